@@ -1,0 +1,146 @@
+/*
+ * cuhre.h -- C-ABI of the MI355X adaptive cubature engine.
+ *
+ * Plain pointers and sizes only.  All functions return 0 on success, a
+ * positive "not converged" flag where stated, or a negative error code
+ * (message available from cuhre_last_error()).  No function calls exit().
+ *
+ * Reference interface replaced (paths relative to the reference tree):
+ *   quad::GPUcuhre<T>::GPUcuhre(argc, argv, dim, key, verbose, numDevices)
+ *       quad/GPUquad/GPUquad.h:46-63           -> cuhre_context_create
+ *   quad::GPUcuhre<T>::integrate(epsrel, epsabs, integral&, error&,
+ *       nregions&, neval&, integT* f, Volume* vol)
+ *       quad/GPUquad/GPUquad.h:611-727         -> cuhre_integrate
+ *   IntegralGPU::Cuhre typedef, CuhreInteface.h:9-12 (the facade that
+ *       include/gpucuhre.hpp mirrors on top of this ABI)
+ *   INTEGRATE_GPU_PHASE1 / SampleRegionBlock (per-region rule values)
+ *       quad/GPUquad/GPUQuadPhases.cu:37-79,
+ *       quad/GPUquad/GPUQuadSample.cu:99-220   -> cuhre_eval_regions
+ *   alignRegions + divideIntervalsGPU (compaction + bisection)
+ *       quad/GPUquad/GPUKernelquad.cu:63-149   -> cuhre_split_regions
+ *   Volume (integration box), quad/GPUquad/GPUQuadVolume.cu:4-25
+ *       -> the lows/highs arguments (FP64 instead of float textures).
+ */
+#ifndef CUHRE_H
+#define CUHRE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CUHRE_MAXDIM 16
+#define CUHRE_MAXTERMS 8
+
+/* error codes */
+#define CUHRE_OK 0
+#define CUHRE_NOT_CONVERGED 1
+#define CUHRE_ERR_ARG (-1)
+#define CUHRE_ERR_NONFINITE (-2)
+#define CUHRE_ERR_ALLOC (-3)
+#define CUHRE_ERR_DEVICE (-4)
+#define CUHRE_ERR_UNSUPPORTED (-5)
+#define CUHRE_ERR_CAPACITY (-6)
+
+/* integrand families compiled into the library */
+#define CUHRE_GAUSSIAN 1     /* exp(-sum a_i^2 (x_i-w_i)^2)                    */
+#define CUHRE_PRODUCT_PEAK 2 /* prod 1/(a_i^-2 + (x_i-w_i)^2)                  */
+#define CUHRE_OSCILLATORY 3  /* cos(2 pi u + sum a_i x_i)                      */
+#define CUHRE_CORNER_PEAK 4  /* (1 + sum a_i x_i)^-exponent                    */
+#define CUHRE_CONTINUOUS 5   /* exp(-sum a_i |x_i - w_i|)                      */
+#define CUHRE_POLYNOMIAL 6   /* sum_t coef_t prod_i x_i^powers[t][i]           */
+#define CUHRE_COSINE_SUM 7   /* -(1/2) sum_i cos(10 x_i)/0.054402111088937     */
+#define CUHRE_TABLE6 8       /* B(x1,x2) x3^-1.5 exp(-(x4-.5)^2/.02)(1+x5 x6)  */
+
+/* Integrand description (the C-ABI stand-in for the reference's functor
+ * with __device__ double operator()(double x[], int dim)).  The table of
+ * CUHRE_TABLE6 is a host pointer to nx*ny doubles, table[i*ny + j] =
+ * B(i/(nx-1), j/(ny-1)); the library copies it to the device. */
+typedef struct {
+  int32_t family;
+  int32_t dim;
+  double a[CUHRE_MAXDIM];
+  double w[CUHRE_MAXDIM];
+  double u;
+  int32_t exponent;
+  int32_t nterms;
+  double coef[CUHRE_MAXTERMS];
+  int32_t powers[CUHRE_MAXTERMS][CUHRE_MAXDIM];
+  const double* table;
+  int32_t table_nx, table_ny;
+} cuhre_integrand;
+
+/* Result of one integration (the reference returns integral, error,
+ * nregions, neval by reference plus an int flag).  fail: 0 converged,
+ * 1 not converged (maxeval, capacity, iteration cap or no active region
+ * left while the global test still fails). */
+typedef struct {
+  double integral;
+  double error;          /* Genz-Malik error estimate sum            */
+  double fp64_err_bound; /* first-order FP64 rounding bound          */
+  int64_t nregions;      /* regions evaluated (Phase-1 meaning)      */
+  int64_t neval;         /* nregions * points per region             */
+  int32_t fail;
+  int32_t iterations;
+  int64_t peak_regions;  /* largest list evaluated in one iteration  */
+  int64_t nfinished;     /* regions retired as finished              */
+  double ms_total;       /* host wall time inside the call           */
+  double ms_eval;        /* device time of rule-evaluation launches  */
+  double ms_split;       /* device time of classify/split launches   */
+  int32_t eval_launches;
+  int32_t split_launches;
+  double eval_flops;     /* algorithmic FP64 flops of the eval launches */
+} cuhre_result;
+
+typedef struct cuhre_context cuhre_context;
+
+/* Creates an engine on HIP device `device`.  capacity_regions bounds the
+ * region list (0 = default of 64 Mi regions); memory grows on demand. */
+int cuhre_context_create(cuhre_context** ctx, int device, int64_t capacity_regions);
+int cuhre_context_destroy(cuhre_context* ctx);
+
+/* Engine knobs.  profile != 0 records HIP events around every launch
+ * (fills ms_eval/ms_split).  lanes_per_region: 0 automatic, 1 (one region
+ * per lane) or 64 (one region per wavefront). */
+int cuhre_context_set(cuhre_context* ctx, int profile, int lanes_per_region);
+
+/* Plain adaptive integration over the box [lows, highs].  key selects the
+ * rule: 0 or 7 = Genz-Malik degree 7/5.  Returns the fail flag (0/1) or a
+ * negative error code. */
+int cuhre_integrate(cuhre_context* ctx, const cuhre_integrand* f, int dim,
+                    double epsrel, double epsabs, int64_t maxeval, int key,
+                    const double* lows, const double* highs, cuhre_result* out);
+
+/* Rule evaluation of n regions given on the host (SoA: lower[m*n + r],
+ * width[m*n + r]).  Outputs per region: est, err, axis, abssum (volume *
+ * sum |w7 f|); i5 and dvec (d*n, SoA) may be NULL. */
+int cuhre_eval_regions(cuhre_context* ctx, const cuhre_integrand* f, int dim,
+                       int key, int64_t n, const double* lower,
+                       const double* width, double* est, double* err,
+                       int32_t* axis, double* abssum, double* i5, double* dvec);
+
+/* One classify + compact + bisect step on a host list: regions with
+ * err > max(epsrel |est|, epsabs) are split on axis[r]; child 0 of the j-th
+ * such region lands at 2j, child 1 at 2j+1 (SoA with stride 2A).  out_*
+ * must hold 2n*dim doubles.  *n_active receives A. */
+int cuhre_split_regions(cuhre_context* ctx, int dim, int64_t n,
+                        const double* lower, const double* width,
+                        const double* est, const double* err,
+                        const int32_t* axis, double epsrel, double epsabs,
+                        double* out_lower, double* out_width,
+                        int64_t* n_active);
+
+/* Deterministic FP64 sum of n values (the reduction the engine uses). */
+int cuhre_sum(cuhre_context* ctx, int64_t n, const double* x, double* out);
+
+/* Thread-local message of the last error. */
+const char* cuhre_last_error(void);
+
+/* Points per region of the rule selected by key for dimension dim. */
+int64_t cuhre_points_per_region(int dim, int key);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

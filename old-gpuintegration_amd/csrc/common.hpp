@@ -1,0 +1,93 @@
+// common.hpp -- definitions shared by the host engine and the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "cuhre.h"
+
+namespace cuhre {
+
+constexpr int kMaxDim = CUHRE_MAXDIM;
+constexpr int kBlock = 256;  // threads per block in every kernel
+constexpr int kWave = 64;    // CDNA wavefront
+
+// Device copy of an integrand: the C-ABI description plus constants derived
+// on the host once per call (1/a^2 for the product peak, 2 pi u for the
+// oscillatory family) so every evaluation point reuses them.
+struct DevIntegrand {
+  int32_t family;
+  int32_t dim;
+  int32_t exponent;
+  int32_t nterms;
+  double a[kMaxDim];
+  double w[kMaxDim];
+  double b[kMaxDim];  // product peak: 1/(a*a)
+  double s0;          // oscillatory: 2*pi*u
+  double coef[CUHRE_MAXTERMS];
+  int8_t powers[CUHRE_MAXTERMS][kMaxDim];
+  const double* table;  // device pointer (TABLE6)
+  int32_t nx, ny;
+};
+
+// Genz-Malik degree-7/5 rule constants, weights normalised to sum to one
+// (est = volume * sum w f).  Built on the host by rule.cpp.
+struct RuleConst {
+  double l2, l3, l4, l5;
+  double w7c, w7a2, w7a3, w7p, w7v;  // centre, axis l2, axis l3, pair l4, corner l5
+  double w5c, w5a2, w5a3, w5p;       // embedded degree-5 rule (no corners)
+  double ratio;                      // (l2/l3)^2 = 1/7
+  int32_t npts;
+};
+
+// Per-block partial sums written by the evaluation kernel.  The *_s/*_c
+// pairs are value + carried rounding error (TwoSum), so the final sums are
+// accurate to about one ulp whatever the block count.
+struct Partial {
+  double est_s, est_c, err_s, err_c;      // all regions of the iteration
+  double fest_s, fest_c, ferr_s, ferr_c;  // regions retired as finished
+  double abs_all, abs_fin;                // sum volume*sum|w7 f|
+  double aest_all, aest_fin;              // sum |est|
+  int64_t nactive, nnonfinite;
+};
+
+// Iteration outcome, written by the last evaluation block.
+struct IterState {
+  Partial tot;        // sums over the iteration's regions
+  double rG_s, rG_c;  // finished-so-far + all regions of this iteration
+  double eG_s, eG_c;
+  int32_t stop;       // 1: converged, 2: no active region, 3: budget, 4: nonfinite
+  int32_t pad;
+};
+
+struct EvalArgs {
+  const double* lower;  // SoA, lower[m*ld + r]
+  const double* width;
+  int64_t ld;
+  int64_t n;
+  double epsrel, epsabs;
+  double* est;
+  double* err;
+  int8_t* axis;
+  double* abssum;  // optional
+  double* i5;      // optional
+  double* dvec;    // optional, dvec[m*ld + r]
+  Partial* partials;
+  int64_t* offsets;  // exclusive scan of active counts per block
+  uint32_t* ticket;
+  IterState* state;
+  double fin_est_s, fin_est_c, fin_err_s, fin_err_c;
+  int64_t max_active;  // stop (budget) when the active count exceeds it
+  int32_t finalize;    // 0: per-region outputs only (no partials/state)
+};
+
+using EvalLauncher = void (*)(const EvalArgs&, const DevIntegrand&, const RuleConst&,
+                              hipStream_t);
+
+// Returns the launcher for (family, dim, lanes per region) or nullptr.
+EvalLauncher find_eval_launcher(int family, int dim, int lanes);
+
+// Regions handled by one evaluation block for `lanes` lanes per region.
+inline int regions_per_block(int lanes) { return kBlock / lanes; }
+
+}  // namespace cuhre

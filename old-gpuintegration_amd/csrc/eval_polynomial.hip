@@ -1,0 +1,6 @@
+// Instantiates the rule-evaluation kernels of the polynomial family for d = 2..10.
+#include "eval_kernels.hpp"
+
+namespace cuhre {
+EvalLauncher launcher_polynomial(int dim, int lanes) { return family_launcher<CUHRE_POLYNOMIAL>(dim, lanes); }
+}  // namespace cuhre

@@ -1,0 +1,143 @@
+// integrands.hpp -- FP64 device integrand families.
+//
+// The reference evaluates a user functor `double operator()(double x[], int
+// dim)` at each cubature point (quad/GPUquad/GPUQuadSample.cu:84,
+// cudaCuhre.cu:11-22).  Functors cannot cross a C ABI, so the families are
+// compiled in and selected by DevIntegrand::family; each is templated on the
+// dimension so the point vector stays in registers.
+#pragma once
+
+#include "common.hpp"
+
+namespace cuhre {
+
+__device__ __forceinline__ double ipow_dev(double b, int e)
+{
+  double r = 1.0;
+  while (e > 0) {
+    if (e & 1) r *= b;
+    b *= b;
+    e >>= 1;
+  }
+  return r;
+}
+
+template <int FAM, int D>
+struct Family;
+
+template <int D>
+struct Family<CUHRE_GAUSSIAN, D> {
+  __device__ __forceinline__ static double eval(const DevIntegrand& P, const double (&x)[D])
+  {
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      double t = P.a[i] * (x[i] - P.w[i]);
+      s = fma(t, t, s);
+    }
+    return exp(-s);
+  }
+};
+
+template <int D>
+struct Family<CUHRE_PRODUCT_PEAK, D> {
+  __device__ __forceinline__ static double eval(const DevIntegrand& P, const double (&x)[D])
+  {
+    double p = 1.0;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      double t = x[i] - P.w[i];
+      p *= fma(t, t, P.b[i]);
+    }
+    return 1.0 / p;
+  }
+};
+
+template <int D>
+struct Family<CUHRE_OSCILLATORY, D> {
+  __device__ __forceinline__ static double eval(const DevIntegrand& P, const double (&x)[D])
+  {
+    double s = P.s0;
+#pragma unroll
+    for (int i = 0; i < D; ++i) s = fma(P.a[i], x[i], s);
+    return cos(s);
+  }
+};
+
+template <int D>
+struct Family<CUHRE_CORNER_PEAK, D> {
+  __device__ __forceinline__ static double eval(const DevIntegrand& P, const double (&x)[D])
+  {
+    double s = 1.0;
+#pragma unroll
+    for (int i = 0; i < D; ++i) s = fma(P.a[i], x[i], s);
+    return ipow_dev(1.0 / s, P.exponent);
+  }
+};
+
+template <int D>
+struct Family<CUHRE_CONTINUOUS, D> {
+  __device__ __forceinline__ static double eval(const DevIntegrand& P, const double (&x)[D])
+  {
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < D; ++i) s = fma(P.a[i], fabs(x[i] - P.w[i]), s);
+    return exp(-s);
+  }
+};
+
+template <int D>
+struct Family<CUHRE_POLYNOMIAL, D> {
+  __device__ __forceinline__ static double eval(const DevIntegrand& P, const double (&x)[D])
+  {
+    double s = 0.0;
+    for (int t = 0; t < P.nterms; ++t) {
+      double p = P.coef[t];
+#pragma unroll
+      for (int i = 0; i < D; ++i) p *= ipow_dev(x[i], P.powers[t][i]);
+      s += p;
+    }
+    return s;
+  }
+};
+
+// The reference driver's test functor (cudaCuhre.cu:11-22, function.cu FUN 5).
+template <int D>
+struct Family<CUHRE_COSINE_SUM, D> {
+  __device__ __forceinline__ static double eval(const DevIntegrand&, const double (&x)[D])
+  {
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < D; ++i) s = s - cos(10.0 * x[i]) / 0.054402111088937;
+    return s / 2.0;
+  }
+};
+
+// Table-driven 6-D integrand: B(x1,x2) x3^-1.5 exp(-(x4-0.5)^2/0.02) (1+x5 x6),
+// B bilinear on an nx*ny FP64 grid over [0,1]^2 (table[i*ny + j]).  The grid
+// (512 KiB at 256x256) exceeds one CU's LDS, so it is read through L2.
+template <int D>
+struct Family<CUHRE_TABLE6, D> {
+  __device__ __forceinline__ static double eval(const DevIntegrand& P, const double (&x)[D])
+  {
+    const int nx = P.nx, ny = P.ny;
+    double fx = x[0] * (double)(nx - 1), fy = x[1] * (double)(ny - 1);
+    int ix = (int)floor(fx), iy = (int)floor(fy);
+    ix = ix < 0 ? 0 : (ix > nx - 2 ? nx - 2 : ix);
+    iy = iy < 0 ? 0 : (iy > ny - 2 ? ny - 2 : iy);
+    double tx = fx - (double)ix, ty = fy - (double)iy;
+    const double* T = P.table + (size_t)ix * ny + iy;
+    double t00 = T[0], t01 = T[1], t10 = T[ny], t11 = T[ny + 1];
+    double b0 = fma(tx, t10 - t00, t00);
+    double b1 = fma(tx, t11 - t01, t01);
+    double B = fma(ty, b1 - b0, b0);
+    double x3 = x[2];
+    double p3 = 1.0 / (x3 * sqrt(x3));
+    double t4 = x[3] - 0.5;
+    double g4 = exp(-(t4 * t4) / 0.02);
+    double q = fma(x[4], x[5], 1.0);
+    return B * p3 * g4 * q;
+  }
+};
+
+}  // namespace cuhre

@@ -1,0 +1,196 @@
+// reduce.hpp -- deterministic FP64 wave/block reductions and the
+// last-block finalisation of an iteration.
+//
+// The reference sums est/err with four thrust::reduce calls and a blocking
+// copy per iteration (quad/GPUquad/GPUKernelquad.cu:565-577) and counts the
+// active regions with thrust::exclusive_scan + two device-to-host copies
+// (:364-386).  Here each evaluation block folds its regions into one Partial
+// (wave shuffles, then LDS across the four waves), and the block that
+// finishes last -- found with an acq_rel ticket at agent scope -- reduces the
+// partials in a fixed order, scans the per-block active counts and takes the
+// stop decision, so an iteration needs no extra launch and no host sync.
+// Sums are carried as (value, rounding error) pairs built with TwoSum.
+#pragma once
+
+#include "common.hpp"
+
+namespace cuhre {
+
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e)
+{
+  s = a + b;
+  double bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+
+// (s1,c1) += (s2,c2)
+__device__ __forceinline__ void dd_add(double& s1, double& c1, double s2, double c2)
+{
+  double s, e;
+  two_sum(s1, s2, s, e);
+  s1 = s;
+  c1 = c1 + c2 + e;
+}
+
+__device__ __forceinline__ double shfl_xor_d(double v, int mask)
+{
+  return __shfl_xor(v, mask, kWave);
+}
+
+__device__ __forceinline__ void partial_zero(Partial& p)
+{
+  p.est_s = p.est_c = p.err_s = p.err_c = 0.0;
+  p.fest_s = p.fest_c = p.ferr_s = p.ferr_c = 0.0;
+  p.abs_all = p.abs_fin = p.aest_all = p.aest_fin = 0.0;
+  p.nactive = p.nnonfinite = 0;
+}
+
+__device__ __forceinline__ void partial_add(Partial& a, const Partial& b)
+{
+  dd_add(a.est_s, a.est_c, b.est_s, b.est_c);
+  dd_add(a.err_s, a.err_c, b.err_s, b.err_c);
+  dd_add(a.fest_s, a.fest_c, b.fest_s, b.fest_c);
+  dd_add(a.ferr_s, a.ferr_c, b.ferr_s, b.ferr_c);
+  a.abs_all += b.abs_all;
+  a.abs_fin += b.abs_fin;
+  a.aest_all += b.aest_all;
+  a.aest_fin += b.aest_fin;
+  a.nactive += b.nactive;
+  a.nnonfinite += b.nnonfinite;
+}
+
+__device__ __forceinline__ void partial_shfl_add(Partial& p, int mask)
+{
+  Partial q;
+  q.est_s = shfl_xor_d(p.est_s, mask);
+  q.est_c = shfl_xor_d(p.est_c, mask);
+  q.err_s = shfl_xor_d(p.err_s, mask);
+  q.err_c = shfl_xor_d(p.err_c, mask);
+  q.fest_s = shfl_xor_d(p.fest_s, mask);
+  q.fest_c = shfl_xor_d(p.fest_c, mask);
+  q.ferr_s = shfl_xor_d(p.ferr_s, mask);
+  q.ferr_c = shfl_xor_d(p.ferr_c, mask);
+  q.abs_all = shfl_xor_d(p.abs_all, mask);
+  q.abs_fin = shfl_xor_d(p.abs_fin, mask);
+  q.aest_all = shfl_xor_d(p.aest_all, mask);
+  q.aest_fin = shfl_xor_d(p.aest_fin, mask);
+  q.nactive = (int64_t)__shfl_xor((long long)p.nactive, mask, kWave);
+  q.nnonfinite = (int64_t)__shfl_xor((long long)p.nnonfinite, mask, kWave);
+  // combine in lane order so both partners compute the same bits
+  if ((__lane_id() & mask) == 0) {
+    partial_add(p, q);
+  } else {
+    partial_add(q, p);
+    p = q;
+  }
+}
+
+// Region classification of the reference: active iff err > MaxErr(est)
+// (quad/GPUquad/GPUQuadPhases.cu:63-73, MaxErr in quad/util/cudaUtil.h:218).
+__device__ __forceinline__ bool region_active(double est, double err, double epsrel,
+                                              double epsabs)
+{
+  return err > fmax(epsrel * fabs(est), epsabs);
+}
+
+__device__ __forceinline__ void partial_from_region(Partial& p, double est, double err,
+                                                    double ab, bool valid, double epsrel,
+                                                    double epsabs)
+{
+  partial_zero(p);
+  if (!valid) return;
+  const bool finite = isfinite(est) && isfinite(err);
+  const bool act = finite && region_active(est, err, epsrel, epsabs);
+  p.est_s = est;
+  p.err_s = err;
+  p.abs_all = ab;
+  p.aest_all = fabs(est);
+  if (!act) {
+    p.fest_s = est;
+    p.ferr_s = err;
+    p.abs_fin = ab;
+    p.aest_fin = fabs(est);
+  }
+  p.nactive = act ? 1 : 0;
+  p.nnonfinite = finite ? 0 : 1;
+}
+
+// Reduce one Partial per thread to thread 0 of the block (256 threads).
+__device__ __forceinline__ void block_reduce_partial(Partial& p, Partial* lds4)
+{
+#pragma unroll
+  for (int mask = 1; mask < kWave; mask <<= 1) partial_shfl_add(p, mask);
+  const int wave = threadIdx.x / kWave;
+  if (__lane_id() == 0) lds4[wave] = p;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    Partial t = lds4[0];
+    for (int w = 1; w < kBlock / kWave; ++w) partial_add(t, lds4[w]);
+    p = t;
+  }
+}
+
+// Called by every block after its thread 0 holds the block's Partial in p.
+// The last block to arrive reduces all partials, scans active counts into
+// offsets and writes the iteration state.
+__device__ __forceinline__ void finish_iteration(const EvalArgs& A, Partial& p, Partial* lds4)
+{
+  __shared__ uint32_t s_last;
+  const uint32_t nb = gridDim.x;
+  if (threadIdx.x == 0) {
+    A.partials[blockIdx.x] = p;
+    uint32_t t = __hip_atomic_fetch_add(A.ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    s_last = (t == nb - 1) ? 1u : 0u;
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+
+  // contiguous chunk of blocks per thread, summed in order
+  const uint32_t chunk = (nb + kBlock - 1) / kBlock;
+  const uint32_t b0 = threadIdx.x * chunk;
+  const uint32_t b1 = b0 + chunk < nb ? b0 + chunk : nb;
+  Partial q;
+  partial_zero(q);
+  for (uint32_t b = b0; b < b1; ++b) partial_add(q, A.partials[b]);
+  // exclusive scan of per-thread active counts (Hillis-Steele in LDS)
+  __shared__ int64_t s_scan[kBlock];
+  int64_t mine = q.nactive;
+  s_scan[threadIdx.x] = mine;
+  __syncthreads();
+  for (int off = 1; off < kBlock; off <<= 1) {
+    int64_t v = threadIdx.x >= (unsigned)off ? s_scan[threadIdx.x - off] : 0;
+    __syncthreads();
+    s_scan[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int64_t run = s_scan[threadIdx.x] - mine;
+  for (uint32_t b = b0; b < b1; ++b) {
+    A.offsets[b] = run;
+    run += A.partials[b].nactive;
+  }
+  block_reduce_partial(q, lds4);
+  if (threadIdx.x == 0) {
+    IterState st;
+    st.tot = q;
+    st.rG_s = A.fin_est_s;
+    st.rG_c = A.fin_est_c;
+    dd_add(st.rG_s, st.rG_c, q.est_s, q.est_c);
+    st.eG_s = A.fin_err_s;
+    st.eG_c = A.fin_err_c;
+    dd_add(st.eG_s, st.eG_c, q.err_s, q.err_c);
+    const double rG = st.rG_s + st.rG_c, eG = st.eG_s + st.eG_c;
+    int stop = 0;
+    if (q.nnonfinite > 0) stop = 4;
+    else if (eG <= fmax(A.epsrel * fabs(rG), A.epsabs)) stop = 1;
+    else if (q.nactive == 0) stop = 2;
+    else if (q.nactive > A.max_active) stop = 3;
+    st.stop = stop;
+    st.pad = 0;
+    *A.state = st;
+    // reset for the next launch (stream order makes this visible to it)
+    __hip_atomic_store(A.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+}  // namespace cuhre

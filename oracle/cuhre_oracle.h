@@ -1,0 +1,124 @@
+/*
+ * cuhre_oracle.h -- CPU FP64 restatement of the adaptive cubature path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This code is the checker: only tests/, the
+ * smoke() entry and bench.py's CPU-baseline leg may load it.  The product
+ * library never links or calls it.
+ *
+ * What it restates (reference file:line, paths relative to the reference):
+ *   - per-region rule evaluation, fourth-difference axis choice and its tie
+ *     rules: quad/GPUquad/GPUQuadSample.cu:99-220 (the rule itself is the
+ *     Genz-Malik degree-7/5 pair, Genz & Malik 1980, not the reference's
+ *     hard-wired degree-9 rule);
+ *   - region classification err > max(epsrel*|est|, epsabs):
+ *     quad/GPUquad/GPUQuadPhases.cu:60-78, quad/util/cudaUtil.h:218;
+ *   - the breadth-first loop, global error test and counters:
+ *     quad/GPUquad/GPUKernelquad.cu:507-668;
+ *   - initial k^d grid: quad/GPUquad/GPUKernelquad.cu:23-61, 300-316;
+ *   - bisection: quad/GPUquad/GPUKernelquad.cu:97-149;
+ *   - final flag: quad/GPUquad/GPUquad.h:713-714.
+ * Closed forms of every integrand family are evaluated in __float128.
+ */
+#ifndef CUHRE_ORACLE_H
+#define CUHRE_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OR_MAXDIM 16
+#define OR_MAXTERMS 8
+
+enum {
+  OR_GAUSSIAN = 1,      /* exp(-sum a_i^2 (x_i-w_i)^2)                     */
+  OR_PRODUCT_PEAK = 2,  /* prod 1/(a_i^-2 + (x_i-w_i)^2)                   */
+  OR_OSCILLATORY = 3,   /* cos(2 pi u + sum a_i x_i)                       */
+  OR_CORNER_PEAK = 4,   /* (1 + sum a_i x_i)^-k, k = exponent              */
+  OR_CONTINUOUS = 5,    /* exp(-sum a_i |x_i - w_i|)                       */
+  OR_POLYNOMIAL = 6,    /* sum_t c_t prod_i x_i^e_ti                       */
+  OR_COSINE_SUM = 7,    /* -(1/2) sum cos(10 x_i)/0.054402111088937        */
+  OR_TABLE6 = 8         /* B(x1,x2) x3^-1.5 exp(-(x4-.5)^2/.02) (1+x5 x6)  */
+};
+
+typedef struct {
+  int32_t family;
+  int32_t dim;
+  double a[OR_MAXDIM];
+  double w[OR_MAXDIM];
+  double u;
+  int32_t exponent;
+  int32_t nterms;
+  double coef[OR_MAXTERMS];
+  int32_t powers[OR_MAXTERMS][OR_MAXDIM];
+  const double* table; /* TABLE6: nx*ny values, table[i*ny + j] = B(i/(nx-1), j/(ny-1)) */
+  int32_t table_nx, table_ny;
+} or_integrand;
+
+/* Genz-Malik degree-7 rule with embedded degree-5 rule on [-1,1]^d, weights
+ * normalised to sum to 1 (so est = volume * sum w_k f(x_k)). */
+typedef struct {
+  int dim, npts;
+  double* g;   /* npts*dim generator offsets, point k row g[k*dim ..] */
+  double* w7;  /* npts */
+  double* w5;  /* npts (0 for the 2^d corner points) */
+} or_rule;
+
+int or_rule_init(or_rule* r, int dim);
+void or_rule_free(or_rule* r);
+int or_gm_npoints(int dim);
+
+double or_f(const or_integrand* f, const double* x);
+
+/* One region.  lower/width are the region's d-vectors.  dvec (may be NULL)
+ * receives the d fourth differences; abssum receives volume*sum|w7 f|. */
+void or_region_eval(const or_rule* r, const or_integrand* f,
+                    const double* lower, const double* width,
+                    double* est, double* err, int* axis, double* abssum,
+                    double* i5, double* dvec);
+
+typedef struct {
+  double integral, error, bound;
+  int64_t nregions, neval;
+  int32_t fail, iterations;
+  int64_t peak_regions;
+  int64_t nfinished;
+} or_result;
+
+/* Breadth-first loop (the plain method).  nthreads>1 uses OpenMP over the
+ * regions of an iteration.  Returns 0 (converged), 1 (not converged), <0
+ * error (-2 nonfinite integrand value, -3 allocation, -1 bad argument). */
+int or_integrate_bfs(const or_integrand* f, const double* lows,
+                     const double* highs, double epsrel, double epsabs,
+                     int64_t maxeval, int64_t capacity, int nthreads,
+                     or_result* out);
+
+/* Cuhre-style global adaptive (heap of regions, always bisect the region
+ * with the largest error), same GM7/5 rule: the CPU baseline flavour (i). */
+int or_integrate_heap(const or_integrand* f, const double* lows,
+                      const double* highs, double epsrel, double epsabs,
+                      int64_t maxeval, or_result* out);
+
+/* Exact integral over the box, evaluated in __float128, returned as double
+ * (hi part) plus the residual (lo part). */
+int or_exact(const or_integrand* f, const double* lows, const double* highs,
+             double* hi, double* lo);
+
+/* Splitting step on a host list (for compaction/bisection checks):
+ * given n regions SoA (lower[m*n+r], width[m*n+r]) and flags/axes, writes
+ * 2*A children (child 0 at 2j, child 1 at 2j+1, SoA with stride 2A).
+ * Returns A. */
+int64_t or_split(int dim, int64_t n, const double* lower, const double* width,
+                 const int32_t* active, const int32_t* axis,
+                 double* out_lower, double* out_width);
+
+/* Initial grid: k^d regions, k = 4 (d<=4), 2 (5<=d<=10), 1 (d>10).
+ * Writes SoA with stride n; returns n. lower/width may be NULL to query n. */
+int64_t or_initial_grid(int dim, const double* lows, const double* highs,
+                        double* lower, double* width);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

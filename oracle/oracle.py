@@ -1,0 +1,228 @@
+"""ctypes loader for the CPU oracle (test infrastructure only).
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's CPU-baseline leg may
+import this module; the product path never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "libcuhre_oracle.so")
+sys.path.insert(0, os.path.join(os.path.dirname(HERE), "old-gpuintegration_amd"))
+import workloads as W  # noqa: E402
+
+MAXDIM = 16
+MAXTERMS = 8
+
+
+class OrIntegrand(C.Structure):
+    _fields_ = [
+        ("family", C.c_int32),
+        ("dim", C.c_int32),
+        ("a", C.c_double * MAXDIM),
+        ("w", C.c_double * MAXDIM),
+        ("u", C.c_double),
+        ("exponent", C.c_int32),
+        ("nterms", C.c_int32),
+        ("coef", C.c_double * MAXTERMS),
+        ("powers", (C.c_int32 * MAXDIM) * MAXTERMS),
+        ("table", C.POINTER(C.c_double)),
+        ("table_nx", C.c_int32),
+        ("table_ny", C.c_int32),
+    ]
+
+
+class OrResult(C.Structure):
+    _fields_ = [
+        ("integral", C.c_double),
+        ("error", C.c_double),
+        ("bound", C.c_double),
+        ("nregions", C.c_int64),
+        ("neval", C.c_int64),
+        ("fail", C.c_int32),
+        ("iterations", C.c_int32),
+        ("peak_regions", C.c_int64),
+        ("nfinished", C.c_int64),
+    ]
+
+    def asdict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class OrRule(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int),
+        ("npts", C.c_int),
+        ("g", C.POINTER(C.c_double)),
+        ("w7", C.POINTER(C.c_double)),
+        ("w5", C.POINTER(C.c_double)),
+    ]
+
+
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        dp = C.POINTER(C.c_double)
+        ip = C.POINTER(C.c_int32)
+        L.or_rule_init.argtypes = [C.POINTER(OrRule), C.c_int]
+        L.or_rule_free.argtypes = [C.POINTER(OrRule)]
+        L.or_f.argtypes = [C.POINTER(OrIntegrand), dp]
+        L.or_f.restype = C.c_double
+        L.or_region_eval.argtypes = [C.POINTER(OrRule), C.POINTER(OrIntegrand), dp, dp,
+                                     dp, dp, C.POINTER(C.c_int), dp, dp, dp]
+        L.or_integrate_bfs.argtypes = [C.POINTER(OrIntegrand), dp, dp, C.c_double, C.c_double,
+                                       C.c_int64, C.c_int64, C.c_int, C.POINTER(OrResult)]
+        L.or_integrate_heap.argtypes = [C.POINTER(OrIntegrand), dp, dp, C.c_double, C.c_double,
+                                        C.c_int64, C.POINTER(OrResult)]
+        L.or_exact.argtypes = [C.POINTER(OrIntegrand), dp, dp, dp, dp]
+        L.or_split.argtypes = [C.c_int, C.c_int64, dp, dp, ip, ip, dp, dp]
+        L.or_split.restype = C.c_int64
+        L.or_initial_grid.argtypes = [C.c_int, dp, dp, dp, dp]
+        L.or_initial_grid.restype = C.c_int64
+        _lib = L
+    return _lib
+
+
+def _dp(arr):
+    return arr.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def to_struct(f: W.Integrand):
+    s = OrIntegrand()
+    s.family = f.family
+    s.dim = f.dim
+    for i, v in enumerate(f.a):
+        s.a[i] = v
+    for i, v in enumerate(f.w):
+        s.w[i] = v
+    s.u = f.u
+    s.exponent = f.exponent
+    s.nterms = len(f.coef)
+    for t, (c, pw) in enumerate(zip(f.coef, f.powers)):
+        s.coef[t] = c
+        for i, e in enumerate(pw):
+            s.powers[t][i] = e
+    keep = None
+    if f.table is not None:
+        keep = np.ascontiguousarray(f.table, dtype=np.float64)
+        s.table = _dp(keep)
+        s.table_nx, s.table_ny = keep.shape
+    return s, keep
+
+
+def rule(d):
+    """Return (g[p,d], w7[p], w5[p]) of the oracle's GM7/5 rule."""
+    r = OrRule()
+    rc = lib().or_rule_init(C.byref(r), d)
+    assert rc == 0
+    p = r.npts
+    g = np.ctypeslib.as_array(r.g, shape=(p * d,)).reshape(p, d).copy()
+    w7 = np.ctypeslib.as_array(r.w7, shape=(p,)).copy()
+    w5 = np.ctypeslib.as_array(r.w5, shape=(p,)).copy()
+    lib().or_rule_free(C.byref(r))
+    return g, w7, w5
+
+
+def f_values(f: W.Integrand, X):
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    s, keep = to_struct(f)
+    out = np.empty(X.shape[0])
+    for i in range(X.shape[0]):
+        out[i] = lib().or_f(C.byref(s), _dp(X[i]))
+    return out
+
+
+def region_eval(f: W.Integrand, lower, width):
+    """Per-region (est, err, axis, abssum, i5, dvec) for arrays lower/width [n, d]."""
+    lower = np.ascontiguousarray(lower, dtype=np.float64)
+    width = np.ascontiguousarray(width, dtype=np.float64)
+    n, d = lower.shape
+    r = OrRule()
+    assert lib().or_rule_init(C.byref(r), d) == 0
+    s, keep = to_struct(f)
+    est = np.empty(n); err = np.empty(n); ab = np.empty(n); i5 = np.empty(n)
+    ax = np.empty(n, dtype=np.int32)
+    dv = np.empty((n, d))
+    e = C.c_double(); er = C.c_double(); a = C.c_int(); abv = C.c_double(); i5v = C.c_double()
+    for i in range(n):
+        lib().or_region_eval(C.byref(r), C.byref(s), _dp(lower[i]), _dp(width[i]), C.byref(e),
+                             C.byref(er), C.byref(a), C.byref(abv), C.byref(i5v), _dp(dv[i]))
+        est[i], err[i], ax[i], ab[i], i5[i] = e.value, er.value, a.value, abv.value, i5v.value
+    lib().or_rule_free(C.byref(r))
+    return est, err, ax, ab, i5, dv
+
+
+def integrate_bfs(p: W.Problem, nthreads=1, capacity=None, maxeval=None, epsrel=None):
+    s, keep = to_struct(p.f)
+    lo = np.array(p.lows, dtype=np.float64)
+    hi = np.array(p.highs, dtype=np.float64)
+    res = OrResult()
+    me = p.maxeval if maxeval is None else maxeval
+    cap = capacity if capacity is not None else max(1, me // W.gm_npoints(p.f.dim)) + 64
+    rc = lib().or_integrate_bfs(C.byref(s), _dp(lo), _dp(hi), p.epsrel if epsrel is None else epsrel,
+                                p.epsabs, me, cap, nthreads, C.byref(res))
+    d = res.asdict()
+    d["rc"] = rc
+    return d
+
+
+def integrate_heap(p: W.Problem, maxeval=None):
+    s, keep = to_struct(p.f)
+    lo = np.array(p.lows, dtype=np.float64)
+    hi = np.array(p.highs, dtype=np.float64)
+    res = OrResult()
+    rc = lib().or_integrate_heap(C.byref(s), _dp(lo), _dp(hi), p.epsrel, p.epsabs,
+                                 p.maxeval if maxeval is None else maxeval, C.byref(res))
+    d = res.asdict()
+    d["rc"] = rc
+    return d
+
+
+def exact(f: W.Integrand, lows, highs):
+    s, keep = to_struct(f)
+    lo = np.array(lows, dtype=np.float64)
+    hi = np.array(highs, dtype=np.float64)
+    h = C.c_double(); l = C.c_double()
+    rc = lib().or_exact(C.byref(s), _dp(lo), _dp(hi), C.byref(h), C.byref(l))
+    if rc != 0:
+        raise ValueError("no closed form for this integrand/box")
+    return h.value, l.value
+
+
+def initial_grid(d, lows, highs):
+    lo = np.array(lows, dtype=np.float64)
+    hi = np.array(highs, dtype=np.float64)
+    n = lib().or_initial_grid(d, _dp(lo), _dp(hi), None, None)
+    L = np.empty(n * d); Wd = np.empty(n * d)
+    lib().or_initial_grid(d, _dp(lo), _dp(hi), _dp(L), _dp(Wd))
+    return L.reshape(d, n).T.copy(), Wd.reshape(d, n).T.copy()
+
+
+def split(lower, width, active, axis):
+    """lower/width [n,d]; returns children [2A, d] (child 0 at 2j, child 1 at 2j+1)."""
+    n, d = lower.shape
+    L = np.ascontiguousarray(lower.T).ravel()
+    Wd = np.ascontiguousarray(width.T).ravel()
+    act = np.ascontiguousarray(active, dtype=np.int32)
+    ax = np.ascontiguousarray(axis, dtype=np.int32)
+    A = int(act.astype(bool).sum())
+    oL = np.empty(max(1, 2 * A * d)); oW = np.empty(max(1, 2 * A * d))
+    lib().or_split(d, n, _dp(L), _dp(Wd), act.ctypes.data_as(C.POINTER(C.c_int32)),
+                   ax.ctypes.data_as(C.POINTER(C.c_int32)), _dp(oL), _dp(oW))
+    return (oL[: 2 * A * d].reshape(d, 2 * A).T.copy(), oW[: 2 * A * d].reshape(d, 2 * A).T.copy())
