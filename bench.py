@@ -1,0 +1,173 @@
+#!/usr/bin/env python3
+"""Benchmark: integrand evaluations per second of the plain GPU method on the
+8-D workloads (oscillatory + product peak on [0,1]^8, FP64, Genz-Malik
+degree-7/5, 401 points per region, epsrel 1e-6, maxeval 1e9).
+
+One step = one cuhre_integrate call per 8-D integrand, from the initial grid
+to the stop (global test met, or the evaluation budget reached).  The value
+is whole-step evals/s = sum(neval) / wall time.  Inputs (the integrand
+parameters) are a few hundred bytes; the region list is created on the GPU.
+
+Also reported: the roofline of the dominant kernel (one-region-per-lane rule
+evaluation of the product peak, FP64-bound), and the CPU baseline (the oracle
+port, OpenMP over the host cores) on the same workload.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "old-gpuintegration_amd"))
+
+import workloads as W  # noqa: E402
+
+FP64_PEAK_TFLOPS = 78.6
+
+
+def flop_costs():
+    path = os.path.join(ROOT, "old-gpuintegration_amd", "flop_costs.json")
+    with open(path) as fh:
+        return json.load(fh)["fp64_flops"]
+
+
+def family_flops(fam, d, c):
+    """Algorithmic FP64 flops of one integrand evaluation (add/sub/mul 1,
+    fma 2, div/exp/cos/sqrt charged their measured instruction flops)."""
+    if fam == W.GAUSSIAN:
+        return 4 * d + c["exp"]
+    if fam == W.PRODUCT_PEAK:
+        return 4 * d + c["div"]
+    if fam == W.OSCILLATORY:
+        return 2 * d + 1 + c["cos"]
+    if fam == W.CORNER_PEAK:
+        return 2 * d + 4 + c["div"]
+    if fam == W.TABLE6:
+        return 20 + c["exp"] + c["sqrt"] + 2 * c["div"]
+    raise ValueError(fam)
+
+
+def region_flops(fam, d, c):
+    p = W.gm_npoints(d)
+    return p * (family_flops(fam, d, c) + 2 * d + 4) + (8 * d + 12)
+
+
+def cpu_baseline(problems):
+    """The oracle's breadth-first loop (same algorithm, C + OpenMP) on the
+    same workload, on the host cores."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    threads = min(16, os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    ne = 0
+    for p in problems:
+        r = oracle.integrate_bfs(p, nthreads=threads)
+        ne += r["neval"]
+    dt = time.perf_counter() - t0
+    return {"value": ne / dt, "unit": "evals/s", "cores": threads, "kind": "port",
+            "sample": f"full workload: oracle breadth-first GM7/5 loop on both 8-D integrands "
+                      f"(maxeval 1e9), {ne} evals in {dt:.2f} s, OpenMP over regions"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--converge", action="store_true",
+                    help="also run the product peak with maxeval 1e11 to reach epsrel 1e-6")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the path runs on a single MI355X "
+              "(multi-GPU sharding is a design note, not built)", file=sys.stderr)
+        sys.exit(2)
+
+    import cuhre
+    problems = [W.c2_oscillatory8(), W.c2_product_peak8()]
+    ctx = cuhre.Context(0)
+    for _ in range(args.warmup):
+        for p in problems:
+            ctx.integrate(p)
+    # timed region: plain steps, no profiling events
+    t0 = time.perf_counter()
+    runs = []
+    for _ in range(args.steps):
+        runs.append([ctx.integrate(p) for p in problems])
+    dt = time.perf_counter() - t0
+    neval = sum(r["neval"] for step in runs for r in step)
+    value = neval / dt
+
+    # kernel timing with HIP events on the engine's stream over the same steps
+    ctx.set(True, 0)
+    prof = [[ctx.integrate(p) for p in problems] for _ in range(args.steps)]
+    ctx.set(False, 0)
+    costs = flop_costs()
+    pp = [step[1] for step in prof]
+    lane_ms = sum(r["ms_eval_lane"] for r in pp)
+    lane_launches = sum(r["eval_lane_launches"] for r in pp)
+    lane_regions = sum(r["lane_regions"] for r in pp)
+    fl = region_flops(W.PRODUCT_PEAK, 8, costs)
+    achieved = (fl * lane_regions / lane_launches) / (lane_ms / lane_launches * 1e-3) / 1e12 if lane_ms else 0.0
+
+    last = runs[-1]
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    per = []
+    for p, r, pr in zip(problems, last, prof[-1]):
+        item = {"integrand": p.name, "integral": r["integral"], "error": r["error"],
+                "fp64_err_bound": r["fp64_err_bound"], "nregions": r["nregions"],
+                "neval": r["neval"], "fail": r["fail"], "iterations": r["iterations"],
+                "peak_regions": r["peak_regions"], "ms": r["ms_total"],
+                "ms_eval": pr["ms_eval"], "ms_split": pr["ms_split"]}
+        try:
+            import oracle
+            ex, _ = oracle.exact(p.f, p.lows, p.highs)
+            item["exact"] = ex
+            item["abs_err_vs_exact"] = abs(r["integral"] - ex)
+        except Exception:
+            pass
+        per.append(item)
+
+    out = {
+        "metric": "integrand evals/s to epsrel 1e-6 (8-D), plain GPU method",
+        "value": value,
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded SplitMix64 integrand parameters)",
+        "config": {"workload": "oscillatory8d + product_peak8d on [0,1]^8, GM7/5 (401 pts/region), "
+                               "epsrel 1e-6, maxeval 1e9 each",
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "fp64", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "k_eval_lane<8, product_peak>",
+                     "launches": lane_launches, "avg_launch_ms": lane_ms / max(lane_launches, 1),
+                     "flops_per_region": fl},
+        "per_integrand": per,
+    }
+    if args.converge:
+        p = W.c2_product_peak8()
+        t1 = time.perf_counter()
+        r = ctx.integrate(p, maxeval=10**11)
+        t2 = time.perf_counter()
+        out["product_peak_to_convergence"] = {"maxeval": 10**11, "seconds": t2 - t1,
+                                              "evals_per_s": r["neval"] / (t2 - t1), **{k: r[k] for k in (
+                                                  "integral", "error", "nregions", "neval", "fail",
+                                                  "iterations", "peak_regions")}}
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(problems)
+    ctx.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

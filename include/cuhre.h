@@ -91,6 +91,10 @@ typedef struct {
   int32_t eval_launches;
   int32_t split_launches;
   double eval_flops;     /* algorithmic FP64 flops of the eval launches */
+  double ms_eval_lane;   /* device time of one-region-per-lane launches */
+  int32_t eval_lane_launches;
+  int32_t pad0;
+  int64_t lane_regions;  /* regions evaluated by those launches         */
 } cuhre_result;
 
 typedef struct cuhre_context cuhre_context;

@@ -56,7 +56,7 @@ struct IterState {
   Partial tot;        // sums over the iteration's regions
   double rG_s, rG_c;  // finished-so-far + all regions of this iteration
   double eG_s, eG_c;
-  int32_t stop;       // 1: converged, 2: no active region, 3: budget, 4: nonfinite
+  int32_t stop;  // 1 converged, 2 no active region, 3 budget, 4 nonfinite, 5 grow
   int32_t pad;
 };
 
@@ -77,7 +77,8 @@ struct EvalArgs {
   uint32_t* ticket;
   IterState* state;
   double fin_est_s, fin_est_c, fin_err_s, fin_err_c;
-  int64_t max_active;  // stop (budget) when the active count exceeds it
+  int64_t max_active;    // stop (eval budget / capacity) above this count
+  int64_t alloc_active;  // grow the list buffers above this count
   int32_t finalize;    // 0: per-region outputs only (no partials/state)
 };
 

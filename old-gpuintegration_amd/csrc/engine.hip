@@ -1,0 +1,864 @@
+// engine.hip -- host loop of the plain adaptive cubature method, the
+// classify/compact/bisect kernel and the C-ABI (include/cuhre.h).
+//
+// Reference path replaced: GPUcuhre<T>::integrate (quad/GPUquad/GPUquad.h:
+// 611-727) -> GPUKernelCuhre::IntegrateFirstPhase / FirstPhaseIteration /
+// GenerateActiveIntervals (quad/GPUquad/GPUKernelquad.cu:356-668).  The
+// reference allocates and frees ~7 device buffers per iteration, reduces
+// with four thrust calls and synchronises the host three times; here the
+// region list lives in two preallocated SoA buffers in HBM (ping-pong), an
+// iteration is two launches (evaluate+classify+reduce, then
+// compact+bisect) and one 160-byte state copy, and the loop runs until the
+// global test, the evaluation budget or the capacity stops it (no 2^14
+// region Phase-1 cap and no Phase 2).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "reduce.hpp"
+
+namespace cuhre {
+
+EvalLauncher launcher_gaussian(int, int);
+EvalLauncher launcher_product_peak(int, int);
+EvalLauncher launcher_oscillatory(int, int);
+EvalLauncher launcher_corner_peak(int, int);
+EvalLauncher launcher_continuous(int, int);
+EvalLauncher launcher_polynomial(int, int);
+EvalLauncher launcher_cosine_sum(int, int);
+EvalLauncher launcher_table6(int, int);
+
+EvalLauncher find_eval_launcher(int family, int dim, int lanes)
+{
+  switch (family) {
+  case CUHRE_GAUSSIAN: return launcher_gaussian(dim, lanes);
+  case CUHRE_PRODUCT_PEAK: return launcher_product_peak(dim, lanes);
+  case CUHRE_OSCILLATORY: return launcher_oscillatory(dim, lanes);
+  case CUHRE_CORNER_PEAK: return launcher_corner_peak(dim, lanes);
+  case CUHRE_CONTINUOUS: return launcher_continuous(dim, lanes);
+  case CUHRE_POLYNOMIAL: return launcher_polynomial(dim, lanes);
+  case CUHRE_COSINE_SUM: return launcher_cosine_sum(dim, lanes);
+  case CUHRE_TABLE6: return launcher_table6(dim, lanes);
+  default: return nullptr;
+  }
+}
+
+// ------------------------------------------------------------------ errors
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char* fmt, ...)
+{
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return fail(CUHRE_ERR_DEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                  __FILE__, __LINE__);                                                  \
+  } while (0)
+
+// ------------------------------------------------------------------ rule
+
+int gm_points(int d) { return (1 << d) + 2 * d * d + 2 * d + 1; }
+
+// Genz-Malik (1980) degree-7 rule and embedded degree-5 rule; weights on
+// [-1,1]^d divided by the volume 2^d.
+RuleConst make_rule(int d)
+{
+  RuleConst R;
+  const double dd = (double)d;
+  R.l2 = std::sqrt(9.0 / 70.0);
+  R.l3 = std::sqrt(9.0 / 10.0);
+  R.l4 = std::sqrt(9.0 / 10.0);
+  R.l5 = std::sqrt(9.0 / 19.0);
+  R.w7c = (12824.0 - 9120.0 * dd + 400.0 * dd * dd) / 19683.0;
+  R.w7a2 = 980.0 / 6561.0;
+  R.w7a3 = (1820.0 - 400.0 * dd) / 19683.0;
+  R.w7p = 200.0 / 19683.0;
+  R.w7v = 6859.0 / 19683.0 / (double)(1 << d);
+  R.w5c = (729.0 - 950.0 * dd + 50.0 * dd * dd) / 729.0;
+  R.w5a2 = 245.0 / 486.0;
+  R.w5a3 = (265.0 - 100.0 * dd) / 1458.0;
+  R.w5p = 25.0 / 729.0;
+  R.ratio = 1.0 / 7.0;
+  R.npts = gm_points(d);
+  return R;
+}
+
+// Algorithmic FP64 flops of one region's evaluation for a family, by the
+// counting convention add/sub/mul = 1, fma = 2, and div / exp / cos / sqrt
+// charged the FP64 VALU flops of their gfx950 instruction sequences
+// (measured with the SQ_INSTS_VALU_*_F64 counters on a one-op kernel).
+struct FlopCosts { double div = 21, exp = 42, cos = 66, sqrt = 20; };
+static FlopCosts g_costs;
+
+double family_flops(int fam, int d)
+{
+  switch (fam) {
+  case CUHRE_GAUSSIAN: return 4.0 * d + g_costs.exp;
+  case CUHRE_PRODUCT_PEAK: return 4.0 * d + g_costs.div;
+  case CUHRE_OSCILLATORY: return 2.0 * d + 1 + g_costs.cos;
+  case CUHRE_CORNER_PEAK: return 2.0 * d + 4 + g_costs.div;
+  case CUHRE_CONTINUOUS: return 4.0 * d + g_costs.exp;
+  case CUHRE_COSINE_SUM: return d * (2.0 + g_costs.cos + g_costs.div) + 1;
+  case CUHRE_TABLE6: return 20.0 + g_costs.exp + g_costs.sqrt + 2 * g_costs.div;
+  default: return 4.0 * d;
+  }
+}
+
+double region_flops(int fam, int d)
+{
+  const double p = gm_points(d);
+  return p * (family_flops(fam, d) + 2.0 * d + 4.0) + (8.0 * d + 12.0);
+}
+
+// ------------------------------------------------------------------ split
+
+struct SplitArgs {
+  const double* lower;
+  const double* width;
+  int64_t ld;
+  int64_t n;
+  const double* est;
+  const double* err;
+  const int8_t* axis;
+  double epsrel, epsabs;
+  const int64_t* offsets;  // per evaluation block (rpb regions)
+  int32_t rpb;
+  int32_t dim;
+  double* out_lower;
+  double* out_width;
+  int64_t out_ld;
+  const IterState* state;  // nullptr: always split
+};
+
+// Classify + stream-compact + bisect in one pass (replaces the reference's
+// thrust::exclusive_scan, alignRegions and divideIntervalsGPU,
+// quad/GPUquad/GPUKernelquad.cu:63-149, 356-505).  The rank of an active
+// region inside its evaluation block comes from a wavefront ballot (plus
+// the counts of earlier waves of the same block through LDS); the block's
+// base offset was scanned by the evaluation kernel's last block.  Child 0
+// of the j-th active region goes to slot 2j and child 1 to 2j+1, so the
+// two children of a region are adjacent and written with one 16-byte store
+// per coordinate.
+__global__ __launch_bounds__(kBlock) void k_split(SplitArgs S)
+{
+  if (S.state != nullptr && S.state->stop != 0) return;
+  __shared__ uint32_t wcnt[kBlock / kWave];
+  const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const bool valid = r < S.n;
+  bool act = false;
+  if (valid) {
+    const double e = S.est[r], er = S.err[r];
+    act = isfinite(e) && isfinite(er) && region_active(e, er, S.epsrel, S.epsabs);
+  }
+  const uint64_t mask = __ballot(act);
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  const uint64_t below = lane ? (~0ull >> (kWave - lane)) : 0ull;
+  int64_t rank;
+  if (S.rpb >= kWave) {
+    if (lane == 0) wcnt[wave] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    const int wpb = S.rpb / kWave;
+    const int first = wave - wave % wpb;
+    rank = __popcll(mask & below);
+    for (int w = first; w < wave; ++w) rank += wcnt[w];
+  } else {
+    const int g0 = lane - lane % S.rpb;
+    const uint64_t grp = (S.rpb == kWave ? ~0ull : ((1ull << S.rpb) - 1ull)) << g0;
+    rank = __popcll(mask & grp & below);
+  }
+  if (!act) return;
+  const int64_t dst = S.offsets[r / S.rpb] + rank;
+  const int a = S.axis[r];
+  for (int m = 0; m < S.dim; ++m) {
+    const double lo = S.lower[m * S.ld + r];
+    const double w = S.width[m * S.ld + r];
+    double2 L, W;
+    if (m == a) {
+      const double hw = 0.5 * w;
+      L = make_double2(lo, lo + hw);
+      W = make_double2(hw, hw);
+    } else {
+      L = make_double2(lo, lo);
+      W = make_double2(w, w);
+    }
+    *reinterpret_cast<double2*>(S.out_lower + m * S.out_ld + 2 * dst) = L;
+    *reinterpret_cast<double2*>(S.out_width + m * S.out_ld + 2 * dst) = W;
+  }
+}
+
+// Active-region count per 256-region block (standalone split entry point).
+__global__ __launch_bounds__(kBlock) void k_count(const double* est, const double* err, int64_t n,
+                                                   double epsrel, double epsabs, int64_t* counts)
+{
+  __shared__ uint32_t wcnt[kBlock / kWave];
+  const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  bool act = false;
+  if (r < n) act = isfinite(est[r]) && isfinite(err[r]) && region_active(est[r], err[r], epsrel, epsabs);
+  const uint64_t mask = __ballot(act);
+  if ((threadIdx.x & (kWave - 1)) == 0) wcnt[threadIdx.x / kWave] = (uint32_t)__popcll(mask);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t c = 0;
+    for (int w = 0; w < kBlock / kWave; ++w) c += wcnt[w];
+    counts[blockIdx.x] = c;
+  }
+}
+
+// Deterministic two-pass FP64 sum with TwoSum-carried errors.
+__global__ __launch_bounds__(kBlock) void k_sum_blocks(const double* x, int64_t n, int64_t per_block,
+                                                        double2* part)
+{
+  __shared__ double2 lds[kBlock / kWave];
+  const int64_t b0 = (int64_t)blockIdx.x * per_block;
+  const int64_t b1 = b0 + per_block < n ? b0 + per_block : n;
+  double s = 0.0, c = 0.0;
+  for (int64_t i = b0 + threadIdx.x; i < b1; i += kBlock) dd_add(s, c, x[i], 0.0);
+#pragma unroll
+  for (int mask = 1; mask < kWave; mask <<= 1) {
+    const double os = shfl_xor_d(s, mask), oc = shfl_xor_d(c, mask);
+    dd_add(s, c, os, oc);
+  }
+  if ((threadIdx.x & (kWave - 1)) == 0) lds[threadIdx.x / kWave] = make_double2(s, c);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double2 t = lds[0];
+    for (int w = 1; w < kBlock / kWave; ++w) dd_add(t.x, t.y, lds[w].x, lds[w].y);
+    part[blockIdx.x] = t;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_sum_final(const double2* part, int64_t nb, double* out)
+{
+  __shared__ double2 lds[kBlock / kWave];
+  const int64_t chunk = (nb + kBlock - 1) / kBlock;
+  const int64_t b0 = threadIdx.x * chunk;
+  const int64_t b1 = b0 + chunk < nb ? b0 + chunk : nb;
+  double s = 0.0, c = 0.0;
+  for (int64_t b = b0; b < b1; ++b) dd_add(s, c, part[b].x, part[b].y);
+#pragma unroll
+  for (int mask = 1; mask < kWave; mask <<= 1) {
+    const double os = shfl_xor_d(s, mask), oc = shfl_xor_d(c, mask);
+    dd_add(s, c, os, oc);
+  }
+  if ((threadIdx.x & (kWave - 1)) == 0) lds[threadIdx.x / kWave] = make_double2(s, c);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double2 t = lds[0];
+    for (int w = 1; w < kBlock / kWave; ++w) dd_add(t.x, t.y, lds[w].x, lds[w].y);
+    out[0] = t.x + t.y;
+  }
+}
+
+}  // namespace cuhre
+
+using namespace cuhre;
+
+// ------------------------------------------------------------------ context
+
+struct cuhre_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int64_t capacity = 64ll << 20;  // regions
+  int profile = 0;
+  int lanes = 0;
+  int64_t lane_threshold = 131072;  // auto: one region per lane at/above this n
+
+  int64_t cap = 0;  // allocated regions per list buffer
+  int dim_alloc = 0;
+  double* lo[2] = {nullptr, nullptr};
+  double* wd[2] = {nullptr, nullptr};
+  double* est = nullptr;
+  double* err = nullptr;
+  double* abssum = nullptr;
+  int8_t* axis = nullptr;
+  Partial* partials = nullptr;
+  int64_t* offsets = nullptr;
+  uint32_t* ticket = nullptr;
+  IterState* state = nullptr;
+  IterState* h_state = nullptr;
+  double* aux = nullptr;  // i5 (cap) + dvec (cap*dim) on request
+  int64_t aux_cap = 0;
+  double2* sum_part = nullptr;
+  double* sum_out = nullptr;
+
+  double* table = nullptr;
+  std::vector<double> table_host;
+
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+};
+
+static int64_t blocks_for(int64_t n, int lanes) { return (n + kBlock / lanes - 1) / (kBlock / lanes); }
+
+static int ctx_free_lists(cuhre_context* c)
+{
+  for (int i = 0; i < 2; ++i) {
+    if (c->lo[i]) (void)hipFree(c->lo[i]);
+    if (c->wd[i]) (void)hipFree(c->wd[i]);
+    c->lo[i] = c->wd[i] = nullptr;
+  }
+  if (c->est) (void)hipFree(c->est);
+  if (c->err) (void)hipFree(c->err);
+  if (c->abssum) (void)hipFree(c->abssum);
+  if (c->axis) (void)hipFree(c->axis);
+  if (c->partials) (void)hipFree(c->partials);
+  if (c->offsets) (void)hipFree(c->offsets);
+  c->est = c->err = c->abssum = nullptr;
+  c->axis = nullptr;
+  c->partials = nullptr;
+  c->offsets = nullptr;
+  c->cap = 0;
+  return 0;
+}
+
+// Allocates list buffers for `cap` regions of `dim` coordinates.  When
+// keep_n > 0 the first keep_n regions of list `cur` (and their est/err/axis,
+// and keep_nb block offsets) are carried over into the new buffers.
+static int ctx_reserve(cuhre_context* c, int64_t cap, int dim, int64_t keep_n = 0, int cur = 0,
+                       int64_t keep_nb = 0)
+{
+  if (cap <= c->cap && dim <= c->dim_alloc) return 0;
+  cap = std::max(cap, c->cap);
+  dim = std::max(dim, c->dim_alloc);
+  cap = (cap + 255) & ~255ll;
+  const int64_t nbmax = cap / (kBlock / kWave) + 1;
+  double *lo[2], *wd[2], *est, *err, *ab;
+  int8_t* ax;
+  Partial* part;
+  int64_t* off;
+  const size_t lb = sizeof(double) * (size_t)cap * dim;
+  if (hipMalloc(&lo[0], lb) != hipSuccess || hipMalloc(&lo[1], lb) != hipSuccess ||
+      hipMalloc(&wd[0], lb) != hipSuccess || hipMalloc(&wd[1], lb) != hipSuccess ||
+      hipMalloc(&est, sizeof(double) * cap) != hipSuccess ||
+      hipMalloc(&err, sizeof(double) * cap) != hipSuccess ||
+      hipMalloc(&ab, sizeof(double) * cap) != hipSuccess || hipMalloc(&ax, cap) != hipSuccess ||
+      hipMalloc(&part, sizeof(Partial) * nbmax) != hipSuccess ||
+      hipMalloc(&off, sizeof(int64_t) * nbmax) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(CUHRE_ERR_ALLOC, "cannot allocate a region list of %lld regions x %d dims",
+                (long long)cap, dim);
+  }
+  if (keep_n > 0) {
+    const int dk = c->dim_alloc;
+    HIP_TRY(hipMemcpy2DAsync(lo[cur], sizeof(double) * cap, c->lo[cur], sizeof(double) * c->cap,
+                             sizeof(double) * keep_n, dk, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(hipMemcpy2DAsync(wd[cur], sizeof(double) * cap, c->wd[cur], sizeof(double) * c->cap,
+                             sizeof(double) * keep_n, dk, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(est, c->est, sizeof(double) * keep_n, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(err, c->err, sizeof(double) * keep_n, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(ax, c->axis, keep_n, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(off, c->offsets, sizeof(int64_t) * keep_nb, hipMemcpyDeviceToDevice,
+                           c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  }
+  ctx_free_lists(c);
+  c->lo[0] = lo[0];
+  c->lo[1] = lo[1];
+  c->wd[0] = wd[0];
+  c->wd[1] = wd[1];
+  c->est = est;
+  c->err = err;
+  c->abssum = ab;
+  c->axis = ax;
+  c->partials = part;
+  c->offsets = off;
+  c->cap = cap;
+  c->dim_alloc = dim;
+  return 0;
+}
+
+static int ctx_reserve_aux(cuhre_context* c, int64_t cap, int dim)
+{
+  const int64_t need = cap * (1 + dim);
+  if (need <= c->aux_cap) return 0;
+  if (c->aux) (void)hipFree(c->aux);
+  c->aux = nullptr;
+  c->aux_cap = 0;
+  HIP_TRY(hipMalloc(&c->aux, sizeof(double) * need));
+  c->aux_cap = need;
+  return 0;
+}
+
+static int make_dev_integrand(cuhre_context* c, const cuhre_integrand* f, int dim, DevIntegrand* P)
+{
+  std::memset(P, 0, sizeof *P);
+  P->family = f->family;
+  P->dim = dim;
+  P->exponent = f->exponent;
+  P->nterms = f->nterms;
+  if (f->nterms < 0 || f->nterms > CUHRE_MAXTERMS) return fail(CUHRE_ERR_ARG, "nterms out of range");
+  for (int i = 0; i < kMaxDim; ++i) {
+    P->a[i] = f->a[i];
+    P->w[i] = f->w[i];
+    P->b[i] = (i < dim && f->family == CUHRE_PRODUCT_PEAK) ? 1.0 / (f->a[i] * f->a[i]) : 0.0;
+  }
+  P->s0 = 2.0 * M_PI * f->u;
+  for (int t = 0; t < CUHRE_MAXTERMS; ++t) {
+    P->coef[t] = f->coef[t];
+    for (int i = 0; i < kMaxDim; ++i) {
+      const int e = f->powers[t][i];
+      if (t < f->nterms && (e < 0 || e > 127)) return fail(CUHRE_ERR_ARG, "polynomial power out of range");
+      P->powers[t][i] = (int8_t)e;
+    }
+  }
+  if (f->family == CUHRE_CORNER_PEAK && f->exponent < 0)
+    return fail(CUHRE_ERR_ARG, "corner peak exponent must be >= 0");
+  if (f->family == CUHRE_TABLE6) {
+    if (!f->table || f->table_nx < 2 || f->table_ny < 2)
+      return fail(CUHRE_ERR_ARG, "table integrand needs a table of at least 2x2");
+    const size_t cnt = (size_t)f->table_nx * f->table_ny;
+    const bool same = c->table && c->table_host.size() == cnt &&
+                      std::memcmp(c->table_host.data(), f->table, cnt * sizeof(double)) == 0;
+    if (!same) {
+      if (c->table) (void)hipFree(c->table);
+      c->table = nullptr;
+      c->table_host.assign(f->table, f->table + cnt);
+      HIP_TRY(hipMalloc(&c->table, cnt * sizeof(double)));
+      HIP_TRY(hipMemcpyAsync(c->table, f->table, cnt * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    }
+    P->table = c->table;
+    P->nx = f->table_nx;
+    P->ny = f->table_ny;
+  }
+  return 0;
+}
+
+static int pick_lanes(const cuhre_context* c, int64_t n)
+{
+  if (c->lanes == 1 || c->lanes == kWave) return c->lanes;
+  return n >= c->lane_threshold ? 1 : kWave;
+}
+
+static int grid_k(int d) { return d <= 4 ? 4 : (d <= 10 ? 2 : 1); }
+
+// Initial uniform k^d grid over the box (GPUKernelquad.cu:23-61, 300-316),
+// region r has coordinate index (r / k^m) % k along axis m.
+static int64_t initial_grid(int d, const double* lows, const double* highs, std::vector<double>& lo,
+                            std::vector<double>& wd)
+{
+  const int k = grid_k(d);
+  int64_t n = 1;
+  for (int m = 0; m < d; ++m) n *= k;
+  lo.resize((size_t)n * d);
+  wd.resize((size_t)n * d);
+  for (int64_t r = 0; r < n; ++r) {
+    int64_t q = r;
+    for (int m = 0; m < d; ++m) {
+      const int64_t id = q % k;
+      q /= k;
+      const double w = (highs[m] - lows[m]) / (double)k;
+      lo[(size_t)m * n + r] = lows[m] + (double)id * w;
+      wd[(size_t)m * n + r] = w;
+    }
+  }
+  return n;
+}
+
+static int check_common(cuhre_context* ctx, const cuhre_integrand* f, int dim, int key)
+{
+  if (!ctx) return fail(CUHRE_ERR_ARG, "null context");
+  if (!f) return fail(CUHRE_ERR_ARG, "null integrand");
+  if (dim < 2 || dim > 10) return fail(CUHRE_ERR_UNSUPPORTED, "dimension %d outside 2..10", dim);
+  if (f->dim != dim) return fail(CUHRE_ERR_ARG, "integrand dim %d != dim %d", f->dim, dim);
+  if (key != 0 && key != 7) return fail(CUHRE_ERR_UNSUPPORTED, "rule key %d not available (0 or 7)", key);
+  if (!find_eval_launcher(f->family, dim, 1))
+    return fail(CUHRE_ERR_UNSUPPORTED, "family %d not compiled for dim %d", f->family, dim);
+  HIP_TRY(hipSetDevice(ctx->device));
+  return 0;
+}
+
+static void fill_eval_args(cuhre_context* c, EvalArgs& A, int cur, int64_t n, int dim, double epsrel,
+                           double epsabs)
+{
+  std::memset(&A, 0, sizeof A);
+  A.lower = c->lo[cur];
+  A.width = c->wd[cur];
+  A.ld = c->cap;
+  A.n = n;
+  A.epsrel = epsrel;
+  A.epsabs = epsabs;
+  A.est = c->est;
+  A.err = c->err;
+  A.axis = c->axis;
+  A.abssum = c->abssum;
+  A.partials = c->partials;
+  A.offsets = c->offsets;
+  A.ticket = c->ticket;
+  A.state = c->state;
+  (void)dim;
+}
+
+static void fill_split_args(cuhre_context* c, SplitArgs& S, int cur, int64_t n, int dim, double epsrel,
+                            double epsabs, int rpb, bool use_state)
+{
+  S.lower = c->lo[cur];
+  S.width = c->wd[cur];
+  S.ld = c->cap;
+  S.n = n;
+  S.est = c->est;
+  S.err = c->err;
+  S.axis = c->axis;
+  S.epsrel = epsrel;
+  S.epsabs = epsabs;
+  S.offsets = c->offsets;
+  S.rpb = rpb;
+  S.dim = dim;
+  S.out_lower = c->lo[1 - cur];
+  S.out_width = c->wd[1 - cur];
+  S.out_ld = c->cap;
+  S.state = use_state ? c->state : nullptr;
+}
+
+static void dd_add_host(double& s1, double& c1, double s2, double c2)
+{
+  const double s = s1 + s2;
+  const double bb = s - s1;
+  const double e = (s1 - (s - bb)) + (s2 - bb);
+  s1 = s;
+  c1 = c1 + c2 + e;
+}
+
+extern "C" {
+
+const char* cuhre_last_error(void) { return g_err.c_str(); }
+
+int64_t cuhre_points_per_region(int dim, int key)
+{
+  if (dim < 2 || dim > kMaxDim || (key != 0 && key != 7)) return -1;
+  return gm_points(dim);
+}
+
+int cuhre_context_create(cuhre_context** out, int device, int64_t capacity_regions)
+{
+  if (!out) return fail(CUHRE_ERR_ARG, "null output pointer");
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    (void)hipGetLastError();
+    return fail(CUHRE_ERR_DEVICE, "no HIP device available");
+  }
+  if (device < 0 || device >= ndev) return fail(CUHRE_ERR_ARG, "device %d out of range (%d)", device, ndev);
+  cuhre_context* c = new cuhre_context();
+  c->device = device;
+  if (capacity_regions > 0) c->capacity = capacity_regions;
+  int rc = 0;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc(&c->ticket, sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc(&c->state, sizeof(IterState)) != hipSuccess ||
+      hipHostMalloc(&c->h_state, sizeof(IterState), hipHostMallocDefault) != hipSuccess ||
+      hipMemset(c->ticket, 0, sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc(&c->sum_out, sizeof(double)) != hipSuccess) {
+    rc = fail(CUHRE_ERR_DEVICE, "context setup failed: %s", hipGetErrorString(hipGetLastError()));
+  }
+  for (int i = 0; i < 4 && rc == 0; ++i)
+    if (hipEventCreate(&c->ev[i]) != hipSuccess) rc = fail(CUHRE_ERR_DEVICE, "event create failed");
+  if (rc != 0) {
+    cuhre_context_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return 0;
+}
+
+int cuhre_context_destroy(cuhre_context* c)
+{
+  if (!c) return 0;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  ctx_free_lists(c);
+  if (c->aux) (void)hipFree(c->aux);
+  if (c->ticket) (void)hipFree(c->ticket);
+  if (c->state) (void)hipFree(c->state);
+  if (c->h_state) (void)hipHostFree(c->h_state);
+  if (c->sum_part) (void)hipFree(c->sum_part);
+  if (c->sum_out) (void)hipFree(c->sum_out);
+  if (c->table) (void)hipFree(c->table);
+  for (int i = 0; i < 4; ++i)
+    if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return 0;
+}
+
+int cuhre_context_set(cuhre_context* c, int profile, int lanes_per_region)
+{
+  if (!c) return fail(CUHRE_ERR_ARG, "null context");
+  if (lanes_per_region != 0 && lanes_per_region != 1 && lanes_per_region != kWave)
+    return fail(CUHRE_ERR_ARG, "lanes_per_region must be 0, 1 or 64");
+  c->profile = profile;
+  c->lanes = lanes_per_region;
+  return 0;
+}
+
+int cuhre_integrate(cuhre_context* c, const cuhre_integrand* f, int dim, double epsrel, double epsabs,
+                    int64_t maxeval, int key, const double* lows, const double* highs, cuhre_result* out)
+{
+  const auto t0 = std::chrono::steady_clock::now();
+  if (!out) return fail(CUHRE_ERR_ARG, "null result");
+  std::memset(out, 0, sizeof *out);
+  int rc = check_common(c, f, dim, key);
+  if (rc) return rc;
+  if (!lows || !highs) return fail(CUHRE_ERR_ARG, "null bounds");
+  if (!(epsrel >= 0.0) || !(epsabs >= 0.0)) return fail(CUHRE_ERR_ARG, "tolerances must be >= 0");
+  for (int m = 0; m < dim; ++m)
+    if (!(highs[m] > lows[m]) || !std::isfinite(lows[m]) || !std::isfinite(highs[m]))
+      return fail(CUHRE_ERR_ARG, "empty or non-finite box along axis %d", m);
+
+  DevIntegrand P;
+  if ((rc = make_dev_integrand(c, f, dim, &P))) return rc;
+  const RuleConst R = make_rule(dim);
+  const int64_t p = R.npts;
+  const double rflops = region_flops(f->family, dim);
+
+  std::vector<double> hlo, hwd;
+  const int64_t n0 = initial_grid(dim, lows, highs, hlo, hwd);
+  if (n0 > c->capacity) return fail(CUHRE_ERR_CAPACITY, "initial grid exceeds capacity");
+  const int64_t want = std::min<int64_t>(c->capacity, std::max<int64_t>(4 * n0, 1 << 16));
+  if ((rc = ctx_reserve(c, want, dim))) return rc;
+  int cur = 0;
+  HIP_TRY(hipMemcpy2DAsync(c->lo[cur], sizeof(double) * c->cap, hlo.data(), sizeof(double) * n0,
+                           sizeof(double) * n0, dim, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpy2DAsync(c->wd[cur], sizeof(double) * c->cap, hwd.data(), sizeof(double) * n0,
+                           sizeof(double) * n0, dim, hipMemcpyHostToDevice, c->stream));
+
+  double fin_s = 0.0, fin_c = 0.0, fer_s = 0.0, fer_c = 0.0, fin_abs = 0.0, fin_aest = 0.0;
+  int64_t nfin = 0, n = n0;
+  const double u = 0.5 * 2.220446049250313e-16;
+  int result = CUHRE_NOT_CONVERGED;
+  for (int it = 0; it < 1000; ++it) {
+    const int lanes = pick_lanes(c, n);
+    const EvalLauncher launch = find_eval_launcher(f->family, dim, lanes);
+    EvalArgs A;
+    fill_eval_args(c, A, cur, n, dim, epsrel, epsabs);
+    A.finalize = 1;
+    A.fin_est_s = fin_s;
+    A.fin_est_c = fin_c;
+    A.fin_err_s = fer_s;
+    A.fin_err_c = fer_c;
+    const int64_t left = maxeval - (out->neval + n * p);
+    A.max_active = std::min<int64_t>(left >= 0 ? left / (2 * p) : -1, c->capacity / 2);
+    A.alloc_active = c->cap / 2;
+    SplitArgs S;
+    fill_split_args(c, S, cur, n, dim, epsrel, epsabs, kBlock / lanes, true);
+    const int64_t nb = blocks_for(n, lanes);
+    if (nb > 0x7fffffffll) return fail(CUHRE_ERR_CAPACITY, "grid too large");
+
+    if (c->profile) HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+    launch(A, P, R, c->stream);
+    if (c->profile) HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+    hipLaunchKernelGGL(k_split, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, c->stream, S);
+    if (c->profile) HIP_TRY(hipEventRecord(c->ev[2], c->stream));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(c->h_state, c->state, sizeof(IterState), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    IterState st = *c->h_state;
+    if (c->profile) {
+      float ms_e = 0.f, ms_s = 0.f;
+      HIP_TRY(hipEventElapsedTime(&ms_e, c->ev[0], c->ev[1]));
+      HIP_TRY(hipEventElapsedTime(&ms_s, c->ev[1], c->ev[2]));
+      out->ms_eval += ms_e;
+      out->ms_split += ms_s;
+      if (lanes == 1) out->ms_eval_lane += ms_e;
+    }
+    if (lanes == 1) {
+      out->eval_lane_launches++;
+      out->lane_regions += n;
+    }
+    out->eval_launches++;
+    out->eval_flops += rflops * (double)n;
+    if (st.stop == 5) {
+      // list buffers too small for the children: grow (keeping this
+      // iteration's list and results) and run the split again
+      const int64_t A2 = 2 * st.tot.nactive;
+      const int64_t ncap = std::min<int64_t>(c->capacity, std::max<int64_t>(A2, 2 * c->cap));
+      if ((rc = ctx_reserve(c, ncap, dim, n, cur, nb))) return rc;
+      fill_split_args(c, S, cur, n, dim, epsrel, epsabs, kBlock / lanes, false);
+      if (c->profile) HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+      hipLaunchKernelGGL(k_split, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, c->stream, S);
+      if (c->profile) HIP_TRY(hipEventRecord(c->ev[2], c->stream));
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipStreamSynchronize(c->stream));
+      if (c->profile) {
+        float ms_s = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms_s, c->ev[1], c->ev[2]));
+        out->ms_split += ms_s;
+      }
+      st.stop = 0;
+    }
+    out->split_launches++;
+    out->iterations = it + 1;
+    out->nregions += n;
+    out->neval += n * p;
+    out->peak_regions = std::max(out->peak_regions, n);
+    const double abs_contrib = fin_abs + st.tot.abs_all;
+    const double aest_contrib = fin_aest + st.tot.aest_all;
+    const int64_t ncontrib = nfin + n;
+    dd_add_host(fin_s, fin_c, st.tot.fest_s, st.tot.fest_c);
+    dd_add_host(fer_s, fer_c, st.tot.ferr_s, st.tot.ferr_c);
+    fin_abs += st.tot.abs_fin;
+    fin_aest += st.tot.aest_fin;
+    nfin += n - st.tot.nactive;
+    out->nfinished = nfin;
+    out->integral = st.rG_s + st.rG_c;
+    out->error = st.eG_s + st.eG_c;
+    out->fp64_err_bound = (double)(p + 4) * u * abs_contrib + 2.0 * u * std::fabs(out->integral) +
+                          (double)ncontrib * u * u * aest_contrib;
+    if (st.stop == 4) {
+      result = fail(CUHRE_ERR_NONFINITE, "integrand returned a non-finite value in %lld region(s)",
+                    (long long)st.tot.nnonfinite);
+      break;
+    }
+    if (st.stop == 1) {
+      result = CUHRE_OK;
+      break;
+    }
+    if (st.stop == 2 || st.stop == 3) {
+      result = CUHRE_NOT_CONVERGED;
+      break;
+    }
+    n = 2 * st.tot.nactive;
+    cur = 1 - cur;
+  }
+  out->fail = result > 0 ? result : 0;
+  out->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return result;
+}
+
+int cuhre_eval_regions(cuhre_context* c, const cuhre_integrand* f, int dim, int key, int64_t n,
+                       const double* lower, const double* width, double* est, double* err,
+                       int32_t* axis, double* abssum, double* i5, double* dvec)
+{
+  int rc = check_common(c, f, dim, key);
+  if (rc) return rc;
+  if (n <= 0) return 0;
+  if (!lower || !width || !est || !err || !axis || !abssum) return fail(CUHRE_ERR_ARG, "null buffer");
+  DevIntegrand P;
+  if ((rc = make_dev_integrand(c, f, dim, &P))) return rc;
+  const RuleConst R = make_rule(dim);
+  if ((rc = ctx_reserve(c, n, dim))) return rc;
+  const bool want_aux = i5 != nullptr || dvec != nullptr;
+  if (want_aux && (rc = ctx_reserve_aux(c, c->cap, dim))) return rc;
+  HIP_TRY(hipMemcpy2DAsync(c->lo[0], sizeof(double) * c->cap, lower, sizeof(double) * n,
+                           sizeof(double) * n, dim, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpy2DAsync(c->wd[0], sizeof(double) * c->cap, width, sizeof(double) * n,
+                           sizeof(double) * n, dim, hipMemcpyHostToDevice, c->stream));
+  const int lanes = pick_lanes(c, n);
+  EvalArgs A;
+  fill_eval_args(c, A, 0, n, dim, 0.0, 0.0);
+  A.finalize = 0;
+  if (want_aux) {
+    A.i5 = c->aux;
+    A.dvec = c->aux + c->cap;
+  }
+  find_eval_launcher(f->family, dim, lanes)(A, P, R, c->stream);
+  HIP_TRY(hipGetLastError());
+  std::vector<int8_t> ax8((size_t)n);
+  HIP_TRY(hipMemcpyAsync(est, c->est, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(err, c->err, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(abssum, c->abssum, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(ax8.data(), c->axis, n, hipMemcpyDeviceToHost, c->stream));
+  if (i5) HIP_TRY(hipMemcpyAsync(i5, c->aux, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  if (dvec)
+    HIP_TRY(hipMemcpy2DAsync(dvec, sizeof(double) * n, c->aux + c->cap, sizeof(double) * c->cap,
+                             sizeof(double) * n, dim, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  for (int64_t r = 0; r < n; ++r) axis[r] = ax8[r];
+  return 0;
+}
+
+int cuhre_split_regions(cuhre_context* c, int dim, int64_t n, const double* lower, const double* width,
+                        const double* est, const double* err, const int32_t* axis, double epsrel,
+                        double epsabs, double* out_lower, double* out_width, int64_t* n_active)
+{
+  if (!c) return fail(CUHRE_ERR_ARG, "null context");
+  if (dim < 1 || dim > kMaxDim) return fail(CUHRE_ERR_ARG, "dimension %d out of range", dim);
+  if (!n_active) return fail(CUHRE_ERR_ARG, "null n_active");
+  *n_active = 0;
+  if (n <= 0) return 0;
+  if (!lower || !width || !est || !err || !axis || !out_lower || !out_width)
+    return fail(CUHRE_ERR_ARG, "null buffer");
+  for (int64_t r = 0; r < n; ++r)
+    if (axis[r] < 0 || axis[r] >= dim) return fail(CUHRE_ERR_ARG, "axis[%lld] out of range", (long long)r);
+  HIP_TRY(hipSetDevice(c->device));
+  int rc;
+  if ((rc = ctx_reserve(c, 2 * n, dim))) return rc;
+  std::vector<int8_t> ax8((size_t)n);
+  for (int64_t r = 0; r < n; ++r) ax8[r] = (int8_t)axis[r];
+  HIP_TRY(hipMemcpy2DAsync(c->lo[0], sizeof(double) * c->cap, lower, sizeof(double) * n,
+                           sizeof(double) * n, dim, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpy2DAsync(c->wd[0], sizeof(double) * c->cap, width, sizeof(double) * n,
+                           sizeof(double) * n, dim, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(c->est, est, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(c->err, err, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(c->axis, ax8.data(), n, hipMemcpyHostToDevice, c->stream));
+  const int64_t nb = (n + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(k_count, dim3((unsigned)nb), dim3(kBlock), 0, c->stream, c->est, c->err, n, epsrel,
+                     epsabs, c->offsets);
+  HIP_TRY(hipGetLastError());
+  std::vector<int64_t> counts((size_t)nb);
+  HIP_TRY(hipMemcpyAsync(counts.data(), c->offsets, sizeof(int64_t) * nb, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  int64_t run = 0;
+  for (int64_t b = 0; b < nb; ++b) {
+    const int64_t t = counts[b];
+    counts[b] = run;
+    run += t;
+  }
+  HIP_TRY(hipMemcpyAsync(c->offsets, counts.data(), sizeof(int64_t) * nb, hipMemcpyHostToDevice, c->stream));
+  SplitArgs S;
+  fill_split_args(c, S, 0, n, dim, epsrel, epsabs, kBlock, false);
+  hipLaunchKernelGGL(k_split, dim3((unsigned)nb), dim3(kBlock), 0, c->stream, S);
+  HIP_TRY(hipGetLastError());
+  if (run > 0) {
+    HIP_TRY(hipMemcpy2DAsync(out_lower, sizeof(double) * 2 * run, c->lo[1], sizeof(double) * c->cap,
+                             sizeof(double) * 2 * run, dim, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpy2DAsync(out_width, sizeof(double) * 2 * run, c->wd[1], sizeof(double) * c->cap,
+                             sizeof(double) * 2 * run, dim, hipMemcpyDeviceToHost, c->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  *n_active = run;
+  return 0;
+}
+
+int cuhre_sum(cuhre_context* c, int64_t n, const double* x, double* out)
+{
+  if (!c || !out) return fail(CUHRE_ERR_ARG, "null argument");
+  *out = 0.0;
+  if (n <= 0) return 0;
+  if (!x) return fail(CUHRE_ERR_ARG, "null input");
+  HIP_TRY(hipSetDevice(c->device));
+  int rc;
+  if ((rc = ctx_reserve(c, n, 1))) return rc;
+  HIP_TRY(hipMemcpyAsync(c->est, x, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  const int64_t per = 4096;
+  const int64_t nb = (n + per - 1) / per;
+  if (c->sum_part) (void)hipFree(c->sum_part);
+  c->sum_part = nullptr;
+  HIP_TRY(hipMalloc(&c->sum_part, sizeof(double2) * nb));
+  hipLaunchKernelGGL(k_sum_blocks, dim3((unsigned)nb), dim3(kBlock), 0, c->stream, c->est, n, per, c->sum_part);
+  hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(kBlock), 0, c->stream, c->sum_part, nb, c->sum_out);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(out, c->sum_out, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+}  // extern "C"

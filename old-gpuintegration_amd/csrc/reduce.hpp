@@ -185,6 +185,7 @@ __device__ __forceinline__ void finish_iteration(const EvalArgs& A, Partial& p, 
     else if (eG <= fmax(A.epsrel * fabs(rG), A.epsabs)) stop = 1;
     else if (q.nactive == 0) stop = 2;
     else if (q.nactive > A.max_active) stop = 3;
+    else if (q.nactive > A.alloc_active) stop = 5;
     st.stop = stop;
     st.pad = 0;
     *A.state = st;

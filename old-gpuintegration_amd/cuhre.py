@@ -1,0 +1,240 @@
+"""ctypes binding of libcuhre_amd.so (the C-ABI declared in include/cuhre.h).
+
+This is the Python view of the drop-in boundary used by tests and bench.py.
+It never falls back to anything: if the HIP library is missing or no GPU is
+present, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+import workloads as W
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libcuhre_amd.so")
+MAXDIM = 16
+MAXTERMS = 8
+
+OK = 0
+NOT_CONVERGED = 1
+ERR_ARG = -1
+ERR_NONFINITE = -2
+ERR_ALLOC = -3
+ERR_DEVICE = -4
+ERR_UNSUPPORTED = -5
+ERR_CAPACITY = -6
+
+# Symbols every build exports (kept in step with include/cuhre.h).
+EXPORTS = [
+    "cuhre_context_create", "cuhre_context_destroy", "cuhre_context_set",
+    "cuhre_integrate", "cuhre_eval_regions", "cuhre_split_regions", "cuhre_sum",
+    "cuhre_last_error", "cuhre_points_per_region",
+]
+
+
+class Integrand(C.Structure):
+    _fields_ = [
+        ("family", C.c_int32),
+        ("dim", C.c_int32),
+        ("a", C.c_double * MAXDIM),
+        ("w", C.c_double * MAXDIM),
+        ("u", C.c_double),
+        ("exponent", C.c_int32),
+        ("nterms", C.c_int32),
+        ("coef", C.c_double * MAXTERMS),
+        ("powers", (C.c_int32 * MAXDIM) * MAXTERMS),
+        ("table", C.POINTER(C.c_double)),
+        ("table_nx", C.c_int32),
+        ("table_ny", C.c_int32),
+    ]
+
+
+class Result(C.Structure):
+    _fields_ = [
+        ("integral", C.c_double),
+        ("error", C.c_double),
+        ("fp64_err_bound", C.c_double),
+        ("nregions", C.c_int64),
+        ("neval", C.c_int64),
+        ("fail", C.c_int32),
+        ("iterations", C.c_int32),
+        ("peak_regions", C.c_int64),
+        ("nfinished", C.c_int64),
+        ("ms_total", C.c_double),
+        ("ms_eval", C.c_double),
+        ("ms_split", C.c_double),
+        ("eval_launches", C.c_int32),
+        ("split_launches", C.c_int32),
+        ("eval_flops", C.c_double),
+        ("ms_eval_lane", C.c_double),
+        ("eval_lane_launches", C.c_int32),
+        ("pad0", C.c_int32),
+        ("lane_regions", C.c_int64),
+    ]
+
+    def asdict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class CuhreError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"cuhre error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise RuntimeError(f"{LIBPATH} is not built (run __graft_entry__.build())")
+        L = C.CDLL(LIBPATH)
+        dp = C.POINTER(C.c_double)
+        ctxp = C.c_void_p
+        L.cuhre_context_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_int64]
+        L.cuhre_context_destroy.argtypes = [ctxp]
+        L.cuhre_context_set.argtypes = [ctxp, C.c_int, C.c_int]
+        L.cuhre_integrate.argtypes = [ctxp, C.POINTER(Integrand), C.c_int, C.c_double, C.c_double,
+                                      C.c_int64, C.c_int, dp, dp, C.POINTER(Result)]
+        L.cuhre_eval_regions.argtypes = [ctxp, C.POINTER(Integrand), C.c_int, C.c_int, C.c_int64,
+                                         dp, dp, dp, dp, C.POINTER(C.c_int32), dp, dp, dp]
+        L.cuhre_split_regions.argtypes = [ctxp, C.c_int, C.c_int64, dp, dp, dp, dp,
+                                          C.POINTER(C.c_int32), C.c_double, C.c_double, dp, dp,
+                                          C.POINTER(C.c_int64)]
+        L.cuhre_sum.argtypes = [ctxp, C.c_int64, dp, dp]
+        L.cuhre_last_error.restype = C.c_char_p
+        L.cuhre_points_per_region.argtypes = [C.c_int, C.c_int]
+        L.cuhre_points_per_region.restype = C.c_int64
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _check(rc):
+    if rc < 0:
+        raise CuhreError(rc, lib().cuhre_last_error().decode())
+    return rc
+
+
+def to_struct(f: W.Integrand):
+    s = Integrand()
+    s.family = f.family
+    s.dim = f.dim
+    for i, v in enumerate(f.a):
+        s.a[i] = v
+    for i, v in enumerate(f.w):
+        s.w[i] = v
+    s.u = f.u
+    s.exponent = f.exponent
+    s.nterms = len(f.coef)
+    for t, (c, pw) in enumerate(zip(f.coef, f.powers)):
+        s.coef[t] = c
+        for i, e in enumerate(pw):
+            s.powers[t][i] = e
+    keep = None
+    if f.table is not None:
+        keep = np.ascontiguousarray(f.table, dtype=np.float64)
+        s.table = _dp(keep)
+        s.table_nx, s.table_ny = keep.shape
+    return s, keep
+
+
+class Context:
+    """One engine context on one HIP device (cuhre_context_create)."""
+
+    def __init__(self, device: int = 0, capacity: int = 0, profile: bool = False, lanes: int = 0):
+        self.h = C.c_void_p()
+        _check(lib().cuhre_context_create(C.byref(self.h), device, capacity))
+        self.set(profile, lanes)
+
+    def set(self, profile: bool = False, lanes: int = 0):
+        _check(lib().cuhre_context_set(self.h, int(profile), lanes))
+
+    def close(self):
+        if self.h:
+            lib().cuhre_context_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def integrate(self, p: W.Problem, epsrel: Optional[float] = None, maxeval: Optional[int] = None,
+                  key: int = 0):
+        s, keep = to_struct(p.f)
+        lo = np.ascontiguousarray(p.lows, dtype=np.float64)
+        hi = np.ascontiguousarray(p.highs, dtype=np.float64)
+        res = Result()
+        rc = lib().cuhre_integrate(self.h, C.byref(s), p.f.dim,
+                                   p.epsrel if epsrel is None else epsrel, p.epsabs,
+                                   p.maxeval if maxeval is None else maxeval, key,
+                                   _dp(lo), _dp(hi), C.byref(res))
+        _check(rc)
+        d = res.asdict()
+        d["rc"] = rc
+        return d
+
+    def eval_regions(self, f: W.Integrand, lower, width, want_aux=False, key=0):
+        """lower/width: [n, d] arrays.  Returns est, err, axis, abssum (+ i5, dvec[n,d])."""
+        lower = np.asarray(lower, dtype=np.float64)
+        width = np.asarray(width, dtype=np.float64)
+        n, d = lower.shape
+        L = np.ascontiguousarray(lower.T)
+        Wd = np.ascontiguousarray(width.T)
+        est = np.empty(n); err = np.empty(n); ab = np.empty(n)
+        ax = np.empty(n, dtype=np.int32)
+        i5 = np.empty(n) if want_aux else None
+        dv = np.empty((d, n)) if want_aux else None
+        s, keep = to_struct(f)
+        _check(lib().cuhre_eval_regions(self.h, C.byref(s), d, key, n, _dp(L), _dp(Wd), _dp(est),
+                                        _dp(err), ax.ctypes.data_as(C.POINTER(C.c_int32)), _dp(ab),
+                                        _dp(i5) if want_aux else None,
+                                        _dp(dv) if want_aux else None))
+        if want_aux:
+            return est, err, ax, ab, i5, dv.T.copy()
+        return est, err, ax, ab
+
+    def split_regions(self, lower, width, est, err, axis, epsrel, epsabs=0.0):
+        lower = np.asarray(lower, dtype=np.float64)
+        width = np.asarray(width, dtype=np.float64)
+        n, d = lower.shape
+        L = np.ascontiguousarray(lower.T)
+        Wd = np.ascontiguousarray(width.T)
+        est = np.ascontiguousarray(est, dtype=np.float64)
+        err = np.ascontiguousarray(err, dtype=np.float64)
+        ax = np.ascontiguousarray(axis, dtype=np.int32)
+        oL = np.empty(max(1, 2 * n * d)); oW = np.empty(max(1, 2 * n * d))
+        na = C.c_int64()
+        _check(lib().cuhre_split_regions(self.h, d, n, _dp(L), _dp(Wd), _dp(est), _dp(err),
+                                         ax.ctypes.data_as(C.POINTER(C.c_int32)), epsrel, epsabs,
+                                         _dp(oL), _dp(oW), C.byref(na)))
+        A = na.value
+        m = 2 * A
+        return A, oL[: m * d].reshape(d, m).T.copy(), oW[: m * d].reshape(d, m).T.copy()
+
+    def sum(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = C.c_double()
+        _check(lib().cuhre_sum(self.h, x.size, _dp(x), C.byref(out)))
+        return out.value
+
+
+def points_per_region(dim: int, key: int = 0) -> int:
+    return lib().cuhre_points_per_region(dim, key)

@@ -197,7 +197,7 @@ void or_region_eval(const or_rule* r, const or_integrand* f,
                     double* dvec)
 {
   const int d = r->dim, p = r->npts;
-  double c[OR_MAXDIM], h[OR_MAXDIM], x[OR_MAXDIM];
+  double c[OR_MAXDIM] = {0}, h[OR_MAXDIM], x[OR_MAXDIM];
   double vol = 1.0, maxw = 0.0;
   int widest = 0;
   for (int m = 0; m < d; ++m) {

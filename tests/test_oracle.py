@@ -1,0 +1,263 @@
+"""Pins the CPU oracle (the checker) before anything is compared against it.
+
+Each check is chosen so that a plausible slip in the oracle -- a wrong
+generator radius, weight, sign, index or scaling -- fails it:
+  * rule: point counts, weight sums, exactness on every monomial up to the
+    rule's degree and failure one degree above;
+  * per-region values: SciPy's independent Genz-Malik implementation (golden
+    fixture), exact rational integrals of polynomials;
+  * fourth differences: the closed form D_i = |c_i| h_i^4 * 972/4900 for
+    sum c_i x_i^4 and the widest-axis fallback for cubics
+    (quad/GPUquad/GPUQuadSample.cu:113-125, 150-172);
+  * closed forms of every family: adaptive quadrature (SciPy) in 1-3 D;
+  * loop: convergence on the 3-D Gaussian workload against its erf form,
+    budget stop, grid/bisection tiling.
+"""
+import itertools
+import json
+import math
+import os
+from fractions import Fraction
+
+import numpy as np
+import pytest
+from scipy import integrate
+
+import oracle as O
+import workloads as W
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def test_splitmix64_known_answers():
+    # published first outputs of SplitMix64 seeded with 0
+    r = W.SplitMix64(0)
+    assert [r.next_u64() for _ in range(3)] == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4,
+                                                0x06C45D188009454F]
+    u = W.SplitMix64(7)
+    vals = [u.uniform() for _ in range(1000)]
+    assert min(vals) >= 0.0 and max(vals) < 1.0
+
+
+@pytest.mark.parametrize("d", range(2, 11))
+def test_rule_point_count_and_weights(d):
+    g, w7, w5 = O.rule(d)
+    assert len(w7) == (1 << d) + 2 * d * d + 2 * d + 1
+    assert {3: 33, 5: 93, 6: 149, 7: 241, 8: 401}.get(d, len(w7)) == len(w7)
+    assert abs(w7.sum() - 1.0) < 1e-14
+    assert abs(w5.sum() - 1.0) < 1e-14
+    assert np.all(w5[-(1 << d):] == 0.0)
+    assert np.all(np.abs(g) <= 1.0)
+
+
+def _monomial_exact(alpha):
+    # mean of x^alpha over [-1,1]^d
+    v = Fraction(1)
+    for a in alpha:
+        if a % 2:
+            return Fraction(0)
+        v *= Fraction(1, a + 1)
+    return v
+
+
+@pytest.mark.parametrize("d", [2, 3, 4, 5])
+def test_rule_polynomial_degree(d):
+    g, w7, w5 = O.rule(d)
+    for deg in range(0, 9):
+        for alpha in itertools.product(range(deg + 1), repeat=d):
+            if sum(alpha) != deg:
+                continue
+            val7 = float(np.sum(w7 * np.prod(g ** np.array(alpha), axis=1)))
+            val5 = float(np.sum(w5 * np.prod(g ** np.array(alpha), axis=1)))
+            ex = float(_monomial_exact(alpha))
+            if deg <= 7:
+                assert abs(val7 - ex) < 1e-14, (alpha, val7, ex)
+            if deg <= 5:
+                assert abs(val5 - ex) < 1e-14, (alpha, val5, ex)
+    # one degree higher is not integrated exactly (the rule is not over-fitted)
+    a8 = (8,) + (0,) * (d - 1)
+    assert abs(np.sum(w7 * g[:, 0] ** 8) - 1.0 / 9.0) > 1e-4
+    a6 = (6,) + (0,) * (d - 1)
+    assert abs(np.sum(w5 * g[:, 0] ** 6) - 1.0 / 7.0) > 1e-4
+    del a8, a6
+
+
+def test_oracle_matches_scipy_golden():
+    g = json.load(open(os.path.join(GOLDEN, "gm_scipy_regions.json")))
+    n = 0
+    for c in g["cases"]:
+        f = W.Integrand(c["family"], c["dim"], c["a"], c["w"], u=c["u"], exponent=c["exponent"])
+        L = np.array([r["lower"] for r in c["regions"]])
+        Wd = np.array([r["width"] for r in c["regions"]])
+        est, err, ax, ab, i5, dv = O.region_eval(f, L, Wd)
+        for k, r in enumerate(c["regions"]):
+            assert abs(est[k] - r["est"]) <= 1e-13 * ab[k]
+            assert abs(err[k] - r["err"]) <= 1e-13 * ab[k]
+            n += 1
+    assert n == 120
+
+
+def _poly_exact_fraction(terms, lo, hi):
+    tot = Fraction(0)
+    for c, pw in terms:
+        v = Fraction(c)
+        for e, a, b in zip(pw, lo, hi):
+            fa, fb = Fraction(a), Fraction(b)
+            v *= (fb ** (e + 1) - fa ** (e + 1)) / (e + 1)
+        tot += v
+    return tot
+
+
+def test_region_exact_on_polynomials():
+    rng = W.SplitMix64(11)
+    for d in (2, 3, 4, 6):
+        for _ in range(4):
+            terms = []
+            for _t in range(5):
+                pw = [0] * d
+                deg = int(6 * rng.uniform())  # total degree <= 5
+                for _k in range(deg):
+                    pw[int(d * rng.uniform())] += 1
+                terms.append((round(rng.uniform_range(-2, 2), 3), pw))
+            f = W.polynomial(d, terms)
+            lo = np.array([[round(rng.uniform(), 4) for _ in range(d)]])
+            wd = np.array([[round(rng.uniform_range(0.05, 0.5), 4) for _ in range(d)]])
+            est, err, ax, ab, i5, dv = O.region_eval(f, lo, wd)
+            ex = float(_poly_exact_fraction(terms, lo[0], lo[0] + wd[0]))
+            assert abs(est[0] - ex) <= 1e-14 * max(ab[0], 1e-300)
+            assert err[0] <= 1e-14 * max(ab[0], 1e-300)
+            assert abs(i5[0] - ex) <= 1e-14 * max(ab[0], 1e-300)
+
+
+def test_fourth_difference_closed_form_and_axis():
+    d = 5
+    c = [0.3, -2.0, 1.1, 0.7, -0.2]
+    terms = [(ci, [4 if k == i else 0 for k in range(d)]) for i, ci in enumerate(c)]
+    f = W.polynomial(d, terms)
+    lo = np.array([[0.1, 0.2, 0.3, 0.0, 0.5]])
+    wd = np.array([[0.4, 0.3, 0.5, 0.2, 0.3]])
+    est, err, ax, ab, i5, dv = O.region_eval(f, lo, wd)
+    h = wd[0] / 2
+    expect = np.abs(np.array(c)) * h ** 4 * 972.0 / 4900.0
+    assert np.allclose(dv[0], expect, rtol=1e-9, atol=1e-15)
+    assert ax[0] == int(np.argmax(expect))
+
+
+def test_fourth_difference_zero_picks_widest_axis():
+    d = 4
+    f = W.polynomial(d, [(1.0, [1, 2, 0, 0]), (0.5, [0, 0, 3, 0]), (2.0, [0, 0, 0, 1])])
+    lo = np.array([[0.0, 0.0, 0.0, 0.0]])
+    wd = np.array([[0.1, 0.3, 0.2, 0.3]])  # first widest is axis 1
+    est, err, ax, ab, i5, dv = O.region_eval(f, lo, wd)
+    assert np.all(dv[0] <= 1e-15)
+    assert ax[0] == 1
+
+
+def _nquad(fn, lows, highs):
+    v, e = integrate.nquad(lambda *x: fn(np.array(x)), list(zip(lows, highs)),
+                           opts={"epsabs": 1e-13, "epsrel": 1e-12, "limit": 200})
+    return v
+
+
+@pytest.mark.parametrize("family", ["gaussian", "product_peak", "oscillatory", "corner", "continuous", "cosine"])
+def test_closed_forms_against_quadrature(family):
+    rng = W.SplitMix64(hash(family) & 0xFFFF)
+    d = 2
+    lows = [rng.uniform_range(-0.2, 0.2) for _ in range(d)]
+    highs = [lows[i] + rng.uniform_range(0.5, 1.2) for i in range(d)]
+    a = [rng.uniform_range(0.5, 4) for _ in range(d)]
+    w = [rng.uniform_range(0.1, 0.9) for _ in range(d)]
+    f = {
+        "gaussian": W.gaussian(a, w),
+        "product_peak": W.product_peak(a, w),
+        "oscillatory": W.oscillatory(a, rng.uniform()),
+        "corner": W.corner_peak(a, 6),
+        "continuous": W.continuous(a, w),
+        "cosine": W.cosine_sum(d),
+    }[family]
+    if family == "corner":
+        lows = [0.0, 0.1]
+    hi, lo = O.exact(f, lows, highs)
+    num = _nquad(lambda x: O.f_values(f, x[None, :])[0], lows, highs)
+    assert abs(hi - num) <= 1e-10 * max(1.0, abs(num))
+
+
+def test_corner_peak_closed_form_3d():
+    f = W.corner_peak([0.7, 2.5, 4.0], 8)
+    hi, lo = O.exact(f, [0, 0, 0], [1, 1, 1])
+    num = integrate.tplquad(lambda z, y, x: (1 + 0.7 * x + 2.5 * y + 4.0 * z) ** -8, 0, 1, 0, 1, 0, 1,
+                            epsabs=1e-15, epsrel=1e-12)[0]
+    assert abs(hi - num) <= 1e-10 * abs(num)
+
+
+def test_table6_closed_form_small_grid():
+    # 3x3 table: bilinear interpolant integrated by quadrature vs trapezoid form
+    T = np.array([[0.2, -0.5, 1.0], [0.3, 0.9, -0.1], [1.5, 0.0, 0.4]])
+    f = W.table6(T)
+    lows = [0, 0, 0.1, 0, 0, 0]
+    highs = [1, 1, 1, 1, 1, 1]
+    hi, lo = O.exact(f, lows, highs)
+
+    def B(x1, x2):
+        x = np.zeros(6)
+        x[0], x[1], x[2], x[3], x[4], x[5] = x1, x2, 1.0, 0.5, 0.0, 0.0
+        return O.f_values(f, x[None, :])[0]
+
+    bint = integrate.dblquad(lambda y, x: B(x, y), 0, 1, 0, 1, epsabs=1e-13, epsrel=1e-12)[0]
+    i3 = 2 * (1 / math.sqrt(0.1) - 1)
+    i4 = integrate.quad(lambda t: math.exp(-((t - 0.5) ** 2) / 0.02), 0, 1, epsabs=1e-15)[0]
+    assert abs(hi - bint * i3 * i4 * 1.25) <= 1e-9 * abs(hi)
+
+
+def test_initial_grid_and_split_tile_the_box():
+    for d in (3, 5):
+        lows = [0.0] * d
+        highs = [1.0] * (d - 1) + [2.0]
+        L, Wd = O.initial_grid(d, lows, highs)
+        k = 4 if d <= 4 else 2
+        assert L.shape == (k ** d, d)
+        assert abs(np.prod(Wd, axis=1).sum() - 2.0) < 1e-14
+        # distinct cells
+        assert len({tuple(r) for r in np.round(L, 12)}) == k ** d
+        act = np.array([i % 3 != 0 for i in range(len(L))], dtype=np.int32)
+        ax = np.array([i % d for i in range(len(L))], dtype=np.int32)
+        cL, cW = O.split(L, Wd, act, ax)
+        idx = np.nonzero(act)[0]
+        assert cL.shape[0] == 2 * len(idx)
+        for j, r in enumerate(idx):
+            a = ax[r]
+            for c in (0, 1):
+                assert np.all(cW[2 * j + c, np.arange(d) != a] == Wd[r, np.arange(d) != a])
+                assert cW[2 * j + c, a] == Wd[r, a] / 2
+            assert cL[2 * j, a] == L[r, a]
+            assert cL[2 * j + 1, a] == L[r, a] + Wd[r, a] / 2
+
+
+def test_bfs_gaussian3d_converges_to_erf_form():
+    p = W.c1_gaussian3()
+    r = O.integrate_bfs(p)
+    ex, _ = O.exact(p.f, p.lows, p.highs)
+    assert r["rc"] == 0 and r["fail"] == 0
+    assert r["error"] <= p.epsrel * abs(r["integral"])
+    assert abs(r["integral"] - ex) <= p.epsrel * abs(ex)
+    assert r["neval"] == r["nregions"] * 33
+    assert r["neval"] <= p.maxeval
+
+
+def test_bfs_budget_stop_and_heap():
+    p = W.c2_product_peak8()
+    r = O.integrate_bfs(p, nthreads=4, maxeval=2_000_000)
+    assert r["rc"] == 1 and r["fail"] == 1 and r["neval"] <= 2_000_000
+    p1 = W.c1_gaussian3()
+    h = O.integrate_heap(p1)
+    ex, _ = O.exact(p1.f, p1.lows, p1.highs)
+    assert h["rc"] == 0 and abs(h["integral"] - ex) <= p1.epsrel * abs(ex)
+
+
+def test_bfs_nonfinite_is_an_error():
+    # x^-1.5 on a box touching 0 gives inf at a corner point? No: GM nodes are
+    # interior.  Use a power that overflows instead: (1 + a.x)^-k with a < 0.
+    f = W.corner_peak([-1.0, -1.0], 700)
+    p = W.Problem("bad", f, [0.0, 0.0], [1.0, 1.0], 1e-6, 0.0, 10**6)
+    r = O.integrate_bfs(p)
+    assert r["rc"] == -2
