@@ -59,20 +59,21 @@ def family_instance(fam, d, rng):
     raise ValueError(fam)
 
 
-def assert_region_parity(g, o, d, tag):
+def assert_region_parity(g, o, d, tag, vol):
     est, err, ax, ab, i5, dv = g
     oest, oerr, oax, oab, oi5, odv = o
     scale = np.maximum(oab, 1e-300)
+    fscale = scale / vol  # sum |w7 f|: the size of the f values behind D_i
     assert np.all(np.abs(est - oest) <= 1e-13 * scale), tag
     assert np.all(np.abs(err - oerr) <= 1e-13 * scale), tag
     assert np.all(np.abs(i5 - oi5) <= 1e-13 * scale), tag
     assert np.all(np.abs(ab - oab) <= 1e-12 * scale), tag
     dscale = np.maximum(np.max(odv, axis=1, keepdims=True), 1e-300)
-    assert np.all(np.abs(dv - odv) <= 1e-10 * dscale + 1e-14 * scale[:, None]), tag
+    assert np.all(np.abs(dv - odv) <= 1e-10 * dscale + 1e-14 * fscale[:, None]), tag
     for r in np.nonzero(ax != oax)[0]:
         # only a rounding-level tie may flip the chosen axis
         gap = abs(odv[r, ax[r]] - odv[r, oax[r]])
-        assert gap <= 1e-10 * dscale[r, 0] + 1e-14 * scale[r], (tag, r, ax[r], oax[r], odv[r])
+        assert gap <= 1e-10 * dscale[r, 0] + 1e-14 * fscale[r], (tag, r, ax[r], oax[r], odv[r])
 
 
 FAMILIES = [W.GAUSSIAN, W.PRODUCT_PEAK, W.OSCILLATORY, W.CORNER_PEAK, W.CONTINUOUS, W.POLYNOMIAL,
@@ -93,7 +94,7 @@ def test_region_values_match_oracle(ctx, fam, d, lanes):
     finally:
         ctx.set(False, 0)
     o = O.region_eval(f, L, Wd)
-    assert_region_parity(g, o, d, (fam, d, lanes))
+    assert_region_parity(g, o, d, (fam, d, lanes), np.prod(Wd, axis=1))
 
 
 @pytest.mark.parametrize("lanes", [1, 64])
@@ -105,14 +106,14 @@ def test_region_values_dim10_and_table(ctx, lanes):
     try:
         g = ctx.eval_regions(f, L, Wd, want_aux=True)
         o = O.region_eval(f, L, Wd)
-        assert_region_parity(g, o, 10, ("pp10", lanes))
+        assert_region_parity(g, o, 10, ("pp10", lanes), np.prod(Wd, axis=1))
         p = W.c3_table6()
         L, Wd = random_regions(rng, 200, 6)
         L[:, 2] = 0.1 + 0.9 * L[:, 2]
         Wd[:, 2] *= 0.9
         g = ctx.eval_regions(p.f, L, Wd, want_aux=True)
         o = O.region_eval(p.f, L, Wd)
-        assert_region_parity(g, o, 6, ("table6", lanes))
+        assert_region_parity(g, o, 6, ("table6", lanes), np.prod(Wd, axis=1))
     finally:
         ctx.set(False, 0)
 
