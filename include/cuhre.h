@@ -91,7 +91,7 @@ typedef struct {
   int32_t eval_launches;
   int32_t split_launches;
   double eval_flops;     /* algorithmic FP64 flops of the eval launches */
-  double ms_eval_lane;   /* device time of one-region-per-lane launches */
+  double ms_eval_lane;   /* device time of single-segment eval launches */
   int32_t eval_lane_launches;
   int32_t pad0;
   int64_t lane_regions;  /* regions evaluated by those launches         */
@@ -105,9 +105,10 @@ int cuhre_context_create(cuhre_context** ctx, int device, int64_t capacity_regio
 int cuhre_context_destroy(cuhre_context* ctx);
 
 /* Engine knobs.  profile != 0 records HIP events around every launch
- * (fills ms_eval/ms_split).  lanes_per_region: 0 automatic, 1 (one region
- * per lane) or 64 (one region per wavefront). */
-int cuhre_context_set(cuhre_context* ctx, int profile, int lanes_per_region);
+ * (fills ms_eval/ms_split).  point_segments: 0 automatic, else 1, 2, 4, 8
+ * or 16 wavefronts share the points of each group of 64 regions (one region
+ * per lane; more segments give short lists more parallelism). */
+int cuhre_context_set(cuhre_context* ctx, int profile, int point_segments);
 
 /* Plain adaptive integration over the box [lows, highs].  key selects the
  * rule: 0 or 7 = Genz-Malik degree 7/5.  Returns the fail flag (0/1) or a

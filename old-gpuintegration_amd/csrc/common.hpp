@@ -11,6 +11,7 @@ namespace cuhre {
 constexpr int kMaxDim = CUHRE_MAXDIM;
 constexpr int kBlock = 256;  // threads per block in every kernel
 constexpr int kWave = 64;    // CDNA wavefront
+constexpr int kMaxWaves = 16;  // largest block: 1024 threads
 
 // Device copy of an integrand: the C-ABI description plus constants derived
 // on the host once per call (1/a^2 for the product peak, 2 pi u for the
@@ -82,13 +83,11 @@ struct EvalArgs {
   int32_t finalize;    // 0: per-region outputs only (no partials/state)
 };
 
-using EvalLauncher = void (*)(const EvalArgs&, const DevIntegrand&, const RuleConst&,
+// Launches the rule evaluation of A.n regions with S point segments.
+using EvalLauncher = void (*)(const EvalArgs&, const DevIntegrand&, const RuleConst&, int S,
                               hipStream_t);
 
-// Returns the launcher for (family, dim, lanes per region) or nullptr.
-EvalLauncher find_eval_launcher(int family, int dim, int lanes);
-
-// Regions handled by one evaluation block for `lanes` lanes per region.
-inline int regions_per_block(int lanes) { return kBlock / lanes; }
+// Returns the launcher for (family, dim) or nullptr.
+EvalLauncher find_eval_launcher(int family, int dim);
 
 }  // namespace cuhre

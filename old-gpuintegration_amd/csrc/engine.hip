@@ -23,30 +23,31 @@
 #include <vector>
 
 #include "common.hpp"
+#include "eval_kernels.hpp"
 #include "reduce.hpp"
 
 namespace cuhre {
 
-EvalLauncher launcher_gaussian(int, int);
-EvalLauncher launcher_product_peak(int, int);
-EvalLauncher launcher_oscillatory(int, int);
-EvalLauncher launcher_corner_peak(int, int);
-EvalLauncher launcher_continuous(int, int);
-EvalLauncher launcher_polynomial(int, int);
-EvalLauncher launcher_cosine_sum(int, int);
-EvalLauncher launcher_table6(int, int);
+EvalLauncher launcher_gaussian(int);
+EvalLauncher launcher_product_peak(int);
+EvalLauncher launcher_oscillatory(int);
+EvalLauncher launcher_corner_peak(int);
+EvalLauncher launcher_continuous(int);
+EvalLauncher launcher_polynomial(int);
+EvalLauncher launcher_cosine_sum(int);
+EvalLauncher launcher_table6(int);
 
-EvalLauncher find_eval_launcher(int family, int dim, int lanes)
+EvalLauncher find_eval_launcher(int family, int dim)
 {
   switch (family) {
-  case CUHRE_GAUSSIAN: return launcher_gaussian(dim, lanes);
-  case CUHRE_PRODUCT_PEAK: return launcher_product_peak(dim, lanes);
-  case CUHRE_OSCILLATORY: return launcher_oscillatory(dim, lanes);
-  case CUHRE_CORNER_PEAK: return launcher_corner_peak(dim, lanes);
-  case CUHRE_CONTINUOUS: return launcher_continuous(dim, lanes);
-  case CUHRE_POLYNOMIAL: return launcher_polynomial(dim, lanes);
-  case CUHRE_COSINE_SUM: return launcher_cosine_sum(dim, lanes);
-  case CUHRE_TABLE6: return launcher_table6(dim, lanes);
+  case CUHRE_GAUSSIAN: return launcher_gaussian(dim);
+  case CUHRE_PRODUCT_PEAK: return launcher_product_peak(dim);
+  case CUHRE_OSCILLATORY: return launcher_oscillatory(dim);
+  case CUHRE_CORNER_PEAK: return launcher_corner_peak(dim);
+  case CUHRE_CONTINUOUS: return launcher_continuous(dim);
+  case CUHRE_POLYNOMIAL: return launcher_polynomial(dim);
+  case CUHRE_COSINE_SUM: return launcher_cosine_sum(dim);
+  case CUHRE_TABLE6: return launcher_table6(dim);
   default: return nullptr;
   }
 }
@@ -106,7 +107,7 @@ RuleConst make_rule(int d)
 // counting convention add/sub/mul = 1, fma = 2, and div / exp / cos / sqrt
 // charged the FP64 VALU flops of their gfx950 instruction sequences
 // (measured with the SQ_INSTS_VALU_*_F64 counters on a one-op kernel).
-struct FlopCosts { double div = 21, exp = 42, cos = 66, sqrt = 20; };
+struct FlopCosts { double div = 14, exp = 27, cos = 57, sqrt = 17; };
 static FlopCosts g_costs;
 
 double family_flops(int fam, int d)
@@ -279,8 +280,8 @@ struct cuhre_context {
   hipStream_t stream = nullptr;
   int64_t capacity = 64ll << 20;  // regions
   int profile = 0;
-  int lanes = 0;
-  int64_t lane_threshold = 131072;  // auto: one region per lane at/above this n
+  int segments = 0;                // 0: automatic, else 1, 2, 4, 8 or 16
+  int64_t fill_lanes = 262144;     // auto: smallest S with n*S >= fill_lanes
 
   int64_t cap = 0;  // allocated regions per list buffer
   int dim_alloc = 0;
@@ -306,7 +307,11 @@ struct cuhre_context {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 };
 
-static int64_t blocks_for(int64_t n, int lanes) { return (n + kBlock / lanes - 1) / (kBlock / lanes); }
+static int64_t blocks_for(int64_t n, int S)
+{
+  const int rpb = seg_regions_per_block(S);
+  return (n + rpb - 1) / rpb;
+}
 
 static int ctx_free_lists(cuhre_context* c)
 {
@@ -339,7 +344,7 @@ static int ctx_reserve(cuhre_context* c, int64_t cap, int dim, int64_t keep_n = 
   cap = std::max(cap, c->cap);
   dim = std::max(dim, c->dim_alloc);
   cap = (cap + 255) & ~255ll;
-  const int64_t nbmax = cap / (kBlock / kWave) + 1;
+  const int64_t nbmax = cap / kWave + 1;
   double *lo[2], *wd[2], *est, *err, *ab;
   int8_t* ax;
   Partial* part;
@@ -441,10 +446,13 @@ static int make_dev_integrand(cuhre_context* c, const cuhre_integrand* f, int di
   return 0;
 }
 
-static int pick_lanes(const cuhre_context* c, int64_t n)
+// Point segments per region: enough that n*S lanes fill the chip.
+static int pick_segments(const cuhre_context* c, int64_t n)
 {
-  if (c->lanes == 1 || c->lanes == kWave) return c->lanes;
-  return n >= c->lane_threshold ? 1 : kWave;
+  if (c->segments > 0) return c->segments;
+  int S = 1;
+  while (S < kMaxWaves && n * S < c->fill_lanes) S *= 2;
+  return S;
 }
 
 static int grid_k(int d) { return d <= 4 ? 4 : (d <= 10 ? 2 : 1); }
@@ -479,7 +487,7 @@ static int check_common(cuhre_context* ctx, const cuhre_integrand* f, int dim, i
   if (dim < 2 || dim > 10) return fail(CUHRE_ERR_UNSUPPORTED, "dimension %d outside 2..10", dim);
   if (f->dim != dim) return fail(CUHRE_ERR_ARG, "integrand dim %d != dim %d", f->dim, dim);
   if (key != 0 && key != 7) return fail(CUHRE_ERR_UNSUPPORTED, "rule key %d not available (0 or 7)", key);
-  if (!find_eval_launcher(f->family, dim, 1))
+  if (!find_eval_launcher(f->family, dim))
     return fail(CUHRE_ERR_UNSUPPORTED, "family %d not compiled for dim %d", f->family, dim);
   HIP_TRY(hipSetDevice(ctx->device));
   return 0;
@@ -598,13 +606,14 @@ int cuhre_context_destroy(cuhre_context* c)
   return 0;
 }
 
-int cuhre_context_set(cuhre_context* c, int profile, int lanes_per_region)
+int cuhre_context_set(cuhre_context* c, int profile, int point_segments)
 {
   if (!c) return fail(CUHRE_ERR_ARG, "null context");
-  if (lanes_per_region != 0 && lanes_per_region != 1 && lanes_per_region != kWave)
-    return fail(CUHRE_ERR_ARG, "lanes_per_region must be 0, 1 or 64");
+  if (point_segments != 0 && point_segments != 1 && point_segments != 2 && point_segments != 4 &&
+      point_segments != 8 && point_segments != 16)
+    return fail(CUHRE_ERR_ARG, "point_segments must be 0, 1, 2, 4, 8 or 16");
   c->profile = profile;
-  c->lanes = lanes_per_region;
+  c->segments = point_segments;
   return 0;
 }
 
@@ -644,8 +653,8 @@ int cuhre_integrate(cuhre_context* c, const cuhre_integrand* f, int dim, double 
   const double u = 0.5 * 2.220446049250313e-16;
   int result = CUHRE_NOT_CONVERGED;
   for (int it = 0; it < 1000; ++it) {
-    const int lanes = pick_lanes(c, n);
-    const EvalLauncher launch = find_eval_launcher(f->family, dim, lanes);
+    const int S = pick_segments(c, n);
+    const EvalLauncher launch = find_eval_launcher(f->family, dim);
     EvalArgs A;
     fill_eval_args(c, A, cur, n, dim, epsrel, epsabs);
     A.finalize = 1;
@@ -656,15 +665,15 @@ int cuhre_integrate(cuhre_context* c, const cuhre_integrand* f, int dim, double 
     const int64_t left = maxeval - (out->neval + n * p);
     A.max_active = std::min<int64_t>(left >= 0 ? left / (2 * p) : -1, c->capacity / 2);
     A.alloc_active = c->cap / 2;
-    SplitArgs S;
-    fill_split_args(c, S, cur, n, dim, epsrel, epsabs, kBlock / lanes, true);
-    const int64_t nb = blocks_for(n, lanes);
+    SplitArgs SA;
+    fill_split_args(c, SA, cur, n, dim, epsrel, epsabs, seg_regions_per_block(S), true);
+    const int64_t nb = blocks_for(n, S);
     if (nb > 0x7fffffffll) return fail(CUHRE_ERR_CAPACITY, "grid too large");
 
     if (c->profile) HIP_TRY(hipEventRecord(c->ev[0], c->stream));
-    launch(A, P, R, c->stream);
+    launch(A, P, R, S, c->stream);
     if (c->profile) HIP_TRY(hipEventRecord(c->ev[1], c->stream));
-    hipLaunchKernelGGL(k_split, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, c->stream, S);
+    hipLaunchKernelGGL(k_split, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, c->stream, SA);
     if (c->profile) HIP_TRY(hipEventRecord(c->ev[2], c->stream));
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(c->h_state, c->state, sizeof(IterState), hipMemcpyDeviceToHost, c->stream));
@@ -676,9 +685,9 @@ int cuhre_integrate(cuhre_context* c, const cuhre_integrand* f, int dim, double 
       HIP_TRY(hipEventElapsedTime(&ms_s, c->ev[1], c->ev[2]));
       out->ms_eval += ms_e;
       out->ms_split += ms_s;
-      if (lanes == 1) out->ms_eval_lane += ms_e;
+      if (S == 1) out->ms_eval_lane += ms_e;
     }
-    if (lanes == 1) {
+    if (S == 1) {
       out->eval_lane_launches++;
       out->lane_regions += n;
     }
@@ -690,9 +699,9 @@ int cuhre_integrate(cuhre_context* c, const cuhre_integrand* f, int dim, double 
       const int64_t A2 = 2 * st.tot.nactive;
       const int64_t ncap = std::min<int64_t>(c->capacity, std::max<int64_t>(A2, 2 * c->cap));
       if ((rc = ctx_reserve(c, ncap, dim, n, cur, nb))) return rc;
-      fill_split_args(c, S, cur, n, dim, epsrel, epsabs, kBlock / lanes, false);
+      fill_split_args(c, SA, cur, n, dim, epsrel, epsabs, seg_regions_per_block(S), false);
       if (c->profile) HIP_TRY(hipEventRecord(c->ev[1], c->stream));
-      hipLaunchKernelGGL(k_split, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, c->stream, S);
+      hipLaunchKernelGGL(k_split, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, c->stream, SA);
       if (c->profile) HIP_TRY(hipEventRecord(c->ev[2], c->stream));
       HIP_TRY(hipGetLastError());
       HIP_TRY(hipStreamSynchronize(c->stream));
@@ -760,7 +769,7 @@ int cuhre_eval_regions(cuhre_context* c, const cuhre_integrand* f, int dim, int 
                            sizeof(double) * n, dim, hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipMemcpy2DAsync(c->wd[0], sizeof(double) * c->cap, width, sizeof(double) * n,
                            sizeof(double) * n, dim, hipMemcpyHostToDevice, c->stream));
-  const int lanes = pick_lanes(c, n);
+  const int S = pick_segments(c, n);
   EvalArgs A;
   fill_eval_args(c, A, 0, n, dim, 0.0, 0.0);
   A.finalize = 0;
@@ -768,7 +777,7 @@ int cuhre_eval_regions(cuhre_context* c, const cuhre_integrand* f, int dim, int 
     A.i5 = c->aux;
     A.dvec = c->aux + c->cap;
   }
-  find_eval_launcher(f->family, dim, lanes)(A, P, R, c->stream);
+  find_eval_launcher(f->family, dim)(A, P, R, S, c->stream);
   HIP_TRY(hipGetLastError());
   std::vector<int8_t> ax8((size_t)n);
   HIP_TRY(hipMemcpyAsync(est, c->est, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));

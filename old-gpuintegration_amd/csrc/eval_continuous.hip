@@ -2,5 +2,5 @@
 #include "eval_kernels.hpp"
 
 namespace cuhre {
-EvalLauncher launcher_continuous(int dim, int lanes) { return family_launcher<CUHRE_CONTINUOUS>(dim, lanes); }
+EvalLauncher launcher_continuous(int dim) { return family_launcher<CUHRE_CONTINUOUS>(dim); }
 }  // namespace cuhre

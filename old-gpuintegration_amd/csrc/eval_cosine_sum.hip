@@ -2,5 +2,5 @@
 #include "eval_kernels.hpp"
 
 namespace cuhre {
-EvalLauncher launcher_cosine_sum(int dim, int lanes) { return family_launcher<CUHRE_COSINE_SUM>(dim, lanes); }
+EvalLauncher launcher_cosine_sum(int dim) { return family_launcher<CUHRE_COSINE_SUM>(dim); }
 }  // namespace cuhre

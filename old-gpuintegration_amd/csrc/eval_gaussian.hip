@@ -2,5 +2,5 @@
 #include "eval_kernels.hpp"
 
 namespace cuhre {
-EvalLauncher launcher_gaussian(int dim, int lanes) { return family_launcher<CUHRE_GAUSSIAN>(dim, lanes); }
+EvalLauncher launcher_gaussian(int dim) { return family_launcher<CUHRE_GAUSSIAN>(dim); }
 }  // namespace cuhre

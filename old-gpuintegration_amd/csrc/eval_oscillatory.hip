@@ -2,5 +2,5 @@
 #include "eval_kernels.hpp"
 
 namespace cuhre {
-EvalLauncher launcher_oscillatory(int dim, int lanes) { return family_launcher<CUHRE_OSCILLATORY>(dim, lanes); }
+EvalLauncher launcher_oscillatory(int dim) { return family_launcher<CUHRE_OSCILLATORY>(dim); }
 }  // namespace cuhre

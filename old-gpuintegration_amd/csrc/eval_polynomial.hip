@@ -2,5 +2,5 @@
 #include "eval_kernels.hpp"
 
 namespace cuhre {
-EvalLauncher launcher_polynomial(int dim, int lanes) { return family_launcher<CUHRE_POLYNOMIAL>(dim, lanes); }
+EvalLauncher launcher_polynomial(int dim) { return family_launcher<CUHRE_POLYNOMIAL>(dim); }
 }  // namespace cuhre

@@ -2,5 +2,5 @@
 #include "eval_kernels.hpp"
 
 namespace cuhre {
-EvalLauncher launcher_product_peak(int dim, int lanes) { return family_launcher<CUHRE_PRODUCT_PEAK>(dim, lanes); }
+EvalLauncher launcher_product_peak(int dim) { return family_launcher<CUHRE_PRODUCT_PEAK>(dim); }
 }  // namespace cuhre

@@ -2,8 +2,5 @@
 #include "eval_kernels.hpp"
 
 namespace cuhre {
-EvalLauncher launcher_table6(int dim, int lanes)
-{
-  return dim == 6 ? pick_lanes<CUHRE_TABLE6, 6>(lanes) : nullptr;
-}
+EvalLauncher launcher_table6(int dim) { return dim == 6 ? &launch_eval<6, CUHRE_TABLE6> : nullptr; }
 }  // namespace cuhre

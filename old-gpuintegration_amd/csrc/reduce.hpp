@@ -115,17 +115,24 @@ __device__ __forceinline__ void partial_from_region(Partial& p, double est, doub
   p.nnonfinite = finite ? 0 : 1;
 }
 
-// Reduce one Partial per thread to thread 0 of the block (256 threads).
-__device__ __forceinline__ void block_reduce_partial(Partial& p, Partial* lds4)
+// Reduce one Partial per thread to thread 0 of a block of W waves.  Waves
+// whose lanes hold no region (contributes == false, wave-uniform) skip the
+// shuffles and add zeros.
+__device__ __forceinline__ void block_reduce_partial(Partial& p, Partial* ldsw, int W,
+                                                     bool contributes = true)
 {
+  if (contributes) {
 #pragma unroll
-  for (int mask = 1; mask < kWave; mask <<= 1) partial_shfl_add(p, mask);
+    for (int mask = 1; mask < kWave; mask <<= 1) partial_shfl_add(p, mask);
+  } else {
+    partial_zero(p);
+  }
   const int wave = threadIdx.x / kWave;
-  if (__lane_id() == 0) lds4[wave] = p;
+  if (__lane_id() == 0) ldsw[wave] = p;
   __syncthreads();
   if (threadIdx.x == 0) {
-    Partial t = lds4[0];
-    for (int w = 1; w < kBlock / kWave; ++w) partial_add(t, lds4[w]);
+    Partial t = ldsw[0];
+    for (int w = 1; w < W; ++w) partial_add(t, ldsw[w]);
     p = t;
   }
 }
@@ -133,7 +140,7 @@ __device__ __forceinline__ void block_reduce_partial(Partial& p, Partial* lds4)
 // Called by every block after its thread 0 holds the block's Partial in p.
 // The last block to arrive reduces all partials, scans active counts into
 // offsets and writes the iteration state.
-__device__ __forceinline__ void finish_iteration(const EvalArgs& A, Partial& p, Partial* lds4)
+__device__ __forceinline__ void finish_iteration(const EvalArgs& A, Partial& p, Partial* ldsw, int W)
 {
   __shared__ uint32_t s_last;
   const uint32_t nb = gridDim.x;
@@ -147,19 +154,20 @@ __device__ __forceinline__ void finish_iteration(const EvalArgs& A, Partial& p, 
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 
   // contiguous chunk of blocks per thread, summed in order
-  const uint32_t chunk = (nb + kBlock - 1) / kBlock;
+  const uint32_t T = blockDim.x;
+  const uint32_t chunk = (nb + T - 1) / T;
   const uint32_t b0 = threadIdx.x * chunk;
   const uint32_t b1 = b0 + chunk < nb ? b0 + chunk : nb;
   Partial q;
   partial_zero(q);
   for (uint32_t b = b0; b < b1; ++b) partial_add(q, A.partials[b]);
   // exclusive scan of per-thread active counts (Hillis-Steele in LDS)
-  __shared__ int64_t s_scan[kBlock];
+  __shared__ int64_t s_scan[kMaxWaves * kWave];
   int64_t mine = q.nactive;
   s_scan[threadIdx.x] = mine;
   __syncthreads();
-  for (int off = 1; off < kBlock; off <<= 1) {
-    int64_t v = threadIdx.x >= (unsigned)off ? s_scan[threadIdx.x - off] : 0;
+  for (uint32_t off = 1; off < T; off <<= 1) {
+    int64_t v = threadIdx.x >= off ? s_scan[threadIdx.x - off] : 0;
     __syncthreads();
     s_scan[threadIdx.x] += v;
     __syncthreads();
@@ -169,7 +177,7 @@ __device__ __forceinline__ void finish_iteration(const EvalArgs& A, Partial& p, 
     A.offsets[b] = run;
     run += A.partials[b].nactive;
   }
-  block_reduce_partial(q, lds4);
+  block_reduce_partial(q, ldsw, W);
   if (threadIdx.x == 0) {
     IterState st;
     st.tot = q;

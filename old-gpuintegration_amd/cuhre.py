@@ -151,13 +151,15 @@ def to_struct(f: W.Integrand):
 class Context:
     """One engine context on one HIP device (cuhre_context_create)."""
 
-    def __init__(self, device: int = 0, capacity: int = 0, profile: bool = False, lanes: int = 0):
+    def __init__(self, device: int = 0, capacity: int = 0, profile: bool = False, segments: int = 0):
         self.h = C.c_void_p()
         _check(lib().cuhre_context_create(C.byref(self.h), device, capacity))
-        self.set(profile, lanes)
+        self.set(profile, segments)
 
-    def set(self, profile: bool = False, lanes: int = 0):
-        _check(lib().cuhre_context_set(self.h, int(profile), lanes))
+    def set(self, profile: bool = False, segments: int = 0):
+        """profile: HIP-event timing of every launch; segments: 0 automatic or
+        1/2/4/8/16 wavefronts per group of 64 regions."""
+        _check(lib().cuhre_context_set(self.h, int(profile), segments))
 
     def close(self):
         if self.h:
