@@ -91,10 +91,9 @@ typedef struct {
   int32_t eval_launches;
   int32_t split_launches;
   double eval_flops;     /* algorithmic FP64 flops of the eval launches */
-  double ms_eval_lane;   /* device time of single-segment eval launches */
-  int32_t eval_lane_launches;
-  int32_t pad0;
-  int64_t lane_regions;  /* regions evaluated by those launches         */
+  double ms_eval_largest;  /* device time of the largest eval launch    */
+  int32_t pad0, pad1;
+  int64_t largest_regions; /* regions of that launch                   */
 } cuhre_result;
 
 typedef struct cuhre_context cuhre_context;

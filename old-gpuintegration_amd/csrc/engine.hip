@@ -446,12 +446,15 @@ static int make_dev_integrand(cuhre_context* c, const cuhre_integrand* f, int di
   return 0;
 }
 
-// Point segments per region: enough that n*S lanes fill the chip.
+// Point segments per region: the smallest S with n*S >= 256k lanes, and at
+// least 2 below 2M regions (measured on MI355X, 8-D product peak and
+// oscillatory: S=16 is best up to 16k regions, S=4 at 64k-256k, S=2 at 1M).
 static int pick_segments(const cuhre_context* c, int64_t n)
 {
   if (c->segments > 0) return c->segments;
   int S = 1;
   while (S < kMaxWaves && n * S < c->fill_lanes) S *= 2;
+  if (S == 1 && n < (2ll << 20)) S = 2;
   return S;
 }
 
@@ -685,11 +688,10 @@ int cuhre_integrate(cuhre_context* c, const cuhre_integrand* f, int dim, double 
       HIP_TRY(hipEventElapsedTime(&ms_s, c->ev[1], c->ev[2]));
       out->ms_eval += ms_e;
       out->ms_split += ms_s;
-      if (S == 1) out->ms_eval_lane += ms_e;
-    }
-    if (S == 1) {
-      out->eval_lane_launches++;
-      out->lane_regions += n;
+      if (n >= out->largest_regions) {
+        out->largest_regions = n;
+        out->ms_eval_largest = ms_e;
+      }
     }
     out->eval_launches++;
     out->eval_flops += rflops * (double)n;

@@ -70,10 +70,10 @@ class Result(C.Structure):
         ("eval_launches", C.c_int32),
         ("split_launches", C.c_int32),
         ("eval_flops", C.c_double),
-        ("ms_eval_lane", C.c_double),
-        ("eval_lane_launches", C.c_int32),
+        ("ms_eval_largest", C.c_double),
         ("pad0", C.c_int32),
-        ("lane_regions", C.c_int64),
+        ("pad1", C.c_int32),
+        ("largest_regions", C.c_int64),
     ]
 
     def asdict(self):

@@ -1,0 +1,49 @@
+"""The C++ facade (include/gpucuhre.hpp) builds against the C-ABI library and,
+on a GPU, reproduces the reference driver's call pattern (cudaCuhre.cu)."""
+import os
+import subprocess
+
+import pytest
+
+import cuhre
+import oracle as O
+import workloads as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def build_driver(tmp_path):
+    exe = str(tmp_path / "facade_driver")
+    libdir = os.path.dirname(cuhre.LIBPATH)
+    subprocess.check_call(["g++", "-std=c++17", "-O2", "-I", os.path.join(ROOT, "include"),
+                           os.path.join(ROOT, "tests", "facade_driver.cpp"), "-o", exe,
+                           "-L", libdir, "-lcuhre_amd", f"-Wl,-rpath,{libdir}"])
+    return exe
+
+
+def test_facade_compiles_and_links(tmp_path):
+    exe = build_driver(tmp_path)
+    assert os.path.exists(exe)
+
+
+@pytest.mark.gpu
+def test_facade_driver_matches_oracle(tmp_path):
+    exe = build_driver(tmp_path)
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=120, check=True).stdout
+    lines = [l.split("\t") for l in out.strip().splitlines()]
+    f = W.cosine_sum(3)
+    import numpy as np
+    # the driver's Volume holds float bounds, widened exactly to double
+    lows = [1.0, float(np.float32(0.1)), 0.0]
+    highs = [2.0, float(np.float32(0.3)), 1.0]
+    ex, _ = O.exact(f, lows, highs)
+    for row, eps in zip(lines[:2], (1e-7, 1e-10)):
+        integral, error, nreg, neval, flag = float(row[2]), float(row[3]), int(row[4]), int(row[5]), int(row[6])
+        o = O.integrate_bfs(W.Problem("drv", f, lows, highs, eps, 1e-12, 10**9), nthreads=8)
+        assert flag == 0 and o["rc"] == 0
+        assert nreg == o["nregions"] and neval == o["neval"]
+        assert abs(integral - o["integral"]) <= 1e-12 * abs(o["integral"])
+        assert abs(integral - ex) <= max(eps * abs(ex), 1e-12)
+    acc, acc_r = float(lines[2][1]), int(lines[2][2])
+    assert abs(acc - 2 * float(lines[0][2])) <= 1e-12 * abs(acc)
+    assert acc_r == 2 * int(lines[0][4])
