@@ -72,12 +72,39 @@ def cpu_baseline(problems):
                       f"(maxeval 1e9), {ne} evals in {dt:.2f} s, OpenMP over regions"}
 
 
+def bench_batched(ctx, count=4096):
+    """4096 independent 5-D integrals (Gaussian / product peak / corner peak,
+    epsrel 1e-5 each): one batched call vs one plain call per integral."""
+    probs = W.c4_batch(count=count)
+    cfg, table = ctx.batch_autotune(probs)
+    ctx.integrate_batch(probs)  # warm-up (graph capture)
+    t0 = time.perf_counter()
+    res, info, rc = ctx.integrate_batch(probs)
+    tb = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    plain = [ctx.integrate(p) for p in probs]
+    tp = time.perf_counter() - t0
+    same = sum(1 for a, b in zip(res, plain) if a["nregions"] == b["nregions"]
+               and abs(a["integral"] - b["integral"]) <= 1e-12 * abs(b["integral"]))
+    regions = sum(r["nregions"] for r in res)
+    return {"instances": count, "batched_s": tb, "plain_sequential_s": tp, "wall_ratio": tb / tp,
+            "batched_evals_per_s": info["total_neval"] / tb,
+            "plain_evals_per_s": sum(r["neval"] for r in plain) / tp,
+            "regions": regions, "regions_plain": sum(r["nregions"] for r in plain),
+            "identical_instances": same, "not_converged": info["not_converged"],
+            "iterations": info["iterations"], "graph_used": info["graph_used"], "autotuned": cfg,
+            "autotune_ms": table.tolist(),
+            "per_instance_regions_min_max": [min(r["nregions"] for r in res), max(r["nregions"] for r in res)]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batched", action="store_true",
+                    help="also time the 4096-integral batched form against sequential plain calls")
     ap.add_argument("--converge", action="store_true",
                     help="also run the product peak with maxeval 1e11 to reach epsrel 1e-6")
     args = ap.parse_args()
@@ -170,6 +197,8 @@ def main():
                                               "evals_per_s": r["neval"] / (t2 - t1), **{k: r[k] for k in (
                                                   "integral", "error", "nregions", "neval", "fail",
                                                   "iterations", "peak_regions")}}
+    if args.batched:
+        out["batched"] = bench_batched(ctx)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(problems)
     ctx.close()

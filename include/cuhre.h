@@ -138,6 +138,50 @@ int cuhre_split_regions(cuhre_context* ctx, int dim, int64_t n,
 /* Deterministic FP64 sum of n values (the reduction the engine uses). */
 int cuhre_sum(cuhre_context* ctx, int64_t n, const double* x, double* out);
 
+/* ---------------------------------------------------------------- batches
+ * Many independent integrals of one dimension in one call (no reference
+ * counterpart: the reference integrates one functor per call).  Each
+ * instance runs the plain method with its own tolerances and budget; all
+ * share one region list on the device and the iterations are replayed from
+ * a hipGraph.  Families: GAUSSIAN, PRODUCT_PEAK, OSCILLATORY, CORNER_PEAK,
+ * CONTINUOUS. */
+typedef struct {
+  int32_t point_segments;       /* 0: from cuhre_batch_autotune, else 1..16   */
+  int32_t grid_blocks;          /* 0: from cuhre_batch_autotune               */
+  int32_t iterations_per_check; /* graph replays between host checks (0: 2)   */
+  int32_t use_graph;            /* 1: hipGraph replay, 0: direct launches     */
+} cuhre_batch_config;
+
+typedef struct {
+  int32_t iterations;           /* largest per-instance iteration count       */
+  int32_t launched_iterations;  /* iterations replayed (>= iterations)        */
+  int64_t total_regions;
+  int64_t total_neval;
+  int32_t not_converged;
+  int32_t graph_used;
+  int32_t point_segments;
+  int32_t grid_blocks;
+  double ms_total;
+} cuhre_batch_info;
+
+/* count instances; fs[count]; epsrel/epsabs/maxeval per instance (NULL:
+ * 1e-3 / 0 / 1e9); lows/highs count*dim (NULL: unit cube); out[count].
+ * Returns 0 when every instance converged, 1 if some did not, <0 on error
+ * (a non-finite integrand value marks that instance and returns -2). */
+int cuhre_integrate_batch(cuhre_context* ctx, int32_t count, const cuhre_integrand* fs, int dim,
+                          const double* epsrel, const double* epsabs, const int64_t* maxeval,
+                          const double* lows, const double* highs,
+                          const cuhre_batch_config* cfg, cuhre_result* out,
+                          cuhre_batch_info* info);
+
+/* Times the batch evaluation kernel over point_segments {1,2,4} x grid
+ * blocks {256..4096} on a uniformly refined sample of the batch's own list
+ * and stores the fastest shape (also returned in *best; table_ms, if not
+ * NULL, receives the 15 timings in that order). */
+int cuhre_batch_autotune(cuhre_context* ctx, int32_t count, const cuhre_integrand* fs, int dim,
+                         const double* lows, const double* highs, cuhre_batch_config* best,
+                         double* table_ms);
+
 /* Thread-local message of the last error. */
 const char* cuhre_last_error(void);
 

@@ -1,0 +1,877 @@
+// batch.hip -- many independent integrals in one region list.
+//
+// Batched form of the plain method (no reference counterpart; the reference
+// integrates one functor per GPUcuhre::integrate call, quad/GPUquad/
+// GPUquad.h:611-727).  Every instance runs exactly the plain loop -- same
+// rule, same per-region classification, same global test on the instance's
+// own sums, same evaluation budget -- but all instances share one SoA region
+// list in HBM, tagged with an instance id and kept grouped by instance (the
+// compaction is stable and children stay adjacent).  One iteration is six
+// launches with fixed shapes that read the list length from device memory:
+//
+//   k_batch_eval    rule evaluation, grid-stride over 64-region groups, the
+//                   lane's integrand parameters fetched by instance id;
+//   k_batch_reduce  one block per instance: TwoSum-carried sums of its slice,
+//                   global test, budget test, active count;
+//   k_batch_scan    one block: next slice offsets of the instances (with a
+//                   capacity cut) and the next list length;
+//   k_tile_count /  per-256-region active counts and their scan;
+//   k_tile_scan
+//   k_batch_split   stable compaction + bisection into the other buffer.
+//
+// Because no launch depends on a host-side size, two iterations (one per
+// ping-pong parity) are captured once in a hipGraph and replayed until the
+// list is empty; finished instances cost nothing.  The evaluation launch
+// shape (point segments, grid blocks) comes from cuhre_batch_autotune, which
+// times the candidates on the batch's own integrands.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "common.hpp"
+#include "eval_kernels.hpp"
+#include "reduce.hpp"
+
+namespace cuhre {
+
+RuleConst make_rule(int d);
+int gm_points(int d);
+int set_error(int code, const char* msg);
+
+// Per-instance parameters and tolerances (uploaded once per call).
+struct BatchParams {
+  int32_t family, exponent;
+  double a[kMaxDim], w[kMaxDim], b[kMaxDim];
+  double s0;
+  double epsrel, epsabs;
+  int64_t maxeval;
+  double lows[kMaxDim], highs[kMaxDim];
+};
+
+// Per-instance loop state, updated on the device.
+struct InstState {
+  double fin_s, fin_c, fer_s, fer_c;  // finished-region sums (value, carry)
+  double abs_fin, aest_fin;
+  double integral, error, bound;
+  int64_t neval, nregions, nfin, peak;
+  int64_t start, count;  // slice of the current list
+  int64_t next;          // children for the next iteration (2 * active)
+  int32_t done, fail, iters, pad;
+};
+
+// Integrand of one lane in a batch: family FAM with the lane's instance
+// parameters (held in registers).
+template <int D, int FAM>
+struct LaneIntegrand {
+  DevIntegrand P;
+  __device__ __forceinline__ double operator()(const double (&x)[D]) const
+  {
+    return Family<FAM, D>::eval(P, x);
+  }
+};
+
+struct BatchBuffers {
+  double* lower;
+  double* width;
+  int32_t* inst;
+  double* out_lower;
+  double* out_width;
+  int32_t* out_inst;
+  int64_t ld;
+  const int64_t* n;  // current list length
+  int64_t* n_next;
+  double* est;
+  double* err;
+  double* abssum;
+  int8_t* axis;
+  int64_t* tile_cnt;
+  int64_t* tile_off;
+  const BatchParams* params;
+  InstState* state;
+  int32_t count;
+  int32_t dim;
+  int32_t npts;
+  int64_t cap;
+};
+
+__device__ __forceinline__ bool batch_active(const InstState& st, const BatchParams& bp, double e,
+                                             double er)
+{
+  return !st.done && isfinite(e) && isfinite(er) && region_active(e, er, bp.epsrel, bp.epsabs);
+}
+
+// Initial k^d grid of every instance plus the instance states.
+__global__ void k_batch_init(BatchBuffers B, int k, int64_t n0)
+{
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t total = n0 * B.count;
+  if (r < B.count) {
+    InstState st;
+    memset(&st, 0, sizeof st);
+    st.start = r * n0;
+    st.count = n0;
+    B.state[r] = st;
+  }
+  if (r >= total) return;
+  const int i = (int)(r / n0);
+  int64_t q = r - (int64_t)i * n0;
+  const BatchParams& bp = B.params[i];
+  for (int m = 0; m < B.dim; ++m) {
+    const int64_t id = q % k;
+    q /= k;
+    const double w = (bp.highs[m] - bp.lows[m]) / (double)k;
+    B.lower[m * B.ld + r] = bp.lows[m] + (double)id * w;
+    B.width[m * B.ld + r] = w;
+  }
+  B.inst[r] = i;
+}
+
+template <int D, int FAM>
+__device__ __forceinline__ RegionOut batch_seg_eval(const BatchBuffers& B, const RuleConst& R,
+                                                    int64_t r, bool valid, int64_t rr, int S, int s,
+                                                    int wave, int lane, int W, double* seg_lds)
+{
+  const BatchParams& bp = B.params[B.inst[rr]];
+  LaneIntegrand<D, FAM> f;
+  f.P.family = FAM;
+  f.P.exponent = bp.exponent;
+#pragma unroll
+  for (int m = 0; m < D; ++m) {
+    f.P.a[m] = bp.a[m];
+    f.P.w[m] = bp.w[m];
+    f.P.b[m] = bp.b[m];
+  }
+  f.P.s0 = bp.s0;
+  return seg_eval<D>(f, B.lower, B.width, B.ld, r, valid, rr, S, s, wave, lane, W, seg_lds, R, nullptr);
+}
+
+// One launch per family present in the batch.  Instances are ordered by
+// family, so the families along the list are non-decreasing: a tile whose
+// first and last regions bracket FAM may hold it, any other tile is skipped
+// (a block-uniform test, which keeps the segment barriers convergent).
+// Lanes of another family in a boundary tile compute but do not store.
+template <int D, int FAM>
+__global__ __launch_bounds__(kBlock) void k_batch_eval(BatchBuffers B, RuleConst R, int S)
+{
+  extern __shared__ double seg_lds[];
+  const int W = S < 4 ? 4 : S;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & (kWave - 1);
+  const int g = wave / S, s = wave - g * S;
+  const int64_t n = *B.n;
+  const int64_t rpb = kWave * (W / S);
+  const int64_t ntiles = (n + rpb - 1) / rpb;
+  for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int64_t t0 = t * rpb;
+    const int64_t t1 = (t0 + rpb < n ? t0 + rpb : n) - 1;
+    const int f_lo = B.params[B.inst[t0]].family;
+    const int f_hi = B.params[B.inst[t1]].family;
+    if (FAM < f_lo || FAM > f_hi) continue;
+    const int64_t r = t0 + g * kWave + lane;
+    const bool valid = r < n;
+    const int64_t rr = valid ? r : n - 1;
+    const RegionOut o = batch_seg_eval<D, FAM>(B, R, r, valid, rr, S, s, wave, lane, W, seg_lds);
+    if (valid && s == 0 && B.params[B.inst[r]].family == FAM) {
+      B.est[r] = o.est;
+      B.err[r] = o.err;
+      B.abssum[r] = o.ab;
+      B.axis[r] = (int8_t)o.axis;
+    }
+  }
+}
+
+// One block per instance: sums over the instance's slice and its loop step.
+__global__ __launch_bounds__(kBlock) void k_batch_reduce(BatchBuffers B)
+{
+  __shared__ Partial ldsw[kMaxWaves];
+  const int i = blockIdx.x;
+  InstState st = B.state[i];
+  if (st.done) return;
+  const BatchParams& bp = B.params[i];
+  Partial q;
+  partial_zero(q);
+  for (int64_t r = st.start + threadIdx.x; r < st.start + st.count; r += kBlock) {
+    Partial one;
+    partial_from_region(one, B.est[r], B.err[r], B.abssum[r], true, bp.epsrel, bp.epsabs);
+    partial_add(q, one);
+  }
+  block_reduce_partial(q, ldsw, kBlock / kWave);
+  if (threadIdx.x != 0) return;
+  const double u = 0.5 * 2.220446049250313e-16;
+  const int64_t p = B.npts;
+  double rs = st.fin_s, rc = st.fin_c, es = st.fer_s, ec = st.fer_c;
+  dd_add(rs, rc, q.est_s, q.est_c);
+  dd_add(es, ec, q.err_s, q.err_c);
+  const double rG = rs + rc, eG = es + ec;
+  st.iters += 1;
+  st.nregions += st.count;
+  st.neval += st.count * p;
+  st.peak = st.count > st.peak ? st.count : st.peak;
+  st.integral = rG;
+  st.error = eG;
+  st.bound = (double)(p + 4) * u * (st.abs_fin + q.abs_all) + 2.0 * u * fabs(rG) +
+             (double)(st.nfin + st.count) * u * u * (st.aest_fin + q.aest_all);
+  dd_add(st.fin_s, st.fin_c, q.fest_s, q.fest_c);
+  dd_add(st.fer_s, st.fer_c, q.ferr_s, q.ferr_c);
+  st.abs_fin += q.abs_fin;
+  st.aest_fin += q.aest_fin;
+  st.nfin += st.count - q.nactive;
+  st.next = 0;
+  if (q.nnonfinite > 0) {
+    st.done = 1;
+    st.fail = CUHRE_ERR_NONFINITE;
+  } else if (eG <= fmax(bp.epsrel * fabs(rG), bp.epsabs)) {
+    st.done = 1;
+    st.fail = 0;
+  } else if (q.nactive == 0 || st.neval + 2 * q.nactive * p > bp.maxeval || st.iters >= 1000) {
+    st.done = 1;
+    st.fail = 1;
+  } else {
+    st.next = 2 * q.nactive;
+  }
+  B.state[i] = st;
+}
+
+// Slice offsets of the next list (exclusive scan of the children counts in
+// instance order).  If the list would exceed the capacity, the instances
+// from the first one that does not fit onward stop with fail = 1.
+__global__ __launch_bounds__(1024) void k_batch_scan(BatchBuffers B)
+{
+  __shared__ int64_t sc[1024];
+  __shared__ int s_cut;
+  __shared__ unsigned long long s_tot;
+  const int T = blockDim.x;
+  const int chunk = (B.count + T - 1) / T;
+  const int i0 = min((int)threadIdx.x * chunk, B.count);
+  const int i1 = min(i0 + chunk, B.count);
+  int64_t mine = 0;
+  for (int i = i0; i < i1; ++i) mine += B.state[i].done ? 0 : B.state[i].next;
+  sc[threadIdx.x] = mine;
+  if (threadIdx.x == 0) {
+    s_cut = B.count;
+    s_tot = 0ull;
+  }
+  __syncthreads();
+  for (int off = 1; off < T; off <<= 1) {
+    const int64_t v = (int)threadIdx.x >= off ? sc[threadIdx.x - off] : 0;
+    __syncthreads();
+    sc[threadIdx.x] += v;
+    __syncthreads();
+  }
+  const int64_t excl = sc[threadIdx.x] - mine;
+  int64_t run = excl;
+  for (int i = i0; i < i1; ++i) {
+    const int64_t c = B.state[i].done ? 0 : B.state[i].next;
+    if (run + c > B.cap) {
+      atomicMin(&s_cut, i);
+      break;
+    }
+    run += c;
+  }
+  __syncthreads();
+  const int cut = s_cut;
+  run = excl;
+  unsigned long long kept = 0ull;
+  for (int i = i0; i < i1; ++i) {
+    InstState& st = B.state[i];
+    const int64_t c = st.done ? 0 : st.next;
+    if (!st.done && i >= cut) {
+      st.done = 1;
+      st.fail = 1;
+      st.next = 0;
+    }
+    if (!st.done) {
+      st.start = run;
+      st.count = c;
+      kept += (unsigned long long)c;
+    } else {
+      st.count = 0;
+    }
+    run += c;
+  }
+  atomicAdd(&s_tot, kept);
+  __syncthreads();
+  if (threadIdx.x == 0) *B.n_next = (int64_t)s_tot;
+}
+
+__global__ __launch_bounds__(kBlock) void k_tile_count(BatchBuffers B)
+{
+  __shared__ uint32_t wcnt[kBlock / kWave];
+  const int64_t n = *B.n;
+  const int64_t ntiles = (n + kBlock - 1) / kBlock;
+  for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int64_t r = t * kBlock + threadIdx.x;
+    bool act = false;
+    if (r < n) {
+      const int i = B.inst[r];
+      act = batch_active(B.state[i], B.params[i], B.est[r], B.err[r]);
+    }
+    const uint64_t mask = __ballot(act);
+    if ((threadIdx.x & (kWave - 1)) == 0) wcnt[threadIdx.x / kWave] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    if (threadIdx.x == 0) B.tile_cnt[t] = (int64_t)wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_tile_scan(BatchBuffers B)
+{
+  __shared__ int64_t sc[1024];
+  const int64_t n = *B.n;
+  const int64_t ntiles = (n + kBlock - 1) / kBlock;
+  const int T = blockDim.x;
+  const int64_t chunk = (ntiles + T - 1) / T;
+  const int64_t t0 = threadIdx.x * chunk;
+  const int64_t t1 = t0 + chunk < ntiles ? t0 + chunk : ntiles;
+  int64_t mine = 0;
+  for (int64_t t = t0; t < t1; ++t) mine += B.tile_cnt[t];
+  sc[threadIdx.x] = mine;
+  __syncthreads();
+  for (int off = 1; off < T; off <<= 1) {
+    const int64_t v = threadIdx.x >= off ? sc[threadIdx.x - off] : 0;
+    __syncthreads();
+    sc[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int64_t run = sc[threadIdx.x] - mine;
+  for (int64_t t = t0; t < t1; ++t) {
+    B.tile_off[t] = run;
+    run += B.tile_cnt[t];
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_batch_split(BatchBuffers B)
+{
+  __shared__ uint32_t wcnt[kBlock / kWave];
+  const int64_t n = *B.n;
+  const int64_t ntiles = (n + kBlock - 1) / kBlock;
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  const uint64_t below = lane ? (~0ull >> (kWave - lane)) : 0ull;
+  for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int64_t r = t * kBlock + threadIdx.x;
+    bool act = false;
+    int i = 0;
+    if (r < n) {
+      i = B.inst[r];
+      act = batch_active(B.state[i], B.params[i], B.est[r], B.err[r]);
+    }
+    const uint64_t mask = __ballot(act);
+    if (lane == 0) wcnt[wave] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    int64_t rank = __popcll(mask & below);
+    for (int w = 0; w < wave; ++w) rank += wcnt[w];
+    __syncthreads();
+    if (!act) continue;
+    const int64_t dst = B.tile_off[t] + rank;
+    const int a = B.axis[r];
+    for (int m = 0; m < B.dim; ++m) {
+      const double lo = B.lower[m * B.ld + r];
+      const double w = B.width[m * B.ld + r];
+      double2 L, Wd;
+      if (m == a) {
+        const double hw = 0.5 * w;
+        L = make_double2(lo, lo + hw);
+        Wd = make_double2(hw, hw);
+      } else {
+        L = make_double2(lo, lo);
+        Wd = make_double2(w, w);
+      }
+      *reinterpret_cast<double2*>(B.out_lower + m * B.ld + 2 * dst) = L;
+      *reinterpret_cast<double2*>(B.out_width + m * B.ld + 2 * dst) = Wd;
+    }
+    *reinterpret_cast<int2*>(B.out_inst + 2 * dst) = make_int2(i, i);
+  }
+}
+
+// Every region of the list split on axis (round mod d): a uniform
+// refinement used to build autotuning samples.
+__global__ __launch_bounds__(kBlock) void k_refine_all(BatchBuffers B, int64_t n, int axis)
+{
+  const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (r >= n) return;
+  for (int m = 0; m < B.dim; ++m) {
+    const double lo = B.lower[m * B.ld + r];
+    const double w = B.width[m * B.ld + r];
+    double2 L, Wd;
+    if (m == axis) {
+      const double hw = 0.5 * w;
+      L = make_double2(lo, lo + hw);
+      Wd = make_double2(hw, hw);
+    } else {
+      L = make_double2(lo, lo);
+      Wd = make_double2(w, w);
+    }
+    *reinterpret_cast<double2*>(B.out_lower + m * B.ld + 2 * r) = L;
+    *reinterpret_cast<double2*>(B.out_width + m * B.ld + 2 * r) = Wd;
+  }
+  const int i = B.inst[r];
+  *reinterpret_cast<int2*>(B.out_inst + 2 * r) = make_int2(i, i);
+}
+
+template <int D>
+static void launch_batch_eval(const BatchBuffers& B, const RuleConst& R, int S, int grid,
+                              unsigned fam_mask, hipStream_t st)
+{
+  const dim3 g(grid), b(kWave * seg_waves(S));
+  const size_t lds = seg_lds_bytes(S);
+  if (fam_mask & (1u << CUHRE_GAUSSIAN))
+    hipLaunchKernelGGL((k_batch_eval<D, CUHRE_GAUSSIAN>), g, b, lds, st, B, R, S);
+  if (fam_mask & (1u << CUHRE_PRODUCT_PEAK))
+    hipLaunchKernelGGL((k_batch_eval<D, CUHRE_PRODUCT_PEAK>), g, b, lds, st, B, R, S);
+  if (fam_mask & (1u << CUHRE_OSCILLATORY))
+    hipLaunchKernelGGL((k_batch_eval<D, CUHRE_OSCILLATORY>), g, b, lds, st, B, R, S);
+  if (fam_mask & (1u << CUHRE_CORNER_PEAK))
+    hipLaunchKernelGGL((k_batch_eval<D, CUHRE_CORNER_PEAK>), g, b, lds, st, B, R, S);
+  if (fam_mask & (1u << CUHRE_CONTINUOUS))
+    hipLaunchKernelGGL((k_batch_eval<D, CUHRE_CONTINUOUS>), g, b, lds, st, B, R, S);
+}
+
+using BatchEvalLauncher = void (*)(const BatchBuffers&, const RuleConst&, int, int, unsigned,
+                                   hipStream_t);
+
+BatchEvalLauncher batch_eval_launcher(int dim)
+{
+  switch (dim) {
+  case 2: return &launch_batch_eval<2>;
+  case 3: return &launch_batch_eval<3>;
+  case 4: return &launch_batch_eval<4>;
+  case 5: return &launch_batch_eval<5>;
+  case 6: return &launch_batch_eval<6>;
+  case 7: return &launch_batch_eval<7>;
+  case 8: return &launch_batch_eval<8>;
+  case 9: return &launch_batch_eval<9>;
+  case 10: return &launch_batch_eval<10>;
+  default: return nullptr;
+  }
+}
+
+// Host side of the batch engine (state lives in the context).
+struct BatchEngine {
+  int64_t cap = 0;
+  int dim_alloc = 0;
+  int count_alloc = 0;
+  double* lo[2] = {nullptr, nullptr};
+  double* wd[2] = {nullptr, nullptr};
+  int32_t* inst[2] = {nullptr, nullptr};
+  double *est = nullptr, *err = nullptr, *abssum = nullptr;
+  int8_t* axis = nullptr;
+  int64_t *tile_cnt = nullptr, *tile_off = nullptr;
+  int64_t* n = nullptr;  // [2]
+  BatchParams* params = nullptr;
+  InstState* state = nullptr;
+  std::vector<InstState> h_state;
+  std::vector<BatchParams> h_params;
+  std::vector<int> perm;  // list position k holds caller instance perm[k]
+  unsigned fam_mask = 0;
+  int64_t h_n[2] = {0, 0};
+  // cached graph of two iterations for one (count, dim, cap, S, grid) shape
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  int g_count = -1, g_dim = -1, g_S = -1, g_grid = -1;
+  unsigned g_mask = 0;
+  int64_t g_cap = -1;
+  // autotune cache
+  int tuned_dim = -1, tuned_S = 0, tuned_grid = 0;
+  uint64_t tuned_key = 0;
+};
+
+static void free_batch(BatchEngine* e)
+{
+  for (int p = 0; p < 2; ++p) {
+    if (e->lo[p]) (void)hipFree(e->lo[p]);
+    if (e->wd[p]) (void)hipFree(e->wd[p]);
+    if (e->inst[p]) (void)hipFree(e->inst[p]);
+    e->lo[p] = e->wd[p] = nullptr;
+    e->inst[p] = nullptr;
+  }
+  for (void* q : {(void*)e->est, (void*)e->err, (void*)e->abssum, (void*)e->axis, (void*)e->tile_cnt,
+                  (void*)e->tile_off, (void*)e->n, (void*)e->params, (void*)e->state})
+    if (q) (void)hipFree(q);
+  e->est = e->err = e->abssum = nullptr;
+  e->axis = nullptr;
+  e->tile_cnt = e->tile_off = e->n = nullptr;
+  e->params = nullptr;
+  e->state = nullptr;
+  if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+  if (e->graph) (void)hipGraphDestroy(e->graph);
+  e->gexec = nullptr;
+  e->graph = nullptr;
+  e->g_count = -1;
+  e->cap = 0;
+}
+
+BatchEngine* batch_engine_new() { return new BatchEngine(); }
+void batch_engine_delete(BatchEngine* e)
+{
+  if (!e) return;
+  free_batch(e);
+  delete e;
+}
+
+static int batch_reserve(BatchEngine* e, int64_t cap, int dim, int count)
+{
+  if (cap <= e->cap && dim <= e->dim_alloc && count <= e->count_alloc) return 0;
+  cap = std::max(cap, e->cap);
+  dim = std::max(dim, e->dim_alloc);
+  count = std::max(count, e->count_alloc);
+  free_batch(e);
+  cap = (cap + 255) & ~255ll;
+  const size_t lb = sizeof(double) * (size_t)cap * dim;
+  const int64_t ntiles = cap / kBlock + 1;
+  bool ok = true;
+  for (int p = 0; p < 2; ++p) {
+    ok = ok && hipMalloc(&e->lo[p], lb) == hipSuccess && hipMalloc(&e->wd[p], lb) == hipSuccess &&
+         hipMalloc(&e->inst[p], sizeof(int32_t) * cap) == hipSuccess;
+  }
+  ok = ok && hipMalloc(&e->est, sizeof(double) * cap) == hipSuccess &&
+       hipMalloc(&e->err, sizeof(double) * cap) == hipSuccess &&
+       hipMalloc(&e->abssum, sizeof(double) * cap) == hipSuccess && hipMalloc(&e->axis, cap) == hipSuccess &&
+       hipMalloc(&e->tile_cnt, sizeof(int64_t) * ntiles) == hipSuccess &&
+       hipMalloc(&e->tile_off, sizeof(int64_t) * ntiles) == hipSuccess &&
+       hipMalloc(&e->n, sizeof(int64_t) * 2) == hipSuccess &&
+       hipMalloc(&e->params, sizeof(BatchParams) * count) == hipSuccess &&
+       hipMalloc(&e->state, sizeof(InstState) * count) == hipSuccess;
+  if (!ok) {
+    (void)hipGetLastError();
+    free_batch(e);
+    return set_error(CUHRE_ERR_ALLOC, "cannot allocate the batch region list");
+  }
+  e->cap = cap;
+  e->dim_alloc = dim;
+  e->count_alloc = count;
+  return 0;
+}
+
+static BatchBuffers make_buffers(BatchEngine* e, int parity, int count, int dim)
+{
+  BatchBuffers B;
+  B.lower = e->lo[parity];
+  B.width = e->wd[parity];
+  B.inst = e->inst[parity];
+  B.out_lower = e->lo[1 - parity];
+  B.out_width = e->wd[1 - parity];
+  B.out_inst = e->inst[1 - parity];
+  B.ld = e->cap;
+  B.n = e->n + parity;
+  B.n_next = e->n + (1 - parity);
+  B.est = e->est;
+  B.err = e->err;
+  B.abssum = e->abssum;
+  B.axis = e->axis;
+  B.tile_cnt = e->tile_cnt;
+  B.tile_off = e->tile_off;
+  B.params = e->params;
+  B.state = e->state;
+  B.count = count;
+  B.dim = dim;
+  B.npts = gm_points(dim);
+  B.cap = e->cap;
+  return B;
+}
+
+static void enqueue_iteration(BatchEngine* e, int parity, int count, int dim, const RuleConst& R,
+                              int S, int grid, hipStream_t st)
+{
+  const BatchBuffers B = make_buffers(e, parity, count, dim);
+  batch_eval_launcher(dim)(B, R, S, grid, e->fam_mask, st);
+  hipLaunchKernelGGL(k_batch_reduce, dim3(count), dim3(kBlock), 0, st, B);
+  hipLaunchKernelGGL(k_batch_scan, dim3(1), dim3(1024), 0, st, B);
+  hipLaunchKernelGGL(k_tile_count, dim3(grid), dim3(kBlock), 0, st, B);
+  hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, st, B);
+  hipLaunchKernelGGL(k_batch_split, dim3(grid), dim3(kBlock), 0, st, B);
+}
+
+static int check_batch_args(int count, const cuhre_integrand* fs, int dim)
+{
+  if (count <= 0 || !fs) return set_error(CUHRE_ERR_ARG, "empty batch");
+  if (dim < 2 || dim > 10) return set_error(CUHRE_ERR_UNSUPPORTED, "batch dimension outside 2..10");
+  for (int i = 0; i < count; ++i) {
+    const int f = fs[i].family;
+    if (fs[i].dim != dim) return set_error(CUHRE_ERR_ARG, "instance dimension differs from dim");
+    if (f != CUHRE_GAUSSIAN && f != CUHRE_PRODUCT_PEAK && f != CUHRE_OSCILLATORY &&
+        f != CUHRE_CORNER_PEAK && f != CUHRE_CONTINUOUS)
+      return set_error(CUHRE_ERR_UNSUPPORTED, "family not available in batches");
+    if (f == CUHRE_CORNER_PEAK && fs[i].exponent < 0)
+      return set_error(CUHRE_ERR_ARG, "corner peak exponent must be >= 0");
+  }
+  return 0;
+}
+
+// Parameters in list order: instances sorted (stably) by family.
+static void fill_params(std::vector<BatchParams>& hp, std::vector<int>& perm, unsigned& mask, int count,
+                        const cuhre_integrand* fs, int dim, const double* epsrel, const double* epsabs,
+                        const int64_t* maxeval, const double* lows, const double* highs)
+{
+  perm.resize(count);
+  for (int i = 0; i < count; ++i) perm[i] = i;
+  std::stable_sort(perm.begin(), perm.end(),
+                   [&](int x, int y) { return fs[x].family < fs[y].family; });
+  mask = 0;
+  hp.assign(count, BatchParams{});
+  for (int k = 0; k < count; ++k) {
+    const int i = perm[k];
+    mask |= 1u << fs[i].family;
+    BatchParams& b = hp[k];
+    std::memset(&b, 0, sizeof b);
+    const cuhre_integrand& f = fs[i];
+    b.family = f.family;
+    b.exponent = f.exponent;
+    for (int m = 0; m < kMaxDim; ++m) {
+      b.a[m] = f.a[m];
+      b.w[m] = f.w[m];
+      b.b[m] = (m < dim && f.family == CUHRE_PRODUCT_PEAK) ? 1.0 / (f.a[m] * f.a[m]) : 0.0;
+    }
+    b.s0 = 2.0 * M_PI * f.u;
+    b.epsrel = epsrel ? epsrel[i] : 1e-3;
+    b.epsabs = epsabs ? epsabs[i] : 0.0;
+    b.maxeval = maxeval ? maxeval[i] : 1000000000LL;
+    for (int m = 0; m < dim; ++m) {
+      b.lows[m] = lows ? lows[(size_t)i * dim + m] : 0.0;
+      b.highs[m] = highs ? highs[(size_t)i * dim + m] : 1.0;
+    }
+  }
+}
+
+static int grid_k_batch(int d) { return d <= 4 ? 4 : (d <= 10 ? 2 : 1); }
+
+// Uploads the batch and builds its initial list in buffer 0.
+static int batch_setup(BatchEngine* e, hipStream_t st, int count, const cuhre_integrand* fs, int dim,
+                       const double* epsrel, const double* epsabs, const int64_t* maxeval,
+                       const double* lows, const double* highs, int64_t cap, int64_t* n0_out)
+{
+  int k = grid_k_batch(dim);
+  int64_t n0 = 1;
+  for (int m = 0; m < dim; ++m) n0 *= k;
+  const int64_t total = n0 * count;
+  if (2 * total > cap) cap = 2 * total;
+  int rc = batch_reserve(e, cap, dim, count);
+  if (rc) return rc;
+  std::vector<BatchParams>& hp = e->h_params;
+  fill_params(hp, e->perm, e->fam_mask, count, fs, dim, epsrel, epsabs, maxeval, lows, highs);
+  if (hipMemcpyAsync(e->params, hp.data(), sizeof(BatchParams) * count, hipMemcpyHostToDevice, st) !=
+      hipSuccess)
+    return set_error(CUHRE_ERR_DEVICE, "batch parameter upload failed");
+  const BatchBuffers B = make_buffers(e, 0, count, dim);
+  const int64_t nthreads = std::max<int64_t>(total, count);
+  hipLaunchKernelGGL(k_batch_init, dim3((unsigned)((nthreads + 255) / 256)), dim3(256), 0, st, B, k, n0);
+  e->h_n[0] = total;
+  e->h_n[1] = 0;
+  if (hipMemcpyAsync(e->n, e->h_n, sizeof e->h_n, hipMemcpyHostToDevice, st) != hipSuccess)
+    return set_error(CUHRE_ERR_DEVICE, "batch setup copy failed");
+  if (hipGetLastError() != hipSuccess) return set_error(CUHRE_ERR_DEVICE, "batch init launch failed");
+  *n0_out = total;
+  return 0;
+}
+
+}  // namespace cuhre
+
+using namespace cuhre;
+
+// The context owns one batch engine (defined in engine.hip).
+extern "C" BatchEngine* cuhre_ctx_batch(cuhre_context* c, hipStream_t* stream, int* device, int64_t* capacity);
+
+extern "C" {
+
+int cuhre_batch_autotune(cuhre_context* ctx, int32_t count, const cuhre_integrand* fs, int dim,
+                         const double* lows, const double* highs, cuhre_batch_config* best,
+                         double* table_ms)
+{
+  if (!ctx || !best) return set_error(CUHRE_ERR_ARG, "null argument");
+  int rc = check_batch_args(count, fs, dim);
+  if (rc) return rc;
+  hipStream_t st;
+  int device;
+  int64_t capacity;
+  BatchEngine* e = cuhre_ctx_batch(ctx, &st, &device, &capacity);
+  if (hipSetDevice(device) != hipSuccess) return set_error(CUHRE_ERR_DEVICE, "hipSetDevice failed");
+  // sample list: the batch's initial grids refined uniformly to >= 512k regions
+  int64_t n0 = 0;
+  const int64_t want = 1 << 19;
+  int64_t cap = want * 4;
+  rc = batch_setup(e, st, count, fs, dim, nullptr, nullptr, nullptr, lows, highs,
+                   std::min<int64_t>(capacity, cap), &n0);
+  if (rc) return rc;
+  int64_t n = n0;
+  int parity = 0, round = 0;
+  while (n < want && 2 * n <= e->cap) {
+    const BatchBuffers B = make_buffers(e, parity, count, dim);
+    hipLaunchKernelGGL(k_refine_all, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, B, n,
+                       round % dim);
+    n *= 2;
+    parity = 1 - parity;
+    ++round;
+  }
+  e->h_n[parity] = n;
+  if (hipMemcpyAsync(e->n + parity, e->h_n + parity, sizeof n, hipMemcpyHostToDevice, st) != hipSuccess)
+    return set_error(CUHRE_ERR_DEVICE, "autotune copy failed");
+  const RuleConst R = make_rule(dim);
+  const BatchBuffers B = make_buffers(e, parity, count, dim);
+  const int Ss[3] = {1, 2, 4};
+  const int Gs[5] = {256, 512, 1024, 2048, 4096};
+  hipEvent_t a, b;
+  if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess)
+    return set_error(CUHRE_ERR_DEVICE, "event create failed");
+  float best_ms = 1e30f;
+  int bs = 1, bg = 1024, idx = 0;
+  for (int S : Ss)
+    for (int G : Gs) {
+      batch_eval_launcher(dim)(B, R, S, G, e->fam_mask, st);  // warm
+      (void)hipEventRecord(a, st);
+      for (int rep = 0; rep < 3; ++rep) batch_eval_launcher(dim)(B, R, S, G, e->fam_mask, st);
+      (void)hipEventRecord(b, st);
+      (void)hipEventSynchronize(b);
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, a, b);
+      ms /= 3.0f;
+      if (table_ms) table_ms[idx] = ms;
+      ++idx;
+      if (ms < best_ms) {
+        best_ms = ms;
+        bs = S;
+        bg = G;
+      }
+    }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  if (hipGetLastError() != hipSuccess) return set_error(CUHRE_ERR_DEVICE, "autotune launch failed");
+  best->point_segments = bs;
+  best->grid_blocks = bg;
+  best->iterations_per_check = 2;
+  best->use_graph = 1;
+  e->tuned_dim = dim;
+  e->tuned_S = bs;
+  e->tuned_grid = bg;
+  return 0;
+}
+
+int cuhre_integrate_batch(cuhre_context* ctx, int32_t count, const cuhre_integrand* fs, int dim,
+                          const double* epsrel, const double* epsabs, const int64_t* maxeval,
+                          const double* lows, const double* highs, const cuhre_batch_config* cfg,
+                          cuhre_result* out, cuhre_batch_info* info)
+{
+  const auto t0 = std::chrono::steady_clock::now();
+  if (!ctx || !out) return set_error(CUHRE_ERR_ARG, "null argument");
+  int rc = check_batch_args(count, fs, dim);
+  if (rc) return rc;
+  hipStream_t st;
+  int device;
+  int64_t capacity;
+  BatchEngine* e = cuhre_ctx_batch(ctx, &st, &device, &capacity);
+  if (hipSetDevice(device) != hipSuccess) return set_error(CUHRE_ERR_DEVICE, "hipSetDevice failed");
+  int S = cfg && cfg->point_segments > 0 ? cfg->point_segments : 0;
+  int G = cfg && cfg->grid_blocks > 0 ? cfg->grid_blocks : 0;
+  if (S == 0 || G == 0) {
+    if (e->tuned_dim == dim) {
+      S = S ? S : e->tuned_S;
+      G = G ? G : e->tuned_grid;
+    } else {
+      S = S ? S : 1;
+      G = G ? G : 2048;
+    }
+  }
+  if (S != 1 && S != 2 && S != 4) return set_error(CUHRE_ERR_ARG, "batch point_segments must be 1, 2 or 4");
+  const int per_check = cfg && cfg->iterations_per_check > 0 ? cfg->iterations_per_check : 2;
+  const bool use_graph = !cfg || cfg->use_graph != 0;
+  const int64_t cap = std::min<int64_t>(capacity, 32ll << 20);
+  int64_t n0 = 0;
+  rc = batch_setup(e, st, count, fs, dim, epsrel, epsabs, maxeval, lows, highs, cap, &n0);
+  if (rc) return rc;
+  const RuleConst R = make_rule(dim);
+
+  bool graph_ok = false;
+  if (use_graph) {
+    const bool same = e->gexec && e->g_count == count && e->g_dim == dim && e->g_S == S &&
+                      e->g_grid == G && e->g_cap == e->cap && e->g_mask == e->fam_mask;
+    if (same) {
+      graph_ok = true;
+    } else {
+      if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+      if (e->graph) (void)hipGraphDestroy(e->graph);
+      e->gexec = nullptr;
+      e->graph = nullptr;
+      if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+        enqueue_iteration(e, 0, count, dim, R, S, G, st);
+        enqueue_iteration(e, 1, count, dim, R, S, G, st);
+        if (hipStreamEndCapture(st, &e->graph) == hipSuccess &&
+            hipGraphInstantiate(&e->gexec, e->graph, nullptr, nullptr, 0) == hipSuccess) {
+          graph_ok = true;
+          e->g_count = count;
+          e->g_dim = dim;
+          e->g_S = S;
+          e->g_grid = G;
+          e->g_cap = e->cap;
+          e->g_mask = e->fam_mask;
+        }
+      }
+      (void)hipGetLastError();
+    }
+  }
+  int64_t h_n[2] = {n0, 0};
+  (void)n0;
+  int iters = 0;
+  int64_t total_regions = 0, peak = 0;
+  for (int check = 0; check < 1000; ++check) {
+    for (int k = 0; k < per_check; ++k) {
+      if (graph_ok) {
+        if (hipGraphLaunch(e->gexec, st) != hipSuccess) return set_error(CUHRE_ERR_DEVICE, "graph launch failed");
+      } else {
+        enqueue_iteration(e, 0, count, dim, R, S, G, st);
+        enqueue_iteration(e, 1, count, dim, R, S, G, st);
+      }
+      iters += 2;
+    }
+    if (hipGetLastError() != hipSuccess) return set_error(CUHRE_ERR_DEVICE, "batch iteration launch failed");
+    if (hipMemcpyAsync(h_n, e->n, sizeof h_n, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return set_error(CUHRE_ERR_DEVICE, "batch sync failed");
+    if (h_n[0] == 0) break;  // an even number of iterations leaves the list in buffer 0
+  }
+  e->h_state.resize(count);
+  if (hipMemcpyAsync(e->h_state.data(), e->state, sizeof(InstState) * count, hipMemcpyDeviceToHost, st) !=
+          hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return set_error(CUHRE_ERR_DEVICE, "batch result copy failed");
+  int worst = 0, nonconv = 0, maxit = 0;
+  const int64_t p = gm_points(dim);
+  for (int k = 0; k < count; ++k) {
+    const InstState& s = e->h_state[k];
+    cuhre_result& r = out[e->perm[k]];
+    std::memset(&r, 0, sizeof r);
+    r.integral = s.integral;
+    r.error = s.error;
+    r.fp64_err_bound = s.bound;
+    r.nregions = s.nregions;
+    r.neval = s.nregions * p;
+    r.fail = s.fail > 0 ? s.fail : 0;
+    r.iterations = s.iters;
+    r.peak_regions = s.peak;
+    r.nfinished = s.nfin;
+    total_regions += s.nregions;
+    peak = std::max(peak, s.peak);
+    maxit = std::max(maxit, s.iters);
+    if (s.fail < 0) worst = s.fail;
+    if (s.fail > 0 || !s.done) ++nonconv;
+  }
+  const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (info) {
+    std::memset(info, 0, sizeof *info);
+    info->iterations = maxit;
+    info->launched_iterations = iters;
+    info->total_regions = total_regions;
+    info->total_neval = total_regions * p;
+    info->not_converged = nonconv;
+    info->ms_total = ms;
+    info->graph_used = graph_ok ? 1 : 0;
+    info->point_segments = S;
+    info->grid_blocks = G;
+  }
+  for (int i = 0; i < count; ++i) out[i].ms_total = ms;
+  if (worst < 0) return set_error(worst, "an integrand returned a non-finite value in some instance");
+  return nonconv ? CUHRE_NOT_CONVERGED : CUHRE_OK;
+}
+
+}  // extern "C"

@@ -75,6 +75,12 @@ static int fail(int code, const char* fmt, ...)
                   __FILE__, __LINE__);                                                  \
   } while (0)
 
+int set_error(int code, const char* msg) { return fail(code, "%s", msg); }
+
+struct BatchEngine;
+BatchEngine* batch_engine_new();
+void batch_engine_delete(BatchEngine* e);
+
 // ------------------------------------------------------------------ rule
 
 int gm_points(int d) { return (1 << d) + 2 * d * d + 2 * d + 1; }
@@ -305,7 +311,18 @@ struct cuhre_context {
   std::vector<double> table_host;
 
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  BatchEngine* batch = nullptr;
 };
+
+extern "C" __attribute__((visibility("hidden"))) BatchEngine* cuhre_ctx_batch(cuhre_context* c, hipStream_t* stream, int* device,
+                                        int64_t* capacity)
+{
+  if (!c->batch) c->batch = batch_engine_new();
+  *stream = c->stream;
+  *device = c->device;
+  *capacity = c->capacity;
+  return c->batch;
+}
 
 static int64_t blocks_for(int64_t n, int S)
 {
@@ -595,6 +612,8 @@ int cuhre_context_destroy(cuhre_context* c)
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   ctx_free_lists(c);
+  batch_engine_delete(c->batch);
+  c->batch = nullptr;
   if (c->aux) (void)hipFree(c->aux);
   if (c->ticket) (void)hipFree(c->ticket);
   if (c->state) (void)hipFree(c->state);

@@ -45,27 +45,39 @@ inline size_t seg_lds_bytes(int S)
 inline int seg_waves(int S) { return S < 4 ? 4 : S; }
 inline int seg_regions_per_block(int S) { return kWave * (seg_waves(S) / S); }
 
+// Integrand of the plain kernel: one family, parameters in kernel arguments.
 template <int D, int FAM>
-__global__ __launch_bounds__(1024) void k_eval_seg(EvalArgs A, DevIntegrand P, RuleConst R, int S)
-{
-  using F = Family<FAM, D>;
-  extern __shared__ double seg_lds[];
-  __shared__ Partial ldsw[kMaxWaves];
-  const int W = S < 4 ? 4 : S;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int lane = threadIdx.x & (kWave - 1);
-  const int g = wave / S, s = wave - g * S;
-  const int64_t r = (int64_t)blockIdx.x * (kWave * (W / S)) + g * kWave + lane;
-  const bool valid = r < A.n;
-  const int64_t rr = valid ? r : A.n - 1;
+struct PlainIntegrand {
+  const DevIntegrand& P;
+  __device__ __forceinline__ double operator()(const double (&x)[D]) const
+  {
+    return Family<FAM, D>::eval(P, x);
+  }
+};
 
+// Per-region outputs of seg_eval (valid in segment 0 of each group).
+struct RegionOut {
+  double est, err, ab, i5;
+  int axis;
+};
+
+// Segmented Genz-Malik evaluation of the region held by this lane (see the
+// file comment).  All waves of the block must call it together when S > 1
+// (it contains a block barrier).  F(x) evaluates the lane's integrand.
+template <int D, class F>
+__device__ __forceinline__ RegionOut seg_eval(const F& f, const double* __restrict__ lower,
+                                              const double* __restrict__ width, int64_t ld,
+                                              int64_t r, bool valid, int64_t rr, int S, int s,
+                                              int wave, int lane, int W, double* seg_lds,
+                                              const RuleConst& R, double* dvec)
+{
   double c[D], h[D], x[D];
   double vol = 1.0, maxw = 0.0;
   int widest = 0;
 #pragma unroll
   for (int m = 0; m < D; ++m) {
-    const double lo = A.lower[m * A.ld + rr];
-    const double w = A.width[m * A.ld + rr];
+    const double lo = lower[m * ld + rr];
+    const double w = width[m * ld + rr];
     c[m] = fma(0.5, w, lo);
     h[m] = 0.5 * w;
     vol *= w;
@@ -77,14 +89,13 @@ __global__ __launch_bounds__(1024) void k_eval_seg(EvalArgs A, DevIntegrand P, R
 
   // every segment needs f at the centre for its fourth differences; only
   // segment 0 adds its weight
-  const double f0 = F::eval(P, c);
+  const double f0 = f(c);
   const double c0 = s == 0 ? 1.0 : 0.0;
   double s7 = fma(c0 * R.w7c, f0, 0.0);
   double s5 = fma(c0 * R.w5c, f0, 0.0);
   double sa = fma(c0 * fabs(R.w7c), fabs(f0), 0.0);
   double maxd = 0.0;
   int ax = -1;
-  double* const dvec = A.dvec;
 
   // axis points: four per axis, then the fourth difference of that axis
 #pragma unroll 1
@@ -95,7 +106,7 @@ __global__ __launch_bounds__(1024) void k_eval_seg(EvalArgs A, DevIntegrand P, R
       const double gv = t == 0 ? R.l2 : (t == 1 ? -R.l2 : (t == 2 ? R.l3 : -R.l3));
 #pragma unroll
       for (int m = 0; m < D; ++m) x[m] = fma(h[m], m == i ? gv : 0.0, c[m]);
-      fv[t] = F::eval(P, x);
+      fv[t] = f(x);
       const double w7 = t < 2 ? R.w7a2 : R.w7a3;
       const double w5 = t < 2 ? R.w5a2 : R.w5a3;
       s7 = fma(w7, fv[t], s7);
@@ -105,7 +116,7 @@ __global__ __launch_bounds__(1024) void k_eval_seg(EvalArgs A, DevIntegrand P, R
     const double t2 = fma(-2.0, f0, fv[0] + fv[1]);
     const double t3 = fma(-2.0, f0, fv[2] + fv[3]);
     const double di = fabs(fma(-R.ratio, t3, t2));
-    if (dvec != nullptr && valid) dvec[i * A.ld + r] = di;
+    if (dvec != nullptr && valid) dvec[i * ld + r] = di;
     if (di > maxd) {
       maxd = di;
       ax = i;
@@ -128,7 +139,7 @@ __global__ __launch_bounds__(1024) void k_eval_seg(EvalArgs A, DevIntegrand P, R
       const double gj = (t & 1) ? -R.l4 : R.l4;
 #pragma unroll
       for (int m = 0; m < D; ++m) x[m] = fma(h[m], m == i ? gi : (m == j ? gj : 0.0), c[m]);
-      const double v = F::eval(P, x);
+      const double v = f(x);
       s7 = fma(R.w7p, v, s7);
       s5 = fma(R.w5p, v, s5);
       sa = fma(fabs(R.w7p), fabs(v), sa);
@@ -140,7 +151,7 @@ __global__ __launch_bounds__(1024) void k_eval_seg(EvalArgs A, DevIntegrand P, R
   for (int k = s; k < (1 << D); k += S) {
 #pragma unroll
     for (int m = 0; m < D; ++m) x[m] = fma(h[m], ((k >> m) & 1) ? -R.l5 : R.l5, c[m]);
-    const double v = F::eval(P, x);
+    const double v = f(x);
     s7 = fma(R.w7v, v, s7);
     sa = fma(fabs(R.w7v), fabs(v), sa);
   }
@@ -172,23 +183,45 @@ __global__ __launch_bounds__(1024) void k_eval_seg(EvalArgs A, DevIntegrand P, R
         }
       }
     }
+    __syncthreads();  // the LDS block may be reused by the caller
   }
   if (ax < 0) ax = widest;  // every fourth difference is zero
 
+  RegionOut o;
+  o.est = vol * s7;
+  o.err = vol * fabs(s7 - s5);
+  o.ab = vol * sa;
+  o.i5 = vol * s5;
+  o.axis = ax;
+  return o;
+}
+
+template <int D, int FAM>
+__global__ __launch_bounds__(1024) void k_eval_seg(EvalArgs A, DevIntegrand P, RuleConst R, int S)
+{
+  extern __shared__ double seg_lds[];
+  __shared__ Partial ldsw[kMaxWaves];
+  const int W = S < 4 ? 4 : S;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & (kWave - 1);
+  const int g = wave / S, s = wave - g * S;
+  const int64_t r = (int64_t)blockIdx.x * (kWave * (W / S)) + g * kWave + lane;
+  const bool valid = r < A.n;
+  const int64_t rr = valid ? r : A.n - 1;
+  const PlainIntegrand<D, FAM> f{P};
+  const RegionOut o = seg_eval<D>(f, A.lower, A.width, A.ld, r, valid, rr, S, s, wave, lane, W,
+                                  seg_lds, R, A.dvec);
   const bool owner = s == 0;
-  const double est = vol * s7;
-  const double err = vol * fabs(s7 - s5);
-  const double ab = vol * sa;
   if (valid && owner) {
-    A.est[r] = est;
-    A.err[r] = err;
-    A.axis[r] = (int8_t)ax;
-    if (A.abssum) A.abssum[r] = ab;
-    if (A.i5) A.i5[r] = vol * s5;
+    A.est[r] = o.est;
+    A.err[r] = o.err;
+    A.axis[r] = (int8_t)o.axis;
+    if (A.abssum) A.abssum[r] = o.ab;
+    if (A.i5) A.i5[r] = o.i5;
   }
   if (A.finalize) {
     Partial p;
-    partial_from_region(p, est, err, ab, valid && owner, A.epsrel, A.epsabs);
+    partial_from_region(p, o.est, o.err, o.ab, valid && owner, A.epsrel, A.epsabs);
     block_reduce_partial(p, ldsw, W, owner);
     finish_iteration(A, p, ldsw, W);
   }

@@ -32,7 +32,7 @@ ERR_CAPACITY = -6
 EXPORTS = [
     "cuhre_context_create", "cuhre_context_destroy", "cuhre_context_set",
     "cuhre_integrate", "cuhre_eval_regions", "cuhre_split_regions", "cuhre_sum",
-    "cuhre_last_error", "cuhre_points_per_region",
+    "cuhre_last_error", "cuhre_points_per_region", "cuhre_integrate_batch", "cuhre_batch_autotune",
 ]
 
 
@@ -80,6 +80,35 @@ class Result(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class BatchConfig(C.Structure):
+    _fields_ = [
+        ("point_segments", C.c_int32),
+        ("grid_blocks", C.c_int32),
+        ("iterations_per_check", C.c_int32),
+        ("use_graph", C.c_int32),
+    ]
+
+    def asdict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class BatchInfo(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int32),
+        ("launched_iterations", C.c_int32),
+        ("total_regions", C.c_int64),
+        ("total_neval", C.c_int64),
+        ("not_converged", C.c_int32),
+        ("graph_used", C.c_int32),
+        ("point_segments", C.c_int32),
+        ("grid_blocks", C.c_int32),
+        ("ms_total", C.c_double),
+    ]
+
+    def asdict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class CuhreError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"cuhre error {code}: {msg}")
@@ -108,6 +137,12 @@ def lib():
                                           C.POINTER(C.c_int32), C.c_double, C.c_double, dp, dp,
                                           C.POINTER(C.c_int64)]
         L.cuhre_sum.argtypes = [ctxp, C.c_int64, dp, dp]
+        i64p = C.POINTER(C.c_int64)
+        L.cuhre_integrate_batch.argtypes = [ctxp, C.c_int32, C.POINTER(Integrand), C.c_int, dp, dp, i64p,
+                                            dp, dp, C.POINTER(BatchConfig), C.POINTER(Result),
+                                            C.POINTER(BatchInfo)]
+        L.cuhre_batch_autotune.argtypes = [ctxp, C.c_int32, C.POINTER(Integrand), C.c_int, dp, dp,
+                                           C.POINTER(BatchConfig), dp]
         L.cuhre_last_error.restype = C.c_char_p
         L.cuhre_points_per_region.argtypes = [C.c_int, C.c_int]
         L.cuhre_points_per_region.restype = C.c_int64
@@ -230,6 +265,44 @@ class Context:
         A = na.value
         m = 2 * A
         return A, oL[: m * d].reshape(d, m).T.copy(), oW[: m * d].reshape(d, m).T.copy()
+
+    def _batch_inputs(self, problems):
+        count = len(problems)
+        d = problems[0].f.dim
+        arr = (Integrand * count)()
+        keep = []
+        for i, p in enumerate(problems):
+            st, k = to_struct(p.f)
+            arr[i] = st
+            keep.append(k)
+        lows = np.ascontiguousarray([p.lows for p in problems], dtype=np.float64).ravel()
+        highs = np.ascontiguousarray([p.highs for p in problems], dtype=np.float64).ravel()
+        return count, d, arr, lows, highs, keep
+
+    def integrate_batch(self, problems, point_segments=0, grid_blocks=0, iterations_per_check=0,
+                        use_graph=True):
+        """Integrates every problem (same dimension) in one batched call.
+        Returns (results list in input order, info dict, return code)."""
+        count, d, arr, lows, highs, keep = self._batch_inputs(problems)
+        er = np.ascontiguousarray([p.epsrel for p in problems], dtype=np.float64)
+        ea = np.ascontiguousarray([p.epsabs for p in problems], dtype=np.float64)
+        me = np.ascontiguousarray([p.maxeval for p in problems], dtype=np.int64)
+        cfg = BatchConfig(point_segments, grid_blocks, iterations_per_check, int(bool(use_graph)))
+        out = (Result * count)()
+        info = BatchInfo()
+        rc = lib().cuhre_integrate_batch(self.h, count, arr, d, _dp(er), _dp(ea),
+                                         me.ctypes.data_as(C.POINTER(C.c_int64)), _dp(lows), _dp(highs),
+                                         C.byref(cfg), out, C.byref(info))
+        _check(rc)
+        return [out[i].asdict() for i in range(count)], info.asdict(), rc
+
+    def batch_autotune(self, problems):
+        count, d, arr, lows, highs, keep = self._batch_inputs(problems)
+        best = BatchConfig()
+        table = np.zeros(15)
+        _check(lib().cuhre_batch_autotune(self.h, count, arr, d, _dp(lows), _dp(highs), C.byref(best),
+                                          _dp(table)))
+        return best.asdict(), table
 
     def sum(self, x):
         x = np.ascontiguousarray(x, dtype=np.float64)

@@ -1,0 +1,101 @@
+"""Batched form: every instance of a batch must reproduce the plain method
+run on that instance alone (same region counts and iterations, estimates
+within 1e-12 relative), whatever the launch shape or graph use."""
+import numpy as np
+import pytest
+
+import cuhre
+import oracle as O
+import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _same(b, g, tag):
+    assert b["fail"] == g["fail"], (tag, b, g)
+    assert b["nregions"] == g["nregions"] and b["iterations"] == g["iterations"], (tag, b, g)
+    assert b["neval"] == g["neval"]
+    assert abs(b["integral"] - g["integral"]) <= 1e-12 * abs(g["integral"]) + 1e-300, (tag, b, g)
+    assert abs(b["error"] - g["error"]) <= 1e-6 * g["error"] + 1e-300, (tag, b, g)
+
+
+def test_batch_matches_plain_and_oracle(ctx):
+    probs = W.c4_batch(count=48)
+    res, info, rc = ctx.integrate_batch(probs)
+    assert info["graph_used"] == 1
+    assert rc == (0 if all(r["fail"] == 0 for r in res) else 1)
+    for i, (p, b) in enumerate(zip(probs, res)):
+        g = ctx.integrate(p)
+        _same(b, g, ("plain", i))
+        o = O.integrate_bfs(p, nthreads=8)
+        assert b["nregions"] == o["nregions"] and b["iterations"] == o["iterations"], (i, b, o)
+        assert abs(b["integral"] - o["integral"]) <= 1e-12 * abs(o["integral"])
+        if b["fail"] == 0:
+            ex, _ = O.exact(p.f, p.lows, p.highs)
+            assert abs(b["integral"] - ex) <= max(b["error"], 1e-3 * p.epsrel * abs(ex)), (i, b, ex)
+
+
+@pytest.mark.parametrize("segments,grid,graph", [(1, 256, True), (2, 1024, True), (4, 4096, False), (1, 2048, False)])
+def test_batch_shapes_and_graph_agree(ctx, segments, grid, graph):
+    probs = W.c4_batch(seed=9, count=64)
+    ref, _, _ = ctx.integrate_batch(probs, point_segments=1, grid_blocks=2048, use_graph=True)
+    res, info, _ = ctx.integrate_batch(probs, point_segments=segments, grid_blocks=grid, use_graph=graph)
+    assert info["graph_used"] == int(graph)
+    for i, (a, b) in enumerate(zip(res, ref)):
+        assert a["nregions"] == b["nregions"] and a["iterations"] == b["iterations"]
+        assert a["integral"] == b["integral"] and a["error"] == b["error"], i
+
+
+def test_batch_mixed_families_tolerances_budget_and_boxes(ctx):
+    rng = W.SplitMix64(31)
+    probs = []
+    for i in range(24):
+        d = 3
+        a = [rng.uniform_range(0.5, 6) for _ in range(d)]
+        w = [rng.uniform() for _ in range(d)]
+        f = [W.oscillatory(a, rng.uniform()), W.continuous(a, w), W.gaussian(a, w),
+             W.product_peak(a, w), W.corner_peak(a, 4)][i % 5]
+        lo = [rng.uniform_range(-0.5, 0.2) for _ in range(d)]
+        if f.family == W.CORNER_PEAK:
+            lo = [abs(l) for l in lo]  # keep 1 + a.x away from zero
+        hi = [l + rng.uniform_range(0.3, 1.5) for l in lo]
+        eps = [1e-4, 1e-6, 1e-8][i % 3]
+        maxeval = [10**9, 20_000, 10**7][i % 3]
+        probs.append(W.Problem(f"m{i}", f, lo, hi, eps, 1e-14, maxeval))
+    res, info, rc = ctx.integrate_batch(probs)
+    for i, (p, b) in enumerate(zip(probs, res)):
+        g = ctx.integrate(p)
+        _same(b, g, ("mixed", i))
+    assert info["total_regions"] == sum(r["nregions"] for r in res)
+
+
+def test_batch_autotune_returns_a_valid_shape(ctx):
+    probs = W.c4_batch(count=256)
+    cfg, table = ctx.batch_autotune(probs)
+    assert cfg["point_segments"] in (1, 2, 4) and cfg["grid_blocks"] in (256, 512, 1024, 2048, 4096)
+    assert np.all(table > 0)
+    res, info, _ = ctx.integrate_batch(probs)  # uses the tuned shape
+    assert info["point_segments"] == cfg["point_segments"] and info["grid_blocks"] == cfg["grid_blocks"]
+    ref, _, _ = ctx.integrate_batch(probs, point_segments=1, grid_blocks=1024)
+    for a, b in zip(res, ref):
+        assert a["nregions"] == b["nregions"] and a["integral"] == b["integral"]
+
+
+def test_batch_capacity_cut_and_errors(ctx):
+    probs = W.c4_batch(seed=12, count=32)
+    with cuhre.Context(0, capacity=4096) as small:
+        res, info, rc = small.integrate_batch(probs)
+    assert rc == 1 and info["not_converged"] > 0
+    full, _, _ = ctx.integrate_batch(probs)
+    for a, b in zip(res, full):
+        if a["fail"] == 0:
+            assert a["integral"] == b["integral"] and a["nregions"] == b["nregions"]
+    bad = probs[:3] + [W.Problem("bad", W.corner_peak([-1.0] * 5, 700), [0] * 5, [1] * 5, 1e-5, 0, 10**8)]
+    with pytest.raises(cuhre.CuhreError) as e:
+        ctx.integrate_batch(bad)
+    assert e.value.code == cuhre.ERR_NONFINITE
+    with pytest.raises(cuhre.CuhreError) as e:
+        ctx.integrate_batch([W.c1_gaussian3(), W.c2_oscillatory8()])
+    assert e.value.code == cuhre.ERR_ARG
+    # the context is still usable
+    assert ctx.integrate(W.c1_gaussian3())["rc"] == 0
