@@ -43,7 +43,11 @@ def test_batch_shapes_and_graph_agree(ctx, segments, grid, graph):
     assert info["graph_used"] == int(graph)
     for i, (a, b) in enumerate(zip(res, ref)):
         assert a["nregions"] == b["nregions"] and a["iterations"] == b["iterations"]
-        assert a["integral"] == b["integral"] and a["error"] == b["error"], i
+        if segments == 1:  # same per-region summation order: bit-identical
+            assert a["integral"] == b["integral"] and a["error"] == b["error"], i
+        else:  # segments change the order of the per-region sums
+            assert abs(a["integral"] - b["integral"]) <= 1e-12 * abs(b["integral"]), i
+            assert abs(a["error"] - b["error"]) <= 1e-6 * b["error"], i
 
 
 def test_batch_mixed_families_tolerances_budget_and_boxes(ctx):
