@@ -275,8 +275,8 @@ class Context:
             st, k = to_struct(p.f)
             arr[i] = st
             keep.append(k)
-        lows = np.ascontiguousarray([p.lows for p in problems], dtype=np.float64).ravel()
-        highs = np.ascontiguousarray([p.highs for p in problems], dtype=np.float64).ravel()
+        lows = np.ascontiguousarray([x for p in problems for x in p.lows], dtype=np.float64)
+        highs = np.ascontiguousarray([x for p in problems for x in p.highs], dtype=np.float64)
         return count, d, arr, lows, highs, keep
 
     def integrate_batch(self, problems, point_segments=0, grid_blocks=0, iterations_per_check=0,
