@@ -150,6 +150,9 @@ typedef struct {
   int32_t grid_blocks;          /* 0: from cuhre_batch_autotune               */
   int32_t iterations_per_check; /* graph replays between host checks (0: 2)   */
   int32_t use_graph;            /* 1: hipGraph replay, 0: direct launches     */
+  int32_t pad;
+  int64_t initial_capacity;     /* first list allocation in regions (0: 4 Mi);
+                                   the list grows on demand                   */
 } cuhre_batch_config;
 
 typedef struct {
@@ -162,6 +165,10 @@ typedef struct {
   int32_t point_segments;
   int32_t grid_blocks;
   double ms_total;
+  int64_t peak_list;            /* longest region list of the run            */
+  int64_t capacity;             /* regions per list buffer at the end        */
+  int32_t grows;                /* list growths during the run               */
+  int32_t pad;
 } cuhre_batch_info;
 
 /* count instances; fs[count]; epsrel/epsabs/maxeval per instance (NULL:

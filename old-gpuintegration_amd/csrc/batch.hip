@@ -84,6 +84,10 @@ struct BatchBuffers {
   int64_t ld;
   const int64_t* n;  // current list length
   int64_t* n_next;
+  unsigned long long* n_peak;  // largest list length so far
+  int64_t* flags;  // [0] overflow: parity + 1 of the iteration that ran out of room, [1] its need
+  int32_t parity;
+  int32_t allow_cut;  // at the memory limit: stop instances that do not fit
   double* est;
   double* err;
   double* abssum;
@@ -189,6 +193,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_reduce(BatchBuffers B)
 {
   __shared__ Partial ldsw[kMaxWaves];
   const int i = blockIdx.x;
+  if (B.flags[0] != 0) return;
   InstState st = B.state[i];
   if (st.done) return;
   const BatchParams& bp = B.params[i];
@@ -237,13 +242,16 @@ __global__ __launch_bounds__(kBlock) void k_batch_reduce(BatchBuffers B)
 }
 
 // Slice offsets of the next list (exclusive scan of the children counts in
-// instance order).  If the list would exceed the capacity, the instances
-// from the first one that does not fit onward stop with fail = 1.
+// instance order).  If the next list does not fit, the iteration is halted
+// here (overflow flag, empty next list) so that the host can grow the
+// buffers and finish it; only at the memory limit (allow_cut) do the
+// instances from the first one that does not fit onward stop with fail = 1.
 __global__ __launch_bounds__(1024) void k_batch_scan(BatchBuffers B)
 {
   __shared__ int64_t sc[1024];
   __shared__ int s_cut;
   __shared__ unsigned long long s_tot;
+  if (B.flags[0] != 0) return;
   const int T = blockDim.x;
   const int chunk = (B.count + T - 1) / T;
   const int i0 = min((int)threadIdx.x * chunk, B.count);
@@ -261,6 +269,15 @@ __global__ __launch_bounds__(1024) void k_batch_scan(BatchBuffers B)
     __syncthreads();
     sc[threadIdx.x] += v;
     __syncthreads();
+  }
+  const int64_t total = sc[T - 1];
+  if (total > B.cap && !B.allow_cut) {
+    if (threadIdx.x == 0) {
+      B.flags[0] = B.parity + 1;
+      B.flags[1] = total;
+      *B.n_next = 0;
+    }
+    return;
   }
   const int64_t excl = sc[threadIdx.x] - mine;
   int64_t run = excl;
@@ -295,12 +312,16 @@ __global__ __launch_bounds__(1024) void k_batch_scan(BatchBuffers B)
   }
   atomicAdd(&s_tot, kept);
   __syncthreads();
-  if (threadIdx.x == 0) *B.n_next = (int64_t)s_tot;
+  if (threadIdx.x == 0) {
+    *B.n_next = (int64_t)s_tot;
+    if (s_tot > *B.n_peak) *B.n_peak = s_tot;
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void k_tile_count(BatchBuffers B)
 {
   __shared__ uint32_t wcnt[kBlock / kWave];
+  if (B.flags[0] != 0) return;
   const int64_t n = *B.n;
   const int64_t ntiles = (n + kBlock - 1) / kBlock;
   for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -321,6 +342,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_count(BatchBuffers B)
 __global__ __launch_bounds__(1024) void k_tile_scan(BatchBuffers B)
 {
   __shared__ int64_t sc[1024];
+  if (B.flags[0] != 0) return;
   const int64_t n = *B.n;
   const int64_t ntiles = (n + kBlock - 1) / kBlock;
   const int T = blockDim.x;
@@ -347,6 +369,7 @@ __global__ __launch_bounds__(1024) void k_tile_scan(BatchBuffers B)
 __global__ __launch_bounds__(kBlock) void k_batch_split(BatchBuffers B)
 {
   __shared__ uint32_t wcnt[kBlock / kWave];
+  if (B.flags[0] != 0) return;
   const int64_t n = *B.n;
   const int64_t ntiles = (n + kBlock - 1) / kBlock;
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
@@ -451,7 +474,7 @@ BatchEvalLauncher batch_eval_launcher(int dim)
 
 // Host side of the batch engine (state lives in the context).
 struct BatchEngine {
-  int64_t cap = 0;
+  int64_t cap = 0;  // regions per list buffer
   int dim_alloc = 0;
   int count_alloc = 0;
   double* lo[2] = {nullptr, nullptr};
@@ -460,48 +483,122 @@ struct BatchEngine {
   double *est = nullptr, *err = nullptr, *abssum = nullptr;
   int8_t* axis = nullptr;
   int64_t *tile_cnt = nullptr, *tile_off = nullptr;
-  int64_t* n = nullptr;  // [2]
+  int64_t* n = nullptr;  // [0,1] list lengths, [2] peak, [3] overflow, [4] needed length
   BatchParams* params = nullptr;
   InstState* state = nullptr;
   std::vector<InstState> h_state;
   std::vector<BatchParams> h_params;
   std::vector<int> perm;  // list position k holds caller instance perm[k]
   unsigned fam_mask = 0;
-  int64_t h_n[2] = {0, 0};
-  // cached graph of two iterations for one (count, dim, cap, S, grid) shape
+  int allow_cut = 0;
+  int64_t h_n[5] = {0, 0, 0, 0, 0};
+  // cached graph of two iterations for one buffer set and launch shape
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
-  int g_count = -1, g_dim = -1, g_S = -1, g_grid = -1;
+  int g_count = -1, g_dim = -1, g_S = -1, g_grid = -1, g_cut = -1;
   unsigned g_mask = 0;
   int64_t g_cap = -1;
   // autotune cache
   int tuned_dim = -1, tuned_S = 0, tuned_grid = 0;
-  uint64_t tuned_key = 0;
 };
 
-static void free_batch(BatchEngine* e)
+static void drop_graph(BatchEngine* e)
 {
-  for (int p = 0; p < 2; ++p) {
-    if (e->lo[p]) (void)hipFree(e->lo[p]);
-    if (e->wd[p]) (void)hipFree(e->wd[p]);
-    if (e->inst[p]) (void)hipFree(e->inst[p]);
-    e->lo[p] = e->wd[p] = nullptr;
-    e->inst[p] = nullptr;
-  }
-  for (void* q : {(void*)e->est, (void*)e->err, (void*)e->abssum, (void*)e->axis, (void*)e->tile_cnt,
-                  (void*)e->tile_off, (void*)e->n, (void*)e->params, (void*)e->state})
-    if (q) (void)hipFree(q);
-  e->est = e->err = e->abssum = nullptr;
-  e->axis = nullptr;
-  e->tile_cnt = e->tile_off = e->n = nullptr;
-  e->params = nullptr;
-  e->state = nullptr;
   if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
   if (e->graph) (void)hipGraphDestroy(e->graph);
   e->gexec = nullptr;
   e->graph = nullptr;
   e->g_count = -1;
+}
+
+struct ListArrays {
+  double* lo[2] = {nullptr, nullptr};
+  double* wd[2] = {nullptr, nullptr};
+  int32_t* inst[2] = {nullptr, nullptr};
+  double *est = nullptr, *err = nullptr, *abssum = nullptr;
+  int8_t* axis = nullptr;
+  int64_t *tile_cnt = nullptr, *tile_off = nullptr;
+};
+
+static void free_arrays(ListArrays& a)
+{
+  for (int p = 0; p < 2; ++p) {
+    if (a.lo[p]) (void)hipFree(a.lo[p]);
+    if (a.wd[p]) (void)hipFree(a.wd[p]);
+    if (a.inst[p]) (void)hipFree(a.inst[p]);
+  }
+  for (void* q : {(void*)a.est, (void*)a.err, (void*)a.abssum, (void*)a.axis, (void*)a.tile_cnt,
+                  (void*)a.tile_off})
+    if (q) (void)hipFree(q);
+  a = ListArrays();
+}
+
+static bool alloc_arrays(ListArrays& a, int64_t cap, int dim)
+{
+  const size_t lb = sizeof(double) * (size_t)cap * dim;
+  const int64_t ntiles = cap / kBlock + 1;
+  bool ok = true;
+  for (int p = 0; p < 2; ++p)
+    ok = ok && hipMalloc(&a.lo[p], lb) == hipSuccess && hipMalloc(&a.wd[p], lb) == hipSuccess &&
+         hipMalloc(&a.inst[p], sizeof(int32_t) * cap) == hipSuccess;
+  ok = ok && hipMalloc(&a.est, sizeof(double) * cap) == hipSuccess &&
+       hipMalloc(&a.err, sizeof(double) * cap) == hipSuccess &&
+       hipMalloc(&a.abssum, sizeof(double) * cap) == hipSuccess && hipMalloc(&a.axis, cap) == hipSuccess &&
+       hipMalloc(&a.tile_cnt, sizeof(int64_t) * ntiles) == hipSuccess &&
+       hipMalloc(&a.tile_off, sizeof(int64_t) * ntiles) == hipSuccess;
+  if (!ok) {
+    (void)hipGetLastError();
+    free_arrays(a);
+  }
+  return ok;
+}
+
+static ListArrays take_arrays(BatchEngine* e)
+{
+  ListArrays a;
+  for (int p = 0; p < 2; ++p) {
+    a.lo[p] = e->lo[p];
+    a.wd[p] = e->wd[p];
+    a.inst[p] = e->inst[p];
+  }
+  a.est = e->est;
+  a.err = e->err;
+  a.abssum = e->abssum;
+  a.axis = e->axis;
+  a.tile_cnt = e->tile_cnt;
+  a.tile_off = e->tile_off;
+  return a;
+}
+
+static void put_arrays(BatchEngine* e, const ListArrays& a)
+{
+  for (int p = 0; p < 2; ++p) {
+    e->lo[p] = a.lo[p];
+    e->wd[p] = a.wd[p];
+    e->inst[p] = a.inst[p];
+  }
+  e->est = a.est;
+  e->err = a.err;
+  e->abssum = a.abssum;
+  e->axis = a.axis;
+  e->tile_cnt = a.tile_cnt;
+  e->tile_off = a.tile_off;
+}
+
+static void free_batch(BatchEngine* e)
+{
+  drop_graph(e);
+  ListArrays a = take_arrays(e);
+  free_arrays(a);
+  put_arrays(e, a);
+  for (void* q : {(void*)e->n, (void*)e->params, (void*)e->state})
+    if (q) (void)hipFree(q);
+  e->n = nullptr;
+  e->params = nullptr;
+  e->state = nullptr;
   e->cap = 0;
+  e->dim_alloc = 0;
+  e->count_alloc = 0;
 }
 
 BatchEngine* batch_engine_new() { return new BatchEngine(); }
@@ -512,29 +609,36 @@ void batch_engine_delete(BatchEngine* e)
   delete e;
 }
 
+static size_t bytes_per_region(int dim)
+{
+  return (size_t)dim * 4 * sizeof(double) + 2 * sizeof(int32_t) + 3 * sizeof(double) + 1 +
+         2 * sizeof(int64_t) / kBlock + 1;
+}
+
+// Regions per list buffer that three quarters of the device memory holds
+// (counting what this engine already owns as free).
+static int64_t memory_cap(BatchEngine* e, int dim)
+{
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) return 1 << 20;
+  const double held = (double)e->cap * (double)bytes_per_region(std::max(dim, e->dim_alloc));
+  return (int64_t)(((double)fr + held) * 0.75 / (double)bytes_per_region(dim));
+}
+
+// Fresh buffers (contents not kept).
 static int batch_reserve(BatchEngine* e, int64_t cap, int dim, int count)
 {
   if (cap <= e->cap && dim <= e->dim_alloc && count <= e->count_alloc) return 0;
-  cap = std::max(cap, e->cap);
+  cap = (std::max(cap, e->cap) + 255) & ~255ll;
   dim = std::max(dim, e->dim_alloc);
   count = std::max(count, e->count_alloc);
   free_batch(e);
-  cap = (cap + 255) & ~255ll;
-  const size_t lb = sizeof(double) * (size_t)cap * dim;
-  const int64_t ntiles = cap / kBlock + 1;
-  bool ok = true;
-  for (int p = 0; p < 2; ++p) {
-    ok = ok && hipMalloc(&e->lo[p], lb) == hipSuccess && hipMalloc(&e->wd[p], lb) == hipSuccess &&
-         hipMalloc(&e->inst[p], sizeof(int32_t) * cap) == hipSuccess;
-  }
-  ok = ok && hipMalloc(&e->est, sizeof(double) * cap) == hipSuccess &&
-       hipMalloc(&e->err, sizeof(double) * cap) == hipSuccess &&
-       hipMalloc(&e->abssum, sizeof(double) * cap) == hipSuccess && hipMalloc(&e->axis, cap) == hipSuccess &&
-       hipMalloc(&e->tile_cnt, sizeof(int64_t) * ntiles) == hipSuccess &&
-       hipMalloc(&e->tile_off, sizeof(int64_t) * ntiles) == hipSuccess &&
-       hipMalloc(&e->n, sizeof(int64_t) * 2) == hipSuccess &&
+  ListArrays a;
+  bool ok = alloc_arrays(a, cap, dim);
+  ok = ok && hipMalloc(&e->n, sizeof(int64_t) * 5) == hipSuccess &&
        hipMalloc(&e->params, sizeof(BatchParams) * count) == hipSuccess &&
        hipMalloc(&e->state, sizeof(InstState) * count) == hipSuccess;
+  put_arrays(e, a);
   if (!ok) {
     (void)hipGetLastError();
     free_batch(e);
@@ -543,6 +647,40 @@ static int batch_reserve(BatchEngine* e, int64_t cap, int dim, int count)
   e->cap = cap;
   e->dim_alloc = dim;
   e->count_alloc = count;
+  return 0;
+}
+
+// Larger list buffers keeping list `parity` (n regions) and its per-region
+// results; the instance states and parameters stay where they are.
+static int batch_grow(BatchEngine* e, int64_t cap, int parity, int64_t n, hipStream_t st)
+{
+  cap = (cap + 255) & ~255ll;
+  ListArrays a;
+  if (!alloc_arrays(a, cap, e->dim_alloc)) return set_error(CUHRE_ERR_ALLOC, "cannot grow the batch list");
+  bool ok = true;
+  if (n > 0) {
+    const size_t rb = sizeof(double) * (size_t)n;
+    ok = hipMemcpy2DAsync(a.lo[parity], sizeof(double) * cap, e->lo[parity], sizeof(double) * e->cap, rb,
+                          e->dim_alloc, hipMemcpyDeviceToDevice, st) == hipSuccess &&
+         hipMemcpy2DAsync(a.wd[parity], sizeof(double) * cap, e->wd[parity], sizeof(double) * e->cap, rb,
+                          e->dim_alloc, hipMemcpyDeviceToDevice, st) == hipSuccess &&
+         hipMemcpyAsync(a.inst[parity], e->inst[parity], sizeof(int32_t) * n, hipMemcpyDeviceToDevice, st) ==
+             hipSuccess &&
+         hipMemcpyAsync(a.est, e->est, rb, hipMemcpyDeviceToDevice, st) == hipSuccess &&
+         hipMemcpyAsync(a.err, e->err, rb, hipMemcpyDeviceToDevice, st) == hipSuccess &&
+         hipMemcpyAsync(a.abssum, e->abssum, rb, hipMemcpyDeviceToDevice, st) == hipSuccess &&
+         hipMemcpyAsync(a.axis, e->axis, n, hipMemcpyDeviceToDevice, st) == hipSuccess;
+  }
+  ok = ok && hipStreamSynchronize(st) == hipSuccess;
+  if (!ok) {
+    free_arrays(a);
+    return set_error(CUHRE_ERR_DEVICE, "batch list copy failed");
+  }
+  drop_graph(e);
+  ListArrays old = take_arrays(e);
+  free_arrays(old);
+  put_arrays(e, a);
+  e->cap = cap;
   return 0;
 }
 
@@ -558,6 +696,10 @@ static BatchBuffers make_buffers(BatchEngine* e, int parity, int count, int dim)
   B.ld = e->cap;
   B.n = e->n + parity;
   B.n_next = e->n + (1 - parity);
+  B.n_peak = reinterpret_cast<unsigned long long*>(e->n + 2);
+  B.flags = e->n + 3;
+  B.parity = parity;
+  B.allow_cut = e->allow_cut;
   B.est = e->est;
   B.err = e->err;
   B.abssum = e->abssum;
@@ -573,16 +715,28 @@ static BatchBuffers make_buffers(BatchEngine* e, int parity, int count, int dim)
   return B;
 }
 
-static void enqueue_iteration(BatchEngine* e, int parity, int count, int dim, const RuleConst& R,
-                              int S, int grid, hipStream_t st)
+static void enqueue_eval_reduce(BatchEngine* e, int parity, int count, int dim, const RuleConst& R, int S,
+                                int grid, hipStream_t st)
 {
   const BatchBuffers B = make_buffers(e, parity, count, dim);
   batch_eval_launcher(dim)(B, R, S, grid, e->fam_mask, st);
   hipLaunchKernelGGL(k_batch_reduce, dim3(count), dim3(kBlock), 0, st, B);
+}
+
+static void enqueue_scan_split(BatchEngine* e, int parity, int count, int dim, int grid, hipStream_t st)
+{
+  const BatchBuffers B = make_buffers(e, parity, count, dim);
   hipLaunchKernelGGL(k_batch_scan, dim3(1), dim3(1024), 0, st, B);
   hipLaunchKernelGGL(k_tile_count, dim3(grid), dim3(kBlock), 0, st, B);
   hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, st, B);
   hipLaunchKernelGGL(k_batch_split, dim3(grid), dim3(kBlock), 0, st, B);
+}
+
+static void enqueue_iteration(BatchEngine* e, int parity, int count, int dim, const RuleConst& R, int S,
+                              int grid, hipStream_t st)
+{
+  enqueue_eval_reduce(e, parity, count, dim, R, S, grid, st);
+  enqueue_scan_split(e, parity, count, dim, grid, st);
 }
 
 static int check_batch_args(int count, const cuhre_integrand* fs, int dim)
@@ -608,8 +762,7 @@ static void fill_params(std::vector<BatchParams>& hp, std::vector<int>& perm, un
 {
   perm.resize(count);
   for (int i = 0; i < count; ++i) perm[i] = i;
-  std::stable_sort(perm.begin(), perm.end(),
-                   [&](int x, int y) { return fs[x].family < fs[y].family; });
+  std::stable_sort(perm.begin(), perm.end(), [&](int x, int y) { return fs[x].family < fs[y].family; });
   mask = 0;
   hp.assign(count, BatchParams{});
   for (int k = 0; k < count; ++k) {
@@ -643,11 +796,11 @@ static int batch_setup(BatchEngine* e, hipStream_t st, int count, const cuhre_in
                        const double* epsrel, const double* epsabs, const int64_t* maxeval,
                        const double* lows, const double* highs, int64_t cap, int64_t* n0_out)
 {
-  int k = grid_k_batch(dim);
+  const int k = grid_k_batch(dim);
   int64_t n0 = 1;
   for (int m = 0; m < dim; ++m) n0 *= k;
   const int64_t total = n0 * count;
-  if (2 * total > cap) cap = 2 * total;
+  cap = std::max(cap, 2 * total);
   int rc = batch_reserve(e, cap, dim, count);
   if (rc) return rc;
   std::vector<BatchParams>& hp = e->h_params;
@@ -660,11 +813,40 @@ static int batch_setup(BatchEngine* e, hipStream_t st, int count, const cuhre_in
   hipLaunchKernelGGL(k_batch_init, dim3((unsigned)((nthreads + 255) / 256)), dim3(256), 0, st, B, k, n0);
   e->h_n[0] = total;
   e->h_n[1] = 0;
+  e->h_n[2] = total;
+  e->h_n[3] = 0;
+  e->h_n[4] = 0;
   if (hipMemcpyAsync(e->n, e->h_n, sizeof e->h_n, hipMemcpyHostToDevice, st) != hipSuccess)
     return set_error(CUHRE_ERR_DEVICE, "batch setup copy failed");
   if (hipGetLastError() != hipSuccess) return set_error(CUHRE_ERR_DEVICE, "batch init launch failed");
   *n0_out = total;
   return 0;
+}
+
+static bool capture_graph(BatchEngine* e, int count, int dim, const RuleConst& R, int S, int G, hipStream_t st)
+{
+  const bool same = e->gexec && e->g_count == count && e->g_dim == dim && e->g_S == S && e->g_grid == G &&
+                    e->g_cap == e->cap && e->g_mask == e->fam_mask && e->g_cut == e->allow_cut;
+  if (same) return true;
+  drop_graph(e);
+  bool ok = false;
+  if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+    enqueue_iteration(e, 0, count, dim, R, S, G, st);
+    enqueue_iteration(e, 1, count, dim, R, S, G, st);
+    if (hipStreamEndCapture(st, &e->graph) == hipSuccess &&
+        hipGraphInstantiate(&e->gexec, e->graph, nullptr, nullptr, 0) == hipSuccess) {
+      ok = true;
+      e->g_count = count;
+      e->g_dim = dim;
+      e->g_S = S;
+      e->g_grid = G;
+      e->g_cap = e->cap;
+      e->g_mask = e->fam_mask;
+      e->g_cut = e->allow_cut;
+    }
+  }
+  (void)hipGetLastError();
+  return ok;
 }
 
 }  // namespace cuhre
@@ -689,11 +871,9 @@ int cuhre_batch_autotune(cuhre_context* ctx, int32_t count, const cuhre_integran
   BatchEngine* e = cuhre_ctx_batch(ctx, &st, &device, &capacity);
   if (hipSetDevice(device) != hipSuccess) return set_error(CUHRE_ERR_DEVICE, "hipSetDevice failed");
   // sample list: the batch's initial grids refined uniformly to >= 512k regions
-  int64_t n0 = 0;
   const int64_t want = 1 << 19;
-  int64_t cap = want * 4;
-  rc = batch_setup(e, st, count, fs, dim, nullptr, nullptr, nullptr, lows, highs,
-                   std::min<int64_t>(capacity, cap), &n0);
+  int64_t n0 = 0;
+  rc = batch_setup(e, st, count, fs, dim, nullptr, nullptr, nullptr, lows, highs, 4 * want, &n0);
   if (rc) return rc;
   int64_t n = n0;
   int parity = 0, round = 0;
@@ -742,6 +922,8 @@ int cuhre_batch_autotune(cuhre_context* ctx, int32_t count, const cuhre_integran
   best->grid_blocks = bg;
   best->iterations_per_check = 2;
   best->use_graph = 1;
+  best->pad = 0;
+  best->initial_capacity = 0;
   e->tuned_dim = dim;
   e->tuned_S = bs;
   e->tuned_grid = bg;
@@ -764,57 +946,29 @@ int cuhre_integrate_batch(cuhre_context* ctx, int32_t count, const cuhre_integra
   if (hipSetDevice(device) != hipSuccess) return set_error(CUHRE_ERR_DEVICE, "hipSetDevice failed");
   int S = cfg && cfg->point_segments > 0 ? cfg->point_segments : 0;
   int G = cfg && cfg->grid_blocks > 0 ? cfg->grid_blocks : 0;
-  if (S == 0 || G == 0) {
-    if (e->tuned_dim == dim) {
-      S = S ? S : e->tuned_S;
-      G = G ? G : e->tuned_grid;
-    } else {
-      S = S ? S : 1;
-      G = G ? G : 2048;
-    }
+  if (e->tuned_dim == dim) {
+    S = S ? S : e->tuned_S;
+    G = G ? G : e->tuned_grid;
   }
+  S = S ? S : 1;
+  G = G ? G : 2048;
   if (S != 1 && S != 2 && S != 4) return set_error(CUHRE_ERR_ARG, "batch point_segments must be 1, 2 or 4");
   const int per_check = cfg && cfg->iterations_per_check > 0 ? cfg->iterations_per_check : 2;
   const bool use_graph = !cfg || cfg->use_graph != 0;
-  const int64_t cap = std::min<int64_t>(capacity, 32ll << 20);
+  // list capacity: an explicit engine capacity is a hard limit; otherwise the
+  // list starts small and grows, up to three quarters of the device memory
+  const int64_t limit = capacity > 0 ? capacity : memory_cap(e, dim);
   int64_t n0 = 0;
-  rc = batch_setup(e, st, count, fs, dim, epsrel, epsabs, maxeval, lows, highs, cap, &n0);
+  const int64_t first = cfg && cfg->initial_capacity > 0 ? cfg->initial_capacity : (1 << 22);
+  if (cfg && cfg->initial_capacity > 0 && e->cap > first) free_batch(e);  // honour a smaller start
+  rc = batch_setup(e, st, count, fs, dim, epsrel, epsabs, maxeval, lows, highs,
+                   std::min<int64_t>(limit, std::max<int64_t>(e->cap, first)), &n0);
   if (rc) return rc;
+  e->allow_cut = e->cap >= limit ? 1 : 0;
   const RuleConst R = make_rule(dim);
-
-  bool graph_ok = false;
-  if (use_graph) {
-    const bool same = e->gexec && e->g_count == count && e->g_dim == dim && e->g_S == S &&
-                      e->g_grid == G && e->g_cap == e->cap && e->g_mask == e->fam_mask;
-    if (same) {
-      graph_ok = true;
-    } else {
-      if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
-      if (e->graph) (void)hipGraphDestroy(e->graph);
-      e->gexec = nullptr;
-      e->graph = nullptr;
-      if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess) {
-        enqueue_iteration(e, 0, count, dim, R, S, G, st);
-        enqueue_iteration(e, 1, count, dim, R, S, G, st);
-        if (hipStreamEndCapture(st, &e->graph) == hipSuccess &&
-            hipGraphInstantiate(&e->gexec, e->graph, nullptr, nullptr, 0) == hipSuccess) {
-          graph_ok = true;
-          e->g_count = count;
-          e->g_dim = dim;
-          e->g_S = S;
-          e->g_grid = G;
-          e->g_cap = e->cap;
-          e->g_mask = e->fam_mask;
-        }
-      }
-      (void)hipGetLastError();
-    }
-  }
-  int64_t h_n[2] = {n0, 0};
-  (void)n0;
-  int iters = 0;
-  int64_t total_regions = 0, peak = 0;
-  for (int check = 0; check < 1000; ++check) {
+  bool graph_ok = use_graph && capture_graph(e, count, dim, R, S, G, st);
+  int iters = 0, grows = 0;
+  for (int check = 0; check < 2000; ++check) {
     for (int k = 0; k < per_check; ++k) {
       if (graph_ok) {
         if (hipGraphLaunch(e->gexec, st) != hipSuccess) return set_error(CUHRE_ERR_DEVICE, "graph launch failed");
@@ -825,10 +979,27 @@ int cuhre_integrate_batch(cuhre_context* ctx, int32_t count, const cuhre_integra
       iters += 2;
     }
     if (hipGetLastError() != hipSuccess) return set_error(CUHRE_ERR_DEVICE, "batch iteration launch failed");
-    if (hipMemcpyAsync(h_n, e->n, sizeof h_n, hipMemcpyDeviceToHost, st) != hipSuccess ||
+    if (hipMemcpyAsync(e->h_n, e->n, sizeof e->h_n, hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)
       return set_error(CUHRE_ERR_DEVICE, "batch sync failed");
-    if (h_n[0] == 0) break;  // an even number of iterations leaves the list in buffer 0
+    if (e->h_n[3] != 0) {
+      // an iteration ran out of room after its reduction: grow, then finish it
+      const int p = (int)e->h_n[3] - 1;
+      const int64_t need = e->h_n[4];
+      const int64_t ncap = std::min<int64_t>(limit, std::max<int64_t>(2 * e->cap, need + need / 4));
+      if ((rc = batch_grow(e, ncap, p, e->h_n[p], st))) return rc;
+      e->allow_cut = e->cap >= limit || ncap < need ? 1 : 0;
+      e->h_n[3] = 0;
+      e->h_n[4] = 0;
+      if (hipMemcpyAsync(e->n + 3, e->h_n + 3, 2 * sizeof(int64_t), hipMemcpyHostToDevice, st) != hipSuccess)
+        return set_error(CUHRE_ERR_DEVICE, "batch flag reset failed");
+      enqueue_scan_split(e, p, count, dim, G, st);
+      if (p == 0) enqueue_iteration(e, 1, count, dim, R, S, G, st);  // back in step with the graph
+      ++grows;
+      graph_ok = use_graph && capture_graph(e, count, dim, R, S, G, st);
+      continue;
+    }
+    if (e->h_n[0] == 0) break;  // an even number of iterations leaves the list in buffer 0
   }
   e->h_state.resize(count);
   if (hipMemcpyAsync(e->h_state.data(), e->state, sizeof(InstState) * count, hipMemcpyDeviceToHost, st) !=
@@ -836,6 +1007,7 @@ int cuhre_integrate_batch(cuhre_context* ctx, int32_t count, const cuhre_integra
       hipStreamSynchronize(st) != hipSuccess)
     return set_error(CUHRE_ERR_DEVICE, "batch result copy failed");
   int worst = 0, nonconv = 0, maxit = 0;
+  int64_t total_regions = 0;
   const int64_t p = gm_points(dim);
   for (int k = 0; k < count; ++k) {
     const InstState& s = e->h_state[k];
@@ -846,12 +1018,11 @@ int cuhre_integrate_batch(cuhre_context* ctx, int32_t count, const cuhre_integra
     r.fp64_err_bound = s.bound;
     r.nregions = s.nregions;
     r.neval = s.nregions * p;
-    r.fail = s.fail > 0 ? s.fail : 0;
+    r.fail = s.fail > 0 ? s.fail : (s.done ? 0 : 1);
     r.iterations = s.iters;
     r.peak_regions = s.peak;
     r.nfinished = s.nfin;
     total_regions += s.nregions;
-    peak = std::max(peak, s.peak);
     maxit = std::max(maxit, s.iters);
     if (s.fail < 0) worst = s.fail;
     if (s.fail > 0 || !s.done) ++nonconv;
@@ -868,6 +1039,9 @@ int cuhre_integrate_batch(cuhre_context* ctx, int32_t count, const cuhre_integra
     info->graph_used = graph_ok ? 1 : 0;
     info->point_segments = S;
     info->grid_blocks = G;
+    info->peak_list = e->h_n[2];
+    info->capacity = e->cap;
+    info->grows = grows;
   }
   for (int i = 0; i < count; ++i) out[i].ms_total = ms;
   if (worst < 0) return set_error(worst, "an integrand returned a non-finite value in some instance");

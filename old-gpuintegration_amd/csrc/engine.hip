@@ -285,6 +285,7 @@ struct cuhre_context {
   int device = 0;
   hipStream_t stream = nullptr;
   int64_t capacity = 64ll << 20;  // regions
+  int64_t batch_capacity = 0;     // 0: sized from free HBM per batch call
   int profile = 0;
   int segments = 0;                // 0: automatic, else 1, 2, 4, 8 or 16
   int64_t fill_lanes = 262144;     // auto: smallest S with n*S >= fill_lanes
@@ -320,7 +321,7 @@ extern "C" __attribute__((visibility("hidden"))) BatchEngine* cuhre_ctx_batch(cu
   if (!c->batch) c->batch = batch_engine_new();
   *stream = c->stream;
   *device = c->device;
-  *capacity = c->capacity;
+  *capacity = c->batch_capacity;
   return c->batch;
 }
 
@@ -586,7 +587,10 @@ int cuhre_context_create(cuhre_context** out, int device, int64_t capacity_regio
   if (device < 0 || device >= ndev) return fail(CUHRE_ERR_ARG, "device %d out of range (%d)", device, ndev);
   cuhre_context* c = new cuhre_context();
   c->device = device;
-  if (capacity_regions > 0) c->capacity = capacity_regions;
+  if (capacity_regions > 0) {
+    c->capacity = capacity_regions;
+    c->batch_capacity = capacity_regions;
+  }
   int rc = 0;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc(&c->ticket, sizeof(uint32_t)) != hipSuccess ||

@@ -86,6 +86,8 @@ class BatchConfig(C.Structure):
         ("grid_blocks", C.c_int32),
         ("iterations_per_check", C.c_int32),
         ("use_graph", C.c_int32),
+        ("pad", C.c_int32),
+        ("initial_capacity", C.c_int64),
     ]
 
     def asdict(self):
@@ -103,6 +105,10 @@ class BatchInfo(C.Structure):
         ("point_segments", C.c_int32),
         ("grid_blocks", C.c_int32),
         ("ms_total", C.c_double),
+        ("peak_list", C.c_int64),
+        ("capacity", C.c_int64),
+        ("grows", C.c_int32),
+        ("pad", C.c_int32),
     ]
 
     def asdict(self):
@@ -280,14 +286,15 @@ class Context:
         return count, d, arr, lows, highs, keep
 
     def integrate_batch(self, problems, point_segments=0, grid_blocks=0, iterations_per_check=0,
-                        use_graph=True):
+                        use_graph=True, initial_capacity=0):
         """Integrates every problem (same dimension) in one batched call.
         Returns (results list in input order, info dict, return code)."""
         count, d, arr, lows, highs, keep = self._batch_inputs(problems)
         er = np.ascontiguousarray([p.epsrel for p in problems], dtype=np.float64)
         ea = np.ascontiguousarray([p.epsabs for p in problems], dtype=np.float64)
         me = np.ascontiguousarray([p.maxeval for p in problems], dtype=np.int64)
-        cfg = BatchConfig(point_segments, grid_blocks, iterations_per_check, int(bool(use_graph)))
+        cfg = BatchConfig(point_segments, grid_blocks, iterations_per_check, int(bool(use_graph)), 0,
+                          initial_capacity)
         out = (Result * count)()
         info = BatchInfo()
         rc = lib().cuhre_integrate_batch(self.h, count, arr, d, _dp(er), _dp(ea),

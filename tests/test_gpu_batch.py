@@ -105,3 +105,16 @@ def test_batch_capacity_cut_and_errors(ctx):
     assert e.value.code == cuhre.ERR_ARG
     # the context is still usable
     assert ctx.integrate(W.c1_gaussian3())["rc"] == 0
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_batch_list_growth_keeps_results(ctx, graph):
+    # start from a 4096-region list so the run has to grow it several times
+    probs = W.c4_batch(seed=21, count=40)
+    ref, rinfo, _ = ctx.integrate_batch(probs, point_segments=1, grid_blocks=1024)
+    res, info, _ = ctx.integrate_batch(probs, point_segments=1, grid_blocks=1024, use_graph=graph,
+                                       initial_capacity=4096)
+    assert info["grows"] >= 2 and info["not_converged"] == rinfo["not_converged"]
+    for a, b in zip(res, ref):
+        assert a["nregions"] == b["nregions"] and a["iterations"] == b["iterations"]
+        assert a["integral"] == b["integral"] and a["error"] == b["error"]
