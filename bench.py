@@ -26,6 +26,9 @@ sys.path.insert(0, os.path.join(ROOT, "old-gpuintegration_amd"))
 import workloads as W  # noqa: E402
 
 FP64_PEAK_TFLOPS = 78.6
+# measured HBM traffic per launch of k_eval_seg<8, product_peak> on this workload
+# (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, calibrated on a copy kernel)
+TRAFFIC_PER_LAUNCH = 2 * 10150.76 * 1024 + 4149.64 * 1024
 
 
 def flop_costs():
@@ -53,6 +56,21 @@ def family_flops(fam, d, c):
 def region_flops(fam, d, c):
     p = W.gm_npoints(d)
     return p * (family_flops(fam, d, c) + 2 * d + 4) + (8 * d + 12)
+
+
+def check_against_exact(problems, per, conv):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    out = {}
+    for p, item in zip(problems, per):
+        ex, _ = oracle.exact(p.f, p.lows, p.highs)
+        out[p.name] = {"exact": ex, "rel_err": abs(item["integral"] - ex) / abs(ex),
+                       "reported_rel_error": item["error"] / abs(item["integral"])}
+    if conv:
+        ex, _ = oracle.exact(problems[1].f, problems[1].lows, problems[1].highs)
+        out["product_peak8d_to_convergence"] = {"exact": ex, "rel_err": abs(conv["integral"] - ex) / abs(ex),
+                                                "reported_rel_error": conv["error"] / abs(conv["integral"])}
+    return out
 
 
 def cpu_baseline(problems):
@@ -106,7 +124,7 @@ def main():
     ap.add_argument("--batched", action="store_true",
                     help="also time the 4096-integral batched form against sequential plain calls")
     ap.add_argument("--converge", action="store_true",
-                    help="also run the product peak with maxeval 1e11 to reach epsrel 1e-6")
+                    help="also run the product peak with maxeval 1e12 to reach epsrel 1e-6")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the path runs on a single MI355X "
@@ -146,7 +164,6 @@ def main():
     big_tf = fl * big["largest_regions"] / (big["ms_eval_largest"] * 1e-3) / 1e12
 
     last = runs[-1]
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
     per = []
     for p, r, pr in zip(problems, last, prof[-1]):
         item = {"integrand": p.name, "integral": r["integral"], "error": r["error"],
@@ -154,13 +171,6 @@ def main():
                 "neval": r["neval"], "fail": r["fail"], "iterations": r["iterations"],
                 "peak_regions": r["peak_regions"], "ms": r["ms_total"],
                 "ms_eval": pr["ms_eval"], "ms_split": pr["ms_split"]}
-        try:
-            import oracle
-            ex, _ = oracle.exact(p.f, p.lows, p.highs)
-            item["exact"] = ex
-            item["abs_err_vs_exact"] = abs(r["integral"] - ex)
-        except Exception:
-            pass
         per.append(item)
 
     out = {
@@ -180,7 +190,10 @@ def main():
                                "epsrel 1e-6, maxeval 1e9 each",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "fp64", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": TRAFFIC_PER_LAUNCH,
+                     "traffic_note": "HBM bytes per launch of this kernel on this workload, FETCH_SIZE x2 + "
+                                     "WRITE_SIZE x1 (copy-kernel calibration), profiles/eval_seg_product_peak8d.md; "
+                                     "algorithmic 153 B/region x 159,134 regions = 24.3 MB",
                      "kernel": "k_eval_seg<8, product_peak>",
                      "launches": ev_launches, "avg_launch_ms": avg_ms,
                      "flops_per_region": fl, "regions_per_launch": ev_regions / ev_launches,
@@ -191,9 +204,9 @@ def main():
     if args.converge:
         p = W.c2_product_peak8()
         t1 = time.perf_counter()
-        r = ctx.integrate(p, maxeval=10**11)
+        r = ctx.integrate(p, maxeval=10**12)
         t2 = time.perf_counter()
-        out["product_peak_to_convergence"] = {"maxeval": 10**11, "seconds": t2 - t1,
+        out["product_peak_to_convergence"] = {"maxeval": 10**12, "seconds": t2 - t1,
                                               "evals_per_s": r["neval"] / (t2 - t1), **{k: r[k] for k in (
                                                   "integral", "error", "nregions", "neval", "fail",
                                                   "iterations", "peak_regions")}}
@@ -201,6 +214,10 @@ def main():
         out["batched"] = bench_batched(ctx)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(problems)
+        # the CPU leg also holds the closed forms: report each GPU result's
+        # distance from the exact integral there
+        out["cpu_baseline"]["gpu_vs_exact"] = check_against_exact(problems, per,
+                                                                  out.get("product_peak_to_convergence"))
     ctx.close()
     print(json.dumps(out))
 

@@ -99,7 +99,8 @@ typedef struct {
 typedef struct cuhre_context cuhre_context;
 
 /* Creates an engine on HIP device `device`.  capacity_regions bounds the
- * region list (0 = default of 64 Mi regions); memory grows on demand. */
+ * region list (0 = what three quarters of the free HBM holds); the list
+ * buffers start small and grow on demand. */
 int cuhre_context_create(cuhre_context** ctx, int device, int64_t capacity_regions);
 int cuhre_context_destroy(cuhre_context* ctx);
 

@@ -284,8 +284,8 @@ using namespace cuhre;
 struct cuhre_context {
   int device = 0;
   hipStream_t stream = nullptr;
-  int64_t capacity = 64ll << 20;  // regions
-  int64_t batch_capacity = 0;     // 0: sized from free HBM per batch call
+  int64_t capacity = 0;        // regions per list; 0: three quarters of free HBM
+  int64_t batch_capacity = 0;  // 0: sized from free HBM per batch call
   int profile = 0;
   int segments = 0;                // 0: automatic, else 1, 2, 4, 8 or 16
   int64_t fill_lanes = 262144;     // auto: smallest S with n*S >= fill_lanes
@@ -467,6 +467,19 @@ static int make_dev_integrand(cuhre_context* c, const cuhre_integrand* f, int di
 // Point segments per region: the smallest S with n*S >= 256k lanes, and at
 // least 2 below 2M regions (measured on MI355X, 8-D product peak and
 // oscillatory: S=16 is best up to 16k regions, S=4 at 64k-256k, S=2 at 1M).
+// Regions per list: the context's setting, else what three quarters of the
+// free HBM holds for two lists of this dimension (buffers the context
+// already owns count as free).
+static int64_t list_capacity(cuhre_context* c, int dim)
+{
+  if (c->capacity > 0) return c->capacity;
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) return 64ll << 20;
+  const double per = 4.0 * sizeof(double) * dim + 3 * sizeof(double) + 1 + sizeof(Partial) / 64.0 + 1;
+  const double held = (double)c->cap * (4.0 * sizeof(double) * c->dim_alloc + 3 * sizeof(double) + 1);
+  return (int64_t)(((double)fr + held) * 0.75 / per);
+}
+
 static int pick_segments(const cuhre_context* c, int64_t n)
 {
   if (c->segments > 0) return c->segments;
@@ -665,8 +678,9 @@ int cuhre_integrate(cuhre_context* c, const cuhre_integrand* f, int dim, double 
 
   std::vector<double> hlo, hwd;
   const int64_t n0 = initial_grid(dim, lows, highs, hlo, hwd);
-  if (n0 > c->capacity) return fail(CUHRE_ERR_CAPACITY, "initial grid exceeds capacity");
-  const int64_t want = std::min<int64_t>(c->capacity, std::max<int64_t>(4 * n0, 1 << 16));
+  const int64_t capacity = list_capacity(c, dim);
+  if (n0 > capacity) return fail(CUHRE_ERR_CAPACITY, "initial grid exceeds capacity");
+  const int64_t want = std::min<int64_t>(capacity, std::max<int64_t>(4 * n0, 1 << 16));
   if ((rc = ctx_reserve(c, want, dim))) return rc;
   int cur = 0;
   HIP_TRY(hipMemcpy2DAsync(c->lo[cur], sizeof(double) * c->cap, hlo.data(), sizeof(double) * n0,
@@ -689,7 +703,7 @@ int cuhre_integrate(cuhre_context* c, const cuhre_integrand* f, int dim, double 
     A.fin_err_s = fer_s;
     A.fin_err_c = fer_c;
     const int64_t left = maxeval - (out->neval + n * p);
-    A.max_active = std::min<int64_t>(left >= 0 ? left / (2 * p) : -1, c->capacity / 2);
+    A.max_active = std::min<int64_t>(left >= 0 ? left / (2 * p) : -1, capacity / 2);
     A.alloc_active = c->cap / 2;
     SplitArgs SA;
     fill_split_args(c, SA, cur, n, dim, epsrel, epsabs, seg_regions_per_block(S), true);
@@ -722,7 +736,7 @@ int cuhre_integrate(cuhre_context* c, const cuhre_integrand* f, int dim, double 
       // list buffers too small for the children: grow (keeping this
       // iteration's list and results) and run the split again
       const int64_t A2 = 2 * st.tot.nactive;
-      const int64_t ncap = std::min<int64_t>(c->capacity, std::max<int64_t>(A2, 2 * c->cap));
+      const int64_t ncap = std::min<int64_t>(capacity, std::max<int64_t>(A2, 2 * c->cap));
       if ((rc = ctx_reserve(c, ncap, dim, n, cur, nb))) return rc;
       fill_split_args(c, SA, cur, n, dim, epsrel, epsabs, seg_regions_per_block(S), false);
       if (c->profile) HIP_TRY(hipEventRecord(c->ev[1], c->stream));
