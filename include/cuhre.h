@@ -190,6 +190,65 @@ int cuhre_batch_autotune(cuhre_context* ctx, int32_t count, const cuhre_integran
                          const double* lows, const double* highs, cuhre_batch_config* best,
                          double* table_ms);
 
+/* ------------------------------------------------------ learned split guide
+ * A BF16 MLP (32-64-64-64-16, FP32 accumulation on MFMA) trained on the GPU
+ * by the library's own forward/backward kernels.  It proposes split axes
+ * and early finishes; the Genz-Malik error sum remains the only
+ * convergence test.  No reference counterpart. */
+#define CUHRE_GUIDE_PARAMS 11472 /* W0 32x64, W1 64x64, W2 64x64, W3 64x16, b0..b3 */
+
+typedef struct cuhre_guide cuhre_guide;
+
+typedef struct {
+  int32_t axis_mode;        /* 0: fourth difference; 1: guide when confident   */
+  int32_t finish_mode;      /* 0: none; 1: guide proposal + volume-share guard;
+                               2: guard alone (no network)                     */
+  float axis_confidence;    /* softmax probability needed to override the axis */
+  float finish_threshold;   /* sigmoid output needed to propose a finish       */
+  double budget;            /* share of epsrel*|I| retired regions may use     */
+} cuhre_guide_policy;
+
+typedef struct {
+  int64_t retired;          /* regions retired early by the finish rule        */
+  int64_t samples;          /* samples logged (collection runs)                */
+  int32_t guide_iterations;
+  int32_t pad;
+} cuhre_guide_stats;
+
+typedef struct {
+  double loss_first, loss_last;      /* mean loss of the first / last epoch  */
+  double axis_accuracy, finish_accuracy; /* last epoch, training samples     */
+  int32_t epochs, steps;
+  int64_t samples;
+  double ms;
+} cuhre_train_stats;
+
+int cuhre_guide_create(cuhre_context* ctx, uint64_t seed, cuhre_guide** guide);
+int cuhre_guide_destroy(cuhre_guide* guide);
+int cuhre_guide_set_params(cuhre_guide* guide, const float* params);
+int cuhre_guide_get_params(cuhre_guide* guide, float* params);
+/* logits[n*16] of features[n*32] */
+int cuhre_guide_forward(cuhre_guide* guide, int64_t n, const float* features, float* logits);
+/* gradient of the mean loss over n samples (no update) */
+int cuhre_guide_gradient(cuhre_guide* guide, int64_t n, const float* features, const int32_t* axis,
+                         const float* finish, int dim, float* grad, double* loss);
+/* plain run that logs up to max_samples (features, look-ahead axis, finish) */
+int cuhre_guide_collect(cuhre_guide* guide, const cuhre_integrand* f, int dim, double epsrel,
+                        double epsabs, int64_t maxeval, const double* lows, const double* highs,
+                        int64_t max_samples, cuhre_result* out);
+int64_t cuhre_guide_samples(const cuhre_guide* guide);
+int cuhre_guide_get_samples(cuhre_guide* guide, int64_t first, int64_t count, float* features,
+                            int32_t* axis, float* finish);
+/* Adam over the logged samples */
+int cuhre_guide_train(cuhre_guide* guide, int dim, int epochs, int batch, float lr, uint64_t seed,
+                      cuhre_train_stats* stats);
+/* the plain loop with the guide between evaluation and split (pol NULL:
+ * axis_mode 1, finish_mode 1, confidence 0.6, threshold 0.5, budget 0.25) */
+int cuhre_integrate_guided(cuhre_context* ctx, cuhre_guide* guide, const cuhre_integrand* f, int dim,
+                           double epsrel, double epsabs, int64_t maxeval, const double* lows,
+                           const double* highs, const cuhre_guide_policy* pol, cuhre_result* out,
+                           cuhre_guide_stats* stats);
+
 /* Thread-local message of the last error. */
 const char* cuhre_last_error(void);
 

@@ -1,0 +1,92 @@
+// context.hpp -- the engine context and host helpers shared by engine.hip
+// (plain loop) and learn.hip (guided loop).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "common.hpp"
+
+namespace cuhre {
+
+struct BatchEngine;
+
+struct SplitArgs {
+  const double* lower;
+  const double* width;
+  int64_t ld;
+  int64_t n;
+  const double* est;
+  const double* err;
+  const int8_t* axis;
+  double epsrel, epsabs;
+  const int64_t* offsets;  // per evaluation block (rpb regions)
+  int32_t rpb;
+  int32_t dim;
+  double* out_lower;
+  double* out_width;
+  int64_t out_ld;
+  const IterState* state;  // nullptr: always split
+  const int8_t* retire;    // optional: regions retired as finished are not split
+};
+
+void launch_split(const SplitArgs& S, hipStream_t st);
+
+}  // namespace cuhre
+
+struct cuhre_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int64_t capacity = 0;        // regions per list; 0: three quarters of free HBM
+  int64_t batch_capacity = 0;  // 0: sized from free HBM per batch call
+  int profile = 0;
+  int segments = 0;                // 0: automatic, else 1, 2, 4, 8 or 16
+  int64_t fill_lanes = 262144;     // auto: smallest S with n*S >= fill_lanes
+
+  int64_t cap = 0;  // allocated regions per list buffer
+  int dim_alloc = 0;
+  double* lo[2] = {nullptr, nullptr};
+  double* wd[2] = {nullptr, nullptr};
+  double* est = nullptr;
+  double* err = nullptr;
+  double* abssum = nullptr;
+  int8_t* axis = nullptr;
+  cuhre::Partial* partials = nullptr;
+  int64_t* offsets = nullptr;
+  uint32_t* ticket = nullptr;
+  cuhre::IterState* state = nullptr;
+  cuhre::IterState* h_state = nullptr;
+  double* aux = nullptr;  // i5 (cap) + dvec (cap*dim) on request
+  int64_t aux_cap = 0;
+  double2* sum_part = nullptr;
+  double* sum_out = nullptr;
+
+  double* table = nullptr;
+  std::vector<double> table_host;
+
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  cuhre::BatchEngine* batch = nullptr;
+};
+
+namespace cuhre {
+
+int fail_msg(int code, const char* msg);
+int ctx_reserve(cuhre_context* c, int64_t cap, int dim, int64_t keep_n = 0, int cur = 0, int64_t keep_nb = 0);
+int ctx_reserve_aux(cuhre_context* c, int64_t cap, int dim);
+int make_dev_integrand(cuhre_context* c, const cuhre_integrand* f, int dim, DevIntegrand* P);
+int64_t list_capacity(cuhre_context* c, int dim);
+int pick_segments(const cuhre_context* c, int64_t n);
+int64_t initial_grid(int d, const double* lows, const double* highs, std::vector<double>& lo,
+                     std::vector<double>& wd);
+int check_common(cuhre_context* ctx, const cuhre_integrand* f, int dim, int key);
+void fill_eval_args(cuhre_context* c, EvalArgs& A, int cur, int64_t n, int dim, double epsrel, double epsabs);
+void fill_split_args(cuhre_context* c, SplitArgs& S, int cur, int64_t n, int dim, double epsrel, double epsabs,
+                     int rpb, bool use_state);
+void dd_add_host(double& s1, double& c1, double s2, double c2);
+int64_t blocks_for(int64_t n, int S);
+RuleConst make_rule(int d);
+int gm_points(int d);
+int set_error(int code, const char* msg);
+
+}  // namespace cuhre

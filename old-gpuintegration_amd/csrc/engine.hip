@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "context.hpp"
 #include "eval_kernels.hpp"
 #include "reduce.hpp"
 
@@ -138,23 +139,7 @@ double region_flops(int fam, int d)
 
 // ------------------------------------------------------------------ split
 
-struct SplitArgs {
-  const double* lower;
-  const double* width;
-  int64_t ld;
-  int64_t n;
-  const double* est;
-  const double* err;
-  const int8_t* axis;
-  double epsrel, epsabs;
-  const int64_t* offsets;  // per evaluation block (rpb regions)
-  int32_t rpb;
-  int32_t dim;
-  double* out_lower;
-  double* out_width;
-  int64_t out_ld;
-  const IterState* state;  // nullptr: always split
-};
+
 
 // Classify + stream-compact + bisect in one pass (replaces the reference's
 // thrust::exclusive_scan, alignRegions and divideIntervalsGPU,
@@ -175,6 +160,7 @@ __global__ __launch_bounds__(kBlock) void k_split(SplitArgs S)
   if (valid) {
     const double e = S.est[r], er = S.err[r];
     act = isfinite(e) && isfinite(er) && region_active(e, er, S.epsrel, S.epsabs);
+    if (S.retire != nullptr && S.retire[r]) act = false;
   }
   const uint64_t mask = __ballot(act);
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
@@ -210,6 +196,11 @@ __global__ __launch_bounds__(kBlock) void k_split(SplitArgs S)
     *reinterpret_cast<double2*>(S.out_lower + m * S.out_ld + 2 * dst) = L;
     *reinterpret_cast<double2*>(S.out_width + m * S.out_ld + 2 * dst) = W;
   }
+}
+
+void launch_split(const SplitArgs& S, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_split, dim3((unsigned)((S.n + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, S);
 }
 
 // Active-region count per 256-region block (standalone split entry point).
@@ -281,39 +272,7 @@ using namespace cuhre;
 
 // ------------------------------------------------------------------ context
 
-struct cuhre_context {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  int64_t capacity = 0;        // regions per list; 0: three quarters of free HBM
-  int64_t batch_capacity = 0;  // 0: sized from free HBM per batch call
-  int profile = 0;
-  int segments = 0;                // 0: automatic, else 1, 2, 4, 8 or 16
-  int64_t fill_lanes = 262144;     // auto: smallest S with n*S >= fill_lanes
-
-  int64_t cap = 0;  // allocated regions per list buffer
-  int dim_alloc = 0;
-  double* lo[2] = {nullptr, nullptr};
-  double* wd[2] = {nullptr, nullptr};
-  double* est = nullptr;
-  double* err = nullptr;
-  double* abssum = nullptr;
-  int8_t* axis = nullptr;
-  Partial* partials = nullptr;
-  int64_t* offsets = nullptr;
-  uint32_t* ticket = nullptr;
-  IterState* state = nullptr;
-  IterState* h_state = nullptr;
-  double* aux = nullptr;  // i5 (cap) + dvec (cap*dim) on request
-  int64_t aux_cap = 0;
-  double2* sum_part = nullptr;
-  double* sum_out = nullptr;
-
-  double* table = nullptr;
-  std::vector<double> table_host;
-
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-  BatchEngine* batch = nullptr;
-};
+namespace cuhre {
 
 extern "C" __attribute__((visibility("hidden"))) BatchEngine* cuhre_ctx_batch(cuhre_context* c, hipStream_t* stream, int* device,
                                         int64_t* capacity)
@@ -325,7 +284,7 @@ extern "C" __attribute__((visibility("hidden"))) BatchEngine* cuhre_ctx_batch(cu
   return c->batch;
 }
 
-static int64_t blocks_for(int64_t n, int S)
+int64_t blocks_for(int64_t n, int S)
 {
   const int rpb = seg_regions_per_block(S);
   return (n + rpb - 1) / rpb;
@@ -355,8 +314,7 @@ static int ctx_free_lists(cuhre_context* c)
 // Allocates list buffers for `cap` regions of `dim` coordinates.  When
 // keep_n > 0 the first keep_n regions of list `cur` (and their est/err/axis,
 // and keep_nb block offsets) are carried over into the new buffers.
-static int ctx_reserve(cuhre_context* c, int64_t cap, int dim, int64_t keep_n = 0, int cur = 0,
-                       int64_t keep_nb = 0)
+int ctx_reserve(cuhre_context* c, int64_t cap, int dim, int64_t keep_n, int cur, int64_t keep_nb)
 {
   if (cap <= c->cap && dim <= c->dim_alloc) return 0;
   cap = std::max(cap, c->cap);
@@ -408,7 +366,7 @@ static int ctx_reserve(cuhre_context* c, int64_t cap, int dim, int64_t keep_n = 
   return 0;
 }
 
-static int ctx_reserve_aux(cuhre_context* c, int64_t cap, int dim)
+int ctx_reserve_aux(cuhre_context* c, int64_t cap, int dim)
 {
   const int64_t need = cap * (1 + dim);
   if (need <= c->aux_cap) return 0;
@@ -420,7 +378,7 @@ static int ctx_reserve_aux(cuhre_context* c, int64_t cap, int dim)
   return 0;
 }
 
-static int make_dev_integrand(cuhre_context* c, const cuhre_integrand* f, int dim, DevIntegrand* P)
+int make_dev_integrand(cuhre_context* c, const cuhre_integrand* f, int dim, DevIntegrand* P)
 {
   std::memset(P, 0, sizeof *P);
   P->family = f->family;
@@ -470,7 +428,7 @@ static int make_dev_integrand(cuhre_context* c, const cuhre_integrand* f, int di
 // Regions per list: the context's setting, else what three quarters of the
 // free HBM holds for two lists of this dimension (buffers the context
 // already owns count as free).
-static int64_t list_capacity(cuhre_context* c, int dim)
+int64_t list_capacity(cuhre_context* c, int dim)
 {
   if (c->capacity > 0) return c->capacity;
   size_t fr = 0, tot = 0;
@@ -480,7 +438,7 @@ static int64_t list_capacity(cuhre_context* c, int dim)
   return (int64_t)(((double)fr + held) * 0.75 / per);
 }
 
-static int pick_segments(const cuhre_context* c, int64_t n)
+int pick_segments(const cuhre_context* c, int64_t n)
 {
   if (c->segments > 0) return c->segments;
   int S = 1;
@@ -493,7 +451,7 @@ static int grid_k(int d) { return d <= 4 ? 4 : (d <= 10 ? 2 : 1); }
 
 // Initial uniform k^d grid over the box (GPUKernelquad.cu:23-61, 300-316),
 // region r has coordinate index (r / k^m) % k along axis m.
-static int64_t initial_grid(int d, const double* lows, const double* highs, std::vector<double>& lo,
+int64_t initial_grid(int d, const double* lows, const double* highs, std::vector<double>& lo,
                             std::vector<double>& wd)
 {
   const int k = grid_k(d);
@@ -514,7 +472,7 @@ static int64_t initial_grid(int d, const double* lows, const double* highs, std:
   return n;
 }
 
-static int check_common(cuhre_context* ctx, const cuhre_integrand* f, int dim, int key)
+int check_common(cuhre_context* ctx, const cuhre_integrand* f, int dim, int key)
 {
   if (!ctx) return fail(CUHRE_ERR_ARG, "null context");
   if (!f) return fail(CUHRE_ERR_ARG, "null integrand");
@@ -527,7 +485,7 @@ static int check_common(cuhre_context* ctx, const cuhre_integrand* f, int dim, i
   return 0;
 }
 
-static void fill_eval_args(cuhre_context* c, EvalArgs& A, int cur, int64_t n, int dim, double epsrel,
+void fill_eval_args(cuhre_context* c, EvalArgs& A, int cur, int64_t n, int dim, double epsrel,
                            double epsabs)
 {
   std::memset(&A, 0, sizeof A);
@@ -548,7 +506,7 @@ static void fill_eval_args(cuhre_context* c, EvalArgs& A, int cur, int64_t n, in
   (void)dim;
 }
 
-static void fill_split_args(cuhre_context* c, SplitArgs& S, int cur, int64_t n, int dim, double epsrel,
+void fill_split_args(cuhre_context* c, SplitArgs& S, int cur, int64_t n, int dim, double epsrel,
                             double epsabs, int rpb, bool use_state)
 {
   S.lower = c->lo[cur];
@@ -567,9 +525,10 @@ static void fill_split_args(cuhre_context* c, SplitArgs& S, int cur, int64_t n, 
   S.out_width = c->wd[1 - cur];
   S.out_ld = c->cap;
   S.state = use_state ? c->state : nullptr;
+  S.retire = nullptr;
 }
 
-static void dd_add_host(double& s1, double& c1, double s2, double c2)
+void dd_add_host(double& s1, double& c1, double s2, double c2)
 {
   const double s = s1 + s2;
   const double bb = s - s1;
@@ -577,6 +536,8 @@ static void dd_add_host(double& s1, double& c1, double s2, double c2)
   s1 = s;
   c1 = c1 + c2 + e;
 }
+
+}  // namespace cuhre
 
 extern "C" {
 

@@ -1,0 +1,1263 @@
+// learn.hip -- the learned split guide: network training and inference on
+// the MFMA units, sample collection from plain runs, and the guided loop.
+//
+// No reference counterpart (the reference always splits on the largest
+// fourth difference, quad/GPUquad/GPUQuadSample.cu:150-172, and finishes a
+// region only by its local test, quad/GPUquad/GPUQuadPhases.cu:63-73).
+//
+// The guide sees 25 features of a region (d = 7): its position and depth per
+// axis in the box, its fourth differences, its degree-7 estimate against the
+// mean density, its error against the volume share of the tolerance, the
+// relative I7 - I5 gap and its total depth.  It proposes
+//   * a split axis (trained to imitate the look-ahead axis: the bisection
+//     whose two children have the smallest summed Genz-Malik error), used
+//     when its softmax confidence passes a threshold, else the fourth
+//     difference decides as in the plain method;
+//   * a finish flag (trained on regions whose error fits inside their volume
+//     share of epsrel * |I|, I being the run's final estimate), honoured only
+//     when the region's error really fits inside its volume share of
+//     budget * epsrel * |current estimate|, so the retired error stays below
+//     budget * epsrel * |I| in total.
+// The Genz-Malik error sum stays the only convergence test.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "common.hpp"
+#include "context.hpp"
+#include "eval_kernels.hpp"
+#include "mlp.hpp"
+#include "reduce.hpp"
+
+namespace cuhre {
+
+int set_error(int code, const char* msg);
+
+// ---------------------------------------------------------------- training
+
+// Master FP32 parameters -> BF16 W ([K][N]) and W^T ([N][K]) copies.
+__global__ void k_mlp_prepare(const float* params, uint16_t* W, uint16_t* WT)
+{
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= kMlpBOff[0]) return;
+  int l = 3;
+  while (p < kMlpWOff[l]) --l;
+  const int K = kMlpK[l], N = kMlpN[l];
+  const int q = p - kMlpWOff[l];
+  const int k = q / N, n = q - k * N;
+  const uint16_t h = f2bf(params[p]);
+  W[kMlpWOff[l] + k * N + n] = h;
+  WT[kMlpWOff[l] + n * K + k] = h;
+}
+
+// Inference on rows given as FP32 features (n x 32) -> logits (n x 16).
+__global__ __launch_bounds__(256) void k_mlp_forward(MlpDev M, const float* feat, int64_t n, float* out)
+{
+  __shared__ __align__(16) uint16_t X[kMlpRows * kMlpIn];
+  __shared__ __align__(16) uint16_t H1[kMlpRows * kMlpHid], H2[kMlpRows * kMlpHid], H3[kMlpRows * kMlpHid];
+  __shared__ float Y[kMlpRows * kMlpOut];
+  const int64_t r0 = (int64_t)blockIdx.x * kMlpRows;
+  for (int e = threadIdx.x; e < kMlpRows * kMlpIn; e += blockDim.x) {
+    const int64_t r = r0 + e / kMlpIn;
+    X[e] = r < n ? f2bf(feat[r * kMlpIn + e % kMlpIn]) : (uint16_t)0;
+  }
+  __syncthreads();
+  const int w = threadIdx.x >> 6;
+  mlp_forward16(M, X + w * 16 * kMlpIn, H1 + w * 16 * kMlpHid, H2 + w * 16 * kMlpHid, H3 + w * 16 * kMlpHid,
+                Y + w * 16 * kMlpOut);
+  for (int e = threadIdx.x; e < kMlpRows * kMlpOut; e += blockDim.x) {
+    const int64_t r = r0 + e / kMlpOut;
+    if (r < n) out[r * kMlpOut + e % kMlpOut] = Y[e];
+  }
+}
+
+// dH[16 x K] = dZ[16 x N] . W^T for this wave's rows, then dZprev = dH * (H > 0)
+// stored as BF16.  W row-major [K][N] gives contiguous B operands.
+template <int K, int N>
+__device__ __forceinline__ void mlp_layer_bwd(const uint16_t* dZs, const uint16_t* W, const uint16_t* Hs,
+                                              uint16_t* dZprev)
+{
+  const int l = __lane_id(), row = l & 15, kq = (l >> 4) * 4;
+  f32x4 acc[K / 16];
+#pragma unroll
+  for (int jt = 0; jt < K / 16; ++jt) acc[jt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int kt = 0; kt < N / 16; ++kt) {
+    const bf16x4 a = *reinterpret_cast<const bf16x4*>(dZs + row * N + kt * 16 + kq);
+#pragma unroll
+    for (int jt = 0; jt < K / 16; ++jt) {
+      const bf16x4 b = *reinterpret_cast<const bf16x4*>(W + (jt * 16 + row) * N + kt * 16 + kq);
+      acc[jt] = mfma16(a, b, acc[jt]);
+    }
+  }
+#pragma unroll
+  for (int jt = 0; jt < K / 16; ++jt)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int idx = (kq + v) * K + jt * 16 + row;
+      dZprev[idx] = Hs[idx] != 0 ? f2bf(acc[jt][v]) : (uint16_t)0;
+    }
+}
+
+// One 16x16 tile of dW = H^T dZ over the block's 64 rows (4 MFMA steps).
+__device__ __forceinline__ f32x4 mlp_dw_tile(const uint16_t* Hs, int K, const uint16_t* dZs, int N, int mt, int nt)
+{
+  const int l = __lane_id(), row = l & 15, kq = (l >> 4) * 4;
+  f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int kt = 0; kt < kMlpRows / 16; ++kt) {
+    bf16x4 a, b;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int s = kt * 16 + kq + v;
+      a[v] = (short)Hs[s * K + mt * 16 + row];
+      b[v] = (short)dZs[s * N + nt * 16 + row];
+    }
+    acc = mfma16(a, b, acc);
+  }
+  return acc;
+}
+
+struct TrainArgs {
+  MlpDev M;
+  const float* feat;     // samples x 32
+  const int8_t* axis;    // samples
+  const float* finish;   // samples
+  const int32_t* order;  // batch rows -> sample index
+  int64_t rows;          // rows in this batch
+  int dim;
+  float inv_batch;
+  float* part;           // blocks x kMlpParams partial gradients
+  float* loss;           // rows
+  int8_t* correct;       // rows: bit 0 axis argmax right, bit 1 finish right
+};
+
+// Forward + backward of 64 rows per block; the block's weight gradient is
+// written to its own slot (reduced in a fixed order by k_mlp_reduce_adam).
+__global__ __launch_bounds__(256) void k_mlp_train(TrainArgs T)
+{
+  __shared__ __align__(16) uint16_t X[kMlpRows * kMlpIn];
+  __shared__ __align__(16) uint16_t H1[kMlpRows * kMlpHid], H2[kMlpRows * kMlpHid], H3[kMlpRows * kMlpHid];
+  __shared__ float Y[kMlpRows * kMlpOut];
+  __shared__ __align__(16) uint16_t dZ4[kMlpRows * kMlpOut];
+  __shared__ __align__(16) uint16_t dZ3[kMlpRows * kMlpHid], dZ2[kMlpRows * kMlpHid], dZ1[kMlpRows * kMlpHid];
+  const int64_t r0 = (int64_t)blockIdx.x * kMlpRows;
+  for (int e = threadIdx.x; e < kMlpRows * kMlpIn; e += blockDim.x) {
+    const int64_t r = r0 + e / kMlpIn;
+    X[e] = r < T.rows ? f2bf(T.feat[(int64_t)T.order[r] * kMlpIn + e % kMlpIn]) : (uint16_t)0;
+  }
+  __syncthreads();
+  const int w = threadIdx.x >> 6;
+  mlp_forward16(T.M, X + w * 16 * kMlpIn, H1 + w * 16 * kMlpHid, H2 + w * 16 * kMlpHid,
+                H3 + w * 16 * kMlpHid, Y + w * 16 * kMlpOut);
+  // loss and dY (softmax cross-entropy over the d axis logits + BCE finish)
+  if (threadIdx.x < kMlpRows) {
+    const int rr = threadIdx.x;
+    const int64_t r = r0 + rr;
+    uint16_t* dz = dZ4 + rr * kMlpOut;
+    for (int j = 0; j < kMlpOut; ++j) dz[j] = 0;
+    if (r < T.rows) {
+      const int64_t s = T.order[r];
+      const float* y = Y + rr * kMlpOut;
+      const int a = T.axis[s];
+      const float f = T.finish[s];
+      float m = y[0];
+      int am = 0;
+      for (int j = 1; j < T.dim; ++j)
+        if (y[j] > m) {
+          m = y[j];
+          am = j;
+        }
+      float se = 0.f;
+      for (int j = 0; j < T.dim; ++j) se += expf(y[j] - m);
+      const float lse = m + logf(se);
+      const float z = y[kMlpFinish];
+      const float q = 1.f / (1.f + expf(-z));
+      const float bce = fmaxf(z, 0.f) - z * f + log1pf(expf(-fabsf(z)));
+      T.loss[r] = (lse - y[a]) + bce;
+      T.correct[r] = (int8_t)((am == a ? 1 : 0) | (((q >= 0.5f) == (f >= 0.5f)) ? 2 : 0));
+      for (int j = 0; j < T.dim; ++j) {
+        const float pj = expf(y[j] - lse);
+        dz[j] = f2bf((pj - (j == a ? 1.f : 0.f)) * T.inv_batch);
+      }
+      dz[kMlpFinish] = f2bf((q - f) * T.inv_batch);
+    }
+  }
+  __syncthreads();
+  mlp_layer_bwd<64, 16>(dZ4 + w * 16 * kMlpOut, T.M.W + kMlpWOff[3], H3 + w * 16 * kMlpHid,
+                        dZ3 + w * 16 * kMlpHid);
+  __syncthreads();
+  mlp_layer_bwd<64, 64>(dZ3 + w * 16 * kMlpHid, T.M.W + kMlpWOff[2], H2 + w * 16 * kMlpHid,
+                        dZ2 + w * 16 * kMlpHid);
+  __syncthreads();
+  mlp_layer_bwd<64, 64>(dZ2 + w * 16 * kMlpHid, T.M.W + kMlpWOff[1], H1 + w * 16 * kMlpHid,
+                        dZ1 + w * 16 * kMlpHid);
+  __syncthreads();
+  // weight gradients: 44 tiles of 16x16 shared out over the 4 waves
+  float* part = T.part + (int64_t)blockIdx.x * kMlpParams;
+  const int l = __lane_id(), row = l & 15, kq = (l >> 4) * 4;
+  for (int t = w; t < 44; t += 4) {
+    int layer, mt, nt;
+    if (t < 8) { layer = 0; mt = t / 4; nt = t % 4; }
+    else if (t < 24) { layer = 1; mt = (t - 8) / 4; nt = (t - 8) % 4; }
+    else if (t < 40) { layer = 2; mt = (t - 24) / 4; nt = (t - 24) % 4; }
+    else { layer = 3; mt = t - 40; nt = 0; }
+    const uint16_t* Hs = layer == 0 ? X : (layer == 1 ? H1 : (layer == 2 ? H2 : H3));
+    const uint16_t* dZs = layer == 0 ? dZ1 : (layer == 1 ? dZ2 : (layer == 2 ? dZ3 : dZ4));
+    const int K = kMlpK[layer], N = kMlpN[layer];
+    const f32x4 acc = mlp_dw_tile(Hs, K, dZs, N, mt, nt);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) part[kMlpWOff[layer] + (mt * 16 + kq + v) * N + nt * 16 + row] = acc[v];
+  }
+  // bias gradients: column sums of the BF16 upstream gradients
+  for (int c = threadIdx.x; c < kMlpParams - kMlpBOff[0]; c += blockDim.x) {
+    int layer = 3;
+    while (c < kMlpBOff[layer] - kMlpBOff[0]) --layer;
+    const int N = kMlpN[layer], col = c - (kMlpBOff[layer] - kMlpBOff[0]);
+    const uint16_t* dZs = layer == 0 ? dZ1 : (layer == 1 ? dZ2 : (layer == 2 ? dZ3 : dZ4));
+    float sacc = 0.f;
+    for (int s = 0; s < kMlpRows; ++s) sacc += bf2f(dZs[s * N + col]);
+    part[kMlpBOff[0] + c] = sacc;
+  }
+}
+
+// grad[p] = sum over blocks (fixed order); optional Adam step on the FP32
+// master parameters followed by the BF16 copies.
+__global__ void k_mlp_reduce_adam(const float* part, int nblocks, float* grad, float* params, float* m,
+                                  float* v, int step, float lr, int do_update, uint16_t* W, uint16_t* WT)
+{
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= kMlpParams) return;
+  float g = 0.f;
+  for (int b = 0; b < nblocks; ++b) g += part[(int64_t)b * kMlpParams + p];
+  grad[p] = g;
+  if (!do_update) return;
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  const float mm = b1 * m[p] + (1.f - b1) * g;
+  const float vv = b2 * v[p] + (1.f - b2) * g * g;
+  m[p] = mm;
+  v[p] = vv;
+  const float mh = mm / (1.f - powf(b1, (float)step));
+  const float vh = vv / (1.f - powf(b2, (float)step));
+  const float np = params[p] - lr * mh / (sqrtf(vh) + eps);
+  params[p] = np;
+  if (p < kMlpBOff[0]) {
+    int l = 3;
+    while (p < kMlpWOff[l]) --l;
+    const int K = kMlpK[l], N = kMlpN[l];
+    const int q = p - kMlpWOff[l];
+    const int k = q / N, n = q - k * N;
+    const uint16_t h = f2bf(np);
+    W[kMlpWOff[l] + k * N + n] = h;
+    WT[kMlpWOff[l] + n * K + k] = h;
+  }
+}
+
+
+// ---------------------------------------------------------------- features
+
+struct GuideArgs {
+  const double* lower;
+  const double* width;
+  int64_t ld;
+  int64_t n;
+  const double* est;
+  const double* err;
+  const double* i5;
+  const double* dvec;
+  const double* abssum;
+  int8_t* axis;    // in/out: the fourth-difference axis, replaced by a confident guide
+  int8_t* retire;  // out: 1 = retired as finished, not split
+  int dim;
+  double lows[kMaxDim], highs[kMaxDim];
+  double absI;  // |current estimate of the integral|
+  double epsrel, epsabs;
+  int axis_mode;    // 0: fourth difference, 1: guide when confident
+  int finish_mode;  // 0: none, 1: guide proposal + volume-share guard, 2: guard alone
+  int use_mlp;
+  float axis_conf, finish_thr;
+  double budget;  // fraction of epsrel*|I| the retired regions may spend
+  MlpDev M;
+  int64_t* tile_cnt;     // regions split per 64-region tile
+  int64_t* tile_logcnt;  // regions logged per tile (collection)
+  double* tile_ret;      // per tile: retired est (s,c), err (s,c), abssum, |est|, count
+  uint64_t log_stride;   // 0: no collection
+  uint64_t log_salt;
+};
+
+constexpr int kRetStride = 8;
+
+struct GuideSums {
+  int64_t n_split, n_retired, n_logged, pad;
+  double est_s, est_c, err_s, err_c, abs_r, aest_r;
+};
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ bool guide_selected(const GuideArgs& G, int64_t r, bool active)
+{
+  return active && G.log_stride > 0 && (mix64((uint64_t)r ^ G.log_salt) % G.log_stride) == 0;
+}
+
+// The 32 input features of region r (25 used at d = 7) and its relative volume.
+__device__ __forceinline__ double guide_features(const GuideArgs& G, int64_t r, float* f)
+{
+  const int d = G.dim;
+  double vrel = 1.0, maxd = 0.0;
+  for (int j = 0; j < 32; ++j) f[j] = 0.f;
+  for (int m = 0; m < d; ++m) {
+    const double span = G.highs[m] - G.lows[m];
+    const double lo = G.lower[m * G.ld + r], w = G.width[m * G.ld + r];
+    const double wr = w / span;
+    vrel *= wr;
+    f[m] = (float)((lo - G.lows[m]) / span);
+    f[8 + m] = (float)(-log2(wr) / 16.0);
+    maxd = fmax(maxd, G.dvec[m * G.ld + r]);
+  }
+  for (int m = 0; m < d; ++m) f[16 + m] = maxd > 0.0 ? (float)(G.dvec[m * G.ld + r] / maxd) : 0.f;
+  const double e = G.est[r], er = G.err[r], i5 = G.i5[r];
+  const double scale = G.absI * vrel;
+  f[24] = (float)(log10(fabs(e) / scale + 1e-30) / 8.0);
+  f[25] = (float)(log10(er / (G.epsrel * scale) + 1e-30) / 8.0);
+  f[26] = (float)((e - i5) / (fabs(e) + fabs(i5) + 1e-300));
+  f[27] = (float)(-log2(vrel) / 64.0);
+  return vrel;
+}
+
+// Guide step of one iteration on 64 regions per block: features, network,
+// axis / finish decisions, per-tile counts and retired sums.
+__global__ __launch_bounds__(256) void k_guide_apply(GuideArgs G)
+{
+  __shared__ __align__(16) uint16_t X[kMlpRows * kMlpIn];
+  __shared__ __align__(16) uint16_t H1[kMlpRows * kMlpHid], H2[kMlpRows * kMlpHid], H3[kMlpRows * kMlpHid];
+  __shared__ float Y[kMlpRows * kMlpOut];
+  __shared__ int s_split[kMlpRows], s_log[kMlpRows], s_ret[kMlpRows];
+  __shared__ double s_e[kMlpRows], s_er[kMlpRows], s_ab[kMlpRows];
+  const int t = threadIdx.x;
+  const int64_t r = (int64_t)blockIdx.x * kMlpRows + t;
+  bool active = false;
+  double vrel = 0.0;
+  if (t < kMlpRows) {
+    float f[32];
+    for (int j = 0; j < 32; ++j) f[j] = 0.f;
+    if (r < G.n) {
+      const double e = G.est[r], er = G.err[r];
+      active = isfinite(e) && isfinite(er) && region_active(e, er, G.epsrel, G.epsabs);
+      vrel = guide_features(G, r, f);
+    }
+    for (int j = 0; j < 32; ++j) X[t * kMlpIn + j] = f2bf(f[j]);
+  }
+  __syncthreads();
+  if (G.use_mlp) {
+    const int w = t >> 6;
+    mlp_forward16(G.M, X + w * 16 * kMlpIn, H1 + w * 16 * kMlpHid, H2 + w * 16 * kMlpHid, H3 + w * 16 * kMlpHid,
+                  Y + w * 16 * kMlpOut);
+  }
+  if (t < kMlpRows) {
+    int retire = 0;
+    if (active) {
+      const double e = G.est[r], er = G.err[r];
+      const bool guard = er <= G.budget * G.epsrel * G.absI * vrel;
+      if (G.use_mlp) {
+        const float* y = Y + t * kMlpOut;
+        float m = y[0];
+        int am = 0;
+        for (int j = 1; j < G.dim; ++j)
+          if (y[j] > m) {
+            m = y[j];
+            am = j;
+          }
+        float se = 0.f;
+        for (int j = 0; j < G.dim; ++j) se += expf(y[j] - m);
+        const float conf = 1.f / se;  // softmax probability of the argmax
+        if (G.axis_mode == 1 && conf >= G.axis_conf) G.axis[r] = (int8_t)am;
+        const float q = 1.f / (1.f + expf(-y[kMlpFinish]));
+        if (G.finish_mode == 1 && q >= G.finish_thr && guard) retire = 1;
+      }
+      if (G.finish_mode == 2 && guard) retire = 1;
+      s_e[t] = e;
+      s_er[t] = er;
+      s_ab[t] = G.abssum[r];
+    } else {
+      s_e[t] = s_er[t] = s_ab[t] = 0.0;
+    }
+    if (r < G.n) G.retire[r] = (int8_t)retire;
+    s_ret[t] = retire;
+    s_split[t] = active && !retire ? 1 : 0;
+    s_log[t] = guide_selected(G, r, active) ? 1 : 0;
+  }
+  __syncthreads();
+  if (t == 0) {
+    int64_t ns = 0, nl = 0, nr = 0;
+    double es = 0, ec = 0, rs = 0, rc = 0, ab = 0, ae = 0;
+    for (int j = 0; j < kMlpRows; ++j) {
+      ns += s_split[j];
+      nl += s_log[j];
+      if (s_ret[j]) {
+        ++nr;
+        dd_add(es, ec, s_e[j], 0.0);
+        dd_add(rs, rc, s_er[j], 0.0);
+        ab += s_ab[j];
+        ae += fabs(s_e[j]);
+      }
+    }
+    G.tile_cnt[blockIdx.x] = ns;
+    G.tile_logcnt[blockIdx.x] = nl;
+    double* tr = G.tile_ret + (int64_t)blockIdx.x * kRetStride;
+    tr[0] = es;
+    tr[1] = ec;
+    tr[2] = rs;
+    tr[3] = rc;
+    tr[4] = ab;
+    tr[5] = ae;
+    tr[6] = (double)nr;
+    tr[7] = 0.0;
+  }
+}
+
+// One block: exclusive scans of the split / logged counts per tile and the
+// retired sums in tile order.
+__global__ __launch_bounds__(1024) void k_guide_scan(const int64_t* cnt, const int64_t* logcnt, const double* ret,
+                                                     int64_t ntiles, int64_t* off, int64_t* logoff, GuideSums* out)
+{
+  __shared__ int64_t sa[1024], sb[1024];
+  __shared__ double sd[1024][7];
+  const int T = blockDim.x;
+  const int64_t chunk = (ntiles + T - 1) / T;
+  const int64_t t0 = threadIdx.x * chunk;
+  const int64_t t1 = t0 + chunk < ntiles ? t0 + chunk : ntiles;
+  int64_t ma = 0, mb = 0;
+  double es = 0, ec = 0, rs = 0, rc = 0, ab = 0, ae = 0, nr = 0;
+  for (int64_t t = t0; t < t1; ++t) {
+    ma += cnt[t];
+    mb += logcnt[t];
+    const double* tr = ret + t * kRetStride;
+    dd_add(es, ec, tr[0], tr[1]);
+    dd_add(rs, rc, tr[2], tr[3]);
+    ab += tr[4];
+    ae += tr[5];
+    nr += tr[6];
+  }
+  sa[threadIdx.x] = ma;
+  sb[threadIdx.x] = mb;
+  sd[threadIdx.x][0] = es;
+  sd[threadIdx.x][1] = ec;
+  sd[threadIdx.x][2] = rs;
+  sd[threadIdx.x][3] = rc;
+  sd[threadIdx.x][4] = ab;
+  sd[threadIdx.x][5] = ae;
+  sd[threadIdx.x][6] = nr;
+  __syncthreads();
+  for (int o = 1; o < T; o <<= 1) {
+    const int64_t va = (int)threadIdx.x >= o ? sa[threadIdx.x - o] : 0;
+    const int64_t vb = (int)threadIdx.x >= o ? sb[threadIdx.x - o] : 0;
+    __syncthreads();
+    sa[threadIdx.x] += va;
+    sb[threadIdx.x] += vb;
+    __syncthreads();
+  }
+  int64_t ra = sa[threadIdx.x] - ma, rb = sb[threadIdx.x] - mb;
+  for (int64_t t = t0; t < t1; ++t) {
+    off[t] = ra;
+    logoff[t] = rb;
+    ra += cnt[t];
+    rb += logcnt[t];
+  }
+  if (threadIdx.x == 0) {
+    GuideSums g;
+    g.n_split = sa[T - 1];
+    g.n_logged = sb[T - 1];
+    double a0 = 0, a1 = 0, b0 = 0, b1 = 0, c = 0, e = 0, n = 0;
+    for (int i = 0; i < T; ++i) {
+      dd_add(a0, a1, sd[i][0], sd[i][1]);
+      dd_add(b0, b1, sd[i][2], sd[i][3]);
+      c += sd[i][4];
+      e += sd[i][5];
+      n += sd[i][6];
+    }
+    g.est_s = a0;
+    g.est_c = a1;
+    g.err_s = b0;
+    g.err_c = b1;
+    g.abs_r = c;
+    g.aest_r = e;
+    g.n_retired = (int64_t)n;
+    g.pad = 0;
+    *out = g;
+  }
+}
+
+struct CollectArgs {
+  GuideArgs G;
+  const int64_t* logoff;
+  int64_t base;     // dataset rows before this iteration
+  float* feat;      // dataset features
+  double* s_err;    // dataset: region error
+  double* s_vrel;   // dataset: relative volume
+  double* la_lo;    // look-ahead children, SoA with stride la_ld
+  double* la_wd;
+  int64_t la_ld;
+};
+
+// Writes the logged regions' features and their 2d look-ahead children
+// (each axis bisected) in deterministic order (tile offset + rank).
+__global__ __launch_bounds__(64) void k_collect_write(CollectArgs C)
+{
+  const GuideArgs& G = C.G;
+  const int64_t r = (int64_t)blockIdx.x * kMlpRows + threadIdx.x;
+  bool active = false;
+  if (r < G.n) {
+    const double e = G.est[r], er = G.err[r];
+    active = isfinite(e) && isfinite(er) && region_active(e, er, G.epsrel, G.epsabs);
+  }
+  const bool sel = guide_selected(G, r, active);
+  const uint64_t mask = __ballot(sel);
+  const int lane = threadIdx.x;
+  const uint64_t below = lane ? (~0ull >> (kWave - lane)) : 0ull;
+  if (!sel) return;
+  const int64_t j = C.logoff[blockIdx.x] + __popcll(mask & below);
+  float f[32];
+  const double vrel = guide_features(G, r, f);
+  const int64_t s = C.base + j;
+  for (int q = 0; q < 32; ++q) C.feat[s * 32 + q] = f[q];
+  C.s_err[s] = G.err[r];
+  C.s_vrel[s] = vrel;
+  const int d = G.dim;
+  for (int a = 0; a < d; ++a)
+    for (int c = 0; c < 2; ++c) {
+      const int64_t k = j * 2 * d + 2 * a + c;
+      for (int m = 0; m < d; ++m) {
+        const double lo = G.lower[m * G.ld + r], w = G.width[m * G.ld + r];
+        const double hw = 0.5 * w;
+        C.la_lo[m * C.la_ld + k] = m == a ? (c ? lo + hw : lo) : lo;
+        C.la_wd[m * C.la_ld + k] = m == a ? hw : w;
+      }
+    }
+}
+
+// Axis label of each logged region: the bisection whose two children have the
+// smallest summed error (first minimum).
+__global__ void k_label_axis(const double* la_err, int64_t nlog, int d, int64_t base, int8_t* s_axis)
+{
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nlog) return;
+  double best = 0.0;
+  int ba = 0;
+  for (int a = 0; a < d; ++a) {
+    const double e = la_err[j * 2 * d + 2 * a] + la_err[j * 2 * d + 2 * a + 1];
+    if (a == 0 || e < best) {
+      best = e;
+      ba = a;
+    }
+  }
+  s_axis[base + j] = (int8_t)ba;
+}
+
+// Finish label of a run's samples: the region's error fits in its volume
+// share of epsrel * |I| (I = the run's final estimate).
+__global__ void k_label_finish(const double* s_err, const double* s_vrel, int64_t b0, int64_t b1, double tol,
+                               float* s_fin)
+{
+  const int64_t s = b0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= b1) return;
+  s_fin[s] = s_err[s] <= tol * s_vrel[s] ? 1.f : 0.f;
+}
+
+__global__ void k_train_stats(const float* loss, const int8_t* correct, int64_t rows, double* acc)
+{
+  double l = 0, a = 0, f = 0;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < rows; r += (int64_t)gridDim.x * blockDim.x) {
+    l += loss[r];
+    a += (correct[r] & 1) ? 1 : 0;
+    f += (correct[r] & 2) ? 1 : 0;
+  }
+  atomicAdd(acc + 0, l);
+  atomicAdd(acc + 1, a);
+  atomicAdd(acc + 2, f);
+}
+
+}  // namespace cuhre
+
+using namespace cuhre;
+
+// ------------------------------------------------------------------ host
+
+struct cuhre_guide {
+  cuhre_context* ctx = nullptr;
+  float* params = nullptr;  // FP32 master
+  float* m = nullptr;
+  float* v = nullptr;
+  uint16_t* W = nullptr;
+  uint16_t* WT = nullptr;
+  float* grad = nullptr;
+  float* part = nullptr;
+  int part_blocks = 0;
+  int adam_step = 0;
+  // dataset
+  int64_t ds_cap = 0, ds_n = 0;
+  float* ds_feat = nullptr;
+  int8_t* ds_axis = nullptr;
+  float* ds_fin = nullptr;
+  double* ds_err = nullptr;
+  double* ds_vrel = nullptr;
+  // look-ahead buffers
+  int64_t la_cap = 0;
+  double *la_lo = nullptr, *la_wd = nullptr, *la_est = nullptr, *la_err = nullptr, *la_ab = nullptr;
+  int8_t* la_axis = nullptr;
+  // per-iteration buffers of the guided loop
+  int64_t reg_cap = 0;
+  int8_t* retire = nullptr;
+  int64_t *tile_cnt = nullptr, *tile_off = nullptr, *tile_logcnt = nullptr, *tile_logoff = nullptr;
+  double* tile_ret = nullptr;
+  GuideSums* d_sums = nullptr;
+  GuideSums* h_sums = nullptr;
+  // training scratch
+  int64_t train_cap = 0;
+  int32_t* order = nullptr;
+  float* loss = nullptr;
+  int8_t* correct = nullptr;
+  double* d_stats = nullptr;
+};
+
+namespace {
+
+void gfree(void* p)
+{
+  if (p) (void)hipFree(p);
+}
+
+int guide_reserve_regions(cuhre_guide* g, int64_t cap)
+{
+  if (cap <= g->reg_cap) return 0;
+  cap = std::max(cap, 2 * g->reg_cap);
+  gfree(g->retire);
+  gfree(g->tile_cnt);
+  gfree(g->tile_off);
+  gfree(g->tile_logcnt);
+  gfree(g->tile_logoff);
+  gfree(g->tile_ret);
+  const int64_t nt = cap / kMlpRows + 1;
+  if (hipMalloc(&g->retire, cap) != hipSuccess || hipMalloc(&g->tile_cnt, sizeof(int64_t) * nt) != hipSuccess ||
+      hipMalloc(&g->tile_off, sizeof(int64_t) * nt) != hipSuccess ||
+      hipMalloc(&g->tile_logcnt, sizeof(int64_t) * nt) != hipSuccess ||
+      hipMalloc(&g->tile_logoff, sizeof(int64_t) * nt) != hipSuccess ||
+      hipMalloc(&g->tile_ret, sizeof(double) * kRetStride * nt) != hipSuccess) {
+    (void)hipGetLastError();
+    g->reg_cap = 0;
+    return set_error(CUHRE_ERR_ALLOC, "cannot allocate guide buffers");
+  }
+  g->reg_cap = cap;
+  return 0;
+}
+
+int guide_reserve_dataset(cuhre_guide* g, int64_t need)
+{
+  if (need <= g->ds_cap) return 0;
+  const int64_t cap = std::max(need, 2 * g->ds_cap);
+  float *feat, *fin;
+  int8_t* ax;
+  double *er, *vr;
+  if (hipMalloc(&feat, sizeof(float) * 32 * cap) != hipSuccess || hipMalloc(&fin, sizeof(float) * cap) != hipSuccess ||
+      hipMalloc(&ax, cap) != hipSuccess || hipMalloc(&er, sizeof(double) * cap) != hipSuccess ||
+      hipMalloc(&vr, sizeof(double) * cap) != hipSuccess) {
+    (void)hipGetLastError();
+    return set_error(CUHRE_ERR_ALLOC, "cannot allocate the guide's sample store");
+  }
+  if (g->ds_n > 0) {
+    const int64_t n = g->ds_n;
+    if (hipMemcpy(feat, g->ds_feat, sizeof(float) * 32 * n, hipMemcpyDeviceToDevice) != hipSuccess ||
+        hipMemcpy(fin, g->ds_fin, sizeof(float) * n, hipMemcpyDeviceToDevice) != hipSuccess ||
+        hipMemcpy(ax, g->ds_axis, n, hipMemcpyDeviceToDevice) != hipSuccess ||
+        hipMemcpy(er, g->ds_err, sizeof(double) * n, hipMemcpyDeviceToDevice) != hipSuccess ||
+        hipMemcpy(vr, g->ds_vrel, sizeof(double) * n, hipMemcpyDeviceToDevice) != hipSuccess)
+      return set_error(CUHRE_ERR_DEVICE, "sample store copy failed");
+  }
+  gfree(g->ds_feat);
+  gfree(g->ds_fin);
+  gfree(g->ds_axis);
+  gfree(g->ds_err);
+  gfree(g->ds_vrel);
+  g->ds_feat = feat;
+  g->ds_fin = fin;
+  g->ds_axis = ax;
+  g->ds_err = er;
+  g->ds_vrel = vr;
+  g->ds_cap = cap;
+  return 0;
+}
+
+int guide_reserve_lookahead(cuhre_guide* g, int64_t need, int dim)
+{
+  if (need <= g->la_cap) return 0;
+  const int64_t cap = (std::max(need, 2 * g->la_cap) + 255) & ~255ll;
+  for (void* p : {(void*)g->la_lo, (void*)g->la_wd, (void*)g->la_est, (void*)g->la_err, (void*)g->la_ab,
+                  (void*)g->la_axis})
+    gfree(p);
+  if (hipMalloc(&g->la_lo, sizeof(double) * cap * kMaxDim) != hipSuccess ||
+      hipMalloc(&g->la_wd, sizeof(double) * cap * kMaxDim) != hipSuccess ||
+      hipMalloc(&g->la_est, sizeof(double) * cap) != hipSuccess ||
+      hipMalloc(&g->la_err, sizeof(double) * cap) != hipSuccess ||
+      hipMalloc(&g->la_ab, sizeof(double) * cap) != hipSuccess || hipMalloc(&g->la_axis, cap) != hipSuccess) {
+    (void)hipGetLastError();
+    g->la_cap = 0;
+    return set_error(CUHRE_ERR_ALLOC, "cannot allocate look-ahead buffers");
+  }
+  (void)dim;
+  g->la_cap = cap;
+  return 0;
+}
+
+MlpDev mlp_dev(const cuhre_guide* g)
+{
+  MlpDev M;
+  M.W = g->W;
+  M.WT = g->WT;
+  M.b = g->params + kMlpBOff[0];
+  return M;
+}
+
+// He-initialised weights from SplitMix64 + Box-Muller, zero biases.
+void init_params(std::vector<float>& p, uint64_t seed)
+{
+  p.assign(kMlpParams, 0.f);
+  uint64_t st = seed;
+  auto u01 = [&]() {
+    st += 0x9E3779B97F4A7C15ull;
+    uint64_t z = st;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+  };
+  for (int l = 0; l < 4; ++l) {
+    const double sd = std::sqrt(2.0 / kMlpK[l]);
+    for (int q = 0; q < kMlpK[l] * kMlpN[l]; ++q) {
+      const double a = 1.0 - u01(), b = u01();
+      p[kMlpWOff[l] + q] = (float)(sd * std::sqrt(-2.0 * std::log(a)) * std::cos(2.0 * M_PI * b));
+    }
+  }
+}
+
+int upload_params(cuhre_guide* g, const float* host)
+{
+  hipStream_t st = g->ctx->stream;
+  if (hipMemcpyAsync(g->params, host, sizeof(float) * kMlpParams, hipMemcpyHostToDevice, st) != hipSuccess)
+    return set_error(CUHRE_ERR_DEVICE, "parameter upload failed");
+  (void)hipMemsetAsync(g->m, 0, sizeof(float) * kMlpParams, st);
+  (void)hipMemsetAsync(g->v, 0, sizeof(float) * kMlpParams, st);
+  g->adam_step = 0;
+  hipLaunchKernelGGL(k_mlp_prepare, dim3((kMlpBOff[0] + 255) / 256), dim3(256), 0, st, g->params, g->W, g->WT);
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
+    return set_error(CUHRE_ERR_DEVICE, "parameter preparation failed");
+  return 0;
+}
+
+// Scratch for steps of `rows` rows and an order array of `order_len` entries.
+int reserve_train(cuhre_guide* g, int64_t rows, int64_t order_len)
+{
+  const int blocks = (int)((rows + kMlpRows - 1) / kMlpRows);
+  if (blocks > g->part_blocks) {
+    gfree(g->part);
+    if (hipMalloc(&g->part, sizeof(float) * kMlpParams * (size_t)blocks) != hipSuccess) {
+      g->part_blocks = 0;
+      return set_error(CUHRE_ERR_ALLOC, "cannot allocate gradient partials");
+    }
+    g->part_blocks = blocks;
+  }
+  const int64_t need = std::max(rows, order_len);
+  if (need > g->train_cap) {
+    gfree(g->order);
+    gfree(g->loss);
+    gfree(g->correct);
+    if (hipMalloc(&g->order, sizeof(int32_t) * need) != hipSuccess ||
+        hipMalloc(&g->loss, sizeof(float) * need) != hipSuccess || hipMalloc(&g->correct, need) != hipSuccess) {
+      g->train_cap = 0;
+      return set_error(CUHRE_ERR_ALLOC, "cannot allocate training buffers");
+    }
+    g->train_cap = need;
+  }
+  return 0;
+}
+
+// One optimisation step (or gradient only) over `rows` sample indices.
+void train_step(cuhre_guide* g, const float* feat, const int8_t* ax, const float* fin, const int32_t* order,
+                int64_t rows, int dim, float lr, bool update, hipStream_t st)
+{
+  TrainArgs T;
+  T.M = mlp_dev(g);
+  T.feat = feat;
+  T.axis = ax;
+  T.finish = fin;
+  T.order = order;
+  T.rows = rows;
+  T.dim = dim;
+  T.inv_batch = 1.0f / (float)rows;
+  T.part = g->part;
+  T.loss = g->loss;
+  T.correct = g->correct;
+  const int blocks = (int)((rows + kMlpRows - 1) / kMlpRows);
+  hipLaunchKernelGGL(k_mlp_train, dim3(blocks), dim3(256), 0, st, T);
+  if (update) ++g->adam_step;
+  hipLaunchKernelGGL(k_mlp_reduce_adam, dim3((kMlpParams + 255) / 256), dim3(256), 0, st, g->part, blocks, g->grad,
+                     g->params, g->m, g->v, std::max(g->adam_step, 1), lr, update ? 1 : 0, g->W, g->WT);
+}
+
+struct GuidedRun {
+  int axis_mode, finish_mode, use_mlp;
+  float axis_conf, finish_thr;
+  double budget;
+  int64_t collect_max;  // collection: samples wanted from this run (0: none)
+};
+
+// The plain loop of engine.hip plus the guide step between evaluation and
+// split; with collect_max > 0 it also logs samples and their labels.
+int guided_run(cuhre_context* c, cuhre_guide* g, const cuhre_integrand* f, int dim, double epsrel, double epsabs,
+               int64_t maxeval, const double* lows, const double* highs, const GuidedRun& run, cuhre_result* out,
+               cuhre_guide_stats* gs)
+{
+  const auto t0 = std::chrono::steady_clock::now();
+  std::memset(out, 0, sizeof *out);
+  if (gs) std::memset(gs, 0, sizeof *gs);
+  int rc = check_common(c, f, dim, 0);
+  if (rc) return rc;
+  if (dim > 8) return set_error(CUHRE_ERR_UNSUPPORTED, "the guide handles dimensions up to 8");
+  if (!lows || !highs) return set_error(CUHRE_ERR_ARG, "null bounds");
+  DevIntegrand P;
+  if ((rc = make_dev_integrand(c, f, dim, &P))) return rc;
+  const RuleConst R = make_rule(dim);
+  const int64_t p = R.npts;
+  std::vector<double> hlo, hwd;
+  const int64_t n0 = initial_grid(dim, lows, highs, hlo, hwd);
+  const int64_t capacity = list_capacity(c, dim);
+  if ((rc = ctx_reserve(c, std::min<int64_t>(capacity, std::max<int64_t>(4 * n0, 1 << 16)), dim))) return rc;
+  if ((rc = ctx_reserve_aux(c, c->cap, dim))) return rc;
+  if ((rc = guide_reserve_regions(g, c->cap))) return rc;
+  hipStream_t st = c->stream;
+  int cur = 0;
+  if (hipMemcpy2DAsync(c->lo[cur], sizeof(double) * c->cap, hlo.data(), sizeof(double) * n0, sizeof(double) * n0,
+                       dim, hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemcpy2DAsync(c->wd[cur], sizeof(double) * c->cap, hwd.data(), sizeof(double) * n0, sizeof(double) * n0,
+                       dim, hipMemcpyHostToDevice, st) != hipSuccess)
+    return set_error(CUHRE_ERR_DEVICE, "initial grid upload failed");
+  double vdom = 1.0;
+  for (int m = 0; m < dim; ++m) vdom *= highs[m] - lows[m];
+  (void)vdom;
+  const int64_t ds_base0 = g->ds_n;
+  double fin_s = 0, fin_c = 0, fer_s = 0, fer_c = 0, fin_abs = 0, fin_aest = 0;
+  int64_t nfin = 0, n = n0, logged = 0;
+  const double u = 0.5 * 2.220446049250313e-16;
+  int result = CUHRE_NOT_CONVERGED;
+  for (int it = 0; it < 1000; ++it) {
+    const int S = pick_segments(c, n);
+    EvalArgs A;
+    fill_eval_args(c, A, cur, n, dim, epsrel, epsabs);
+    A.finalize = 1;
+    A.i5 = c->aux;
+    A.dvec = c->aux + c->cap;
+    A.fin_est_s = fin_s;
+    A.fin_est_c = fin_c;
+    A.fin_err_s = fer_s;
+    A.fin_err_c = fer_c;
+    const int64_t left = maxeval - (out->neval + n * p);
+    A.max_active = std::min<int64_t>(left >= 0 ? left / (2 * p) : -1, capacity / 2);
+    A.alloc_active = c->cap / 2;
+    find_eval_launcher(f->family, dim)(A, P, R, S, st);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(c->h_state, c->state, sizeof(IterState), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return set_error(CUHRE_ERR_DEVICE, "guided evaluation failed");
+    IterState s = *c->h_state;
+    out->eval_launches++;
+    out->iterations = it + 1;
+    out->nregions += n;
+    out->neval += n * p;
+    out->peak_regions = std::max(out->peak_regions, n);
+    out->integral = s.rG_s + s.rG_c;
+    out->error = s.eG_s + s.eG_c;
+    if (s.stop == 5) {
+      // children would not fit: grow the lists (keeping this iteration) and
+      // the guide's per-region buffers
+      const int64_t ncap = std::min<int64_t>(capacity, std::max<int64_t>(2 * s.tot.nactive, 2 * c->cap));
+      const int64_t nb = blocks_for(n, S);
+      if ((rc = ctx_reserve(c, ncap, dim, n, cur, nb))) return rc;
+      // i5 / dvec were computed into the old aux block: re-evaluate this list
+      if ((rc = ctx_reserve_aux(c, c->cap, dim))) return rc;
+      if ((rc = guide_reserve_regions(g, c->cap))) return rc;
+      fill_eval_args(c, A, cur, n, dim, epsrel, epsabs);
+      A.finalize = 0;
+      A.i5 = c->aux;
+      A.dvec = c->aux + c->cap;
+      find_eval_launcher(f->family, dim)(A, P, R, S, st);
+      s.stop = 0;
+    }
+    const double abs_contrib = fin_abs + s.tot.abs_all;
+    const double aest_contrib = fin_aest + s.tot.aest_all;
+    out->fp64_err_bound = (double)(p + 4) * u * abs_contrib + 2.0 * u * std::fabs(out->integral) +
+                          (double)(nfin + n) * u * u * aest_contrib;
+    if (s.stop == 4) {
+      result = set_error(CUHRE_ERR_NONFINITE, "integrand returned a non-finite value");
+      break;
+    }
+    if (s.stop == 1) {
+      result = CUHRE_OK;
+      break;
+    }
+    if (s.stop == 2 || s.stop == 3) {
+      result = CUHRE_NOT_CONVERGED;
+      break;
+    }
+    // guide step
+    GuideArgs G;
+    std::memset(&G, 0, sizeof G);
+    G.lower = c->lo[cur];
+    G.width = c->wd[cur];
+    G.ld = c->cap;
+    G.n = n;
+    G.est = c->est;
+    G.err = c->err;
+    G.i5 = c->aux;
+    G.dvec = c->aux + c->cap;
+    G.abssum = c->abssum;
+    G.axis = c->axis;
+    G.retire = g->retire;
+    G.dim = dim;
+    for (int m = 0; m < dim; ++m) {
+      G.lows[m] = lows[m];
+      G.highs[m] = highs[m];
+    }
+    G.absI = std::fabs(out->integral);
+    G.epsrel = epsrel;
+    G.epsabs = epsabs;
+    G.axis_mode = run.axis_mode;
+    G.finish_mode = run.finish_mode;
+    G.use_mlp = run.use_mlp;
+    G.axis_conf = run.axis_conf;
+    G.finish_thr = run.finish_thr;
+    G.budget = run.budget;
+    G.M = mlp_dev(g);
+    G.tile_cnt = g->tile_cnt;
+    G.tile_logcnt = g->tile_logcnt;
+    G.tile_ret = g->tile_ret;
+    const int64_t want = run.collect_max - logged;
+    if (run.collect_max > 0 && want > 0 && s.tot.nactive > 0) {
+      // sample about `want / 2` regions of this iteration (later iterations are larger)
+      const int64_t target = std::max<int64_t>(1, std::min<int64_t>(want, std::max<int64_t>(want / 2, 4096)));
+      G.log_stride = (uint64_t)std::max<int64_t>(1, (s.tot.nactive + target - 1) / target);
+      G.log_salt = 0x5DEECE66Dull * (uint64_t)(it + 1);
+    }
+    const int64_t ntiles = (n + kMlpRows - 1) / kMlpRows;
+    hipLaunchKernelGGL(k_guide_apply, dim3((unsigned)ntiles), dim3(256), 0, st, G);
+    hipLaunchKernelGGL(k_guide_scan, dim3(1), dim3(1024), 0, st, g->tile_cnt, g->tile_logcnt, g->tile_ret, ntiles,
+                       g->tile_off, g->tile_logoff, g->d_sums);
+    SplitArgs SA;
+    fill_split_args(c, SA, cur, n, dim, epsrel, epsabs, kMlpRows, false);
+    SA.offsets = g->tile_off;
+    SA.retire = g->retire;
+    launch_split(SA, st);
+    if (hipMemcpyAsync(g->h_sums, g->d_sums, sizeof(GuideSums), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return set_error(CUHRE_ERR_DEVICE, "guide step failed");
+    const GuideSums gsum = *g->h_sums;
+    if (G.log_stride > 0 && gsum.n_logged > 0) {
+      const int64_t L = std::min<int64_t>(gsum.n_logged, run.collect_max - logged);
+      if ((rc = guide_reserve_dataset(g, g->ds_n + gsum.n_logged))) return rc;
+      if ((rc = guide_reserve_lookahead(g, gsum.n_logged * 2 * dim, dim))) return rc;
+      CollectArgs CA;
+      CA.G = G;
+      CA.logoff = g->tile_logoff;
+      CA.base = g->ds_n;
+      CA.feat = g->ds_feat;
+      CA.s_err = g->ds_err;
+      CA.s_vrel = g->ds_vrel;
+      CA.la_lo = g->la_lo;
+      CA.la_wd = g->la_wd;
+      CA.la_ld = g->la_cap;
+      hipLaunchKernelGGL(k_collect_write, dim3((unsigned)ntiles), dim3(64), 0, st, CA);
+      EvalArgs LA;
+      std::memset(&LA, 0, sizeof LA);
+      LA.lower = g->la_lo;
+      LA.width = g->la_wd;
+      LA.ld = g->la_cap;
+      LA.n = gsum.n_logged * 2 * dim;
+      LA.epsrel = epsrel;
+      LA.epsabs = epsabs;
+      LA.est = g->la_est;
+      LA.err = g->la_err;
+      LA.axis = g->la_axis;
+      LA.abssum = g->la_ab;
+      LA.finalize = 0;
+      find_eval_launcher(f->family, dim)(LA, P, R, pick_segments(c, LA.n), st);
+      hipLaunchKernelGGL(k_label_axis, dim3((unsigned)((gsum.n_logged + 255) / 256)), dim3(256), 0, st, g->la_err,
+                         gsum.n_logged, dim, g->ds_n, g->ds_axis);
+      if (hipGetLastError() != hipSuccess) return set_error(CUHRE_ERR_DEVICE, "sample collection failed");
+      g->ds_n += L;  // keep at most collect_max (the extra rows are overwritten later)
+      logged += L;
+    }
+    if (gs) {
+      gs->retired += gsum.n_retired;
+      gs->guide_iterations += 1;
+    }
+    out->split_launches++;
+    dd_add_host(fin_s, fin_c, s.tot.fest_s, s.tot.fest_c);
+    dd_add_host(fer_s, fer_c, s.tot.ferr_s, s.tot.ferr_c);
+    dd_add_host(fin_s, fin_c, gsum.est_s, gsum.est_c);
+    dd_add_host(fer_s, fer_c, gsum.err_s, gsum.err_c);
+    fin_abs += s.tot.abs_fin + gsum.abs_r;
+    fin_aest += s.tot.aest_fin + gsum.aest_r;
+    nfin += n - gsum.n_split;
+    out->nfinished = nfin;
+    if (gsum.n_split == 0) {
+      result = CUHRE_NOT_CONVERGED;
+      break;
+    }
+    n = 2 * gsum.n_split;
+    cur = 1 - cur;
+  }
+  if (result >= 0 && g->ds_n > ds_base0) {
+    // finish labels of this run's samples against its final estimate
+    const double tol = epsrel * std::fabs(out->integral);
+    hipLaunchKernelGGL(k_label_finish, dim3((unsigned)((g->ds_n - ds_base0 + 255) / 256)), dim3(256), 0, st,
+                       g->ds_err, g->ds_vrel, ds_base0, g->ds_n, tol, g->ds_fin);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
+      return set_error(CUHRE_ERR_DEVICE, "finish labelling failed");
+  }
+  if (gs) gs->samples = g->ds_n - ds_base0;
+  out->fail = result > 0 ? result : 0;
+  out->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return result;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cuhre_guide_create(cuhre_context* ctx, uint64_t seed, cuhre_guide** out)
+{
+  if (!ctx || !out) return set_error(CUHRE_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (hipSetDevice(ctx->device) != hipSuccess) return set_error(CUHRE_ERR_DEVICE, "hipSetDevice failed");
+  cuhre_guide* g = new cuhre_guide();
+  g->ctx = ctx;
+  if (hipMalloc(&g->params, sizeof(float) * kMlpParams) != hipSuccess ||
+      hipMalloc(&g->m, sizeof(float) * kMlpParams) != hipSuccess ||
+      hipMalloc(&g->v, sizeof(float) * kMlpParams) != hipSuccess ||
+      hipMalloc(&g->grad, sizeof(float) * kMlpParams) != hipSuccess ||
+      hipMalloc(&g->W, sizeof(uint16_t) * kMlpBOff[0]) != hipSuccess ||
+      hipMalloc(&g->WT, sizeof(uint16_t) * kMlpBOff[0]) != hipSuccess ||
+      hipMalloc(&g->d_sums, sizeof(GuideSums)) != hipSuccess ||
+      hipMalloc(&g->d_stats, sizeof(double) * 4) != hipSuccess ||
+      hipHostMalloc(&g->h_sums, sizeof(GuideSums), hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    cuhre_guide_destroy(g);
+    return set_error(CUHRE_ERR_ALLOC, "cannot allocate the guide");
+  }
+  std::vector<float> p;
+  init_params(p, seed);
+  const int rc = upload_params(g, p.data());
+  if (rc) {
+    cuhre_guide_destroy(g);
+    return rc;
+  }
+  *out = g;
+  return 0;
+}
+
+int cuhre_guide_destroy(cuhre_guide* g)
+{
+  if (!g) return 0;
+  if (g->ctx) (void)hipStreamSynchronize(g->ctx->stream);
+  for (void* q : {(void*)g->params, (void*)g->m, (void*)g->v, (void*)g->W, (void*)g->WT, (void*)g->grad,
+                  (void*)g->part, (void*)g->ds_feat, (void*)g->ds_axis, (void*)g->ds_fin, (void*)g->ds_err,
+                  (void*)g->ds_vrel, (void*)g->la_lo, (void*)g->la_wd, (void*)g->la_est, (void*)g->la_err,
+                  (void*)g->la_ab, (void*)g->la_axis, (void*)g->retire, (void*)g->tile_cnt, (void*)g->tile_off,
+                  (void*)g->tile_logcnt, (void*)g->tile_logoff, (void*)g->tile_ret, (void*)g->d_sums,
+                  (void*)g->order, (void*)g->loss, (void*)g->correct, (void*)g->d_stats})
+    gfree(q);
+  if (g->h_sums) (void)hipHostFree(g->h_sums);
+  delete g;
+  return 0;
+}
+
+int cuhre_guide_set_params(cuhre_guide* g, const float* params)
+{
+  if (!g || !params) return set_error(CUHRE_ERR_ARG, "null argument");
+  return upload_params(g, params);
+}
+
+int cuhre_guide_get_params(cuhre_guide* g, float* params)
+{
+  if (!g || !params) return set_error(CUHRE_ERR_ARG, "null argument");
+  if (hipMemcpy(params, g->params, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost) != hipSuccess)
+    return set_error(CUHRE_ERR_DEVICE, "parameter download failed");
+  return 0;
+}
+
+int cuhre_guide_forward(cuhre_guide* g, int64_t n, const float* features, float* logits)
+{
+  if (!g || !features || !logits) return set_error(CUHRE_ERR_ARG, "null argument");
+  if (n <= 0) return 0;
+  hipStream_t st = g->ctx->stream;
+  float *dx, *dy;
+  if (hipMalloc(&dx, sizeof(float) * 32 * n) != hipSuccess || hipMalloc(&dy, sizeof(float) * 16 * n) != hipSuccess)
+    return set_error(CUHRE_ERR_ALLOC, "cannot allocate forward buffers");
+  int rc = 0;
+  if (hipMemcpyAsync(dx, features, sizeof(float) * 32 * n, hipMemcpyHostToDevice, st) != hipSuccess) rc = 1;
+  hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)((n + kMlpRows - 1) / kMlpRows)), dim3(256), 0, st, mlp_dev(g),
+                     dx, n, dy);
+  if (hipGetLastError() != hipSuccess ||
+      hipMemcpyAsync(logits, dy, sizeof(float) * 16 * n, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    rc = 1;
+  (void)hipFree(dx);
+  (void)hipFree(dy);
+  return rc ? set_error(CUHRE_ERR_DEVICE, "guide forward failed") : 0;
+}
+
+int cuhre_guide_gradient(cuhre_guide* g, int64_t n, const float* features, const int32_t* axis,
+                         const float* finish, int dim, float* grad, double* loss)
+{
+  if (!g || !features || !axis || !finish || !grad) return set_error(CUHRE_ERR_ARG, "null argument");
+  if (n <= 0 || dim < 2 || dim > 8) return set_error(CUHRE_ERR_ARG, "bad size or dimension");
+  hipStream_t st = g->ctx->stream;
+  int rc = reserve_train(g, n, n);
+  if (rc) return rc;
+  float *dx, *df;
+  int8_t* da;
+  std::vector<int8_t> ax8(n);
+  std::vector<int32_t> ord(n);
+  for (int64_t i = 0; i < n; ++i) {
+    if (axis[i] < 0 || axis[i] >= dim) return set_error(CUHRE_ERR_ARG, "axis label out of range");
+    ax8[i] = (int8_t)axis[i];
+    ord[i] = (int32_t)i;
+  }
+  if (hipMalloc(&dx, sizeof(float) * 32 * n) != hipSuccess || hipMalloc(&df, sizeof(float) * n) != hipSuccess ||
+      hipMalloc(&da, n) != hipSuccess)
+    return set_error(CUHRE_ERR_ALLOC, "cannot allocate gradient inputs");
+  bool ok = hipMemcpyAsync(dx, features, sizeof(float) * 32 * n, hipMemcpyHostToDevice, st) == hipSuccess &&
+            hipMemcpyAsync(df, finish, sizeof(float) * n, hipMemcpyHostToDevice, st) == hipSuccess &&
+            hipMemcpyAsync(da, ax8.data(), n, hipMemcpyHostToDevice, st) == hipSuccess &&
+            hipMemcpyAsync(g->order, ord.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, st) == hipSuccess;
+  if (ok) {
+    train_step(g, dx, da, df, g->order, n, dim, 0.f, false, st);
+    std::vector<float> lh(n);
+    ok = hipGetLastError() == hipSuccess &&
+         hipMemcpyAsync(grad, g->grad, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, st) == hipSuccess &&
+         hipMemcpyAsync(lh.data(), g->loss, sizeof(float) * n, hipMemcpyDeviceToHost, st) == hipSuccess &&
+         hipStreamSynchronize(st) == hipSuccess;
+    if (ok && loss) {
+      double s = 0;
+      for (float v : lh) s += v;
+      *loss = s / (double)n;
+    }
+  }
+  (void)hipFree(dx);
+  (void)hipFree(df);
+  (void)hipFree(da);
+  return ok ? 0 : set_error(CUHRE_ERR_DEVICE, "guide gradient failed");
+}
+
+int64_t cuhre_guide_samples(const cuhre_guide* g) { return g ? g->ds_n : -1; }
+
+int cuhre_guide_get_samples(cuhre_guide* g, int64_t first, int64_t count, float* features, int32_t* axis,
+                            float* finish)
+{
+  if (!g || first < 0 || count < 0 || first + count > g->ds_n) return set_error(CUHRE_ERR_ARG, "bad sample range");
+  if (count == 0) return 0;
+  std::vector<int8_t> a8(count);
+  if ((features && hipMemcpy(features, g->ds_feat + first * 32, sizeof(float) * 32 * count,
+                             hipMemcpyDeviceToHost) != hipSuccess) ||
+      hipMemcpy(a8.data(), g->ds_axis + first, count, hipMemcpyDeviceToHost) != hipSuccess ||
+      (finish && hipMemcpy(finish, g->ds_fin + first, sizeof(float) * count, hipMemcpyDeviceToHost) != hipSuccess))
+    return set_error(CUHRE_ERR_DEVICE, "sample download failed");
+  if (axis)
+    for (int64_t i = 0; i < count; ++i) axis[i] = a8[i];
+  return 0;
+}
+
+int cuhre_guide_collect(cuhre_guide* g, const cuhre_integrand* f, int dim, double epsrel, double epsabs,
+                        int64_t maxeval, const double* lows, const double* highs, int64_t max_samples,
+                        cuhre_result* out)
+{
+  if (!g || !out) return set_error(CUHRE_ERR_ARG, "null argument");
+  GuidedRun run{0, 0, 0, 1.f, 1.f, 0.0, max_samples};
+  cuhre_guide_stats gs;
+  return guided_run(g->ctx, g, f, dim, epsrel, epsabs, maxeval, lows, highs, run, out, &gs);
+}
+
+int cuhre_guide_train(cuhre_guide* g, int dim, int epochs, int batch, float lr, uint64_t seed,
+                      cuhre_train_stats* stats)
+{
+  if (!g || !stats) return set_error(CUHRE_ERR_ARG, "null argument");
+  std::memset(stats, 0, sizeof *stats);
+  if (dim < 2 || dim > 8 || epochs <= 0 || batch < kMlpRows) return set_error(CUHRE_ERR_ARG, "bad training settings");
+  const int64_t n = g->ds_n;
+  if (n < batch) return set_error(CUHRE_ERR_ARG, "fewer samples than one batch");
+  hipStream_t st = g->ctx->stream;
+  int rc = reserve_train(g, batch, n);
+  if (rc) return rc;
+  std::vector<int32_t> ord(n);
+  uint64_t sm = seed;
+  auto next = [&]() {
+    sm += 0x9E3779B97F4A7C15ull;
+    uint64_t z = sm;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  };
+  const int64_t steps = n / batch;
+  const auto t0 = std::chrono::steady_clock::now();
+  for (int ep = 0; ep < epochs; ++ep) {
+    for (int64_t i = 0; i < n; ++i) ord[i] = (int32_t)i;
+    for (int64_t i = n - 1; i > 0; --i) std::swap(ord[i], ord[next() % (uint64_t)(i + 1)]);
+    if (hipMemcpyAsync(g->order, ord.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, st) != hipSuccess)
+      return set_error(CUHRE_ERR_DEVICE, "order upload failed");
+    double acc_h[4] = {0, 0, 0, 0};
+    (void)hipMemsetAsync(g->d_stats, 0, sizeof(double) * 4, st);
+    for (int64_t k = 0; k < steps; ++k) {
+      train_step(g, g->ds_feat, g->ds_axis, g->ds_fin, g->order + k * batch, batch, dim, lr, true, st);
+      hipLaunchKernelGGL(k_train_stats, dim3(16), dim3(256), 0, st, g->loss, g->correct, (int64_t)batch, g->d_stats);
+    }
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(acc_h, g->d_stats, sizeof acc_h, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return set_error(CUHRE_ERR_DEVICE, "training step failed");
+    const double rows = (double)(steps * batch);
+    if (ep == 0) stats->loss_first = acc_h[0] / rows;
+    stats->loss_last = acc_h[0] / rows;
+    stats->axis_accuracy = acc_h[1] / rows;
+    stats->finish_accuracy = acc_h[2] / rows;
+  }
+  stats->epochs = epochs;
+  stats->steps = (int32_t)(steps * epochs);
+  stats->samples = n;
+  stats->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return 0;
+}
+
+int cuhre_integrate_guided(cuhre_context* ctx, cuhre_guide* g, const cuhre_integrand* f, int dim, double epsrel,
+                           double epsabs, int64_t maxeval, const double* lows, const double* highs,
+                           const cuhre_guide_policy* pol, cuhre_result* out, cuhre_guide_stats* stats)
+{
+  if (!ctx || !g || !out) return set_error(CUHRE_ERR_ARG, "null argument");
+  if (g->ctx != ctx) return set_error(CUHRE_ERR_ARG, "the guide belongs to another context");
+  cuhre_guide_policy d = {1, 1, 0.6f, 0.5f, 0.25};
+  const cuhre_guide_policy& P = pol ? *pol : d;
+  if (P.axis_mode < 0 || P.axis_mode > 1 || P.finish_mode < 0 || P.finish_mode > 2 || !(P.budget >= 0.0) ||
+      P.budget > 1.0)
+    return set_error(CUHRE_ERR_ARG, "bad guide policy");
+  GuidedRun run{P.axis_mode, P.finish_mode, (P.axis_mode == 1 || P.finish_mode == 1) ? 1 : 0, P.axis_confidence,
+                P.finish_threshold, P.budget, 0};
+  return guided_run(ctx, g, f, dim, epsrel, epsabs, maxeval, lows, highs, run, out, stats);
+}
+
+}  // extern "C"

@@ -1,0 +1,117 @@
+// mlp.hpp -- the learned split guide's network on the MFMA units.
+//
+// 32 -> 64 -> 64 -> 64 -> 16, ReLU.  BF16 weights and BF16 activations enter
+// v_mfma_f32_16x16x16_bf16 (FP32 accumulation).  One wavefront owns 16 rows
+// (regions / samples); a 256-thread block holds 64 rows and their
+// activations in LDS.  Output columns 0..d-1 are split-axis logits, column 15
+// the finish logit.
+//
+// Operand layout of v_mfma_f32_16x16x16_bf16 (lane l, item v = 0..3):
+//   A[i][k]: i = l % 16, k = 4*(l/16) + v
+//   B[k][j]: j = l % 16, k = 4*(l/16) + v
+//   D[i][j]: j = l % 16, i = 4*(l/16) + v
+#pragma once
+
+#include "common.hpp"
+
+namespace cuhre {
+
+constexpr int kMlpIn = 32, kMlpHid = 64, kMlpOut = 16, kMlpFinish = 15;
+constexpr int kMlpRows = 64;  // rows per 256-thread block
+// parameter vector: W0 [32x64], W1 [64x64], W2 [64x64], W3 [64x16], b0, b1, b2, b3
+constexpr int kMlpK[4] = {32, 64, 64, 64};
+constexpr int kMlpN[4] = {64, 64, 64, 16};
+constexpr int kMlpWOff[4] = {0, 2048, 6144, 10240};
+constexpr int kMlpBOff[4] = {11264, 11328, 11392, 11456};
+constexpr int kMlpParams = 11472;
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t f2bf(float f)
+{
+  const uint32_t u = __float_as_uint(f);
+  return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// Device view of the weights: W[l] row-major [K][N] and WT[l] = W^T [N][K], both BF16.
+struct MlpDev {
+  const uint16_t* W;   // concatenated at kMlpWOff
+  const uint16_t* WT;  // concatenated at kMlpWOff
+  const float* b;      // biases at kMlpBOff - kMlpBOff[0]
+};
+
+__device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c)
+{
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// Y[16 x N] = X[16 x K] . W for this wave's 16 rows; X in LDS (row stride K,
+// BF16), W read from WT ([N][K]) so each B operand is 8 contiguous bytes.
+template <int K, int N>
+__device__ __forceinline__ void mlp_layer_fwd(const uint16_t* Xs, const uint16_t* WT, f32x4 (&acc)[N / 16])
+{
+  const int l = __lane_id(), row = l & 15, kq = (l >> 4) * 4;
+#pragma unroll
+  for (int nt = 0; nt < N / 16; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int kt = 0; kt < K / 16; ++kt) {
+    const bf16x4 a = *reinterpret_cast<const bf16x4*>(Xs + row * K + kt * 16 + kq);
+#pragma unroll
+    for (int nt = 0; nt < N / 16; ++nt) {
+      const bf16x4 b = *reinterpret_cast<const bf16x4*>(WT + (nt * 16 + row) * K + kt * 16 + kq);
+      acc[nt] = mfma16(a, b, acc[nt]);
+    }
+  }
+}
+
+// Bias + ReLU + BF16 store of a layer's output tile into LDS [16][N].
+template <int N>
+__device__ __forceinline__ void mlp_store_relu(const f32x4 (&acc)[N / 16], const float* bias, uint16_t* Hs)
+{
+  const int l = __lane_id(), col = l & 15, r0 = (l >> 4) * 4;
+#pragma unroll
+  for (int nt = 0; nt < N / 16; ++nt)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const float z = acc[nt][v] + bias[nt * 16 + col];
+      Hs[(r0 + v) * N + nt * 16 + col] = f2bf(z > 0.f ? z : 0.f);
+    }
+}
+
+// Forward pass of this wave's 16 rows.  Xs..H3s are this wave's LDS rows;
+// Ys receives the 16 x 16 logits in FP32.  Every wave of the block calls it
+// together (block barriers between layers).
+__device__ __forceinline__ void mlp_forward16(const MlpDev& M, const uint16_t* Xs, uint16_t* H1s, uint16_t* H2s,
+                                              uint16_t* H3s, float* Ys)
+{
+  {
+    f32x4 acc[4];
+    mlp_layer_fwd<32, 64>(Xs, M.WT + kMlpWOff[0], acc);
+    mlp_store_relu<64>(acc, M.b + 0, H1s);
+  }
+  __syncthreads();
+  {
+    f32x4 acc[4];
+    mlp_layer_fwd<64, 64>(H1s, M.WT + kMlpWOff[1], acc);
+    mlp_store_relu<64>(acc, M.b + 64, H2s);
+  }
+  __syncthreads();
+  {
+    f32x4 acc[4];
+    mlp_layer_fwd<64, 64>(H2s, M.WT + kMlpWOff[2], acc);
+    mlp_store_relu<64>(acc, M.b + 128, H3s);
+  }
+  __syncthreads();
+  {
+    f32x4 acc[1];
+    mlp_layer_fwd<64, 16>(H3s, M.WT + kMlpWOff[3], acc);
+    const int l = __lane_id(), col = l & 15, r0 = (l >> 4) * 4;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) Ys[(r0 + v) * 16 + col] = acc[0][v] + M.b[192 + col];
+  }
+  __syncthreads();
+}
+
+}  // namespace cuhre

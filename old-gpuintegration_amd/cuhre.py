@@ -33,7 +33,11 @@ EXPORTS = [
     "cuhre_context_create", "cuhre_context_destroy", "cuhre_context_set",
     "cuhre_integrate", "cuhre_eval_regions", "cuhre_split_regions", "cuhre_sum",
     "cuhre_last_error", "cuhre_points_per_region", "cuhre_integrate_batch", "cuhre_batch_autotune",
+    "cuhre_guide_create", "cuhre_guide_destroy", "cuhre_guide_set_params", "cuhre_guide_get_params",
+    "cuhre_guide_forward", "cuhre_guide_gradient", "cuhre_guide_collect", "cuhre_guide_samples",
+    "cuhre_guide_get_samples", "cuhre_guide_train", "cuhre_integrate_guided",
 ]
+GUIDE_PARAMS = 11472
 
 
 class Integrand(C.Structure):
@@ -115,6 +119,44 @@ class BatchInfo(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class GuidePolicy(C.Structure):
+    _fields_ = [
+        ("axis_mode", C.c_int32),
+        ("finish_mode", C.c_int32),
+        ("axis_confidence", C.c_float),
+        ("finish_threshold", C.c_float),
+        ("budget", C.c_double),
+    ]
+
+
+class GuideStats(C.Structure):
+    _fields_ = [
+        ("retired", C.c_int64),
+        ("samples", C.c_int64),
+        ("guide_iterations", C.c_int32),
+        ("pad", C.c_int32),
+    ]
+
+    def asdict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class TrainStats(C.Structure):
+    _fields_ = [
+        ("loss_first", C.c_double),
+        ("loss_last", C.c_double),
+        ("axis_accuracy", C.c_double),
+        ("finish_accuracy", C.c_double),
+        ("epochs", C.c_int32),
+        ("steps", C.c_int32),
+        ("samples", C.c_int64),
+        ("ms", C.c_double),
+    ]
+
+    def asdict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class CuhreError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"cuhre error {code}: {msg}")
@@ -149,6 +191,25 @@ def lib():
                                             C.POINTER(BatchInfo)]
         L.cuhre_batch_autotune.argtypes = [ctxp, C.c_int32, C.POINTER(Integrand), C.c_int, dp, dp,
                                            C.POINTER(BatchConfig), dp]
+        fp = C.POINTER(C.c_float)
+        i32p = C.POINTER(C.c_int32)
+        L.cuhre_guide_create.argtypes = [ctxp, C.c_uint64, C.POINTER(C.c_void_p)]
+        L.cuhre_guide_destroy.argtypes = [C.c_void_p]
+        L.cuhre_guide_set_params.argtypes = [C.c_void_p, fp]
+        L.cuhre_guide_get_params.argtypes = [C.c_void_p, fp]
+        L.cuhre_guide_forward.argtypes = [C.c_void_p, C.c_int64, fp, fp]
+        L.cuhre_guide_gradient.argtypes = [C.c_void_p, C.c_int64, fp, i32p, fp, C.c_int, fp,
+                                           C.POINTER(C.c_double)]
+        L.cuhre_guide_collect.argtypes = [C.c_void_p, C.POINTER(Integrand), C.c_int, C.c_double, C.c_double,
+                                          C.c_int64, dp, dp, C.c_int64, C.POINTER(Result)]
+        L.cuhre_guide_samples.argtypes = [C.c_void_p]
+        L.cuhre_guide_samples.restype = C.c_int64
+        L.cuhre_guide_get_samples.argtypes = [C.c_void_p, C.c_int64, C.c_int64, fp, i32p, fp]
+        L.cuhre_guide_train.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_uint64,
+                                        C.POINTER(TrainStats)]
+        L.cuhre_integrate_guided.argtypes = [ctxp, C.c_void_p, C.POINTER(Integrand), C.c_int, C.c_double,
+                                             C.c_double, C.c_int64, dp, dp, C.POINTER(GuidePolicy),
+                                             C.POINTER(Result), C.POINTER(GuideStats)]
         L.cuhre_last_error.restype = C.c_char_p
         L.cuhre_points_per_region.argtypes = [C.c_int, C.c_int]
         L.cuhre_points_per_region.restype = C.c_int64
@@ -316,6 +377,106 @@ class Context:
         out = C.c_double()
         _check(lib().cuhre_sum(self.h, x.size, _dp(x), C.byref(out)))
         return out.value
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Guide:
+    """Learned split guide (BF16 MLP on MFMA) attached to a Context."""
+
+    def __init__(self, ctx: Context, seed: int = 1):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        _check(lib().cuhre_guide_create(ctx.h, seed, C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().cuhre_guide_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        p = np.empty(GUIDE_PARAMS, dtype=np.float32)
+        _check(lib().cuhre_guide_get_params(self.h, _fp(p)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == GUIDE_PARAMS
+        _check(lib().cuhre_guide_set_params(self.h, _fp(p)))
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        n = X.shape[0]
+        Y = np.empty((n, 16), dtype=np.float32)
+        _check(lib().cuhre_guide_forward(self.h, n, _fp(X), _fp(Y)))
+        return Y
+
+    def gradient(self, X, axis, finish, dim):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        a = np.ascontiguousarray(axis, dtype=np.int32)
+        f = np.ascontiguousarray(finish, dtype=np.float32)
+        g = np.empty(GUIDE_PARAMS, dtype=np.float32)
+        loss = C.c_double()
+        _check(lib().cuhre_guide_gradient(self.h, X.shape[0], _fp(X), a.ctypes.data_as(C.POINTER(C.c_int32)),
+                                          _fp(f), dim, _fp(g), C.byref(loss)))
+        return g, loss.value
+
+    def collect(self, p: W.Problem, max_samples: int, maxeval=None):
+        s, keep = to_struct(p.f)
+        lo = np.ascontiguousarray(p.lows, dtype=np.float64)
+        hi = np.ascontiguousarray(p.highs, dtype=np.float64)
+        res = Result()
+        rc = lib().cuhre_guide_collect(self.h, C.byref(s), p.f.dim, p.epsrel, p.epsabs,
+                                       p.maxeval if maxeval is None else maxeval, _dp(lo), _dp(hi),
+                                       max_samples, C.byref(res))
+        _check(rc)
+        d = res.asdict()
+        d["rc"] = rc
+        return d
+
+    def samples(self):
+        return lib().cuhre_guide_samples(self.h)
+
+    def get_samples(self, first=0, count=None):
+        n = self.samples()
+        count = n - first if count is None else count
+        X = np.empty((count, 32), dtype=np.float32)
+        a = np.empty(count, dtype=np.int32)
+        f = np.empty(count, dtype=np.float32)
+        _check(lib().cuhre_guide_get_samples(self.h, first, count, _fp(X), a.ctypes.data_as(C.POINTER(C.c_int32)),
+                                             _fp(f)))
+        return X, a, f
+
+    def train(self, dim, epochs=4, batch=4096, lr=1e-3, seed=7):
+        st = TrainStats()
+        _check(lib().cuhre_guide_train(self.h, dim, epochs, batch, lr, seed, C.byref(st)))
+        return st.asdict()
+
+    def integrate(self, p: W.Problem, axis_mode=1, finish_mode=1, axis_confidence=0.6, finish_threshold=0.5,
+                  budget=0.25, maxeval=None, epsrel=None):
+        s, keep = to_struct(p.f)
+        lo = np.ascontiguousarray(p.lows, dtype=np.float64)
+        hi = np.ascontiguousarray(p.highs, dtype=np.float64)
+        pol = GuidePolicy(axis_mode, finish_mode, axis_confidence, finish_threshold, budget)
+        res = Result()
+        gst = GuideStats()
+        rc = lib().cuhre_integrate_guided(self.ctx.h, self.h, C.byref(s), p.f.dim,
+                                          p.epsrel if epsrel is None else epsrel, p.epsabs,
+                                          p.maxeval if maxeval is None else maxeval, _dp(lo), _dp(hi),
+                                          C.byref(pol), C.byref(res), C.byref(gst))
+        _check(rc)
+        d = res.asdict()
+        d["rc"] = rc
+        d["guide"] = gst.asdict()
+        return d
 
 
 def points_per_region(dim: int, key: int = 0) -> int:

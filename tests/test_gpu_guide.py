@@ -1,0 +1,131 @@
+"""Learned split guide on the GPU: MFMA forward/backward against the CPU
+oracle (same BF16 rounding points, FP32 vs float64 accumulation), training,
+sample collection, and the guided loop's guarantees."""
+import numpy as np
+import pytest
+
+import cuhre
+import mlp_oracle as M
+import oracle as O
+import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+WOFF = [0, 2048, 6144, 10240]
+BOFF = [11264, 11328, 11392, 11456]
+
+
+def to_vec(Ws, bs):
+    p = np.zeros(cuhre.GUIDE_PARAMS, dtype=np.float32)
+    for l in range(4):
+        p[WOFF[l]:WOFF[l] + Ws[l].size] = Ws[l].ravel()
+        p[BOFF[l]:BOFF[l] + bs[l].size] = bs[l]
+    return p
+
+
+def from_vec(p):
+    Ws, bs = [], []
+    for l, (k, n) in enumerate(M.SHAPES):
+        Ws.append(p[WOFF[l]:WOFF[l] + k * n].reshape(k, n).astype(np.float64))
+        bs.append(p[BOFF[l]:BOFF[l] + n].astype(np.float64))
+    return Ws, bs
+
+
+def random_setup(seed, n):
+    rng = np.random.default_rng(seed)
+    Ws, bs = M.init_params(seed)
+    bs = [rng.standard_normal(b.shape) * 0.1 for b in bs]
+    p = to_vec(Ws, bs)
+    Ws, bs = from_vec(p)  # the float32 values the GPU holds
+    X = rng.standard_normal((n, 32)).astype(np.float32)
+    X[:, 28:] = 0
+    return p, Ws, bs, X, rng
+
+
+def test_forward_matches_oracle(ctx):
+    g = cuhre.Guide(ctx, seed=1)
+    p, Ws, bs, X, _ = random_setup(5, 1000)
+    g.set_params(p)
+    Yg = g.forward(X)
+    _, Yo = M.forward(Ws, bs, X.astype(np.float64))
+    scale = np.abs(Yo).max()
+    assert np.max(np.abs(Yg - Yo)) <= 2e-2 * scale
+    assert np.median(np.abs(Yg - Yo) / (np.abs(Yo) + 1e-3 * scale)) <= 2e-3
+
+
+def test_gradient_matches_oracle(ctx):
+    g = cuhre.Guide(ctx, seed=1)
+    p, Ws, bs, X, rng = random_setup(9, 1024)
+    g.set_params(p)
+    d = 7
+    axis = rng.integers(0, d, X.shape[0]).astype(np.int32)
+    fin = (rng.random(X.shape[0]) < 0.3).astype(np.float32)
+    grad, loss = g.gradient(X, axis, fin, d)
+    Lo, gW, gb, _ = M.grads(Ws, bs, X.astype(np.float64), axis, fin.astype(np.float64), d)
+    assert abs(loss - Lo) <= 1e-3 * abs(Lo)
+    go = to_vec(gW, gb).astype(np.float64)
+    for l in range(4):
+        a = grad[WOFF[l]:WOFF[l] + Ws[l].size].astype(np.float64)
+        b = go[WOFF[l]:WOFF[l] + Ws[l].size]
+        rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+        cos = a @ b / (np.linalg.norm(a) * np.linalg.norm(b))
+        assert rel <= 3e-2 and cos >= 0.999, (l, rel, cos)
+        a = grad[BOFF[l]:BOFF[l] + bs[l].size].astype(np.float64)
+        b = go[BOFF[l]:BOFF[l] + bs[l].size]
+        assert np.linalg.norm(a - b) <= 3e-2 * np.linalg.norm(b) + 1e-7, l
+
+
+def test_guided_without_guide_is_the_plain_method(ctx):
+    g = cuhre.Guide(ctx, seed=2)
+    for p in (W.c1_gaussian3(), W.c5_corner7()):
+        a = g.integrate(p, axis_mode=0, finish_mode=0, maxeval=30_000_000)
+        b = ctx.integrate(p, maxeval=30_000_000)
+        for k in ("nregions", "iterations", "integral", "error", "rc"):
+            assert a[k] == b[k], (p.name, k, a[k], b[k])
+        assert a["guide"]["retired"] == 0
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_guided_finish_keeps_the_certificate(ctx, mode):
+    g = cuhre.Guide(ctx, seed=3)  # untrained network: proposals are arbitrary
+    for p in (W.c1_gaussian3(), W.Problem("corner5", W.corner_peak([0.7, 1.5, 2.0, 3.0, 4.0], 6),
+                                          [0.0] * 5, [1.0] * 5, 1e-6, 0.0, 10**9)):
+        r = g.integrate(p, axis_mode=1, finish_mode=mode, axis_confidence=0.0, budget=0.25)
+        ex, _ = O.exact(p.f, p.lows, p.highs)
+        assert r["rc"] == 0, r
+        assert r["error"] <= p.epsrel * abs(r["integral"])
+        assert abs(r["integral"] - ex) <= r["error"] + 1e-12 * abs(ex)
+        plain = ctx.integrate(p)
+        if mode == 2:
+            assert r["guide"]["retired"] > 0
+            assert r["nregions"] <= plain["nregions"]
+
+
+def test_collection_labels(ctx):
+    g = cuhre.Guide(ctx, seed=4)
+    # only x_2 varies (degree 6): the look-ahead axis is 2 for every region
+    f = W.polynomial(3, [(1.0, [0, 0, 6])])
+    p = W.Problem("x2^6", f, [0.0] * 3, [1.0] * 3, 1e-9, 0.0, 10**7)
+    r = g.collect(p, max_samples=5000)
+    n = g.samples()
+    assert 0 < n <= 5000
+    X, a, fin = g.get_samples()
+    assert np.all(a == 2)
+    assert set(np.unique(fin)) <= {0.0, 1.0}
+    assert np.all(np.abs(X[:, 28:]) == 0)
+    # fourth differences: only axis 2 is non-zero, normalised to 1
+    assert np.allclose(X[:, 18], 1.0) and np.allclose(X[:, 16], 0.0) and np.allclose(X[:, 17], 0.0)
+    assert r["nregions"] == ctx.integrate(p)["nregions"]
+
+
+def test_training_learns_a_rule(ctx):
+    g = cuhre.Guide(ctx, seed=5)
+    f = W.corner_peak([0.7, 1.5, 2.0, 3.0, 4.5, 1.1, 2.2], 8)
+    p = W.Problem("corner7small", f, [0.0] * 7, [1.0] * 7, 1e-6, 0.0, 10**9)
+    g.collect(p, max_samples=200_000, maxeval=5 * 10**8)
+    n = g.samples()
+    assert n >= 50_000
+    st = g.train(7, epochs=6, batch=4096, lr=2e-3, seed=1)
+    assert st["loss_last"] < st["loss_first"]
+    assert st["finish_accuracy"] >= 0.8
+    assert st["axis_accuracy"] >= 0.5
