@@ -427,11 +427,11 @@ __global__ __launch_bounds__(256) void k_guide_apply(GuideArgs G)
 
 // One block: exclusive scans of the split / logged counts per tile and the
 // retired sums in tile order.
-__global__ __launch_bounds__(1024) void k_guide_scan(const int64_t* cnt, const int64_t* logcnt, const double* ret,
-                                                     int64_t ntiles, int64_t* off, int64_t* logoff, GuideSums* out)
+__global__ __launch_bounds__(256) void k_guide_scan(const int64_t* cnt, const int64_t* logcnt, const double* ret,
+                                                    int64_t ntiles, int64_t* off, int64_t* logoff, GuideSums* out)
 {
-  __shared__ int64_t sa[1024], sb[1024];
-  __shared__ double sd[1024][7];
+  __shared__ int64_t sa[256], sb[256];
+  __shared__ double sd[256][7];
   const int T = blockDim.x;
   const int64_t chunk = (ntiles + T - 1) / T;
   const int64_t t0 = threadIdx.x * chunk;
@@ -957,7 +957,7 @@ int guided_run(cuhre_context* c, cuhre_guide* g, const cuhre_integrand* f, int d
     }
     const int64_t ntiles = (n + kMlpRows - 1) / kMlpRows;
     hipLaunchKernelGGL(k_guide_apply, dim3((unsigned)ntiles), dim3(256), 0, st, G);
-    hipLaunchKernelGGL(k_guide_scan, dim3(1), dim3(1024), 0, st, g->tile_cnt, g->tile_logcnt, g->tile_ret, ntiles,
+    hipLaunchKernelGGL(k_guide_scan, dim3(1), dim3(256), 0, st, g->tile_cnt, g->tile_logcnt, g->tile_ret, ntiles,
                        g->tile_off, g->tile_logoff, g->d_sums);
     SplitArgs SA;
     fill_split_args(c, SA, cur, n, dim, epsrel, epsabs, kMlpRows, false);
