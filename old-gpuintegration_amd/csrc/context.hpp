@@ -11,6 +11,7 @@
 namespace cuhre {
 
 struct BatchEngine;
+void guide_detach(cuhre_guide* g);  // the guide's context is going away
 
 struct SplitArgs {
   const double* lower;
@@ -29,6 +30,7 @@ struct SplitArgs {
   int64_t out_ld;
   const IterState* state;  // nullptr: always split
   const int8_t* retire;    // optional: regions retired as finished are not split
+  const int* retire_on;    // optional device flag: retire honoured only when nonzero
 };
 
 void launch_split(const SplitArgs& S, hipStream_t st);
@@ -67,6 +69,7 @@ struct cuhre_context {
 
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   cuhre::BatchEngine* batch = nullptr;
+  std::vector<cuhre_guide*> guides;  // attached guides, detached on destroy
 };
 
 namespace cuhre {

@@ -160,7 +160,7 @@ __global__ __launch_bounds__(kBlock) void k_split(SplitArgs S)
   if (valid) {
     const double e = S.est[r], er = S.err[r];
     act = isfinite(e) && isfinite(er) && region_active(e, er, S.epsrel, S.epsabs);
-    if (S.retire != nullptr && S.retire[r]) act = false;
+    if (S.retire != nullptr && (S.retire_on == nullptr || *S.retire_on) && S.retire[r]) act = false;
   }
   const uint64_t mask = __ballot(act);
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
@@ -526,6 +526,7 @@ void fill_split_args(cuhre_context* c, SplitArgs& S, int cur, int64_t n, int dim
   S.out_ld = c->cap;
   S.state = use_state ? c->state : nullptr;
   S.retire = nullptr;
+  S.retire_on = nullptr;
 }
 
 void dd_add_host(double& s1, double& c1, double s2, double c2)
@@ -589,6 +590,8 @@ int cuhre_context_destroy(cuhre_context* c)
   if (!c) return 0;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (cuhre_guide* g : c->guides) cuhre::guide_detach(g);
+  c->guides.clear();
   ctx_free_lists(c);
   batch_engine_delete(c->batch);
   c->batch = nullptr;

@@ -16,9 +16,13 @@
 //   * a finish flag (trained on regions whose error fits inside their volume
 //     share of epsrel * |I|, I being the run's final estimate), honoured only
 //     when the region's error really fits inside its volume share of
-//     budget * epsrel * |current estimate|, so the retired error stays below
-//     budget * epsrel * |I| in total.
-// The Genz-Malik error sum stays the only convergence test.
+//     budget * tol (tol = max(epsrel * |current estimate|, epsabs)).
+// The retirements of one iteration are taken all or none: only when the
+// retired error of the run stays within budget * tol and the frozen error
+// (finished plus retired) within (1 - budget) * tol, so the regions still
+// being split keep budget * tol or more for themselves and a run whose every
+// region is retired has met the global test.  The Genz-Malik error sum stays
+// the only convergence test.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -276,6 +280,7 @@ struct GuideArgs {
   double lows[kMaxDim], highs[kMaxDim];
   double absI;  // |current estimate of the integral|
   double epsrel, epsabs;
+  double tol;   // max(epsrel * absI, epsabs)
   int axis_mode;    // 0: fourth difference, 1: guide when confident
   int finish_mode;  // 0: none, 1: guide proposal + volume-share guard, 2: guard alone
   int use_mlp;
@@ -367,7 +372,7 @@ __global__ __launch_bounds__(256) void k_guide_apply(GuideArgs G)
     int retire = 0;
     if (active) {
       const double e = G.est[r], er = G.err[r];
-      const bool guard = er <= G.budget * G.epsrel * G.absI * vrel;
+      const bool guard = er <= G.budget * G.tol * vrel;
       if (G.use_mlp) {
         const float* y = Y + t * kMlpOut;
         float m = y[0];
@@ -426,22 +431,27 @@ __global__ __launch_bounds__(256) void k_guide_apply(GuideArgs G)
 }
 
 // One block: exclusive scans of the split / logged counts per tile and the
-// retired sums in tile order.
+// retired sums in tile order.  The iteration's retirements are honoured
+// (*retire_on = 1) only when their summed error fits in `room`; otherwise the
+// offsets count every active region and the retired sums are zero.
 __global__ __launch_bounds__(256) void k_guide_scan(const int64_t* cnt, const int64_t* logcnt, const double* ret,
-                                                    int64_t ntiles, int64_t* off, int64_t* logoff, GuideSums* out)
+                                                    int64_t ntiles, double room, int64_t* off, int64_t* logoff,
+                                                    GuideSums* out, int* retire_on)
 {
-  __shared__ int64_t sa[256], sb[256];
+  __shared__ int64_t sa[256], sb[256], sc[256];
   __shared__ double sd[256][7];
+  __shared__ int s_allow;
   const int T = blockDim.x;
   const int64_t chunk = (ntiles + T - 1) / T;
   const int64_t t0 = threadIdx.x * chunk;
   const int64_t t1 = t0 + chunk < ntiles ? t0 + chunk : ntiles;
-  int64_t ma = 0, mb = 0;
+  int64_t ma = 0, mb = 0, mc = 0;
   double es = 0, ec = 0, rs = 0, rc = 0, ab = 0, ae = 0, nr = 0;
   for (int64_t t = t0; t < t1; ++t) {
+    const double* tr = ret + t * kRetStride;
     ma += cnt[t];
     mb += logcnt[t];
-    const double* tr = ret + t * kRetStride;
+    mc += cnt[t] + (int64_t)tr[6];
     dd_add(es, ec, tr[0], tr[1]);
     dd_add(rs, rc, tr[2], tr[3]);
     ab += tr[4];
@@ -450,6 +460,7 @@ __global__ __launch_bounds__(256) void k_guide_scan(const int64_t* cnt, const in
   }
   sa[threadIdx.x] = ma;
   sb[threadIdx.x] = mb;
+  sc[threadIdx.x] = mc;
   sd[threadIdx.x][0] = es;
   sd[threadIdx.x][1] = ec;
   sd[threadIdx.x][2] = rs;
@@ -461,22 +472,15 @@ __global__ __launch_bounds__(256) void k_guide_scan(const int64_t* cnt, const in
   for (int o = 1; o < T; o <<= 1) {
     const int64_t va = (int)threadIdx.x >= o ? sa[threadIdx.x - o] : 0;
     const int64_t vb = (int)threadIdx.x >= o ? sb[threadIdx.x - o] : 0;
+    const int64_t vc = (int)threadIdx.x >= o ? sc[threadIdx.x - o] : 0;
     __syncthreads();
     sa[threadIdx.x] += va;
     sb[threadIdx.x] += vb;
+    sc[threadIdx.x] += vc;
     __syncthreads();
-  }
-  int64_t ra = sa[threadIdx.x] - ma, rb = sb[threadIdx.x] - mb;
-  for (int64_t t = t0; t < t1; ++t) {
-    off[t] = ra;
-    logoff[t] = rb;
-    ra += cnt[t];
-    rb += logcnt[t];
   }
   if (threadIdx.x == 0) {
     GuideSums g;
-    g.n_split = sa[T - 1];
-    g.n_logged = sb[T - 1];
     double a0 = 0, a1 = 0, b0 = 0, b1 = 0, c = 0, e = 0, n = 0;
     for (int i = 0; i < T; ++i) {
       dd_add(a0, a1, sd[i][0], sd[i][1]);
@@ -485,15 +489,36 @@ __global__ __launch_bounds__(256) void k_guide_scan(const int64_t* cnt, const in
       e += sd[i][5];
       n += sd[i][6];
     }
-    g.est_s = a0;
-    g.est_c = a1;
-    g.err_s = b0;
-    g.err_c = b1;
-    g.abs_r = c;
-    g.aest_r = e;
-    g.n_retired = (int64_t)n;
+    const int allow = n > 0.0 && b0 + b1 <= room;
+    g.n_logged = sb[T - 1];
     g.pad = 0;
+    if (allow) {
+      g.n_split = sa[T - 1];
+      g.est_s = a0;
+      g.est_c = a1;
+      g.err_s = b0;
+      g.err_c = b1;
+      g.abs_r = c;
+      g.aest_r = e;
+      g.n_retired = (int64_t)n;
+    } else {
+      g.n_split = sc[T - 1];
+      g.est_s = g.est_c = g.err_s = g.err_c = g.abs_r = g.aest_r = 0.0;
+      g.n_retired = 0;
+    }
     *out = g;
+    *retire_on = allow;
+    s_allow = allow;
+  }
+  __syncthreads();
+  const int allow = s_allow;
+  int64_t ra = allow ? sa[threadIdx.x] - ma : sc[threadIdx.x] - mc;
+  int64_t rb = sb[threadIdx.x] - mb;
+  for (int64_t t = t0; t < t1; ++t) {
+    off[t] = ra;
+    logoff[t] = rb;
+    ra += allow ? cnt[t] : cnt[t] + (int64_t)ret[t * kRetStride + 6];
+    rb += logcnt[t];
   }
 }
 
@@ -620,6 +645,7 @@ struct cuhre_guide {
   int64_t *tile_cnt = nullptr, *tile_off = nullptr, *tile_logcnt = nullptr, *tile_logoff = nullptr;
   double* tile_ret = nullptr;
   GuideSums* d_sums = nullptr;
+  int* d_retire_on = nullptr;  // the scan's verdict on this iteration's retirements
   GuideSums* h_sums = nullptr;
   // training scratch
   int64_t train_cap = 0;
@@ -853,7 +879,7 @@ int guided_run(cuhre_context* c, cuhre_guide* g, const cuhre_integrand* f, int d
   for (int m = 0; m < dim; ++m) vdom *= highs[m] - lows[m];
   (void)vdom;
   const int64_t ds_base0 = g->ds_n;
-  double fin_s = 0, fin_c = 0, fer_s = 0, fer_c = 0, fin_abs = 0, fin_aest = 0;
+  double fin_s = 0, fin_c = 0, fer_s = 0, fer_c = 0, fin_abs = 0, fin_aest = 0, ret_err = 0;
   int64_t nfin = 0, n = n0, logged = 0;
   const double u = 0.5 * 2.220446049250313e-16;
   int result = CUHRE_NOT_CONVERGED;
@@ -938,6 +964,7 @@ int guided_run(cuhre_context* c, cuhre_guide* g, const cuhre_integrand* f, int d
     G.absI = std::fabs(out->integral);
     G.epsrel = epsrel;
     G.epsabs = epsabs;
+    G.tol = std::max(epsrel * G.absI, epsabs);
     G.axis_mode = run.axis_mode;
     G.finish_mode = run.finish_mode;
     G.use_mlp = run.use_mlp;
@@ -957,12 +984,18 @@ int guided_run(cuhre_context* c, cuhre_guide* g, const cuhre_integrand* f, int d
     }
     const int64_t ntiles = (n + kMlpRows - 1) / kMlpRows;
     hipLaunchKernelGGL(k_guide_apply, dim3((unsigned)ntiles), dim3(256), 0, st, G);
+    // retirement room: retired error within budget * tol, frozen error
+    // (finished so far, finished now, retired) within (1 - budget) * tol
+    const double frozen = (fer_s + fer_c) + (s.tot.ferr_s + s.tot.ferr_c);
+    const double room =
+        run.finish_mode ? std::min(run.budget * G.tol - ret_err, (1.0 - run.budget) * G.tol - frozen) : -1.0;
     hipLaunchKernelGGL(k_guide_scan, dim3(1), dim3(256), 0, st, g->tile_cnt, g->tile_logcnt, g->tile_ret, ntiles,
-                       g->tile_off, g->tile_logoff, g->d_sums);
+                       room, g->tile_off, g->tile_logoff, g->d_sums, g->d_retire_on);
     SplitArgs SA;
     fill_split_args(c, SA, cur, n, dim, epsrel, epsabs, kMlpRows, false);
     SA.offsets = g->tile_off;
     SA.retire = g->retire;
+    SA.retire_on = g->d_retire_on;
     launch_split(SA, st);
     if (hipMemcpyAsync(g->h_sums, g->d_sums, sizeof(GuideSums), hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)
@@ -1012,12 +1045,16 @@ int guided_run(cuhre_context* c, cuhre_guide* g, const cuhre_integrand* f, int d
     dd_add_host(fer_s, fer_c, s.tot.ferr_s, s.tot.ferr_c);
     dd_add_host(fin_s, fin_c, gsum.est_s, gsum.est_c);
     dd_add_host(fer_s, fer_c, gsum.err_s, gsum.err_c);
+    ret_err += gsum.err_s + gsum.err_c;
     fin_abs += s.tot.abs_fin + gsum.abs_r;
     fin_aest += s.tot.aest_fin + gsum.aest_r;
     nfin += n - gsum.n_split;
     out->nfinished = nfin;
     if (gsum.n_split == 0) {
-      result = CUHRE_NOT_CONVERGED;
+      // every region is finished or retired: the estimate is final
+      out->integral = fin_s + fin_c;
+      out->error = fer_s + fer_c;
+      result = out->error <= std::max(epsrel * std::fabs(out->integral), epsabs) ? CUHRE_OK : CUHRE_NOT_CONVERGED;
       break;
     }
     n = 2 * gsum.n_split;
@@ -1039,6 +1076,22 @@ int guided_run(cuhre_context* c, cuhre_guide* g, const cuhre_integrand* f, int d
 
 }  // namespace
 
+namespace cuhre {
+void guide_detach(cuhre_guide* g) { g->ctx = nullptr; }
+}  // namespace cuhre
+
+namespace {
+
+// A guide is usable while its context lives.
+int guide_check(const cuhre_guide* g)
+{
+  if (!g) return set_error(CUHRE_ERR_ARG, "null guide");
+  if (!g->ctx) return set_error(CUHRE_ERR_ARG, "the guide's context was destroyed");
+  return 0;
+}
+
+}  // namespace
+
 extern "C" {
 
 int cuhre_guide_create(cuhre_context* ctx, uint64_t seed, cuhre_guide** out)
@@ -1055,6 +1108,7 @@ int cuhre_guide_create(cuhre_context* ctx, uint64_t seed, cuhre_guide** out)
       hipMalloc(&g->W, sizeof(uint16_t) * kMlpBOff[0]) != hipSuccess ||
       hipMalloc(&g->WT, sizeof(uint16_t) * kMlpBOff[0]) != hipSuccess ||
       hipMalloc(&g->d_sums, sizeof(GuideSums)) != hipSuccess ||
+      hipMalloc(&g->d_retire_on, sizeof(int)) != hipSuccess ||
       hipMalloc(&g->d_stats, sizeof(double) * 4) != hipSuccess ||
       hipHostMalloc(&g->h_sums, sizeof(GuideSums), hipHostMallocDefault) != hipSuccess) {
     (void)hipGetLastError();
@@ -1068,6 +1122,7 @@ int cuhre_guide_create(cuhre_context* ctx, uint64_t seed, cuhre_guide** out)
     cuhre_guide_destroy(g);
     return rc;
   }
+  ctx->guides.push_back(g);
   *out = g;
   return 0;
 }
@@ -1075,12 +1130,16 @@ int cuhre_guide_create(cuhre_context* ctx, uint64_t seed, cuhre_guide** out)
 int cuhre_guide_destroy(cuhre_guide* g)
 {
   if (!g) return 0;
-  if (g->ctx) (void)hipStreamSynchronize(g->ctx->stream);
+  if (g->ctx) {
+    (void)hipStreamSynchronize(g->ctx->stream);
+    auto& v = g->ctx->guides;
+    v.erase(std::remove(v.begin(), v.end(), g), v.end());
+  }
   for (void* q : {(void*)g->params, (void*)g->m, (void*)g->v, (void*)g->W, (void*)g->WT, (void*)g->grad,
                   (void*)g->part, (void*)g->ds_feat, (void*)g->ds_axis, (void*)g->ds_fin, (void*)g->ds_err,
                   (void*)g->ds_vrel, (void*)g->la_lo, (void*)g->la_wd, (void*)g->la_est, (void*)g->la_err,
                   (void*)g->la_ab, (void*)g->la_axis, (void*)g->retire, (void*)g->tile_cnt, (void*)g->tile_off,
-                  (void*)g->tile_logcnt, (void*)g->tile_logoff, (void*)g->tile_ret, (void*)g->d_sums,
+                  (void*)g->tile_logcnt, (void*)g->tile_logoff, (void*)g->tile_ret, (void*)g->d_sums, (void*)g->d_retire_on,
                   (void*)g->order, (void*)g->loss, (void*)g->correct, (void*)g->d_stats})
     gfree(q);
   if (g->h_sums) (void)hipHostFree(g->h_sums);
@@ -1090,13 +1149,15 @@ int cuhre_guide_destroy(cuhre_guide* g)
 
 int cuhre_guide_set_params(cuhre_guide* g, const float* params)
 {
-  if (!g || !params) return set_error(CUHRE_ERR_ARG, "null argument");
+  if (guide_check(g)) return CUHRE_ERR_ARG;
+  if (!params) return set_error(CUHRE_ERR_ARG, "null argument");
   return upload_params(g, params);
 }
 
 int cuhre_guide_get_params(cuhre_guide* g, float* params)
 {
-  if (!g || !params) return set_error(CUHRE_ERR_ARG, "null argument");
+  if (guide_check(g)) return CUHRE_ERR_ARG;
+  if (!params) return set_error(CUHRE_ERR_ARG, "null argument");
   if (hipMemcpy(params, g->params, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost) != hipSuccess)
     return set_error(CUHRE_ERR_DEVICE, "parameter download failed");
   return 0;
@@ -1104,7 +1165,8 @@ int cuhre_guide_get_params(cuhre_guide* g, float* params)
 
 int cuhre_guide_forward(cuhre_guide* g, int64_t n, const float* features, float* logits)
 {
-  if (!g || !features || !logits) return set_error(CUHRE_ERR_ARG, "null argument");
+  if (guide_check(g)) return CUHRE_ERR_ARG;
+  if (!features || !logits) return set_error(CUHRE_ERR_ARG, "null argument");
   if (n <= 0) return 0;
   hipStream_t st = g->ctx->stream;
   float *dx, *dy;
@@ -1126,7 +1188,8 @@ int cuhre_guide_forward(cuhre_guide* g, int64_t n, const float* features, float*
 int cuhre_guide_gradient(cuhre_guide* g, int64_t n, const float* features, const int32_t* axis,
                          const float* finish, int dim, float* grad, double* loss)
 {
-  if (!g || !features || !axis || !finish || !grad) return set_error(CUHRE_ERR_ARG, "null argument");
+  if (guide_check(g)) return CUHRE_ERR_ARG;
+  if (!features || !axis || !finish || !grad) return set_error(CUHRE_ERR_ARG, "null argument");
   if (n <= 0 || dim < 2 || dim > 8) return set_error(CUHRE_ERR_ARG, "bad size or dimension");
   hipStream_t st = g->ctx->stream;
   int rc = reserve_train(g, n, n);
@@ -1171,7 +1234,8 @@ int64_t cuhre_guide_samples(const cuhre_guide* g) { return g ? g->ds_n : -1; }
 int cuhre_guide_get_samples(cuhre_guide* g, int64_t first, int64_t count, float* features, int32_t* axis,
                             float* finish)
 {
-  if (!g || first < 0 || count < 0 || first + count > g->ds_n) return set_error(CUHRE_ERR_ARG, "bad sample range");
+  if (guide_check(g)) return CUHRE_ERR_ARG;
+  if (first < 0 || count < 0 || first + count > g->ds_n) return set_error(CUHRE_ERR_ARG, "bad sample range");
   if (count == 0) return 0;
   std::vector<int8_t> a8(count);
   if ((features && hipMemcpy(features, g->ds_feat + first * 32, sizeof(float) * 32 * count,
@@ -1188,7 +1252,8 @@ int cuhre_guide_collect(cuhre_guide* g, const cuhre_integrand* f, int dim, doubl
                         int64_t maxeval, const double* lows, const double* highs, int64_t max_samples,
                         cuhre_result* out)
 {
-  if (!g || !out) return set_error(CUHRE_ERR_ARG, "null argument");
+  if (guide_check(g)) return CUHRE_ERR_ARG;
+  if (!out) return set_error(CUHRE_ERR_ARG, "null argument");
   GuidedRun run{0, 0, 0, 1.f, 1.f, 0.0, max_samples};
   cuhre_guide_stats gs;
   return guided_run(g->ctx, g, f, dim, epsrel, epsabs, maxeval, lows, highs, run, out, &gs);
@@ -1197,7 +1262,8 @@ int cuhre_guide_collect(cuhre_guide* g, const cuhre_integrand* f, int dim, doubl
 int cuhre_guide_train(cuhre_guide* g, int dim, int epochs, int batch, float lr, uint64_t seed,
                       cuhre_train_stats* stats)
 {
-  if (!g || !stats) return set_error(CUHRE_ERR_ARG, "null argument");
+  if (guide_check(g)) return CUHRE_ERR_ARG;
+  if (!stats) return set_error(CUHRE_ERR_ARG, "null argument");
   std::memset(stats, 0, sizeof *stats);
   if (dim < 2 || dim > 8 || epochs <= 0 || batch < kMlpRows) return set_error(CUHRE_ERR_ARG, "bad training settings");
   const int64_t n = g->ds_n;
