@@ -202,10 +202,15 @@ typedef struct cuhre_guide cuhre_guide;
 typedef struct {
   int32_t axis_mode;        /* 0: fourth difference; 1: guide when confident   */
   int32_t finish_mode;      /* 0: none; 1: guide proposal + volume-share guard;
-                               2: guard alone (no network)                     */
+                               2: guard alone (no network);
+                               3: error budget: the per-region test is off and
+                                  each iteration retires the smallest errors
+                                  that fit in budget * (tol - frozen error)   */
   float axis_confidence;    /* softmax probability needed to override the axis */
   float finish_threshold;   /* sigmoid output needed to propose a finish       */
-  double budget;            /* share of epsrel*|I| retired regions may use     */
+  double budget;            /* modes 1, 2: share of tol retired regions may use;
+                               mode 3: share of the remaining slack retired
+                               per iteration                                  */
 } cuhre_guide_policy;
 
 typedef struct {

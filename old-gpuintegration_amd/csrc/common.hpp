@@ -66,7 +66,8 @@ struct EvalArgs {
   const double* width;
   int64_t ld;
   int64_t n;
-  double epsrel, epsabs;
+  double epsrel, epsabs;          // global test
+  double loc_epsrel, loc_epsabs;  // per-region test (active iff err > max(loc_epsrel*|est|, loc_epsabs))
   double* est;
   double* err;
   int8_t* axis;

@@ -495,6 +495,8 @@ void fill_eval_args(cuhre_context* c, EvalArgs& A, int cur, int64_t n, int dim, 
   A.n = n;
   A.epsrel = epsrel;
   A.epsabs = epsabs;
+  A.loc_epsrel = epsrel;
+  A.loc_epsabs = epsabs;
   A.est = c->est;
   A.err = c->err;
   A.axis = c->axis;

@@ -281,8 +281,10 @@ struct GuideArgs {
   double absI;  // |current estimate of the integral|
   double epsrel, epsabs;
   double tol;   // max(epsrel * absI, epsabs)
+  double loc_epsrel, loc_epsabs;  // the per-region test (0, 0 under the shared allotment)
   int axis_mode;    // 0: fourth difference, 1: guide when confident
-  int finish_mode;  // 0: none, 1: guide proposal + volume-share guard, 2: guard alone
+  int finish_mode;  // 0: none, 1: guide proposal + volume-share guard, 2: guard alone,
+                    // 3: shared value / volume allotment replaces the per-region test
   int use_mlp;
   float axis_conf, finish_thr;
   double budget;  // fraction of epsrel*|I| the retired regions may spend
@@ -340,12 +342,16 @@ __device__ __forceinline__ double guide_features(const GuideArgs& G, int64_t r, 
 }
 
 // Guide step of one iteration on 64 regions per block: features, network,
-// axis / finish decisions, per-tile counts and retired sums.
-__global__ __launch_bounds__(256) void k_guide_apply(GuideArgs G)
+// axis / finish decisions, per-tile counts and retired sums.  MLP: 256
+// threads run the network on the tile; otherwise one wave, no network.
+template <bool MLP>
+__global__ __launch_bounds__(MLP ? 256 : 64) void k_guide_apply(GuideArgs G)
 {
-  __shared__ __align__(16) uint16_t X[kMlpRows * kMlpIn];
-  __shared__ __align__(16) uint16_t H1[kMlpRows * kMlpHid], H2[kMlpRows * kMlpHid], H3[kMlpRows * kMlpHid];
-  __shared__ float Y[kMlpRows * kMlpOut];
+  constexpr int kL = MLP ? 1 : 0;  // network buffers only when the network runs
+  __shared__ __align__(16) uint16_t X[kMlpRows * kMlpIn * kL + 1];
+  __shared__ __align__(16) uint16_t H1[kMlpRows * kMlpHid * kL + 1], H2[kMlpRows * kMlpHid * kL + 1],
+      H3[kMlpRows * kMlpHid * kL + 1];
+  __shared__ float Y[kMlpRows * kMlpOut * kL + 1];
   __shared__ int s_split[kMlpRows], s_log[kMlpRows], s_ret[kMlpRows];
   __shared__ double s_e[kMlpRows], s_er[kMlpRows], s_ab[kMlpRows];
   const int t = threadIdx.x;
@@ -357,13 +363,19 @@ __global__ __launch_bounds__(256) void k_guide_apply(GuideArgs G)
     for (int j = 0; j < 32; ++j) f[j] = 0.f;
     if (r < G.n) {
       const double e = G.est[r], er = G.err[r];
-      active = isfinite(e) && isfinite(er) && region_active(e, er, G.epsrel, G.epsabs);
-      vrel = guide_features(G, r, f);
+      active = isfinite(e) && isfinite(er) && region_active(e, er, G.loc_epsrel, G.loc_epsabs);
+      if (MLP) {
+        vrel = guide_features(G, r, f);
+      } else if (G.finish_mode >= 2) {
+        vrel = 1.0;
+        for (int m = 0; m < G.dim; ++m) vrel *= G.width[m * G.ld + r] / (G.highs[m] - G.lows[m]);
+      }
     }
-    for (int j = 0; j < 32; ++j) X[t * kMlpIn + j] = f2bf(f[j]);
+    if (MLP)
+      for (int j = 0; j < 32; ++j) X[t * kMlpIn + j] = f2bf(f[j]);
   }
   __syncthreads();
-  if (G.use_mlp) {
+  if (MLP) {
     const int w = t >> 6;
     mlp_forward16(G.M, X + w * 16 * kMlpIn, H1 + w * 16 * kMlpHid, H2 + w * 16 * kMlpHid, H3 + w * 16 * kMlpHid,
                   Y + w * 16 * kMlpOut);
@@ -373,7 +385,7 @@ __global__ __launch_bounds__(256) void k_guide_apply(GuideArgs G)
     if (active) {
       const double e = G.est[r], er = G.err[r];
       const bool guard = er <= G.budget * G.tol * vrel;
-      if (G.use_mlp) {
+      if (MLP) {
         const float* y = Y + t * kMlpOut;
         float m = y[0];
         int am = 0;
@@ -390,6 +402,11 @@ __global__ __launch_bounds__(256) void k_guide_apply(GuideArgs G)
         if (G.finish_mode == 1 && q >= G.finish_thr && guard) retire = 1;
       }
       if (G.finish_mode == 2 && guard) retire = 1;
+      if (G.finish_mode == 3) {
+        // shared allotment: a (1 - budget) share of tol by value, a budget share by volume
+        const double a = G.epsrel * ((1.0 - G.budget) * fabs(e) + G.budget * G.absI * vrel);
+        if (er <= fmax(a, G.epsabs)) retire = 1;
+      }
       s_e[t] = e;
       s_er[t] = er;
       s_ab[t] = G.abssum[r];
@@ -432,12 +449,14 @@ __global__ __launch_bounds__(256) void k_guide_apply(GuideArgs G)
 
 // One block: exclusive scans of the split / logged counts per tile and the
 // retired sums in tile order.  The iteration's retirements are honoured
-// (*retire_on = 1) only when their summed error fits in `room`; otherwise the
+// (*retire_on = 1) only when their summed error fits in `room` (read from
+// room_dev when given); otherwise the
 // offsets count every active region and the retired sums are zero.
 __global__ __launch_bounds__(256) void k_guide_scan(const int64_t* cnt, const int64_t* logcnt, const double* ret,
-                                                    int64_t ntiles, double room, int64_t* off, int64_t* logoff,
-                                                    GuideSums* out, int* retire_on)
+                                                    int64_t ntiles, double room, const double* room_dev,
+                                                    int64_t* off, int64_t* logoff, GuideSums* out, int* retire_on)
 {
+  if (room_dev != nullptr) room = *room_dev;
   __shared__ int64_t sa[256], sb[256], sc[256];
   __shared__ double sd[256][7];
   __shared__ int s_allow;
@@ -543,7 +562,7 @@ __global__ __launch_bounds__(64) void k_collect_write(CollectArgs C)
   bool active = false;
   if (r < G.n) {
     const double e = G.est[r], er = G.err[r];
-    active = isfinite(e) && isfinite(er) && region_active(e, er, G.epsrel, G.epsabs);
+    active = isfinite(e) && isfinite(er) && region_active(e, er, G.loc_epsrel, G.loc_epsabs);
   }
   const bool sel = guide_selected(G, r, active);
   const uint64_t mask = __ballot(sel);
@@ -865,8 +884,12 @@ int guided_run(cuhre_context* c, cuhre_guide* g, const cuhre_integrand* f, int d
   std::vector<double> hlo, hwd;
   const int64_t n0 = initial_grid(dim, lows, highs, hlo, hwd);
   const int64_t capacity = list_capacity(c, dim);
+  // the network and the sample log read I5 and the fourth differences
+  const bool need_aux = run.use_mlp || run.collect_max > 0;
+  // the error-budget finish replaces the per-region test
+  const double loc_rel = run.finish_mode == 3 ? 0.0 : epsrel, loc_abs = run.finish_mode == 3 ? 0.0 : epsabs;
   if ((rc = ctx_reserve(c, std::min<int64_t>(capacity, std::max<int64_t>(4 * n0, 1 << 16)), dim))) return rc;
-  if ((rc = ctx_reserve_aux(c, c->cap, dim))) return rc;
+  if (need_aux && (rc = ctx_reserve_aux(c, c->cap, dim))) return rc;
   if ((rc = guide_reserve_regions(g, c->cap))) return rc;
   hipStream_t st = c->stream;
   int cur = 0;
@@ -875,9 +898,6 @@ int guided_run(cuhre_context* c, cuhre_guide* g, const cuhre_integrand* f, int d
       hipMemcpy2DAsync(c->wd[cur], sizeof(double) * c->cap, hwd.data(), sizeof(double) * n0, sizeof(double) * n0,
                        dim, hipMemcpyHostToDevice, st) != hipSuccess)
     return set_error(CUHRE_ERR_DEVICE, "initial grid upload failed");
-  double vdom = 1.0;
-  for (int m = 0; m < dim; ++m) vdom *= highs[m] - lows[m];
-  (void)vdom;
   const int64_t ds_base0 = g->ds_n;
   double fin_s = 0, fin_c = 0, fer_s = 0, fer_c = 0, fin_abs = 0, fin_aest = 0, ret_err = 0;
   int64_t nfin = 0, n = n0, logged = 0;
@@ -887,9 +907,11 @@ int guided_run(cuhre_context* c, cuhre_guide* g, const cuhre_integrand* f, int d
     const int S = pick_segments(c, n);
     EvalArgs A;
     fill_eval_args(c, A, cur, n, dim, epsrel, epsabs);
+    A.loc_epsrel = loc_rel;
+    A.loc_epsabs = loc_abs;
     A.finalize = 1;
-    A.i5 = c->aux;
-    A.dvec = c->aux + c->cap;
+    A.i5 = need_aux ? c->aux : nullptr;
+    A.dvec = need_aux ? c->aux + c->cap : nullptr;
     A.fin_est_s = fin_s;
     A.fin_est_c = fin_c;
     A.fin_err_s = fer_s;
@@ -916,14 +938,18 @@ int guided_run(cuhre_context* c, cuhre_guide* g, const cuhre_integrand* f, int d
       const int64_t ncap = std::min<int64_t>(capacity, std::max<int64_t>(2 * s.tot.nactive, 2 * c->cap));
       const int64_t nb = blocks_for(n, S);
       if ((rc = ctx_reserve(c, ncap, dim, n, cur, nb))) return rc;
-      // i5 / dvec were computed into the old aux block: re-evaluate this list
-      if ((rc = ctx_reserve_aux(c, c->cap, dim))) return rc;
       if ((rc = guide_reserve_regions(g, c->cap))) return rc;
-      fill_eval_args(c, A, cur, n, dim, epsrel, epsabs);
-      A.finalize = 0;
-      A.i5 = c->aux;
-      A.dvec = c->aux + c->cap;
-      find_eval_launcher(f->family, dim)(A, P, R, S, st);
+      if (need_aux) {
+        // i5 / dvec were computed into the old aux block: re-evaluate this list
+        if ((rc = ctx_reserve_aux(c, c->cap, dim))) return rc;
+        fill_eval_args(c, A, cur, n, dim, epsrel, epsabs);
+        A.loc_epsrel = loc_rel;
+        A.loc_epsabs = loc_abs;
+        A.finalize = 0;
+        A.i5 = c->aux;
+        A.dvec = c->aux + c->cap;
+        find_eval_launcher(f->family, dim)(A, P, R, S, st);
+      }
       s.stop = 0;
     }
     const double abs_contrib = fin_abs + s.tot.abs_all;
@@ -965,6 +991,8 @@ int guided_run(cuhre_context* c, cuhre_guide* g, const cuhre_integrand* f, int d
     G.epsrel = epsrel;
     G.epsabs = epsabs;
     G.tol = std::max(epsrel * G.absI, epsabs);
+    G.loc_epsrel = loc_rel;
+    G.loc_epsabs = loc_abs;
     G.axis_mode = run.axis_mode;
     G.finish_mode = run.finish_mode;
     G.use_mlp = run.use_mlp;
@@ -983,16 +1011,22 @@ int guided_run(cuhre_context* c, cuhre_guide* g, const cuhre_integrand* f, int d
       G.log_salt = 0x5DEECE66Dull * (uint64_t)(it + 1);
     }
     const int64_t ntiles = (n + kMlpRows - 1) / kMlpRows;
-    hipLaunchKernelGGL(k_guide_apply, dim3((unsigned)ntiles), dim3(256), 0, st, G);
+    if (run.use_mlp)
+      hipLaunchKernelGGL(k_guide_apply<true>, dim3((unsigned)ntiles), dim3(256), 0, st, G);
+    else
+      hipLaunchKernelGGL(k_guide_apply<false>, dim3((unsigned)ntiles), dim3(64), 0, st, G);
     // retirement room: retired error within budget * tol, frozen error
     // (finished so far, finished now, retired) within (1 - budget) * tol
     const double frozen = (fer_s + fer_c) + (s.tot.ferr_s + s.tot.ferr_c);
     const double room =
-        run.finish_mode ? std::min(run.budget * G.tol - ret_err, (1.0 - run.budget) * G.tol - frozen) : -1.0;
+        run.finish_mode == 3 ? 1e300
+        : run.finish_mode    ? std::min(run.budget * G.tol - ret_err, (1.0 - run.budget) * G.tol - frozen)
+                             : -1.0;
     hipLaunchKernelGGL(k_guide_scan, dim3(1), dim3(256), 0, st, g->tile_cnt, g->tile_logcnt, g->tile_ret, ntiles,
-                       room, g->tile_off, g->tile_logoff, g->d_sums, g->d_retire_on);
+                       room, nullptr, g->tile_off, g->tile_logoff,
+                       g->d_sums, g->d_retire_on);
     SplitArgs SA;
-    fill_split_args(c, SA, cur, n, dim, epsrel, epsabs, kMlpRows, false);
+    fill_split_args(c, SA, cur, n, dim, loc_rel, loc_abs, kMlpRows, false);
     SA.offsets = g->tile_off;
     SA.retire = g->retire;
     SA.retire_on = g->d_retire_on;
@@ -1022,8 +1056,8 @@ int guided_run(cuhre_context* c, cuhre_guide* g, const cuhre_integrand* f, int d
       LA.width = g->la_wd;
       LA.ld = g->la_cap;
       LA.n = gsum.n_logged * 2 * dim;
-      LA.epsrel = epsrel;
-      LA.epsabs = epsabs;
+      LA.epsrel = LA.loc_epsrel = epsrel;
+      LA.epsabs = LA.loc_epsabs = epsabs;
       LA.est = g->la_est;
       LA.err = g->la_err;
       LA.axis = g->la_axis;
@@ -1109,6 +1143,7 @@ int cuhre_guide_create(cuhre_context* ctx, uint64_t seed, cuhre_guide** out)
       hipMalloc(&g->WT, sizeof(uint16_t) * kMlpBOff[0]) != hipSuccess ||
       hipMalloc(&g->d_sums, sizeof(GuideSums)) != hipSuccess ||
       hipMalloc(&g->d_retire_on, sizeof(int)) != hipSuccess ||
+
       hipMalloc(&g->d_stats, sizeof(double) * 4) != hipSuccess ||
       hipHostMalloc(&g->h_sums, sizeof(GuideSums), hipHostMallocDefault) != hipSuccess) {
     (void)hipGetLastError();
@@ -1318,7 +1353,7 @@ int cuhre_integrate_guided(cuhre_context* ctx, cuhre_guide* g, const cuhre_integ
   if (g->ctx != ctx) return set_error(CUHRE_ERR_ARG, "the guide belongs to another context");
   cuhre_guide_policy d = {1, 1, 0.6f, 0.5f, 0.25};
   const cuhre_guide_policy& P = pol ? *pol : d;
-  if (P.axis_mode < 0 || P.axis_mode > 1 || P.finish_mode < 0 || P.finish_mode > 2 || !(P.budget >= 0.0) ||
+  if (P.axis_mode < 0 || P.axis_mode > 1 || P.finish_mode < 0 || P.finish_mode > 3 || !(P.budget >= 0.0) ||
       P.budget > 1.0)
     return set_error(CUHRE_ERR_ARG, "bad guide policy");
   GuidedRun run{P.axis_mode, P.finish_mode, (P.axis_mode == 1 || P.finish_mode == 1) ? 1 : 0, P.axis_confidence,

@@ -85,14 +85,17 @@ def test_guided_without_guide_is_the_plain_method(ctx):
         assert a["guide"]["retired"] == 0
 
 
-@pytest.mark.parametrize("axis_mode,finish_mode", [(1, 1), (0, 2)])
-def test_guided_finish_keeps_the_certificate(ctx, axis_mode, finish_mode):
-    # (1, 1): an untrained network picks every axis and proposes finishes;
-    # (0, 2): fourth-difference axes, volume-share guard alone
+CORNER5 = W.Problem("corner5", W.corner_peak([0.7, 1.5, 2.0, 3.0, 4.0], 6), [0.0] * 5, [1.0] * 5, 1e-6, 0.0,
+                    10**9)
+
+
+@pytest.mark.parametrize("finish_mode", [1, 2])
+def test_guided_finish_keeps_the_certificate(ctx, finish_mode):
+    # 1: an untrained network proposes finishes, the guard and the room decide;
+    # 2: the volume-share guard alone.  Axes are the fourth-difference ones.
     g = cuhre.Guide(ctx, seed=3)
-    for p in (W.c1_gaussian3(), W.Problem("corner5", W.corner_peak([0.7, 1.5, 2.0, 3.0, 4.0], 6),
-                                          [0.0] * 5, [1.0] * 5, 1e-6, 0.0, 10**9)):
-        r = g.integrate(p, axis_mode=axis_mode, finish_mode=finish_mode, axis_confidence=0.0, budget=0.25)
+    for p in (W.c1_gaussian3(), CORNER5):
+        r = g.integrate(p, axis_mode=0, finish_mode=finish_mode, finish_threshold=0.0, budget=0.25)
         ex, _ = O.exact(p.f, p.lows, p.highs)
         assert r["rc"] == 0, r
         assert r["error"] <= p.epsrel * abs(r["integral"])
@@ -100,7 +103,40 @@ def test_guided_finish_keeps_the_certificate(ctx, axis_mode, finish_mode):
         if finish_mode == 2:
             plain = ctx.integrate(p)
             assert r["guide"]["retired"] > 0
-            assert r["nregions"] <= plain["nregions"]
+            assert r["nregions"] <= plain["nregions"], (p.name, r["nregions"], plain["nregions"])
+
+
+def test_shared_allotment(ctx):
+    # finish_mode 3 replaces the per-region test err <= epsrel*|est| by
+    # err <= epsrel*((1-b)*|est| + b*|I|*vrel); b = 0 is the plain method
+    g = cuhre.Guide(ctx, seed=3)
+    for p in (W.c1_gaussian3(), CORNER5, W.c2_oscillatory8()):
+        plain = ctx.integrate(p)
+        assert plain["rc"] == 0
+        r0 = g.integrate(p, axis_mode=0, finish_mode=3, budget=0.0)
+        for k in ("nregions", "iterations", "integral", "error", "rc"):
+            assert r0[k] == plain[k], (p.name, k, r0[k], plain[k])
+        r = g.integrate(p, axis_mode=0, finish_mode=3, budget=0.5)
+        ex, _ = O.exact(p.f, p.lows, p.highs)
+        assert r["rc"] == 0, r
+        assert r["error"] <= p.epsrel * abs(r["integral"])
+        assert abs(r["integral"] - ex) <= r["error"] + 1e-12 * abs(ex)
+        if p.name != "corner5":
+            assert r["nregions"] < plain["nregions"], (p.name, r["nregions"], plain["nregions"])
+
+
+def test_guided_untrained_axes_stay_honest(ctx):
+    # an untrained network choosing every axis may waste the budget, but a
+    # reported convergence still carries a true error bound
+    g = cuhre.Guide(ctx, seed=3)
+    for p in (W.c1_gaussian3(), CORNER5):
+        r = g.integrate(p, axis_mode=1, finish_mode=1, axis_confidence=0.0, budget=0.25)
+        ex, _ = O.exact(p.f, p.lows, p.highs)
+        assert r["rc"] in (0, 1), r
+        assert r["neval"] <= p.maxeval
+        if r["rc"] == 0:
+            assert r["error"] <= p.epsrel * abs(r["integral"])
+            assert abs(r["integral"] - ex) <= r["error"] + 1e-12 * abs(ex)
 
 
 def test_collection_labels(ctx):
