@@ -114,16 +114,29 @@ RuleConst make_rule(int d)
 // counting convention add/sub/mul = 1, fma = 2, and div / exp / cos / sqrt
 // charged the FP64 VALU flops of their gfx950 instruction sequences
 // (measured with the SQ_INSTS_VALU_*_F64 counters on a one-op kernel).
-struct FlopCosts { double div = 14, exp = 27, cos = 57, sqrt = 17; };
+// FP64 flops per call (add/sub/mul 1, fma 2, transcendental estimate 1) of
+// the device math each family uses, counted with the SQ_INSTS_VALU_*_F64
+// counters on one-operation kernels (flopcost.cpp); recip is recip_dev.
+struct FlopCosts { double div = 14, exp = 27, cos = 57, sqrt = 17, recip = 9; };
 static FlopCosts g_costs;
 
-double family_flops(int fam, int d)
+// multiplies of b^e: ipow_const for Genz's exponent d + 1, else the loop
+double ipow_flops(int e, int d)
 {
-  switch (fam) {
+  if (e <= 0) return 0.0;
+  int sq = 0, bits = 0;
+  for (int t = e; t > 1; t >>= 1) ++sq;
+  for (int t = e; t; t >>= 1) bits += t & 1;
+  return e == d + 1 ? (double)(sq + bits - 1) : (double)(sq + bits);
+}
+
+double family_flops(const cuhre_integrand* f, int d)
+{
+  switch (f->family) {
   case CUHRE_GAUSSIAN: return 4.0 * d + g_costs.exp;
-  case CUHRE_PRODUCT_PEAK: return 4.0 * d + g_costs.div;
+  case CUHRE_PRODUCT_PEAK: return 4.0 * d - 1 + g_costs.recip;
   case CUHRE_OSCILLATORY: return 2.0 * d + 1 + g_costs.cos;
-  case CUHRE_CORNER_PEAK: return 2.0 * d + 4 + g_costs.div;
+  case CUHRE_CORNER_PEAK: return 2.0 * d + ipow_flops(f->exponent, d) + g_costs.recip;
   case CUHRE_CONTINUOUS: return 4.0 * d + g_costs.exp;
   case CUHRE_COSINE_SUM: return d * (2.0 + g_costs.cos + g_costs.div) + 1;
   case CUHRE_TABLE6: return 20.0 + g_costs.exp + g_costs.sqrt + 2 * g_costs.div;
@@ -131,10 +144,13 @@ double family_flops(int fam, int d)
   }
 }
 
-double region_flops(int fam, int d)
+// Per region: every point costs its integrand plus d coordinate FMAs; the
+// sums take three FMAs (I7, I5, sum |w f|) at centre/axis/pair points and
+// two at the corners (no degree-5 weight); plus the per-region setup.
+double region_flops(const cuhre_integrand* f, int d)
 {
-  const double p = gm_points(d);
-  return p * (family_flops(fam, d) + 2.0 * d + 4.0) + (8.0 * d + 12.0);
+  const double p = gm_points(d), corners = (double)(1 << d);
+  return p * (family_flops(f, d) + 2.0 * d) + 6.0 * (p - corners) + 4.0 * corners + (8.0 * d + 12.0);
 }
 
 // ------------------------------------------------------------------ split
@@ -640,7 +656,7 @@ int cuhre_integrate(cuhre_context* c, const cuhre_integrand* f, int dim, double 
   if ((rc = make_dev_integrand(c, f, dim, &P))) return rc;
   const RuleConst R = make_rule(dim);
   const int64_t p = R.npts;
-  const double rflops = region_flops(f->family, dim);
+  const double rflops = region_flops(f, dim);
 
   std::vector<double> hlo, hwd;
   const int64_t n0 = initial_grid(dim, lows, highs, hlo, hwd);

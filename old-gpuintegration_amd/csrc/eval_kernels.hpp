@@ -125,7 +125,7 @@ __device__ __forceinline__ RegionOut seg_eval(const F& f, const double* __restri
 
   // pair points at l4, pair q = (i, j) in the order (0,1),(0,2),...,(D-2,D-1)
   constexpr int NPAIR = D * (D - 1) / 2;
-#pragma unroll 1
+#pragma unroll 2
   for (int q = s; q < NPAIR; q += S) {
     int i = 0, rem = q;
     while (rem >= D - 1 - i) {
@@ -147,7 +147,7 @@ __device__ __forceinline__ RegionOut seg_eval(const F& f, const double* __restri
   }
 
   // 2^D corner points at l5 (degree-7 rule only)
-#pragma unroll 1
+#pragma unroll 2
   for (int k = s; k < (1 << D); k += S) {
 #pragma unroll
     for (int m = 0; m < D; ++m) x[m] = fma(h[m], ((k >> m) & 1) ? -R.l5 : R.l5, c[m]);

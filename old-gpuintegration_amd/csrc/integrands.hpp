@@ -11,15 +11,46 @@
 
 namespace cuhre {
 
+// b^e, e >= 0: right-to-left binary powering.
 __device__ __forceinline__ double ipow_dev(double b, int e)
 {
   double r = 1.0;
   while (e > 0) {
     if (e & 1) r *= b;
-    b *= b;
     e >>= 1;
+    if (e == 0) break;
+    b *= b;
   }
   return r;
+}
+
+// b^E for a compile-time E, unrolled (E = 8: three squarings).
+template <int E>
+__device__ __forceinline__ double ipow_const(double b)
+{
+  if constexpr (E == 0) {
+    return 1.0;
+  } else if constexpr (E == 1) {
+    return b;
+  } else if constexpr (E % 2 == 0) {
+    const double h = ipow_const<E / 2>(b);
+    return h * h;
+  } else {
+    return b * ipow_const<E - 1>(b);
+  }
+}
+
+// 1/p from the hardware estimate (v_rcp_f64) refined by two Newton-Raphson
+// steps r <- r + r (1 - p r): within one ulp of the rounded quotient for
+// normal finite p.  p = +-inf or 0 gives NaN (the steps meet inf * 0), which
+// the evaluation reports as a non-finite integrand value.
+__device__ __forceinline__ double recip_dev(double p)
+{
+  double r = __builtin_amdgcn_rcp(p);
+  double e = fma(-p, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-p, r, 1.0);
+  return fma(r, e, r);
 }
 
 template <int FAM, int D>
@@ -49,7 +80,7 @@ struct Family<CUHRE_PRODUCT_PEAK, D> {
       double t = x[i] - P.w[i];
       p *= fma(t, t, P.b[i]);
     }
-    return 1.0 / p;
+    return recip_dev(p);
   }
 };
 
@@ -71,7 +102,9 @@ struct Family<CUHRE_CORNER_PEAK, D> {
     double s = 1.0;
 #pragma unroll
     for (int i = 0; i < D; ++i) s = fma(P.a[i], x[i], s);
-    return ipow_dev(1.0 / s, P.exponent);
+    const double r = recip_dev(s);
+    // Genz's exponent d + 1 unrolled; any other exponent by the loop
+    return P.exponent == D + 1 ? ipow_const<D + 1>(r) : ipow_dev(r, P.exponent);
   }
 };
 

@@ -15,7 +15,7 @@ import numpy as np
 import workloads as W
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIBPATH = os.path.join(HERE, "libcuhre_amd.so")
+LIBPATH = os.environ.get("CUHRE_AMD_LIB") or os.path.join(HERE, "libcuhre_amd.so")
 MAXDIM = 16
 MAXTERMS = 8
 
