@@ -82,6 +82,9 @@ struct EvalArgs {
   int64_t max_active;    // stop (eval budget / capacity) above this count
   int64_t alloc_active;  // grow the list buffers above this count
   int32_t finalize;    // 0: per-region outputs only (no partials/state)
+  uint32_t seq;        // published with the state when host_state is set
+  IterState* host_state;  // optional: coherent host copy of *state
+  uint32_t* host_seq;     // written (system-scope release) after host_state
 };
 
 // Launches the rule evaluation of A.n regions with S point segments.

@@ -59,6 +59,13 @@ struct cuhre_context {
   uint32_t* ticket = nullptr;
   cuhre::IterState* state = nullptr;
   cuhre::IterState* h_state = nullptr;
+  // coherent host memory the evaluation's last block publishes its state to
+  // (host and device views), with a sequence number written last
+  cuhre::IterState* pub_state = nullptr;
+  uint32_t* pub_seq = nullptr;
+  cuhre::IterState* pub_state_dev = nullptr;
+  uint32_t* pub_seq_dev = nullptr;
+  uint32_t seq = 0;
   double* aux = nullptr;  // i5 (cap) + dvec (cap*dim) on request
   int64_t aux_cap = 0;
   double2* sum_part = nullptr;

@@ -589,6 +589,10 @@ int cuhre_context_create(cuhre_context** out, int device, int64_t capacity_regio
       hipMalloc(&c->ticket, sizeof(uint32_t)) != hipSuccess ||
       hipMalloc(&c->state, sizeof(IterState)) != hipSuccess ||
       hipHostMalloc(&c->h_state, sizeof(IterState), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc(&c->pub_state, sizeof(IterState), hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
+      hipHostMalloc(&c->pub_seq, sizeof(uint32_t), hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
+      hipHostGetDevicePointer((void**)&c->pub_state_dev, c->pub_state, 0) != hipSuccess ||
+      hipHostGetDevicePointer((void**)&c->pub_seq_dev, c->pub_seq, 0) != hipSuccess ||
       hipMemset(c->ticket, 0, sizeof(uint32_t)) != hipSuccess ||
       hipMalloc(&c->sum_out, sizeof(double)) != hipSuccess) {
     rc = fail(CUHRE_ERR_DEVICE, "context setup failed: %s", hipGetErrorString(hipGetLastError()));
@@ -617,6 +621,8 @@ int cuhre_context_destroy(cuhre_context* c)
   if (c->ticket) (void)hipFree(c->ticket);
   if (c->state) (void)hipFree(c->state);
   if (c->h_state) (void)hipHostFree(c->h_state);
+  if (c->pub_state) (void)hipHostFree(c->pub_state);
+  if (c->pub_seq) (void)hipHostFree(c->pub_seq);
   if (c->sum_part) (void)hipFree(c->sum_part);
   if (c->sum_out) (void)hipFree(c->sum_out);
   if (c->table) (void)hipFree(c->table);
@@ -637,6 +643,30 @@ int cuhre_context_set(cuhre_context* c, int profile, int point_segments)
   c->segments = point_segments;
   return 0;
 }
+
+namespace {
+
+// Waits for the evaluation launch numbered seq to publish its state (its
+// last block writes the state, then the number, to coherent host memory).
+// Meanwhile the split already queued behind it keeps the GPU busy.
+int wait_published(cuhre_context* c, uint32_t seq, IterState* out)
+{
+  for (uint64_t spin = 0;; ++spin) {
+    if (__atomic_load_n(c->pub_seq, __ATOMIC_ACQUIRE) == seq) {
+      std::memcpy(out, (const void*)c->pub_state, sizeof *out);
+      return 0;
+    }
+    if ((spin & 4095) == 4095) {
+      const hipError_t q = hipStreamQuery(c->stream);
+      if (q == hipSuccess && __atomic_load_n(c->pub_seq, __ATOMIC_ACQUIRE) != seq)
+        return fail(CUHRE_ERR_DEVICE, "evaluation ended without publishing its state");
+      if (q != hipSuccess && q != hipErrorNotReady)
+        return fail(CUHRE_ERR_DEVICE, "HIP error: %s", hipGetErrorString(q));
+    }
+  }
+}
+
+}  // namespace
 
 int cuhre_integrate(cuhre_context* c, const cuhre_integrand* f, int dim, double epsrel, double epsabs,
                     int64_t maxeval, int key, const double* lows, const double* highs, cuhre_result* out)
@@ -692,15 +722,28 @@ int cuhre_integrate(cuhre_context* c, const cuhre_integrand* f, int dim, double 
     const int64_t nb = blocks_for(n, S);
     if (nb > 0x7fffffffll) return fail(CUHRE_ERR_CAPACITY, "grid too large");
 
+    // without profiling the state comes back through coherent host memory,
+    // so the next launches are queued while this split still runs
+    const bool publish = !c->profile;
+    if (publish) {
+      A.seq = ++c->seq;
+      A.host_state = c->pub_state_dev;
+      A.host_seq = c->pub_seq_dev;
+    }
     if (c->profile) HIP_TRY(hipEventRecord(c->ev[0], c->stream));
     launch(A, P, R, S, c->stream);
     if (c->profile) HIP_TRY(hipEventRecord(c->ev[1], c->stream));
     hipLaunchKernelGGL(k_split, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, c->stream, SA);
     if (c->profile) HIP_TRY(hipEventRecord(c->ev[2], c->stream));
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(c->h_state, c->state, sizeof(IterState), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    IterState st = *c->h_state;
+    IterState st;
+    if (publish) {
+      if ((rc = wait_published(c, A.seq, &st))) return rc;
+    } else {
+      HIP_TRY(hipMemcpyAsync(c->h_state, c->state, sizeof(IterState), hipMemcpyDeviceToHost, c->stream));
+      HIP_TRY(hipStreamSynchronize(c->stream));
+      st = *c->h_state;
+    }
     if (c->profile) {
       float ms_e = 0.f, ms_s = 0.f;
       HIP_TRY(hipEventElapsedTime(&ms_e, c->ev[0], c->ev[1]));
@@ -725,8 +768,8 @@ int cuhre_integrate(cuhre_context* c, const cuhre_integrand* f, int dim, double 
       hipLaunchKernelGGL(k_split, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, c->stream, SA);
       if (c->profile) HIP_TRY(hipEventRecord(c->ev[2], c->stream));
       HIP_TRY(hipGetLastError());
-      HIP_TRY(hipStreamSynchronize(c->stream));
       if (c->profile) {
+        HIP_TRY(hipStreamSynchronize(c->stream));
         float ms_s = 0.f;
         HIP_TRY(hipEventElapsedTime(&ms_s, c->ev[1], c->ev[2]));
         out->ms_split += ms_s;
@@ -767,6 +810,7 @@ int cuhre_integrate(cuhre_context* c, const cuhre_integrand* f, int dim, double 
     n = 2 * st.tot.nactive;
     cur = 1 - cur;
   }
+  HIP_TRY(hipStreamSynchronize(c->stream));
   out->fail = result > 0 ? result : 0;
   out->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return result;

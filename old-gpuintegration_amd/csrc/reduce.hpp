@@ -197,6 +197,11 @@ __device__ __forceinline__ void finish_iteration(const EvalArgs& A, Partial& p, 
     st.stop = stop;
     st.pad = 0;
     *A.state = st;
+    if (A.host_state != nullptr) {
+      // publish to the host, which polls host_seq instead of synchronising
+      *A.host_state = st;
+      __hip_atomic_store(A.host_seq, A.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
     // reset for the next launch (stream order makes this visible to it)
     __hip_atomic_store(A.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
