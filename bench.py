@@ -128,6 +128,42 @@ def bench_batched(ctx, count=4096):
             "per_instance_regions_min_max": [min(r["nregions"] for r in res), max(r["nregions"] for r in res)]}
 
 
+def bench_learned(ctx, samples_per_run=1_000_000, maxeval=10**11):
+    """Learned split guide on 7-D corner peaks (epsrel 1e-6): sample
+    collection from plain runs on training seeds 100..115, on-GPU training,
+    then guided vs plain on the held-out seeds 6, 7, 8."""
+    import cuhre
+    g = cuhre.Guide(ctx, seed=11)
+    t0 = time.perf_counter()
+    for p in W.c5_train_problems(100, 16):
+        g.collect(p, max_samples=samples_per_run, maxeval=maxeval)
+    t_collect = time.perf_counter() - t0
+    train = [g.train(7, epochs=ep, batch=8192, lr=lr, seed=ep) for ep, lr in ((3, 2e-3), (3, 1e-3), (2, 3e-4))]
+    policies = {
+        "plain": None,
+        "learned_axis": dict(axis_mode=1, finish_mode=0, axis_confidence=0.6),
+        "learned_axis_and_finish": dict(axis_mode=1, finish_mode=1, axis_confidence=0.6, budget=0.25),
+        "volume_guard": dict(axis_mode=0, finish_mode=2, budget=0.25),
+        "shared_allotment": dict(axis_mode=0, finish_mode=3, budget=0.5),
+    }
+    held = []
+    for seed in (6, 7, 8):
+        p = W.c5_corner7(seed)
+        row = {"seed": seed}
+        for name, pol in policies.items():
+            best = None
+            for _ in range(2):
+                r = ctx.integrate(p, maxeval=maxeval) if pol is None else g.integrate(p, maxeval=maxeval, **pol)
+                if best is None or r["ms_total"] < best["ms_total"]:
+                    best = r
+            row[name] = {"rc": best["rc"], "nregions": best["nregions"], "ms": best["ms_total"],
+                         "integral": best["integral"], "error": best["error"],
+                         "retired": best.get("guide", {}).get("retired", 0)}
+        held.append(row)
+    return {"samples": g.samples(), "collect_s": t_collect, "train": train, "held_out": held,
+            "maxeval": maxeval}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -136,6 +172,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batched", action="store_true",
                     help="also time the 4096-integral batched form against sequential plain calls")
+    ap.add_argument("--learned", action="store_true",
+                    help="also train the split guide and compare it with the plain method on 7-D corner peaks")
     ap.add_argument("--converge", action="store_true",
                     help="also run the product peak with maxeval 1e12 to reach epsrel 1e-6")
     args = ap.parse_args()
@@ -225,6 +263,8 @@ def main():
                                                   "iterations", "peak_regions")}}
     if args.batched:
         out["batched"] = bench_batched(ctx)
+    if args.learned:
+        out["learned"] = bench_learned(ctx)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(problems)
         # the CPU leg also holds the closed forms: report each GPU result's

@@ -216,6 +216,7 @@ typedef struct {
 typedef struct {
   int64_t retired;          /* regions retired early by the finish rule        */
   int64_t samples;          /* samples logged (collection runs)                */
+  int64_t overridden;       /* split axes the network changed                  */
   int32_t guide_iterations;
   int32_t pad;
 } cuhre_guide_stats;

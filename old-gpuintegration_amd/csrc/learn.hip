@@ -126,6 +126,15 @@ __device__ __forceinline__ f32x4 mlp_dw_tile(const uint16_t* Hs, int K, const ui
   return acc;
 }
 
+// SplitMix64 finaliser: sample hashes for logging and axis relabelling.
+__host__ __device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
 struct TrainArgs {
   MlpDev M;
   const float* feat;     // samples x 32
@@ -138,6 +147,7 @@ struct TrainArgs {
   float* part;           // blocks x kMlpParams partial gradients
   float* loss;           // rows
   int8_t* correct;       // rows: bit 0 axis argmax right, bit 1 finish right
+  uint64_t perm_seed;    // 0: samples as logged; else a random axis relabelling per row
 };
 
 // Forward + backward of 64 rows per block; the block's weight gradient is
@@ -149,10 +159,47 @@ __global__ __launch_bounds__(256) void k_mlp_train(TrainArgs T)
   __shared__ float Y[kMlpRows * kMlpOut];
   __shared__ __align__(16) uint16_t dZ4[kMlpRows * kMlpOut];
   __shared__ __align__(16) uint16_t dZ3[kMlpRows * kMlpHid], dZ2[kMlpRows * kMlpHid], dZ1[kMlpRows * kMlpHid];
+  __shared__ int8_t s_perm[kMlpRows][8];  // new axis j reads logged axis s_perm[r][j]
+  __shared__ int s_label[kMlpRows];
   const int64_t r0 = (int64_t)blockIdx.x * kMlpRows;
+  if (threadIdx.x < kMlpRows) {
+    // Axis relabelling (the rule is symmetric under permuting the axes, so
+    // is the label): position, width and fourth-difference features of axis
+    // j come from axis perm[j], the label moves with them.  Fisher-Yates
+    // from a per-sample hash.
+    const int rr = threadIdx.x;
+    const int64_t r = r0 + rr;
+    int8_t* pm = s_perm[rr];
+    for (int j = 0; j < 8; ++j) pm[j] = (int8_t)j;
+    int lab = 0;
+    if (r < T.rows) {
+      const int64_t smp = T.order[r];
+      lab = T.axis[smp];
+      if (T.perm_seed != 0) {
+        uint64_t h = mix64(T.perm_seed ^ (uint64_t)smp * 0x9E3779B97F4A7C15ull);
+        for (int j = T.dim - 1; j > 0; --j) {
+          const int k = (int)(h % (uint64_t)(j + 1));
+          h = mix64(h);
+          const int8_t t = pm[j];
+          pm[j] = pm[k];
+          pm[k] = t;
+        }
+        for (int j = 0; j < T.dim; ++j)
+          if (pm[j] == lab) {
+            lab = j;
+            break;
+          }
+      }
+    }
+    s_label[rr] = lab;
+  }
+  __syncthreads();
   for (int e = threadIdx.x; e < kMlpRows * kMlpIn; e += blockDim.x) {
-    const int64_t r = r0 + e / kMlpIn;
-    X[e] = r < T.rows ? f2bf(T.feat[(int64_t)T.order[r] * kMlpIn + e % kMlpIn]) : (uint16_t)0;
+    const int rr = e / kMlpIn;
+    const int64_t r = r0 + rr;
+    int q = e % kMlpIn;
+    if (q < 24 && (q & 7) < T.dim) q = (q & ~7) | s_perm[rr][q & 7];
+    X[e] = r < T.rows ? f2bf(T.feat[(int64_t)T.order[r] * kMlpIn + q]) : (uint16_t)0;
   }
   __syncthreads();
   const int w = threadIdx.x >> 6;
@@ -167,7 +214,7 @@ __global__ __launch_bounds__(256) void k_mlp_train(TrainArgs T)
     if (r < T.rows) {
       const int64_t s = T.order[r];
       const float* y = Y + rr * kMlpOut;
-      const int a = T.axis[s];
+      const int a = s_label[rr];
       const float f = T.finish[s];
       float m = y[0];
       int am = 0;
@@ -281,6 +328,8 @@ struct GuideArgs {
   double absI;  // |current estimate of the integral|
   double epsrel, epsabs;
   double tol;   // max(epsrel * absI, epsabs)
+  const IterState* state;  // when set: absI and tol come from this iteration's state,
+                           // and a stopped iteration (stop != 0) skips the guide step
   double loc_epsrel, loc_epsabs;  // the per-region test (0, 0 under the shared allotment)
   int axis_mode;    // 0: fourth difference, 1: guide when confident
   int finish_mode;  // 0: none, 1: guide proposal + volume-share guard, 2: guard alone,
@@ -299,17 +348,9 @@ struct GuideArgs {
 constexpr int kRetStride = 8;
 
 struct GuideSums {
-  int64_t n_split, n_retired, n_logged, pad;
+  int64_t n_split, n_retired, n_logged, n_overridden;
   double est_s, est_c, err_s, err_c, abs_r, aest_r;
 };
-
-__device__ __forceinline__ uint64_t mix64(uint64_t z)
-{
-  z += 0x9E3779B97F4A7C15ull;
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-  return z ^ (z >> 31);
-}
 
 __device__ __forceinline__ bool guide_selected(const GuideArgs& G, int64_t r, bool active)
 {
@@ -317,7 +358,7 @@ __device__ __forceinline__ bool guide_selected(const GuideArgs& G, int64_t r, bo
 }
 
 // The 32 input features of region r (25 used at d = 7) and its relative volume.
-__device__ __forceinline__ double guide_features(const GuideArgs& G, int64_t r, float* f)
+__device__ __forceinline__ double guide_features(const GuideArgs& G, double absI, int64_t r, float* f)
 {
   const int d = G.dim;
   double vrel = 1.0, maxd = 0.0;
@@ -333,7 +374,7 @@ __device__ __forceinline__ double guide_features(const GuideArgs& G, int64_t r, 
   }
   for (int m = 0; m < d; ++m) f[16 + m] = maxd > 0.0 ? (float)(G.dvec[m * G.ld + r] / maxd) : 0.f;
   const double e = G.est[r], er = G.err[r], i5 = G.i5[r];
-  const double scale = G.absI * vrel;
+  const double scale = absI * vrel;
   f[24] = (float)(log10(fabs(e) / scale + 1e-30) / 8.0);
   f[25] = (float)(log10(er / (G.epsrel * scale) + 1e-30) / 8.0);
   f[26] = (float)((e - i5) / (fabs(e) + fabs(i5) + 1e-300));
@@ -352,8 +393,14 @@ __global__ __launch_bounds__(MLP ? 256 : 64) void k_guide_apply(GuideArgs G)
   __shared__ __align__(16) uint16_t H1[kMlpRows * kMlpHid * kL + 1], H2[kMlpRows * kMlpHid * kL + 1],
       H3[kMlpRows * kMlpHid * kL + 1];
   __shared__ float Y[kMlpRows * kMlpOut * kL + 1];
-  __shared__ int s_split[kMlpRows], s_log[kMlpRows], s_ret[kMlpRows];
+  __shared__ int s_split[kMlpRows], s_log[kMlpRows], s_ret[kMlpRows], s_ovr[kMlpRows];
   __shared__ double s_e[kMlpRows], s_er[kMlpRows], s_ab[kMlpRows];
+  if (G.state != nullptr && G.state->stop != 0) return;  // block-uniform
+  double absI = G.absI, tol = G.tol;
+  if (G.state != nullptr) {
+    absI = fabs(G.state->rG_s + G.state->rG_c);
+    tol = fmax(G.epsrel * absI, G.epsabs);
+  }
   const int t = threadIdx.x;
   const int64_t r = (int64_t)blockIdx.x * kMlpRows + t;
   bool active = false;
@@ -365,7 +412,7 @@ __global__ __launch_bounds__(MLP ? 256 : 64) void k_guide_apply(GuideArgs G)
       const double e = G.est[r], er = G.err[r];
       active = isfinite(e) && isfinite(er) && region_active(e, er, G.loc_epsrel, G.loc_epsabs);
       if (MLP) {
-        vrel = guide_features(G, r, f);
+        vrel = guide_features(G, absI, r, f);
       } else if (G.finish_mode >= 2) {
         vrel = 1.0;
         for (int m = 0; m < G.dim; ++m) vrel *= G.width[m * G.ld + r] / (G.highs[m] - G.lows[m]);
@@ -381,10 +428,10 @@ __global__ __launch_bounds__(MLP ? 256 : 64) void k_guide_apply(GuideArgs G)
                   Y + w * 16 * kMlpOut);
   }
   if (t < kMlpRows) {
-    int retire = 0;
+    int retire = 0, ovr = 0;
     if (active) {
       const double e = G.est[r], er = G.err[r];
-      const bool guard = er <= G.budget * G.tol * vrel;
+      const bool guard = er <= G.budget * tol * vrel;
       if (MLP) {
         const float* y = Y + t * kMlpOut;
         float m = y[0];
@@ -397,14 +444,17 @@ __global__ __launch_bounds__(MLP ? 256 : 64) void k_guide_apply(GuideArgs G)
         float se = 0.f;
         for (int j = 0; j < G.dim; ++j) se += expf(y[j] - m);
         const float conf = 1.f / se;  // softmax probability of the argmax
-        if (G.axis_mode == 1 && conf >= G.axis_conf) G.axis[r] = (int8_t)am;
+        if (G.axis_mode == 1 && conf >= G.axis_conf && G.axis[r] != am) {
+          G.axis[r] = (int8_t)am;
+          ovr = 1;
+        }
         const float q = 1.f / (1.f + expf(-y[kMlpFinish]));
         if (G.finish_mode == 1 && q >= G.finish_thr && guard) retire = 1;
       }
       if (G.finish_mode == 2 && guard) retire = 1;
       if (G.finish_mode == 3) {
         // shared allotment: a (1 - budget) share of tol by value, a budget share by volume
-        const double a = G.epsrel * ((1.0 - G.budget) * fabs(e) + G.budget * G.absI * vrel);
+        const double a = G.epsrel * ((1.0 - G.budget) * fabs(e) + G.budget * absI * vrel);
         if (er <= fmax(a, G.epsabs)) retire = 1;
       }
       s_e[t] = e;
@@ -415,16 +465,18 @@ __global__ __launch_bounds__(MLP ? 256 : 64) void k_guide_apply(GuideArgs G)
     }
     if (r < G.n) G.retire[r] = (int8_t)retire;
     s_ret[t] = retire;
+    s_ovr[t] = ovr;
     s_split[t] = active && !retire ? 1 : 0;
     s_log[t] = guide_selected(G, r, active) ? 1 : 0;
   }
   __syncthreads();
   if (t == 0) {
-    int64_t ns = 0, nl = 0, nr = 0;
+    int64_t ns = 0, nl = 0, nr = 0, no = 0;
     double es = 0, ec = 0, rs = 0, rc = 0, ab = 0, ae = 0;
     for (int j = 0; j < kMlpRows; ++j) {
       ns += s_split[j];
       nl += s_log[j];
+      no += s_ovr[j];
       if (s_ret[j]) {
         ++nr;
         dd_add(es, ec, s_e[j], 0.0);
@@ -443,50 +495,120 @@ __global__ __launch_bounds__(MLP ? 256 : 64) void k_guide_apply(GuideArgs G)
     tr[4] = ab;
     tr[5] = ae;
     tr[6] = (double)nr;
-    tr[7] = 0.0;
+    tr[7] = (double)no;
   }
 }
 
-// One block: exclusive scans of the split / logged counts per tile and the
-// retired sums in tile order.  The iteration's retirements are honoured
-// (*retire_on = 1) only when their summed error fits in `room` (read from
-// room_dev when given); otherwise the
-// offsets count every active region and the retired sums are zero.
-__global__ __launch_bounds__(256) void k_guide_scan(const int64_t* cnt, const int64_t* logcnt, const double* ret,
-                                                    int64_t ntiles, double room, const double* room_dev,
-                                                    int64_t* off, int64_t* logoff, GuideSums* out, int* retire_on)
+// Retirement room of one iteration: computed on the device from the
+// iteration's state when `state` is set, else `room` as given.
+struct RoomArgs {
+  const IterState* state;
+  double room;
+  double frozen_prev;  // finished + retired error before this iteration
+  double ret_prev;     // retired error before this iteration
+  double budget, epsrel, epsabs;
+  int mode;            // finish mode
+};
+
+// Scan of the per-tile counts and retired sums, in three launches over
+// chunks of kScanChunk tiles (every sum in a fixed order, so the result does
+// not depend on scheduling):
+//   k_gscan_part  per chunk: tile counts and retired sums;
+//   k_gscan_top   one block: the retirement decision (honoured, *retire_on
+//                 = 1, only when the retired error fits in the room; else
+//                 every active region is split and the retired sums are
+//                 zero), chunk offsets and the iteration's totals;
+//   k_gscan_tiles per chunk: tile offsets = chunk offset + scan in chunk.
+constexpr int kScanPer = 4;                   // tiles per thread
+constexpr int kScanChunk = 256 * kScanPer;    // tiles per chunk
+
+struct ChunkPart {
+  int64_t cnt, all, log, pad;  // split with / without retirements, logged
+  double es, ec, rs, rc, ab, ae, nr, no;
+};
+
+__device__ __forceinline__ void cp_zero(ChunkPart& q)
 {
-  if (room_dev != nullptr) room = *room_dev;
-  __shared__ int64_t sa[256], sb[256], sc[256];
-  __shared__ double sd[256][7];
-  __shared__ int s_allow;
-  const int T = blockDim.x;
-  const int64_t chunk = (ntiles + T - 1) / T;
-  const int64_t t0 = threadIdx.x * chunk;
-  const int64_t t1 = t0 + chunk < ntiles ? t0 + chunk : ntiles;
-  int64_t ma = 0, mb = 0, mc = 0;
-  double es = 0, ec = 0, rs = 0, rc = 0, ab = 0, ae = 0, nr = 0;
-  for (int64_t t = t0; t < t1; ++t) {
-    const double* tr = ret + t * kRetStride;
-    ma += cnt[t];
-    mb += logcnt[t];
-    mc += cnt[t] + (int64_t)tr[6];
-    dd_add(es, ec, tr[0], tr[1]);
-    dd_add(rs, rc, tr[2], tr[3]);
-    ab += tr[4];
-    ae += tr[5];
-    nr += tr[6];
+  q.cnt = q.all = q.log = q.pad = 0;
+  q.es = q.ec = q.rs = q.rc = q.ab = q.ae = q.nr = q.no = 0.0;
+}
+
+__device__ __forceinline__ void cp_add(ChunkPart& a, const ChunkPart& b)
+{
+  a.cnt += b.cnt;
+  a.all += b.all;
+  a.log += b.log;
+  dd_add(a.es, a.ec, b.es, b.ec);
+  dd_add(a.rs, a.rc, b.rs, b.rc);
+  a.ab += b.ab;
+  a.ae += b.ae;
+  a.nr += b.nr;
+  a.no += b.no;
+}
+
+__device__ __forceinline__ void cp_tile(ChunkPart& q, const int64_t* cnt, const int64_t* logcnt, const double* ret,
+                                        int64_t t)
+{
+  const double* tr = ret + t * kRetStride;
+  q.cnt += cnt[t];
+  q.all += cnt[t] + (int64_t)tr[6];
+  q.log += logcnt[t];
+  dd_add(q.es, q.ec, tr[0], tr[1]);
+  dd_add(q.rs, q.rc, tr[2], tr[3]);
+  q.ab += tr[4];
+  q.ae += tr[5];
+  q.nr += tr[6];
+  q.no += tr[7];
+}
+
+__device__ __forceinline__ bool scan_stopped(const IterState* st) { return st != nullptr && st->stop != 0; }
+
+__global__ __launch_bounds__(256) void k_gscan_part(const int64_t* cnt, const int64_t* logcnt, const double* ret,
+                                                    int64_t ntiles, const IterState* state, ChunkPart* part)
+{
+  if (scan_stopped(state)) return;
+  __shared__ ChunkPart sp[256];
+  ChunkPart q;
+  cp_zero(q);
+  const int64_t t0 = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * kScanPer;
+  for (int k = 0; k < kScanPer; ++k)
+    if (t0 + k < ntiles) cp_tile(q, cnt, logcnt, ret, t0 + k);
+  sp[threadIdx.x] = q;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) cp_add(sp[threadIdx.x], sp[threadIdx.x + o]);
+    __syncthreads();
   }
-  sa[threadIdx.x] = ma;
-  sb[threadIdx.x] = mb;
-  sc[threadIdx.x] = mc;
-  sd[threadIdx.x][0] = es;
-  sd[threadIdx.x][1] = ec;
-  sd[threadIdx.x][2] = rs;
-  sd[threadIdx.x][3] = rc;
-  sd[threadIdx.x][4] = ab;
-  sd[threadIdx.x][5] = ae;
-  sd[threadIdx.x][6] = nr;
+  if (threadIdx.x == 0) part[blockIdx.x] = sp[0];
+}
+
+__global__ __launch_bounds__(256) void k_gscan_top(const ChunkPart* part, int64_t nchunks, RoomArgs RA,
+                                                   int64_t* choff, int64_t* chlog, GuideSums* out, int* retire_on)
+{
+  if (scan_stopped(RA.state)) return;
+  __shared__ ChunkPart sp[256];
+  __shared__ int64_t sa[256], sb[256], sc[256];
+  __shared__ int s_allow;
+  double room = RA.room;
+  if (RA.state != nullptr) {
+    const IterState& st = *RA.state;
+    const double tol = fmax(RA.epsrel * fabs(st.rG_s + st.rG_c), RA.epsabs);
+    const double frozen = RA.frozen_prev + (st.tot.ferr_s + st.tot.ferr_c);
+    room = RA.mode == 3   ? 1e300
+           : RA.mode != 0 ? fmin(RA.budget * tol - RA.ret_prev, (1.0 - RA.budget) * tol - frozen)
+                          : -1.0;
+  }
+  const int T = blockDim.x;
+  const int64_t per = (nchunks + T - 1) / T;
+  const int64_t c0 = threadIdx.x * per;
+  const int64_t c1 = c0 + per < nchunks ? c0 + per : nchunks;
+  ChunkPart q;
+  cp_zero(q);
+  for (int64_t c = c0; c < c1; ++c) cp_add(q, part[c]);
+  sp[threadIdx.x] = q;
+  sa[threadIdx.x] = q.cnt;
+  sb[threadIdx.x] = q.all;
+  sc[threadIdx.x] = q.log;
   __syncthreads();
   for (int o = 1; o < T; o <<= 1) {
     const int64_t va = (int)threadIdx.x >= o ? sa[threadIdx.x - o] : 0;
@@ -499,29 +621,23 @@ __global__ __launch_bounds__(256) void k_guide_scan(const int64_t* cnt, const in
     __syncthreads();
   }
   if (threadIdx.x == 0) {
+    ChunkPart tot = sp[0];
+    for (int i = 1; i < T; ++i) cp_add(tot, sp[i]);
+    const int allow = tot.nr > 0.0 && tot.rs + tot.rc <= room;
     GuideSums g;
-    double a0 = 0, a1 = 0, b0 = 0, b1 = 0, c = 0, e = 0, n = 0;
-    for (int i = 0; i < T; ++i) {
-      dd_add(a0, a1, sd[i][0], sd[i][1]);
-      dd_add(b0, b1, sd[i][2], sd[i][3]);
-      c += sd[i][4];
-      e += sd[i][5];
-      n += sd[i][6];
-    }
-    const int allow = n > 0.0 && b0 + b1 <= room;
-    g.n_logged = sb[T - 1];
-    g.pad = 0;
+    g.n_logged = tot.log;
+    g.n_overridden = (int64_t)tot.no;
     if (allow) {
-      g.n_split = sa[T - 1];
-      g.est_s = a0;
-      g.est_c = a1;
-      g.err_s = b0;
-      g.err_c = b1;
-      g.abs_r = c;
-      g.aest_r = e;
-      g.n_retired = (int64_t)n;
+      g.n_split = tot.cnt;
+      g.est_s = tot.es;
+      g.est_c = tot.ec;
+      g.err_s = tot.rs;
+      g.err_c = tot.rc;
+      g.abs_r = tot.ab;
+      g.aest_r = tot.ae;
+      g.n_retired = (int64_t)tot.nr;
     } else {
-      g.n_split = sc[T - 1];
+      g.n_split = tot.all;
       g.est_s = g.est_c = g.err_s = g.err_c = g.abs_r = g.aest_r = 0.0;
       g.n_retired = 0;
     }
@@ -531,13 +647,54 @@ __global__ __launch_bounds__(256) void k_guide_scan(const int64_t* cnt, const in
   }
   __syncthreads();
   const int allow = s_allow;
-  int64_t ra = allow ? sa[threadIdx.x] - ma : sc[threadIdx.x] - mc;
-  int64_t rb = sb[threadIdx.x] - mb;
-  for (int64_t t = t0; t < t1; ++t) {
-    off[t] = ra;
-    logoff[t] = rb;
-    ra += allow ? cnt[t] : cnt[t] + (int64_t)ret[t * kRetStride + 6];
-    rb += logcnt[t];
+  int64_t ra = allow ? sa[threadIdx.x] - q.cnt : sb[threadIdx.x] - q.all;
+  int64_t rb = sc[threadIdx.x] - q.log;
+  for (int64_t c = c0; c < c1; ++c) {
+    choff[c] = ra;
+    chlog[c] = rb;
+    ra += allow ? part[c].cnt : part[c].all;
+    rb += part[c].log;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_gscan_tiles(const int64_t* cnt, const int64_t* logcnt, const double* ret,
+                                                     int64_t ntiles, const IterState* state, const int64_t* choff,
+                                                     const int64_t* chlog, const int* retire_on, int64_t* off,
+                                                     int64_t* logoff)
+{
+  if (scan_stopped(state)) return;
+  __shared__ int64_t sa[256], sb[256];
+  const int allow = *retire_on;
+  const int64_t t0 = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * kScanPer;
+  int64_t ma = 0, mb = 0;
+  for (int k = 0; k < kScanPer; ++k) {
+    const int64_t t = t0 + k;
+    if (t < ntiles) {
+      ma += allow ? cnt[t] : cnt[t] + (int64_t)ret[t * kRetStride + 6];
+      mb += logcnt[t];
+    }
+  }
+  sa[threadIdx.x] = ma;
+  sb[threadIdx.x] = mb;
+  __syncthreads();
+  for (int o = 1; o < 256; o <<= 1) {
+    const int64_t va = (int)threadIdx.x >= o ? sa[threadIdx.x - o] : 0;
+    const int64_t vb = (int)threadIdx.x >= o ? sb[threadIdx.x - o] : 0;
+    __syncthreads();
+    sa[threadIdx.x] += va;
+    sb[threadIdx.x] += vb;
+    __syncthreads();
+  }
+  int64_t ra = choff[blockIdx.x] + sa[threadIdx.x] - ma;
+  int64_t rb = chlog[blockIdx.x] + sb[threadIdx.x] - mb;
+  for (int k = 0; k < kScanPer; ++k) {
+    const int64_t t = t0 + k;
+    if (t < ntiles) {
+      off[t] = ra;
+      logoff[t] = rb;
+      ra += allow ? cnt[t] : cnt[t] + (int64_t)ret[t * kRetStride + 6];
+      rb += logcnt[t];
+    }
   }
 }
 
@@ -571,7 +728,7 @@ __global__ __launch_bounds__(64) void k_collect_write(CollectArgs C)
   if (!sel) return;
   const int64_t j = C.logoff[blockIdx.x] + __popcll(mask & below);
   float f[32];
-  const double vrel = guide_features(G, r, f);
+  const double vrel = guide_features(G, G.absI, r, f);
   const int64_t s = C.base + j;
   for (int q = 0; q < 32; ++q) C.feat[s * 32 + q] = f[q];
   C.s_err[s] = G.err[r];
@@ -665,6 +822,8 @@ struct cuhre_guide {
   double* tile_ret = nullptr;
   GuideSums* d_sums = nullptr;
   int* d_retire_on = nullptr;  // the scan's verdict on this iteration's retirements
+  ChunkPart* chunk_part = nullptr;  // scan chunks: partial sums and offsets
+  int64_t *chunk_off = nullptr, *chunk_log = nullptr;
   GuideSums* h_sums = nullptr;
   // training scratch
   int64_t train_cap = 0;
@@ -691,12 +850,19 @@ int guide_reserve_regions(cuhre_guide* g, int64_t cap)
   gfree(g->tile_logcnt);
   gfree(g->tile_logoff);
   gfree(g->tile_ret);
+  gfree(g->chunk_part);
+  gfree(g->chunk_off);
+  gfree(g->chunk_log);
   const int64_t nt = cap / kMlpRows + 1;
+  const int64_t nc = nt / kScanChunk + 1;
   if (hipMalloc(&g->retire, cap) != hipSuccess || hipMalloc(&g->tile_cnt, sizeof(int64_t) * nt) != hipSuccess ||
       hipMalloc(&g->tile_off, sizeof(int64_t) * nt) != hipSuccess ||
       hipMalloc(&g->tile_logcnt, sizeof(int64_t) * nt) != hipSuccess ||
       hipMalloc(&g->tile_logoff, sizeof(int64_t) * nt) != hipSuccess ||
-      hipMalloc(&g->tile_ret, sizeof(double) * kRetStride * nt) != hipSuccess) {
+      hipMalloc(&g->tile_ret, sizeof(double) * kRetStride * nt) != hipSuccess ||
+      hipMalloc(&g->chunk_part, sizeof(ChunkPart) * nc) != hipSuccess ||
+      hipMalloc(&g->chunk_off, sizeof(int64_t) * nc) != hipSuccess ||
+      hipMalloc(&g->chunk_log, sizeof(int64_t) * nc) != hipSuccess) {
     (void)hipGetLastError();
     g->reg_cap = 0;
     return set_error(CUHRE_ERR_ALLOC, "cannot allocate guide buffers");
@@ -836,9 +1002,10 @@ int reserve_train(cuhre_guide* g, int64_t rows, int64_t order_len)
 
 // One optimisation step (or gradient only) over `rows` sample indices.
 void train_step(cuhre_guide* g, const float* feat, const int8_t* ax, const float* fin, const int32_t* order,
-                int64_t rows, int dim, float lr, bool update, hipStream_t st)
+                int64_t rows, int dim, float lr, bool update, hipStream_t st, uint64_t perm_seed = 0)
 {
   TrainArgs T;
+  T.perm_seed = perm_seed;
   T.M = mlp_dev(g);
   T.feat = feat;
   T.axis = ax;
@@ -920,10 +1087,105 @@ int guided_run(cuhre_context* c, cuhre_guide* g, const cuhre_integrand* f, int d
     A.max_active = std::min<int64_t>(left >= 0 ? left / (2 * p) : -1, capacity / 2);
     A.alloc_active = c->cap / 2;
     find_eval_launcher(f->family, dim)(A, P, R, S, st);
-    if (hipGetLastError() != hipSuccess ||
-        hipMemcpyAsync(c->h_state, c->state, sizeof(IterState), hipMemcpyDeviceToHost, st) != hipSuccess ||
+    if (hipGetLastError() != hipSuccess) return set_error(CUHRE_ERR_DEVICE, "guided evaluation failed");
+
+    // The guide step (network, finish rule, scan) and the split.  With
+    // `dev` they read this iteration's estimate from the device state and
+    // skip themselves when the evaluation stopped the run, so they follow
+    // the evaluation without a host round trip; otherwise they take the
+    // host's values (after a list growth, and in sampling runs, whose
+    // buffers are sized on the host).
+    const int64_t ntiles = (n + kMlpRows - 1) / kMlpRows;
+    GuideArgs G;
+    RoomArgs RA;
+    auto guide_step = [&](bool dev) {
+      std::memset(&G, 0, sizeof G);
+      G.lower = c->lo[cur];
+      G.width = c->wd[cur];
+      G.ld = c->cap;
+      G.n = n;
+      G.est = c->est;
+      G.err = c->err;
+      G.i5 = c->aux;
+      G.dvec = c->aux + c->cap;
+      G.abssum = c->abssum;
+      G.axis = c->axis;
+      G.retire = g->retire;
+      G.dim = dim;
+      for (int m = 0; m < dim; ++m) {
+        G.lows[m] = lows[m];
+        G.highs[m] = highs[m];
+      }
+      G.absI = std::fabs(out->integral);
+      G.epsrel = epsrel;
+      G.epsabs = epsabs;
+      G.tol = std::max(epsrel * G.absI, epsabs);
+      G.state = dev ? c->state : nullptr;
+      G.loc_epsrel = loc_rel;
+      G.loc_epsabs = loc_abs;
+      G.axis_mode = run.axis_mode;
+      G.finish_mode = run.finish_mode;
+      G.use_mlp = run.use_mlp;
+      G.axis_conf = run.axis_conf;
+      G.finish_thr = run.finish_thr;
+      G.budget = run.budget;
+      G.M = mlp_dev(g);
+      G.tile_cnt = g->tile_cnt;
+      G.tile_logcnt = g->tile_logcnt;
+      G.tile_ret = g->tile_ret;
+      const int64_t want = run.collect_max - logged;
+      const int64_t nact = c->h_state->tot.nactive;  // read only when !dev (synchronised)
+      if (!dev && run.collect_max > 0 && want > 0 && nact > 0) {
+        // about collect_max / 24 regions per iteration, so that the deep
+        // iterations of a long run are represented as well as the early ones
+        const int64_t target =
+            std::max<int64_t>(1, std::min<int64_t>(want, std::max<int64_t>(run.collect_max / 24, 4096)));
+        G.log_stride = (uint64_t)std::max<int64_t>(1, (nact + target - 1) / target);
+        G.log_salt = 0x5DEECE66Dull * (uint64_t)(it + 1);
+      }
+      if (run.use_mlp)
+        hipLaunchKernelGGL(k_guide_apply<true>, dim3((unsigned)ntiles), dim3(256), 0, st, G);
+      else
+        hipLaunchKernelGGL(k_guide_apply<false>, dim3((unsigned)ntiles), dim3(64), 0, st, G);
+      // retirement room: retired error within budget * tol, frozen error
+      // (finished so far, finished now, retired) within (1 - budget) * tol
+      RA.state = G.state;
+      RA.frozen_prev = fer_s + fer_c;
+      RA.ret_prev = ret_err;
+      RA.budget = run.budget;
+      RA.epsrel = epsrel;
+      RA.epsabs = epsabs;
+      RA.mode = run.finish_mode;
+      RA.room = -1.0;
+      if (!dev) {
+        const IterState& hs = *c->h_state;
+        const double frozen = RA.frozen_prev + (hs.tot.ferr_s + hs.tot.ferr_c);
+        RA.room = run.finish_mode == 3 ? 1e300
+                  : run.finish_mode    ? std::min(run.budget * G.tol - ret_err, (1.0 - run.budget) * G.tol - frozen)
+                                       : -1.0;
+      }
+      const int64_t nchunks = (ntiles + kScanChunk - 1) / kScanChunk;
+      hipLaunchKernelGGL(k_gscan_part, dim3((unsigned)nchunks), dim3(256), 0, st, g->tile_cnt, g->tile_logcnt,
+                         g->tile_ret, ntiles, RA.state, g->chunk_part);
+      hipLaunchKernelGGL(k_gscan_top, dim3(1), dim3(256), 0, st, g->chunk_part, nchunks, RA, g->chunk_off,
+                         g->chunk_log, g->d_sums, g->d_retire_on);
+      hipLaunchKernelGGL(k_gscan_tiles, dim3((unsigned)nchunks), dim3(256), 0, st, g->tile_cnt, g->tile_logcnt,
+                         g->tile_ret, ntiles, RA.state, g->chunk_off, g->chunk_log, g->d_retire_on, g->tile_off,
+                         g->tile_logoff);
+      SplitArgs SA;
+      fill_split_args(c, SA, cur, n, dim, loc_rel, loc_abs, kMlpRows, dev);
+      SA.offsets = g->tile_off;
+      SA.retire = g->retire;
+      SA.retire_on = g->d_retire_on;
+      launch_split(SA, st);
+    };
+    const bool sampling = run.collect_max > 0;
+    if (!sampling) guide_step(true);
+    if (hipMemcpyAsync(c->h_state, c->state, sizeof(IterState), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        (!sampling && hipMemcpyAsync(g->h_sums, g->d_sums, sizeof(GuideSums), hipMemcpyDeviceToHost, st) !=
+                          hipSuccess) ||
         hipStreamSynchronize(st) != hipSuccess)
-      return set_error(CUHRE_ERR_DEVICE, "guided evaluation failed");
+      return set_error(CUHRE_ERR_DEVICE, "guided iteration failed");
     IterState s = *c->h_state;
     out->eval_launches++;
     out->iterations = it + 1;
@@ -932,9 +1194,10 @@ int guided_run(cuhre_context* c, cuhre_guide* g, const cuhre_integrand* f, int d
     out->peak_regions = std::max(out->peak_regions, n);
     out->integral = s.rG_s + s.rG_c;
     out->error = s.eG_s + s.eG_c;
+    bool rerun = sampling;
     if (s.stop == 5) {
       // children would not fit: grow the lists (keeping this iteration) and
-      // the guide's per-region buffers
+      // the guide's per-region buffers, then run the guide step again
       const int64_t ncap = std::min<int64_t>(capacity, std::max<int64_t>(2 * s.tot.nactive, 2 * c->cap));
       const int64_t nb = blocks_for(n, S);
       if ((rc = ctx_reserve(c, ncap, dim, n, cur, nb))) return rc;
@@ -951,6 +1214,7 @@ int guided_run(cuhre_context* c, cuhre_guide* g, const cuhre_integrand* f, int d
         find_eval_launcher(f->family, dim)(A, P, R, S, st);
       }
       s.stop = 0;
+      rerun = true;
     }
     const double abs_contrib = fin_abs + s.tot.abs_all;
     const double aest_contrib = fin_aest + s.tot.aest_all;
@@ -968,72 +1232,12 @@ int guided_run(cuhre_context* c, cuhre_guide* g, const cuhre_integrand* f, int d
       result = CUHRE_NOT_CONVERGED;
       break;
     }
-    // guide step
-    GuideArgs G;
-    std::memset(&G, 0, sizeof G);
-    G.lower = c->lo[cur];
-    G.width = c->wd[cur];
-    G.ld = c->cap;
-    G.n = n;
-    G.est = c->est;
-    G.err = c->err;
-    G.i5 = c->aux;
-    G.dvec = c->aux + c->cap;
-    G.abssum = c->abssum;
-    G.axis = c->axis;
-    G.retire = g->retire;
-    G.dim = dim;
-    for (int m = 0; m < dim; ++m) {
-      G.lows[m] = lows[m];
-      G.highs[m] = highs[m];
+    if (rerun) {
+      guide_step(false);
+      if (hipMemcpyAsync(g->h_sums, g->d_sums, sizeof(GuideSums), hipMemcpyDeviceToHost, st) != hipSuccess ||
+          hipStreamSynchronize(st) != hipSuccess)
+        return set_error(CUHRE_ERR_DEVICE, "guide step failed");
     }
-    G.absI = std::fabs(out->integral);
-    G.epsrel = epsrel;
-    G.epsabs = epsabs;
-    G.tol = std::max(epsrel * G.absI, epsabs);
-    G.loc_epsrel = loc_rel;
-    G.loc_epsabs = loc_abs;
-    G.axis_mode = run.axis_mode;
-    G.finish_mode = run.finish_mode;
-    G.use_mlp = run.use_mlp;
-    G.axis_conf = run.axis_conf;
-    G.finish_thr = run.finish_thr;
-    G.budget = run.budget;
-    G.M = mlp_dev(g);
-    G.tile_cnt = g->tile_cnt;
-    G.tile_logcnt = g->tile_logcnt;
-    G.tile_ret = g->tile_ret;
-    const int64_t want = run.collect_max - logged;
-    if (run.collect_max > 0 && want > 0 && s.tot.nactive > 0) {
-      // sample about `want / 2` regions of this iteration (later iterations are larger)
-      const int64_t target = std::max<int64_t>(1, std::min<int64_t>(want, std::max<int64_t>(want / 2, 4096)));
-      G.log_stride = (uint64_t)std::max<int64_t>(1, (s.tot.nactive + target - 1) / target);
-      G.log_salt = 0x5DEECE66Dull * (uint64_t)(it + 1);
-    }
-    const int64_t ntiles = (n + kMlpRows - 1) / kMlpRows;
-    if (run.use_mlp)
-      hipLaunchKernelGGL(k_guide_apply<true>, dim3((unsigned)ntiles), dim3(256), 0, st, G);
-    else
-      hipLaunchKernelGGL(k_guide_apply<false>, dim3((unsigned)ntiles), dim3(64), 0, st, G);
-    // retirement room: retired error within budget * tol, frozen error
-    // (finished so far, finished now, retired) within (1 - budget) * tol
-    const double frozen = (fer_s + fer_c) + (s.tot.ferr_s + s.tot.ferr_c);
-    const double room =
-        run.finish_mode == 3 ? 1e300
-        : run.finish_mode    ? std::min(run.budget * G.tol - ret_err, (1.0 - run.budget) * G.tol - frozen)
-                             : -1.0;
-    hipLaunchKernelGGL(k_guide_scan, dim3(1), dim3(256), 0, st, g->tile_cnt, g->tile_logcnt, g->tile_ret, ntiles,
-                       room, nullptr, g->tile_off, g->tile_logoff,
-                       g->d_sums, g->d_retire_on);
-    SplitArgs SA;
-    fill_split_args(c, SA, cur, n, dim, loc_rel, loc_abs, kMlpRows, false);
-    SA.offsets = g->tile_off;
-    SA.retire = g->retire;
-    SA.retire_on = g->d_retire_on;
-    launch_split(SA, st);
-    if (hipMemcpyAsync(g->h_sums, g->d_sums, sizeof(GuideSums), hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess)
-      return set_error(CUHRE_ERR_DEVICE, "guide step failed");
     const GuideSums gsum = *g->h_sums;
     if (G.log_stride > 0 && gsum.n_logged > 0) {
       const int64_t L = std::min<int64_t>(gsum.n_logged, run.collect_max - logged);
@@ -1073,6 +1277,7 @@ int guided_run(cuhre_context* c, cuhre_guide* g, const cuhre_integrand* f, int d
     if (gs) {
       gs->retired += gsum.n_retired;
       gs->guide_iterations += 1;
+      gs->overridden += gsum.n_overridden;
     }
     out->split_launches++;
     dd_add_host(fin_s, fin_c, s.tot.fest_s, s.tot.fest_c);
@@ -1175,6 +1380,7 @@ int cuhre_guide_destroy(cuhre_guide* g)
                   (void*)g->ds_vrel, (void*)g->la_lo, (void*)g->la_wd, (void*)g->la_est, (void*)g->la_err,
                   (void*)g->la_ab, (void*)g->la_axis, (void*)g->retire, (void*)g->tile_cnt, (void*)g->tile_off,
                   (void*)g->tile_logcnt, (void*)g->tile_logoff, (void*)g->tile_ret, (void*)g->d_sums, (void*)g->d_retire_on,
+                  (void*)g->chunk_part, (void*)g->chunk_off, (void*)g->chunk_log,
                   (void*)g->order, (void*)g->loss, (void*)g->correct, (void*)g->d_stats})
     gfree(q);
   if (g->h_sums) (void)hipHostFree(g->h_sums);
@@ -1325,7 +1531,8 @@ int cuhre_guide_train(cuhre_guide* g, int dim, int epochs, int batch, float lr, 
     double acc_h[4] = {0, 0, 0, 0};
     (void)hipMemsetAsync(g->d_stats, 0, sizeof(double) * 4, st);
     for (int64_t k = 0; k < steps; ++k) {
-      train_step(g, g->ds_feat, g->ds_axis, g->ds_fin, g->order + k * batch, batch, dim, lr, true, st);
+      train_step(g, g->ds_feat, g->ds_axis, g->ds_fin, g->order + k * batch, batch, dim, lr, true, st,
+                 mix64(seed ^ (0xA0761D6478BD642Full * (uint64_t)(ep * steps + k + 1))));
       hipLaunchKernelGGL(k_train_stats, dim3(16), dim3(256), 0, st, g->loss, g->correct, (int64_t)batch, g->d_stats);
     }
     if (hipGetLastError() != hipSuccess ||

@@ -133,6 +133,7 @@ class GuideStats(C.Structure):
     _fields_ = [
         ("retired", C.c_int64),
         ("samples", C.c_int64),
+        ("overridden", C.c_int64),
         ("guide_iterations", C.c_int32),
         ("pad", C.c_int32),
     ]
