@@ -128,17 +128,24 @@ def bench_batched(ctx, count=4096):
             "per_instance_regions_min_max": [min(r["nregions"] for r in res), max(r["nregions"] for r in res)]}
 
 
-def bench_learned(ctx, samples_per_run=1_000_000, maxeval=10**11):
+def bench_learned(ctx, samples_per_run=500_000, maxeval=10**11, rounds=3):
     """Learned split guide on 7-D corner peaks (epsrel 1e-6): sample
-    collection from plain runs on training seeds 100..115, on-GPU training,
-    then guided vs plain on the held-out seeds 6, 7, 8."""
+    collection on training seeds 100..115 (round 0 under the plain axes,
+    later rounds under the network's own axes, every sample labelled by the
+    look-ahead), on-GPU training after each round, then guided vs plain on
+    the held-out seeds 6, 7, 8."""
     import cuhre
     g = cuhre.Guide(ctx, seed=11)
     t0 = time.perf_counter()
-    for p in W.c5_train_problems(100, 16):
-        g.collect(p, max_samples=samples_per_run, maxeval=maxeval)
-    t_collect = time.perf_counter() - t0
-    train = [g.train(7, epochs=ep, batch=8192, lr=lr, seed=ep) for ep, lr in ((3, 2e-3), (3, 1e-3), (2, 3e-4))]
+    train = []
+    for rnd in range(rounds):
+        for p in W.c5_train_problems(100, 16):
+            g.collect(p, max_samples=samples_per_run, maxeval=maxeval, axis_mode=0 if rnd == 0 else 1,
+                      axis_confidence=0.6)
+        for ep, lr in ((2, 2e-3), (2, 1e-3), (1, 3e-4)):
+            st = g.train(7, epochs=ep, batch=8192, lr=lr, seed=10 * rnd + ep)
+        train.append(dict(st, round=rnd))
+    t_learn = time.perf_counter() - t0
     policies = {
         "plain": None,
         "learned_axis": dict(axis_mode=1, finish_mode=0, axis_confidence=0.6),
@@ -158,17 +165,18 @@ def bench_learned(ctx, samples_per_run=1_000_000, maxeval=10**11):
                     best = r
             row[name] = {"rc": best["rc"], "nregions": best["nregions"], "ms": best["ms_total"],
                          "integral": best["integral"], "error": best["error"],
-                         "retired": best.get("guide", {}).get("retired", 0)}
+                         "retired": best.get("guide", {}).get("retired", 0),
+                         "overridden": best.get("guide", {}).get("overridden", 0)}
         held.append(row)
-    return {"samples": g.samples(), "collect_s": t_collect, "train": train, "held_out": held,
+    return {"samples": g.samples(), "collect_and_train_s": t_learn, "train": train, "held_out": held,
             "maxeval": maxeval}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batched", action="store_true",
                     help="also time the 4096-integral batched form against sequential plain calls")

@@ -238,10 +238,12 @@ int cuhre_guide_forward(cuhre_guide* guide, int64_t n, const float* features, fl
 /* gradient of the mean loss over n samples (no update) */
 int cuhre_guide_gradient(cuhre_guide* guide, int64_t n, const float* features, const int32_t* axis,
                          const float* finish, int dim, float* grad, double* loss);
-/* plain run that logs up to max_samples (features, look-ahead axis, finish) */
+/* run that logs up to max_samples (features, look-ahead axis, finish);
+ * pol NULL: plain axes; pol->axis_mode 1: the network's confident axes
+ * (only axis_mode and axis_confidence are read) */
 int cuhre_guide_collect(cuhre_guide* guide, const cuhre_integrand* f, int dim, double epsrel,
                         double epsabs, int64_t maxeval, const double* lows, const double* highs,
-                        int64_t max_samples, cuhre_result* out);
+                        int64_t max_samples, const cuhre_guide_policy* pol, cuhre_result* out);
 int64_t cuhre_guide_samples(const cuhre_guide* guide);
 int cuhre_guide_get_samples(cuhre_guide* guide, int64_t first, int64_t count, float* features,
                             int32_t* axis, float* finish);

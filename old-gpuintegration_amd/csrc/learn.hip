@@ -1491,11 +1491,16 @@ int cuhre_guide_get_samples(cuhre_guide* g, int64_t first, int64_t count, float*
 
 int cuhre_guide_collect(cuhre_guide* g, const cuhre_integrand* f, int dim, double epsrel, double epsabs,
                         int64_t maxeval, const double* lows, const double* highs, int64_t max_samples,
-                        cuhre_result* out)
+                        const cuhre_guide_policy* pol, cuhre_result* out)
 {
   if (guide_check(g)) return CUHRE_ERR_ARG;
   if (!out) return set_error(CUHRE_ERR_ARG, "null argument");
-  GuidedRun run{0, 0, 0, 1.f, 1.f, 0.0, max_samples};
+  // pol NULL: the plain method decides every axis; with axis_mode 1 the
+  // network decides where confident, so the log holds the regions its own
+  // choices produce (labelled by the look-ahead as always)
+  if (pol && (pol->axis_mode < 0 || pol->axis_mode > 1)) return set_error(CUHRE_ERR_ARG, "bad guide policy");
+  const int am = pol ? pol->axis_mode : 0;
+  GuidedRun run{am, 0, am, pol ? pol->axis_confidence : 1.f, 1.f, 0.0, max_samples};
   cuhre_guide_stats gs;
   return guided_run(g->ctx, g, f, dim, epsrel, epsabs, maxeval, lows, highs, run, out, &gs);
 }

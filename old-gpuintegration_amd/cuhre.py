@@ -202,7 +202,7 @@ def lib():
         L.cuhre_guide_gradient.argtypes = [C.c_void_p, C.c_int64, fp, i32p, fp, C.c_int, fp,
                                            C.POINTER(C.c_double)]
         L.cuhre_guide_collect.argtypes = [C.c_void_p, C.POINTER(Integrand), C.c_int, C.c_double, C.c_double,
-                                          C.c_int64, dp, dp, C.c_int64, C.POINTER(Result)]
+                                          C.c_int64, dp, dp, C.c_int64, C.POINTER(GuidePolicy), C.POINTER(Result)]
         L.cuhre_guide_samples.argtypes = [C.c_void_p]
         L.cuhre_guide_samples.restype = C.c_int64
         L.cuhre_guide_get_samples.argtypes = [C.c_void_p, C.c_int64, C.c_int64, fp, i32p, fp]
@@ -430,14 +430,17 @@ class Guide:
                                           _fp(f), dim, _fp(g), C.byref(loss)))
         return g, loss.value
 
-    def collect(self, p: W.Problem, max_samples: int, maxeval=None):
+    def collect(self, p: W.Problem, max_samples: int, maxeval=None, axis_mode=0, axis_confidence=0.6):
+        """Run that logs up to max_samples; axis_mode 1 lets the network pick
+        the axes it is confident about (so the log covers its own regions)."""
         s, keep = to_struct(p.f)
         lo = np.ascontiguousarray(p.lows, dtype=np.float64)
         hi = np.ascontiguousarray(p.highs, dtype=np.float64)
         res = Result()
+        pol = GuidePolicy(axis_mode, 0, axis_confidence, 1.0, 0.0)
         rc = lib().cuhre_guide_collect(self.h, C.byref(s), p.f.dim, p.epsrel, p.epsabs,
                                        p.maxeval if maxeval is None else maxeval, _dp(lo), _dp(hi),
-                                       max_samples, C.byref(res))
+                                       max_samples, C.byref(pol), C.byref(res))
         _check(rc)
         d = res.asdict()
         d["rc"] = rc

@@ -156,6 +156,21 @@ def test_collection_labels(ctx):
     assert r["nregions"] == ctx.integrate(p)["nregions"]
 
 
+def test_collection_under_the_network(ctx):
+    # an untrained network choosing every axis: the log holds the regions
+    # its choices produce, labelled by the look-ahead like any other
+    g = cuhre.Guide(ctx, seed=9)
+    p = CORNER5
+    r = g.collect(p, max_samples=20000, maxeval=10**8, axis_mode=1, axis_confidence=0.0)
+    n = g.samples()
+    assert 0 < n <= 20000
+    X, a, fin = g.get_samples()
+    assert a.min() >= 0 and a.max() < 5
+    assert np.all(np.isfinite(X))
+    plain = g.collect(p, max_samples=1, maxeval=10**8)
+    assert r["nregions"] != plain["nregions"] or r["error"] != plain["error"]
+
+
 def test_training_learns_a_rule(ctx):
     g = cuhre.Guide(ctx, seed=5)
     f = W.corner_peak([0.7, 1.5, 2.0, 3.0, 4.5, 1.1, 2.2], 8)
