@@ -175,8 +175,8 @@ def bench_learned(ctx, samples_per_run=500_000, maxeval=10**11, rounds=3):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batched", action="store_true",
                     help="also time the 4096-integral batched form against sequential plain calls")

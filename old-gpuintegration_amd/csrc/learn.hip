@@ -61,17 +61,18 @@ __global__ void k_mlp_prepare(const float* params, uint16_t* W, uint16_t* WT)
 // Inference on rows given as FP32 features (n x 32) -> logits (n x 16).
 __global__ __launch_bounds__(256) void k_mlp_forward(MlpDev M, const float* feat, int64_t n, float* out)
 {
-  __shared__ __align__(16) uint16_t X[kMlpRows * kMlpIn];
-  __shared__ __align__(16) uint16_t H1[kMlpRows * kMlpHid], H2[kMlpRows * kMlpHid], H3[kMlpRows * kMlpHid];
+  __shared__ __align__(16) uint16_t X[kMlpRows * kLdIn];
+  __shared__ __align__(16) uint16_t H1[kMlpRows * kLdHid], H2[kMlpRows * kLdHid], H3[kMlpRows * kLdHid];
   __shared__ float Y[kMlpRows * kMlpOut];
   const int64_t r0 = (int64_t)blockIdx.x * kMlpRows;
   for (int e = threadIdx.x; e < kMlpRows * kMlpIn; e += blockDim.x) {
-    const int64_t r = r0 + e / kMlpIn;
-    X[e] = r < n ? f2bf(feat[r * kMlpIn + e % kMlpIn]) : (uint16_t)0;
+    const int rr = e / kMlpIn, q = e % kMlpIn;
+    const int64_t r = r0 + rr;
+    X[rr * kLdIn + q] = r < n ? f2bf(feat[r * kMlpIn + q]) : (uint16_t)0;
   }
   __syncthreads();
   const int w = threadIdx.x >> 6;
-  mlp_forward16(M, X + w * 16 * kMlpIn, H1 + w * 16 * kMlpHid, H2 + w * 16 * kMlpHid, H3 + w * 16 * kMlpHid,
+  mlp_forward16(M, X + w * 16 * kLdIn, H1 + w * 16 * kLdHid, H2 + w * 16 * kLdHid, H3 + w * 16 * kLdHid,
                 Y + w * 16 * kMlpOut);
   for (int e = threadIdx.x; e < kMlpRows * kMlpOut; e += blockDim.x) {
     const int64_t r = r0 + e / kMlpOut;
@@ -80,7 +81,8 @@ __global__ __launch_bounds__(256) void k_mlp_forward(MlpDev M, const float* feat
 }
 
 // dH[16 x K] = dZ[16 x N] . W^T for this wave's rows, then dZprev = dH * (H > 0)
-// stored as BF16.  W row-major [K][N] gives contiguous B operands.
+// stored as BF16.  W row-major [K][N] (global) gives contiguous B operands;
+// dZ, H and dZprev are LDS rows with padded strides.
 template <int K, int N>
 __device__ __forceinline__ void mlp_layer_bwd(const uint16_t* dZs, const uint16_t* W, const uint16_t* Hs,
                                               uint16_t* dZprev)
@@ -91,7 +93,7 @@ __device__ __forceinline__ void mlp_layer_bwd(const uint16_t* dZs, const uint16_
   for (int jt = 0; jt < K / 16; ++jt) acc[jt] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int kt = 0; kt < N / 16; ++kt) {
-    const bf16x4 a = *reinterpret_cast<const bf16x4*>(dZs + row * N + kt * 16 + kq);
+    const bf16x4 a = *reinterpret_cast<const bf16x4*>(dZs + row * (N + kPad) + kt * 16 + kq);
 #pragma unroll
     for (int jt = 0; jt < K / 16; ++jt) {
       const bf16x4 b = *reinterpret_cast<const bf16x4*>(W + (jt * 16 + row) * N + kt * 16 + kq);
@@ -102,13 +104,14 @@ __device__ __forceinline__ void mlp_layer_bwd(const uint16_t* dZs, const uint16_
   for (int jt = 0; jt < K / 16; ++jt)
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      const int idx = (kq + v) * K + jt * 16 + row;
+      const int idx = (kq + v) * (K + kPad) + jt * 16 + row;
       dZprev[idx] = Hs[idx] != 0 ? f2bf(acc[jt][v]) : (uint16_t)0;
     }
 }
 
-// One 16x16 tile of dW = H^T dZ over the block's 64 rows (4 MFMA steps).
-__device__ __forceinline__ f32x4 mlp_dw_tile(const uint16_t* Hs, int K, const uint16_t* dZs, int N, int mt, int nt)
+// One 16x16 tile of dW = H^T dZ over the block's 64 rows (4 MFMA steps);
+// ldh, ldz: LDS row strides of H and dZ.
+__device__ __forceinline__ f32x4 mlp_dw_tile(const uint16_t* Hs, int ldh, const uint16_t* dZs, int ldz, int mt, int nt)
 {
   const int l = __lane_id(), row = l & 15, kq = (l >> 4) * 4;
   f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -118,8 +121,8 @@ __device__ __forceinline__ f32x4 mlp_dw_tile(const uint16_t* Hs, int K, const ui
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int s = kt * 16 + kq + v;
-      a[v] = (short)Hs[s * K + mt * 16 + row];
-      b[v] = (short)dZs[s * N + nt * 16 + row];
+      a[v] = (short)Hs[s * ldh + mt * 16 + row];
+      b[v] = (short)dZs[s * ldz + nt * 16 + row];
     }
     acc = mfma16(a, b, acc);
   }
@@ -154,11 +157,11 @@ struct TrainArgs {
 // written to its own slot (reduced in a fixed order by k_mlp_reduce_adam).
 __global__ __launch_bounds__(256) void k_mlp_train(TrainArgs T)
 {
-  __shared__ __align__(16) uint16_t X[kMlpRows * kMlpIn];
-  __shared__ __align__(16) uint16_t H1[kMlpRows * kMlpHid], H2[kMlpRows * kMlpHid], H3[kMlpRows * kMlpHid];
+  __shared__ __align__(16) uint16_t X[kMlpRows * kLdIn];
+  __shared__ __align__(16) uint16_t H1[kMlpRows * kLdHid], H2[kMlpRows * kLdHid], H3[kMlpRows * kLdHid];
   __shared__ float Y[kMlpRows * kMlpOut];
-  __shared__ __align__(16) uint16_t dZ4[kMlpRows * kMlpOut];
-  __shared__ __align__(16) uint16_t dZ3[kMlpRows * kMlpHid], dZ2[kMlpRows * kMlpHid], dZ1[kMlpRows * kMlpHid];
+  __shared__ __align__(16) uint16_t dZ4[kMlpRows * kLdOut];
+  __shared__ __align__(16) uint16_t dZ3[kMlpRows * kLdHid], dZ2[kMlpRows * kLdHid], dZ1[kMlpRows * kLdHid];
   __shared__ int8_t s_perm[kMlpRows][8];  // new axis j reads logged axis s_perm[r][j]
   __shared__ int s_label[kMlpRows];
   const int64_t r0 = (int64_t)blockIdx.x * kMlpRows;
@@ -198,18 +201,19 @@ __global__ __launch_bounds__(256) void k_mlp_train(TrainArgs T)
     const int rr = e / kMlpIn;
     const int64_t r = r0 + rr;
     int q = e % kMlpIn;
+    const int q0 = q;
     if (q < 24 && (q & 7) < T.dim) q = (q & ~7) | s_perm[rr][q & 7];
-    X[e] = r < T.rows ? f2bf(T.feat[(int64_t)T.order[r] * kMlpIn + q]) : (uint16_t)0;
+    X[rr * kLdIn + q0] = r < T.rows ? f2bf(T.feat[(int64_t)T.order[r] * kMlpIn + q]) : (uint16_t)0;
   }
   __syncthreads();
   const int w = threadIdx.x >> 6;
-  mlp_forward16(T.M, X + w * 16 * kMlpIn, H1 + w * 16 * kMlpHid, H2 + w * 16 * kMlpHid,
-                H3 + w * 16 * kMlpHid, Y + w * 16 * kMlpOut);
+  mlp_forward16(T.M, X + w * 16 * kLdIn, H1 + w * 16 * kLdHid, H2 + w * 16 * kLdHid,
+                H3 + w * 16 * kLdHid, Y + w * 16 * kMlpOut);
   // loss and dY (softmax cross-entropy over the d axis logits + BCE finish)
   if (threadIdx.x < kMlpRows) {
     const int rr = threadIdx.x;
     const int64_t r = r0 + rr;
-    uint16_t* dz = dZ4 + rr * kMlpOut;
+    uint16_t* dz = dZ4 + rr * kLdOut;
     for (int j = 0; j < kMlpOut; ++j) dz[j] = 0;
     if (r < T.rows) {
       const int64_t s = T.order[r];
@@ -239,14 +243,14 @@ __global__ __launch_bounds__(256) void k_mlp_train(TrainArgs T)
     }
   }
   __syncthreads();
-  mlp_layer_bwd<64, 16>(dZ4 + w * 16 * kMlpOut, T.M.W + kMlpWOff[3], H3 + w * 16 * kMlpHid,
-                        dZ3 + w * 16 * kMlpHid);
+  mlp_layer_bwd<64, 16>(dZ4 + w * 16 * kLdOut, T.M.W + kMlpWOff[3], H3 + w * 16 * kLdHid,
+                        dZ3 + w * 16 * kLdHid);
   __syncthreads();
-  mlp_layer_bwd<64, 64>(dZ3 + w * 16 * kMlpHid, T.M.W + kMlpWOff[2], H2 + w * 16 * kMlpHid,
-                        dZ2 + w * 16 * kMlpHid);
+  mlp_layer_bwd<64, 64>(dZ3 + w * 16 * kLdHid, T.M.W + kMlpWOff[2], H2 + w * 16 * kLdHid,
+                        dZ2 + w * 16 * kLdHid);
   __syncthreads();
-  mlp_layer_bwd<64, 64>(dZ2 + w * 16 * kMlpHid, T.M.W + kMlpWOff[1], H1 + w * 16 * kMlpHid,
-                        dZ1 + w * 16 * kMlpHid);
+  mlp_layer_bwd<64, 64>(dZ2 + w * 16 * kLdHid, T.M.W + kMlpWOff[1], H1 + w * 16 * kLdHid,
+                        dZ1 + w * 16 * kLdHid);
   __syncthreads();
   // weight gradients: 44 tiles of 16x16 shared out over the 4 waves
   float* part = T.part + (int64_t)blockIdx.x * kMlpParams;
@@ -260,7 +264,7 @@ __global__ __launch_bounds__(256) void k_mlp_train(TrainArgs T)
     const uint16_t* Hs = layer == 0 ? X : (layer == 1 ? H1 : (layer == 2 ? H2 : H3));
     const uint16_t* dZs = layer == 0 ? dZ1 : (layer == 1 ? dZ2 : (layer == 2 ? dZ3 : dZ4));
     const int K = kMlpK[layer], N = kMlpN[layer];
-    const f32x4 acc = mlp_dw_tile(Hs, K, dZs, N, mt, nt);
+    const f32x4 acc = mlp_dw_tile(Hs, K + kPad, dZs, N + kPad, mt, nt);
 #pragma unroll
     for (int v = 0; v < 4; ++v) part[kMlpWOff[layer] + (mt * 16 + kq + v) * N + nt * 16 + row] = acc[v];
   }
@@ -271,7 +275,7 @@ __global__ __launch_bounds__(256) void k_mlp_train(TrainArgs T)
     const int N = kMlpN[layer], col = c - (kMlpBOff[layer] - kMlpBOff[0]);
     const uint16_t* dZs = layer == 0 ? dZ1 : (layer == 1 ? dZ2 : (layer == 2 ? dZ3 : dZ4));
     float sacc = 0.f;
-    for (int s = 0; s < kMlpRows; ++s) sacc += bf2f(dZs[s * N + col]);
+    for (int s = 0; s < kMlpRows; ++s) sacc += bf2f(dZs[s * (N + kPad) + col]);
     part[kMlpBOff[0] + c] = sacc;
   }
 }
@@ -389,9 +393,9 @@ template <bool MLP>
 __global__ __launch_bounds__(MLP ? 256 : 64) void k_guide_apply(GuideArgs G)
 {
   constexpr int kL = MLP ? 1 : 0;  // network buffers only when the network runs
-  __shared__ __align__(16) uint16_t X[kMlpRows * kMlpIn * kL + 1];
-  __shared__ __align__(16) uint16_t H1[kMlpRows * kMlpHid * kL + 1], H2[kMlpRows * kMlpHid * kL + 1],
-      H3[kMlpRows * kMlpHid * kL + 1];
+  __shared__ __align__(16) uint16_t X[kMlpRows * kLdIn * kL + 1];
+  __shared__ __align__(16) uint16_t H1[kMlpRows * kLdHid * kL + 1], H2[kMlpRows * kLdHid * kL + 1],
+      H3[kMlpRows * kLdHid * kL + 1];
   __shared__ float Y[kMlpRows * kMlpOut * kL + 1];
   __shared__ int s_split[kMlpRows], s_log[kMlpRows], s_ret[kMlpRows], s_ovr[kMlpRows];
   __shared__ double s_e[kMlpRows], s_er[kMlpRows], s_ab[kMlpRows];
@@ -419,12 +423,12 @@ __global__ __launch_bounds__(MLP ? 256 : 64) void k_guide_apply(GuideArgs G)
       }
     }
     if (MLP)
-      for (int j = 0; j < 32; ++j) X[t * kMlpIn + j] = f2bf(f[j]);
+      for (int j = 0; j < 32; ++j) X[t * kLdIn + j] = f2bf(f[j]);
   }
   __syncthreads();
   if (MLP) {
     const int w = t >> 6;
-    mlp_forward16(G.M, X + w * 16 * kMlpIn, H1 + w * 16 * kMlpHid, H2 + w * 16 * kMlpHid, H3 + w * 16 * kMlpHid,
+    mlp_forward16(G.M, X + w * 16 * kLdIn, H1 + w * 16 * kLdHid, H2 + w * 16 * kLdHid, H3 + w * 16 * kLdHid,
                   Y + w * 16 * kMlpOut);
   }
   if (t < kMlpRows) {

@@ -25,6 +25,13 @@ constexpr int kMlpWOff[4] = {0, 2048, 6144, 10240};
 constexpr int kMlpBOff[4] = {11264, 11328, 11392, 11456};
 constexpr int kMlpParams = 11472;
 
+// LDS rows (activations and gradients) carry kPad extra BF16 (8 bytes):
+// the 16 rows one MFMA operand read touches then start in different banks
+// instead of all in the same one (a row of 64 BF16 is exactly one pass over
+// the 32 banks).
+constexpr int kPad = 4;
+constexpr int kLdIn = kMlpIn + kPad, kLdHid = kMlpHid + kPad, kLdOut = kMlpOut + kPad;
+
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
@@ -47,9 +54,10 @@ __device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c)
   return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
-// Y[16 x N] = X[16 x K] . W for this wave's 16 rows; X in LDS (row stride K,
-// BF16), W read from WT ([N][K]) so each B operand is 8 contiguous bytes.
-template <int K, int N>
+// Y[16 x N] = X[16 x K] . W for this wave's 16 rows; X in LDS (row stride
+// K + kPad, BF16), W read from W^T ([N][K], row stride LDW) so each B
+// operand is 8 contiguous bytes.
+template <int K, int N, int LDW>
 __device__ __forceinline__ void mlp_layer_fwd(const uint16_t* Xs, const uint16_t* WT, f32x4 (&acc)[N / 16])
 {
   const int l = __lane_id(), row = l & 15, kq = (l >> 4) * 4;
@@ -57,16 +65,16 @@ __device__ __forceinline__ void mlp_layer_fwd(const uint16_t* Xs, const uint16_t
   for (int nt = 0; nt < N / 16; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int kt = 0; kt < K / 16; ++kt) {
-    const bf16x4 a = *reinterpret_cast<const bf16x4*>(Xs + row * K + kt * 16 + kq);
+    const bf16x4 a = *reinterpret_cast<const bf16x4*>(Xs + row * (K + kPad) + kt * 16 + kq);
 #pragma unroll
     for (int nt = 0; nt < N / 16; ++nt) {
-      const bf16x4 b = *reinterpret_cast<const bf16x4*>(WT + (nt * 16 + row) * K + kt * 16 + kq);
+      const bf16x4 b = *reinterpret_cast<const bf16x4*>(WT + (nt * 16 + row) * LDW + kt * 16 + kq);
       acc[nt] = mfma16(a, b, acc[nt]);
     }
   }
 }
 
-// Bias + ReLU + BF16 store of a layer's output tile into LDS [16][N].
+// Bias + ReLU + BF16 store of a layer's output tile into LDS [16][N + kPad].
 template <int N>
 __device__ __forceinline__ void mlp_store_relu(const f32x4 (&acc)[N / 16], const float* bias, uint16_t* Hs)
 {
@@ -76,37 +84,39 @@ __device__ __forceinline__ void mlp_store_relu(const f32x4 (&acc)[N / 16], const
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const float z = acc[nt][v] + bias[nt * 16 + col];
-      Hs[(r0 + v) * N + nt * 16 + col] = f2bf(z > 0.f ? z : 0.f);
+      Hs[(r0 + v) * (N + kPad) + nt * 16 + col] = f2bf(z > 0.f ? z : 0.f);
     }
 }
 
-// Forward pass of this wave's 16 rows.  Xs..H3s are this wave's LDS rows;
-// Ys receives the 16 x 16 logits in FP32.  Every wave of the block calls it
-// together (block barriers between layers).
+// Forward pass of this wave's 16 rows.  Xs..H3s are this wave's LDS rows
+// (padded strides); Ys receives the 16 x 16 logits in FP32.  The weights
+// come from W^T in global memory (L2-resident, 22 KB): staging them in LDS
+// per block was measured slower.  Every wave of the block calls it together
+// (block barriers between layers).
 __device__ __forceinline__ void mlp_forward16(const MlpDev& M, const uint16_t* Xs, uint16_t* H1s, uint16_t* H2s,
                                               uint16_t* H3s, float* Ys)
 {
   {
     f32x4 acc[4];
-    mlp_layer_fwd<32, 64>(Xs, M.WT + kMlpWOff[0], acc);
+    mlp_layer_fwd<32, 64, 32>(Xs, M.WT + kMlpWOff[0], acc);
     mlp_store_relu<64>(acc, M.b + 0, H1s);
   }
   __syncthreads();
   {
     f32x4 acc[4];
-    mlp_layer_fwd<64, 64>(H1s, M.WT + kMlpWOff[1], acc);
+    mlp_layer_fwd<64, 64, 64>(H1s, M.WT + kMlpWOff[1], acc);
     mlp_store_relu<64>(acc, M.b + 64, H2s);
   }
   __syncthreads();
   {
     f32x4 acc[4];
-    mlp_layer_fwd<64, 64>(H2s, M.WT + kMlpWOff[2], acc);
+    mlp_layer_fwd<64, 64, 64>(H2s, M.WT + kMlpWOff[2], acc);
     mlp_store_relu<64>(acc, M.b + 128, H3s);
   }
   __syncthreads();
   {
     f32x4 acc[1];
-    mlp_layer_fwd<64, 16>(H3s, M.WT + kMlpWOff[3], acc);
+    mlp_layer_fwd<64, 16, 64>(H3s, M.WT + kMlpWOff[3], acc);
     const int l = __lane_id(), col = l & 15, r0 = (l >> 4) * 4;
 #pragma unroll
     for (int v = 0; v < 4; ++v) Ys[(r0 + v) * 16 + col] = acc[0][v] + M.b[192 + col];
