@@ -28,7 +28,7 @@ import workloads as W  # noqa: E402
 FP64_PEAK_TFLOPS = 78.6
 # measured HBM traffic per launch of k_eval_seg<8, product_peak> on this workload
 # (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, calibrated on a copy kernel)
-TRAFFIC_PER_LAUNCH = 2 * 10150.76 * 1024 + 4149.64 * 1024
+TRAFFIC_PER_LAUNCH = 2 * 10175.49 * 1024 + 4141.44 * 1024
 
 
 def flop_costs():
