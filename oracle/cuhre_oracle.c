@@ -252,6 +252,171 @@ void or_region_eval(const or_rule* r, const or_integrand* f,
 }
 
 /* ------------------------------------------------------------------------ */
+/* Degree-9 rule (KEY 9), quad/GPUquad/GPUQuadRule.cu:30-226.  The weight    */
+/* table and the radii are the reference's (they define the rule); the       */
+/* point enumeration is set by set; scale/norm as GPUQuadRule.cu:209-225;    */
+/* error and axis as GPUQuadSample.cu:113-172, 195-216.                      */
+/* ------------------------------------------------------------------------ */
+
+int or_r9_npoints(int d)
+{
+  return 1 + 8 * d + 2 * d * (d - 1) + 4 * d * (d - 1) + 4 * d * (d - 1) * (d - 2) / 3 + (1 << d);
+}
+
+int or_rule9_init(or_rule9* r, int d)
+{
+  if (d < 2 || d > OR_MAXDIM) return -1;
+  const double n = (double)d;
+  /* radii as fractions of the region's width (GPUQuadRule.cu:172-176) */
+  const double a[5] = {.47795365790226950619, .20302858736911986780, .44762735462617812882, .125,
+                       .34303789878087814570};
+  for (int k = 0; k < 5; ++k) r->t[k] = 2.0 * a[k]; /* half-width units */
+  /* weights (GPUQuadRule.cu:33-116): recurring polynomials in d */
+  const double A = n * (n * (n * (-.0023611709677855117884) + .11415390023857325268) + (-.63833920076702389094)) +
+                   .74849988504685208004;
+  const double B = n * (n * (n * (-.0014324017033399125142) + .057471507864489725949) + (-.14225104571434243234)) -
+                   (-.062875028738286979989);
+  const double C = n * (n * .0035417564516782676826 + (-.072609367395893679605)) + .10557491625218991012;
+  const double D = n * (n * .0021486025550098687713 + (-.032268563892953949998)) + .010636783990231217481;
+  const double E = n * (-.04508628929435784076) + .21415883524352793401;
+  const double F = n * (-.027351546526545644722) + .054941067048711234101;
+  const double G = .015532417276607053264 - n * .0035417564516782676826;
+  const double H = .0035328099607090870236 - n * .0021486025550098687713;
+  const double c = (double)(1 << d);
+  const double w[OR_R9_SETS][OR_R9_RULES] = {
+      {A, B, n * .254591133248959089 - B, n * (n * (-1.207328566678236261) + .89567365764160676508) - 1 + A,
+       n * (-.36479356986049146661) + 1 - A},
+      {C, D, .014689102496143490175 - D, n * .51134708346467591431 + .45976448120806344646 + C,
+       .18239678493024573331 - C},
+      {E, F, .11937596202570775297 - F, n * .65089519391920250593 + .14744939829434460168, -E},
+      {.057693384490973483573, .034999626602143583822, -.057693384490973483573, -1.3868627719278281436,
+       -.057693384490973483573},
+      {0, 0, -.2386668732575008879, 0, 0},
+      {G, H, -H, .09231719987444221619 + G, -G},
+      {.02254314464717892038, .013675773263272822361, -.013675773263272822361, -.32544759695960125297,
+       -.02254314464717892038},
+      {.0017708782258391338413, .0010743012775049343856, -.0010743012775049343856, .0017708782258391338413,
+       -.0017708782258391338413},
+      {.25150011495314791996 / c, -.062875028738286979989 / c, .062875028738286979989 / c,
+       .25150011495314791996 / c, -.25150011495314791996 / c}};
+  const int cnt[OR_R9_SETS] = {1, 2 * d, 2 * d, 2 * d, 2 * d, 2 * d * (d - 1), 4 * d * (d - 1),
+                               4 * d * (d - 1) * (d - 2) / 3, 1 << d};
+  r->dim = d;
+  r->npts = or_r9_npoints(d);
+  for (int s = 0; s < OR_R9_SETS; ++s) {
+    r->count[s] = cnt[s];
+    for (int q = 0; q < OR_R9_RULES; ++q) {
+      r->w[s][q] = w[s][q];
+      r->scale[s][q] = 0.0;
+      r->norm[s][q] = 0.0;
+    }
+  }
+  /* null-rule combinations: for rules 1..3 and each set, the scale that
+   * cancels that set's weight in w[r+1] + scale*w[r], and the inverse
+   * 1-norm of the combined rule (GPUQuadRule.cu:209-225) */
+  for (int s = 0; s < OR_R9_SETS; ++s)
+    for (int q = 1; q < OR_R9_RULES - 1; ++q) {
+      const double sc = w[s][q] == 0.0 ? 100.0 : -w[s][q + 1] / w[s][q];
+      double sum = 0.0;
+      for (int x = 0; x < OR_R9_SETS; ++x) sum += (double)cnt[x] * fabs(w[x][q + 1] + sc * w[x][q]);
+      r->scale[s][q] = sc;
+      r->norm[s][q] = 1.0 / sum;
+    }
+  return 0;
+}
+
+/* accumulates one point of set s */
+static void r9_add(const or_rule9* r, int s, double v, double* sums, double* sa)
+{
+  for (int q = 0; q < OR_R9_RULES; ++q) sums[q] = fma(r->w[s][q], v, sums[q]);
+  *sa = fma(fabs(r->w[s][0]), fabs(v), *sa);
+}
+
+void or_region_eval9(const or_rule9* r, const or_integrand* f, const double* lower,
+                     const double* width, double* est, double* err, int* axis, double* abssum,
+                     double* dvec, double* sums_out)
+{
+  const int d = r->dim;
+  double c[OR_MAXDIM] = {0}, h[OR_MAXDIM], x[OR_MAXDIM];
+  double vol = 1.0, maxw = 0.0;
+  int widest = 0;
+  for (int m = 0; m < d; ++m) {
+    c[m] = fma(0.5, width[m], lower[m]);
+    h[m] = 0.5 * width[m];
+    vol *= width[m];
+    if (width[m] > maxw) { maxw = width[m]; widest = m; }
+  }
+  double sums[OR_R9_RULES] = {0, 0, 0, 0, 0}, sa = 0.0;
+  const double f0 = or_f(f, c);
+  r9_add(r, 0, f0, sums, &sa);
+  /* axis sets 1..4, fourth difference from sets 1 and 2 */
+  const double ratio = (r->t[1] / r->t[0]) * (r->t[1] / r->t[0]);
+  double maxd = 0.0;
+  int ax = widest;
+  for (int i = 0; i < d; ++i) {
+    double fv[4][2];
+    for (int k = 0; k < 4; ++k)
+      for (int sg = 0; sg < 2; ++sg) {
+        for (int m = 0; m < d; ++m) x[m] = c[m];
+        x[i] = fma(h[i], sg ? -r->t[k] : r->t[k], c[i]);
+        fv[k][sg] = or_f(f, x);
+        r9_add(r, 1 + k, fv[k][sg], sums, &sa);
+      }
+    const double di = fabs(2.0 * (1.0 - ratio) * f0 + ratio * (fv[0][0] + fv[0][1]) - (fv[1][0] + fv[1][1]));
+    if (dvec) dvec[i] = di;
+    if (di > maxd) { maxd = di; ax = i; }
+  }
+  /* pair sets 5 (t1, t1) and 6 (t1, t2 in both orders) */
+  for (int i = 0; i < d; ++i)
+    for (int j = i + 1; j < d; ++j) {
+      for (int sg = 0; sg < 4; ++sg) {
+        for (int m = 0; m < d; ++m) x[m] = c[m];
+        x[i] = fma(h[i], (sg & 1) ? -r->t[0] : r->t[0], c[i]);
+        x[j] = fma(h[j], (sg & 2) ? -r->t[0] : r->t[0], c[j]);
+        r9_add(r, 5, or_f(f, x), sums, &sa);
+      }
+      for (int o = 0; o < 2; ++o)
+        for (int sg = 0; sg < 4; ++sg) {
+          for (int m = 0; m < d; ++m) x[m] = c[m];
+          const double ti = o ? r->t[1] : r->t[0], tj = o ? r->t[0] : r->t[1];
+          x[i] = fma(h[i], (sg & 1) ? -ti : ti, c[i]);
+          x[j] = fma(h[j], (sg & 2) ? -tj : tj, c[j]);
+          r9_add(r, 6, or_f(f, x), sums, &sa);
+        }
+    }
+  /* triple set 7 */
+  for (int i = 0; i < d; ++i)
+    for (int j = i + 1; j < d; ++j)
+      for (int k = j + 1; k < d; ++k)
+        for (int sg = 0; sg < 8; ++sg) {
+          for (int m = 0; m < d; ++m) x[m] = c[m];
+          x[i] = fma(h[i], (sg & 1) ? -r->t[0] : r->t[0], c[i]);
+          x[j] = fma(h[j], (sg & 2) ? -r->t[0] : r->t[0], c[j]);
+          x[k] = fma(h[k], (sg & 4) ? -r->t[0] : r->t[0], c[k]);
+          r9_add(r, 7, or_f(f, x), sums, &sa);
+        }
+  /* corner set 8 */
+  for (int q = 0; q < (1 << d); ++q) {
+    for (int m = 0; m < d; ++m) x[m] = fma(h[m], ((q >> m) & 1) ? -r->t[4] : r->t[4], c[m]);
+    r9_add(r, 8, or_f(f, x), sums, &sa);
+  }
+  if (sums_out)
+    for (int q = 0; q < OR_R9_RULES; ++q) sums_out[q] = sums[q];
+  /* the largest normalised null rule in the span of two successive ones */
+  double e[OR_R9_RULES];
+  for (int q = 1; q < OR_R9_RULES - 1; ++q) {
+    double m = 0.0;
+    for (int s = 0; s < OR_R9_SETS; ++s) m = fmax(m, fabs(sums[q + 1] + r->scale[s][q] * sums[q]) * r->norm[s][q]);
+    e[q] = m;
+  }
+  const double e1 = e[1], e2 = e[2], e3 = e[3];
+  *est = vol * sums[0];
+  *err = vol * ((5.0 * e1 <= e2 && 5.0 * e2 <= e3) ? e1 : 5.0 * fmax(fmax(e1, e2), e3));
+  *abssum = vol * sa;
+  *axis = ax;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Initial grid (GPUKernelquad.cu:23-61, 300-316) and bisection              */
 /* (GPUKernelquad.cu:97-149; children interleaved 2j, 2j+1).                 */
 /* ------------------------------------------------------------------------ */
@@ -331,11 +496,22 @@ int or_integrate_bfs(const or_integrand* f, const double* lows,
                      int64_t maxeval, int64_t capacity, int nthreads,
                      or_result* out)
 {
+  return or_integrate_bfs_key(f, lows, highs, epsrel, epsabs, maxeval, capacity, nthreads, 7, out);
+}
+
+int or_integrate_bfs_key(const or_integrand* f, const double* lows,
+                         const double* highs, double epsrel, double epsabs,
+                         int64_t maxeval, int64_t capacity, int nthreads,
+                         int key, or_result* out)
+{
   memset(out, 0, sizeof(*out));
   const int d = f->dim;
+  if (key != 0 && key != 7 && key != 9) return -1;
   or_rule rule;
+  or_rule9 rule9;
   if (or_rule_init(&rule, d) != 0) return -1;
-  const int p = rule.npts;
+  if (key == 9 && or_rule9_init(&rule9, d) != 0) { or_rule_free(&rule); return -1; }
+  const int p = key == 9 ? rule9.npts : rule.npts;
   int64_t n = or_initial_grid(d, lows, highs, NULL, NULL);
   if (capacity < n) capacity = n;
   int64_t cap = capacity;
@@ -363,7 +539,10 @@ int or_integrate_bfs(const or_integrand* f, const double* lows,
       double l[OR_MAXDIM], w[OR_MAXDIM];
       for (int m = 0; m < d; ++m) { l[m] = lo[(size_t)m * n + r]; w[m] = wd[(size_t)m * n + r]; }
       int a;
-      or_region_eval(&rule, f, l, w, &est[r], &err[r], &a, &ab[r], NULL, NULL);
+      if (key == 9)
+        or_region_eval9(&rule9, f, l, w, &est[r], &err[r], &a, &ab[r], NULL, NULL);
+      else
+        or_region_eval(&rule, f, l, w, &est[r], &err[r], &a, &ab[r], NULL, NULL);
       ax[r] = a;
       if (!isfinite(est[r]) || !isfinite(err[r])) nonfinite = 1;
       double tol = fmax(epsrel * fabs(est[r]), epsabs);

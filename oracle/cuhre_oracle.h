@@ -78,6 +78,33 @@ void or_region_eval(const or_rule* r, const or_integrand* f,
                     double* est, double* err, int* axis, double* abssum,
                     double* i5, double* dvec);
 
+/* The reference's degree-9 rule (KEY 9): Cuba's Cuhre rule of degree 9 with
+ * four null rules and nine generator sets, as the reference hard-wires it
+ * (quad/GPUquad/GPUQuadRule.cu:30-226, quad/quad.h:80-97); error from the
+ * null rules with errcoeff {5, 1, 5} (quad/GPUquad/GPUQuadSample.cu:195-216).
+ * Points: centre; +-t_k e_i for four radii; (+-t1, +-t1) on pairs i<j;
+ * (+-t1, +-t2) on ordered pairs i != j; (+-t1)^3 on triples i<j<k; the 2^d
+ * corners at +-t5.  t_k are in half-width units.  Weights sum to 1 in rule
+ * 0 (est = volume * sum_k w_k f(x_k)). */
+#define OR_R9_SETS 9
+#define OR_R9_RULES 5
+typedef struct {
+  int dim, npts;
+  double t[5];
+  double w[OR_R9_SETS][OR_R9_RULES];
+  double scale[OR_R9_SETS][OR_R9_RULES], norm[OR_R9_SETS][OR_R9_RULES];
+  int count[OR_R9_SETS];
+} or_rule9;
+
+int or_rule9_init(or_rule9* r, int dim);
+int or_r9_npoints(int dim);
+/* One region under the degree-9 rule; sums[5] (may be NULL) receives the
+ * five rule sums (rule 0 = the estimate's, before the volume factor). */
+void or_region_eval9(const or_rule9* r, const or_integrand* f,
+                     const double* lower, const double* width,
+                     double* est, double* err, int* axis, double* abssum,
+                     double* dvec, double* sums);
+
 typedef struct {
   double integral, error, bound;
   int64_t nregions, neval;
@@ -93,6 +120,11 @@ int or_integrate_bfs(const or_integrand* f, const double* lows,
                      const double* highs, double epsrel, double epsabs,
                      int64_t maxeval, int64_t capacity, int nthreads,
                      or_result* out);
+/* The same loop with the rule chosen by key (0 or 7: GM7/5, 9: degree 9). */
+int or_integrate_bfs_key(const or_integrand* f, const double* lows,
+                         const double* highs, double epsrel, double epsabs,
+                         int64_t maxeval, int64_t capacity, int nthreads,
+                         int key, or_result* out);
 
 /* Cuhre-style global adaptive (heap of regions, always bisect the region
  * with the largest error), same GM7/5 rule: the CPU baseline flavour (i). */

@@ -65,6 +65,18 @@ class OrRule(C.Structure):
     ]
 
 
+class OrRule9(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int),
+        ("npts", C.c_int),
+        ("t", C.c_double * 5),
+        ("w", (C.c_double * 5) * 9),
+        ("scale", (C.c_double * 5) * 9),
+        ("norm", (C.c_double * 5) * 9),
+        ("count", C.c_int * 9),
+    ]
+
+
 _lib = None
 
 
@@ -88,6 +100,12 @@ def lib():
                                      dp, dp, C.POINTER(C.c_int), dp, dp, dp]
         L.or_integrate_bfs.argtypes = [C.POINTER(OrIntegrand), dp, dp, C.c_double, C.c_double,
                                        C.c_int64, C.c_int64, C.c_int, C.POINTER(OrResult)]
+        L.or_rule9_init.argtypes = [C.POINTER(OrRule9), C.c_int]
+        L.or_r9_npoints.argtypes = [C.c_int]
+        L.or_region_eval9.argtypes = [C.POINTER(OrRule9), C.POINTER(OrIntegrand), dp, dp,
+                                      dp, dp, C.POINTER(C.c_int), dp, dp, dp]
+        L.or_integrate_bfs_key.argtypes = [C.POINTER(OrIntegrand), dp, dp, C.c_double, C.c_double,
+                                           C.c_int64, C.c_int64, C.c_int, C.c_int, C.POINTER(OrResult)]
         L.or_integrate_heap.argtypes = [C.POINTER(OrIntegrand), dp, dp, C.c_double, C.c_double,
                                         C.c_int64, C.POINTER(OrResult)]
         L.or_exact.argtypes = [C.POINTER(OrIntegrand), dp, dp, dp, dp]
@@ -168,15 +186,48 @@ def region_eval(f: W.Integrand, lower, width):
     return est, err, ax, ab, i5, dv
 
 
-def integrate_bfs(p: W.Problem, nthreads=1, capacity=None, maxeval=None, epsrel=None):
+def rule9(d):
+    """The degree-9 rule's tables: dict of t[5], w[9][5], scale, norm, count[9], npts."""
+    r = OrRule9()
+    assert lib().or_rule9_init(C.byref(r), d) == 0
+    return {"t": np.array(r.t[:]), "w": np.array([r.w[i][:] for i in range(9)]),
+            "scale": np.array([r.scale[i][:] for i in range(9)]),
+            "norm": np.array([r.norm[i][:] for i in range(9)]),
+            "count": np.array(r.count[:]), "npts": r.npts}
+
+
+def region_eval9(f: W.Integrand, lower, width):
+    """Per-region (est, err, axis, abssum, dvec, sums[5]) under the degree-9 rule."""
+    lower = np.ascontiguousarray(lower, dtype=np.float64)
+    width = np.ascontiguousarray(width, dtype=np.float64)
+    n, d = lower.shape
+    r = OrRule9()
+    assert lib().or_rule9_init(C.byref(r), d) == 0
+    s, keep = to_struct(f)
+    est = np.empty(n); err = np.empty(n); ab = np.empty(n)
+    ax = np.empty(n, dtype=np.int32)
+    dv = np.empty((n, d)); sums = np.empty((n, 5))
+    e = C.c_double(); er = C.c_double(); a = C.c_int(); abv = C.c_double()
+    for i in range(n):
+        lib().or_region_eval9(C.byref(r), C.byref(s), _dp(lower[i]), _dp(width[i]), C.byref(e),
+                              C.byref(er), C.byref(a), C.byref(abv), _dp(dv[i]), _dp(sums[i]))
+        est[i], err[i], ax[i], ab[i] = e.value, er.value, a.value, abv.value
+    return est, err, ax, ab, dv, sums
+
+
+def npoints(d, key=7):
+    return lib().or_r9_npoints(d) if key == 9 else W.gm_npoints(d)
+
+
+def integrate_bfs(p: W.Problem, nthreads=1, capacity=None, maxeval=None, epsrel=None, key=7):
     s, keep = to_struct(p.f)
     lo = np.array(p.lows, dtype=np.float64)
     hi = np.array(p.highs, dtype=np.float64)
     res = OrResult()
     me = p.maxeval if maxeval is None else maxeval
-    cap = capacity if capacity is not None else max(1, me // W.gm_npoints(p.f.dim)) + 64
-    rc = lib().or_integrate_bfs(C.byref(s), _dp(lo), _dp(hi), p.epsrel if epsrel is None else epsrel,
-                                p.epsabs, me, cap, nthreads, C.byref(res))
+    cap = capacity if capacity is not None else max(1, me // npoints(p.f.dim, key)) + 64
+    rc = lib().or_integrate_bfs_key(C.byref(s), _dp(lo), _dp(hi), p.epsrel if epsrel is None else epsrel,
+                                    p.epsabs, me, cap, nthreads, key, C.byref(res))
     d = res.asdict()
     d["rc"] = rc
     return d

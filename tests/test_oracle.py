@@ -262,3 +262,92 @@ def test_bfs_nonfinite_is_an_error():
     p = W.Problem("bad", f, [0.0, 0.0], [1.0, 1.0], 1e-6, 0.0, 10**6)
     r = O.integrate_bfs(p)
     assert r["rc"] == -2
+
+
+# ---------------------------------------------------------------- degree 9
+# The reference's hard-wired rule (KEY 9, quad/GPUquad/GPUQuadRule.cu:30-226):
+# pinned by its point count, normalisation and polynomial degrees, which a
+# wrong radius or weight breaks.
+
+@pytest.mark.parametrize("d", range(2, 11))
+def test_rule9_counts_and_normalisation(d):
+    R = O.rule9(d)
+    assert R["npts"] == R["count"].sum()
+    assert {3: 77, 5: 273, 7: 717, 8: 1105}.get(d, R["npts"]) == R["npts"]
+    assert abs((R["count"] * R["w"][:, 0]).sum() - 1.0) < 1e-13
+    for q in range(1, 5):  # null rules integrate constants to zero
+        assert abs((R["count"] * R["w"][:, q]).sum()) < 1e-13
+
+
+def _r9_monomial_dev(d, deg):
+    """Largest |rule - exact| over the monomials of total degree deg on
+    [-1,1]^d, for rule 0 and each null rule (whose exact value is 0)."""
+    dev = np.zeros(5)
+    for alpha in itertools.product(range(deg + 1), repeat=d):
+        if sum(alpha) != deg:
+            continue
+        f = W.polynomial(d, [(1.0, list(alpha))])
+        est, err, ax, ab, dv, sums = O.region_eval9(f, np.full((1, d), -1.0), np.full((1, d), 2.0))
+        ex = float(_monomial_exact(alpha)) * 2 ** d
+        dev[0] = max(dev[0], abs(est[0] - ex))
+        for q in range(1, 5):
+            dev[q] = max(dev[q], abs(sums[0][q]))
+    return dev
+
+
+@pytest.mark.parametrize("d", [2, 3, 4])
+def test_rule9_polynomial_degrees(d):
+    # rule 0 has degree 9; the null rules 1..4 vanish up to degrees 7, 7, 5, 3
+    exact_to = [9, 7, 7, 5, 3]
+    devs = {deg: _r9_monomial_dev(d, deg) for deg in range(0, 11)}
+    for q in range(5):
+        for deg in range(0, exact_to[q] + 1):
+            assert devs[deg][q] < 1e-13, (q, deg, devs[deg][q])
+        # and not one even degree above (odd monomials vanish by symmetry)
+        above = exact_to[q] + 1 if (exact_to[q] + 1) % 2 == 0 else exact_to[q] + 2
+        assert devs[above][q] > 1e-4, (q, above, devs[above][q])
+
+
+def test_rule9_fourth_difference_closed_form():
+    # f = sum c_i x_i^4: the quadratic parts cancel by the choice of ratio,
+    # leaving D_i = 2 |c_i| h_i^4 t2^2 (t1^2 - t2^2) (t in half-width units)
+    d = 5
+    c = [0.3, -2.0, 1.1, 0.7, -0.2]
+    f = W.polynomial(d, [(ci, [4 if k == i else 0 for k in range(d)]) for i, ci in enumerate(c)])
+    lo = np.array([[0.1, 0.2, 0.3, 0.0, 0.5]])
+    wd = np.array([[0.4, 0.3, 0.5, 0.2, 0.3]])
+    est, err, ax, ab, dv, sums = O.region_eval9(f, lo, wd)
+    t = O.rule9(d)["t"]
+    h = wd[0] / 2
+    expect = 2 * np.abs(c) * h ** 4 * t[1] ** 2 * (t[0] ** 2 - t[1] ** 2)
+    assert np.allclose(dv[0], expect, rtol=1e-9, atol=1e-15)
+    assert ax[0] == int(np.argmax(expect))
+
+
+def test_rule9_error_vanishes_for_cubics_and_axis_falls_back():
+    d = 4
+    f = W.polynomial(d, [(1.0, [1, 2, 0, 0]), (0.5, [0, 0, 3, 0]), (2.0, [0, 0, 0, 1])])
+    lo = np.array([[0.0, 0.0, 0.0, 0.0]])
+    wd = np.array([[0.1, 0.3, 0.2, 0.3]])
+    est, err, ax, ab, dv, sums = O.region_eval9(f, lo, wd)
+    ex = float(_poly_exact_fraction([(1.0, [1, 2, 0, 0]), (0.5, [0, 0, 3, 0]), (2.0, [0, 0, 0, 1])],
+                                    lo[0], lo[0] + wd[0]))
+    assert abs(est[0] - ex) <= 1e-14 * ab[0]
+    assert err[0] <= 1e-15 * ab[0]
+    assert np.all(dv[0] <= 1e-15 * ab[0] / np.prod(wd[0]))  # zero up to rounding
+    # every D_i exactly 0 (the zero function): the first widest axis
+    z = W.polynomial(d, [(0.0, [0, 0, 0, 0])])
+    est, err, ax, ab, dv, sums = O.region_eval9(z, lo, wd)
+    assert np.all(dv[0] == 0.0) and ax[0] == 1
+
+
+def test_rule9_loop_gaussian():
+    p = W.c1_gaussian3()
+    r9 = O.integrate_bfs(p, key=9)
+    r7 = O.integrate_bfs(p, key=7)
+    ex, _ = O.exact(p.f, p.lows, p.highs)
+    assert r9["rc"] == 0
+    assert r9["error"] <= p.epsrel * abs(r9["integral"])
+    assert abs(r9["integral"] - ex) <= r9["error"]
+    assert r9["neval"] == r9["nregions"] * 77
+    assert r7["rc"] == 0 and r7["neval"] == r7["nregions"] * 33
