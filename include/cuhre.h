@@ -111,15 +111,18 @@ int cuhre_context_destroy(cuhre_context* ctx);
 int cuhre_context_set(cuhre_context* ctx, int profile, int point_segments);
 
 /* Plain adaptive integration over the box [lows, highs].  key selects the
- * rule: 0 or 7 = Genz-Malik degree 7/5.  Returns the fail flag (0/1) or a
- * negative error code. */
+ * rule: 0 or 7 = Genz-Malik degree 7/5 (err = |I7 - I5|); 9 = the
+ * reference's degree-9 rule with four null rules (quad/GPUquad/GPUQuadRule.cu:30-226,
+ * error as quad/GPUquad/GPUQuadSample.cu:195-216).  Returns the fail flag
+ * (0/1) or a negative error code. */
 int cuhre_integrate(cuhre_context* ctx, const cuhre_integrand* f, int dim,
                     double epsrel, double epsabs, int64_t maxeval, int key,
                     const double* lows, const double* highs, cuhre_result* out);
 
 /* Rule evaluation of n regions given on the host (SoA: lower[m*n + r],
  * width[m*n + r]).  Outputs per region: est, err, axis, abssum (volume *
- * sum |w7 f|); i5 and dvec (d*n, SoA) may be NULL. */
+ * sum |w f| of the basic rule); i5 (degree-5 estimate; 0 under key 9) and
+ * dvec (d*n, SoA) may be NULL. */
 int cuhre_eval_regions(cuhre_context* ctx, const cuhre_integrand* f, int dim,
                        int key, int64_t n, const double* lower,
                        const double* width, double* est, double* err,

@@ -38,7 +38,7 @@
 
 namespace cuhre {
 
-RuleConst make_rule(int d);
+RuleConst make_rule(int d, int key);
 int gm_points(int d);
 int set_error(int code, const char* msg);
 
@@ -888,7 +888,7 @@ int cuhre_batch_autotune(cuhre_context* ctx, int32_t count, const cuhre_integran
   e->h_n[parity] = n;
   if (hipMemcpyAsync(e->n + parity, e->h_n + parity, sizeof n, hipMemcpyHostToDevice, st) != hipSuccess)
     return set_error(CUHRE_ERR_DEVICE, "autotune copy failed");
-  const RuleConst R = make_rule(dim);
+  const RuleConst R = make_rule(dim, 7);
   const BatchBuffers B = make_buffers(e, parity, count, dim);
   const int Ss[3] = {1, 2, 4};
   const int Gs[5] = {256, 512, 1024, 2048, 4096};
@@ -965,7 +965,7 @@ int cuhre_integrate_batch(cuhre_context* ctx, int32_t count, const cuhre_integra
                    std::min<int64_t>(limit, std::max<int64_t>(e->cap, first)), &n0);
   if (rc) return rc;
   e->allow_cut = e->cap >= limit ? 1 : 0;
-  const RuleConst R = make_rule(dim);
+  const RuleConst R = make_rule(dim, 7);
   bool graph_ok = use_graph && capture_graph(e, count, dim, R, S, G, st);
   int iters = 0, grows = 0;
   for (int check = 0; check < 2000; ++check) {

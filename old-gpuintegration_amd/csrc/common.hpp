@@ -33,12 +33,24 @@ struct DevIntegrand {
 
 // Genz-Malik degree-7/5 rule constants, weights normalised to sum to one
 // (est = volume * sum w f).  Built on the host by rule.cpp.
+// The reference's degree-9 rule (KEY 9): nine generator sets, a basic rule
+// and four null rules (weights per set and rule), radii t[] in half-width
+// units, and the null-rule combination constants.
+struct Rule9Const {
+  double t[5];
+  double w[9][5];
+  double scale[9][5], norm[9][5];
+  double ratio;  // (t2/t1)^2, fourth differences
+};
+
 struct RuleConst {
   double l2, l3, l4, l5;
   double w7c, w7a2, w7a3, w7p, w7v;  // centre, axis l2, axis l3, pair l4, corner l5
   double w5c, w5a2, w5a3, w5p;       // embedded degree-5 rule (no corners)
   double ratio;                      // (l2/l3)^2 = 1/7
   int32_t npts;
+  int32_t key;                       // 7: the rule above; 9: r9
+  Rule9Const r9;
 };
 
 // Per-block partial sums written by the evaluation kernel.  The *_s/*_c
@@ -91,7 +103,7 @@ struct EvalArgs {
 using EvalLauncher = void (*)(const EvalArgs&, const DevIntegrand&, const RuleConst&, int S,
                               hipStream_t);
 
-// Returns the launcher for (family, dim) or nullptr.
-EvalLauncher find_eval_launcher(int family, int dim);
+// Returns the launcher for (family, dim, rule key 0/7 or 9) or nullptr.
+EvalLauncher find_eval_launcher(int family, int dim, int key = 7);
 
 }  // namespace cuhre

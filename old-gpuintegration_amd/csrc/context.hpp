@@ -86,7 +86,7 @@ int ctx_reserve(cuhre_context* c, int64_t cap, int dim, int64_t keep_n = 0, int 
 int ctx_reserve_aux(cuhre_context* c, int64_t cap, int dim);
 int make_dev_integrand(cuhre_context* c, const cuhre_integrand* f, int dim, DevIntegrand* P);
 int64_t list_capacity(cuhre_context* c, int dim);
-int pick_segments(const cuhre_context* c, int64_t n);
+int pick_segments(const cuhre_context* c, int64_t n, int key = 7);
 int64_t initial_grid(int d, const double* lows, const double* highs, std::vector<double>& lo,
                      std::vector<double>& wd);
 int check_common(cuhre_context* ctx, const cuhre_integrand* f, int dim, int key);
@@ -95,8 +95,9 @@ void fill_split_args(cuhre_context* c, SplitArgs& S, int cur, int64_t n, int dim
                      int rpb, bool use_state);
 void dd_add_host(double& s1, double& c1, double s2, double c2);
 int64_t blocks_for(int64_t n, int S);
-RuleConst make_rule(int d);
+RuleConst make_rule(int d, int key = 7);
 int gm_points(int d);
+int r9_points(int d);
 int set_error(int code, const char* msg);
 
 }  // namespace cuhre

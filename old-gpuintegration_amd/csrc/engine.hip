@@ -29,26 +29,28 @@
 
 namespace cuhre {
 
-EvalLauncher launcher_gaussian(int);
-EvalLauncher launcher_product_peak(int);
-EvalLauncher launcher_oscillatory(int);
-EvalLauncher launcher_corner_peak(int);
-EvalLauncher launcher_continuous(int);
-EvalLauncher launcher_polynomial(int);
-EvalLauncher launcher_cosine_sum(int);
-EvalLauncher launcher_table6(int);
+EvalLauncher launcher_gaussian(int, int);
+EvalLauncher launcher_product_peak(int, int);
+EvalLauncher launcher_oscillatory(int, int);
+EvalLauncher launcher_corner_peak(int, int);
+EvalLauncher launcher_continuous(int, int);
+EvalLauncher launcher_polynomial(int, int);
+EvalLauncher launcher_cosine_sum(int, int);
+EvalLauncher launcher_table6(int, int);
 
-EvalLauncher find_eval_launcher(int family, int dim)
+EvalLauncher find_eval_launcher(int family, int dim, int key)
 {
+  if (key == 0) key = 7;
+  if (key != 7 && key != 9) return nullptr;
   switch (family) {
-  case CUHRE_GAUSSIAN: return launcher_gaussian(dim);
-  case CUHRE_PRODUCT_PEAK: return launcher_product_peak(dim);
-  case CUHRE_OSCILLATORY: return launcher_oscillatory(dim);
-  case CUHRE_CORNER_PEAK: return launcher_corner_peak(dim);
-  case CUHRE_CONTINUOUS: return launcher_continuous(dim);
-  case CUHRE_POLYNOMIAL: return launcher_polynomial(dim);
-  case CUHRE_COSINE_SUM: return launcher_cosine_sum(dim);
-  case CUHRE_TABLE6: return launcher_table6(dim);
+  case CUHRE_GAUSSIAN: return launcher_gaussian(dim, key);
+  case CUHRE_PRODUCT_PEAK: return launcher_product_peak(dim, key);
+  case CUHRE_OSCILLATORY: return launcher_oscillatory(dim, key);
+  case CUHRE_CORNER_PEAK: return launcher_corner_peak(dim, key);
+  case CUHRE_CONTINUOUS: return launcher_continuous(dim, key);
+  case CUHRE_POLYNOMIAL: return launcher_polynomial(dim, key);
+  case CUHRE_COSINE_SUM: return launcher_cosine_sum(dim, key);
+  case CUHRE_TABLE6: return launcher_table6(dim, key);
   default: return nullptr;
   }
 }
@@ -88,9 +90,72 @@ int gm_points(int d) { return (1 << d) + 2 * d * d + 2 * d + 1; }
 
 // Genz-Malik (1980) degree-7 rule and embedded degree-5 rule; weights on
 // [-1,1]^d divided by the volume 2^d.
-RuleConst make_rule(int d)
+int r9_points(int d)
+{
+  return 1 + 8 * d + 2 * d * (d - 1) + 4 * d * (d - 1) + 4 * d * (d - 1) * (d - 2) / 3 + (1 << d);
+}
+
+// The reference's degree-9 rule (KEY 9): radii and weights as
+// quad/GPUquad/GPUQuadRule.cu:33-176 defines them (they define the rule),
+// null-rule scale/norm as :209-225.
+static Rule9Const make_rule9(int d)
+{
+  Rule9Const R;
+  const double n = (double)d;
+  const double a[5] = {.47795365790226950619, .20302858736911986780, .44762735462617812882, .125,
+                       .34303789878087814570};
+  for (int k = 0; k < 5; ++k) R.t[k] = 2.0 * a[k];  // fractions of the width -> half-width units
+  const double A = n * (n * (n * (-.0023611709677855117884) + .11415390023857325268) + (-.63833920076702389094)) +
+                   .74849988504685208004;
+  const double B = n * (n * (n * (-.0014324017033399125142) + .057471507864489725949) + (-.14225104571434243234)) -
+                   (-.062875028738286979989);
+  const double C = n * (n * .0035417564516782676826 + (-.072609367395893679605)) + .10557491625218991012;
+  const double D = n * (n * .0021486025550098687713 + (-.032268563892953949998)) + .010636783990231217481;
+  const double E = n * (-.04508628929435784076) + .21415883524352793401;
+  const double F = n * (-.027351546526545644722) + .054941067048711234101;
+  const double G = .015532417276607053264 - n * .0035417564516782676826;
+  const double H = .0035328099607090870236 - n * .0021486025550098687713;
+  const double c = (double)(1 << d);
+  const double w[9][5] = {
+      {A, B, n * .254591133248959089 - B, n * (n * (-1.207328566678236261) + .89567365764160676508) - 1 + A,
+       n * (-.36479356986049146661) + 1 - A},
+      {C, D, .014689102496143490175 - D, n * .51134708346467591431 + .45976448120806344646 + C,
+       .18239678493024573331 - C},
+      {E, F, .11937596202570775297 - F, n * .65089519391920250593 + .14744939829434460168, -E},
+      {.057693384490973483573, .034999626602143583822, -.057693384490973483573, -1.3868627719278281436,
+       -.057693384490973483573},
+      {0, 0, -.2386668732575008879, 0, 0},
+      {G, H, -H, .09231719987444221619 + G, -G},
+      {.02254314464717892038, .013675773263272822361, -.013675773263272822361, -.32544759695960125297,
+       -.02254314464717892038},
+      {.0017708782258391338413, .0010743012775049343856, -.0010743012775049343856, .0017708782258391338413,
+       -.0017708782258391338413},
+      {.25150011495314791996 / c, -.062875028738286979989 / c, .062875028738286979989 / c,
+       .25150011495314791996 / c, -.25150011495314791996 / c}};
+  const int cnt[9] = {1, 2 * d, 2 * d, 2 * d, 2 * d, 2 * d * (d - 1), 4 * d * (d - 1),
+                      4 * d * (d - 1) * (d - 2) / 3, 1 << d};
+  for (int s = 0; s < 9; ++s)
+    for (int q = 0; q < 5; ++q) {
+      R.w[s][q] = w[s][q];
+      R.scale[s][q] = 0.0;
+      R.norm[s][q] = 0.0;
+    }
+  for (int s = 0; s < 9; ++s)
+    for (int q = 1; q < 4; ++q) {
+      const double sc = w[s][q] == 0.0 ? 100.0 : -w[s][q + 1] / w[s][q];
+      double sum = 0.0;
+      for (int x = 0; x < 9; ++x) sum += (double)cnt[x] * std::fabs(w[x][q + 1] + sc * w[x][q]);
+      R.scale[s][q] = sc;
+      R.norm[s][q] = 1.0 / sum;
+    }
+  R.ratio = (R.t[1] / R.t[0]) * (R.t[1] / R.t[0]);
+  return R;
+}
+
+RuleConst make_rule(int d, int key)
 {
   RuleConst R;
+  std::memset(&R, 0, sizeof R);
   const double dd = (double)d;
   R.l2 = std::sqrt(9.0 / 70.0);
   R.l3 = std::sqrt(9.0 / 10.0);
@@ -106,7 +171,9 @@ RuleConst make_rule(int d)
   R.w5a3 = (265.0 - 100.0 * dd) / 1458.0;
   R.w5p = 25.0 / 729.0;
   R.ratio = 1.0 / 7.0;
-  R.npts = gm_points(d);
+  R.key = key == 9 ? 9 : 7;
+  R.npts = key == 9 ? r9_points(d) : gm_points(d);
+  if (key == 9) R.r9 = make_rule9(d);
   return R;
 }
 
@@ -144,11 +211,18 @@ double family_flops(const cuhre_integrand* f, int d)
   }
 }
 
-// Per region: every point costs its integrand plus d coordinate FMAs; the
-// sums take three FMAs (I7, I5, sum |w f|) at centre/axis/pair points and
-// two at the corners (no degree-5 weight); plus the per-region setup.
-double region_flops(const cuhre_integrand* f, int d)
+// Per region: every point costs its integrand plus d coordinate FMAs.
+// GM7/5: the sums take three FMAs (I7, I5, sum |w f|) at centre/axis/pair
+// points and two at the corners (no degree-5 weight).  Degree 9: two adds
+// per point (f and |f| into the set group), six FMAs per set group, the
+// null-rule combination.  Plus the per-region setup.
+double region_flops(const cuhre_integrand* f, int d, int key)
 {
+  if (key == 9) {
+    const double p = r9_points(d);
+    const double groups = 1 + 4.0 * d + d * (d - 1) + d * (d - 1) * (d - 2) / 6.0 + 1;
+    return p * (family_flops(f, d) + 2.0 * d + 2.0) + 12.0 * groups + 3 * 9 * 4.0 + (8.0 * d + 12.0);
+  }
   const double p = gm_points(d), corners = (double)(1 << d);
   return p * (family_flops(f, d) + 2.0 * d) + 6.0 * (p - corners) + 4.0 * corners + (8.0 * d + 12.0);
 }
@@ -454,11 +528,12 @@ int64_t list_capacity(cuhre_context* c, int dim)
   return (int64_t)(((double)fr + held) * 0.75 / per);
 }
 
-int pick_segments(const cuhre_context* c, int64_t n)
+int pick_segments(const cuhre_context* c, int64_t n, int key)
 {
-  if (c->segments > 0) return c->segments;
+  const int smax = key == 9 ? 8 : kMaxWaves;  // the degree-9 merge needs more LDS per wave
+  if (c->segments > 0) return std::min(c->segments, smax);
   int S = 1;
-  while (S < kMaxWaves && n * S < c->fill_lanes) S *= 2;
+  while (S < smax && n * S < c->fill_lanes) S *= 2;
   if (S == 1 && n < (2ll << 20)) S = 2;
   return S;
 }
@@ -494,8 +569,9 @@ int check_common(cuhre_context* ctx, const cuhre_integrand* f, int dim, int key)
   if (!f) return fail(CUHRE_ERR_ARG, "null integrand");
   if (dim < 2 || dim > 10) return fail(CUHRE_ERR_UNSUPPORTED, "dimension %d outside 2..10", dim);
   if (f->dim != dim) return fail(CUHRE_ERR_ARG, "integrand dim %d != dim %d", f->dim, dim);
-  if (key != 0 && key != 7) return fail(CUHRE_ERR_UNSUPPORTED, "rule key %d not available (0 or 7)", key);
-  if (!find_eval_launcher(f->family, dim))
+  if (key != 0 && key != 7 && key != 9)
+    return fail(CUHRE_ERR_UNSUPPORTED, "rule key %d not available (0 or 7: Genz-Malik 7/5; 9: degree 9)", key);
+  if (!find_eval_launcher(f->family, dim, key))
     return fail(CUHRE_ERR_UNSUPPORTED, "family %d not compiled for dim %d", f->family, dim);
   HIP_TRY(hipSetDevice(ctx->device));
   return 0;
@@ -564,8 +640,8 @@ const char* cuhre_last_error(void) { return g_err.c_str(); }
 
 int64_t cuhre_points_per_region(int dim, int key)
 {
-  if (dim < 2 || dim > kMaxDim || (key != 0 && key != 7)) return -1;
-  return gm_points(dim);
+  if (dim < 2 || dim > kMaxDim || (key != 0 && key != 7 && key != 9)) return -1;
+  return key == 9 ? r9_points(dim) : gm_points(dim);
 }
 
 int cuhre_context_create(cuhre_context** out, int device, int64_t capacity_regions)
@@ -684,9 +760,9 @@ int cuhre_integrate(cuhre_context* c, const cuhre_integrand* f, int dim, double 
 
   DevIntegrand P;
   if ((rc = make_dev_integrand(c, f, dim, &P))) return rc;
-  const RuleConst R = make_rule(dim);
+  const RuleConst R = make_rule(dim, key);
   const int64_t p = R.npts;
-  const double rflops = region_flops(f, dim);
+  const double rflops = region_flops(f, dim, key);
 
   std::vector<double> hlo, hwd;
   const int64_t n0 = initial_grid(dim, lows, highs, hlo, hwd);
@@ -705,8 +781,8 @@ int cuhre_integrate(cuhre_context* c, const cuhre_integrand* f, int dim, double 
   const double u = 0.5 * 2.220446049250313e-16;
   int result = CUHRE_NOT_CONVERGED;
   for (int it = 0; it < 1000; ++it) {
-    const int S = pick_segments(c, n);
-    const EvalLauncher launch = find_eval_launcher(f->family, dim);
+    const int S = pick_segments(c, n, key);
+    const EvalLauncher launch = find_eval_launcher(f->family, dim, key);
     EvalArgs A;
     fill_eval_args(c, A, cur, n, dim, epsrel, epsabs);
     A.finalize = 1;
@@ -826,7 +902,7 @@ int cuhre_eval_regions(cuhre_context* c, const cuhre_integrand* f, int dim, int 
   if (!lower || !width || !est || !err || !axis || !abssum) return fail(CUHRE_ERR_ARG, "null buffer");
   DevIntegrand P;
   if ((rc = make_dev_integrand(c, f, dim, &P))) return rc;
-  const RuleConst R = make_rule(dim);
+  const RuleConst R = make_rule(dim, key);
   if ((rc = ctx_reserve(c, n, dim))) return rc;
   const bool want_aux = i5 != nullptr || dvec != nullptr;
   if (want_aux && (rc = ctx_reserve_aux(c, c->cap, dim))) return rc;
@@ -834,7 +910,7 @@ int cuhre_eval_regions(cuhre_context* c, const cuhre_integrand* f, int dim, int 
                            sizeof(double) * n, dim, hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipMemcpy2DAsync(c->wd[0], sizeof(double) * c->cap, width, sizeof(double) * n,
                            sizeof(double) * n, dim, hipMemcpyHostToDevice, c->stream));
-  const int S = pick_segments(c, n);
+  const int S = pick_segments(c, n, key);
   EvalArgs A;
   fill_eval_args(c, A, 0, n, dim, 0.0, 0.0);
   A.finalize = 0;
@@ -842,7 +918,7 @@ int cuhre_eval_regions(cuhre_context* c, const cuhre_integrand* f, int dim, int 
     A.i5 = c->aux;
     A.dvec = c->aux + c->cap;
   }
-  find_eval_launcher(f->family, dim)(A, P, R, S, c->stream);
+  find_eval_launcher(f->family, dim, key)(A, P, R, S, c->stream);
   HIP_TRY(hipGetLastError());
   std::vector<int8_t> ax8((size_t)n);
   HIP_TRY(hipMemcpyAsync(est, c->est, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));

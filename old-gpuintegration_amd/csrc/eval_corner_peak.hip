@@ -1,6 +1,7 @@
-// Instantiates the rule-evaluation kernels of the corner_peak family for d = 2..10.
+// Instantiates the rule-evaluation kernels of the corner_peak family for d = 2..10, for the
+// degree-7/5 rule and the degree-9 rule.
 #include "eval_kernels.hpp"
 
 namespace cuhre {
-EvalLauncher launcher_corner_peak(int dim) { return family_launcher<CUHRE_CORNER_PEAK>(dim); }
+EvalLauncher launcher_corner_peak(int dim, int key) { return family_launcher<CUHRE_CORNER_PEAK>(dim, key); }
 }  // namespace cuhre

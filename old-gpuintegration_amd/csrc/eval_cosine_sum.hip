@@ -1,6 +1,7 @@
-// Instantiates the rule-evaluation kernels of the cosine_sum family for d = 2..10.
+// Instantiates the rule-evaluation kernels of the cosine_sum family for d = 2..10, for the
+// degree-7/5 rule and the degree-9 rule.
 #include "eval_kernels.hpp"
 
 namespace cuhre {
-EvalLauncher launcher_cosine_sum(int dim) { return family_launcher<CUHRE_COSINE_SUM>(dim); }
+EvalLauncher launcher_cosine_sum(int dim, int key) { return family_launcher<CUHRE_COSINE_SUM>(dim, key); }
 }  // namespace cuhre

@@ -237,10 +237,274 @@ void launch_eval(const EvalArgs& A, const DevIntegrand& P, const RuleConst& R, i
                      seg_lds_bytes(S), st, A, P, R, S);
 }
 
-// Launchers for one family over dimensions 2..10.
-template <int FAM>
-EvalLauncher family_launcher(int dim)
+// ---------------------------------------------------------------- degree 9
+// The reference's degree-9 rule (KEY 9; quad/GPUquad/GPUQuadRule.cu:30-226,
+// evaluated per region by quad/GPUquad/GPUQuadSample.cu:99-220), on the same
+// lane-per-region mapping.  The point sets are walked set by set: the
+// centre; per axis i the eight points +-t1..t4 along i (whose first two
+// radii give the fourth difference); per pair i<j the four (+-t1, +-t1) and
+// the eight (+-t1, +-t2) / (+-t2, +-t1) points; per triple i<j<k the eight
+// (+-t1)^3 points; the 2^d corners at +-t5.  f is summed per set group and
+// the group sums enter the five rule sums with the set's weights.  Segments
+// take axes, pairs, triples and corners round robin as in seg_eval.
+
+inline size_t seg9_lds_bytes(int S)
 {
+  if (S <= 1) return 0;
+  const int W = S < 4 ? 4 : S;
+  return (size_t)W * kWave * (7 * sizeof(double) + sizeof(int32_t));
+}
+
+template <int D, class F>
+__device__ __forceinline__ RegionOut seg_eval9(const F& f, const double* __restrict__ lower,
+                                               const double* __restrict__ width, int64_t ld, int64_t r,
+                                               bool valid, int64_t rr, int S, int s, int wave, int lane, int W,
+                                               double* seg_lds, const Rule9Const& R, double* dvec)
+{
+  double c[D], h[D], x[D];
+  double vol = 1.0, maxw = 0.0;
+  int widest = 0;
+#pragma unroll
+  for (int m = 0; m < D; ++m) {
+    const double lo = lower[m * ld + rr];
+    const double w = width[m * ld + rr];
+    c[m] = fma(0.5, w, lo);
+    h[m] = 0.5 * w;
+    vol *= w;
+    if (w > maxw) {
+      maxw = w;
+      widest = m;
+    }
+  }
+  double sum[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  double sa = 0.0;
+  // adds one set group (sum of f, sum of |f|) to the rule sums
+  auto group = [&](int set, double fs, double fa) {
+#pragma unroll
+    for (int q = 0; q < 5; ++q) sum[q] = fma(R.w[set][q], fs, sum[q]);
+    sa = fma(fabs(R.w[set][0]), fa, sa);
+  };
+  const double f0 = f(c);
+  if (s == 0) group(0, f0, fabs(f0));
+  double maxd = 0.0;
+  int ax = -1;
+
+  // axis sets 1..4
+#pragma unroll 1
+  for (int i = s; i < D; i += S) {
+    double fv[2][2];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      double fs = 0.0, fa = 0.0;
+#pragma unroll
+      for (int sg = 0; sg < 2; ++sg) {
+        const double gv = sg ? -R.t[k] : R.t[k];
+#pragma unroll
+        for (int m = 0; m < D; ++m) x[m] = fma(h[m], m == i ? gv : 0.0, c[m]);
+        const double v = f(x);
+        if (k < 2) fv[k][sg] = v;
+        fs += v;
+        fa += fabs(v);
+      }
+      group(1 + k, fs, fa);
+    }
+    const double di = fabs(2.0 * (1.0 - R.ratio) * f0 + R.ratio * (fv[0][0] + fv[0][1]) - (fv[1][0] + fv[1][1]));
+    if (dvec != nullptr && valid) dvec[i * ld + r] = di;
+    if (di > maxd) {
+      maxd = di;
+      ax = i;
+    }
+  }
+
+  // pair sets 5 and 6, pair q = (i, j) in the order (0,1),(0,2),...,(D-2,D-1)
+  constexpr int NPAIR = D * (D - 1) / 2;
+#pragma unroll 1
+  for (int q = s; q < NPAIR; q += S) {
+    int i = 0, rem = q;
+    while (rem >= D - 1 - i) {
+      rem -= D - 1 - i;
+      ++i;
+    }
+    const int j = i + 1 + rem;
+    double fs = 0.0, fa = 0.0;
+#pragma unroll
+    for (int sg = 0; sg < 4; ++sg) {
+      const double gi = (sg & 1) ? -R.t[0] : R.t[0], gj = (sg & 2) ? -R.t[0] : R.t[0];
+#pragma unroll
+      for (int m = 0; m < D; ++m) x[m] = fma(h[m], m == i ? gi : (m == j ? gj : 0.0), c[m]);
+      const double v = f(x);
+      fs += v;
+      fa += fabs(v);
+    }
+    group(5, fs, fa);
+    fs = 0.0;
+    fa = 0.0;
+#pragma unroll
+    for (int o = 0; o < 2; ++o)
+#pragma unroll
+      for (int sg = 0; sg < 4; ++sg) {
+        const double ti = o ? R.t[1] : R.t[0], tj = o ? R.t[0] : R.t[1];
+        const double gi = (sg & 1) ? -ti : ti, gj = (sg & 2) ? -tj : tj;
+#pragma unroll
+        for (int m = 0; m < D; ++m) x[m] = fma(h[m], m == i ? gi : (m == j ? gj : 0.0), c[m]);
+        const double v = f(x);
+        fs += v;
+        fa += fabs(v);
+      }
+    group(6, fs, fa);
+  }
+
+  // triple set 7, triples in lexicographic order
+  constexpr int NTRI = D * (D - 1) * (D - 2) / 6;
+#pragma unroll 1
+  for (int q = s; q < NTRI; q += S) {
+    int i = 0, rem = q;
+    while (rem >= (D - 1 - i) * (D - 2 - i) / 2) {
+      rem -= (D - 1 - i) * (D - 2 - i) / 2;
+      ++i;
+    }
+    int j = i + 1;
+    while (rem >= D - 1 - j) {
+      rem -= D - 1 - j;
+      ++j;
+    }
+    const int k = j + 1 + rem;
+    double fs = 0.0, fa = 0.0;
+#pragma unroll 2
+    for (int sg = 0; sg < 8; ++sg) {
+      const double gi = (sg & 1) ? -R.t[0] : R.t[0], gj = (sg & 2) ? -R.t[0] : R.t[0],
+                   gk = (sg & 4) ? -R.t[0] : R.t[0];
+#pragma unroll
+      for (int m = 0; m < D; ++m)
+        x[m] = fma(h[m], m == i ? gi : (m == j ? gj : (m == k ? gk : 0.0)), c[m]);
+      const double v = f(x);
+      fs += v;
+      fa += fabs(v);
+    }
+    group(7, fs, fa);
+  }
+
+  // corner set 8
+  {
+    double fs = 0.0, fa = 0.0;
+#pragma unroll 2
+    for (int k = s; k < (1 << D); k += S) {
+#pragma unroll
+      for (int m = 0; m < D; ++m) x[m] = fma(h[m], ((k >> m) & 1) ? -R.t[4] : R.t[4], c[m]);
+      const double v = f(x);
+      fs += v;
+      fa += fabs(v);
+    }
+    group(8, fs, fa);
+  }
+
+  if (S > 1) {
+    double* const sh = seg_lds;  // [W][7][64] doubles, then [W][64] ints
+    int32_t* const shax = reinterpret_cast<int32_t*>(seg_lds + (size_t)W * 7 * kWave);
+    if (s != 0) {
+#pragma unroll
+      for (int q = 0; q < 5; ++q) sh[(wave * 7 + q) * kWave + lane] = sum[q];
+      sh[(wave * 7 + 5) * kWave + lane] = sa;
+      sh[(wave * 7 + 6) * kWave + lane] = maxd;
+      shax[wave * kWave + lane] = ax;
+    }
+    __syncthreads();
+    if (s == 0) {
+      for (int t = 1; t < S; ++t) {
+        const int w2 = wave + t;
+#pragma unroll
+        for (int q = 0; q < 5; ++q) sum[q] += sh[(w2 * 7 + q) * kWave + lane];
+        sa += sh[(w2 * 7 + 5) * kWave + lane];
+        const double d2 = sh[(w2 * 7 + 6) * kWave + lane];
+        const int a2 = shax[w2 * kWave + lane];
+        if (d2 > maxd || (d2 == maxd && d2 > 0.0 && a2 < ax)) {
+          maxd = d2;
+          ax = a2;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (ax < 0) ax = widest;
+
+  // the largest normalised null rule in the span of two successive ones
+  // (GPUQuadSample.cu:195-216, errcoeff {5, 1, 5})
+  double e[4];
+#pragma unroll
+  for (int q = 1; q < 4; ++q) {
+    double mx = 0.0;
+#pragma unroll
+    for (int t = 0; t < 9; ++t) mx = fmax(mx, fabs(sum[q + 1] + R.scale[t][q] * sum[q]) * R.norm[t][q]);
+    e[q] = mx;
+  }
+  RegionOut o;
+  o.est = vol * sum[0];
+  o.err = vol * ((5.0 * e[1] <= e[2] && 5.0 * e[2] <= e[3]) ? e[1] : 5.0 * fmax(fmax(e[1], e[2]), e[3]));
+  o.ab = vol * sa;
+  o.i5 = 0.0;
+  o.axis = ax;
+  return o;
+}
+
+template <int D, int FAM>
+__global__ __launch_bounds__(512) void k_eval9_seg(EvalArgs A, DevIntegrand P, RuleConst R, int S)
+{
+  extern __shared__ double seg_lds[];
+  __shared__ Partial ldsw[kMaxWaves];
+  const int W = S < 4 ? 4 : S;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & (kWave - 1);
+  const int g = wave / S, s = wave - g * S;
+  const int64_t r = (int64_t)blockIdx.x * (kWave * (W / S)) + g * kWave + lane;
+  const bool valid = r < A.n;
+  const int64_t rr = valid ? r : A.n - 1;
+  const PlainIntegrand<D, FAM> f{P};
+  const RegionOut o = seg_eval9<D>(f, A.lower, A.width, A.ld, r, valid, rr, S, s, wave, lane, W, seg_lds, R.r9,
+                                   A.dvec);
+  const bool owner = s == 0;
+  if (valid && owner) {
+    A.est[r] = o.est;
+    A.err[r] = o.err;
+    A.axis[r] = (int8_t)o.axis;
+    if (A.abssum) A.abssum[r] = o.ab;
+    if (A.i5) A.i5[r] = o.i5;
+  }
+  if (A.finalize) {
+    Partial p;
+    partial_from_region(p, o.est, o.err, o.ab, valid && owner, A.loc_epsrel, A.loc_epsabs);
+    block_reduce_partial(p, ldsw, W, owner);
+    finish_iteration(A, p, ldsw, W);
+  }
+}
+
+template <int D, int FAM>
+void launch_eval9(const EvalArgs& A, const DevIntegrand& P, const RuleConst& R, int S, hipStream_t st)
+{
+  if (S > 8) S = 8;  // LDS of the merge: 7 doubles + 1 int per lane and wave
+  const int rpb = seg_regions_per_block(S);
+  const int64_t nb = (A.n + rpb - 1) / rpb;
+  hipLaunchKernelGGL((k_eval9_seg<D, FAM>), dim3((unsigned)nb), dim3(kWave * seg_waves(S)), seg9_lds_bytes(S), st,
+                     A, P, R, S);
+}
+
+// Launchers for one family over dimensions 2..10 (key 9: the degree-9 rule).
+template <int FAM>
+EvalLauncher family_launcher(int dim, int key = 7)
+{
+  if (key == 9) {
+    switch (dim) {
+    case 2: return &launch_eval9<2, FAM>;
+    case 3: return &launch_eval9<3, FAM>;
+    case 4: return &launch_eval9<4, FAM>;
+    case 5: return &launch_eval9<5, FAM>;
+    case 6: return &launch_eval9<6, FAM>;
+    case 7: return &launch_eval9<7, FAM>;
+    case 8: return &launch_eval9<8, FAM>;
+    case 9: return &launch_eval9<9, FAM>;
+    case 10: return &launch_eval9<10, FAM>;
+    default: return nullptr;
+    }
+  }
   switch (dim) {
   case 2: return &launch_eval<2, FAM>;
   case 3: return &launch_eval<3, FAM>;

@@ -1,6 +1,7 @@
-// Instantiates the rule-evaluation kernels of the polynomial family for d = 2..10.
+// Instantiates the rule-evaluation kernels of the polynomial family for d = 2..10, for the
+// degree-7/5 rule and the degree-9 rule.
 #include "eval_kernels.hpp"
 
 namespace cuhre {
-EvalLauncher launcher_polynomial(int dim) { return family_launcher<CUHRE_POLYNOMIAL>(dim); }
+EvalLauncher launcher_polynomial(int dim, int key) { return family_launcher<CUHRE_POLYNOMIAL>(dim, key); }
 }  // namespace cuhre

@@ -2,5 +2,9 @@
 #include "eval_kernels.hpp"
 
 namespace cuhre {
-EvalLauncher launcher_table6(int dim) { return dim == 6 ? &launch_eval<6, CUHRE_TABLE6> : nullptr; }
+EvalLauncher launcher_table6(int dim, int key)
+{
+  if (dim != 6) return nullptr;
+  return key == 9 ? &launch_eval9<6, CUHRE_TABLE6> : &launch_eval<6, CUHRE_TABLE6>;
+}
 }  // namespace cuhre

@@ -34,7 +34,9 @@ def test_library_exports_every_declared_symbol():
 def test_points_per_region_without_device():
     assert [cuhre.points_per_region(d) for d in (3, 5, 6, 7, 8)] == [33, 93, 149, 241, 401]
     assert cuhre.points_per_region(1) == -1
-    assert cuhre.points_per_region(8, key=9) == -1
+    # key 9: the reference's degree-9 rule, 1 + 8d + 6d(d-1) + 4d(d-1)(d-2)/3 + 2^d points
+    assert [cuhre.points_per_region(d, key=9) for d in (2, 3, 5, 7, 8)] == [33, 77, 273, 717, 1105]
+    assert cuhre.points_per_region(8, key=5) == -1
 
 
 def test_context_without_device_is_an_error_code():

@@ -250,7 +250,7 @@ def test_errors_are_codes(ctx):
         ctx.integrate(bad)
     assert e.value.code == cuhre.ERR_NONFINITE
     with pytest.raises(cuhre.CuhreError) as e:
-        ctx.integrate(p, key=9)
+        ctx.integrate(p, key=5)
     assert e.value.code == cuhre.ERR_UNSUPPORTED
     with pytest.raises(cuhre.CuhreError) as e:
         ctx.integrate(W.Problem("box", p.f, [0, 0, 1], [1, 1, 1], 1e-6, 0, 10**6))
