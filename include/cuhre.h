@@ -119,6 +119,23 @@ int cuhre_integrate(cuhre_context* ctx, const cuhre_integrand* f, int dim,
                     double epsrel, double epsabs, int64_t maxeval, int key,
                     const double* lows, const double* highs, cuhre_result* out);
 
+/* The reference's two-phase method (quad/GPUquad/GPUquad.h:611-727):
+ * Phase 1 is the breadth-first loop above, at most 100 iterations, until
+ * the global test holds, no region is active, or a split leaves
+ * phase1_regions regions or more (0: 2^14, FIRST_PHASE_MAXREGIONS); Phase 2
+ * then refines each of those regions on its own
+ * (quad/GPUquad/GPUQuadPhases.cu:422-536): a max-error heap, at most
+ * pool_max bisections (0: 2048, MAX_GLOBALPOOL_SIZE), until the region's
+ * own sum meets max(epsrel |sum|, epsabs), children's errors refined from
+ * the parent difference (quad/GPUquad/GPUQuadPhases.cu:498-515).  Returns
+ * 0 when the global test holds, else 1 with out->fail = the number of
+ * Phase-2 regions that missed their own test (the reference's flag). */
+int cuhre_integrate_two_phase(cuhre_context* ctx, const cuhre_integrand* f, int dim,
+                              double epsrel, double epsabs, int key,
+                              int64_t phase1_regions, int32_t pool_max,
+                              const double* lows, const double* highs,
+                              cuhre_result* out);
+
 /* Rule evaluation of n regions given on the host (SoA: lower[m*n + r],
  * width[m*n + r]).  Outputs per region: est, err, axis, abssum (volume *
  * sum |w f| of the basic rule); i5 (degree-5 estimate; 0 under key 9) and

@@ -25,6 +25,7 @@
 #include "common.hpp"
 #include "context.hpp"
 #include "eval_kernels.hpp"
+#include "phase2.hpp"
 #include "reduce.hpp"
 
 namespace cuhre {
@@ -37,6 +38,32 @@ EvalLauncher launcher_continuous(int, int);
 EvalLauncher launcher_polynomial(int, int);
 EvalLauncher launcher_cosine_sum(int, int);
 EvalLauncher launcher_table6(int, int);
+
+Phase2Launcher p2launcher_gaussian(int, int);
+Phase2Launcher p2launcher_product_peak(int, int);
+Phase2Launcher p2launcher_oscillatory(int, int);
+Phase2Launcher p2launcher_corner_peak(int, int);
+Phase2Launcher p2launcher_continuous(int, int);
+Phase2Launcher p2launcher_polynomial(int, int);
+Phase2Launcher p2launcher_cosine_sum(int, int);
+Phase2Launcher p2launcher_table6(int, int);
+
+Phase2Launcher find_phase2_launcher(int family, int dim, int key)
+{
+  if (key == 0) key = 7;
+  if (key != 7 && key != 9) return nullptr;
+  switch (family) {
+  case CUHRE_GAUSSIAN: return p2launcher_gaussian(dim, key);
+  case CUHRE_PRODUCT_PEAK: return p2launcher_product_peak(dim, key);
+  case CUHRE_OSCILLATORY: return p2launcher_oscillatory(dim, key);
+  case CUHRE_CORNER_PEAK: return p2launcher_corner_peak(dim, key);
+  case CUHRE_CONTINUOUS: return p2launcher_continuous(dim, key);
+  case CUHRE_POLYNOMIAL: return p2launcher_polynomial(dim, key);
+  case CUHRE_COSINE_SUM: return p2launcher_cosine_sum(dim, key);
+  case CUHRE_TABLE6: return p2launcher_table6(dim, key);
+  default: return nullptr;
+  }
+}
 
 EvalLauncher find_eval_launcher(int family, int dim, int key)
 {
@@ -744,8 +771,29 @@ int wait_published(cuhre_context* c, uint32_t seq, IterState* out)
 
 }  // namespace
 
-int cuhre_integrate(cuhre_context* c, const cuhre_integrand* f, int dim, double epsrel, double epsabs,
-                    int64_t maxeval, int key, const double* lows, const double* highs, cuhre_result* out)
+}  // extern "C"
+
+namespace {
+
+// Options of the breadth-first loop: stop after max_iter iterations; with
+// handoff_n > 0, stop once a split leaves handoff_n regions or more (or the
+// iterations run out with regions left) and hand that list over.
+struct LoopOpts {
+  int64_t handoff_n = 0;
+  int max_iter = 1000;
+};
+
+struct LoopHandoff {
+  int handed = 0;
+  int cur = 0;
+  int64_t n = 0;
+  double fin_s = 0, fin_c = 0, fer_s = 0, fer_c = 0, fin_abs = 0, fin_aest = 0;
+  int64_t nfin = 0;
+};
+
+int plain_loop(cuhre_context* c, const cuhre_integrand* f, int dim, double epsrel, double epsabs, int64_t maxeval,
+               int key, const double* lows, const double* highs, cuhre_result* out, const LoopOpts& opt,
+               LoopHandoff* ho)
 {
   const auto t0 = std::chrono::steady_clock::now();
   if (!out) return fail(CUHRE_ERR_ARG, "null result");
@@ -780,7 +828,7 @@ int cuhre_integrate(cuhre_context* c, const cuhre_integrand* f, int dim, double 
   int64_t nfin = 0, n = n0;
   const double u = 0.5 * 2.220446049250313e-16;
   int result = CUHRE_NOT_CONVERGED;
-  for (int it = 0; it < 1000; ++it) {
+  for (int it = 0; it < opt.max_iter; ++it) {
     const int S = pick_segments(c, n, key);
     const EvalLauncher launch = find_eval_launcher(f->family, dim, key);
     EvalArgs A;
@@ -885,9 +933,141 @@ int cuhre_integrate(cuhre_context* c, const cuhre_integrand* f, int dim, double 
     }
     n = 2 * st.tot.nactive;
     cur = 1 - cur;
+    if (opt.handoff_n > 0 && (n >= opt.handoff_n || it + 1 == opt.max_iter)) {
+      ho->handed = 1;
+      ho->cur = cur;
+      ho->n = n;
+      ho->fin_s = fin_s;
+      ho->fin_c = fin_c;
+      ho->fer_s = fer_s;
+      ho->fer_c = fer_c;
+      ho->fin_abs = fin_abs;
+      ho->fin_aest = fin_aest;
+      ho->nfin = nfin;
+      break;
+    }
   }
   HIP_TRY(hipStreamSynchronize(c->stream));
   out->fail = result > 0 ? result : 0;
+  out->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return result;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cuhre_integrate(cuhre_context* c, const cuhre_integrand* f, int dim, double epsrel, double epsabs,
+                    int64_t maxeval, int key, const double* lows, const double* highs, cuhre_result* out)
+{
+  LoopOpts opt;
+  return plain_loop(c, f, dim, epsrel, epsabs, maxeval, key, lows, highs, out, opt, nullptr);
+}
+
+int cuhre_integrate_two_phase(cuhre_context* c, const cuhre_integrand* f, int dim, double epsrel, double epsabs,
+                              int key, int64_t phase1_regions, int32_t pool_max, const double* lows,
+                              const double* highs, cuhre_result* out)
+{
+  const auto t0 = std::chrono::steady_clock::now();
+  if (!out) return fail(CUHRE_ERR_ARG, "null result");
+  if (phase1_regions <= 0) phase1_regions = 1 << 14;  // the reference's FIRST_PHASE_MAXREGIONS
+  if (pool_max <= 0) pool_max = 2048;                 // the reference's MAX_GLOBALPOOL_SIZE
+  if (pool_max > (1 << 20)) return fail(CUHRE_ERR_ARG, "pool_max above 2^20");
+  LoopOpts opt;
+  opt.handoff_n = phase1_regions;
+  opt.max_iter = 100;  // GPUKernelquad.cu:631
+  LoopHandoff ho;
+  const int64_t no_budget = INT64_MAX / 4;
+  int rc = plain_loop(c, f, dim, epsrel, epsabs, no_budget, key, lows, highs, out, opt, &ho);
+  if (rc < 0 || !ho.handed) {
+    // converged in Phase 1 (or nothing left active): the loop's result
+    out->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return rc;
+  }
+  // Phase 2 on the ho.n regions of list ho.cur (split, not yet evaluated)
+  const Phase2Launcher launch = find_phase2_launcher(f->family, dim, key);
+  if (!launch) return fail(CUHRE_ERR_UNSUPPORTED, "no Phase-2 kernel for family %d, dim %d", f->family, dim);
+  DevIntegrand P;
+  if ((rc = make_dev_integrand(c, f, dim, &P))) return rc;
+  const RuleConst R = make_rule(dim, key);
+  const int64_t p = R.npts;
+  const int64_t n = ho.n, L = (n + 63) / 64 * 64;
+  const int64_t slots = (int64_t)pool_max + 2;
+  const size_t per_slot = sizeof(double) * (2 * (size_t)dim + 2) + sizeof(int8_t) + 2 * sizeof(int32_t);
+  const size_t ws = (size_t)L * (size_t)slots * per_slot;
+  size_t free_b = 0, total_b = 0;
+  HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+  if (ws > free_b / 4 * 3)
+    return fail(CUHRE_ERR_CAPACITY, "Phase 2 needs %.1f GB for %lld regions x %d slots", ws / 1e9, (long long)n,
+                pool_max);
+  char* base = nullptr;
+  const size_t outb = (size_t)L * (2 * sizeof(double) + sizeof(int32_t) + 2);
+  if (hipMalloc(&base, ws + outb + 1024) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(CUHRE_ERR_ALLOC, "cannot allocate the Phase-2 workspace (%.1f GB)", ws / 1e9);
+  }
+  Phase2Args A;
+  A.lower = c->lo[ho.cur];
+  A.width = c->wd[ho.cur];
+  A.ld = c->cap;
+  A.n = n;
+  A.L = L;
+  char* q = base;
+  auto take = [&](size_t bytes) {
+    char* r = q;
+    q += (bytes + 255) & ~(size_t)255;
+    return r;
+  };
+  A.slo = (double*)take(sizeof(double) * (size_t)L * slots * dim);
+  A.swd = (double*)take(sizeof(double) * (size_t)L * slots * dim);
+  A.sest = (double*)take(sizeof(double) * (size_t)L * slots);
+  A.serr = (double*)take(sizeof(double) * (size_t)L * slots);
+  A.sseq = (int32_t*)take(sizeof(int32_t) * (size_t)L * slots);
+  A.heap = (int32_t*)take(sizeof(int32_t) * (size_t)L * slots);
+  A.sax = (int8_t*)take((size_t)L * slots);
+  A.res = (double*)take(sizeof(double) * L);
+  A.err = (double*)take(sizeof(double) * L);
+  A.cnt = (int32_t*)take(sizeof(int32_t) * L);
+  A.act = (int8_t*)take(L);
+  A.nonfinite = (int8_t*)take(L);
+  A.epsrel = epsrel;
+  A.epsabs = epsabs;
+  A.pool_max = pool_max;
+  int result = 0;
+  std::vector<double> hres(n), herr(n);
+  std::vector<int32_t> hcnt(n);
+  std::vector<int8_t> hact(n), hbad(n);
+  launch(A, P, R, c->stream);
+  if (hipGetLastError() != hipSuccess ||
+      hipMemcpyAsync(hres.data(), A.res, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      hipMemcpyAsync(herr.data(), A.err, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      hipMemcpyAsync(hcnt.data(), A.cnt, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      hipMemcpyAsync(hact.data(), A.act, n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      hipMemcpyAsync(hbad.data(), A.nonfinite, n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    result = fail(CUHRE_ERR_DEVICE, "Phase 2 failed: %s", hipGetErrorString(hipGetLastError()));
+  (void)hipFree(base);
+  if (result) return result;
+  double I_s = ho.fin_s, I_c = ho.fin_c, E_s = ho.fer_s, E_c = ho.fer_c;
+  int64_t failed = 0, bad = 0;
+  for (int64_t r = 0; r < n; ++r) {
+    dd_add_host(I_s, I_c, hres[r], 0.0);
+    dd_add_host(E_s, E_c, herr[r], 0.0);
+    out->nregions += hcnt[r];
+    out->neval += (int64_t)(hcnt[r] - 1) * 2 * p + p;
+    failed += hact[r];
+    bad += hbad[r];
+  }
+  if (bad) return fail(CUHRE_ERR_NONFINITE, "integrand returned a non-finite value in Phase 2 (%lld regions)",
+                       (long long)bad);
+  out->integral = I_s + I_c;
+  out->error = E_s + E_c;
+  out->nfinished = ho.nfin + n;
+  const bool ok = out->error <= std::max(epsrel * std::fabs(out->integral), epsabs);
+  result = ok ? CUHRE_OK : CUHRE_NOT_CONVERGED;
+  // the reference's flag: Phase-2 regions that missed their test, 0 when
+  // the global test holds (GPUquad.h:713-714)
+  out->fail = ok ? 0 : (int32_t)std::min<int64_t>(std::max<int64_t>(failed, 1), INT32_MAX);
   out->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return result;
 }

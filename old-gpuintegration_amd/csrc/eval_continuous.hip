@@ -1,7 +1,9 @@
 // Instantiates the rule-evaluation kernels of the continuous family for d = 2..10, for the
-// degree-7/5 rule and the degree-9 rule.
+// degree-7/5 rule and the degree-9 rule, and its Phase-2 kernels.
 #include "eval_kernels.hpp"
+#include "phase2.hpp"
 
 namespace cuhre {
 EvalLauncher launcher_continuous(int dim, int key) { return family_launcher<CUHRE_CONTINUOUS>(dim, key); }
+Phase2Launcher p2launcher_continuous(int dim, int key) { return phase2_launcher<CUHRE_CONTINUOUS>(dim, key); }
 }  // namespace cuhre

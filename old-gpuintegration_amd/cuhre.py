@@ -31,7 +31,7 @@ ERR_CAPACITY = -6
 # Symbols every build exports (kept in step with include/cuhre.h).
 EXPORTS = [
     "cuhre_context_create", "cuhre_context_destroy", "cuhre_context_set",
-    "cuhre_integrate", "cuhre_eval_regions", "cuhre_split_regions", "cuhre_sum",
+    "cuhre_integrate", "cuhre_integrate_two_phase", "cuhre_eval_regions", "cuhre_split_regions", "cuhre_sum",
     "cuhre_last_error", "cuhre_points_per_region", "cuhre_integrate_batch", "cuhre_batch_autotune",
     "cuhre_guide_create", "cuhre_guide_destroy", "cuhre_guide_set_params", "cuhre_guide_get_params",
     "cuhre_guide_forward", "cuhre_guide_gradient", "cuhre_guide_collect", "cuhre_guide_samples",
@@ -180,6 +180,8 @@ def lib():
         L.cuhre_context_set.argtypes = [ctxp, C.c_int, C.c_int]
         L.cuhre_integrate.argtypes = [ctxp, C.POINTER(Integrand), C.c_int, C.c_double, C.c_double,
                                       C.c_int64, C.c_int, dp, dp, C.POINTER(Result)]
+        L.cuhre_integrate_two_phase.argtypes = [ctxp, C.POINTER(Integrand), C.c_int, C.c_double, C.c_double,
+                                                C.c_int, C.c_int64, C.c_int32, dp, dp, C.POINTER(Result)]
         L.cuhre_eval_regions.argtypes = [ctxp, C.POINTER(Integrand), C.c_int, C.c_int, C.c_int64,
                                          dp, dp, dp, dp, C.POINTER(C.c_int32), dp, dp, dp]
         L.cuhre_split_regions.argtypes = [ctxp, C.c_int, C.c_int64, dp, dp, dp, dp,
@@ -291,6 +293,21 @@ class Context:
                                    p.epsrel if epsrel is None else epsrel, p.epsabs,
                                    p.maxeval if maxeval is None else maxeval, key,
                                    _dp(lo), _dp(hi), C.byref(res))
+        _check(rc)
+        d = res.asdict()
+        d["rc"] = rc
+        return d
+
+    def integrate_two_phase(self, p: W.Problem, key: int = 9, phase1_regions: int = 0, pool_max: int = 0):
+        """The reference's two-phase method: breadth-first Phase 1 (at most 100
+        iterations, until phase1_regions regions; 0: 2^14), then per-region
+        Phase 2 (at most pool_max bisections each; 0: 2048)."""
+        s, keep = to_struct(p.f)
+        lo = np.ascontiguousarray(p.lows, dtype=np.float64)
+        hi = np.ascontiguousarray(p.highs, dtype=np.float64)
+        res = Result()
+        rc = lib().cuhre_integrate_two_phase(self.h, C.byref(s), p.f.dim, p.epsrel, p.epsabs, key,
+                                             phase1_regions, pool_max, _dp(lo), _dp(hi), C.byref(res))
         _check(rc)
         d = res.asdict()
         d["rc"] = rc

@@ -593,6 +593,240 @@ done:
 }
 
 /* ------------------------------------------------------------------------ */
+/* Two-phase method of the reference (Phase 1 + per-region Phase 2).         */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+  int key, d;
+  or_rule r7;
+  or_rule9 r9;
+} anyrule;
+
+static void any_eval(const anyrule* R, const or_integrand* f, const double* l, const double* w,
+                     double* est, double* err, int* ax, double* ab)
+{
+  if (R->key == 9)
+    or_region_eval9(&R->r9, f, l, w, est, err, ax, ab, NULL, NULL);
+  else
+    or_region_eval(&R->r7, f, l, w, est, err, ax, ab, NULL, NULL);
+}
+
+/* heap of slots ordered by (err desc, seq asc) */
+static int p2_before(const double* err, const int64_t* seq, int a, int b)
+{
+  return err[a] > err[b] || (err[a] == err[b] && seq[a] < seq[b]);
+}
+
+static void p2_push(int* heap, int* hn, int slot, const double* err, const int64_t* seq)
+{
+  int i = (*hn)++;
+  heap[i] = slot;
+  while (i > 0) {
+    int pa = (i - 1) / 2;
+    if (!p2_before(err, seq, heap[i], heap[pa])) break;
+    int t = heap[i]; heap[i] = heap[pa]; heap[pa] = t;
+    i = pa;
+  }
+}
+
+static int p2_pop(int* heap, int* hn, const double* err, const int64_t* seq)
+{
+  int top = heap[0];
+  heap[0] = heap[--(*hn)];
+  int i = 0;
+  for (;;) {
+    int l = 2 * i + 1, r = l + 1, b = i;
+    if (l < *hn && p2_before(err, seq, heap[l], heap[b])) b = l;
+    if (r < *hn && p2_before(err, seq, heap[r], heap[b])) b = r;
+    if (b == i) break;
+    int t = heap[i]; heap[i] = heap[b]; heap[b] = t;
+    i = b;
+  }
+  return top;
+}
+
+/* Phase 2 of one region (GPUQuadPhases.cu:422-536). */
+static void phase2_region(const anyrule* R, const or_integrand* f, const double* lo0, const double* w0,
+                          double epsrel, double epsabs, int pool_max, double* res_out, double* err_out,
+                          int* nreg_out, int* active_out)
+{
+  const int d = R->d, cap = pool_max + 2;
+  double* lo = (double*)malloc(sizeof(double) * (size_t)cap * d);
+  double* wd = (double*)malloc(sizeof(double) * (size_t)cap * d);
+  double* est = (double*)malloc(sizeof(double) * cap);
+  double* er = (double*)malloc(sizeof(double) * cap);
+  int* ax = (int*)malloc(sizeof(int) * cap);
+  int64_t* seq = (int64_t*)malloc(sizeof(int64_t) * cap);
+  int* heap = (int*)malloc(sizeof(int) * cap);
+  int hn = 0;
+  int64_t next = 0;
+  double ab;
+  for (int m = 0; m < d; ++m) { lo[m] = lo0[m]; wd[m] = w0[m]; }
+  any_eval(R, f, lo, wd, &est[0], &er[0], &ax[0], &ab);
+  seq[0] = next++;
+  p2_push(heap, &hn, 0, er, seq);
+  double RES = est[0], ERR = er[0];
+  int count = 1;
+  while (count <= pool_max && (count == 1 || ERR > fmax(epsrel * fabs(RES), epsabs))) {
+    const int a = p2_pop(heap, &hn, er, seq);
+    const int b = count;
+    const double pe = est[a], pr = er[a];
+    const int k = ax[a];
+    for (int m = 0; m < d; ++m) { lo[(size_t)b * d + m] = lo[(size_t)a * d + m]; wd[(size_t)b * d + m] = wd[(size_t)a * d + m]; }
+    const double hw = 0.5 * wd[(size_t)a * d + k];
+    wd[(size_t)a * d + k] = hw;
+    wd[(size_t)b * d + k] = hw;
+    lo[(size_t)b * d + k] = lo[(size_t)a * d + k] + hw;
+    any_eval(R, f, &lo[(size_t)a * d], &wd[(size_t)a * d], &est[a], &er[a], &ax[a], &ab);
+    any_eval(R, f, &lo[(size_t)b * d], &wd[(size_t)b * d], &est[b], &er[b], &ax[b], &ab);
+    /* Cuba's refinement from the parent difference (GPUQuadPhases.cu:498-515) */
+    const double diff = fabs(0.25 * (est[a] + est[b] - pe));
+    const double e = er[a] + er[b];
+    if (e > 0) {
+      const double c = 1 + 2 * diff / e;
+      er[a] *= c;
+      er[b] *= c;
+    }
+    er[a] += diff;
+    er[b] += diff;
+    ERR += er[a] + er[b] - pr;
+    RES += est[a] + est[b] - pe;
+    seq[a] = next++;
+    seq[b] = next++;
+    p2_push(heap, &hn, a, er, seq);
+    p2_push(heap, &hn, b, er, seq);
+    ++count;
+  }
+  int active = ERR > fmax(epsrel * fabs(RES), epsabs);
+  if (ERR > 1e10) { /* the reference zeroes a diverging region and flags it */
+    RES = 0.0;
+    ERR = 0.0;
+    active = 1;
+  }
+  *res_out = RES;
+  *err_out = ERR;
+  *nreg_out = count;
+  *active_out = active;
+  free(lo); free(wd); free(est); free(er); free(ax); free(seq); free(heap);
+}
+
+int or_integrate_two_phase(const or_integrand* f, const double* lows, const double* highs,
+                           double epsrel, double epsabs, int key, int64_t phase1_max,
+                           int pool_max, int nthreads, or_result* out)
+{
+  memset(out, 0, sizeof(*out));
+  const int d = f->dim;
+  if (key != 0 && key != 7 && key != 9) return -1;
+  if (phase1_max < 1 || pool_max < 1) return -1;
+  anyrule R;
+  R.key = key == 9 ? 9 : 7;
+  R.d = d;
+  if (or_rule_init(&R.r7, d) != 0) return -1;
+  if (R.key == 9 && or_rule9_init(&R.r9, d) != 0) { or_rule_free(&R.r7); return -1; }
+  const int p = R.key == 9 ? R.r9.npts : R.r7.npts;
+  int64_t n = or_initial_grid(d, lows, highs, NULL, NULL);
+  int64_t cap = 2 * (phase1_max > n ? phase1_max : n) + 2;
+  double* lo = (double*)malloc(sizeof(double) * (size_t)cap * d);
+  double* wd = (double*)malloc(sizeof(double) * (size_t)cap * d);
+  double* lo2 = (double*)malloc(sizeof(double) * (size_t)cap * d);
+  double* wd2 = (double*)malloc(sizeof(double) * (size_t)cap * d);
+  double* est = (double*)malloc(sizeof(double) * (size_t)cap);
+  double* err = (double*)malloc(sizeof(double) * (size_t)cap);
+  double* ab = (double*)malloc(sizeof(double) * (size_t)cap);
+  int32_t* ax = (int32_t*)malloc(sizeof(int32_t) * (size_t)cap);
+  int32_t* act = (int32_t*)malloc(sizeof(int32_t) * (size_t)cap);
+  int rc = 0;
+  if (!lo || !wd || !lo2 || !wd2 || !est || !err || !ab || !ax || !act) { rc = -3; goto done2; }
+  or_initial_grid(d, lows, highs, lo, wd);
+  {
+    ksum ifin = {0, 0}, efin = {0, 0};
+    int phase2 = 0;
+    for (int it = 0; it < 100; ++it) {
+      int nonfinite = 0;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nthreads > 0 ? nthreads : 1) schedule(static) reduction(| : nonfinite)
+#endif
+      for (int64_t r = 0; r < n; ++r) {
+        double l[OR_MAXDIM], w[OR_MAXDIM];
+        for (int m = 0; m < d; ++m) { l[m] = lo[(size_t)m * n + r]; w[m] = wd[(size_t)m * n + r]; }
+        int a;
+        any_eval(&R, f, l, w, &est[r], &err[r], &a, &ab[r]);
+        ax[r] = a;
+        if (!isfinite(est[r]) || !isfinite(err[r])) nonfinite = 1;
+        act[r] = err[r] > fmax(epsrel * fabs(est[r]), epsabs) ? 1 : 0;
+      }
+      out->iterations = it + 1;
+      out->nregions += n;
+      out->neval += n * (int64_t)p;
+      if (n > out->peak_regions) out->peak_regions = n;
+      if (nonfinite) { rc = -2; goto done2; }
+      ksum sall = ifin, eall = efin;
+      int64_t A = 0;
+      for (int64_t r = 0; r < n; ++r) {
+        kadd(&sall, est[r]);
+        kadd(&eall, err[r]);
+        if (!act[r]) { kadd(&ifin, est[r]); kadd(&efin, err[r]); out->nfinished++; }
+        else ++A;
+      }
+      const double rG = kval(&sall), eG = kval(&eall);
+      if (eG <= fmax(epsrel * fabs(rG), epsabs)) { /* converged in Phase 1 */
+        out->integral = rG;
+        out->error = eG;
+        out->fail = 0;
+        rc = 0;
+        goto done2;
+      }
+      out->integral = kval(&ifin);
+      out->error = kval(&efin);
+      if (A == 0) { phase2 = 0; break; } /* nothing left for Phase 2 */
+      or_split(d, n, lo, wd, act, ax, lo2, wd2);
+      double* t;
+      t = lo; lo = lo2; lo2 = t;
+      t = wd; wd = wd2; wd2 = t;
+      n = 2 * A;
+      phase2 = 1;
+      if (n >= phase1_max) break;
+    }
+    if (phase2) {
+      /* Phase 2 on the n (not yet evaluated) regions of the split list */
+      double* pres = (double*)malloc(sizeof(double) * (size_t)n);
+      double* perr = (double*)malloc(sizeof(double) * (size_t)n);
+      int* pcnt = (int*)malloc(sizeof(int) * (size_t)n);
+      int* pact = (int*)malloc(sizeof(int) * (size_t)n);
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nthreads > 0 ? nthreads : 1) schedule(dynamic, 4)
+#endif
+      for (int64_t r = 0; r < n; ++r) {
+        double l[OR_MAXDIM], w[OR_MAXDIM];
+        for (int m = 0; m < d; ++m) { l[m] = lo[(size_t)m * n + r]; w[m] = wd[(size_t)m * n + r]; }
+        phase2_region(&R, f, l, w, epsrel, epsabs, pool_max, &pres[r], &perr[r], &pcnt[r], &pact[r]);
+      }
+      ksum I = ifin, E = efin;
+      int64_t failed = 0;
+      for (int64_t r = 0; r < n; ++r) {
+        kadd(&I, pres[r]);
+        kadd(&E, perr[r]);
+        out->nregions += pcnt[r];
+        out->neval += (int64_t)(pcnt[r] - 1) * 2 * p + p;
+        failed += pact[r];
+      }
+      out->integral = kval(&I);
+      out->error = kval(&E);
+      free(pres); free(perr); free(pcnt); free(pact);
+      rc = out->error <= fmax(epsrel * fabs(out->integral), epsabs) ? 0 : (int)(failed > 0 ? 1 : 0);
+      out->fail = (int32_t)(rc == 0 ? 0 : (failed > 0x7fffffff ? 0x7fffffff : failed));
+    } else {
+      rc = out->error <= fmax(epsrel * fabs(out->integral), epsabs) ? 0 : 1;
+      out->fail = rc;
+    }
+  }
+done2:
+  free(lo); free(wd); free(lo2); free(wd2); free(est); free(err); free(ab); free(ax); free(act);
+  or_rule_free(&R.r7);
+  return rc;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Global adaptive (Cuhre-style) CPU baseline: max-heap on err.              */
 /* ------------------------------------------------------------------------ */
 

@@ -126,6 +126,20 @@ int or_integrate_bfs_key(const or_integrand* f, const double* lows,
                          int64_t maxeval, int64_t capacity, int nthreads,
                          int key, or_result* out);
 
+/* The reference's two-phase method (GPUquad.h:611-727): Phase 1 is the
+ * breadth-first loop (at most 100 iterations) until the global test holds,
+ * no region is active, or the split list reaches phase1_max regions
+ * (GPUKernelquad.cu:507-668); Phase 2 then refines every region of that
+ * list on its own (GPUQuadPhases.cu:422-536): a max-error heap, up to
+ * pool_max bisections, until the region's own sum meets
+ * max(epsrel |RESULT|, epsabs), children's errors refined from the parent
+ * difference as Cuba does (GPUQuadPhases.cu:498-515).  Ties in the heap go
+ * to the region created first.  fail = number of Phase-2 regions that did
+ * not meet their test, 0 when the global test holds (GPUquad.h:713-714). */
+int or_integrate_two_phase(const or_integrand* f, const double* lows, const double* highs,
+                           double epsrel, double epsabs, int key, int64_t phase1_max,
+                           int pool_max, int nthreads, or_result* out);
+
 /* Cuhre-style global adaptive (heap of regions, always bisect the region
  * with the largest error), same GM7/5 rule: the CPU baseline flavour (i). */
 int or_integrate_heap(const or_integrand* f, const double* lows,

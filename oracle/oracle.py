@@ -106,6 +106,8 @@ def lib():
                                       dp, dp, C.POINTER(C.c_int), dp, dp, dp]
         L.or_integrate_bfs_key.argtypes = [C.POINTER(OrIntegrand), dp, dp, C.c_double, C.c_double,
                                            C.c_int64, C.c_int64, C.c_int, C.c_int, C.POINTER(OrResult)]
+        L.or_integrate_two_phase.argtypes = [C.POINTER(OrIntegrand), dp, dp, C.c_double, C.c_double, C.c_int,
+                                             C.c_int64, C.c_int, C.c_int, C.POINTER(OrResult)]
         L.or_integrate_heap.argtypes = [C.POINTER(OrIntegrand), dp, dp, C.c_double, C.c_double,
                                         C.c_int64, C.POINTER(OrResult)]
         L.or_exact.argtypes = [C.POINTER(OrIntegrand), dp, dp, dp, dp]
@@ -228,6 +230,19 @@ def integrate_bfs(p: W.Problem, nthreads=1, capacity=None, maxeval=None, epsrel=
     cap = capacity if capacity is not None else max(1, me // npoints(p.f.dim, key)) + 64
     rc = lib().or_integrate_bfs_key(C.byref(s), _dp(lo), _dp(hi), p.epsrel if epsrel is None else epsrel,
                                     p.epsabs, me, cap, nthreads, key, C.byref(res))
+    d = res.asdict()
+    d["rc"] = rc
+    return d
+
+
+def integrate_two_phase(p: W.Problem, key=9, phase1_max=16384, pool_max=2048, nthreads=8):
+    """The reference's two-phase method (breadth-first Phase 1, per-region Phase 2)."""
+    s, keep = to_struct(p.f)
+    lo = np.array(p.lows, dtype=np.float64)
+    hi = np.array(p.highs, dtype=np.float64)
+    res = OrResult()
+    rc = lib().or_integrate_two_phase(C.byref(s), _dp(lo), _dp(hi), p.epsrel, p.epsabs, key, phase1_max,
+                                      pool_max, nthreads, C.byref(res))
     d = res.asdict()
     d["rc"] = rc
     return d

@@ -351,3 +351,39 @@ def test_rule9_loop_gaussian():
     assert abs(r9["integral"] - ex) <= r9["error"]
     assert r9["neval"] == r9["nregions"] * 77
     assert r7["rc"] == 0 and r7["neval"] == r7["nregions"] * 33
+
+
+# ------------------------------------------------------- two-phase method
+# The reference's Phase 1 + per-region Phase 2 (GPUquad.h:611-727,
+# GPUQuadPhases.cu:422-536), restated in the oracle.
+
+def test_two_phase_equals_breadth_first_when_phase1_converges():
+    p = W.c1_gaussian3()
+    bfs = O.integrate_bfs(p, key=9)
+    tp = O.integrate_two_phase(p, key=9, phase1_max=1 << 20)
+    assert bfs["rc"] == tp["rc"] == 0
+    assert tp["nregions"] == bfs["nregions"] and tp["iterations"] == bfs["iterations"]
+    assert tp["integral"] == bfs["integral"]
+
+
+def test_two_phase_region_accounting():
+    # phase1_max = 1: the initial 64 regions are evaluated, their active
+    # ones split, and each child gets exactly one forced bisection when
+    # pool_max = 1: it counts 2 regions (its pool: the two halves, the
+    # reference's nregions) and 1 + 2 evaluations (GPUKernelquad.cu:883-888)
+    p = W.c1_gaussian3()
+    tp = O.integrate_two_phase(p, key=7, phase1_max=1, pool_max=1)
+    assert tp["iterations"] == 1
+    n2 = tp["nregions"] - 64
+    assert n2 % 2 == 0 and n2 > 0
+    assert tp["neval"] == 64 * 33 + (n2 // 2) * 3 * 33
+
+
+@pytest.mark.parametrize("key", [7, 9])
+def test_two_phase_phase2_converges(key):
+    p = W.c1_gaussian3()
+    ex, _ = O.exact(p.f, p.lows, p.highs)
+    tp = O.integrate_two_phase(p, key=key, phase1_max=256)
+    assert tp["rc"] == 0 and tp["fail"] == 0
+    assert tp["error"] <= p.epsrel * abs(tp["integral"])
+    assert abs(tp["integral"] - ex) <= tp["error"]
