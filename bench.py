@@ -172,6 +172,23 @@ def bench_learned(ctx, samples_per_run=500_000, maxeval=10**11, rounds=3):
             "maxeval": maxeval}
 
 
+def bench_reference(ctx):
+    """The reference's own method on MI355X: degree-9 rule (KEY 9), Phase 1
+    breadth-first to 2^14 regions, Phase 2 per region (2048 bisections max),
+    on the 8-D workload and the 7-D corner peak, against the default plain
+    GM7/5 method on the same integrals."""
+    rows = []
+    for p, me in ((W.c2_oscillatory8(), 10**9), (W.c2_product_peak8(), 10**12), (W.c5_corner7(), 10**11)):
+        ref = min((ctx.integrate_two_phase(p, key=9) for _ in range(3)), key=lambda r: r["ms_total"])
+        plain = min((ctx.integrate(p, maxeval=me) for _ in range(3)), key=lambda r: r["ms_total"])
+        rows.append({"integrand": p.name,
+                     "reference_method": {k: ref[k] for k in ("rc", "fail", "integral", "error", "nregions", "neval",
+                                                              "iterations")} | {"ms": ref["ms_total"]},
+                     "plain_gm75": {k: plain[k] for k in ("rc", "integral", "error", "nregions", "neval")}
+                     | {"ms": plain["ms_total"], "maxeval": me}})
+    return rows
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -182,6 +199,8 @@ def main():
                     help="also time the 4096-integral batched form against sequential plain calls")
     ap.add_argument("--learned", action="store_true",
                     help="also train the split guide and compare it with the plain method on 7-D corner peaks")
+    ap.add_argument("--reference", action="store_true",
+                    help="also run the reference's own method (degree-9 rule, two phases) beside the plain one")
     ap.add_argument("--converge", action="store_true",
                     help="also run the product peak with maxeval 1e12 to reach epsrel 1e-6")
     args = ap.parse_args()
@@ -273,12 +292,21 @@ def main():
         out["batched"] = bench_batched(ctx)
     if args.learned:
         out["learned"] = bench_learned(ctx)
+    if args.reference:
+        out["reference_method"] = bench_reference(ctx)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(problems)
         # the CPU leg also holds the closed forms: report each GPU result's
         # distance from the exact integral there
         out["cpu_baseline"]["gpu_vs_exact"] = check_against_exact(problems, per,
                                                                   out.get("product_peak_to_convergence"))
+        if args.reference:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import oracle
+            for row, p in zip(out["reference_method"], (W.c2_oscillatory8(), W.c2_product_peak8(), W.c5_corner7())):
+                ex, _ = oracle.exact(p.f, p.lows, p.highs)
+                for k in ("reference_method", "plain_gm75"):
+                    row[k]["true_rel_err"] = abs(row[k]["integral"] - ex) / abs(ex)
     ctx.close()
     print(json.dumps(out))
 
