@@ -68,6 +68,8 @@ struct cuhre_context {
   uint32_t seq = 0;
   double* aux = nullptr;  // i5 (cap) + dvec (cap*dim) on request
   int64_t aux_cap = 0;
+  char* p2_ws = nullptr;  // Phase-2 workspace of the two-phase method, kept between calls
+  size_t p2_bytes = 0;
   double2* sum_part = nullptr;
   double* sum_out = nullptr;
 

@@ -721,6 +721,7 @@ int cuhre_context_destroy(cuhre_context* c)
   batch_engine_delete(c->batch);
   c->batch = nullptr;
   if (c->aux) (void)hipFree(c->aux);
+  if (c->p2_ws) (void)hipFree(c->p2_ws);
   if (c->ticket) (void)hipFree(c->ticket);
   if (c->state) (void)hipFree(c->state);
   if (c->h_state) (void)hipHostFree(c->h_state);
@@ -972,7 +973,17 @@ int cuhre_integrate_two_phase(cuhre_context* c, const cuhre_integrand* f, int di
   if (!out) return fail(CUHRE_ERR_ARG, "null result");
   if (phase1_regions <= 0) phase1_regions = 1 << 14;  // the reference's FIRST_PHASE_MAXREGIONS
   if (pool_max <= 0) pool_max = 2048;                 // the reference's MAX_GLOBALPOOL_SIZE
-  if (pool_max > (1 << 20)) return fail(CUHRE_ERR_ARG, "pool_max above 2^20");
+  if (!c) return fail(CUHRE_ERR_ARG, "null context");
+  {
+    // the pool's keys live in LDS: pool_max + 2 slots of 12 bytes per region
+    int lds_max = 0;
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
+    const size_t lds = phase2_lds_bytes((int64_t)pool_max + 2);
+    if (pool_max > (1 << 24) || lds > (size_t)lds_max)
+      return fail(CUHRE_ERR_ARG, "pool_max %d needs %zu bytes of LDS per region (at most %d)", pool_max, lds,
+                  lds_max);
+  }
   LoopOpts opt;
   opt.handoff_n = phase1_regions;
   opt.max_iter = 100;  // GPUKernelquad.cu:631
@@ -991,45 +1002,50 @@ int cuhre_integrate_two_phase(cuhre_context* c, const cuhre_integrand* f, int di
   if ((rc = make_dev_integrand(c, f, dim, &P))) return rc;
   const RuleConst R = make_rule(dim, key);
   const int64_t p = R.npts;
-  const int64_t n = ho.n, L = (n + 63) / 64 * 64;
+  const int64_t n = ho.n;
   const int64_t slots = (int64_t)pool_max + 2;
-  const size_t per_slot = sizeof(double) * (2 * (size_t)dim + 2) + sizeof(int8_t) + 2 * sizeof(int32_t);
-  const size_t ws = (size_t)L * (size_t)slots * per_slot;
-  size_t free_b = 0, total_b = 0;
-  HIP_TRY(hipMemGetInfo(&free_b, &total_b));
-  if (ws > free_b / 4 * 3)
-    return fail(CUHRE_ERR_CAPACITY, "Phase 2 needs %.1f GB for %lld regions x %d slots", ws / 1e9, (long long)n,
-                pool_max);
-  char* base = nullptr;
-  const size_t outb = (size_t)L * (2 * sizeof(double) + sizeof(int32_t) + 2);
-  if (hipMalloc(&base, ws + outb + 1024) != hipSuccess) {
-    (void)hipGetLastError();
-    return fail(CUHRE_ERR_ALLOC, "cannot allocate the Phase-2 workspace (%.1f GB)", ws / 1e9);
+  const size_t per_slot = sizeof(double) * (2 * (size_t)dim + 1) + sizeof(int8_t);
+  const size_t ws = (size_t)n * (size_t)slots * per_slot;
+  const size_t outb = (size_t)n * (2 * sizeof(double) + sizeof(int32_t) + 2);
+  const size_t need = ws + outb + 8 * 256;
+  if (c->p2_bytes < need) {
+    size_t free_b = 0, total_b = 0;
+    if (c->p2_ws) {
+      (void)hipFree(c->p2_ws);
+      c->p2_ws = nullptr;
+      c->p2_bytes = 0;
+    }
+    HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+    if (need > free_b / 4 * 3)
+      return fail(CUHRE_ERR_CAPACITY, "Phase 2 needs %.1f GB for %lld regions x %d slots", need / 1e9, (long long)n,
+                  pool_max);
+    if (hipMalloc(&c->p2_ws, need) != hipSuccess) {
+      (void)hipGetLastError();
+      c->p2_ws = nullptr;
+      return fail(CUHRE_ERR_ALLOC, "cannot allocate the Phase-2 workspace (%.1f GB)", need / 1e9);
+    }
+    c->p2_bytes = need;
   }
   Phase2Args A;
   A.lower = c->lo[ho.cur];
   A.width = c->wd[ho.cur];
   A.ld = c->cap;
   A.n = n;
-  A.L = L;
-  char* q = base;
+  A.slots = slots;
+  char* q = c->p2_ws;
   auto take = [&](size_t bytes) {
     char* r = q;
     q += (bytes + 255) & ~(size_t)255;
     return r;
   };
-  A.slo = (double*)take(sizeof(double) * (size_t)L * slots * dim);
-  A.swd = (double*)take(sizeof(double) * (size_t)L * slots * dim);
-  A.sest = (double*)take(sizeof(double) * (size_t)L * slots);
-  A.serr = (double*)take(sizeof(double) * (size_t)L * slots);
-  A.sseq = (int32_t*)take(sizeof(int32_t) * (size_t)L * slots);
-  A.heap = (int32_t*)take(sizeof(int32_t) * (size_t)L * slots);
-  A.sax = (int8_t*)take((size_t)L * slots);
-  A.res = (double*)take(sizeof(double) * L);
-  A.err = (double*)take(sizeof(double) * L);
-  A.cnt = (int32_t*)take(sizeof(int32_t) * L);
-  A.act = (int8_t*)take(L);
-  A.nonfinite = (int8_t*)take(L);
+  A.geo = (double*)take(sizeof(double) * (size_t)n * slots * 2 * dim);
+  A.sest = (double*)take(sizeof(double) * (size_t)n * slots);
+  A.sax = (int8_t*)take((size_t)n * slots);
+  A.res = (double*)take(sizeof(double) * n);
+  A.err = (double*)take(sizeof(double) * n);
+  A.cnt = (int32_t*)take(sizeof(int32_t) * n);
+  A.act = (int8_t*)take(n);
+  A.nonfinite = (int8_t*)take(n);
   A.epsrel = epsrel;
   A.epsabs = epsabs;
   A.pool_max = pool_max;
@@ -1046,7 +1062,6 @@ int cuhre_integrate_two_phase(cuhre_context* c, const cuhre_integrand* f, int di
       hipMemcpyAsync(hbad.data(), A.nonfinite, n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess)
     result = fail(CUHRE_ERR_DEVICE, "Phase 2 failed: %s", hipGetErrorString(hipGetLastError()));
-  (void)hipFree(base);
   if (result) return result;
   double I_s = ho.fin_s, I_c = ho.fin_c, E_s = ho.fer_s, E_c = ho.fer_c;
   int64_t failed = 0, bad = 0;

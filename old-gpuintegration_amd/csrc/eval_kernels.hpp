@@ -61,10 +61,38 @@ struct RegionOut {
   int axis;
 };
 
+// Half-wave merges of the segment results (HALF = true below): segments
+// s = lane mod 32 of one region sit in one half of a wavefront and meet by a
+// xor butterfly, so every lane of the half ends with the same values (each
+// level adds the same two operands in both partner lanes).
+__device__ __forceinline__ double half_sum(double v)
+{
+#pragma unroll
+  for (int o = 16; o > 0; o >>= 1) v += shfl_xor_d(v, o);
+  return v;
+}
+
+// first strict argmax over increasing axis index; a lane whose largest
+// difference is 0 holds axis -1, so partners always agree
+__device__ __forceinline__ void half_argmax(double& d, int& a)
+{
+#pragma unroll
+  for (int o = 16; o > 0; o >>= 1) {
+    const double d2 = shfl_xor_d(d, o);
+    const int a2 = __shfl_xor(a, o, kWave);
+    if (d2 > d || (d2 == d && d2 > 0.0 && a2 < a)) {
+      d = d2;
+      a = a2;
+    }
+  }
+}
+
 // Segmented Genz-Malik evaluation of the region held by this lane (see the
 // file comment).  All waves of the block must call it together when S > 1
 // (it contains a block barrier).  F(x) evaluates the lane's integrand.
-template <int D, class F>
+// HALF: S = 32 segments of one region in the 32 lanes of a half wavefront
+// (s = lane mod 32), merged by half_sum / half_argmax instead of LDS.
+template <int D, bool HALF = false, class F>
 __device__ __forceinline__ RegionOut seg_eval(const F& f, const double* __restrict__ lower,
                                               const double* __restrict__ width, int64_t ld,
                                               int64_t r, bool valid, int64_t rr, int S, int s,
@@ -156,7 +184,12 @@ __device__ __forceinline__ RegionOut seg_eval(const F& f, const double* __restri
     sa = fma(fabs(R.w7v), fabs(v), sa);
   }
 
-  if (S > 1) {
+  if constexpr (HALF) {
+    s7 = half_sum(s7);
+    s5 = half_sum(s5);
+    sa = half_sum(sa);
+    half_argmax(maxd, ax);
+  } else if (S > 1) {
     // segments 1..S-1 hand their partial sums to segment 0 of their group
     double* const sh = seg_lds;  // [W][4][64] doubles, then [W][64] ints
     int32_t* const shax = reinterpret_cast<int32_t*>(seg_lds + (size_t)W * 4 * kWave);
@@ -255,7 +288,7 @@ inline size_t seg9_lds_bytes(int S)
   return (size_t)W * kWave * (7 * sizeof(double) + sizeof(int32_t));
 }
 
-template <int D, class F>
+template <int D, bool HALF = false, class F>
 __device__ __forceinline__ RegionOut seg_eval9(const F& f, const double* __restrict__ lower,
                                                const double* __restrict__ width, int64_t ld, int64_t r,
                                                bool valid, int64_t rr, int S, int s, int wave, int lane, int W,
@@ -398,7 +431,12 @@ __device__ __forceinline__ RegionOut seg_eval9(const F& f, const double* __restr
     group(8, fs, fa);
   }
 
-  if (S > 1) {
+  if constexpr (HALF) {
+#pragma unroll
+    for (int q = 0; q < 5; ++q) sum[q] = half_sum(sum[q]);
+    sa = half_sum(sa);
+    half_argmax(maxd, ax);
+  } else if (S > 1) {
     double* const sh = seg_lds;  // [W][7][64] doubles, then [W][64] ints
     int32_t* const shax = reinterpret_cast<int32_t*>(seg_lds + (size_t)W * 7 * kWave);
     if (s != 0) {

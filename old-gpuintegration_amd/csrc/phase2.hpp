@@ -4,15 +4,19 @@
 //
 // The reference gives each region a 256-thread block with a region pool in
 // shared memory, device-side malloc for the overflow and a serial max-error
-// search.  Here each region is one lane: all 64 lanes of a wavefront walk the
-// same point sequence for their own current pair of children (so the rule
-// evaluation is the same wave-uniform code as the breadth-first kernels),
-// and each lane keeps its regions and a binary max-error heap in a per-lane
-// slice of global memory (slot-major, lane-minor, so that lanes touching the
-// same slot index touch consecutive addresses).  A lane stops when its own
-// sum meets max(epsrel |RESULT|, epsabs) or after pool_max bisections; the
-// wavefront runs while any lane is live.  The loop has no barrier, so lanes
-// beyond the list return at once.
+// search.  Here each region is one wavefront (a 64-thread block).  A
+// bisection evaluates both children at once: lanes 0-31 the left child and
+// lanes 32-63 the right one, each half taking the rule's point sets round
+// robin (32 segments) and merging by a xor butterfly.  The pool's keys (error
+// and creation number per slot) sit in LDS, so the next region to bisect --
+// the largest error, the earliest created among equal errors, as the
+// reference's max-heap order -- is one strided scan plus a butterfly; the
+// pool's geometry, estimates and axes sit in the region's own slice of
+// global memory ([slot][lower D, width D]), written by lanes 0 and 32 and
+// read back by the whole wave after a barrier.  A region stops when its own
+// sum meets max(epsrel |RESULT|, epsabs) or after pool_max bisections.
+// Per-region run lengths differ by orders of magnitude; with one wave per
+// region the hardware dispatcher balances them.
 #pragma once
 
 #include "eval_kernels.hpp"
@@ -24,14 +28,10 @@ struct Phase2Args {
   const double* width;
   int64_t ld;
   int64_t n;
-  int64_t L;  // lanes of the workspace (stride of every per-lane array)
-  double* slo;  // [slot][m][lane]
-  double* swd;
-  double* sest;  // [slot][lane]
-  double* serr;
-  int8_t* sax;
-  int32_t* sseq;
-  int32_t* heap;  // [position][lane] -> slot
+  int64_t slots;  // pool slots per region (pool_max + 2)
+  double* geo;    // [region][slot][lower D, width D]
+  double* sest;   // [region][slot]
+  int8_t* sax;    // [region][slot]
   double epsrel, epsabs;
   int32_t pool_max;
   double* res;  // per region of the input list
@@ -41,102 +41,113 @@ struct Phase2Args {
   int8_t* nonfinite;
 };
 
-// heap order: larger error first, then the region created first
-__device__ __forceinline__ bool p2_before(const Phase2Args& A, int64_t g, int32_t a, int32_t b)
+// LDS bytes of one block: error (double) and creation number (int32) per slot
+inline size_t phase2_lds_bytes(int64_t slots) { return (size_t)slots * (sizeof(double) + sizeof(int32_t)); }
+
+__device__ __forceinline__ double read_lane_d(double v, int l)
 {
-  const double ea = A.serr[(int64_t)a * A.L + g], eb = A.serr[(int64_t)b * A.L + g];
-  return ea > eb || (ea == eb && A.sseq[(int64_t)a * A.L + g] < A.sseq[(int64_t)b * A.L + g]);
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-__device__ __forceinline__ void p2_push(const Phase2Args& A, int64_t g, int32_t& hn, int32_t slot)
-{
-  int32_t i = hn++;
-  A.heap[(int64_t)i * A.L + g] = slot;
-  while (i > 0) {
-    const int32_t pa = (i - 1) / 2;
-    const int32_t si = A.heap[(int64_t)i * A.L + g], sp = A.heap[(int64_t)pa * A.L + g];
-    if (!p2_before(A, g, si, sp)) break;
-    A.heap[(int64_t)i * A.L + g] = sp;
-    A.heap[(int64_t)pa * A.L + g] = si;
-    i = pa;
-  }
-}
-
-__device__ __forceinline__ int32_t p2_pop(const Phase2Args& A, int64_t g, int32_t& hn)
-{
-  const int32_t top = A.heap[g];
-  --hn;
-  A.heap[g] = A.heap[(int64_t)hn * A.L + g];
-  int32_t i = 0;
-  for (;;) {
-    const int32_t l = 2 * i + 1, r = l + 1;
-    int32_t b = i;
-    if (l < hn && p2_before(A, g, A.heap[(int64_t)l * A.L + g], A.heap[(int64_t)b * A.L + g])) b = l;
-    if (r < hn && p2_before(A, g, A.heap[(int64_t)r * A.L + g], A.heap[(int64_t)b * A.L + g])) b = r;
-    if (b == i) break;
-    const int32_t t = A.heap[(int64_t)i * A.L + g];
-    A.heap[(int64_t)i * A.L + g] = A.heap[(int64_t)b * A.L + g];
-    A.heap[(int64_t)b * A.L + g] = t;
-    i = b;
-  }
-  return top;
-}
-
-// Evaluates the region in workspace slot `slot` of lane g with the rule
-// selected by KEY (one point segment: no block barrier).
+// Evaluates the region (lo, wd) with the rule selected by KEY, 32 segments
+// per half wavefront; every lane of a half returns that half's result.
 template <int D, int FAM, int KEY>
-__device__ __forceinline__ RegionOut p2_eval(const Phase2Args& A, const DevIntegrand& P, const RuleConst& R,
-                                             int64_t g, int32_t slot)
+__device__ __forceinline__ RegionOut p2_eval(const PlainIntegrand<D, FAM>& f, const RuleConst& R,
+                                             const double (&lo)[D], const double (&wd)[D], int s)
 {
-  const PlainIntegrand<D, FAM> f{P};
-  const double* lo = A.slo + (int64_t)slot * D * A.L;
-  const double* wd = A.swd + (int64_t)slot * D * A.L;
-  const int lane = threadIdx.x & (kWave - 1);
   if constexpr (KEY == 9)
-    return seg_eval9<D>(f, lo, wd, A.L, g, true, g, 1, 0, 0, lane, 4, nullptr, R.r9, nullptr);
+    return seg_eval9<D, true>(f, lo, wd, 1, 0, true, 0, 32, s, 0, s, 0, nullptr, R.r9, nullptr);
   else
-    return seg_eval<D>(f, lo, wd, A.L, g, true, g, 1, 0, 0, lane, 4, nullptr, R, nullptr);
+    return seg_eval<D, true>(f, lo, wd, 1, 0, true, 0, 32, s, 0, s, 0, nullptr, R, nullptr);
 }
 
 template <int D, int FAM, int KEY>
-__global__ __launch_bounds__(256) void k_phase2(Phase2Args A, DevIntegrand P, RuleConst R)
+__global__ __launch_bounds__(64) void k_phase2(Phase2Args A, DevIntegrand P, RuleConst R)
 {
-  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= A.n) return;  // no barrier below
+  extern __shared__ double p2_keys[];  // error[slots], then creation number[slots]
+  double* const kerr = p2_keys;
+  int32_t* const kseq = reinterpret_cast<int32_t*>(p2_keys + A.slots);
+  const int64_t g = blockIdx.x;  // one block per region of the input list
+  const int lane = threadIdx.x;
+  const int half = lane >> 5;
+  double* const geo = A.geo + g * A.slots * (2 * D);
+  double* const sest = A.sest + g * A.slots;
+  int8_t* const sax = A.sax + g * A.slots;
+  const PlainIntegrand<D, FAM> f{P};
+
+  double lo[D], wd[D];
+#pragma unroll
   for (int m = 0; m < D; ++m) {
-    A.slo[(int64_t)m * A.L + g] = A.lower[m * A.ld + g];
-    A.swd[(int64_t)m * A.L + g] = A.width[m * A.ld + g];
+    lo[m] = A.lower[m * A.ld + g];
+    wd[m] = A.width[m * A.ld + g];
   }
-  RegionOut o = p2_eval<D, FAM, KEY>(A, P, R, g, 0);
-  bool bad = !(isfinite(o.est) && isfinite(o.err));
-  A.sest[g] = o.est;
-  A.serr[g] = o.err;
-  A.sax[g] = (int8_t)o.axis;
-  A.sseq[g] = 0;
-  int32_t hn = 0, next = 1, count = 1;
-  p2_push(A, g, hn, 0);
-  double RES = o.est, ERR = o.err;
-  while (count <= A.pool_max && (count == 1 || ERR > fmax(A.epsrel * fabs(RES), A.epsabs))) {
-    const int32_t a = p2_pop(A, g, hn), b = count;
-    const double pe = A.sest[(int64_t)a * A.L + g], pr = A.serr[(int64_t)a * A.L + g];
-    const int k = A.sax[(int64_t)a * A.L + g];
+  // both halves evaluate the input region; lane 0 keeps it in slot 0
+  const RegionOut o = p2_eval<D, FAM, KEY>(f, R, lo, wd, lane & 31);
+  const double e0 = read_lane_d(o.est, 0), r0 = read_lane_d(o.err, 0);
+  bool bad = !(isfinite(e0) && isfinite(r0));
+  if (lane == 0) {
+#pragma unroll
     for (int m = 0; m < D; ++m) {
-      A.slo[((int64_t)b * D + m) * A.L + g] = A.slo[((int64_t)a * D + m) * A.L + g];
-      A.swd[((int64_t)b * D + m) * A.L + g] = A.swd[((int64_t)a * D + m) * A.L + g];
+      geo[m] = lo[m];
+      geo[D + m] = wd[m];
     }
-    const double hw = 0.5 * A.swd[((int64_t)a * D + k) * A.L + g];
-    A.swd[((int64_t)a * D + k) * A.L + g] = hw;
-    A.swd[((int64_t)b * D + k) * A.L + g] = hw;
-    A.slo[((int64_t)b * D + k) * A.L + g] = A.slo[((int64_t)a * D + k) * A.L + g] + hw;
-    double ce[2], cr[2];
-    int cax[2];
-    for (int c = 0; c < 2; ++c) {
-      const RegionOut q = p2_eval<D, FAM, KEY>(A, P, R, g, c ? b : a);
-      ce[c] = q.est;
-      cr[c] = q.err;
-      cax[c] = q.axis;
-      bad = bad || !(isfinite(q.est) && isfinite(q.err));
+    sest[0] = e0;
+    sax[0] = (int8_t)o.axis;
+    kerr[0] = r0;
+    kseq[0] = 0;
+  }
+  int32_t count = 1, next = 1;
+  double RES = e0, ERR = r0;
+  while (count <= A.pool_max && (count == 1 || ERR > fmax(A.epsrel * fabs(RES), A.epsabs))) {
+    __syncthreads();  // the records written by lanes 0 and 32 are visible to the wave
+    // the slot to bisect: largest error, earliest created among equal errors
+    double be = -INFINITY;
+    int32_t bq = INT32_MAX, bs = -1;
+    for (int32_t t = lane; t < count; t += kWave) {
+      const double e = kerr[t];
+      const int32_t q = kseq[t];
+      if (e > be || (e == be && q < bq)) {
+        be = e;
+        bq = q;
+        bs = t;
+      }
     }
+#pragma unroll
+    for (int msk = 32; msk > 0; msk >>= 1) {
+      const double e2 = shfl_xor_d(be, msk);
+      const int32_t q2 = __shfl_xor(bq, msk, kWave), s2 = __shfl_xor(bs, msk, kWave);
+      if (e2 > be || (e2 == be && q2 < bq)) {
+        be = e2;
+        bq = q2;
+        bs = s2;
+      }
+    }
+    int32_t a = __builtin_amdgcn_readfirstlane(bs);
+    if (a < 0) a = 0;  // every error non-finite: the run is reported as such
+    const int32_t b = count;
+    const double pe = sest[a], pr = kerr[a];
+    const int k = __builtin_amdgcn_readfirstlane((int)sax[a]);
+    double plo[D], pwd[D], wk = 0.0;
+#pragma unroll
+    for (int m = 0; m < D; ++m) {
+      plo[m] = geo[(int64_t)a * (2 * D) + m];
+      pwd[m] = geo[(int64_t)a * (2 * D) + D + m];
+      if (m == k) wk = pwd[m];
+    }
+    const double hw = 0.5 * wk;
+    // lanes 0-31: the lower half along axis k (slot a), lanes 32-63 the upper (slot b)
+#pragma unroll
+    for (int m = 0; m < D; ++m) {
+      wd[m] = m == k ? hw : pwd[m];
+      lo[m] = (m == k && half) ? plo[m] + hw : plo[m];
+    }
+    const RegionOut q = p2_eval<D, FAM, KEY>(f, R, lo, wd, lane & 31);
+    double ce[2] = {read_lane_d(q.est, 0), read_lane_d(q.est, 32)};
+    double cr[2] = {read_lane_d(q.err, 0), read_lane_d(q.err, 32)};
+    bad = bad || !(isfinite(ce[0]) && isfinite(cr[0]) && isfinite(ce[1]) && isfinite(cr[1]));
     // Cuba's refinement from the parent difference (GPUQuadPhases.cu:498-515)
     const double diff = fabs(0.25 * (ce[0] + ce[1] - pe));
     const double e = cr[0] + cr[1];
@@ -149,29 +160,35 @@ __global__ __launch_bounds__(256) void k_phase2(Phase2Args A, DevIntegrand P, Ru
     cr[1] += diff;
     ERR += cr[0] + cr[1] - pr;
     RES += ce[0] + ce[1] - pe;
-    A.sest[(int64_t)a * A.L + g] = ce[0];
-    A.serr[(int64_t)a * A.L + g] = cr[0];
-    A.sax[(int64_t)a * A.L + g] = (int8_t)cax[0];
-    A.sseq[(int64_t)a * A.L + g] = next++;
-    A.sest[(int64_t)b * A.L + g] = ce[1];
-    A.serr[(int64_t)b * A.L + g] = cr[1];
-    A.sax[(int64_t)b * A.L + g] = (int8_t)cax[1];
-    A.sseq[(int64_t)b * A.L + g] = next++;
-    p2_push(A, g, hn, a);
-    p2_push(A, g, hn, b);
+    __syncthreads();  // every lane has read slot a before it is overwritten
+    if ((lane & 31) == 0) {
+      const int32_t slot = half ? b : a;
+#pragma unroll
+      for (int m = 0; m < D; ++m) {
+        geo[(int64_t)slot * (2 * D) + m] = lo[m];
+        geo[(int64_t)slot * (2 * D) + D + m] = wd[m];
+      }
+      sest[slot] = half ? ce[1] : ce[0];
+      sax[slot] = (int8_t)q.axis;
+      kerr[slot] = half ? cr[1] : cr[0];
+      kseq[slot] = next + half;
+    }
+    next += 2;
     ++count;
   }
-  int8_t active = ERR > fmax(A.epsrel * fabs(RES), A.epsabs) ? 1 : 0;
-  if (ERR > 1e10) {  // the reference zeroes a diverging region and flags it
-    RES = 0.0;
-    ERR = 0.0;
-    active = 1;
+  if (lane == 0) {
+    int8_t active = ERR > fmax(A.epsrel * fabs(RES), A.epsabs) ? 1 : 0;
+    if (ERR > 1e10) {  // the reference zeroes a diverging region and flags it
+      RES = 0.0;
+      ERR = 0.0;
+      active = 1;
+    }
+    A.res[g] = RES;
+    A.err[g] = ERR;
+    A.cnt[g] = count;
+    A.act[g] = active;
+    A.nonfinite[g] = bad ? 1 : 0;
   }
-  A.res[g] = RES;
-  A.err[g] = ERR;
-  A.cnt[g] = count;
-  A.act[g] = active;
-  A.nonfinite[g] = bad ? 1 : 0;
 }
 
 using Phase2Launcher = void (*)(const Phase2Args&, const DevIntegrand&, const RuleConst&, hipStream_t);
@@ -179,8 +196,8 @@ using Phase2Launcher = void (*)(const Phase2Args&, const DevIntegrand&, const Ru
 template <int D, int FAM, int KEY>
 void launch_phase2(const Phase2Args& A, const DevIntegrand& P, const RuleConst& R, hipStream_t st)
 {
-  const int64_t nb = (A.n + 255) / 256;
-  hipLaunchKernelGGL((k_phase2<D, FAM, KEY>), dim3((unsigned)nb), dim3(256), 0, st, A, P, R);
+  hipLaunchKernelGGL((k_phase2<D, FAM, KEY>), dim3((unsigned)A.n), dim3(kWave), phase2_lds_bytes(A.slots), st, A,
+                     P, R);
 }
 
 template <int FAM>
