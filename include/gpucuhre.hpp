@@ -60,6 +60,15 @@ template <>
 class GPUcuhre<double> {
  public:
   long long maxeval = 1000000000LL;
+  // The reference's method end to end (GPUquad.h:611-727): with key 9 and
+  // two_phase set, integrate() runs its breadth-first Phase 1 (at most 100
+  // iterations, until phase1_regions regions; 0: 2^14) and its per-region
+  // Phase 2 (at most pool_max bisections per region; 0: 2048), and returns
+  // its flag (the number of Phase-2 regions that missed their own test when
+  // the global test fails).  maxeval does not apply to this method.
+  bool two_phase = false;
+  long long phase1_regions = 0;
+  int pool_max = 0;
 
   // argv flags as in the reference's CommandLineArgs (quad/util/cudaUtil.h:
   // 37-56): --device=N selects the GPU, --v turns on verbose output.
@@ -91,7 +100,9 @@ class GPUcuhre<double> {
       hi[i] = vol ? vol->highs[i] : 1.0;
     }
     cuhre_result r;
-    const int rc = cuhre_integrate(ctx_, &d, dim_, epsrel, epsabs, maxeval, key_, lo, hi, &r);
+    const int rc = two_phase ? cuhre_integrate_two_phase(ctx_, &d, dim_, epsrel, epsabs, key_, phase1_regions,
+                                                         pool_max, lo, hi, &r)
+                             : cuhre_integrate(ctx_, &d, dim_, epsrel, epsabs, maxeval, key_, lo, hi, &r);
     if (rc < 0) return rc;
     integral += r.integral;
     error += r.error;

@@ -1,8 +1,11 @@
 // Caller of the C++ facade written the way the reference's driver
 // (cudaCuhre.cu:25-87) calls GPUcuhre: construct once, integrate the same
 // integrand over a Volume at two tolerances, zeroing the accumulated outputs
-// in between, and print one tab-separated line per call.
+// in between, and print one tab-separated line per call.  With
+// --two-phase it runs the reference's own method (degree-9 rule, Phase 1
+// then Phase 2).
 #include <cstdio>
+#include <cstring>
 
 #include "gpucuhre.hpp"
 
@@ -12,7 +15,11 @@ int main(int argc, char** argv)
   double integral = 0, error = 0;
   size_t nregions = 0, neval = 0;
   int dim = 3;
-  IntegralGPU::Cuhre cuhre(argc, argv, dim, 0, 0, 1);
+  bool two_phase = false;
+  for (int i = 1; i < argc; ++i)
+    if (std::strcmp(argv[i], "--two-phase") == 0) two_phase = true;
+  IntegralGPU::Cuhre cuhre(argc, argv, dim, two_phase ? 9 : 0, 0, 1);
+  cuhre.two_phase = two_phase;
   quad::integrands::CosineSum t(dim);
   float highs[3] = {2, .3f, 1};
   float lows[3] = {1, .1f, 0};

@@ -49,3 +49,25 @@ def test_facade_driver_matches_oracle(tmp_path):
     acc, acc_r = float(lines[2][1]), int(lines[2][2])
     assert abs(acc - 2 * float(lines[0][2])) <= 1e-12 * abs(acc)
     assert acc_r == 2 * int(lines[0][4])
+
+
+@pytest.mark.gpu
+def test_facade_driver_two_phase_matches_oracle(tmp_path):
+    """The driver with --two-phase runs the reference's own method through
+    the facade (key 9, Phase 1 then Phase 2) and agrees with the oracle's
+    two-phase restatement."""
+    exe = build_driver(tmp_path)
+    out = subprocess.run([exe, "--two-phase"], capture_output=True, text=True, timeout=120, check=True).stdout
+    lines = [l.split("\t") for l in out.strip().splitlines()]
+    f = W.cosine_sum(3)
+    import numpy as np
+    lows = [1.0, float(np.float32(0.1)), 0.0]
+    highs = [2.0, float(np.float32(0.3)), 1.0]
+    ex, _ = O.exact(f, lows, highs)
+    for row, eps in zip(lines[:2], (1e-7, 1e-10)):
+        integral, error, nreg, neval, flag = float(row[2]), float(row[3]), int(row[4]), int(row[5]), int(row[6])
+        o = O.integrate_two_phase(W.Problem("drv", f, lows, highs, eps, 1e-12, 10**9), key=9)
+        assert (flag == 0) == (o["fail"] == 0), (flag, o)
+        assert abs(nreg - o["nregions"]) <= 5e-3 * o["nregions"], (nreg, o["nregions"])
+        assert abs(integral - o["integral"]) <= max(error, o["error"]) + 1e-12 * abs(o["integral"])
+        assert abs(integral - ex) <= max(eps * abs(ex), 10 * error)

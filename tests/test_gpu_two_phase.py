@@ -1,13 +1,13 @@
 """GPU parity of the reference's two-phase method (breadth-first Phase 1, then
-one lane per region running its own max-error heap with Cuba's error
-refinement; quad/GPUquad/GPUquad.h:611-727, GPUQuadPhases.cu:422-536)
+one wavefront per region refining its own pool by largest error with Cuba's
+error refinement; quad/GPUquad/GPUquad.h:611-727, GPUQuadPhases.cu:422-536)
 against the oracle's restatement.
 
 Phase 1 is the breadth-first loop (same region counts as the oracle).  In
-Phase 2 a lane picks its next region by the largest error; GPU and CPU
-errors differ in the last bits, so a near-tie can pick a different region
-and change that lane's count: totals are compared within 0.5%, estimates
-within the reported error.
+Phase 2 a region's pool picks its next region by the largest error; GPU and
+CPU errors differ in the last bits (the GPU sums a rule's points in another
+order), so a near-tie can pick a different region and change that region's
+count: totals are compared within 0.5%, estimates within the reported error.
 """
 import numpy as np
 import pytest
@@ -57,3 +57,37 @@ def test_two_phase_counts(ctx):
     n2 = g["nregions"] - 64
     assert g["iterations"] == 1 and n2 > 0 and n2 % 2 == 0
     assert g["neval"] == 64 * 33 + (n2 // 2) * 3 * 33
+
+
+def test_two_phase_pool_cap_matches_oracle(ctx):
+    """A pool of 8 bisections leaves most Phase-2 regions short of their
+    test: the run fails with the reference's flag (the number of such
+    regions) as the oracle's does."""
+    p = W.c2_product_peak8()
+    g = ctx.integrate_two_phase(p, key=9, phase1_regions=256, pool_max=8)
+    o = O.integrate_two_phase(p, key=9, phase1_max=256, pool_max=8)
+    assert g["rc"] == o["rc"] == 1
+    assert g["iterations"] == o["iterations"]
+    assert abs(g["nregions"] - o["nregions"]) <= 5e-3 * o["nregions"], (g["nregions"], o["nregions"])
+    assert abs(g["fail"] - o["fail"]) <= max(2, 0.01 * o["fail"]), (g["fail"], o["fail"])
+    assert abs(g["integral"] - o["integral"]) <= max(g["error"], o["error"])
+
+
+def test_two_phase_workspace_reuse(ctx):
+    """The context keeps the Phase-2 workspace: a smaller run after a larger
+    one gives exactly what a fresh context gives."""
+    small = (W.c1_gaussian3(), 7, 64)
+    big = (W.c2_product_peak8(), 9, 1024)
+    first = ctx.integrate_two_phase(big[0], key=big[1], phase1_regions=big[2])
+    again = ctx.integrate_two_phase(small[0], key=small[1], phase1_regions=small[2])
+    with cuhre.Context() as fresh:
+        ref = fresh.integrate_two_phase(small[0], key=small[1], phase1_regions=small[2])
+        ref_big = fresh.integrate_two_phase(big[0], key=big[1], phase1_regions=big[2])
+    for k in ("nregions", "neval", "iterations", "integral", "error", "rc", "fail"):
+        assert again[k] == ref[k], k
+        assert first[k] == ref_big[k], k
+
+
+def test_two_phase_pool_beyond_lds_is_rejected(ctx):
+    with pytest.raises(cuhre.CuhreError, match="LDS"):
+        ctx.integrate_two_phase(W.c1_gaussian3(), key=7, phase1_regions=64, pool_max=1 << 22)
