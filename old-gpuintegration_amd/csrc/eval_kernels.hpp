@@ -255,7 +255,7 @@ __global__ __launch_bounds__(1024) void k_eval_seg(EvalArgs A, DevIntegrand P, R
   if (A.finalize) {
     Partial p;
     partial_from_region(p, o.est, o.err, o.ab, valid && owner, A.loc_epsrel, A.loc_epsabs);
-    block_reduce_partial(p, ldsw, W, owner);
+    block_reduce_partial(p, ldsw, W, owner, S);
     finish_iteration(A, p, ldsw, W);
   }
 }
@@ -510,7 +510,7 @@ __global__ __launch_bounds__(512) void k_eval9_seg(EvalArgs A, DevIntegrand P, R
   if (A.finalize) {
     Partial p;
     partial_from_region(p, o.est, o.err, o.ab, valid && owner, A.loc_epsrel, A.loc_epsabs);
-    block_reduce_partial(p, ldsw, W, owner);
+    block_reduce_partial(p, ldsw, W, owner, S);
     finish_iteration(A, p, ldsw, W);
   }
 }

@@ -117,9 +117,10 @@ __device__ __forceinline__ void partial_from_region(Partial& p, double est, doub
 
 // Reduce one Partial per thread to thread 0 of a block of W waves.  Waves
 // whose lanes hold no region (contributes == false, wave-uniform) skip the
-// shuffles and add zeros.
+// shuffles and add zeros; thread 0 adds the waves 0, wstep, 2 wstep, ...
+// (the others are known to hold nothing).
 __device__ __forceinline__ void block_reduce_partial(Partial& p, Partial* ldsw, int W,
-                                                     bool contributes = true)
+                                                     bool contributes = true, int wstep = 1)
 {
   if (contributes) {
 #pragma unroll
@@ -132,10 +133,46 @@ __device__ __forceinline__ void block_reduce_partial(Partial& p, Partial* ldsw, 
   __syncthreads();
   if (threadIdx.x == 0) {
     Partial t = ldsw[0];
-    for (int w = 1; w < W; ++w) partial_add(t, ldsw[w]);
+    for (int w = wstep; w < W; w += wstep) partial_add(t, ldsw[w]);
     p = t;
   }
 }
+
+// Global stop decision from the iteration's total q; thread 0 of the last
+// block writes the state (and publishes it to the host when asked).
+__device__ __forceinline__ void write_iteration_state(const EvalArgs& A, const Partial& q)
+{
+  IterState st;
+  st.tot = q;
+  st.rG_s = A.fin_est_s;
+  st.rG_c = A.fin_est_c;
+  dd_add(st.rG_s, st.rG_c, q.est_s, q.est_c);
+  st.eG_s = A.fin_err_s;
+  st.eG_c = A.fin_err_c;
+  dd_add(st.eG_s, st.eG_c, q.err_s, q.err_c);
+  const double rG = st.rG_s + st.rG_c, eG = st.eG_s + st.eG_c;
+  int stop = 0;
+  if (q.nnonfinite > 0) stop = 4;
+  else if (eG <= fmax(A.epsrel * fabs(rG), A.epsabs)) stop = 1;
+  else if (q.nactive == 0) stop = 2;
+  else if (q.nactive > A.max_active) stop = 3;
+  else if (q.nactive > A.alloc_active) stop = 5;
+  st.stop = stop;
+  st.pad = 0;
+  *A.state = st;
+  if (A.host_state != nullptr) {
+    // publish to the host, which polls host_seq instead of synchronising
+    *A.host_state = st;
+    __hip_atomic_store(A.host_seq, A.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+  // reset for the next launch (stream order makes this visible to it)
+  __hip_atomic_store(A.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Grids up to this many blocks are finalised by one wavefront of the last
+// block (no barrier): the block-wide path costs ~10 us of barriers and
+// serial combines, which dominated launches of a few thousand regions.
+constexpr uint32_t kWaveFinalize = 4 * kWave;
 
 // Called by every block after its thread 0 holds the block's Partial in p.
 // The last block to arrive reduces all partials, scans active counts into
@@ -153,13 +190,39 @@ __device__ __forceinline__ void finish_iteration(const EvalArgs& A, Partial& p, 
   if (!s_last) return;
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 
+  Partial q;
+  partial_zero(q);
+  if (nb <= kWaveFinalize) {
+    // one wavefront: each lane sums a contiguous chunk of blocks in order,
+    // then a shuffle scan of the active counts and a shuffle reduction
+    if (threadIdx.x >= kWave) return;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t chunk = (nb + kWave - 1) / kWave;
+    const uint32_t b0 = lane * chunk < nb ? lane * chunk : nb;
+    const uint32_t b1 = b0 + chunk < nb ? b0 + chunk : nb;
+    for (uint32_t b = b0; b < b1; ++b) partial_add(q, A.partials[b]);
+    long long incl = (long long)q.nactive;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      const long long v = __shfl_up(incl, off, kWave);
+      if ((int)lane >= off) incl += v;
+    }
+    int64_t run = (int64_t)incl - q.nactive;
+    for (uint32_t b = b0; b < b1; ++b) {
+      A.offsets[b] = run;
+      run += A.partials[b].nactive;
+    }
+#pragma unroll
+    for (int mask = 1; mask < kWave; mask <<= 1) partial_shfl_add(q, mask);
+    if (lane == 0) write_iteration_state(A, q);
+    return;
+  }
+
   // contiguous chunk of blocks per thread, summed in order
   const uint32_t T = blockDim.x;
   const uint32_t chunk = (nb + T - 1) / T;
-  const uint32_t b0 = threadIdx.x * chunk;
+  const uint32_t b0 = threadIdx.x * chunk < nb ? threadIdx.x * chunk : nb;
   const uint32_t b1 = b0 + chunk < nb ? b0 + chunk : nb;
-  Partial q;
-  partial_zero(q);
   for (uint32_t b = b0; b < b1; ++b) partial_add(q, A.partials[b]);
   // exclusive scan of per-thread active counts (Hillis-Steele in LDS)
   __shared__ int64_t s_scan[kMaxWaves * kWave];
@@ -178,33 +241,7 @@ __device__ __forceinline__ void finish_iteration(const EvalArgs& A, Partial& p, 
     run += A.partials[b].nactive;
   }
   block_reduce_partial(q, ldsw, W);
-  if (threadIdx.x == 0) {
-    IterState st;
-    st.tot = q;
-    st.rG_s = A.fin_est_s;
-    st.rG_c = A.fin_est_c;
-    dd_add(st.rG_s, st.rG_c, q.est_s, q.est_c);
-    st.eG_s = A.fin_err_s;
-    st.eG_c = A.fin_err_c;
-    dd_add(st.eG_s, st.eG_c, q.err_s, q.err_c);
-    const double rG = st.rG_s + st.rG_c, eG = st.eG_s + st.eG_c;
-    int stop = 0;
-    if (q.nnonfinite > 0) stop = 4;
-    else if (eG <= fmax(A.epsrel * fabs(rG), A.epsabs)) stop = 1;
-    else if (q.nactive == 0) stop = 2;
-    else if (q.nactive > A.max_active) stop = 3;
-    else if (q.nactive > A.alloc_active) stop = 5;
-    st.stop = stop;
-    st.pad = 0;
-    *A.state = st;
-    if (A.host_state != nullptr) {
-      // publish to the host, which polls host_seq instead of synchronising
-      *A.host_state = st;
-      __hip_atomic_store(A.host_seq, A.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-    }
-    // reset for the next launch (stream order makes this visible to it)
-    __hip_atomic_store(A.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
+  if (threadIdx.x == 0) write_iteration_state(A, q);
 }
 
 }  // namespace cuhre
