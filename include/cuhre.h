@@ -107,7 +107,9 @@ int cuhre_context_destroy(cuhre_context* ctx);
 /* Engine knobs.  profile != 0 records HIP events around every launch
  * (fills ms_eval/ms_split).  point_segments: 0 automatic, else 1, 2, 4, 8
  * or 16 wavefronts share the points of each group of 64 regions (one region
- * per lane; more segments give short lists more parallelism). */
+ * per lane; more segments give short lists more parallelism), or 32: the
+ * 32 lanes of a half wavefront share the points of one region (the
+ * automatic choice for the shortest lists). */
 int cuhre_context_set(cuhre_context* ctx, int profile, int point_segments);
 
 /* Plain adaptive integration over the box [lows, highs].  key selects the

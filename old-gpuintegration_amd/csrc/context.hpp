@@ -45,6 +45,7 @@ struct cuhre_context {
   int profile = 0;
   int segments = 0;                // 0: automatic, else 1, 2, 4, 8 or 16
   int64_t fill_lanes = 262144;     // auto: smallest S with n*S >= fill_lanes
+  int64_t half_max_regions = 4096;  // auto: half-wavefront form up to this many regions
 
   int64_t cap = 0;  // allocated regions per list buffer
   int dim_alloc = 0;
@@ -56,10 +57,9 @@ struct cuhre_context {
   int8_t* axis = nullptr;
   cuhre::Partial* partials = nullptr;
   int64_t* offsets = nullptr;
-  uint32_t* ticket = nullptr;
   cuhre::IterState* state = nullptr;
   cuhre::IterState* h_state = nullptr;
-  // coherent host memory the evaluation's last block publishes its state to
+  // coherent host memory the finalisation kernel publishes the state to
   // (host and device views), with a sequence number written last
   cuhre::IterState* pub_state = nullptr;
   uint32_t* pub_seq = nullptr;
@@ -89,6 +89,9 @@ int ctx_reserve_aux(cuhre_context* c, int64_t cap, int dim);
 int make_dev_integrand(cuhre_context* c, const cuhre_integrand* f, int dim, DevIntegrand* P);
 int64_t list_capacity(cuhre_context* c, int dim);
 int pick_segments(const cuhre_context* c, int64_t n, int key = 7);
+// Block partials / offsets the list buffers of cap regions come with: one
+// per 64 regions, and at least 1024 (the half-wavefront form on 8192 regions).
+inline int64_t block_capacity(int64_t cap) { return cap / kWave + 1 > 1024 ? cap / kWave + 1 : 1024; }
 int64_t initial_grid(int d, const double* lows, const double* highs, std::vector<double>& lo,
                      std::vector<double>& wd);
 int check_common(cuhre_context* ctx, const cuhre_integrand* f, int dim, int key);

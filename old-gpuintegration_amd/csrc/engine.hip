@@ -263,7 +263,7 @@ double region_flops(const cuhre_integrand* f, int d, int key)
 // quad/GPUquad/GPUKernelquad.cu:63-149, 356-505).  The rank of an active
 // region inside its evaluation block comes from a wavefront ballot (plus
 // the counts of earlier waves of the same block through LDS); the block's
-// base offset was scanned by the evaluation kernel's last block.  Child 0
+// base offset was scanned by k_finish.  Child 0
 // of the j-th active region goes to slot 2j and child 1 to 2j+1, so the
 // two children of a region are adjacent and written with one 16-byte store
 // per coordinate.
@@ -401,6 +401,14 @@ extern "C" __attribute__((visibility("hidden"))) BatchEngine* cuhre_ctx_batch(cu
   return c->batch;
 }
 
+__global__ __launch_bounds__(kBlock) void k_finish(EvalArgs A, uint32_t nb) { finish_iteration(A, nb); }
+
+void launch_finish(const EvalArgs& A, int64_t nb, hipStream_t st)
+{
+  const int T = nb <= (int64_t)kWaveFinalize ? kWave : kBlock;
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(T), 0, st, A, (uint32_t)nb);
+}
+
 int64_t blocks_for(int64_t n, int S)
 {
   const int rpb = seg_regions_per_block(S);
@@ -437,7 +445,7 @@ int ctx_reserve(cuhre_context* c, int64_t cap, int dim, int64_t keep_n, int cur,
   cap = std::max(cap, c->cap);
   dim = std::max(dim, c->dim_alloc);
   cap = (cap + 255) & ~255ll;
-  const int64_t nbmax = cap / kWave + 1;
+  const int64_t nbmax = block_capacity(cap);
   double *lo[2], *wd[2], *est, *err, *ab;
   int8_t* ax;
   Partial* part;
@@ -558,7 +566,11 @@ int64_t list_capacity(cuhre_context* c, int dim)
 int pick_segments(const cuhre_context* c, int64_t n, int key)
 {
   const int smax = key == 9 ? 8 : kMaxWaves;  // the degree-9 merge needs more LDS per wave
+  // the half-wavefront form needs n/8 block partials
+  const bool half_fits = (n + 7) / 8 <= block_capacity(c->cap);
+  if (c->segments == kHalfS && half_fits) return kHalfS;
   if (c->segments > 0) return std::min(c->segments, smax);
+  if (n <= c->half_max_regions && half_fits) return kHalfS;
   int S = 1;
   while (S < smax && n * S < c->fill_lanes) S *= 2;
   if (S == 1 && n < (2ll << 20)) S = 2;
@@ -622,7 +634,6 @@ void fill_eval_args(cuhre_context* c, EvalArgs& A, int cur, int64_t n, int dim, 
   A.abssum = c->abssum;
   A.partials = c->partials;
   A.offsets = c->offsets;
-  A.ticket = c->ticket;
   A.state = c->state;
   (void)dim;
 }
@@ -689,14 +700,12 @@ int cuhre_context_create(cuhre_context** out, int device, int64_t capacity_regio
   }
   int rc = 0;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipMalloc(&c->ticket, sizeof(uint32_t)) != hipSuccess ||
       hipMalloc(&c->state, sizeof(IterState)) != hipSuccess ||
       hipHostMalloc(&c->h_state, sizeof(IterState), hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc(&c->pub_state, sizeof(IterState), hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
       hipHostMalloc(&c->pub_seq, sizeof(uint32_t), hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
       hipHostGetDevicePointer((void**)&c->pub_state_dev, c->pub_state, 0) != hipSuccess ||
       hipHostGetDevicePointer((void**)&c->pub_seq_dev, c->pub_seq, 0) != hipSuccess ||
-      hipMemset(c->ticket, 0, sizeof(uint32_t)) != hipSuccess ||
       hipMalloc(&c->sum_out, sizeof(double)) != hipSuccess) {
     rc = fail(CUHRE_ERR_DEVICE, "context setup failed: %s", hipGetErrorString(hipGetLastError()));
   }
@@ -722,7 +731,6 @@ int cuhre_context_destroy(cuhre_context* c)
   c->batch = nullptr;
   if (c->aux) (void)hipFree(c->aux);
   if (c->p2_ws) (void)hipFree(c->p2_ws);
-  if (c->ticket) (void)hipFree(c->ticket);
   if (c->state) (void)hipFree(c->state);
   if (c->h_state) (void)hipHostFree(c->h_state);
   if (c->pub_state) (void)hipHostFree(c->pub_state);
@@ -741,8 +749,8 @@ int cuhre_context_set(cuhre_context* c, int profile, int point_segments)
 {
   if (!c) return fail(CUHRE_ERR_ARG, "null context");
   if (point_segments != 0 && point_segments != 1 && point_segments != 2 && point_segments != 4 &&
-      point_segments != 8 && point_segments != 16)
-    return fail(CUHRE_ERR_ARG, "point_segments must be 0, 1, 2, 4, 8 or 16");
+      point_segments != 8 && point_segments != 16 && point_segments != kHalfS)
+    return fail(CUHRE_ERR_ARG, "point_segments must be 0, 1, 2, 4, 8, 16 or 32");
   c->profile = profile;
   c->segments = point_segments;
   return 0;
@@ -751,7 +759,7 @@ int cuhre_context_set(cuhre_context* c, int profile, int point_segments)
 namespace {
 
 // Waits for the evaluation launch numbered seq to publish its state (its
-// last block writes the state, then the number, to coherent host memory).
+// k_finish writes the state, then the number, to coherent host memory).
 // Meanwhile the split already queued behind it keeps the GPU busy.
 int wait_published(cuhre_context* c, uint32_t seq, IterState* out)
 {
@@ -858,6 +866,7 @@ int plain_loop(cuhre_context* c, const cuhre_integrand* f, int dim, double epsre
     if (c->profile) HIP_TRY(hipEventRecord(c->ev[0], c->stream));
     launch(A, P, R, S, c->stream);
     if (c->profile) HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+    launch_finish(A, nb, c->stream);
     hipLaunchKernelGGL(k_split, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, c->stream, SA);
     if (c->profile) HIP_TRY(hipEventRecord(c->ev[2], c->stream));
     HIP_TRY(hipGetLastError());

@@ -22,8 +22,8 @@
 // because every region then pays a cross-lane reduction and a per-lane
 // generator decode.)
 //
-// The kernel also fuses the classification err > max(epsrel |est|, epsabs),
-// the per-block partial sums and the last-block finalisation (reduce.hpp).
+// The kernel also fuses the classification err > max(epsrel |est|, epsabs)
+// and the per-block partial sums; k_finish (reduce.hpp) reduces them.
 #pragma once
 
 #include "integrands.hpp"
@@ -43,7 +43,17 @@ inline size_t seg_lds_bytes(int S)
 }
 
 inline int seg_waves(int S) { return S < 4 ? 4 : S; }
-inline int seg_regions_per_block(int S) { return kWave * (seg_waves(S) / S); }
+
+// Shortest lists: one region per half wavefront, its points in S = 32
+// segments (lanes 0-31 / 32-63) merged by a xor butterfly, 8 regions per
+// 256-thread block.  A few thousand regions then spread over every CU instead
+// of stacking 16 segment waves of 64 regions on one CU.
+constexpr int kHalfS = 32;
+
+inline int seg_regions_per_block(int S)
+{
+  return S == kHalfS ? kBlock / kHalfS : kWave * (seg_waves(S) / S);
+}
 
 // Integrand of the plain kernel: one family, parameters in kernel arguments.
 template <int D, int FAM>
@@ -256,14 +266,54 @@ __global__ __launch_bounds__(1024) void k_eval_seg(EvalArgs A, DevIntegrand P, R
     Partial p;
     partial_from_region(p, o.est, o.err, o.ab, valid && owner, A.loc_epsrel, A.loc_epsabs);
     block_reduce_partial(p, ldsw, W, owner, S);
-    finish_iteration(A, p, ldsw, W);
+    store_block_partial(A, p);
   }
+}
+
+// Half-wavefront form (S = kHalfS): 8 regions per 256-thread block.
+template <int D, int FAM, int KEY>
+__global__ __launch_bounds__(256) void k_eval_half(EvalArgs A, DevIntegrand P, RuleConst R)
+{
+  __shared__ Partial ldsw[kBlock / kWave];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int s = threadIdx.x & (kHalfS - 1);
+  const int64_t r = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kHalfS;
+  const bool valid = r < A.n;
+  const int64_t rr = valid ? r : A.n - 1;
+  const PlainIntegrand<D, FAM> f{P};
+  RegionOut o;
+  if constexpr (KEY == 9)
+    o = seg_eval9<D, true>(f, A.lower, A.width, A.ld, r, valid, rr, kHalfS, s, 0, lane, 0, nullptr, R.r9, A.dvec);
+  else
+    o = seg_eval<D, true>(f, A.lower, A.width, A.ld, r, valid, rr, kHalfS, s, 0, lane, 0, nullptr, R, A.dvec);
+  const bool owner = s == 0;
+  if (valid && owner) {
+    A.est[r] = o.est;
+    A.err[r] = o.err;
+    A.axis[r] = (int8_t)o.axis;
+    if (A.abssum) A.abssum[r] = o.ab;
+    if (A.i5) A.i5[r] = o.i5;
+  }
+  if (A.finalize) {
+    Partial p;
+    partial_from_region(p, o.est, o.err, o.ab, valid && owner, A.loc_epsrel, A.loc_epsabs);
+    block_reduce_partial(p, ldsw, kBlock / kWave);
+    store_block_partial(A, p);
+  }
+}
+
+template <int D, int FAM, int KEY>
+void launch_half(const EvalArgs& A, const DevIntegrand& P, const RuleConst& R, hipStream_t st)
+{
+  const int64_t nb = (A.n + kBlock / kHalfS - 1) / (kBlock / kHalfS);
+  hipLaunchKernelGGL((k_eval_half<D, FAM, KEY>), dim3((unsigned)nb), dim3(kBlock), 0, st, A, P, R);
 }
 
 template <int D, int FAM>
 void launch_eval(const EvalArgs& A, const DevIntegrand& P, const RuleConst& R, int S,
                  hipStream_t st)
 {
+  if (S == kHalfS) return launch_half<D, FAM, 7>(A, P, R, st);
   const int rpb = seg_regions_per_block(S);
   const int64_t nb = (A.n + rpb - 1) / rpb;
   hipLaunchKernelGGL((k_eval_seg<D, FAM>), dim3((unsigned)nb), dim3(kWave * seg_waves(S)),
@@ -511,13 +561,14 @@ __global__ __launch_bounds__(512) void k_eval9_seg(EvalArgs A, DevIntegrand P, R
     Partial p;
     partial_from_region(p, o.est, o.err, o.ab, valid && owner, A.loc_epsrel, A.loc_epsabs);
     block_reduce_partial(p, ldsw, W, owner, S);
-    finish_iteration(A, p, ldsw, W);
+    store_block_partial(A, p);
   }
 }
 
 template <int D, int FAM>
 void launch_eval9(const EvalArgs& A, const DevIntegrand& P, const RuleConst& R, int S, hipStream_t st)
 {
+  if (S == kHalfS) return launch_half<D, FAM, 9>(A, P, R, st);
   if (S > 8) S = 8;  // LDS of the merge: 7 doubles + 1 int per lane and wave
   const int rpb = seg_regions_per_block(S);
   const int64_t nb = (A.n + rpb - 1) / rpb;

@@ -1091,6 +1091,7 @@ int guided_run(cuhre_context* c, cuhre_guide* g, const cuhre_integrand* f, int d
     A.max_active = std::min<int64_t>(left >= 0 ? left / (2 * p) : -1, capacity / 2);
     A.alloc_active = c->cap / 2;
     find_eval_launcher(f->family, dim)(A, P, R, S, st);
+    launch_finish(A, blocks_for(n, S), st);
     if (hipGetLastError() != hipSuccess) return set_error(CUHRE_ERR_DEVICE, "guided evaluation failed");
 
     // The guide step (network, finish rule, scan) and the split.  With

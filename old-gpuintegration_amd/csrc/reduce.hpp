@@ -1,15 +1,18 @@
 // reduce.hpp -- deterministic FP64 wave/block reductions and the
-// last-block finalisation of an iteration.
+// finalisation of an iteration.
 //
 // The reference sums est/err with four thrust::reduce calls and a blocking
 // copy per iteration (quad/GPUquad/GPUKernelquad.cu:565-577) and counts the
 // active regions with thrust::exclusive_scan + two device-to-host copies
 // (:364-386).  Here each evaluation block folds its regions into one Partial
-// (wave shuffles, then LDS across the four waves), and the block that
-// finishes last -- found with an acq_rel ticket at agent scope -- reduces the
-// partials in a fixed order, scans the per-block active counts and takes the
-// stop decision, so an iteration needs no extra launch and no host sync.
-// Sums are carried as (value, rounding error) pairs built with TwoSum.
+// (wave shuffles, then LDS across the waves) and stores it; a one-block
+// kernel queued behind the evaluation (k_finish) reduces the partials in a
+// fixed order, scans the per-block active counts into split offsets and
+// takes the stop decision, so an iteration needs no host sync.  (A fused
+// last-block ticket was measured first: the acq_rel atomic on one counter
+// serialises at ~23 ns per block, 12 us at 512 blocks and ~190 us at 8192,
+// against ~3 us for the extra launch.)  Sums are carried as (value,
+// rounding error) pairs built with TwoSum.
 #pragma once
 
 #include "common.hpp"
@@ -138,8 +141,8 @@ __device__ __forceinline__ void block_reduce_partial(Partial& p, Partial* ldsw, 
   }
 }
 
-// Global stop decision from the iteration's total q; thread 0 of the last
-// block writes the state (and publishes it to the host when asked).
+// Global stop decision from the iteration's total q; one thread writes the
+// state (and publishes it to the host when asked).
 __device__ __forceinline__ void write_iteration_state(const EvalArgs& A, const Partial& q)
 {
   IterState st;
@@ -165,83 +168,63 @@ __device__ __forceinline__ void write_iteration_state(const EvalArgs& A, const P
     *A.host_state = st;
     __hip_atomic_store(A.host_seq, A.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
   }
-  // reset for the next launch (stream order makes this visible to it)
-  __hip_atomic_store(A.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Grids up to this many blocks are finalised by one wavefront of the last
-// block (no barrier): the block-wide path costs ~10 us of barriers and
-// serial combines, which dominated launches of a few thousand regions.
+// Launches up to this many blocks are finalised by one wavefront, larger
+// ones by four.
 constexpr uint32_t kWaveFinalize = 4 * kWave;
 
-// Called by every block after its thread 0 holds the block's Partial in p.
-// The last block to arrive reduces all partials, scans active counts into
-// offsets and writes the iteration state.
-__device__ __forceinline__ void finish_iteration(const EvalArgs& A, Partial& p, Partial* ldsw, int W)
+// Called by every evaluation block after its thread 0 holds the block's
+// Partial in p.
+__device__ __forceinline__ void store_block_partial(const EvalArgs& A, const Partial& p)
 {
-  __shared__ uint32_t s_last;
-  const uint32_t nb = gridDim.x;
-  if (threadIdx.x == 0) {
-    A.partials[blockIdx.x] = p;
-    uint32_t t = __hip_atomic_fetch_add(A.ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    s_last = (t == nb - 1) ? 1u : 0u;
-  }
-  __syncthreads();
-  if (!s_last) return;
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  if (threadIdx.x == 0) A.partials[blockIdx.x] = p;
+}
 
-  Partial q;
-  partial_zero(q);
-  if (nb <= kWaveFinalize) {
-    // one wavefront: each lane sums a contiguous chunk of blocks in order,
-    // then a shuffle scan of the active counts and a shuffle reduction
-    if (threadIdx.x >= kWave) return;
-    const uint32_t lane = threadIdx.x;
-    const uint32_t chunk = (nb + kWave - 1) / kWave;
-    const uint32_t b0 = lane * chunk < nb ? lane * chunk : nb;
-    const uint32_t b1 = b0 + chunk < nb ? b0 + chunk : nb;
-    for (uint32_t b = b0; b < b1; ++b) partial_add(q, A.partials[b]);
-    long long incl = (long long)q.nactive;
-#pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-      const long long v = __shfl_up(incl, off, kWave);
-      if ((int)lane >= off) incl += v;
-    }
-    int64_t run = (int64_t)incl - q.nactive;
-    for (uint32_t b = b0; b < b1; ++b) {
-      A.offsets[b] = run;
-      run += A.partials[b].nactive;
-    }
-#pragma unroll
-    for (int mask = 1; mask < kWave; mask <<= 1) partial_shfl_add(q, mask);
-    if (lane == 0) write_iteration_state(A, q);
-    return;
-  }
-
-  // contiguous chunk of blocks per thread, summed in order
-  const uint32_t T = blockDim.x;
+// Body of k_finish: one block of W <= 16 wavefronts reduces the nb block
+// partials of the evaluation launch before it (stream order makes them
+// visible).  Each thread sums a contiguous chunk of blocks in order; each
+// wave scans its active counts and reduces its partials with shuffles;
+// one barrier hands the wave totals over, wave 0 reduces them.
+__device__ __forceinline__ void finish_iteration(const EvalArgs& A, uint32_t nb)
+{
+  __shared__ Partial ldsw[kMaxWaves];
+  __shared__ int64_t s_cnt[kMaxWaves];
+  const uint32_t T = blockDim.x, W = T / kWave;
+  const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
   const uint32_t chunk = (nb + T - 1) / T;
   const uint32_t b0 = threadIdx.x * chunk < nb ? threadIdx.x * chunk : nb;
   const uint32_t b1 = b0 + chunk < nb ? b0 + chunk : nb;
+  Partial q;
+  partial_zero(q);
   for (uint32_t b = b0; b < b1; ++b) partial_add(q, A.partials[b]);
-  // exclusive scan of per-thread active counts (Hillis-Steele in LDS)
-  __shared__ int64_t s_scan[kMaxWaves * kWave];
-  int64_t mine = q.nactive;
-  s_scan[threadIdx.x] = mine;
-  __syncthreads();
-  for (uint32_t off = 1; off < T; off <<= 1) {
-    int64_t v = threadIdx.x >= off ? s_scan[threadIdx.x - off] : 0;
-    __syncthreads();
-    s_scan[threadIdx.x] += v;
-    __syncthreads();
+  const int64_t mine = q.nactive;
+  long long incl = (long long)mine;
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const long long v = __shfl_up(incl, off, kWave);
+    if ((int)lane >= off) incl += v;
   }
-  int64_t run = s_scan[threadIdx.x] - mine;
+#pragma unroll
+  for (int mask = 1; mask < kWave; mask <<= 1) partial_shfl_add(q, mask);
+  if (lane == kWave - 1) s_cnt[wave] = (int64_t)incl;
+  if (lane == 0) ldsw[wave] = q;
+  __syncthreads();
+  int64_t run = (int64_t)incl - mine;
+  for (uint32_t w = 0; w < wave; ++w) run += s_cnt[w];
   for (uint32_t b = b0; b < b1; ++b) {
     A.offsets[b] = run;
     run += A.partials[b].nactive;
   }
-  block_reduce_partial(q, ldsw, W);
-  if (threadIdx.x == 0) write_iteration_state(A, q);
+  if (wave != 0) return;
+  Partial t;
+  if (lane < W) t = ldsw[lane];
+  else partial_zero(t);
+  for (uint32_t mask = 1; mask < W; mask <<= 1) partial_shfl_add(t, (int)mask);
+  if (lane == 0) write_iteration_state(A, t);
 }
+
+// Queues k_finish for an evaluation launch of nb blocks.
+void launch_finish(const EvalArgs& A, int64_t nb, hipStream_t st);
 
 }  // namespace cuhre

@@ -80,7 +80,7 @@ FAMILIES = [W.GAUSSIAN, W.PRODUCT_PEAK, W.OSCILLATORY, W.CORNER_PEAK, W.CONTINUO
             W.COSINE_SUM]
 
 
-@pytest.mark.parametrize("lanes", [1, 4, 16])
+@pytest.mark.parametrize("lanes", [1, 4, 16, 32])
 @pytest.mark.parametrize("d", [2, 3, 5, 8])
 @pytest.mark.parametrize("fam", FAMILIES)
 def test_region_values_match_oracle(ctx, fam, d, lanes):
@@ -98,7 +98,7 @@ def test_region_values_match_oracle(ctx, fam, d, lanes):
     assert_region_parity(g, o, d, (fam, d, lanes), np.prod(Wd, axis=1))
 
 
-@pytest.mark.parametrize("lanes", [1, 2, 8])
+@pytest.mark.parametrize("lanes", [1, 2, 8, 32])
 def test_region_values_dim10_and_table(ctx, lanes):
     rng = W.SplitMix64(77 + lanes)
     f = family_instance(W.PRODUCT_PEAK, 10, rng)

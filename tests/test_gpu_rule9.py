@@ -33,7 +33,7 @@ def assert_err_parity(err, oerr, osums, vol, scale, d, tag):
         assert tie <= 1e-12 * scale[r], (tag, r, e, scale[r])
 
 
-@pytest.mark.parametrize("lanes", [1, 8])
+@pytest.mark.parametrize("lanes", [1, 8, 32])
 @pytest.mark.parametrize("d", [2, 3, 5, 8])
 @pytest.mark.parametrize("fam", FAMILIES)
 def test_rule9_region_values_match_oracle(ctx, fam, d, lanes):
