@@ -88,6 +88,8 @@ struct EvalArgs {
   double* dvec;    // optional, dvec[m*ld + r]
   Partial* partials;
   int64_t* offsets;  // exclusive scan of active counts per block
+  Partial* gpart;    // per group of 64 blocks (launches of many blocks)
+  int64_t* gbase;
   IterState* state;
   double fin_est_s, fin_est_c, fin_err_s, fin_err_c;
   int64_t max_active;    // stop (eval budget / capacity) above this count

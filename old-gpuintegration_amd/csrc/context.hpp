@@ -31,6 +31,7 @@ struct SplitArgs {
   const IterState* state;  // nullptr: always split
   const int8_t* retire;    // optional: regions retired as finished are not split
   const int* retire_on;    // optional device flag: retire honoured only when nonzero
+  const int64_t* gbase;    // optional: base offset per group of 64 blocks (two-level finalisation)
 };
 
 void launch_split(const SplitArgs& S, hipStream_t st);
@@ -92,6 +93,8 @@ int pick_segments(const cuhre_context* c, int64_t n, int key = 7);
 // Block partials / offsets the list buffers of cap regions come with: one
 // per 64 regions, and at least 1024 (the half-wavefront form on 8192 regions).
 inline int64_t block_capacity(int64_t cap) { return cap / kWave + 1 > 1024 ? cap / kWave + 1 : 1024; }
+// Group sums of the two-level finalisation: one per 64 blocks.
+inline int64_t group_capacity(int64_t nbmax) { return nbmax / kWave + 2; }
 int64_t initial_grid(int d, const double* lows, const double* highs, std::vector<double>& lo,
                      std::vector<double>& wd);
 int check_common(cuhre_context* ctx, const cuhre_integrand* f, int dim, int key);

@@ -296,7 +296,8 @@ __global__ __launch_bounds__(kBlock) void k_split(SplitArgs S)
     rank = __popcll(mask & grp & below);
   }
   if (!act) return;
-  const int64_t dst = S.offsets[r / S.rpb] + rank;
+  const int64_t blk = r / S.rpb;
+  const int64_t dst = S.offsets[blk] + (S.gbase != nullptr ? S.gbase[blk / kWave] : 0) + rank;
   const int a = S.axis[r];
   for (int m = 0; m < S.dim; ++m) {
     const double lo = S.lower[m * S.ld + r];
@@ -401,12 +402,20 @@ extern "C" __attribute__((visibility("hidden"))) BatchEngine* cuhre_ctx_batch(cu
   return c->batch;
 }
 
-__global__ __launch_bounds__(kBlock) void k_finish(EvalArgs A, uint32_t nb) { finish_iteration(A, nb); }
+__global__ __launch_bounds__(kWave) void k_finish(EvalArgs A, uint32_t nb) { finish_iteration(A, nb); }
+__global__ __launch_bounds__(kWave) void k_finish_groups(EvalArgs A, uint32_t nb) { finish_groups(A, nb); }
+__global__ __launch_bounds__(kMaxWaves * kWave) void k_finish_top(EvalArgs A, uint32_t G) { finish_top(A, G); }
 
 void launch_finish(const EvalArgs& A, int64_t nb, hipStream_t st)
 {
-  const int T = nb <= (int64_t)kWaveFinalize ? kWave : kBlock;
-  hipLaunchKernelGGL(k_finish, dim3(1), dim3(T), 0, st, A, (uint32_t)nb);
+  if (nb <= (int64_t)kWaveFinalize) {
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(kWave), 0, st, A, (uint32_t)nb);
+    return;
+  }
+  const int64_t G = (nb + kWave - 1) / kWave;
+  hipLaunchKernelGGL(k_finish_groups, dim3((unsigned)G), dim3(kWave), 0, st, A, (uint32_t)nb);
+  hipLaunchKernelGGL(k_finish_top, dim3(1), dim3(G <= (int64_t)kWaveFinalize ? kWave : kMaxWaves * kWave), 0, st,
+                     A, (uint32_t)G);
 }
 
 int64_t blocks_for(int64_t n, int S)
@@ -456,8 +465,8 @@ int ctx_reserve(cuhre_context* c, int64_t cap, int dim, int64_t keep_n, int cur,
       hipMalloc(&est, sizeof(double) * cap) != hipSuccess ||
       hipMalloc(&err, sizeof(double) * cap) != hipSuccess ||
       hipMalloc(&ab, sizeof(double) * cap) != hipSuccess || hipMalloc(&ax, cap) != hipSuccess ||
-      hipMalloc(&part, sizeof(Partial) * nbmax) != hipSuccess ||
-      hipMalloc(&off, sizeof(int64_t) * nbmax) != hipSuccess) {
+      hipMalloc(&part, sizeof(Partial) * (nbmax + group_capacity(nbmax))) != hipSuccess ||
+      hipMalloc(&off, sizeof(int64_t) * (nbmax + group_capacity(nbmax))) != hipSuccess) {
     (void)hipGetLastError();
     return fail(CUHRE_ERR_ALLOC, "cannot allocate a region list of %lld regions x %d dims",
                 (long long)cap, dim);
@@ -473,6 +482,10 @@ int ctx_reserve(cuhre_context* c, int64_t cap, int dim, int64_t keep_n, int cur,
     HIP_TRY(hipMemcpyAsync(ax, c->axis, keep_n, hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(off, c->offsets, sizeof(int64_t) * keep_nb, hipMemcpyDeviceToDevice,
                            c->stream));
+    // and the group bases of a two-level finalisation (behind the block offsets)
+    const int64_t keep_ng = (keep_nb + kWave - 1) / kWave;
+    HIP_TRY(hipMemcpyAsync(off + nbmax, c->offsets + block_capacity(c->cap), sizeof(int64_t) * keep_ng,
+                           hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
   }
   ctx_free_lists(c);
@@ -634,6 +647,9 @@ void fill_eval_args(cuhre_context* c, EvalArgs& A, int cur, int64_t n, int dim, 
   A.abssum = c->abssum;
   A.partials = c->partials;
   A.offsets = c->offsets;
+  // the group sums of the two-level finalisation follow the block partials
+  A.gpart = c->partials + block_capacity(c->cap);
+  A.gbase = c->offsets + block_capacity(c->cap);
   A.state = c->state;
   (void)dim;
 }
@@ -659,6 +675,7 @@ void fill_split_args(cuhre_context* c, SplitArgs& S, int cur, int64_t n, int dim
   S.state = use_state ? c->state : nullptr;
   S.retire = nullptr;
   S.retire_on = nullptr;
+  S.gbase = nullptr;
 }
 
 void dd_add_host(double& s1, double& c1, double s2, double c2)
@@ -853,6 +870,7 @@ int plain_loop(cuhre_context* c, const cuhre_integrand* f, int dim, double epsre
     SplitArgs SA;
     fill_split_args(c, SA, cur, n, dim, epsrel, epsabs, seg_regions_per_block(S), true);
     const int64_t nb = blocks_for(n, S);
+    if (nb > (int64_t)kWaveFinalize) SA.gbase = A.gbase;
     if (nb > 0x7fffffffll) return fail(CUHRE_ERR_CAPACITY, "grid too large");
 
     // without profiling the state comes back through coherent host memory,
@@ -898,6 +916,7 @@ int plain_loop(cuhre_context* c, const cuhre_integrand* f, int dim, double epsre
       const int64_t ncap = std::min<int64_t>(capacity, std::max<int64_t>(A2, 2 * c->cap));
       if ((rc = ctx_reserve(c, ncap, dim, n, cur, nb))) return rc;
       fill_split_args(c, SA, cur, n, dim, epsrel, epsabs, seg_regions_per_block(S), false);
+      if (nb > (int64_t)kWaveFinalize) SA.gbase = c->offsets + block_capacity(c->cap);
       if (c->profile) HIP_TRY(hipEventRecord(c->ev[1], c->stream));
       hipLaunchKernelGGL(k_split, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, c->stream, SA);
       if (c->profile) HIP_TRY(hipEventRecord(c->ev[2], c->stream));

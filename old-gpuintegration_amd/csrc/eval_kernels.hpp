@@ -270,44 +270,9 @@ __global__ __launch_bounds__(1024) void k_eval_seg(EvalArgs A, DevIntegrand P, R
   }
 }
 
-// Half-wavefront form (S = kHalfS): 8 regions per 256-thread block.
+// Half-wavefront form (S = kHalfS), defined after the degree-9 evaluator.
 template <int D, int FAM, int KEY>
-__global__ __launch_bounds__(256) void k_eval_half(EvalArgs A, DevIntegrand P, RuleConst R)
-{
-  __shared__ Partial ldsw[kBlock / kWave];
-  const int lane = threadIdx.x & (kWave - 1);
-  const int s = threadIdx.x & (kHalfS - 1);
-  const int64_t r = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kHalfS;
-  const bool valid = r < A.n;
-  const int64_t rr = valid ? r : A.n - 1;
-  const PlainIntegrand<D, FAM> f{P};
-  RegionOut o;
-  if constexpr (KEY == 9)
-    o = seg_eval9<D, true>(f, A.lower, A.width, A.ld, r, valid, rr, kHalfS, s, 0, lane, 0, nullptr, R.r9, A.dvec);
-  else
-    o = seg_eval<D, true>(f, A.lower, A.width, A.ld, r, valid, rr, kHalfS, s, 0, lane, 0, nullptr, R, A.dvec);
-  const bool owner = s == 0;
-  if (valid && owner) {
-    A.est[r] = o.est;
-    A.err[r] = o.err;
-    A.axis[r] = (int8_t)o.axis;
-    if (A.abssum) A.abssum[r] = o.ab;
-    if (A.i5) A.i5[r] = o.i5;
-  }
-  if (A.finalize) {
-    Partial p;
-    partial_from_region(p, o.est, o.err, o.ab, valid && owner, A.loc_epsrel, A.loc_epsabs);
-    block_reduce_partial(p, ldsw, kBlock / kWave);
-    store_block_partial(A, p);
-  }
-}
-
-template <int D, int FAM, int KEY>
-void launch_half(const EvalArgs& A, const DevIntegrand& P, const RuleConst& R, hipStream_t st)
-{
-  const int64_t nb = (A.n + kBlock / kHalfS - 1) / (kBlock / kHalfS);
-  hipLaunchKernelGGL((k_eval_half<D, FAM, KEY>), dim3((unsigned)nb), dim3(kBlock), 0, st, A, P, R);
-}
+void launch_half(const EvalArgs& A, const DevIntegrand& P, const RuleConst& R, hipStream_t st);
 
 template <int D, int FAM>
 void launch_eval(const EvalArgs& A, const DevIntegrand& P, const RuleConst& R, int S,
@@ -532,6 +497,45 @@ __device__ __forceinline__ RegionOut seg_eval9(const F& f, const double* __restr
   o.i5 = 0.0;
   o.axis = ax;
   return o;
+}
+
+// Half-wavefront form (S = kHalfS): 8 regions per 256-thread block.
+template <int D, int FAM, int KEY>
+__global__ __launch_bounds__(256) void k_eval_half(EvalArgs A, DevIntegrand P, RuleConst R)
+{
+  __shared__ Partial ldsw[kBlock / kWave];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int s = threadIdx.x & (kHalfS - 1);
+  const int64_t r = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kHalfS;
+  const bool valid = r < A.n;
+  const int64_t rr = valid ? r : A.n - 1;
+  const PlainIntegrand<D, FAM> f{P};
+  RegionOut o;
+  if constexpr (KEY == 9)
+    o = seg_eval9<D, true>(f, A.lower, A.width, A.ld, r, valid, rr, kHalfS, s, 0, lane, 0, nullptr, R.r9, A.dvec);
+  else
+    o = seg_eval<D, true>(f, A.lower, A.width, A.ld, r, valid, rr, kHalfS, s, 0, lane, 0, nullptr, R, A.dvec);
+  const bool owner = s == 0;
+  if (valid && owner) {
+    A.est[r] = o.est;
+    A.err[r] = o.err;
+    A.axis[r] = (int8_t)o.axis;
+    if (A.abssum) A.abssum[r] = o.ab;
+    if (A.i5) A.i5[r] = o.i5;
+  }
+  if (A.finalize) {
+    Partial p;
+    partial_from_region(p, o.est, o.err, o.ab, valid && owner, A.loc_epsrel, A.loc_epsabs);
+    block_reduce_partial(p, ldsw, kBlock / kWave);
+    store_block_partial(A, p);
+  }
+}
+
+template <int D, int FAM, int KEY>
+void launch_half(const EvalArgs& A, const DevIntegrand& P, const RuleConst& R, hipStream_t st)
+{
+  const int64_t nb = (A.n + kBlock / kHalfS - 1) / (kBlock / kHalfS);
+  hipLaunchKernelGGL((k_eval_half<D, FAM, KEY>), dim3((unsigned)nb), dim3(kBlock), 0, st, A, P, R);
 }
 
 template <int D, int FAM>

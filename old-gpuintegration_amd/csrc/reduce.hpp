@@ -171,7 +171,7 @@ __device__ __forceinline__ void write_iteration_state(const EvalArgs& A, const P
 }
 
 // Launches up to this many blocks are finalised by one wavefront, larger
-// ones by four.
+// ones in groups of 64 blocks.
 constexpr uint32_t kWaveFinalize = 4 * kWave;
 
 // Called by every evaluation block after its thread 0 holds the block's
@@ -181,23 +181,23 @@ __device__ __forceinline__ void store_block_partial(const EvalArgs& A, const Par
   if (threadIdx.x == 0) A.partials[blockIdx.x] = p;
 }
 
-// Body of k_finish: one block of W <= 16 wavefronts reduces the nb block
-// partials of the evaluation launch before it (stream order makes them
-// visible).  Each thread sums a contiguous chunk of blocks in order; each
-// wave scans its active counts and reduces its partials with shuffles;
-// one barrier hands the wave totals over, wave 0 reduces them.
-__device__ __forceinline__ void finish_iteration(const EvalArgs& A, uint32_t nb)
+// Reduces the n partials src[0..n) with one block of W <= 16 wavefronts:
+// dst[b] = exclusive scan of the active counts, the total lands in thread 0.
+// Each thread sums a contiguous chunk in order; each wave scans its counts
+// and reduces its partials with shuffles; one barrier hands the wave totals
+// over and wave 0 reduces them.  Every thread returns (callers may barrier).
+__device__ __forceinline__ void reduce_partials(const Partial* src, uint32_t n, int64_t* dst, Partial& total)
 {
   __shared__ Partial ldsw[kMaxWaves];
   __shared__ int64_t s_cnt[kMaxWaves];
   const uint32_t T = blockDim.x, W = T / kWave;
   const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-  const uint32_t chunk = (nb + T - 1) / T;
-  const uint32_t b0 = threadIdx.x * chunk < nb ? threadIdx.x * chunk : nb;
-  const uint32_t b1 = b0 + chunk < nb ? b0 + chunk : nb;
+  const uint32_t chunk = (n + T - 1) / T;
+  const uint32_t b0 = threadIdx.x * chunk < n ? threadIdx.x * chunk : n;
+  const uint32_t b1 = b0 + chunk < n ? b0 + chunk : n;
   Partial q;
   partial_zero(q);
-  for (uint32_t b = b0; b < b1; ++b) partial_add(q, A.partials[b]);
+  for (uint32_t b = b0; b < b1; ++b) partial_add(q, src[b]);
   const int64_t mine = q.nactive;
   long long incl = (long long)mine;
 #pragma unroll
@@ -213,15 +213,59 @@ __device__ __forceinline__ void finish_iteration(const EvalArgs& A, uint32_t nb)
   int64_t run = (int64_t)incl - mine;
   for (uint32_t w = 0; w < wave; ++w) run += s_cnt[w];
   for (uint32_t b = b0; b < b1; ++b) {
-    A.offsets[b] = run;
-    run += A.partials[b].nactive;
+    dst[b] = run;
+    run += src[b].nactive;
   }
-  if (wave != 0) return;
-  Partial t;
-  if (lane < W) t = ldsw[lane];
-  else partial_zero(t);
-  for (uint32_t mask = 1; mask < W; mask <<= 1) partial_shfl_add(t, (int)mask);
-  if (lane == 0) write_iteration_state(A, t);
+  if (wave == 0) {
+    Partial t;
+    if (lane < W) t = ldsw[lane];
+    else partial_zero(t);
+    for (uint32_t mask = 1; mask < W; mask <<= 1) partial_shfl_add(t, (int)mask);
+    total = t;
+  }
+}
+
+// Body of k_finish (launches of at most kWaveFinalize blocks, one
+// wavefront): the block partials straight to the state.
+__device__ __forceinline__ void finish_iteration(const EvalArgs& A, uint32_t nb)
+{
+  Partial q;
+  reduce_partials(A.partials, nb, A.offsets, q);
+  if (threadIdx.x == 0) write_iteration_state(A, q);
+}
+
+// Larger launches, first level (k_finish_groups, one wavefront per group
+// of 64 blocks): the group's sum and the offsets of its blocks within it.
+// One block reading all partials of a large launch took ~40 us at 8k blocks
+// (its load stream is serial per thread); in groups they are read in
+// parallel.
+__device__ __forceinline__ void finish_groups(const EvalArgs& A, uint32_t nb)
+{
+  const uint32_t lane = threadIdx.x;
+  const uint32_t b = blockIdx.x * kWave + lane;
+  Partial q;
+  if (b < nb) q = A.partials[b];
+  else partial_zero(q);
+  const int64_t mine = q.nactive;
+  long long incl = (long long)mine;
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const long long v = __shfl_up(incl, off, kWave);
+    if ((int)lane >= off) incl += v;
+  }
+  if (b < nb) A.offsets[b] = (int64_t)incl - mine;
+#pragma unroll
+  for (int mask = 1; mask < kWave; mask <<= 1) partial_shfl_add(q, mask);
+  if (lane == 0) A.gpart[blockIdx.x] = q;
+}
+
+// Second level (k_finish_top, one block): the G group sums to the state
+// and the groups' base offsets (the split adds gbase[b/64] to offsets[b]).
+__device__ __forceinline__ void finish_top(const EvalArgs& A, uint32_t G)
+{
+  Partial q;
+  reduce_partials(A.gpart, G, A.gbase, q);
+  if (threadIdx.x == 0) write_iteration_state(A, q);
 }
 
 // Queues k_finish for an evaluation launch of nb blocks.
