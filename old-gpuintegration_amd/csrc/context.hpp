@@ -46,7 +46,7 @@ struct cuhre_context {
   int profile = 0;
   int segments = 0;                // 0: automatic, else 1, 2, 4, 8 or 16
   int64_t fill_lanes = 262144;     // auto: smallest S with n*S >= fill_lanes
-  int64_t half_max_regions = 4096;  // auto: half-wavefront form up to this many regions
+  int64_t half_max_regions = 8192;  // auto: half-wavefront form up to this many regions
 
   int64_t cap = 0;  // allocated regions per list buffer
   int dim_alloc = 0;
