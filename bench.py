@@ -26,9 +26,11 @@ sys.path.insert(0, os.path.join(ROOT, "old-gpuintegration_amd"))
 import workloads as W  # noqa: E402
 
 FP64_PEAK_TFLOPS = 78.6
-# measured HBM traffic per launch of k_eval_seg<8, product_peak> on this workload
-# (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, calibrated on a copy kernel)
-TRAFFIC_PER_LAUNCH = 2 * 10175.49 * 1024 + 4141.44 * 1024
+# measured HBM traffic per rule-evaluation launch of the 8-D product peak on
+# this workload (k_eval_half for lists up to 8192 regions, k_eval_seg above;
+# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE averaged over the 13 launches of a
+# run, FETCH_SIZE scaled x2 and WRITE_SIZE x1 as a 32 MiB copy kernel reads)
+TRAFFIC_PER_LAUNCH = 2 * 10144.6 * 1024 + 4080.8 * 1024
 
 
 def flop_costs():
@@ -272,7 +274,8 @@ def main():
                      "traffic_note": "HBM bytes per launch of this kernel on this workload, FETCH_SIZE x2 + "
                                      "WRITE_SIZE x1 (copy-kernel calibration), profiles/eval_seg_product_peak8d.md; "
                                      "algorithmic 153 B/region x 159,134 regions = 24.3 MB",
-                     "kernel": "k_eval_seg<8, product_peak>",
+                     "kernel": "rule evaluation of product_peak8d: k_eval_half<8, product_peak, 7> (lists up "
+                               "to 8192 regions) + k_eval_seg<8, product_peak>, every launch",
                      "launches": ev_launches, "avg_launch_ms": avg_ms,
                      "flops_per_region": fl, "regions_per_launch": ev_regions / ev_launches,
                      "largest_launch": {"regions": big["largest_regions"], "ms": big["ms_eval_largest"],

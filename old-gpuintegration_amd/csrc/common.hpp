@@ -98,6 +98,10 @@ struct EvalArgs {
   uint32_t seq;        // published with the state when host_state is set
   IterState* host_state;  // optional: coherent host copy of *state
   uint32_t* host_seq;     // written (system-scope release) after host_state
+  // host side only: when set, the evaluation kernel's own start and end are
+  // recorded in these events (hipExtLaunchKernel), so a timed launch does
+  // not include the gap before it reaches the GPU
+  hipEvent_t prof_start, prof_stop;
 };
 
 // Launches the rule evaluation of A.n regions with S point segments.

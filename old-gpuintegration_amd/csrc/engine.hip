@@ -884,9 +884,13 @@ int plain_loop(cuhre_context* c, const cuhre_integrand* f, int dim, double epsre
       A.host_state = c->pub_state_dev;
       A.host_seq = c->pub_seq_dev;
     }
-    if (c->profile) HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+    if (c->profile) {
+      // the kernel's own start and end (no launch gap in the interval)
+      A.prof_start = c->ev[0];
+      A.prof_stop = c->ev[1];
+    }
     launch(A, P, R, S, c->stream);
-    if (c->profile) HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+    A.prof_start = A.prof_stop = nullptr;
     launch_finish(A, nb, c->stream);
     hipLaunchKernelGGL(k_split, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, c->stream, SA);
     if (c->profile) HIP_TRY(hipEventRecord(c->ev[2], c->stream));

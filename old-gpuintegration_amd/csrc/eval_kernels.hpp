@@ -26,6 +26,8 @@
 // and the per-block partial sums; k_finish (reduce.hpp) reduces them.
 #pragma once
 
+#include <hip/hip_ext.h>
+
 #include "integrands.hpp"
 #include "reduce.hpp"
 
@@ -281,8 +283,8 @@ void launch_eval(const EvalArgs& A, const DevIntegrand& P, const RuleConst& R, i
   if (S == kHalfS) return launch_half<D, FAM, 7>(A, P, R, st);
   const int rpb = seg_regions_per_block(S);
   const int64_t nb = (A.n + rpb - 1) / rpb;
-  hipLaunchKernelGGL((k_eval_seg<D, FAM>), dim3((unsigned)nb), dim3(kWave * seg_waves(S)),
-                     seg_lds_bytes(S), st, A, P, R, S);
+  hipExtLaunchKernelGGL((k_eval_seg<D, FAM>), dim3((unsigned)nb), dim3(kWave * seg_waves(S)),
+                        (uint32_t)seg_lds_bytes(S), st, A.prof_start, A.prof_stop, 0, A, P, R, S);
 }
 
 // ---------------------------------------------------------------- degree 9
@@ -535,7 +537,8 @@ template <int D, int FAM, int KEY>
 void launch_half(const EvalArgs& A, const DevIntegrand& P, const RuleConst& R, hipStream_t st)
 {
   const int64_t nb = (A.n + kBlock / kHalfS - 1) / (kBlock / kHalfS);
-  hipLaunchKernelGGL((k_eval_half<D, FAM, KEY>), dim3((unsigned)nb), dim3(kBlock), 0, st, A, P, R);
+  hipExtLaunchKernelGGL((k_eval_half<D, FAM, KEY>), dim3((unsigned)nb), dim3(kBlock), 0, st, A.prof_start,
+                        A.prof_stop, 0, A, P, R);
 }
 
 template <int D, int FAM>
@@ -576,8 +579,8 @@ void launch_eval9(const EvalArgs& A, const DevIntegrand& P, const RuleConst& R, 
   if (S > 8) S = 8;  // LDS of the merge: 7 doubles + 1 int per lane and wave
   const int rpb = seg_regions_per_block(S);
   const int64_t nb = (A.n + rpb - 1) / rpb;
-  hipLaunchKernelGGL((k_eval9_seg<D, FAM>), dim3((unsigned)nb), dim3(kWave * seg_waves(S)), seg9_lds_bytes(S), st,
-                     A, P, R, S);
+  hipExtLaunchKernelGGL((k_eval9_seg<D, FAM>), dim3((unsigned)nb), dim3(kWave * seg_waves(S)),
+                        (uint32_t)seg9_lds_bytes(S), st, A.prof_start, A.prof_stop, 0, A, P, R, S);
 }
 
 // Launchers for one family over dimensions 2..10 (key 9: the degree-9 rule).
