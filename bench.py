@@ -191,6 +191,31 @@ def bench_reference(ctx):
     return rows
 
 
+def lclt_tables():
+    with open(os.path.join(ROOT, "tests", "golden", "lc_lt_t_tables.json")) as fh:
+        return json.load(fh)
+
+
+def bench_lclt(ctx):
+    """The reference driver's run (cudaCuhre.cu:43-79): its LC_LT_t integrand
+    over its Volume at epsrel 1e-12 then 1e-7 (epsabs 1e-12), by the
+    reference's method (two phases, degree 9) and by the plain GM7/5 loop,
+    best of 5."""
+    t = lclt_tables()
+    rows = []
+    for eps in (1e-12, 1e-7):
+        p = W.lclt_driver(t, eps)
+        row = {"epsrel": eps}
+        for name, run in (("reference_method", lambda: ctx.integrate_two_phase(p, key=9)),
+                          ("plain_gm75", lambda: ctx.integrate(p)),
+                          ("plain_degree9", lambda: ctx.integrate(p, key=9))):
+            r = min((run() for _ in range(5)), key=lambda r: r["ms_total"])
+            row[name] = {k: r[k] for k in ("rc", "fail", "integral", "error", "nregions", "neval",
+                                           "iterations")} | {"ms": r["ms_total"]}
+        rows.append(row)
+    return rows
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -203,6 +228,8 @@ def main():
                     help="also train the split guide and compare it with the plain method on 7-D corner peaks")
     ap.add_argument("--reference", action="store_true",
                     help="also run the reference's own method (degree-9 rule, two phases) beside the plain one")
+    ap.add_argument("--lclt", action="store_true",
+                    help="also run the reference driver's LC_LT_t integrand (its tables, its Volume, its tolerances)")
     ap.add_argument("--converge", action="store_true",
                     help="also run the product peak with maxeval 1e12 to reach epsrel 1e-6")
     args = ap.parse_args()
@@ -297,6 +324,8 @@ def main():
         out["learned"] = bench_learned(ctx)
     if args.reference:
         out["reference_method"] = bench_reference(ctx)
+    if args.lclt:
+        out["lclt_driver"] = bench_lclt(ctx)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(problems)
         # the CPU leg also holds the closed forms: report each GPU result's
@@ -310,6 +339,22 @@ def main():
                 ex, _ = oracle.exact(p.f, p.lows, p.highs)
                 for k in ("reference_method", "plain_gm75"):
                     row[k]["true_rel_err"] = abs(row[k]["integral"] - ex) / abs(ex)
+        if args.lclt:
+            # the oracle's restatement of the same runs on the host, timed
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import oracle
+            t = lclt_tables()
+            for row in out["lclt_driver"]:
+                p = W.lclt_driver(t, row["epsrel"])
+                t0 = time.perf_counter()
+                o = oracle.integrate_two_phase(p, key=9, nthreads=16)
+                t1 = time.perf_counter()
+                b = oracle.integrate_bfs(p, nthreads=16)
+                t2 = time.perf_counter()
+                row["cpu_oracle"] = {"reference_method": {"integral": o["integral"], "nregions": o["nregions"],
+                                                          "ms": (t1 - t0) * 1e3},
+                                     "plain_gm75": {"integral": b["integral"], "nregions": b["nregions"],
+                                                    "ms": (t2 - t1) * 1e3}, "threads": 16}
     ctx.close()
     print(json.dumps(out))
 

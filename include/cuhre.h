@@ -52,11 +52,24 @@ extern "C" {
 #define CUHRE_POLYNOMIAL 6   /* sum_t coef_t prod_i x_i^powers[t][i]           */
 #define CUHRE_COSINE_SUM 7   /* -(1/2) sum_i cos(10 x_i)/0.054402111088937     */
 #define CUHRE_TABLE6 8       /* B(x1,x2) x3^-1.5 exp(-(x4-.5)^2/.02)(1+x5 x6)  */
+#define CUHRE_LC_LT_T 9      /* the reference's LC_LT_t in x = (lt, zt, lc)     */
 
 /* Integrand description (the C-ABI stand-in for the reference's functor
  * with __device__ double operator()(double x[], int dim)).  The table of
  * CUHRE_TABLE6 is a host pointer to nx*ny doubles, table[i*ny + j] =
- * B(i/(nx-1), j/(ny-1)); the library copies it to the device. */
+ * B(i/(nx-1), j/(ny-1)); the library copies it to the device.
+ *
+ * CUHRE_LC_LT_T (dim 3, lc_lt_t.h:20-50 with GPUQuadInterp2D.cu:49-100):
+ * table points to nx + ny + 5*nx*ny doubles: the lt nodes (nx, the
+ * interpolation rows), the zt nodes (ny, the columns), then the fields
+ * tau, mu, sigma, fmsk, fprj, each field[j*nx + i] = value at (lt_i, zt_j)
+ * (the layout of the reference's tau_arr ...).  table_nx = nx, table_ny =
+ * ny; the nodes must increase strictly.  Every value is rounded to float,
+ * as the reference's tables are float textures, and the fields are
+ * interpolated bilinearly in (lt, zt) in the reference's float/double
+ * arithmetic.  Beyond the last node the last interval is extended (the
+ * reference reads a texel past the end there); below the first node the
+ * first interval is extended, as in the reference. */
 typedef struct {
   int32_t family;
   int32_t dim;

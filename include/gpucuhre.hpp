@@ -26,6 +26,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "cuhre.h"
 
@@ -169,6 +170,30 @@ struct CornerPeak {
     for (int i = 0; i < dim; ++i) d.a[i] = a[i];
     d.exponent = exponent;
   }
+  cuhre_integrand descriptor() const { return d; }
+};
+
+// The reference's LC_LT_t<IntegralGPU> (lc_lt_t.h:7-259): five fields
+// tau, mu, sigma, fmsk, fprj tabulated on float nodes in lt (rows) and zt
+// (columns), each array laid out as the reference's (value at (lt_i, zt_j)
+// at index j*nrows + i); x = (lt, zt, lc).  The constructor takes the
+// arrays the reference's constructor hands to Interp2D::Initialize.
+struct LcLtT {
+  std::vector<double> packed;
+  cuhre_integrand d;
+  LcLtT(const float* lt, int rows, const float* zt, int cols, const float* tau, const float* mu,
+        const float* sigma, const float* fmsk, const float* fprj)
+      : d(blank(CUHRE_LC_LT_T, 3))
+  {
+    packed.assign(lt, lt + rows);
+    packed.insert(packed.end(), zt, zt + cols);
+    for (const float* f : {tau, mu, sigma, fmsk, fprj}) packed.insert(packed.end(), f, f + (size_t)rows * cols);
+    d.table = packed.data();
+    d.table_nx = rows;
+    d.table_ny = cols;
+  }
+  LcLtT(const LcLtT&) = delete;
+  LcLtT& operator=(const LcLtT&) = delete;
   cuhre_integrand descriptor() const { return d; }
 };
 

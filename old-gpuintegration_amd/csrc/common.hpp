@@ -28,6 +28,7 @@ struct DevIntegrand {
   double coef[CUHRE_MAXTERMS];
   int8_t powers[CUHRE_MAXTERMS][kMaxDim];
   const double* table;  // device pointer (TABLE6)
+  const float* ftab;    // device pointer (LC_LT_T): nodes, then five fields
   int32_t nx, ny;
 };
 

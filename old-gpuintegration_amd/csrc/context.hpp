@@ -76,6 +76,8 @@ struct cuhre_context {
 
   double* table = nullptr;
   std::vector<double> table_host;
+  float* ftable = nullptr;  // LC_LT_t nodes and fields (float, as the reference's textures)
+  std::vector<float> ftable_host;
 
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   cuhre::BatchEngine* batch = nullptr;

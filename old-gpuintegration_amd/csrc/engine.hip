@@ -38,6 +38,7 @@ EvalLauncher launcher_continuous(int, int);
 EvalLauncher launcher_polynomial(int, int);
 EvalLauncher launcher_cosine_sum(int, int);
 EvalLauncher launcher_table6(int, int);
+EvalLauncher launcher_lclt(int, int);
 
 Phase2Launcher p2launcher_gaussian(int, int);
 Phase2Launcher p2launcher_product_peak(int, int);
@@ -47,6 +48,7 @@ Phase2Launcher p2launcher_continuous(int, int);
 Phase2Launcher p2launcher_polynomial(int, int);
 Phase2Launcher p2launcher_cosine_sum(int, int);
 Phase2Launcher p2launcher_table6(int, int);
+Phase2Launcher p2launcher_lclt(int, int);
 
 Phase2Launcher find_phase2_launcher(int family, int dim, int key)
 {
@@ -61,6 +63,7 @@ Phase2Launcher find_phase2_launcher(int family, int dim, int key)
   case CUHRE_POLYNOMIAL: return p2launcher_polynomial(dim, key);
   case CUHRE_COSINE_SUM: return p2launcher_cosine_sum(dim, key);
   case CUHRE_TABLE6: return p2launcher_table6(dim, key);
+  case CUHRE_LC_LT_T: return p2launcher_lclt(dim, key);
   default: return nullptr;
   }
 }
@@ -78,6 +81,7 @@ EvalLauncher find_eval_launcher(int family, int dim, int key)
   case CUHRE_POLYNOMIAL: return launcher_polynomial(dim, key);
   case CUHRE_COSINE_SUM: return launcher_cosine_sum(dim, key);
   case CUHRE_TABLE6: return launcher_table6(dim, key);
+  case CUHRE_LC_LT_T: return launcher_lclt(dim, key);
   default: return nullptr;
   }
 }
@@ -211,7 +215,7 @@ RuleConst make_rule(int d, int key)
 // FP64 flops per call (add/sub/mul 1, fma 2, transcendental estimate 1) of
 // the device math each family uses, counted with the SQ_INSTS_VALU_*_F64
 // counters on one-operation kernels (flopcost.cpp); recip is recip_dev.
-struct FlopCosts { double div = 14, exp = 27, cos = 57, sqrt = 17, recip = 9; };
+struct FlopCosts { double div = 14, exp = 27, cos = 57, sqrt = 17, recip = 9, erfc = 115; };
 static FlopCosts g_costs;
 
 // multiplies of b^e: ipow_const for Genz's exponent d + 1, else the loop
@@ -234,6 +238,9 @@ double family_flops(const cuhre_integrand* f, int d)
   case CUHRE_CONTINUOUS: return 4.0 * d + g_costs.exp;
   case CUHRE_COSINE_SUM: return d * (2.0 + g_costs.cos + g_costs.div) + 1;
   case CUHRE_TABLE6: return 20.0 + g_costs.exp + g_costs.sqrt + 2 * g_costs.div;
+  // five bilinear fields (8 flops each, 2 interpolation weights), the closed
+  // form's ~40 flops, 3 exp, 4 erfc and 11 divisions
+  case CUHRE_LC_LT_T: return 84.0 + 3 * g_costs.exp + 4 * g_costs.erfc + 11 * g_costs.div;
   default: return 4.0 * d;
   }
 }
@@ -557,6 +564,35 @@ int make_dev_integrand(cuhre_context* c, const cuhre_integrand* f, int dim, DevI
     P->nx = f->table_nx;
     P->ny = f->table_ny;
   }
+  if (f->family == CUHRE_LC_LT_T) {
+    const int nx = f->table_nx, ny = f->table_ny;
+    if (dim != 3) return fail(CUHRE_ERR_ARG, "the LC_LT_t integrand has 3 dimensions (lt, zt, lc)");
+    if (!f->table || nx < 2 || ny < 2 || nx > 4096 || ny > 4096)
+      return fail(CUHRE_ERR_ARG, "LC_LT_t needs node arrays of 2..4096 entries and five fields");
+    const size_t cnt = (size_t)nx + ny + 5 * (size_t)nx * ny;
+    std::vector<float> t32(cnt);
+    for (size_t k = 0; k < cnt; ++k) {
+      t32[k] = (float)f->table[k];
+      if (!std::isfinite(t32[k])) return fail(CUHRE_ERR_ARG, "LC_LT_t table value %zu is not finite", k);
+    }
+    for (int i = 1; i < nx; ++i)
+      if (!(t32[i] > t32[i - 1])) return fail(CUHRE_ERR_ARG, "LC_LT_t lt nodes must increase strictly");
+    for (int j = 1; j < ny; ++j)
+      if (!(t32[nx + j] > t32[nx + j - 1])) return fail(CUHRE_ERR_ARG, "LC_LT_t zt nodes must increase strictly");
+    const bool same = c->ftable && c->ftable_host.size() == cnt &&
+                      std::memcmp(c->ftable_host.data(), t32.data(), cnt * sizeof(float)) == 0;
+    if (!same) {
+      if (c->ftable) (void)hipFree(c->ftable);
+      c->ftable = nullptr;
+      c->ftable_host = t32;
+      HIP_TRY(hipMalloc(&c->ftable, cnt * sizeof(float)));
+      HIP_TRY(hipMemcpyAsync(c->ftable, c->ftable_host.data(), cnt * sizeof(float), hipMemcpyHostToDevice,
+                             c->stream));
+    }
+    P->ftab = c->ftable;
+    P->nx = nx;
+    P->ny = ny;
+  }
   return 0;
 }
 
@@ -755,6 +791,7 @@ int cuhre_context_destroy(cuhre_context* c)
   if (c->sum_part) (void)hipFree(c->sum_part);
   if (c->sum_out) (void)hipFree(c->sum_out);
   if (c->table) (void)hipFree(c->table);
+  if (c->ftable) (void)hipFree(c->ftable);
   for (int i = 0; i < 4; ++i)
     if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
   if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -173,4 +173,70 @@ struct Family<CUHRE_TABLE6, D> {
   }
 };
 
+// The reference's LC_LT_t integrand (lc_lt_t.h:20-50) in x = (lt, zt, lc).
+// Five fields tau, mu, sigma, fmsk, fprj are interpolated bilinearly in
+// (lt, zt) on float tables in the arithmetic of the reference's Interp2D
+// (GPUQuadInterp2D.cu:49-100): float nodes and values, double weights, the
+// result rounded to float.  The node interval is the reference's
+// SmallNearestN (the last node below the coordinate, the first interval
+// below the grid), kept inside the grid above its last node, where the
+// reference reads one texel past the end.  The closed form follows in
+// double with the reference's float sub-expressions kept float.
+// Contraction is off so the FP64 operations are the ones written.
+__device__ __forceinline__ int lclt_interval(const float* g, int n, double t)
+{
+  int k = 0;
+  for (int i = 1; i < n; ++i) k += (double)g[i] < t ? 1 : 0;
+  return k > n - 2 ? n - 2 : k;
+}
+
+__device__ __forceinline__ float lclt_field(const float* F, int nx, int ix, int iy, double wy1, double wy2,
+                                            double t1, double t2)
+{
+#pragma clang fp contract(off)
+  const float q11 = F[ix * nx + iy], q12 = F[ix * nx + iy + 1];
+  const float q21 = F[(ix + 1) * nx + iy], q22 = F[(ix + 1) * nx + iy + 1];
+  return (float)((q11 * wy2 + q12 * wy1) * t1 + (q21 * wy2 + q22 * wy1) * t2);
+}
+
+__device__ __forceinline__ double lclt_gaussian(double x, double mu, double sigma)
+{
+#pragma clang fp contract(off)
+  const double z = (x - mu) / sigma;
+  return exp(-z * z / 2.) * 0.3989422804014327 / sigma;
+}
+
+template <int D>
+struct Family<CUHRE_LC_LT_T, D> {
+  __device__ __forceinline__ static double eval(const DevIntegrand& P, const double (&x)[D])
+  {
+#pragma clang fp contract(off)
+    const float* T = P.ftab;
+    const int nx = P.nx, ny = P.ny;
+    const double lt = x[0], zt = x[1], lc = x[2];
+    const int iy = lclt_interval(T, nx, lt);
+    const int ix = lclt_interval(T + nx, ny, zt);
+    const float y1 = T[iy], y2 = T[iy + 1];
+    const float x1 = T[nx + ix], x2 = T[nx + ix + 1];
+    const float den = (x2 - x1) * (y2 - y1);
+    const double t1 = (x2 - zt) / den, t2 = (zt - x1) / den;
+    const double wy2 = y2 - lt, wy1 = lt - y1;
+    const float* F = T + nx + ny;
+    const int fs = nx * ny;
+    const float tau = lclt_field(F, nx, ix, iy, wy1, wy2, t1, t2);
+    const float mu = lclt_field(F + fs, nx, ix, iy, wy1, wy2, t1, t2);
+    const float sigma = lclt_field(F + 2 * fs, nx, ix, iy, wy1, wy2, t1, t2);
+    const float fmsk = lclt_field(F + 3 * fs, nx, ix, iy, wy1, wy2, t1, t2);
+    const float fprj = lclt_field(F + 4 * fs, nx, ix, iy, wy1, wy2, t1, t2);
+    const double exptau = exp(tau * (2.0 * mu + tau * sigma * sigma - 2.0 * lc) / 2.0);
+    const double root_two_sigma = sqrt(2.0) * sigma;
+    const double mu_tau_sig_sqr = mu + tau * sigma * sigma;
+    return (1.0 - fmsk) * (1.0 - fprj) * lclt_gaussian(lc, mu, sigma) +
+           0.5 * ((1.0 - fmsk) * fprj * tau + fmsk * fprj / lt) * exptau *
+               erfc((mu_tau_sig_sqr - lc) / root_two_sigma) +
+           0.5 * fmsk / lt * (erfc((lc - mu) / root_two_sigma) - erfc((lc + lt - mu) / root_two_sigma)) -
+           0.5 * fmsk * fprj / lt * (exp(-tau * lt) * exptau * erfc((mu_tau_sig_sqr - (lc + lt)) / root_two_sigma));
+  }
+};
+
 }  // namespace cuhre

@@ -249,7 +249,7 @@ def to_struct(f: W.Integrand):
     if f.table is not None:
         keep = np.ascontiguousarray(f.table, dtype=np.float64)
         s.table = _dp(keep)
-        s.table_nx, s.table_ny = keep.shape
+        s.table_nx, s.table_ny = f.table_shape if f.table_shape is not None else keep.shape
     return s, keep
 
 

@@ -24,6 +24,7 @@ CONTINUOUS = 5
 POLYNOMIAL = 6
 COSINE_SUM = 7
 TABLE6 = 8
+LC_LT_T = 9
 
 FAMILY_NAMES = {
     GAUSSIAN: "gaussian",
@@ -34,6 +35,7 @@ FAMILY_NAMES = {
     POLYNOMIAL: "polynomial",
     COSINE_SUM: "cosine_sum",
     TABLE6: "table6",
+    LC_LT_T: "lc_lt_t",
 }
 
 MAXDIM = 16
@@ -83,6 +85,7 @@ class Integrand:
     coef: List[float] = field(default_factory=list)
     powers: List[List[int]] = field(default_factory=list)
     table: Optional[np.ndarray] = None  # (nx, ny) float64, table[i, j] = B(i/(nx-1), j/(ny-1))
+    table_shape: Optional[tuple] = None  # (nx, ny) when `table` is a packed 1-D array (LC_LT_T)
 
     @property
     def name(self) -> str:
@@ -140,6 +143,27 @@ def cosine_sum(d) -> Integrand:
 def table6(table: np.ndarray) -> Integrand:
     return Integrand(TABLE6, 6, [0.0] * 6, [0.0] * 6,
                      table=np.ascontiguousarray(table, dtype=np.float64))
+
+
+def lc_lt_t(tables: dict) -> Integrand:
+    """The reference's LC_LT_t integrand (lc_lt_t.h:20-50) in x = (lt, zt, lc):
+    tables holds the lt and zt nodes and the fields tau, mu, sigma, fmsk, fprj
+    (field[j*nx + i] = value at (lt_i, zt_j)), packed as the C-ABI expects."""
+    lt = np.asarray(tables["lt"], dtype=np.float64)
+    zt = np.asarray(tables["zt"], dtype=np.float64)
+    parts = [lt, zt] + [np.asarray(tables[k], dtype=np.float64) for k in ("tau", "mu", "sigma", "fmsk", "fprj")]
+    for k, a in zip(("tau", "mu", "sigma", "fmsk", "fprj"), parts[2:]):
+        assert a.size == lt.size * zt.size, k
+    return Integrand(LC_LT_T, 3, [0.0] * 3, [0.0] * 3, table=np.concatenate(parts),
+                     table_shape=(lt.size, zt.size))
+
+
+def lclt_driver(tables: dict, epsrel: float = 1e-7) -> Problem:
+    """The reference driver's run (cudaCuhre.cu:43-79): LC_LT_t over the
+    Volume lows (1, .1, 0), highs (2, .3, 1) given as floats, epsabs 1e-12."""
+    f32 = lambda v: float(np.float32(v))  # noqa: E731
+    return Problem("lc_lt_t3", lc_lt_t(tables), [f32(1.0), f32(0.1), f32(0.0)], [f32(2.0), f32(0.3), f32(1.0)],
+                   epsrel, 1e-12, 10**9)
 
 
 # ---------------------------------------------------------------- workloads

@@ -105,6 +105,64 @@ static double ipow(double b, int e)
   return r;
 }
 
+/* The reference's LC_LT_t (lc_lt_t.h:20-50).  Interp2D (GPUQuadInterp2D.cu:
+ * 49-100) on float tables: SmallNearestN picks the last node below the
+ * coordinate (index 0 below the grid; the interval is kept inside the grid
+ * above the last node, where the reference reads past the end), the nodes
+ * and values stay float, the weights are double, the result is rounded to
+ * float.  The closed form keeps the reference's float sub-expressions
+ * (tau * sigma * sigma, mu + ..., fmsk * fprj) in float: this file is built
+ * with -ffp-contract=off and FLT_EVAL_METHOD 0, so C evaluates them as the
+ * reference's CUDA does. */
+static int lclt_interval(const double* g, int n, double t)
+{
+  int k = 0;
+  for (int i = 1; i < n; ++i) k += (double)(float)g[i] < t ? 1 : 0;
+  return k > n - 2 ? n - 2 : k;
+}
+
+static float lclt_field(const double* F, int nx, int ix, int iy, double wy1, double wy2, double t1, double t2)
+{
+  const float q11 = (float)F[ix * nx + iy], q12 = (float)F[ix * nx + iy + 1];
+  const float q21 = (float)F[(ix + 1) * nx + iy], q22 = (float)F[(ix + 1) * nx + iy + 1];
+  return (float)((q11 * wy2 + q12 * wy1) * t1 + (q21 * wy2 + q22 * wy1) * t2);
+}
+
+static double lclt_gaussian(double x, double mu, double sigma)
+{
+  const double z = (x - mu) / sigma;
+  return exp(-z * z / 2.) * 0.3989422804014327 / sigma;
+}
+
+static double lclt_eval(const or_integrand* f, const double* x)
+{
+  const double* T = f->table;
+  const int nx = f->table_nx, ny = f->table_ny;
+  const double lt = x[0], zt = x[1], lc = x[2];
+  const int iy = lclt_interval(T, nx, lt);
+  const int ix = lclt_interval(T + nx, ny, zt);
+  const float y1 = (float)T[iy], y2 = (float)T[iy + 1];
+  const float x1 = (float)T[nx + ix], x2 = (float)T[nx + ix + 1];
+  const float den = (x2 - x1) * (y2 - y1);
+  const double t1 = (x2 - zt) / den, t2 = (zt - x1) / den;
+  const double wy2 = y2 - lt, wy1 = lt - y1;
+  const double* F = T + nx + ny;
+  const int fs = nx * ny;
+  const float tau = lclt_field(F, nx, ix, iy, wy1, wy2, t1, t2);
+  const float mu = lclt_field(F + fs, nx, ix, iy, wy1, wy2, t1, t2);
+  const float sigma = lclt_field(F + 2 * fs, nx, ix, iy, wy1, wy2, t1, t2);
+  const float fmsk = lclt_field(F + 3 * fs, nx, ix, iy, wy1, wy2, t1, t2);
+  const float fprj = lclt_field(F + 4 * fs, nx, ix, iy, wy1, wy2, t1, t2);
+  const double exptau = exp(tau * (2.0 * mu + tau * sigma * sigma - 2.0 * lc) / 2.0);
+  const double root_two_sigma = sqrt(2.0) * sigma;
+  const double mu_tau_sig_sqr = mu + tau * sigma * sigma;
+  return (1.0 - fmsk) * (1.0 - fprj) * lclt_gaussian(lc, mu, sigma) +
+         0.5 * ((1.0 - fmsk) * fprj * tau + fmsk * fprj / lt) * exptau *
+             erfc((mu_tau_sig_sqr - lc) / root_two_sigma) +
+         0.5 * fmsk / lt * (erfc((lc - mu) / root_two_sigma) - erfc((lc + lt - mu) / root_two_sigma)) -
+         0.5 * fmsk * fprj / lt * (exp(-tau * lt) * exptau * erfc((mu_tau_sig_sqr - (lc + lt)) / root_two_sigma));
+}
+
 double or_f(const or_integrand* f, const double* x)
 {
   const int d = f->dim;
@@ -156,6 +214,8 @@ double or_f(const or_integrand* f, const double* x)
     for (int i = 0; i < d; ++i) s = s - cos(10.0 * x[i]) / 0.054402111088937;
     return s / 2.0;
   }
+  case OR_LC_LT_T:
+    return lclt_eval(f, x);
   case OR_TABLE6: {
     const int nx = f->table_nx, ny = f->table_ny;
     double fx = x[0] * (double)(nx - 1), fy = x[1] * (double)(ny - 1);

@@ -39,7 +39,8 @@ enum {
   OR_CONTINUOUS = 5,    /* exp(-sum a_i |x_i - w_i|)                       */
   OR_POLYNOMIAL = 6,    /* sum_t c_t prod_i x_i^e_ti                       */
   OR_COSINE_SUM = 7,    /* -(1/2) sum cos(10 x_i)/0.054402111088937        */
-  OR_TABLE6 = 8         /* B(x1,x2) x3^-1.5 exp(-(x4-.5)^2/.02) (1+x5 x6)  */
+  OR_TABLE6 = 8,        /* B(x1,x2) x3^-1.5 exp(-(x4-.5)^2/.02) (1+x5 x6)  */
+  OR_LC_LT_T = 9        /* the reference's LC_LT_t in x = (lt, zt, lc)      */
 };
 
 typedef struct {
@@ -52,7 +53,9 @@ typedef struct {
   int32_t nterms;
   double coef[OR_MAXTERMS];
   int32_t powers[OR_MAXTERMS][OR_MAXDIM];
-  const double* table; /* TABLE6: nx*ny values, table[i*ny + j] = B(i/(nx-1), j/(ny-1)) */
+  const double* table; /* TABLE6: nx*ny values, table[i*ny + j] = B(i/(nx-1), j/(ny-1));
+                          LC_LT_T: lt nodes (nx), zt nodes (ny), then tau, mu, sigma,
+                          fmsk, fprj, each [j*nx + i], all taken as float */
   int32_t table_nx, table_ny;
 } or_integrand;
 

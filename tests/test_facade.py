@@ -71,3 +71,37 @@ def test_facade_driver_two_phase_matches_oracle(tmp_path):
         assert abs(nreg - o["nregions"]) <= 5e-3 * o["nregions"], (nreg, o["nregions"])
         assert abs(integral - o["integral"]) <= max(error, o["error"]) + 1e-12 * abs(o["integral"])
         assert abs(integral - ex) <= max(eps * abs(ex), 10 * error)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("two_phase", [False, True])
+def test_facade_driver_lclt_matches_oracle(tmp_path, two_phase):
+    """The reference driver's actual run: LC_LT_t over its Volume at epsrel
+    1e-12 and 1e-7 (cudaCuhre.cu:43-79), through the facade, against the
+    oracle (breadth-first GM7/5, or the reference's two-phase degree-9
+    method with --two-phase)."""
+    import json
+    import numpy as np
+    with open(os.path.join(ROOT, "tests", "golden", "lc_lt_t_tables.json")) as fh:
+        t = json.load(fh)
+    path = str(tmp_path / "lclt.bin")
+    with open(path, "wb") as fh:
+        np.array([len(t["lt"]), len(t["zt"])], dtype=np.int32).tofile(fh)
+        np.concatenate([np.asarray(t[k], dtype=np.float32) for k in ("lt", "zt", "tau", "mu", "sigma", "fmsk",
+                                                                      "fprj")]).tofile(fh)
+    exe = build_driver(tmp_path)
+    args = [exe, "--lclt", path] + (["--two-phase"] if two_phase else [])
+    out = subprocess.run(args, capture_output=True, text=True, timeout=120, check=True).stdout
+    lines = [l.split("\t") for l in out.strip().splitlines()]
+    assert len(lines) == 2
+    for row, eps in zip(lines, (1e-12, 1e-7)):
+        integral, error, nreg, flag = float(row[2]), float(row[3]), int(row[4]), int(row[6])
+        p = W.lclt_driver(t, eps)
+        o = O.integrate_two_phase(p, key=9) if two_phase else O.integrate_bfs(p, nthreads=8)
+        assert (flag == 0) == (o["rc"] == 0), (flag, o)
+        if two_phase:
+            assert abs(nreg - o["nregions"]) <= 5e-3 * o["nregions"]
+            assert abs(integral - o["integral"]) <= max(error, o["error"])
+        else:
+            assert nreg == o["nregions"]
+            assert abs(integral - o["integral"]) <= 1e-12 * abs(o["integral"])

@@ -387,3 +387,101 @@ def test_two_phase_phase2_converges(key):
     assert tp["rc"] == 0 and tp["fail"] == 0
     assert tp["error"] <= p.epsrel * abs(tp["integral"])
     assert abs(tp["integral"] - ex) <= tp["error"]
+
+
+# ------------------------------------------------------------------ LC_LT_t
+# The reference's only production integrand (lc_lt_t.h:20-50 with the
+# Interp2D of GPUQuadInterp2D.cu:49-100).  Its data tables come from the
+# reference's header through tests/golden/make_lclt_tables.py, which checks
+# them against the FP64 twin tables of integrands/lc_lt_t.cc.
+
+LCLT_PATH = os.path.join(os.path.dirname(__file__), "golden", "lc_lt_t_tables.json")
+
+
+def lclt_tables():
+    with open(LCLT_PATH) as fh:
+        return json.load(fh)
+
+
+def test_lclt_fixture_tables():
+    t = lclt_tables()
+    nx, ny = len(t["lt"]), len(t["zt"])
+    assert (nx, ny) == (22, 5)
+    assert np.all(np.diff(t["lt"]) > 0) and np.all(np.diff(t["zt"]) > 0)
+    for k in ("tau", "mu", "sigma", "fmsk", "fprj"):
+        a = np.asarray(t[k])
+        assert a.size == nx * ny
+        assert np.array_equal(a, a.astype(np.float32).astype(np.float64)), k  # float values
+    assert min(t["sigma"]) > 0 and min(t["tau"]) > 0
+    assert 0 <= min(t["fmsk"]) and max(t["fmsk"]) <= 1 and 0 <= min(t["fprj"]) and max(t["fprj"]) <= 1
+
+
+def _bilinear_tables(t, mu_c, sg_c):
+    """fmsk = fprj = 0 and mu, sigma bilinear in (lt, zt): the integrand is
+    the normal density of lc, and interpolation of a bilinear function is
+    exact (up to the float rounding of nodes, values and result)."""
+    lt = np.asarray(t["lt"], dtype=np.float32).astype(np.float64)
+    zt = np.asarray(t["zt"], dtype=np.float32).astype(np.float64)
+    L, Z = np.meshgrid(lt, zt)  # [j, i]: row j = zt_j, column i = lt_i -> field[j*nx + i]
+    bil = lambda c: c[0] + c[1] * L + c[2] * Z + c[3] * L * Z  # noqa: E731
+    f32 = lambda a: a.astype(np.float32).astype(np.float64).ravel().tolist()  # noqa: E731
+    nx, ny = lt.size, zt.size
+    return dict(t, mu=f32(bil(mu_c)), sigma=f32(bil(sg_c)), fmsk=[0.0] * (nx * ny), fprj=[0.0] * (nx * ny))
+
+
+def test_lclt_interpolation_reduces_to_normal_density():
+    from scipy.stats import norm
+    t = lclt_tables()
+    mu_c, sg_c = (0.3, 0.01, -0.8, 0.05), (0.5, 0.004, 0.6, 0.02)
+    f = W.lc_lt_t(_bilinear_tables(t, mu_c, sg_c))
+    rng = np.random.default_rng(5)
+    # inside the grid, below its first lt node and above its last
+    lt = np.concatenate([rng.uniform(1, 160, 40), rng.uniform(0.2, 1, 5), rng.uniform(160, 200, 5)])
+    zt = np.concatenate([rng.uniform(0.1, 0.3, 40), rng.uniform(0.1, 0.3, 10)])
+    lc = rng.uniform(-1, 3, lt.size)
+    got = O.f_values(f, np.stack([lt, zt, lc], axis=1))
+    mu = mu_c[0] + mu_c[1] * lt + mu_c[2] * zt + mu_c[3] * lt * zt
+    sg = sg_c[0] + sg_c[1] * lt + sg_c[2] * zt + sg_c[3] * lt * zt
+    want = norm.pdf(lc, mu, sg)
+    # mu and sigma pass through float: relative error a few 1e-7, times |dlog f|
+    tol = 2e-6 * (1 + np.abs((lc - mu) / sg) ** 2) * want + 1e-12
+    assert np.all(np.abs(got - want) <= tol), np.max(np.abs(got - want) / tol)
+
+
+def test_lclt_interpolation_axes_are_not_transposed():
+    # a field that depends on lt only, then on zt only: swapping the roles
+    # of the two axes (or of q12 and q21) changes the value
+    from scipy.stats import norm
+    t = lclt_tables()
+    for mu_c in ((0.0, 0.02, 0.0, 0.0), (0.0, 0.0, 3.0, 0.0)):
+        f = W.lc_lt_t(_bilinear_tables(t, mu_c, (1.0, 0.0, 0.0, 0.0)))
+        X = np.array([[50.0, 0.12, 0.4], [7.5, 0.27, 1.1]])
+        mu = mu_c[1] * X[:, 0] + mu_c[2] * X[:, 1]
+        assert np.allclose(O.f_values(f, X), norm.pdf(X[:, 2], mu, 1.0), rtol=1e-6)
+
+
+@pytest.mark.parametrize("lt,zt", [(1.3, 0.13), (1.9, 0.27), (4.0, 0.2), (35.0, 0.11), (120.0, 0.29)])
+def test_lclt_density_integrates_to_one(lt, zt):
+    """For fixed (lt, zt) the integrand is a density in lc: the Gaussian term
+    carries (1-fmsk)(1-fprj), the exponentially modified Gaussian
+    (1-fmsk) fprj, and the fmsk terms (a uniform of width lt convolved
+    with the Gaussian, minus the lt-shifted EMG over tau lt plus the EMG
+    over tau lt) carry fmsk: the total is 1.  A dropped term or a wrong
+    sign moves it by fmsk, fprj or fmsk*fprj/(tau*lt) (each at least 2e-4
+    at these points, with the tables' interpolated values).  The reference forms mu + tau sigma^2 in float, which
+    shifts the EMG terms' argument by a float rounding: the total is 1
+    within ~2e-7, not to double precision.  (Far below the data, lc < -200,
+    exp overflows against an erfc underflow and the formula gives NaN, in
+    the reference as here; the integral starts at -60, where the density
+    is below 1e-40.)"""
+    t = lclt_tables()
+    f = W.lc_lt_t(t)
+    g = lambda c: O.f_values(f, np.array([[lt, zt, c]]))[0]  # noqa: E731
+    edges = np.concatenate([np.linspace(-60.0, lt + 100.0, 200), np.linspace(lt + 100.0, 800.0, 50)[1:]])
+    total = sum(integrate.quad(g, a, b, limit=200, epsabs=1e-15, epsrel=1e-12)[0]
+                for a, b in zip(edges[:-1], edges[1:]))
+    assert abs(total - 1.0) <= 1e-6, total
+    # the terms a mistake would drop are far above that tolerance here
+    fz = lambda k: O.f_values(W.lc_lt_t(dict(t, **{k: [0.0] * len(t[k])})),  # noqa: E731
+                              np.array([[lt, zt, c] for c in (0.5, lt / 2)]))
+    assert np.max(np.abs(fz("fmsk") - O.f_values(f, np.array([[lt, zt, 0.5], [lt, zt, lt / 2]])))) > 1e-5
