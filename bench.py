@@ -311,6 +311,7 @@ def main():
     }
     if args.converge:
         p = W.c2_product_peak8()
+        ctx.integrate(p, maxeval=10**12)  # warm-up: the lists grow to their peak once
         t1 = time.perf_counter()
         r = ctx.integrate(p, maxeval=10**12)
         t2 = time.perf_counter()
@@ -318,6 +319,19 @@ def main():
                                               "evals_per_s": r["neval"] / (t2 - t1), **{k: r[k] for k in (
                                                   "integral", "error", "nregions", "neval", "fail",
                                                   "iterations", "peak_regions")}}
+        # the same certified epsrel with the shared-allotment finish policy of
+        # the guided loop (per-region test err <= epsrel((1-b)|est| + b|I| vrel),
+        # b = 0.5, no network): regions and wall time against the plain run
+        g = cuhre.Guide(ctx, seed=1)
+        g.integrate(p, axis_mode=0, finish_mode=3, budget=0.5, maxeval=10**12)
+        t1 = time.perf_counter()
+        a = g.integrate(p, axis_mode=0, finish_mode=3, budget=0.5, maxeval=10**12)
+        t2 = time.perf_counter()
+        conv = out["product_peak_to_convergence"]
+        conv["shared_allotment"] = {"budget": 0.5, "seconds": t2 - t1, **{k: a[k] for k in (
+            "rc", "integral", "error", "nregions", "neval", "iterations")},
+            "regions_vs_plain": a["nregions"] / conv["nregions"], "time_vs_plain": (t2 - t1) / conv["seconds"]}
+        g.close()
     if args.batched:
         out["batched"] = bench_batched(ctx)
     if args.learned:
