@@ -191,6 +191,34 @@ def bench_reference(ctx):
     return rows
 
 
+SWEEP_SIZES = (256, 4096, 16384, 65536, 262144, 1048576)
+SWEEP_SEGMENTS = (1, 2, 4, 8, 16, 32)
+
+
+def bench_sweep(ctx, reps=3):
+    """Point-segment sweep of the rule evaluation: random 8-D regions of the
+    product peak and the oscillatory integrand, every list size with every
+    segment count (32 = the half-wavefront form, run up to 8192 regions),
+    through cuhre_eval_regions.  The kernel times come from a profiler
+    around this run; the list records the launches in order."""
+    import numpy as np
+    rng = np.random.default_rng(2024)
+    order = []
+    for p in (W.c2_product_peak8(), W.c2_oscillatory8()):
+        for n in SWEEP_SIZES:
+            L = rng.uniform(0.0, 0.5, size=(n, 8))
+            Wd = rng.uniform(0.05, 0.5, size=(n, 8))
+            for S in SWEEP_SEGMENTS:
+                if S == 32 and n > 8192:
+                    continue
+                ctx.set(False, S)
+                for _ in range(reps):
+                    ctx.eval_regions(p.f, L, Wd)
+                order.append({"integrand": p.name, "regions": n, "segments": S, "launches": reps})
+    ctx.set(False, 0)
+    return order
+
+
 def lclt_tables():
     with open(os.path.join(ROOT, "tests", "golden", "lc_lt_t_tables.json")) as fh:
         return json.load(fh)
@@ -228,6 +256,8 @@ def main():
                     help="also train the split guide and compare it with the plain method on 7-D corner peaks")
     ap.add_argument("--reference", action="store_true",
                     help="also run the reference's own method (degree-9 rule, two phases) beside the plain one")
+    ap.add_argument("--sweep", action="store_true",
+                    help="also run the point-segment sweep of the rule evaluation (for a kernel profiler)")
     ap.add_argument("--lclt", action="store_true",
                     help="also run the reference driver's LC_LT_t integrand (its tables, its Volume, its tolerances)")
     ap.add_argument("--converge", action="store_true",
@@ -340,6 +370,8 @@ def main():
         out["reference_method"] = bench_reference(ctx)
     if args.lclt:
         out["lclt_driver"] = bench_lclt(ctx)
+    if args.sweep:
+        out["segment_sweep"] = bench_sweep(ctx)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(problems)
         # the CPU leg also holds the closed forms: report each GPU result's
