@@ -145,10 +145,8 @@ int cuhre_integrate(cuhre_context* ctx, const cuhre_integrand* f, int dim,
  * the parent difference (quad/GPUquad/GPUQuadPhases.cu:498-515).  Returns
  * 0 when the global test holds, else 1 with out->fail = the number of
  * Phase-2 regions that missed their own test (the reference's flag).
- * The pool's keys take 12 bytes of LDS per slot: pool_max + 2 slots must
- * fit one block's LDS (CUHRE_ERR_ARG otherwise; 64 KiB allows 5459).  The
- * Phase-2 workspace ((2 dim + 1) doubles + 1 byte per slot and region) is
- * kept by the context between calls. */
+ * pool_max is at most 2^24.  The Phase-2 workspace ((2 dim + 2) doubles
+ * + 5 bytes per slot and region) is kept by the context between calls. */
 int cuhre_integrate_two_phase(cuhre_context* ctx, const cuhre_integrand* f, int dim,
                               double epsrel, double epsabs, int key,
                               int64_t phase1_regions, int32_t pool_max,

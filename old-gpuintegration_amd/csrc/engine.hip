@@ -1046,16 +1046,7 @@ int cuhre_integrate_two_phase(cuhre_context* c, const cuhre_integrand* f, int di
   if (phase1_regions <= 0) phase1_regions = 1 << 14;  // the reference's FIRST_PHASE_MAXREGIONS
   if (pool_max <= 0) pool_max = 2048;                 // the reference's MAX_GLOBALPOOL_SIZE
   if (!c) return fail(CUHRE_ERR_ARG, "null context");
-  {
-    // the pool's keys live in LDS: pool_max + 2 slots of 12 bytes per region
-    int lds_max = 0;
-    HIP_TRY(hipSetDevice(c->device));
-    HIP_TRY(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
-    const size_t lds = phase2_lds_bytes((int64_t)pool_max + 2);
-    if (pool_max > (1 << 24) || lds > (size_t)lds_max)
-      return fail(CUHRE_ERR_ARG, "pool_max %d needs %zu bytes of LDS per region (at most %d)", pool_max, lds,
-                  lds_max);
-  }
+  if (pool_max > (1 << 24)) return fail(CUHRE_ERR_ARG, "pool_max %d above 2^24", pool_max);
   LoopOpts opt;
   opt.handoff_n = phase1_regions;
   opt.max_iter = 100;  // GPUKernelquad.cu:631
@@ -1076,7 +1067,7 @@ int cuhre_integrate_two_phase(cuhre_context* c, const cuhre_integrand* f, int di
   const int64_t p = R.npts;
   const int64_t n = ho.n;
   const int64_t slots = (int64_t)pool_max + 2;
-  const size_t per_slot = sizeof(double) * (2 * (size_t)dim + 1) + sizeof(int8_t);
+  const size_t per_slot = sizeof(double) * (2 * (size_t)dim + 2) + sizeof(int8_t) + sizeof(int32_t);
   const size_t ws = (size_t)n * (size_t)slots * per_slot;
   const size_t outb = (size_t)n * (2 * sizeof(double) + sizeof(int32_t) + 2);
   const size_t need = ws + outb + 8 * 256;
@@ -1113,6 +1104,8 @@ int cuhre_integrate_two_phase(cuhre_context* c, const cuhre_integrand* f, int di
   A.geo = (double*)take(sizeof(double) * (size_t)n * slots * 2 * dim);
   A.sest = (double*)take(sizeof(double) * (size_t)n * slots);
   A.sax = (int8_t*)take((size_t)n * slots);
+  A.gkerr = (double*)take(sizeof(double) * (size_t)n * slots);
+  A.gkseq = (int32_t*)take(sizeof(int32_t) * (size_t)n * slots);
   A.res = (double*)take(sizeof(double) * n);
   A.err = (double*)take(sizeof(double) * n);
   A.cnt = (int32_t*)take(sizeof(int32_t) * n);

@@ -8,12 +8,15 @@
 // bisection evaluates both children at once: lanes 0-31 the left child and
 // lanes 32-63 the right one, each half taking the rule's point sets round
 // robin (32 segments) and merging by a xor butterfly.  The pool's keys (error
-// and creation number per slot) sit in LDS, so the next region to bisect --
-// the largest error, the earliest created among equal errors, as the
-// reference's max-heap order -- is one strided scan plus a butterfly; the
-// pool's geometry, estimates and axes sit in the region's own slice of
-// global memory ([slot][lower D, width D]), written by lanes 0 and 32 and
-// read back by the whole wave after a barrier.  A region stops when its own
+// and creation number per slot) of the first kP2LdsSlots slots sit in LDS,
+// the rest (rarely used: most regions stop within a few dozen bisections) in
+// the region's slice of global memory, so the next region to bisect -- the
+// largest error, the earliest created among equal errors, as the
+// reference's max-heap order -- is one strided scan plus a butterfly.  The
+// small LDS footprint (3 KB per wavefront) keeps ~20 wavefronts per CU in
+// flight.  The pool's geometry, estimates and axes sit in the region's own
+// slice of global memory ([slot][lower D, width D]), written by lanes 0 and
+// 32 and read back by the whole wave after a barrier.  A region stops when its own
 // sum meets max(epsrel |RESULT|, epsabs) or after pool_max bisections.
 // Per-region run lengths differ by orders of magnitude; with one wave per
 // region the hardware dispatcher balances them.
@@ -30,6 +33,8 @@ struct Phase2Args {
   int64_t n;
   int64_t slots;  // pool slots per region (pool_max + 2)
   double* geo;    // [region][slot][lower D, width D]
+  double* gkerr;  // [region][slot] pool keys of the slots past the LDS ones
+  int32_t* gkseq;
   double* sest;   // [region][slot]
   int8_t* sax;    // [region][slot]
   double epsrel, epsabs;
@@ -41,8 +46,8 @@ struct Phase2Args {
   int8_t* nonfinite;
 };
 
-// LDS bytes of one block: error (double) and creation number (int32) per slot
-inline size_t phase2_lds_bytes(int64_t slots) { return (size_t)slots * (sizeof(double) + sizeof(int32_t)); }
+// Pool slots whose keys (error, creation number) live in LDS.
+constexpr int kP2LdsSlots = 256;
 
 __device__ __forceinline__ double read_lane_d(double v, int l)
 {
@@ -67,9 +72,21 @@ __device__ __forceinline__ RegionOut p2_eval(const PlainIntegrand<D, FAM>& f, co
 template <int D, int FAM, int KEY>
 __global__ __launch_bounds__(64) void k_phase2(Phase2Args A, DevIntegrand P, RuleConst R)
 {
-  extern __shared__ double p2_keys[];  // error[slots], then creation number[slots]
-  double* const kerr = p2_keys;
-  int32_t* const kseq = reinterpret_cast<int32_t*>(p2_keys + A.slots);
+  __shared__ double s_err[kP2LdsSlots];
+  __shared__ int32_t s_seq[kP2LdsSlots];
+  double* const gerr = A.gkerr + blockIdx.x * A.slots;
+  int32_t* const gseq = A.gkseq + blockIdx.x * A.slots;
+  auto key_err = [&](int32_t t) { return t < kP2LdsSlots ? s_err[t] : gerr[t]; };
+  auto key_seq = [&](int32_t t) { return t < kP2LdsSlots ? s_seq[t] : gseq[t]; };
+  auto set_key = [&](int32_t t, double e, int32_t q) {
+    if (t < kP2LdsSlots) {
+      s_err[t] = e;
+      s_seq[t] = q;
+    } else {
+      gerr[t] = e;
+      gseq[t] = q;
+    }
+  };
   const int64_t g = blockIdx.x;  // one block per region of the input list
   const int lane = threadIdx.x;
   const int half = lane >> 5;
@@ -96,8 +113,7 @@ __global__ __launch_bounds__(64) void k_phase2(Phase2Args A, DevIntegrand P, Rul
     }
     sest[0] = e0;
     sax[0] = (int8_t)o.axis;
-    kerr[0] = r0;
-    kseq[0] = 0;
+    set_key(0, r0, 0);
   }
   int32_t count = 1, next = 1;
   double RES = e0, ERR = r0;
@@ -107,8 +123,8 @@ __global__ __launch_bounds__(64) void k_phase2(Phase2Args A, DevIntegrand P, Rul
     double be = -INFINITY;
     int32_t bq = INT32_MAX, bs = -1;
     for (int32_t t = lane; t < count; t += kWave) {
-      const double e = kerr[t];
-      const int32_t q = kseq[t];
+      const double e = key_err(t);
+      const int32_t q = key_seq(t);
       if (e > be || (e == be && q < bq)) {
         be = e;
         bq = q;
@@ -128,7 +144,7 @@ __global__ __launch_bounds__(64) void k_phase2(Phase2Args A, DevIntegrand P, Rul
     int32_t a = __builtin_amdgcn_readfirstlane(bs);
     if (a < 0) a = 0;  // every error non-finite: the run is reported as such
     const int32_t b = count;
-    const double pe = sest[a], pr = kerr[a];
+    const double pe = sest[a], pr = key_err(a);
     const int k = __builtin_amdgcn_readfirstlane((int)sax[a]);
     double plo[D], pwd[D], wk = 0.0;
 #pragma unroll
@@ -170,8 +186,7 @@ __global__ __launch_bounds__(64) void k_phase2(Phase2Args A, DevIntegrand P, Rul
       }
       sest[slot] = half ? ce[1] : ce[0];
       sax[slot] = (int8_t)q.axis;
-      kerr[slot] = half ? cr[1] : cr[0];
-      kseq[slot] = next + half;
+      set_key(slot, half ? cr[1] : cr[0], next + half);
     }
     next += 2;
     ++count;
@@ -196,7 +211,7 @@ using Phase2Launcher = void (*)(const Phase2Args&, const DevIntegrand&, const Ru
 template <int D, int FAM, int KEY>
 void launch_phase2(const Phase2Args& A, const DevIntegrand& P, const RuleConst& R, hipStream_t st)
 {
-  hipLaunchKernelGGL((k_phase2<D, FAM, KEY>), dim3((unsigned)A.n), dim3(kWave), phase2_lds_bytes(A.slots), st, A,
+  hipLaunchKernelGGL((k_phase2<D, FAM, KEY>), dim3((unsigned)A.n), dim3(kWave), 0, st, A,
                      P, R);
 }
 

@@ -88,6 +88,19 @@ def test_two_phase_workspace_reuse(ctx):
         assert first[k] == ref_big[k], k
 
 
-def test_two_phase_pool_beyond_lds_is_rejected(ctx):
-    with pytest.raises(cuhre.CuhreError, match="LDS"):
-        ctx.integrate_two_phase(W.c1_gaussian3(), key=7, phase1_regions=64, pool_max=1 << 22)
+def test_two_phase_pool_beyond_limit_is_rejected(ctx):
+    with pytest.raises(cuhre.CuhreError, match="2\\^24"):
+        ctx.integrate_two_phase(W.c1_gaussian3(), key=7, phase1_regions=64, pool_max=(1 << 24) + 1)
+
+
+def test_two_phase_large_pool_spills_keys_to_global(ctx):
+    """pool_max 3000 takes pools past the 256 slots whose keys sit in LDS:
+    the 3-D Gaussian at a tight tolerance with 64 Phase-2 regions needs
+    hundreds of bisections in some of them; results as the oracle's."""
+    p = W.Problem("g3tight", W.c1_gaussian3().f, W.c1_gaussian3().lows, W.c1_gaussian3().highs, 1e-12, 0.0, 10**9)
+    g = ctx.integrate_two_phase(p, key=7, phase1_regions=64, pool_max=3000)
+    o = O.integrate_two_phase(p, key=7, phase1_max=64, pool_max=3000)
+    assert g["rc"] == o["rc"] and g["iterations"] == o["iterations"]
+    assert abs(g["nregions"] - o["nregions"]) <= 5e-3 * o["nregions"], (g["nregions"], o["nregions"])
+    assert (g["nregions"] - 64) / 64 > 256  # pools go past the LDS slots on average
+    assert abs(g["integral"] - o["integral"]) <= max(g["error"], o["error"]) + 1e-12 * abs(o["integral"])
