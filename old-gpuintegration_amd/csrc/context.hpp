@@ -97,8 +97,10 @@ int pick_segments(const cuhre_context* c, int64_t n, int key = 7);
 inline int64_t block_capacity(int64_t cap) { return cap / kWave + 1 > 1024 ? cap / kWave + 1 : 1024; }
 // Group sums of the two-level finalisation: one per 64 blocks.
 inline int64_t group_capacity(int64_t nbmax) { return nbmax / kWave + 2; }
-int64_t initial_grid(int d, const double* lows, const double* highs, std::vector<double>& lo,
-                     std::vector<double>& wd);
+// Regions of the initial k^d grid, and the grid written into list `list`
+// on the device.
+int64_t initial_grid_count(int d);
+int upload_initial_grid(cuhre_context* c, int list, int d, const double* lows, const double* highs, int64_t n0);
 int check_common(cuhre_context* ctx, const cuhre_integrand* f, int dim, int key);
 void fill_eval_args(cuhre_context* c, EvalArgs& A, int cur, int64_t n, int dim, double epsrel, double epsabs);
 void fill_split_args(cuhre_context* c, SplitArgs& S, int cur, int64_t n, int dim, double epsrel, double epsabs,

@@ -630,25 +630,54 @@ static int grid_k(int d) { return d <= 4 ? 4 : (d <= 10 ? 2 : 1); }
 
 // Initial uniform k^d grid over the box (GPUKernelquad.cu:23-61, 300-316),
 // region r has coordinate index (r / k^m) % k along axis m.
-int64_t initial_grid(int d, const double* lows, const double* highs, std::vector<double>& lo,
-                            std::vector<double>& wd)
+int64_t initial_grid_count(int d)
 {
   const int k = grid_k(d);
   int64_t n = 1;
   for (int m = 0; m < d; ++m) n *= k;
-  lo.resize((size_t)n * d);
-  wd.resize((size_t)n * d);
-  for (int64_t r = 0; r < n; ++r) {
-    int64_t q = r;
-    for (int m = 0; m < d; ++m) {
-      const int64_t id = q % k;
-      q /= k;
-      const double w = (highs[m] - lows[m]) / (double)k;
-      lo[(size_t)m * n + r] = lows[m] + (double)id * w;
-      wd[(size_t)m * n + r] = w;
-    }
-  }
   return n;
+}
+
+// The initial grid written on the device (lower = low + id * w with the
+// product rounded first, width w = (high - low) / k, as the oracle's
+// or_initial_grid), instead of two host-to-device copies per call.
+struct GridArgs {
+  double* lo;
+  double* wd;
+  int64_t ld, n;
+  int32_t d, k;
+  double lows[kMaxDim], highs[kMaxDim];
+};
+
+__global__ __launch_bounds__(kBlock) void k_initial_grid(GridArgs G)
+{
+  const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (r >= G.n) return;
+  int64_t q = r;
+  for (int m = 0; m < G.d; ++m) {
+    const int64_t id = q % G.k;
+    q /= G.k;
+    const double w = (G.highs[m] - G.lows[m]) / (double)G.k;
+    G.lo[m * G.ld + r] = __dadd_rn(G.lows[m], __dmul_rn((double)id, w));
+    G.wd[m * G.ld + r] = w;
+  }
+}
+
+int upload_initial_grid(cuhre_context* c, int list, int d, const double* lows, const double* highs, int64_t n0)
+{
+  GridArgs G;
+  G.lo = c->lo[list];
+  G.wd = c->wd[list];
+  G.ld = c->cap;
+  G.n = n0;
+  G.d = d;
+  G.k = grid_k(d);
+  for (int m = 0; m < kMaxDim; ++m) {
+    G.lows[m] = m < d ? lows[m] : 0.0;
+    G.highs[m] = m < d ? highs[m] : 0.0;
+  }
+  hipLaunchKernelGGL(k_initial_grid, dim3((unsigned)((n0 + kBlock - 1) / kBlock)), dim3(kBlock), 0, c->stream, G);
+  return hipGetLastError() == hipSuccess ? 0 : fail(CUHRE_ERR_DEVICE, "initial grid launch failed");
 }
 
 int check_common(cuhre_context* ctx, const cuhre_integrand* f, int dim, int key)
@@ -878,17 +907,13 @@ int plain_loop(cuhre_context* c, const cuhre_integrand* f, int dim, double epsre
   const int64_t p = R.npts;
   const double rflops = region_flops(f, dim, key);
 
-  std::vector<double> hlo, hwd;
-  const int64_t n0 = initial_grid(dim, lows, highs, hlo, hwd);
+  const int64_t n0 = initial_grid_count(dim);
   const int64_t capacity = list_capacity(c, dim);
   if (n0 > capacity) return fail(CUHRE_ERR_CAPACITY, "initial grid exceeds capacity");
   const int64_t want = std::min<int64_t>(capacity, std::max<int64_t>(4 * n0, 1 << 16));
   if ((rc = ctx_reserve(c, want, dim))) return rc;
   int cur = 0;
-  HIP_TRY(hipMemcpy2DAsync(c->lo[cur], sizeof(double) * c->cap, hlo.data(), sizeof(double) * n0,
-                           sizeof(double) * n0, dim, hipMemcpyHostToDevice, c->stream));
-  HIP_TRY(hipMemcpy2DAsync(c->wd[cur], sizeof(double) * c->cap, hwd.data(), sizeof(double) * n0,
-                           sizeof(double) * n0, dim, hipMemcpyHostToDevice, c->stream));
+  if ((rc = upload_initial_grid(c, cur, dim, lows, highs, n0))) return rc;
 
   double fin_s = 0.0, fin_c = 0.0, fer_s = 0.0, fer_c = 0.0, fin_abs = 0.0, fin_aest = 0.0;
   int64_t nfin = 0, n = n0;

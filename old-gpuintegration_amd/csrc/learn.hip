@@ -1052,8 +1052,7 @@ int guided_run(cuhre_context* c, cuhre_guide* g, const cuhre_integrand* f, int d
   if ((rc = make_dev_integrand(c, f, dim, &P))) return rc;
   const RuleConst R = make_rule(dim);
   const int64_t p = R.npts;
-  std::vector<double> hlo, hwd;
-  const int64_t n0 = initial_grid(dim, lows, highs, hlo, hwd);
+  const int64_t n0 = initial_grid_count(dim);
   const int64_t capacity = list_capacity(c, dim);
   // the network and the sample log read I5 and the fourth differences
   const bool need_aux = run.use_mlp || run.collect_max > 0;
@@ -1064,10 +1063,7 @@ int guided_run(cuhre_context* c, cuhre_guide* g, const cuhre_integrand* f, int d
   if ((rc = guide_reserve_regions(g, c->cap))) return rc;
   hipStream_t st = c->stream;
   int cur = 0;
-  if (hipMemcpy2DAsync(c->lo[cur], sizeof(double) * c->cap, hlo.data(), sizeof(double) * n0, sizeof(double) * n0,
-                       dim, hipMemcpyHostToDevice, st) != hipSuccess ||
-      hipMemcpy2DAsync(c->wd[cur], sizeof(double) * c->cap, hwd.data(), sizeof(double) * n0, sizeof(double) * n0,
-                       dim, hipMemcpyHostToDevice, st) != hipSuccess)
+  if (upload_initial_grid(c, cur, dim, lows, highs, n0) != 0)
     return set_error(CUHRE_ERR_DEVICE, "initial grid upload failed");
   const int64_t ds_base0 = g->ds_n;
   double fin_s = 0, fin_c = 0, fer_s = 0, fer_c = 0, fin_abs = 0, fin_aest = 0, ret_err = 0;
