@@ -210,6 +210,18 @@ def test_integrate_product_peak_budget_and_growth(ctx, segments):
     assert abs(g["integral"] - ex) <= g["error"]
 
 
+def test_integrate_large_lists_two_level_finalisation(ctx):
+    """maxeval 2e10 takes the 8-D product peak to lists of ~12 M regions:
+    evaluation launches of ~47k blocks, whose sums go through more than 256
+    groups of 64 block partials (the block-wide top level of the two-level
+    finalisation) and whose split adds group bases to the block offsets."""
+    p = W.c2_product_peak8()
+    g = ctx.integrate(p, maxeval=20_000_000_000)
+    assert g["peak_regions"] > 256 * 64 * 256  # more than 256 groups of 64 blocks of 256 regions
+    o = O.integrate_bfs(p, nthreads=16, maxeval=20_000_000_000)
+    _compare_integrations(g, o, p)
+
+
 def test_integrate_reference_driver_pattern(ctx):
     # cudaCuhre.cu:43-83: construct once, integrate at two tolerances on the
     # box [1,2]x[.1,.3]x[0,1]; here with the driver's cosine-sum functor
