@@ -160,7 +160,12 @@ __global__ __launch_bounds__(64) void k_phase2(Phase2Args A, DevIntegrand P, Rul
       wd[m] = m == k ? hw : pwd[m];
       lo[m] = (m == k && half) ? plo[m] + hw : plo[m];
     }
-    const RegionOut q = p2_eval<D, FAM, KEY>(f, R, lo, wd, lane & 31);
+    // the segment index is loop-invariant; kept opaque so the compiler does
+    // not hoist the per-lane point decode out of the loop (its results held
+    // in registers across iterations cost ~160 VGPRs and occupancy)
+    int seg = lane & 31;
+    asm volatile("" : "+v"(seg));
+    const RegionOut q = p2_eval<D, FAM, KEY>(f, R, lo, wd, seg);
     double ce[2] = {read_lane_d(q.est, 0), read_lane_d(q.est, 32)};
     double cr[2] = {read_lane_d(q.err, 0), read_lane_d(q.err, 32)};
     bad = bad || !(isfinite(ce[0]) && isfinite(cr[0]) && isfinite(ce[1]) && isfinite(cr[1]));
