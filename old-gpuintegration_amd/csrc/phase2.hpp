@@ -162,7 +162,8 @@ __global__ __launch_bounds__(64) void k_phase2(Phase2Args A, DevIntegrand P, Rul
     }
     // the segment index is loop-invariant; kept opaque so the compiler does
     // not hoist the per-lane point decode out of the loop (its results held
-    // in registers across iterations cost ~160 VGPRs and occupancy)
+    // in registers across iterations took the kernel from 249 to 308 VGPRs,
+    // one wavefront per SIMD instead of two)
     int seg = lane & 31;
     asm volatile("" : "+v"(seg));
     const RegionOut q = p2_eval<D, FAM, KEY>(f, R, lo, wd, seg);
