@@ -34,7 +34,7 @@ def assert_err_parity(err, oerr, osums, vol, scale, d, tag):
 
 
 @pytest.mark.parametrize("lanes", [1, 8, 32])
-@pytest.mark.parametrize("d", [2, 3, 5, 8])
+@pytest.mark.parametrize("d", [2, 3, 5, 8, 10])
 @pytest.mark.parametrize("fam", FAMILIES)
 def test_rule9_region_values_match_oracle(ctx, fam, d, lanes):
     rng = W.SplitMix64(7000 + 100 * fam + 10 * d + lanes)
