@@ -386,60 +386,58 @@ __device__ __forceinline__ double guide_features(const GuideArgs& G, double absI
   return vrel;
 }
 
-// Guide step of one iteration on 64 regions per block: features, network,
-// axis / finish decisions, per-tile counts and retired sums.  MLP: 256
-// threads run the network on the tile; otherwise one wave, no network.
+// Guide step of one iteration, one 64-region tile per wave and four tiles
+// per block: features (a region per lane), network, axis / finish
+// decisions, per-tile counts and retired sums.  With MLP every wave runs the
+// network on its tile in four steps of 16 rows; the layers are separated by
+// block barriers, so each wave takes all four steps, and a step whose 16
+// regions are all inactive skips the arithmetic.
+constexpr int kGuideTiles = 4;
+
 template <bool MLP>
-__global__ __launch_bounds__(MLP ? 256 : 64) void k_guide_apply(GuideArgs G)
+__global__ __launch_bounds__(256) void k_guide_apply(GuideArgs G)
 {
   constexpr int kL = MLP ? 1 : 0;  // network buffers only when the network runs
-  __shared__ __align__(16) uint16_t X[kMlpRows * kLdIn * kL + 1];
-  __shared__ __align__(16) uint16_t H1[kMlpRows * kLdHid * kL + 1], H2[kMlpRows * kLdHid * kL + 1],
-      H3[kMlpRows * kLdHid * kL + 1];
-  __shared__ float Y[kMlpRows * kMlpOut * kL + 1];
-  __shared__ int s_split[kMlpRows], s_log[kMlpRows], s_ret[kMlpRows], s_ovr[kMlpRows];
-  __shared__ double s_e[kMlpRows], s_er[kMlpRows], s_ab[kMlpRows];
+  __shared__ __align__(16) uint16_t X[kGuideTiles * kMlpRows * kLdIn * kL + 1];
+  __shared__ __align__(16) uint16_t H1[kGuideTiles * 16 * kLdHid * kL + 1], H2[kGuideTiles * 16 * kLdHid * kL + 1],
+      H3[kGuideTiles * 16 * kLdHid * kL + 1];
+  __shared__ float Y[kGuideTiles * 16 * kMlpOut * kL + 1];
   if (G.state != nullptr && G.state->stop != 0) return;  // block-uniform
   double absI = G.absI, tol = G.tol;
   if (G.state != nullptr) {
     absI = fabs(G.state->rG_s + G.state->rG_c);
     tol = fmax(G.epsrel * absI, G.epsabs);
   }
-  const int t = threadIdx.x;
-  const int64_t r = (int64_t)blockIdx.x * kMlpRows + t;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t ntiles = (G.n + kMlpRows - 1) / kMlpRows;
+  const int64_t tile = (int64_t)blockIdx.x * kGuideTiles + w;
+  const int64_t r = tile * kMlpRows + lane;
   bool active = false;
   double vrel = 0.0;
-  if (t < kMlpRows) {
+  int am = 0;                 // network's axis
+  float conf = 0.f, q = 0.f;  // its softmax confidence, finish probability
+  if (MLP) {
     float f[32];
     for (int j = 0; j < 32; ++j) f[j] = 0.f;
     if (r < G.n) {
       const double e = G.est[r], er = G.err[r];
       active = isfinite(e) && isfinite(er) && region_active(e, er, G.loc_epsrel, G.loc_epsabs);
-      if (MLP) {
-        vrel = guide_features(G, absI, r, f);
-      } else if (G.finish_mode >= 2) {
-        vrel = 1.0;
-        for (int m = 0; m < G.dim; ++m) vrel *= G.width[m * G.ld + r] / (G.highs[m] - G.lows[m]);
-      }
+      vrel = guide_features(G, absI, r, f);
     }
-    if (MLP)
-      for (int j = 0; j < 32; ++j) X[t * kLdIn + j] = f2bf(f[j]);
-  }
-  __syncthreads();
-  if (MLP) {
-    const int w = t >> 6;
-    mlp_forward16(G.M, X + w * 16 * kLdIn, H1 + w * 16 * kLdHid, H2 + w * 16 * kLdHid, H3 + w * 16 * kLdHid,
-                  Y + w * 16 * kMlpOut);
-  }
-  if (t < kMlpRows) {
-    int retire = 0, ovr = 0;
-    if (active) {
-      const double e = G.est[r], er = G.err[r];
-      const bool guard = er <= G.budget * tol * vrel;
-      if (MLP) {
-        const float* y = Y + t * kMlpOut;
+    uint16_t* xr = X + (w * kMlpRows + lane) * kLdIn;
+    for (int j = 0; j < 32; ++j) xr[j] = f2bf(f[j]);
+    const uint64_t act = __ballot(active);
+    __syncthreads();
+    for (int s = 0; s < kMlpRows / 16; ++s) {
+      const bool work = ((act >> (16 * s)) & 0xFFFFull) != 0;  // wave-uniform
+      float* Yw = Y + w * 16 * kMlpOut;
+      mlp_forward16(G.M, X + (w * kMlpRows + s * 16) * kLdIn, H1 + w * 16 * kLdHid, H2 + w * 16 * kLdHid,
+                    H3 + w * 16 * kLdHid, Yw, work);
+      // Y of this step is read here and overwritten only after the next
+      // step's first barrier
+      if ((lane >> 4) == s && active) {
+        const float* y = Yw + (lane & 15) * kMlpOut;
         float m = y[0];
-        int am = 0;
         for (int j = 1; j < G.dim; ++j)
           if (y[j] > m) {
             m = y[j];
@@ -447,59 +445,64 @@ __global__ __launch_bounds__(MLP ? 256 : 64) void k_guide_apply(GuideArgs G)
           }
         float se = 0.f;
         for (int j = 0; j < G.dim; ++j) se += expf(y[j] - m);
-        const float conf = 1.f / se;  // softmax probability of the argmax
-        if (G.axis_mode == 1 && conf >= G.axis_conf && G.axis[r] != am) {
-          G.axis[r] = (int8_t)am;
-          ovr = 1;
-        }
-        const float q = 1.f / (1.f + expf(-y[kMlpFinish]));
-        if (G.finish_mode == 1 && q >= G.finish_thr && guard) retire = 1;
+        conf = 1.f / se;  // softmax probability of the argmax
+        q = 1.f / (1.f + expf(-y[kMlpFinish]));
       }
-      if (G.finish_mode == 2 && guard) retire = 1;
-      if (G.finish_mode == 3) {
-        // shared allotment: a (1 - budget) share of tol by value, a budget share by volume
-        const double a = G.epsrel * ((1.0 - G.budget) * fabs(e) + G.budget * absI * vrel);
-        if (er <= fmax(a, G.epsabs)) retire = 1;
-      }
-      s_e[t] = e;
-      s_er[t] = er;
-      s_ab[t] = G.abssum[r];
-    } else {
-      s_e[t] = s_er[t] = s_ab[t] = 0.0;
     }
-    if (r < G.n) G.retire[r] = (int8_t)retire;
-    s_ret[t] = retire;
-    s_ovr[t] = ovr;
-    s_split[t] = active && !retire ? 1 : 0;
-    s_log[t] = guide_selected(G, r, active) ? 1 : 0;
+  } else if (r < G.n) {
+    const double e = G.est[r], er = G.err[r];
+    active = isfinite(e) && isfinite(er) && region_active(e, er, G.loc_epsrel, G.loc_epsabs);
+    if (G.finish_mode >= 2) {
+      vrel = 1.0;
+      for (int m = 0; m < G.dim; ++m) vrel *= G.width[m * G.ld + r] / (G.highs[m] - G.lows[m]);
+    }
   }
-  __syncthreads();
-  if (t == 0) {
-    int64_t ns = 0, nl = 0, nr = 0, no = 0;
-    double es = 0, ec = 0, rs = 0, rc = 0, ab = 0, ae = 0;
-    for (int j = 0; j < kMlpRows; ++j) {
-      ns += s_split[j];
-      nl += s_log[j];
-      no += s_ovr[j];
-      if (s_ret[j]) {
-        ++nr;
-        dd_add(es, ec, s_e[j], 0.0);
-        dd_add(rs, rc, s_er[j], 0.0);
-        ab += s_ab[j];
-        ae += fabs(s_e[j]);
+  int retire = 0, ovr = 0;
+  double e = 0.0, er = 0.0, ab = 0.0;
+  if (active) {
+    e = G.est[r];
+    er = G.err[r];
+    const bool guard = er <= G.budget * tol * vrel;
+    if (MLP) {
+      if (G.axis_mode == 1 && conf >= G.axis_conf && G.axis[r] != am) {
+        G.axis[r] = (int8_t)am;
+        ovr = 1;
       }
+      if (G.finish_mode == 1 && q >= G.finish_thr && guard) retire = 1;
     }
-    G.tile_cnt[blockIdx.x] = ns;
-    G.tile_logcnt[blockIdx.x] = nl;
-    double* tr = G.tile_ret + (int64_t)blockIdx.x * kRetStride;
+    if (G.finish_mode == 2 && guard) retire = 1;
+    if (G.finish_mode == 3) {
+      // shared allotment: a (1 - budget) share of tol by value, a budget share by volume
+      const double a = G.epsrel * ((1.0 - G.budget) * fabs(e) + G.budget * absI * vrel);
+      if (er <= fmax(a, G.epsabs)) retire = 1;
+    }
+    ab = G.abssum[r];
+  }
+  if (r < G.n) G.retire[r] = (int8_t)retire;
+  // the tile's counts and retired sums (a butterfly, the same order on every run)
+  const uint64_t b_split = __ballot(active && !retire), b_log = __ballot(guide_selected(G, r, active));
+  const uint64_t b_ovr = __ballot(ovr != 0), b_ret = __ballot(retire != 0);
+  double es = retire ? e : 0.0, ec = 0.0, rs = retire ? er : 0.0, rc = 0.0;
+  double abr = retire ? ab : 0.0, ae = retire ? fabs(e) : 0.0;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    dd_add(es, ec, shfl_xor_d(es, o), shfl_xor_d(ec, o));
+    dd_add(rs, rc, shfl_xor_d(rs, o), shfl_xor_d(rc, o));
+    abr += shfl_xor_d(abr, o);
+    ae += shfl_xor_d(ae, o);
+  }
+  if (lane == 0 && tile < ntiles) {
+    G.tile_cnt[tile] = __popcll(b_split);
+    G.tile_logcnt[tile] = __popcll(b_log);
+    double* tr = G.tile_ret + tile * kRetStride;
     tr[0] = es;
     tr[1] = ec;
     tr[2] = rs;
     tr[3] = rc;
-    tr[4] = ab;
+    tr[4] = abr;
     tr[5] = ae;
-    tr[6] = (double)nr;
-    tr[7] = (double)no;
+    tr[6] = (double)__popcll(b_ret);
+    tr[7] = (double)__popcll(b_ovr);
   }
 }
 
@@ -1144,10 +1147,11 @@ int guided_run(cuhre_context* c, cuhre_guide* g, const cuhre_integrand* f, int d
         G.log_stride = (uint64_t)std::max<int64_t>(1, (nact + target - 1) / target);
         G.log_salt = 0x5DEECE66Dull * (uint64_t)(it + 1);
       }
+      const unsigned gblocks = (unsigned)((ntiles + kGuideTiles - 1) / kGuideTiles);
       if (run.use_mlp)
-        hipLaunchKernelGGL(k_guide_apply<true>, dim3((unsigned)ntiles), dim3(256), 0, st, G);
+        hipLaunchKernelGGL(k_guide_apply<true>, dim3(gblocks), dim3(256), 0, st, G);
       else
-        hipLaunchKernelGGL(k_guide_apply<false>, dim3((unsigned)ntiles), dim3(64), 0, st, G);
+        hipLaunchKernelGGL(k_guide_apply<false>, dim3(gblocks), dim3(256), 0, st, G);
       // retirement room: retired error within budget * tol, frozen error
       // (finished so far, finished now, retired) within (1 - budget) * tol
       RA.state = G.state;

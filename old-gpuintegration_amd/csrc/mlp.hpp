@@ -92,29 +92,30 @@ __device__ __forceinline__ void mlp_store_relu(const f32x4 (&acc)[N / 16], const
 // (padded strides); Ys receives the 16 x 16 logits in FP32.  The weights
 // come from W^T in global memory (L2-resident, 22 KB): staging them in LDS
 // per block was measured slower.  Every wave of the block calls it together
-// (block barriers between layers).
+// (block barriers between layers); a wave with `work` false (wave-uniform)
+// only takes the barriers and leaves its buffers as they are.
 __device__ __forceinline__ void mlp_forward16(const MlpDev& M, const uint16_t* Xs, uint16_t* H1s, uint16_t* H2s,
-                                              uint16_t* H3s, float* Ys)
+                                              uint16_t* H3s, float* Ys, bool work = true)
 {
-  {
+  if (work) {
     f32x4 acc[4];
     mlp_layer_fwd<32, 64, 32>(Xs, M.WT + kMlpWOff[0], acc);
     mlp_store_relu<64>(acc, M.b + 0, H1s);
   }
   __syncthreads();
-  {
+  if (work) {
     f32x4 acc[4];
     mlp_layer_fwd<64, 64, 64>(H1s, M.WT + kMlpWOff[1], acc);
     mlp_store_relu<64>(acc, M.b + 64, H2s);
   }
   __syncthreads();
-  {
+  if (work) {
     f32x4 acc[4];
     mlp_layer_fwd<64, 64, 64>(H2s, M.WT + kMlpWOff[2], acc);
     mlp_store_relu<64>(acc, M.b + 128, H3s);
   }
   __syncthreads();
-  {
+  if (work) {
     f32x4 acc[1];
     mlp_layer_fwd<64, 16, 64>(H3s, M.WT + kMlpWOff[3], acc);
     const int l = __lane_id(), col = l & 15, r0 = (l >> 4) * 4;
