@@ -361,47 +361,73 @@ __device__ __forceinline__ bool guide_selected(const GuideArgs& G, int64_t r, bo
   return active && G.log_stride > 0 && (mix64((uint64_t)r ^ G.log_salt) % G.log_stride) == 0;
 }
 
-// The 32 input features of region r (25 used at d = 7) and its relative volume.
+// log2 of a positive double to float precision: its binary exponent plus
+// the float log2 of its mantissa in [0.5, 1).
+__device__ __forceinline__ float log2_mant(double x)
+{
+  int ex;
+  const double m = frexp(x, &ex);
+  return (float)ex + log2f((float)m);
+}
+
+// The 32 input features of region r (25 used at d = 7) and its relative
+// volume.  The axis loops run over the guide's largest dimension, 8, with
+// the unused axes masked, so the loads of all axes go out together.
 __device__ __forceinline__ double guide_features(const GuideArgs& G, double absI, int64_t r, float* f)
 {
+  constexpr int kAx = 8;
+  constexpr float kLog10Of2 = 0.30102999566398120f;
   const int d = G.dim;
+  double lo[kAx], wd[kAx], dv[kAx];
+#pragma unroll
+  for (int m = 0; m < kAx; ++m)
+    if (m < d) {
+      lo[m] = G.lower[m * G.ld + r];
+      wd[m] = G.width[m * G.ld + r];
+      dv[m] = G.dvec[m * G.ld + r];
+    }
   double vrel = 1.0, maxd = 0.0;
+#pragma unroll
   for (int j = 0; j < 32; ++j) f[j] = 0.f;
-  for (int m = 0; m < d; ++m) {
-    const double span = G.highs[m] - G.lows[m];
-    const double lo = G.lower[m * G.ld + r], w = G.width[m * G.ld + r];
-    const double wr = w / span;
-    vrel *= wr;
-    f[m] = (float)((lo - G.lows[m]) / span);
-    f[8 + m] = (float)(-log2(wr) / 16.0);
-    maxd = fmax(maxd, G.dvec[m * G.ld + r]);
-  }
-  for (int m = 0; m < d; ++m) f[16 + m] = maxd > 0.0 ? (float)(G.dvec[m * G.ld + r] / maxd) : 0.f;
+#pragma unroll
+  for (int m = 0; m < kAx; ++m)
+    if (m < d) {
+      const double span = G.highs[m] - G.lows[m];
+      const double wr = wd[m] / span;
+      vrel *= wr;
+      f[m] = (float)((lo[m] - G.lows[m]) / span);
+      f[8 + m] = -log2_mant(wr) / 16.f;
+      maxd = fmax(maxd, dv[m]);
+    }
+  const double inv = maxd > 0.0 ? 1.0 / maxd : 0.0;
+#pragma unroll
+  for (int m = 0; m < kAx; ++m)
+    if (m < d) f[16 + m] = (float)(dv[m] * inv);
   const double e = G.est[r], er = G.err[r], i5 = G.i5[r];
   const double scale = absI * vrel;
-  f[24] = (float)(log10(fabs(e) / scale + 1e-30) / 8.0);
-  f[25] = (float)(log10(er / (G.epsrel * scale) + 1e-30) / 8.0);
+  f[24] = log2_mant(fabs(e) / scale + 1e-30) * (kLog10Of2 / 8.f);
+  f[25] = log2_mant(er / (G.epsrel * scale) + 1e-30) * (kLog10Of2 / 8.f);
   f[26] = (float)((e - i5) / (fabs(e) + fabs(i5) + 1e-300));
-  f[27] = (float)(-log2(vrel) / 64.0);
+  f[27] = -log2_mant(vrel) / 64.f;
   return vrel;
 }
 
-// Guide step of one iteration, one 64-region tile per wave and four tiles
-// per block: features (a region per lane), network, axis / finish
-// decisions, per-tile counts and retired sums.  With MLP every wave runs the
-// network on its tile in four steps of 16 rows; the layers are separated by
-// block barriers, so each wave takes all four steps, and a step whose 16
-// regions are all inactive skips the arithmetic.
-constexpr int kGuideTiles = 4;
+// Guide step of one iteration, one 64-region tile per wave: features (a
+// region per lane), network, axis / finish decisions, per-tile counts and
+// retired sums.  With MLP a block is one wave, which runs the network on its
+// tile in LDS of its own with no block barrier (16-row groups whose regions
+// are all inactive skip the arithmetic); without, a block holds four tiles.
+template <bool MLP>
+constexpr int guide_tiles()
+{
+  return MLP ? 1 : 4;
+}
 
 template <bool MLP>
-__global__ __launch_bounds__(256) void k_guide_apply(GuideArgs G)
+__global__ __launch_bounds__(64 * guide_tiles<MLP>()) void k_guide_apply(GuideArgs G)
 {
   constexpr int kL = MLP ? 1 : 0;  // network buffers only when the network runs
-  __shared__ __align__(16) uint16_t X[kGuideTiles * kMlpRows * kLdIn * kL + 1];
-  __shared__ __align__(16) uint16_t H1[kGuideTiles * 16 * kLdHid * kL + 1], H2[kGuideTiles * 16 * kLdHid * kL + 1],
-      H3[kGuideTiles * 16 * kLdHid * kL + 1];
-  __shared__ float Y[kGuideTiles * 16 * kMlpOut * kL + 1];
+  __shared__ __align__(16) uint16_t BufA[kMlpWaveBuf * kL + 1], BufB[kMlpWaveBuf * kL + 1];
   if (G.state != nullptr && G.state->stop != 0) return;  // block-uniform
   double absI = G.absI, tol = G.tol;
   if (G.state != nullptr) {
@@ -410,7 +436,7 @@ __global__ __launch_bounds__(256) void k_guide_apply(GuideArgs G)
   }
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t ntiles = (G.n + kMlpRows - 1) / kMlpRows;
-  const int64_t tile = (int64_t)blockIdx.x * kGuideTiles + w;
+  const int64_t tile = (int64_t)blockIdx.x * guide_tiles<MLP>() + w;
   const int64_t r = tile * kMlpRows + lane;
   bool active = false;
   double vrel = 0.0;
@@ -424,30 +450,26 @@ __global__ __launch_bounds__(256) void k_guide_apply(GuideArgs G)
       active = isfinite(e) && isfinite(er) && region_active(e, er, G.loc_epsrel, G.loc_epsabs);
       vrel = guide_features(G, absI, r, f);
     }
-    uint16_t* xr = X + (w * kMlpRows + lane) * kLdIn;
+    uint16_t* xr = BufA + lane * kLdIn;
     for (int j = 0; j < 32; ++j) xr[j] = f2bf(f[j]);
     const uint64_t act = __ballot(active);
-    __syncthreads();
-    for (int s = 0; s < kMlpRows / 16; ++s) {
-      const bool work = ((act >> (16 * s)) & 0xFFFFull) != 0;  // wave-uniform
-      float* Yw = Y + w * 16 * kMlpOut;
-      mlp_forward16(G.M, X + (w * kMlpRows + s * 16) * kLdIn, H1 + w * 16 * kLdHid, H2 + w * 16 * kLdHid,
-                    H3 + w * 16 * kLdHid, Yw, work);
-      // Y of this step is read here and overwritten only after the next
-      // step's first barrier
-      if ((lane >> 4) == s && active) {
-        const float* y = Yw + (lane & 15) * kMlpOut;
-        float m = y[0];
-        for (int j = 1; j < G.dim; ++j)
-          if (y[j] > m) {
-            m = y[j];
-            am = j;
-          }
-        float se = 0.f;
-        for (int j = 0; j < G.dim; ++j) se += expf(y[j] - m);
-        conf = 1.f / se;  // softmax probability of the argmax
-        q = 1.f / (1.f + expf(-y[kMlpFinish]));
-      }
+    unsigned work = 0;
+    for (int s = 0; s < kMlpRows / 16; ++s)
+      if ((act >> (16 * s)) & 0xFFFFull) work |= 1u << s;
+    __syncwarp();
+    mlp_forward64_wave(G.M, BufA, BufB, work);
+    if (active) {
+      const float* y = reinterpret_cast<const float*>(BufA) + lane * kMlpOut;
+      float m = y[0];
+      for (int j = 1; j < G.dim; ++j)
+        if (y[j] > m) {
+          m = y[j];
+          am = j;
+        }
+      float se = 0.f;
+      for (int j = 0; j < G.dim; ++j) se += expf(y[j] - m);
+      conf = 1.f / se;  // softmax probability of the argmax
+      q = 1.f / (1.f + expf(-y[kMlpFinish]));
     }
   } else if (r < G.n) {
     const double e = G.est[r], er = G.err[r];
@@ -1147,11 +1169,10 @@ int guided_run(cuhre_context* c, cuhre_guide* g, const cuhre_integrand* f, int d
         G.log_stride = (uint64_t)std::max<int64_t>(1, (nact + target - 1) / target);
         G.log_salt = 0x5DEECE66Dull * (uint64_t)(it + 1);
       }
-      const unsigned gblocks = (unsigned)((ntiles + kGuideTiles - 1) / kGuideTiles);
       if (run.use_mlp)
-        hipLaunchKernelGGL(k_guide_apply<true>, dim3(gblocks), dim3(256), 0, st, G);
+        hipLaunchKernelGGL(k_guide_apply<true>, dim3((unsigned)ntiles), dim3(64), 0, st, G);
       else
-        hipLaunchKernelGGL(k_guide_apply<false>, dim3(gblocks), dim3(256), 0, st, G);
+        hipLaunchKernelGGL(k_guide_apply<false>, dim3((unsigned)((ntiles + 3) / 4)), dim3(256), 0, st, G);
       // retirement room: retired error within budget * tol, frozen error
       // (finished so far, finished now, retired) within (1 - budget) * tol
       RA.state = G.state;

@@ -125,4 +125,69 @@ __device__ __forceinline__ void mlp_forward16(const MlpDev& M, const uint16_t* X
   __syncthreads();
 }
 
+// One layer over a wave's 64 rows: the B operands (W^T rows) are loaded once
+// and used for the four 16-row groups, each followed by bias + ReLU into Out
+// [64][N + kPad] BF16 (LAST: the plain logits into Yout [64][16] FP32).  A
+// group whose bit in `work` is clear (wave-uniform) is skipped.
+template <int K, int N, int LDW, bool LAST>
+__device__ __forceinline__ void mlp_layer64(const uint16_t* In, int ldin, const uint16_t* WT, const float* bias,
+                                            uint16_t* Out, float* Yout, unsigned work)
+{
+  const int l = __lane_id(), row = l & 15, kq = (l >> 4) * 4, col = l & 15, r0 = (l >> 4) * 4;
+  bf16x4 b[K / 16][N / 16];
+#pragma unroll
+  for (int kt = 0; kt < K / 16; ++kt)
+#pragma unroll
+    for (int nt = 0; nt < N / 16; ++nt)
+      b[kt][nt] = *reinterpret_cast<const bf16x4*>(WT + (nt * 16 + row) * LDW + kt * 16 + kq);
+  float bs[N / 16];
+#pragma unroll
+  for (int nt = 0; nt < N / 16; ++nt) bs[nt] = bias[nt * 16 + col];
+  for (int s = 0; s < 4; ++s) {
+    if (!((work >> s) & 1u)) continue;
+    f32x4 acc[N / 16];
+#pragma unroll
+    for (int nt = 0; nt < N / 16; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kt = 0; kt < K / 16; ++kt) {
+      const bf16x4 a = *reinterpret_cast<const bf16x4*>(In + (s * 16 + row) * ldin + kt * 16 + kq);
+#pragma unroll
+      for (int nt = 0; nt < N / 16; ++nt) acc[nt] = mfma16(a, b[kt][nt], acc[nt]);
+    }
+#pragma unroll
+    for (int nt = 0; nt < N / 16; ++nt)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const float z = acc[nt][v] + bs[nt];
+        if (LAST)
+          Yout[(s * 16 + r0 + v) * kMlpOut + nt * 16 + col] = z;
+        else
+          Out[(s * 16 + r0 + v) * (N + kPad) + nt * 16 + col] = f2bf(z > 0.f ? z : 0.f);
+      }
+  }
+}
+
+// LDS of one wave's 64-row forward pass, in two buffers used in turn:
+// A holds the inputs [64][kLdIn], then the second hidden layer, then the
+// logits [64][16] FP32; B the first hidden layer, then the third.
+constexpr int kMlpWaveBuf = kMlpRows * kLdHid;  // uint16 elements per buffer
+static_assert(kMlpRows * kLdIn <= kMlpWaveBuf && kMlpRows * kMlpOut * 2 <= kMlpWaveBuf, "wave buffers");
+
+// Forward pass of a wave's 64 rows with no block barrier: the wave owns A
+// and B, and the layers are ordered by wavefront-scope fences.  Bit s of
+// `work` asks for rows 16s .. 16s + 15.  On return the logits of row i are
+// reinterpret_cast<float*>(A)[i * 16 .. i * 16 + 15].
+__device__ __forceinline__ void mlp_forward64_wave(const MlpDev& M, uint16_t* A, uint16_t* B, unsigned work)
+{
+  mlp_layer64<32, 64, 32, false>(A, kLdIn, M.WT + kMlpWOff[0], M.b + 0, B, nullptr, work);
+  __syncwarp();
+  mlp_layer64<64, 64, 64, false>(B, kLdHid, M.WT + kMlpWOff[1], M.b + 64, A, nullptr, work);
+  __syncwarp();
+  mlp_layer64<64, 64, 64, false>(A, kLdHid, M.WT + kMlpWOff[2], M.b + 128, B, nullptr, work);
+  __syncwarp();
+  mlp_layer64<64, 16, 64, true>(B, kLdHid, M.WT + kMlpWOff[3], M.b + 192, nullptr, reinterpret_cast<float*>(A),
+                                work);
+  __syncwarp();
+}
+
 }  // namespace cuhre

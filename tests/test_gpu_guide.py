@@ -153,6 +153,12 @@ def test_collection_labels(ctx):
     assert np.all(np.abs(X[:, 28:]) == 0)
     # fourth differences: only axis 2 is non-zero, normalised to 1
     assert np.allclose(X[:, 18], 1.0) and np.allclose(X[:, 16], 0.0) and np.allclose(X[:, 17], 0.0)
+    # bisections of [0, 1]: depth k per axis is -log2(width) = 16 * feature,
+    # an integer, and the position lower / span a multiple of 2^-k
+    depth = X[:, 8:11].astype(np.float64) * 16
+    assert np.all(depth == np.round(depth)) and np.all(depth >= 0)
+    pos = X[:, 0:3].astype(np.float64) * 2.0 ** depth
+    assert np.all(pos == np.round(pos)) and np.all(X[:, 0:3] >= 0) and np.all(X[:, 0:3] < 1)
     assert r["nregions"] == ctx.integrate(p)["nregions"]
 
 
