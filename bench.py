@@ -130,35 +130,26 @@ def bench_batched(ctx, count=4096):
             "per_instance_regions_min_max": [min(r["nregions"] for r in res), max(r["nregions"] for r in res)]}
 
 
-def bench_learned(ctx, samples_per_run=500_000, maxeval=10**11, rounds=3):
-    """Learned split guide on 7-D corner peaks (epsrel 1e-6): sample
-    collection on training seeds 100..115 (round 0 under the plain axes,
-    later rounds under the network's own axes, every sample labelled by the
-    look-ahead), on-GPU training after each round, then guided vs plain on
-    the held-out seeds 6, 7, 8."""
+def _learned_family(ctx, dim, train, held, maxeval, train_maxeval, samples_per_run, policies, rounds=3):
+    """Collect on the training problems (round 0 under the plain axes, later
+    rounds under the network's own axes, every sample labelled by the
+    look-ahead), train on the GPU after each round, then run each policy on
+    the held-out problems (best of two runs)."""
     import cuhre
     g = cuhre.Guide(ctx, seed=11)
     t0 = time.perf_counter()
-    train = []
+    log = []
     for rnd in range(rounds):
-        for p in W.c5_train_problems(100, 16):
-            g.collect(p, max_samples=samples_per_run, maxeval=maxeval, axis_mode=0 if rnd == 0 else 1,
+        for p in train:
+            g.collect(p, max_samples=samples_per_run, maxeval=train_maxeval, axis_mode=0 if rnd == 0 else 1,
                       axis_confidence=0.6)
         for ep, lr in ((2, 2e-3), (2, 1e-3), (1, 3e-4)):
-            st = g.train(7, epochs=ep, batch=8192, lr=lr, seed=10 * rnd + ep)
-        train.append(dict(st, round=rnd))
+            st = g.train(dim, epochs=ep, batch=8192, lr=lr, seed=10 * rnd + ep)
+        log.append(dict(st, round=rnd))
     t_learn = time.perf_counter() - t0
-    policies = {
-        "plain": None,
-        "learned_axis": dict(axis_mode=1, finish_mode=0, axis_confidence=0.6),
-        "learned_axis_and_finish": dict(axis_mode=1, finish_mode=1, axis_confidence=0.6, budget=0.25),
-        "volume_guard": dict(axis_mode=0, finish_mode=2, budget=0.25),
-        "shared_allotment": dict(axis_mode=0, finish_mode=3, budget=0.5),
-    }
-    held = []
-    for seed in (6, 7, 8):
-        p = W.c5_corner7(seed)
-        row = {"seed": seed}
+    rows = []
+    for p in held:
+        row = {"integrand": p.name}
         for name, pol in policies.items():
             best = None
             for _ in range(2):
@@ -169,9 +160,37 @@ def bench_learned(ctx, samples_per_run=500_000, maxeval=10**11, rounds=3):
                          "integral": best["integral"], "error": best["error"],
                          "retired": best.get("guide", {}).get("retired", 0),
                          "overridden": best.get("guide", {}).get("overridden", 0)}
-        held.append(row)
-    return {"samples": g.samples(), "collect_and_train_s": t_learn, "train": train, "held_out": held,
-            "maxeval": maxeval}
+        for name in policies:
+            if name != "plain":
+                row[name]["regions_vs_plain"] = row[name]["nregions"] / row["plain"]["nregions"]
+                row[name]["time_vs_plain"] = row[name]["ms"] / row["plain"]["ms"]
+        rows.append(row)
+    n = g.samples()
+    g.close()
+    return {"samples": n, "collect_and_train_s": t_learn, "train": log, "held_out": rows, "maxeval": maxeval}
+
+
+def bench_learned(ctx):
+    """Learned split guide, epsrel 1e-6, trained and tested on the GPU:
+    * 7-D corner peaks: training seeds 100..115 (maxeval 1e11, 500k samples
+      a run), held-out seeds 6, 7, 8;
+    * 8-D product peaks: training centres 200..211 (maxeval 1e10, 400k
+      samples a run), held out: the 8-D product peak of the main benchmark,
+      run to convergence (maxeval 1e12)."""
+    corner = _learned_family(
+        ctx, 7, W.c5_train_problems(100, 16), [W.c5_corner7(s) for s in (6, 7, 8)], 10**11, 10**11, 500_000,
+        {"plain": None,
+         "learned_axis": dict(axis_mode=1, finish_mode=0, axis_confidence=0.6),
+         "learned_axis_and_finish": dict(axis_mode=1, finish_mode=1, axis_confidence=0.6, budget=0.25),
+         "volume_guard": dict(axis_mode=0, finish_mode=2, budget=0.25),
+         "shared_allotment": dict(axis_mode=0, finish_mode=3, budget=0.5)})
+    peak = _learned_family(
+        ctx, 8, [W.c2_product_peak8(s) for s in range(200, 212)], [W.c2_product_peak8()], 10**12, 10**10, 400_000,
+        {"plain": None,
+         "shared_allotment": dict(axis_mode=0, finish_mode=3, budget=0.5),
+         "learned_axis": dict(axis_mode=1, finish_mode=0, axis_confidence=0.6),
+         "learned_axis_and_allotment": dict(axis_mode=1, finish_mode=3, budget=0.5, axis_confidence=0.6)})
+    return {"corner_peak7": corner, "product_peak8": peak}
 
 
 def bench_reference(ctx):

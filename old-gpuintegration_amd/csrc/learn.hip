@@ -58,23 +58,21 @@ __global__ void k_mlp_prepare(const float* params, uint16_t* W, uint16_t* WT)
   WT[kMlpWOff[l] + n * K + k] = h;
 }
 
-// Inference on rows given as FP32 features (n x 32) -> logits (n x 16).
-__global__ __launch_bounds__(256) void k_mlp_forward(MlpDev M, const float* feat, int64_t n, float* out)
+// Inference on rows given as FP32 features (n x 32) -> logits (n x 16): one
+// wave per 64 rows, the same network pass as the guide step's.
+__global__ __launch_bounds__(64) void k_mlp_forward(MlpDev M, const float* feat, int64_t n, float* out)
 {
-  __shared__ __align__(16) uint16_t X[kMlpRows * kLdIn];
-  __shared__ __align__(16) uint16_t H1[kMlpRows * kLdHid], H2[kMlpRows * kLdHid], H3[kMlpRows * kLdHid];
-  __shared__ float Y[kMlpRows * kMlpOut];
+  __shared__ __align__(16) uint16_t A[kMlpWaveBuf], B[kMlpWaveBuf];
   const int64_t r0 = (int64_t)blockIdx.x * kMlpRows;
-  for (int e = threadIdx.x; e < kMlpRows * kMlpIn; e += blockDim.x) {
+  for (int e = threadIdx.x; e < kMlpRows * kMlpIn; e += kWave) {
     const int rr = e / kMlpIn, q = e % kMlpIn;
     const int64_t r = r0 + rr;
-    X[rr * kLdIn + q] = r < n ? f2bf(feat[r * kMlpIn + q]) : (uint16_t)0;
+    A[rr * kLdIn + q] = r < n ? f2bf(feat[r * kMlpIn + q]) : (uint16_t)0;
   }
-  __syncthreads();
-  const int w = threadIdx.x >> 6;
-  mlp_forward16(M, X + w * 16 * kLdIn, H1 + w * 16 * kLdHid, H2 + w * 16 * kLdHid, H3 + w * 16 * kLdHid,
-                Y + w * 16 * kMlpOut);
-  for (int e = threadIdx.x; e < kMlpRows * kMlpOut; e += blockDim.x) {
+  __syncwarp();
+  mlp_forward64_wave(M, A, B, 0xFu);
+  const float* Y = reinterpret_cast<const float*>(A);
+  for (int e = threadIdx.x; e < kMlpRows * kMlpOut; e += kWave) {
     const int64_t r = r0 + e / kMlpOut;
     if (r < n) out[r * kMlpOut + e % kMlpOut] = Y[e];
   }
@@ -1441,7 +1439,7 @@ int cuhre_guide_forward(cuhre_guide* g, int64_t n, const float* features, float*
     return set_error(CUHRE_ERR_ALLOC, "cannot allocate forward buffers");
   int rc = 0;
   if (hipMemcpyAsync(dx, features, sizeof(float) * 32 * n, hipMemcpyHostToDevice, st) != hipSuccess) rc = 1;
-  hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)((n + kMlpRows - 1) / kMlpRows)), dim3(256), 0, st, mlp_dev(g),
+  hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)((n + kMlpRows - 1) / kMlpRows)), dim3(kWave), 0, st, mlp_dev(g),
                      dx, n, dy);
   if (hipGetLastError() != hipSuccess ||
       hipMemcpyAsync(logits, dy, sizeof(float) * 16 * n, hipMemcpyDeviceToHost, st) != hipSuccess ||
