@@ -278,15 +278,58 @@ __global__ __launch_bounds__(256) void k_mlp_train(TrainArgs T)
   }
 }
 
-// grad[p] = sum over blocks (fixed order); optional Adam step on the FP32
-// master parameters followed by the BF16 copies.
-__global__ void k_mlp_reduce_adam(const float* part, int nblocks, float* grad, float* params, float* m,
-                                  float* v, int step, float lr, int do_update, uint16_t* W, uint16_t* WT)
+// grad[p] = sum over the training blocks' partials in a fixed order (each of
+// kRedWaves wavefronts sums every kRedWaves-th block for 64 parameters, then
+// wavefront 0 adds the kRedWaves sums in order); optional Adam step on the
+// FP32 master parameters followed by the BF16 copies.  With `acc` set, one
+// extra block adds the step's loss and correct counts of `rows` rows into
+// acc[0..2].
+constexpr int kRedWaves = 8;
+
+__global__ __launch_bounds__(kRedWaves * 64) void k_mlp_reduce_adam(const float* part, int nblocks, float* grad,
+                                                                     float* params, float* m, float* v, int step,
+                                                                     float lr, int do_update, uint16_t* W,
+                                                                     uint16_t* WT, const float* loss,
+                                                                     const int8_t* correct, int64_t rows,
+                                                                     double* acc)
 {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= kMlpParams) return;
+  __shared__ float s_g[kRedWaves][64];
+  __shared__ double s_st[kRedWaves][3];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (acc != nullptr && blockIdx.x == gridDim.x - 1) {
+    double l = 0, a = 0, f = 0;
+    for (int64_t r = threadIdx.x; r < rows; r += blockDim.x) {
+      l += loss[r];
+      a += (correct[r] & 1) ? 1 : 0;
+      f += (correct[r] & 2) ? 1 : 0;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      l += shfl_xor_d(l, o);
+      a += shfl_xor_d(a, o);
+      f += shfl_xor_d(f, o);
+    }
+    if (lane == 0) {
+      s_st[w][0] = l;
+      s_st[w][1] = a;
+      s_st[w][2] = f;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+      double t = 0;
+      for (int j = 0; j < kRedWaves; ++j) t += s_st[j][threadIdx.x];
+      acc[threadIdx.x] += t;
+    }
+    return;
+  }
+  const int p = blockIdx.x * 64 + lane;
   float g = 0.f;
-  for (int b = 0; b < nblocks; ++b) g += part[(int64_t)b * kMlpParams + p];
+  if (p < kMlpParams)
+    for (int b = w; b < nblocks; b += kRedWaves) g += part[(int64_t)b * kMlpParams + p];
+  s_g[w][lane] = g;
+  __syncthreads();
+  if (w != 0 || p >= kMlpParams) return;
+  g = s_g[0][lane];
+  for (int j = 1; j < kRedWaves; ++j) g += s_g[j][lane];
   grad[p] = g;
   if (!do_update) return;
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
@@ -801,19 +844,6 @@ __global__ void k_label_finish(const double* s_err, const double* s_vrel, int64_
   s_fin[s] = s_err[s] <= tol * s_vrel[s] ? 1.f : 0.f;
 }
 
-__global__ void k_train_stats(const float* loss, const int8_t* correct, int64_t rows, double* acc)
-{
-  double l = 0, a = 0, f = 0;
-  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < rows; r += (int64_t)gridDim.x * blockDim.x) {
-    l += loss[r];
-    a += (correct[r] & 1) ? 1 : 0;
-    f += (correct[r] & 2) ? 1 : 0;
-  }
-  atomicAdd(acc + 0, l);
-  atomicAdd(acc + 1, a);
-  atomicAdd(acc + 2, f);
-}
-
 }  // namespace cuhre
 
 using namespace cuhre;
@@ -1028,8 +1058,11 @@ int reserve_train(cuhre_guide* g, int64_t rows, int64_t order_len)
 }
 
 // One optimisation step (or gradient only) over `rows` sample indices.
+// With `stats` set, the step's loss and correct counts are added into
+// stats[0..2] (the device's epoch sums).
 void train_step(cuhre_guide* g, const float* feat, const int8_t* ax, const float* fin, const int32_t* order,
-                int64_t rows, int dim, float lr, bool update, hipStream_t st, uint64_t perm_seed = 0)
+                int64_t rows, int dim, float lr, bool update, hipStream_t st, uint64_t perm_seed = 0,
+                double* stats = nullptr)
 {
   TrainArgs T;
   T.perm_seed = perm_seed;
@@ -1047,8 +1080,10 @@ void train_step(cuhre_guide* g, const float* feat, const int8_t* ax, const float
   const int blocks = (int)((rows + kMlpRows - 1) / kMlpRows);
   hipLaunchKernelGGL(k_mlp_train, dim3(blocks), dim3(256), 0, st, T);
   if (update) ++g->adam_step;
-  hipLaunchKernelGGL(k_mlp_reduce_adam, dim3((kMlpParams + 255) / 256), dim3(256), 0, st, g->part, blocks, g->grad,
-                     g->params, g->m, g->v, std::max(g->adam_step, 1), lr, update ? 1 : 0, g->W, g->WT);
+  const unsigned rblocks = (unsigned)((kMlpParams + 63) / 64) + (stats ? 1u : 0u);
+  hipLaunchKernelGGL(k_mlp_reduce_adam, dim3(rblocks), dim3(kRedWaves * 64), 0, st, g->part, blocks, g->grad,
+                     g->params, g->m, g->v, std::max(g->adam_step, 1), lr, update ? 1 : 0, g->W, g->WT, g->loss,
+                     g->correct, rows, stats);
 }
 
 struct GuidedRun {
@@ -1561,8 +1596,7 @@ int cuhre_guide_train(cuhre_guide* g, int dim, int epochs, int batch, float lr, 
     (void)hipMemsetAsync(g->d_stats, 0, sizeof(double) * 4, st);
     for (int64_t k = 0; k < steps; ++k) {
       train_step(g, g->ds_feat, g->ds_axis, g->ds_fin, g->order + k * batch, batch, dim, lr, true, st,
-                 mix64(seed ^ (0xA0761D6478BD642Full * (uint64_t)(ep * steps + k + 1))));
-      hipLaunchKernelGGL(k_train_stats, dim3(16), dim3(256), 0, st, g->loss, g->correct, (int64_t)batch, g->d_stats);
+                 mix64(seed ^ (0xA0761D6478BD642Full * (uint64_t)(ep * steps + k + 1))), g->d_stats);
     }
     if (hipGetLastError() != hipSuccess ||
         hipMemcpyAsync(acc_h, g->d_stats, sizeof acc_h, hipMemcpyDeviceToHost, st) != hipSuccess ||
