@@ -1,10 +1,11 @@
 // mlp.hpp -- the learned split guide's network on the MFMA units.
 //
 // 32 -> 64 -> 64 -> 64 -> 16, ReLU.  BF16 weights and BF16 activations enter
-// v_mfma_f32_16x16x16_bf16 (FP32 accumulation).  One wavefront owns 16 rows
-// (regions / samples); a 256-thread block holds 64 rows and their
-// activations in LDS.  Output columns 0..d-1 are split-axis logits, column 15
-// the finish logit.
+// v_mfma_f32_16x16x16_bf16 (FP32 accumulation).  Two forms of the forward
+// pass: inference (mlp_forward64_wave) runs 64 rows on one wavefront, layer
+// by layer, in LDS of its own; training (mlp_forward16) gives each wavefront
+// of a 256-thread block 16 of the block's 64 rows.  Output columns 0..d-1
+// are split-axis logits, column 15 the finish logit.
 //
 // Operand layout of v_mfma_f32_16x16x16_bf16 (lane l, item v = 0..3):
 //   A[i][k]: i = l % 16, k = 4*(l/16) + v
@@ -17,7 +18,7 @@
 namespace cuhre {
 
 constexpr int kMlpIn = 32, kMlpHid = 64, kMlpOut = 16, kMlpFinish = 15;
-constexpr int kMlpRows = 64;  // rows per 256-thread block
+constexpr int kMlpRows = 64;  // rows per tile (a training block, an inference wavefront)
 // parameter vector: W0 [32x64], W1 [64x64], W2 [64x64], W3 [64x16], b0, b1, b2, b3
 constexpr int kMlpK[4] = {32, 64, 64, 64};
 constexpr int kMlpN[4] = {64, 64, 64, 16};
@@ -88,34 +89,33 @@ __device__ __forceinline__ void mlp_store_relu(const f32x4 (&acc)[N / 16], const
     }
 }
 
-// Forward pass of this wave's 16 rows.  Xs..H3s are this wave's LDS rows
-// (padded strides); Ys receives the 16 x 16 logits in FP32.  The weights
-// come from W^T in global memory (L2-resident, 22 KB): staging them in LDS
-// per block was measured slower.  Every wave of the block calls it together
-// (block barriers between layers); a wave with `work` false (wave-uniform)
-// only takes the barriers and leaves its buffers as they are.
+// Forward pass of this wave's 16 rows (the training kernel's form, whose
+// backward pass reuses each wave's activations).  Xs..H3s are this wave's
+// LDS rows (padded strides); Ys receives the 16 x 16 logits in FP32.  The
+// weights come from W^T in global memory (L2-resident, 22 KB).  Every wave
+// of the block calls it together (block barriers between layers).
 __device__ __forceinline__ void mlp_forward16(const MlpDev& M, const uint16_t* Xs, uint16_t* H1s, uint16_t* H2s,
-                                              uint16_t* H3s, float* Ys, bool work = true)
+                                              uint16_t* H3s, float* Ys)
 {
-  if (work) {
+  {
     f32x4 acc[4];
     mlp_layer_fwd<32, 64, 32>(Xs, M.WT + kMlpWOff[0], acc);
     mlp_store_relu<64>(acc, M.b + 0, H1s);
   }
   __syncthreads();
-  if (work) {
+  {
     f32x4 acc[4];
     mlp_layer_fwd<64, 64, 64>(H1s, M.WT + kMlpWOff[1], acc);
     mlp_store_relu<64>(acc, M.b + 64, H2s);
   }
   __syncthreads();
-  if (work) {
+  {
     f32x4 acc[4];
     mlp_layer_fwd<64, 64, 64>(H2s, M.WT + kMlpWOff[2], acc);
     mlp_store_relu<64>(acc, M.b + 128, H3s);
   }
   __syncthreads();
-  if (work) {
+  {
     f32x4 acc[1];
     mlp_layer_fwd<64, 16, 64>(H3s, M.WT + kMlpWOff[3], acc);
     const int l = __lane_id(), col = l & 15, r0 = (l >> 4) * 4;
