@@ -9,8 +9,11 @@ is whole-step evals/s = sum(neval) / wall time.  Inputs (the integrand
 parameters) are a few hundred bytes; the region list is created on the GPU.
 
 Also reported: the roofline of the dominant kernel (one-region-per-lane rule
-evaluation of the product peak, FP64-bound), and the CPU baseline (the oracle
-port, OpenMP over the host cores) on the same workload.
+evaluation of the product peak, FP64-bound), the CPU baseline (the oracle
+port, OpenMP over the host cores) on the same workload, and the time to
+epsrel 1e-6 on the 8-D product peak (maxeval 1e12) of the plain method
+against the learned split guide, trained in the same run on product peaks
+with other centres (learned_vs_plain_8d).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -170,13 +173,26 @@ def _learned_family(ctx, dim, train, held, maxeval, train_maxeval, samples_per_r
     return {"samples": n, "collect_and_train_s": t_learn, "train": log, "held_out": rows, "maxeval": maxeval}
 
 
+PEAK8_POLICIES = {
+    "plain": None,
+    "shared_allotment": dict(axis_mode=0, finish_mode=3, budget=0.5),
+    "learned_axis": dict(axis_mode=1, finish_mode=0, axis_confidence=0.6),
+    "learned_axis_and_allotment": dict(axis_mode=1, finish_mode=3, budget=0.5, axis_confidence=0.6)}
+
+
+def bench_learned_peak8(ctx, policies=PEAK8_POLICIES):
+    """8-D product peaks, epsrel 1e-6: the guide trained on centres 200..211
+    (maxeval 1e10, 400k samples a run), held out: the 8-D product peak of
+    the main benchmark, run to convergence (maxeval 1e12)."""
+    return _learned_family(ctx, 8, [W.c2_product_peak8(s) for s in range(200, 212)], [W.c2_product_peak8()],
+                           10**12, 10**10, 400_000, policies)
+
+
 def bench_learned(ctx):
     """Learned split guide, epsrel 1e-6, trained and tested on the GPU:
     * 7-D corner peaks: training seeds 100..115 (maxeval 1e11, 500k samples
       a run), held-out seeds 6, 7, 8;
-    * 8-D product peaks: training centres 200..211 (maxeval 1e10, 400k
-      samples a run), held out: the 8-D product peak of the main benchmark,
-      run to convergence (maxeval 1e12)."""
+    * 8-D product peaks (bench_learned_peak8)."""
     corner = _learned_family(
         ctx, 7, W.c5_train_problems(100, 16), [W.c5_corner7(s) for s in (6, 7, 8)], 10**11, 10**11, 500_000,
         {"plain": None,
@@ -184,13 +200,7 @@ def bench_learned(ctx):
          "learned_axis_and_finish": dict(axis_mode=1, finish_mode=1, axis_confidence=0.6, budget=0.25),
          "volume_guard": dict(axis_mode=0, finish_mode=2, budget=0.25),
          "shared_allotment": dict(axis_mode=0, finish_mode=3, budget=0.5)})
-    peak = _learned_family(
-        ctx, 8, [W.c2_product_peak8(s) for s in range(200, 212)], [W.c2_product_peak8()], 10**12, 10**10, 400_000,
-        {"plain": None,
-         "shared_allotment": dict(axis_mode=0, finish_mode=3, budget=0.5),
-         "learned_axis": dict(axis_mode=1, finish_mode=0, axis_confidence=0.6),
-         "learned_axis_and_allotment": dict(axis_mode=1, finish_mode=3, budget=0.5, axis_confidence=0.6)})
-    return {"corner_peak7": corner, "product_peak8": peak}
+    return {"corner_peak7": corner, "product_peak8": bench_learned_peak8(ctx)}
 
 
 def bench_reference(ctx):
@@ -272,7 +282,10 @@ def main():
     ap.add_argument("--batched", action="store_true",
                     help="also time the 4096-integral batched form against sequential plain calls")
     ap.add_argument("--learned", action="store_true",
-                    help="also train the split guide and compare it with the plain method on 7-D corner peaks")
+                    help="also train the split guide and compare it with the plain method on 7-D corner peaks "
+                         "(and 8-D product peaks, which the default run includes)")
+    ap.add_argument("--no-learned8", action="store_true",
+                    help="skip the default learned-vs-plain comparison on the 8-D product peak")
     ap.add_argument("--reference", action="store_true",
                     help="also run the reference's own method (degree-9 rule, two phases) beside the plain one")
     ap.add_argument("--sweep", action="store_true",
@@ -385,6 +398,18 @@ def main():
         out["batched"] = bench_batched(ctx)
     if args.learned:
         out["learned"] = bench_learned(ctx)
+    if not args.no_learned8:
+        # the headline comparison: time to epsrel 1e-6 on the 8-D product
+        # peak, learned guide (trained here, on other centres) vs plain
+        pk = out["learned"]["product_peak8"] if args.learned else bench_learned_peak8(ctx)
+        row = pk["held_out"][0]
+        out["learned_vs_plain_8d"] = {
+            "integrand": row["integrand"], "epsrel": 1e-6, "maxeval": pk["maxeval"],
+            "training": {"samples": pk["samples"], "collect_and_train_s": pk["collect_and_train_s"]},
+            **{name: {k: row[name][k] for k in ("rc", "nregions", "ms", "integral", "error")
+                      if k in row[name]} | {k: row[name][k] for k in ("regions_vs_plain", "time_vs_plain")
+                                            if k in row[name]}
+               for name in PEAK8_POLICIES}}
     if args.reference:
         out["reference_method"] = bench_reference(ctx)
     if args.lclt:
