@@ -161,6 +161,7 @@ def _learned_family(ctx, dim, train, held, maxeval, train_maxeval, samples_per_r
                     best = r
             row[name] = {"rc": best["rc"], "nregions": best["nregions"], "ms": best["ms_total"],
                          "integral": best["integral"], "error": best["error"],
+                         "fp64_err_bound": best["fp64_err_bound"],
                          "retired": best.get("guide", {}).get("retired", 0),
                          "overridden": best.get("guide", {}).get("overridden", 0)}
         for name in policies:
@@ -406,7 +407,7 @@ def main():
         out["learned_vs_plain_8d"] = {
             "integrand": row["integrand"], "epsrel": 1e-6, "maxeval": pk["maxeval"],
             "training": {"samples": pk["samples"], "collect_and_train_s": pk["collect_and_train_s"]},
-            **{name: {k: row[name][k] for k in ("rc", "nregions", "ms", "integral", "error")
+            **{name: {k: row[name][k] for k in ("rc", "nregions", "ms", "integral", "error", "fp64_err_bound")
                       if k in row[name]} | {k: row[name][k] for k in ("regions_vs_plain", "time_vs_plain")
                                             if k in row[name]}
                for name in PEAK8_POLICIES}}
