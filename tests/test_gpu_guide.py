@@ -85,6 +85,47 @@ def test_guided_without_guide_is_the_plain_method(ctx):
         assert a["guide"]["retired"] == 0
 
 
+def forced_axis_params(k, logit=10.0):
+    # zero weights and hidden biases: every region's logits are the last
+    # layer's biases, axis k at `logit`, the others 0, so the network picks
+    # axis k with softmax confidence 1 / (1 + (d - 1) e^-logit)
+    p = np.zeros(cuhre.GUIDE_PARAMS, dtype=np.float32)
+    p[BOFF[3] + k] = logit
+    return p
+
+
+@pytest.mark.parametrize("k", [0, 2])
+def test_guided_axis_override_follows_the_network(ctx, k):
+    # f = x_k^6 has non-zero fourth differences along axis k only, so the
+    # plain method always bisects axis k
+    expo = [0, 0, 0]
+    expo[k] = 6
+    p = W.Problem("x^6", W.polynomial(3, [(1.0, expo)]), [0.0] * 3, [1.0] * 3, 1e-9, 0.0, 10**7)
+    plain = ctx.integrate(p)
+    g = cuhre.Guide(ctx, seed=1)
+    conf = 1.0 / (1.0 + 2.0 * np.exp(-10.0))  # 0.999909
+    # the network agrees: nothing is overridden, the run is the plain run
+    g.set_params(forced_axis_params(k))
+    r = g.integrate(p, axis_mode=1, finish_mode=0, axis_confidence=0.6)
+    assert r["guide"]["overridden"] == 0
+    for key in ("nregions", "iterations", "integral", "error", "rc"):
+        assert r[key] == plain[key], (key, r[key], plain[key])
+    # the network disagrees and is confident enough: every active region of
+    # the first iteration is bisected along its axis instead
+    other = (k + 1) % 3
+    g.set_params(forced_axis_params(other))
+    r = g.integrate(p, axis_mode=1, finish_mode=0, axis_confidence=0.9999)
+    assert conf >= 0.9999
+    assert r["guide"]["overridden"] >= 64  # the 4^3 initial regions at least
+    assert r["nregions"] != plain["nregions"] or r["error"] != plain["error"]
+    # ... but below the confidence threshold it is ignored
+    r = g.integrate(p, axis_mode=1, finish_mode=0, axis_confidence=0.99995)
+    assert conf < 0.99995
+    assert r["guide"]["overridden"] == 0
+    for key in ("nregions", "iterations", "integral", "error", "rc"):
+        assert r[key] == plain[key], (key, r[key], plain[key])
+
+
 CORNER5 = W.Problem("corner5", W.corner_peak([0.7, 1.5, 2.0, 3.0, 4.0], 6), [0.0] * 5, [1.0] * 5, 1e-6, 0.0,
                     10**9)
 
