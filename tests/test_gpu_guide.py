@@ -130,6 +130,30 @@ CORNER5 = W.Problem("corner5", W.corner_peak([0.7, 1.5, 2.0, 3.0, 4.0], 6), [0.0
                     10**9)
 
 
+def test_guided_finish_follows_the_network(ctx):
+    # finish logit (output 15) forced: at +10 the network proposes every
+    # region (probability 0.99995), so finish_mode 1 is the guard alone
+    # (finish_mode 2); at -10 it proposes none, so nothing is retired and
+    # the run is the plain run
+    g = cuhre.Guide(ctx, seed=1)
+    for p in (W.c1_gaussian3(), CORNER5):
+        guard = g.integrate(p, axis_mode=0, finish_mode=2, budget=0.25)
+        plain = ctx.integrate(p)
+        q = np.zeros(cuhre.GUIDE_PARAMS, dtype=np.float32)
+        q[BOFF[3] + 15] = 10.0
+        g.set_params(q)
+        r = g.integrate(p, axis_mode=0, finish_mode=1, finish_threshold=0.5, budget=0.25)
+        assert r["guide"]["retired"] == guard["guide"]["retired"] > 0
+        for key in ("nregions", "iterations", "integral", "error", "rc"):
+            assert r[key] == guard[key], (p.name, key, r[key], guard[key])
+        q[BOFF[3] + 15] = -10.0
+        g.set_params(q)
+        r = g.integrate(p, axis_mode=0, finish_mode=1, finish_threshold=0.5, budget=0.25)
+        assert r["guide"]["retired"] == 0
+        for key in ("nregions", "iterations", "integral", "error", "rc"):
+            assert r[key] == plain[key], (p.name, key, r[key], plain[key])
+
+
 @pytest.mark.parametrize("finish_mode", [1, 2])
 def test_guided_finish_keeps_the_certificate(ctx, finish_mode):
     # 1: an untrained network proposes finishes, the guard and the room decide;
