@@ -616,6 +616,25 @@ __device__ __forceinline__ void cp_add(ChunkPart& a, const ChunkPart& b)
   a.no += b.no;
 }
 
+// the partner lane's ChunkPart (xor butterfly)
+__device__ __forceinline__ ChunkPart cp_shfl_xor(const ChunkPart& q, int mask)
+{
+  ChunkPart r;
+  r.cnt = __shfl_xor((long long)q.cnt, mask, kWave);
+  r.all = __shfl_xor((long long)q.all, mask, kWave);
+  r.log = __shfl_xor((long long)q.log, mask, kWave);
+  r.pad = 0;
+  r.es = shfl_xor_d(q.es, mask);
+  r.ec = shfl_xor_d(q.ec, mask);
+  r.rs = shfl_xor_d(q.rs, mask);
+  r.rc = shfl_xor_d(q.rc, mask);
+  r.ab = shfl_xor_d(q.ab, mask);
+  r.ae = shfl_xor_d(q.ae, mask);
+  r.nr = shfl_xor_d(q.nr, mask);
+  r.no = shfl_xor_d(q.no, mask);
+  return r;
+}
+
 __device__ __forceinline__ void cp_tile(ChunkPart& q, const int64_t* cnt, const int64_t* logcnt, const double* ret,
                                         int64_t t)
 {
@@ -656,8 +675,8 @@ __global__ __launch_bounds__(256) void k_gscan_top(const ChunkPart* part, int64_
                                                    int64_t* choff, int64_t* chlog, GuideSums* out, int* retire_on)
 {
   if (scan_stopped(RA.state)) return;
-  __shared__ ChunkPart sp[256];
-  __shared__ int64_t sa[256], sb[256], sc[256];
+  __shared__ ChunkPart sp[kMaxWaves];
+  __shared__ int64_t sa[kMaxWaves], sb[kMaxWaves], sc[kMaxWaves];
   __shared__ int s_allow;
   double room = RA.room;
   if (RA.state != nullptr) {
@@ -668,31 +687,45 @@ __global__ __launch_bounds__(256) void k_gscan_top(const ChunkPart* part, int64_
            : RA.mode != 0 ? fmin(RA.budget * tol - RA.ret_prev, (1.0 - RA.budget) * tol - frozen)
                           : -1.0;
   }
-  const int T = blockDim.x;
+  const int T = blockDim.x, W = T / kWave;
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
   const int64_t per = (nchunks + T - 1) / T;
   const int64_t c0 = threadIdx.x * per;
   const int64_t c1 = c0 + per < nchunks ? c0 + per : nchunks;
   ChunkPart q;
   cp_zero(q);
   for (int64_t c = c0; c < c1; ++c) cp_add(q, part[c]);
-  sp[threadIdx.x] = q;
-  sa[threadIdx.x] = q.cnt;
-  sb[threadIdx.x] = q.all;
-  sc[threadIdx.x] = q.log;
+  // inclusive scans of the three counts within the wave, the wave's
+  // ChunkPart total by a butterfly (a fixed order), then across the waves
+  long long ia = q.cnt, ib = q.all, ic = q.log;
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const long long va = __shfl_up(ia, off, kWave), vb = __shfl_up(ib, off, kWave), vc = __shfl_up(ic, off, kWave);
+    if (lane >= off) {
+      ia += va;
+      ib += vb;
+      ic += vc;
+    }
+  }
+  ChunkPart t = q;
+#pragma unroll
+  for (int mask = 1; mask < kWave; mask <<= 1) cp_add(t, cp_shfl_xor(t, mask));
+  if (lane == kWave - 1) {
+    sa[wave] = ia;
+    sb[wave] = ib;
+    sc[wave] = ic;
+  }
+  if (lane == 0) sp[wave] = t;
   __syncthreads();
-  for (int o = 1; o < T; o <<= 1) {
-    const int64_t va = (int)threadIdx.x >= o ? sa[threadIdx.x - o] : 0;
-    const int64_t vb = (int)threadIdx.x >= o ? sb[threadIdx.x - o] : 0;
-    const int64_t vc = (int)threadIdx.x >= o ? sc[threadIdx.x - o] : 0;
-    __syncthreads();
-    sa[threadIdx.x] += va;
-    sb[threadIdx.x] += vb;
-    sc[threadIdx.x] += vc;
-    __syncthreads();
+  int64_t pa = ia - q.cnt, pb = ib - q.all, pc = ic - q.log;  // exclusive, within the wave
+  for (int w = 0; w < wave; ++w) {
+    pa += sa[w];
+    pb += sb[w];
+    pc += sc[w];
   }
   if (threadIdx.x == 0) {
     ChunkPart tot = sp[0];
-    for (int i = 1; i < T; ++i) cp_add(tot, sp[i]);
+    for (int w = 1; w < W; ++w) cp_add(tot, sp[w]);
     const int allow = tot.nr > 0.0 && tot.rs + tot.rc <= room;
     GuideSums g;
     g.n_logged = tot.log;
@@ -717,8 +750,8 @@ __global__ __launch_bounds__(256) void k_gscan_top(const ChunkPart* part, int64_
   }
   __syncthreads();
   const int allow = s_allow;
-  int64_t ra = allow ? sa[threadIdx.x] - q.cnt : sb[threadIdx.x] - q.all;
-  int64_t rb = sc[threadIdx.x] - q.log;
+  int64_t ra = allow ? pa : pb;
+  int64_t rb = pc;
   for (int64_t c = c0; c < c1; ++c) {
     choff[c] = ra;
     chlog[c] = rb;
