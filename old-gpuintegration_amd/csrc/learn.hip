@@ -397,6 +397,22 @@ struct GuideSums {
   double est_s, est_c, err_s, err_c, abs_r, aest_r;
 };
 
+// What one guided iteration hands the host through coherent host memory:
+// the iteration's state and the guide step's sums, valid once the sequence
+// word next to it holds the iteration's number.
+struct GuidePub {
+  IterState st;
+  GuideSums sums;
+};
+
+__device__ __forceinline__ void publish_guided(GuidePub* pub, uint32_t* pub_seq, uint32_t seq, const IterState& st,
+                                               const GuideSums& g)
+{
+  pub->st = st;
+  pub->sums = g;
+  __hip_atomic_store(pub_seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 __device__ __forceinline__ bool guide_selected(const GuideArgs& G, int64_t r, bool active)
 {
   return active && G.log_stride > 0 && (mix64((uint64_t)r ^ G.log_salt) % G.log_stride) == 0;
@@ -671,10 +687,19 @@ __global__ __launch_bounds__(256) void k_gscan_part(const int64_t* cnt, const in
   if (threadIdx.x == 0) part[blockIdx.x] = sp[0];
 }
 
+// With `pub` set (and RA.state), it also publishes the state and the sums
+// to the host under sequence number `seq`, stopped iterations included.
 __global__ __launch_bounds__(256) void k_gscan_top(const ChunkPart* part, int64_t nchunks, RoomArgs RA,
-                                                   int64_t* choff, int64_t* chlog, GuideSums* out, int* retire_on)
+                                                   int64_t* choff, int64_t* chlog, GuideSums* out, int* retire_on,
+                                                   GuidePub* pub, uint32_t* pub_seq, uint32_t seq)
 {
-  if (scan_stopped(RA.state)) return;
+  if (scan_stopped(RA.state)) {
+    if (pub != nullptr && threadIdx.x == 0) {
+      GuideSums z = {};
+      publish_guided(pub, pub_seq, seq, *RA.state, z);
+    }
+    return;
+  }
   __shared__ ChunkPart sp[kMaxWaves];
   __shared__ int64_t sa[kMaxWaves], sb[kMaxWaves], sc[kMaxWaves];
   __shared__ int s_allow;
@@ -747,6 +772,7 @@ __global__ __launch_bounds__(256) void k_gscan_top(const ChunkPart* part, int64_
     *out = g;
     *retire_on = allow;
     s_allow = allow;
+    if (pub != nullptr) publish_guided(pub, pub_seq, seq, *RA.state, g);
   }
   __syncthreads();
   const int allow = s_allow;
@@ -915,6 +941,12 @@ struct cuhre_guide {
   ChunkPart* chunk_part = nullptr;  // scan chunks: partial sums and offsets
   int64_t *chunk_off = nullptr, *chunk_log = nullptr;
   GuideSums* h_sums = nullptr;
+  // the guided loop's published iteration (coherent host memory)
+  GuidePub* pub = nullptr;
+  uint32_t* pub_seq = nullptr;
+  GuidePub* pub_dev = nullptr;
+  uint32_t* pub_seq_dev = nullptr;
+  uint32_t seq = 0;
   // training scratch
   int64_t train_cap = 0;
   int32_t* order = nullptr;
@@ -1126,6 +1158,22 @@ struct GuidedRun {
   int64_t collect_max;  // collection: samples wanted from this run (0: none)
 };
 
+// Waits until the guide step's scan has published iteration g->seq (a
+// stream query every ~2^18 polls guards against a launch that never does).
+int wait_guided(cuhre_guide* g, hipStream_t st)
+{
+  for (uint64_t spin = 0;; ++spin) {
+    if (__atomic_load_n(g->pub_seq, __ATOMIC_ACQUIRE) == g->seq) return 0;
+    __builtin_ia32_pause();
+    if ((spin & 262143) == 262143) {
+      const hipError_t q = hipStreamQuery(st);
+      if (q == hipSuccess && __atomic_load_n(g->pub_seq, __ATOMIC_ACQUIRE) != g->seq)
+        return set_error(CUHRE_ERR_DEVICE, "guide step ended without publishing its state");
+      if (q != hipSuccess && q != hipErrorNotReady) return set_error(CUHRE_ERR_DEVICE, hipGetErrorString(q));
+    }
+  }
+}
+
 // The plain loop of engine.hip plus the guide step between evaluation and
 // split; with collect_max > 0 it also logs samples and their labels.
 int guided_run(cuhre_context* c, cuhre_guide* g, const cuhre_integrand* f, int dim, double epsrel, double epsabs,
@@ -1259,8 +1307,11 @@ int guided_run(cuhre_context* c, cuhre_guide* g, const cuhre_integrand* f, int d
       const int64_t nchunks = (ntiles + kScanChunk - 1) / kScanChunk;
       hipLaunchKernelGGL(k_gscan_part, dim3((unsigned)nchunks), dim3(256), 0, st, g->tile_cnt, g->tile_logcnt,
                          g->tile_ret, ntiles, RA.state, g->chunk_part);
+      const bool publish = dev && run.collect_max == 0;
+      if (publish) ++g->seq;
       hipLaunchKernelGGL(k_gscan_top, dim3(1), dim3(256), 0, st, g->chunk_part, nchunks, RA, g->chunk_off,
-                         g->chunk_log, g->d_sums, g->d_retire_on);
+                         g->chunk_log, g->d_sums, g->d_retire_on, publish ? g->pub_dev : nullptr,
+                         publish ? g->pub_seq_dev : nullptr, g->seq);
       hipLaunchKernelGGL(k_gscan_tiles, dim3((unsigned)nchunks), dim3(256), 0, st, g->tile_cnt, g->tile_logcnt,
                          g->tile_ret, ntiles, RA.state, g->chunk_off, g->chunk_log, g->d_retire_on, g->tile_off,
                          g->tile_logoff);
@@ -1272,12 +1323,19 @@ int guided_run(cuhre_context* c, cuhre_guide* g, const cuhre_integrand* f, int d
       launch_split(SA, st);
     };
     const bool sampling = run.collect_max > 0;
-    if (!sampling) guide_step(true);
-    if (hipMemcpyAsync(c->h_state, c->state, sizeof(IterState), hipMemcpyDeviceToHost, st) != hipSuccess ||
-        (!sampling && hipMemcpyAsync(g->h_sums, g->d_sums, sizeof(GuideSums), hipMemcpyDeviceToHost, st) !=
-                          hipSuccess) ||
-        hipStreamSynchronize(st) != hipSuccess)
+    GuideSums gsum_pub = {};
+    if (!sampling) {
+      // the guide step publishes this iteration's state and sums; the host
+      // queues the next iteration while the scan of tiles and the split run
+      guide_step(true);
+      if (hipGetLastError() != hipSuccess) return set_error(CUHRE_ERR_DEVICE, "guided iteration failed");
+      if ((rc = wait_guided(g, st))) return rc;
+      *c->h_state = g->pub->st;
+      gsum_pub = g->pub->sums;
+    } else if (hipMemcpyAsync(c->h_state, c->state, sizeof(IterState), hipMemcpyDeviceToHost, st) != hipSuccess ||
+               hipStreamSynchronize(st) != hipSuccess) {
       return set_error(CUHRE_ERR_DEVICE, "guided iteration failed");
+    }
     IterState s = *c->h_state;
     out->eval_launches++;
     out->iterations = it + 1;
@@ -1330,7 +1388,7 @@ int guided_run(cuhre_context* c, cuhre_guide* g, const cuhre_integrand* f, int d
           hipStreamSynchronize(st) != hipSuccess)
         return set_error(CUHRE_ERR_DEVICE, "guide step failed");
     }
-    const GuideSums gsum = *g->h_sums;
+    const GuideSums gsum = rerun ? *g->h_sums : gsum_pub;
     if (G.log_stride > 0 && gsum.n_logged > 0) {
       const int64_t L = std::min<int64_t>(gsum.n_logged, run.collect_max - logged);
       if ((rc = guide_reserve_dataset(g, g->ds_n + gsum.n_logged))) return rc;
@@ -1400,6 +1458,9 @@ int guided_run(cuhre_context* c, cuhre_guide* g, const cuhre_integrand* f, int d
       return set_error(CUHRE_ERR_DEVICE, "finish labelling failed");
   }
   if (gs) gs->samples = g->ds_n - ds_base0;
+  // the last iteration's tile scan and split (they skip themselves once the
+  // run stopped) may still be queued
+  if (hipStreamSynchronize(st) != hipSuccess) return set_error(CUHRE_ERR_DEVICE, "guided run failed");
   out->fail = result > 0 ? result : 0;
   out->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return result;
@@ -1442,11 +1503,16 @@ int cuhre_guide_create(cuhre_context* ctx, uint64_t seed, cuhre_guide** out)
       hipMalloc(&g->d_retire_on, sizeof(int)) != hipSuccess ||
 
       hipMalloc(&g->d_stats, sizeof(double) * 4) != hipSuccess ||
-      hipHostMalloc(&g->h_sums, sizeof(GuideSums), hipHostMallocDefault) != hipSuccess) {
+      hipHostMalloc(&g->h_sums, sizeof(GuideSums), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc(&g->pub, sizeof(GuidePub), hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
+      hipHostMalloc(&g->pub_seq, sizeof(uint32_t), hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
+      hipHostGetDevicePointer((void**)&g->pub_dev, g->pub, 0) != hipSuccess ||
+      hipHostGetDevicePointer((void**)&g->pub_seq_dev, g->pub_seq, 0) != hipSuccess) {
     (void)hipGetLastError();
     cuhre_guide_destroy(g);
     return set_error(CUHRE_ERR_ALLOC, "cannot allocate the guide");
   }
+  *g->pub_seq = 0;
   std::vector<float> p;
   init_params(p, seed);
   const int rc = upload_params(g, p.data());
@@ -1476,6 +1542,8 @@ int cuhre_guide_destroy(cuhre_guide* g)
                   (void*)g->order, (void*)g->loss, (void*)g->correct, (void*)g->d_stats})
     gfree(q);
   if (g->h_sums) (void)hipHostFree(g->h_sums);
+  if (g->pub) (void)hipHostFree(g->pub);
+  if (g->pub_seq) (void)hipHostFree(g->pub_seq);
   delete g;
   return 0;
 }
