@@ -254,3 +254,31 @@ def test_training_learns_a_rule(ctx):
     print("training", st)
     assert st["finish_accuracy"] >= 0.8
     assert st["axis_accuracy"] >= 0.4
+
+
+def test_trained_network_gains_on_product_peak8(ctx):
+    # what the trained network adds: collected and trained on 8-D product
+    # peaks with other centres, then run on the held-out peak to epsrel 1e-6
+    g = cuhre.Guide(ctx, seed=11)
+    train = [W.c2_product_peak8(s) for s in range(200, 212)]
+    for rnd in range(3):
+        for p in train:
+            g.collect(p, max_samples=400_000, maxeval=10**10, axis_mode=0 if rnd == 0 else 1, axis_confidence=0.6)
+        for ep, lr in ((2, 2e-3), (2, 1e-3), (1, 3e-4)):
+            g.train(8, epochs=ep, batch=8192, lr=lr, seed=10 * rnd + ep)
+    p = W.c2_product_peak8()
+    ex, _ = O.exact(p.f, p.lows, p.highs)
+    plain = ctx.integrate(p, maxeval=10**12)
+    allot = g.integrate(p, axis_mode=0, finish_mode=3, budget=0.5, maxeval=10**12)
+    learned = g.integrate(p, axis_mode=1, finish_mode=3, budget=0.5, axis_confidence=0.6, maxeval=10**12)
+    print("regions plain / allotment / learned + allotment:", plain["nregions"], allot["nregions"],
+          learned["nregions"])
+    for r in (plain, allot, learned):
+        assert r["rc"] == 0, r
+        assert r["error"] <= p.epsrel * abs(r["integral"])
+        assert abs(r["integral"] - ex) <= r["error"]
+    assert learned["guide"]["overridden"] > 0
+    # the network's axes on top of the allotment: at most 0.8x its regions,
+    # and at most half the plain method's
+    assert learned["nregions"] <= 0.8 * allot["nregions"]
+    assert learned["nregions"] <= 0.5 * plain["nregions"]
