@@ -105,7 +105,10 @@ typedef struct {
   int32_t split_launches;
   double eval_flops;     /* algorithmic FP64 flops of the eval launches */
   double ms_eval_largest;  /* device time of the largest eval launch    */
-  int32_t pad0, pad1;
+  int32_t nzeroed;       /* two-phase: regions whose error passed 1e10 and
+                            were zeroed (the reference's rule); the call
+                            then returns 1 even when the sums pass       */
+  int32_t pad1;
   int64_t largest_regions; /* regions of that launch                   */
 } cuhre_result;
 
@@ -211,8 +214,11 @@ typedef struct {
 
 /* count instances; fs[count]; epsrel/epsabs/maxeval per instance (NULL:
  * 1e-3 / 0 / 1e9); lows/highs count*dim (NULL: unit cube); out[count].
- * Returns 0 when every instance converged, 1 if some did not, <0 on error
- * (a non-finite integrand value marks that instance and returns -2). */
+ * Returns 0 when every instance converged, 1 if some did not, <0 on error.
+ * out[i].fail: 0 converged, 1 not converged, CUHRE_ERR_NONFINITE (-2) when
+ * instance i's integrand returned a non-finite value; the call then returns
+ * -2 with every other instance's result complete (and counted in
+ * info->not_converged when it did not converge). */
 int cuhre_integrate_batch(cuhre_context* ctx, int32_t count, const cuhre_integrand* fs, int dim,
                           const double* epsrel, const double* epsabs, const int64_t* maxeval,
                           const double* lows, const double* highs,

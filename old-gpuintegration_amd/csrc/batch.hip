@@ -206,7 +206,6 @@ __global__ __launch_bounds__(kBlock) void k_batch_reduce(BatchBuffers B)
   }
   block_reduce_partial(q, ldsw, kBlock / kWave);
   if (threadIdx.x != 0) return;
-  const double u = 0.5 * 2.220446049250313e-16;
   const int64_t p = B.npts;
   double rs = st.fin_s, rc = st.fin_c, es = st.fer_s, ec = st.fer_c;
   dd_add(rs, rc, q.est_s, q.est_c);
@@ -218,8 +217,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_reduce(BatchBuffers B)
   st.peak = st.count > st.peak ? st.count : st.peak;
   st.integral = rG;
   st.error = eG;
-  st.bound = (double)(p + 4) * u * (st.abs_fin + q.abs_all) + 2.0 * u * fabs(rG) +
-             (double)(st.nfin + st.count) * u * u * (st.aest_fin + q.aest_all);
+  st.bound = fp64_rounding_bound(p, B.dim, st.nfin + st.count, st.abs_fin + q.abs_all, st.aest_fin + q.aest_all, rG);
   dd_add(st.fin_s, st.fin_c, q.fest_s, q.fest_c);
   dd_add(st.fer_s, st.fer_c, q.ferr_s, q.ferr_c);
   st.abs_fin += q.abs_fin;
@@ -1018,14 +1016,16 @@ int cuhre_integrate_batch(cuhre_context* ctx, int32_t count, const cuhre_integra
     r.fp64_err_bound = s.bound;
     r.nregions = s.nregions;
     r.neval = s.nregions * p;
-    r.fail = s.fail > 0 ? s.fail : (s.done ? 0 : 1);
+    // per instance: 0 converged, 1 not converged, CUHRE_ERR_NONFINITE (-2)
+    // when its integrand returned a non-finite value
+    r.fail = s.fail != 0 ? s.fail : (s.done ? 0 : 1);
     r.iterations = s.iters;
     r.peak_regions = s.peak;
     r.nfinished = s.nfin;
     total_regions += s.nregions;
     maxit = std::max(maxit, s.iters);
     if (s.fail < 0) worst = s.fail;
-    if (s.fail > 0 || !s.done) ++nonconv;
+    if (s.fail != 0 || !s.done) ++nonconv;
   }
   const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   if (info) {
@@ -1044,7 +1044,9 @@ int cuhre_integrate_batch(cuhre_context* ctx, int32_t count, const cuhre_integra
     info->grows = grows;
   }
   for (int i = 0; i < count; ++i) out[i].ms_total = ms;
-  if (worst < 0) return set_error(worst, "an integrand returned a non-finite value in some instance");
+  if (worst < 0)
+    return set_error(worst, "an integrand returned a non-finite value in some instance (its result's fail "
+                            "is -2; the other results are complete)");
   return nonconv ? CUHRE_NOT_CONVERGED : CUHRE_OK;
 }
 

@@ -105,6 +105,30 @@ struct EvalArgs {
   hipEvent_t prof_start, prof_stop;
 };
 
+// FP64 rounding bound of a result that sums n regions of a p-point rule in
+// d dimensions: it bounds |I - sum_r V_r sum_k w_k f_k| for the FP64
+// integrand values f_k the kernels obtained (the integrand's own rounding
+// belongs to the integrand).  A = sum of the regions' V sum |w f| (abssum),
+// E = sum |est|.  Standard model, u = 2^-53:
+//   rule sums: every term passes through at most p roundings (an FMA chain,
+//     or segment chains merged by adds), so |s^ - s| <= g_p sum |w f|;
+//   est = fl(V^ s^) with V^ the product of d widths: g_{p+d} V sum |w f|;
+//   K = p + d + 2 and the factor 1 + 4(K + n)u absorb the rounding of the
+//     abssums themselves and of their plain FP64 total;
+//   regions: (value, TwoSum error) pairs over any summation tree of depth
+//     <= n: u|T| (<= 2u|I|) plus the compensation's own rounding,
+//     g_{2n} * n u * E = 2 n^2 u^2 / (1 - 2nu) E.
+// (Higham, Accuracy and Stability of Numerical Algorithms, 2nd ed., 3.1 and
+// 4.2; Ogita, Rump and Oishi 2005, Sum2.)
+__host__ __device__ inline double fp64_rounding_bound(int64_t p, int d, int64_t n, double A, double E, double I)
+{
+  const double u = 0x1p-53;
+  const double K = (double)p + (double)d + 2.0;
+  const double gK = K * u / (1.0 - K * u);
+  const double nn = (double)n;
+  return gK * A * (1.0 + 4.0 * (K + nn) * u) + 2.0 * u * fabs(I) + 2.0 * nn * nn * u * u / (1.0 - 2.0 * nn * u) * E;
+}
+
 // Launches the rule evaluation of A.n regions with S point segments.
 using EvalLauncher = void (*)(const EvalArgs&, const DevIntegrand&, const RuleConst&, int S,
                               hipStream_t);

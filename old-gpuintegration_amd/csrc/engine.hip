@@ -917,7 +917,6 @@ int plain_loop(cuhre_context* c, const cuhre_integrand* f, int dim, double epsre
 
   double fin_s = 0.0, fin_c = 0.0, fer_s = 0.0, fer_c = 0.0, fin_abs = 0.0, fin_aest = 0.0;
   int64_t nfin = 0, n = n0;
-  const double u = 0.5 * 2.220446049250313e-16;
   int result = CUHRE_NOT_CONVERGED;
   for (int it = 0; it < opt.max_iter; ++it) {
     const int S = pick_segments(c, n, key);
@@ -1014,8 +1013,7 @@ int plain_loop(cuhre_context* c, const cuhre_integrand* f, int dim, double epsre
     out->nfinished = nfin;
     out->integral = st.rG_s + st.rG_c;
     out->error = st.eG_s + st.eG_c;
-    out->fp64_err_bound = (double)(p + 4) * u * abs_contrib + 2.0 * u * std::fabs(out->integral) +
-                          (double)ncontrib * u * u * aest_contrib;
+    out->fp64_err_bound = fp64_rounding_bound(p, dim, ncontrib, abs_contrib, aest_contrib, out->integral);
     if (st.stop == 4) {
       result = fail(CUHRE_ERR_NONFINITE, "integrand returned a non-finite value in %lld region(s)",
                     (long long)st.tot.nnonfinite);
@@ -1094,7 +1092,7 @@ int cuhre_integrate_two_phase(cuhre_context* c, const cuhre_integrand* f, int di
   const int64_t slots = (int64_t)pool_max + 2;
   const size_t per_slot = sizeof(double) * (2 * (size_t)dim + 2) + sizeof(int8_t) + sizeof(int32_t);
   const size_t ws = (size_t)n * (size_t)slots * per_slot;
-  const size_t outb = (size_t)n * (2 * sizeof(double) + sizeof(int32_t) + 2);
+  const size_t outb = (size_t)n * (4 * sizeof(double) + sizeof(int32_t) + 3) + 8 * 256;
   const size_t need = ws + outb + 8 * 256;
   if (c->p2_bytes < need) {
     size_t free_b = 0, total_b = 0;
@@ -1136,25 +1134,32 @@ int cuhre_integrate_two_phase(cuhre_context* c, const cuhre_integrand* f, int di
   A.cnt = (int32_t*)take(sizeof(int32_t) * n);
   A.act = (int8_t*)take(n);
   A.nonfinite = (int8_t*)take(n);
+  A.zeroed = (int8_t*)take(n);
+  A.rab = (double*)take(sizeof(double) * n);
+  A.rres = (double*)take(sizeof(double) * n);
   A.epsrel = epsrel;
   A.epsabs = epsabs;
   A.pool_max = pool_max;
   int result = 0;
-  std::vector<double> hres(n), herr(n);
+  std::vector<double> hres(n), herr(n), hrab(n), hrres(n);
   std::vector<int32_t> hcnt(n);
-  std::vector<int8_t> hact(n), hbad(n);
+  std::vector<int8_t> hact(n), hbad(n), hzero(n);
   launch(A, P, R, c->stream);
   if (hipGetLastError() != hipSuccess ||
       hipMemcpyAsync(hres.data(), A.res, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
       hipMemcpyAsync(herr.data(), A.err, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      hipMemcpyAsync(hrab.data(), A.rab, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      hipMemcpyAsync(hrres.data(), A.rres, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
       hipMemcpyAsync(hcnt.data(), A.cnt, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
       hipMemcpyAsync(hact.data(), A.act, n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
       hipMemcpyAsync(hbad.data(), A.nonfinite, n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      hipMemcpyAsync(hzero.data(), A.zeroed, n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess)
     result = fail(CUHRE_ERR_DEVICE, "Phase 2 failed: %s", hipGetErrorString(hipGetLastError()));
   if (result) return result;
   double I_s = ho.fin_s, I_c = ho.fin_c, E_s = ho.fer_s, E_c = ho.fer_c;
-  int64_t failed = 0, bad = 0;
+  double ab_tot = ho.fin_abs, res_round = 0.0, aest = ho.fin_aest;
+  int64_t failed = 0, bad = 0, zeroed = 0;
   for (int64_t r = 0; r < n; ++r) {
     dd_add_host(I_s, I_c, hres[r], 0.0);
     dd_add_host(E_s, E_c, herr[r], 0.0);
@@ -1162,17 +1167,34 @@ int cuhre_integrate_two_phase(cuhre_context* c, const cuhre_integrand* f, int di
     out->neval += (int64_t)(hcnt[r] - 1) * 2 * p + p;
     failed += hact[r];
     bad += hbad[r];
+    zeroed += hzero[r];
+    ab_tot += hrab[r];
+    res_round += hrres[r];
+    aest += std::fabs(hres[r]);
   }
   if (bad) return fail(CUHRE_ERR_NONFINITE, "integrand returned a non-finite value in Phase 2 (%lld regions)",
                        (long long)bad);
   out->integral = I_s + I_c;
   out->error = E_s + E_c;
   out->nfinished = ho.nfin + n;
+  out->nzeroed = (int32_t)std::min<int64_t>(zeroed, INT32_MAX);
+  // rounding: every evaluated region's rule sum (an over-count: bisected
+  // members left the sum), each Phase-2 region's RES update chain, and the
+  // sum over the finished Phase-1 regions and the n Phase-2 results
+  {
+    const double u = 0x1p-53;
+    // the budgets are themselves FP64 sums of at most pool_max + 1 and n terms
+    const double fac = 1.0 + 4.0 * ((double)pool_max + (double)n + 2.0) * u;
+    out->fp64_err_bound = fp64_rounding_bound(p, dim, ho.nfin + n, ab_tot, aest, out->integral) +
+                          u * res_round * fac;
+  }
   const bool ok = out->error <= std::max(epsrel * std::fabs(out->integral), epsabs);
-  result = ok ? CUHRE_OK : CUHRE_NOT_CONVERGED;
+  // a zeroed region's contribution is dropped (the reference's rule,
+  // GPUQuadPhases.cu:519-531): the result is then not certified
+  result = ok && zeroed == 0 ? CUHRE_OK : CUHRE_NOT_CONVERGED;
   // the reference's flag: Phase-2 regions that missed their test, 0 when
   // the global test holds (GPUquad.h:713-714)
-  out->fail = ok ? 0 : (int32_t)std::min<int64_t>(std::max<int64_t>(failed, 1), INT32_MAX);
+  out->fail = ok && zeroed == 0 ? 0 : (int32_t)std::min<int64_t>(std::max<int64_t>(failed, 1), INT32_MAX);
   out->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return result;
 }

@@ -1207,7 +1207,6 @@ int guided_run(cuhre_context* c, cuhre_guide* g, const cuhre_integrand* f, int d
   const int64_t ds_base0 = g->ds_n;
   double fin_s = 0, fin_c = 0, fer_s = 0, fer_c = 0, fin_abs = 0, fin_aest = 0, ret_err = 0;
   int64_t nfin = 0, n = n0, logged = 0;
-  const double u = 0.5 * 2.220446049250313e-16;
   int result = CUHRE_NOT_CONVERGED;
   for (int it = 0; it < 1000; ++it) {
     const int S = pick_segments(c, n);
@@ -1368,8 +1367,7 @@ int guided_run(cuhre_context* c, cuhre_guide* g, const cuhre_integrand* f, int d
     }
     const double abs_contrib = fin_abs + s.tot.abs_all;
     const double aest_contrib = fin_aest + s.tot.aest_all;
-    out->fp64_err_bound = (double)(p + 4) * u * abs_contrib + 2.0 * u * std::fabs(out->integral) +
-                          (double)(nfin + n) * u * u * aest_contrib;
+    out->fp64_err_bound = fp64_rounding_bound(p, dim, nfin + n, abs_contrib, aest_contrib, out->integral);
     if (s.stop == 4) {
       result = set_error(CUHRE_ERR_NONFINITE, "integrand returned a non-finite value");
       break;

@@ -44,6 +44,12 @@ struct Phase2Args {
   int32_t* cnt;
   int8_t* act;
   int8_t* nonfinite;
+  int8_t* zeroed;  // the region's error passed 1e10 and it was zeroed
+  // rounding budget per region: sum of the abssums of every region the
+  // pool evaluated, and sum over the RES updates of
+  // 2 (|child 0| + |child 1| + |parent|) + |RES| (each update rounds three times)
+  double* rab;
+  double* rres;
 };
 
 // Pool slots whose keys (error, creation number) live in LDS.
@@ -105,6 +111,7 @@ __global__ __launch_bounds__(64) void k_phase2(Phase2Args A, DevIntegrand P, Rul
   const RegionOut o = p2_eval<D, FAM, KEY>(f, R, lo, wd, lane & 31);
   const double e0 = read_lane_d(o.est, 0), r0 = read_lane_d(o.err, 0);
   bool bad = !(isfinite(e0) && isfinite(r0));
+  double rab = read_lane_d(o.ab, 0), rres = 0.0;
   if (lane == 0) {
 #pragma unroll
     for (int m = 0; m < D; ++m) {
@@ -182,6 +189,8 @@ __global__ __launch_bounds__(64) void k_phase2(Phase2Args A, DevIntegrand P, Rul
     cr[1] += diff;
     ERR += cr[0] + cr[1] - pr;
     RES += ce[0] + ce[1] - pe;
+    rab += read_lane_d(q.ab, 0) + read_lane_d(q.ab, 32);
+    rres += 2.0 * (fabs(ce[0]) + fabs(ce[1]) + fabs(pe)) + fabs(RES);
     __syncthreads();  // every lane has read slot a before it is overwritten
     if ((lane & 31) == 0) {
       const int32_t slot = half ? b : a;
@@ -199,16 +208,21 @@ __global__ __launch_bounds__(64) void k_phase2(Phase2Args A, DevIntegrand P, Rul
   }
   if (lane == 0) {
     int8_t active = ERR > fmax(A.epsrel * fabs(RES), A.epsabs) ? 1 : 0;
+    int8_t zero = 0;
     if (ERR > 1e10) {  // the reference zeroes a diverging region and flags it
       RES = 0.0;
       ERR = 0.0;
       active = 1;
+      zero = 1;
     }
     A.res[g] = RES;
     A.err[g] = ERR;
     A.cnt[g] = count;
     A.act[g] = active;
     A.nonfinite[g] = bad ? 1 : 0;
+    A.zeroed[g] = zero;
+    A.rab[g] = rab;
+    A.rres[g] = rres;
   }
 }
 

@@ -75,7 +75,7 @@ class Result(C.Structure):
         ("split_launches", C.c_int32),
         ("eval_flops", C.c_double),
         ("ms_eval_largest", C.c_double),
-        ("pad0", C.c_int32),
+        ("nzeroed", C.c_int32),
         ("pad1", C.c_int32),
         ("largest_regions", C.c_int64),
     ]
@@ -365,9 +365,11 @@ class Context:
         return count, d, arr, lows, highs, keep
 
     def integrate_batch(self, problems, point_segments=0, grid_blocks=0, iterations_per_check=0,
-                        use_graph=True, initial_capacity=0):
+                        use_graph=True, initial_capacity=0, raise_nonfinite=True):
         """Integrates every problem (same dimension) in one batched call.
-        Returns (results list in input order, info dict, return code)."""
+        Returns (results list in input order, info dict, return code).  With
+        raise_nonfinite=False a non-finite integrand value returns rc -2 with
+        the per-instance results (that instance's fail is -2)."""
         count, d, arr, lows, highs, keep = self._batch_inputs(problems)
         er = np.ascontiguousarray([p.epsrel for p in problems], dtype=np.float64)
         ea = np.ascontiguousarray([p.epsabs for p in problems], dtype=np.float64)
@@ -379,7 +381,8 @@ class Context:
         rc = lib().cuhre_integrate_batch(self.h, count, arr, d, _dp(er), _dp(ea),
                                          me.ctypes.data_as(C.POINTER(C.c_int64)), _dp(lows), _dp(highs),
                                          C.byref(cfg), out, C.byref(info))
-        _check(rc)
+        if raise_nonfinite or rc != -2:
+            _check(rc)
         return [out[i].asdict() for i in range(count)], info.asdict(), rc
 
     def batch_autotune(self, problems):

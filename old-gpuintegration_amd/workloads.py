@@ -193,6 +193,19 @@ def c2_product_peak8(seed: int = 3) -> Problem:
     return Problem("product_peak8d", product_peak([3.0] * 8, w), lo, hi, 1e-6, 0.0, 10**9)
 
 
+def cancelling_oscillatory(d: int = 5, seed: int = 7, epsabs: float = 1e-8) -> Problem:
+    """Oscillatory integrand whose integral over [0,1]^d is zero: a_i = 9/8
+    U(0,1) and u chosen so that 2 pi u + sum a_i / 2 = pi / 2 (mod 2 pi), so
+    the closed form cos(2 pi u + sum a_i / 2) prod 2 sin(a_i / 2) / a_i
+    vanishes up to the rounding of u.  The positive and negative halves
+    cancel, so |I| is tiny against sum |w f|; tolerance epsabs."""
+    rng = SplitMix64(seed)
+    a = [1.125 * rng.uniform() for _ in range(d)]
+    u = ((0.5 * math.pi - 0.5 * sum(a)) / (2.0 * math.pi)) % 1.0
+    lo, hi = unit_box(d)
+    return Problem(f"cancelling_oscillatory{d}d", oscillatory(a, u), lo, hi, 0.0, epsabs, 10**9)
+
+
 def c3_field(seed: int = 4, n: int = 256, ncos: int = 32) -> np.ndarray:
     """Smooth random field on an n x n grid: sum of ncos cosines
     c_k cos(2 pi (m_k x1 + n_k x2) + phi_k), c_k ~ N(0,1), m_k, n_k in {0..4},

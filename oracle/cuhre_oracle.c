@@ -244,6 +244,124 @@ double or_f(const or_integrand* f, const double* x)
 }
 
 /* ------------------------------------------------------------------------ */
+/* __float128 restatement of the families (rounding check only): the same   */
+/* formulas with every operation in 113-bit arithmetic.  NaN for the        */
+/* tabulated families.                                                      */
+/* ------------------------------------------------------------------------ */
+
+static __float128 ipowq(__float128 b, int e)
+{
+  __float128 r = 1.0Q;
+  while (e > 0) {
+    if (e & 1) r *= b;
+    b *= b;
+    e >>= 1;
+  }
+  return r;
+}
+
+static __float128 or_fq(const or_integrand* f, const __float128* x)
+{
+  const int d = f->dim;
+  switch (f->family) {
+  case OR_GAUSSIAN: {
+    __float128 s = 0.0Q;
+    for (int i = 0; i < d; ++i) {
+      const __float128 t = (__float128)f->a[i] * (x[i] - (__float128)f->w[i]);
+      s += t * t;
+    }
+    return expq(-s);
+  }
+  case OR_PRODUCT_PEAK: {
+    __float128 p = 1.0Q;
+    for (int i = 0; i < d; ++i) {
+      const __float128 a = f->a[i], t = x[i] - (__float128)f->w[i];
+      p *= 1.0Q / (a * a) + t * t;
+    }
+    return 1.0Q / p;
+  }
+  case OR_OSCILLATORY: {
+    __float128 s = 2.0Q * M_PIq * (__float128)f->u;
+    for (int i = 0; i < d; ++i) s += (__float128)f->a[i] * x[i];
+    return cosq(s);
+  }
+  case OR_CORNER_PEAK: {
+    __float128 s = 1.0Q;
+    for (int i = 0; i < d; ++i) s += (__float128)f->a[i] * x[i];
+    return ipowq(1.0Q / s, f->exponent);
+  }
+  case OR_CONTINUOUS: {
+    __float128 s = 0.0Q;
+    for (int i = 0; i < d; ++i) s += (__float128)f->a[i] * fabsq(x[i] - (__float128)f->w[i]);
+    return expq(-s);
+  }
+  case OR_POLYNOMIAL: {
+    __float128 s = 0.0Q;
+    for (int t = 0; t < f->nterms; ++t) {
+      __float128 p = f->coef[t];
+      for (int i = 0; i < d; ++i) p *= ipowq(x[i], f->powers[t][i]);
+      s += p;
+    }
+    return s;
+  }
+  case OR_COSINE_SUM: {
+    __float128 s = 0.0Q;
+    for (int i = 0; i < d; ++i) s -= cosq(10.0Q * x[i]) / (__float128)0.054402111088937;
+    return s / 2.0Q;
+  }
+  default:
+    return nanq("");
+  }
+}
+
+double or_fq_residual(const or_integrand* f, const double* x, double* lo)
+{
+  __float128 xq[OR_MAXDIM];
+  for (int i = 0; i < f->dim; ++i) xq[i] = x[i];
+  const __float128 v = or_fq(f, xq);
+  const double hi = (double)v;
+  *lo = (double)(v - (__float128)hi);
+  return hi;
+}
+
+/* Shadow of one region's basic-rule sum: s_star adds w_k * fhat_k (exact
+ * product of two doubles) in 113 bits; with want_rule, s_rule adds
+ * w_k * f(c + h g_k) with node and integrand in 113 bits. */
+typedef struct {
+  __float128 s_star, s_rule;
+  int want_rule;
+  __float128 cq[OR_MAXDIM], hq[OR_MAXDIM];
+} qshadow;
+
+static void qs_begin(qshadow* q, int d, const double* lower, const double* width)
+{
+  q->s_star = 0.0Q;
+  q->s_rule = 0.0Q;
+  for (int m = 0; m < d; ++m) {
+    q->hq[m] = 0.5Q * (__float128)width[m];
+    q->cq[m] = (__float128)lower[m] + q->hq[m];
+  }
+}
+
+/* point with generator offsets g (half-width units) */
+static void qs_point(qshadow* q, const or_integrand* f, const double* g, double w, double fhat)
+{
+  q->s_star += (__float128)w * (__float128)fhat;
+  if (q->want_rule) {
+    __float128 x[OR_MAXDIM];
+    for (int m = 0; m < f->dim; ++m) x[m] = q->cq[m] + q->hq[m] * (__float128)g[m];
+    q->s_rule += (__float128)w * or_fq(f, x);
+  }
+}
+
+static __float128 qs_volume(int d, const double* width)
+{
+  __float128 v = 1.0Q;
+  for (int m = 0; m < d; ++m) v *= (__float128)width[m];
+  return v;
+}
+
+/* ------------------------------------------------------------------------ */
 /* One region: GPUQuadSample.cu:99-220 semantics with the GM7/5 pair.         */
 /*   est = vol * sum w7 f ;  err = vol * |sum w7 f - sum w5 f|                */
 /*   D_i = |(f(+l2 e_i)+f(-l2 e_i)-2f0) - (l2/l3)^2 (f(+l3 e_i)+f(-l3 e_i)-2f0)| */
@@ -251,10 +369,9 @@ double or_f(const or_integrand* f, const double* x)
 /*   axis (GPUQuadSample.cu:113-125, 150-172).                               */
 /* ------------------------------------------------------------------------ */
 
-void or_region_eval(const or_rule* r, const or_integrand* f,
-                    const double* lower, const double* width, double* est,
-                    double* err, int* axis, double* abssum, double* i5,
-                    double* dvec)
+static void gm_region(const or_rule* r, const or_integrand* f, const double* lower,
+                      const double* width, double* est, double* err, int* axis,
+                      double* abssum, double* i5, double* dvec, qshadow* q)
 {
   const int d = r->dim, p = r->npts;
   double c[OR_MAXDIM] = {0}, h[OR_MAXDIM], x[OR_MAXDIM];
@@ -272,6 +389,7 @@ void or_region_eval(const or_rule* r, const or_integrand* f,
   s7 = fma(r->w7[0], f0, s7);
   s5 = fma(r->w5[0], f0, s5);
   sa = fma(fabs(r->w7[0]), fabs(f0), sa);
+  if (q) qs_point(q, f, r->g, r->w7[0], f0);
   double maxd = 0.0;
   int ax = widest;
   int k = 1;
@@ -283,6 +401,7 @@ void or_region_eval(const or_rule* r, const or_integrand* f,
       s7 = fma(r->w7[k], fv[s], s7);
       s5 = fma(r->w5[k], fv[s], s5);
       sa = fma(fabs(r->w7[k]), fabs(fv[s]), sa);
+      if (q) qs_point(q, f, r->g + (size_t)k * d, r->w7[k], fv[s]);
     }
     double t2 = fma(-2.0, f0, fv[0] + fv[1]);
     double t3 = fma(-2.0, f0, fv[2] + fv[3]);
@@ -297,18 +416,28 @@ void or_region_eval(const or_rule* r, const or_integrand* f,
     s7 = fma(r->w7[k], v, s7);
     s5 = fma(r->w5[k], v, s5);
     sa = fma(fabs(r->w7[k]), fabs(v), sa);
+    if (q) qs_point(q, f, r->g + (size_t)k * d, r->w7[k], v);
   }
   for (; k < p; ++k) {
     for (int m = 0; m < d; ++m) x[m] = fma(h[m], r->g[(size_t)k * d + m], c[m]);
     double v = or_f(f, x);
     s7 = fma(r->w7[k], v, s7);
     sa = fma(fabs(r->w7[k]), fabs(v), sa);
+    if (q) qs_point(q, f, r->g + (size_t)k * d, r->w7[k], v);
   }
   *est = vol * s7;
   *err = vol * fabs(s7 - s5);
   if (i5) *i5 = vol * s5;
   *abssum = vol * sa;
   *axis = ax;
+}
+
+void or_region_eval(const or_rule* r, const or_integrand* f,
+                    const double* lower, const double* width, double* est,
+                    double* err, int* axis, double* abssum, double* i5,
+                    double* dvec)
+{
+  gm_region(r, f, lower, width, est, err, axis, abssum, i5, dvec, NULL);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -386,15 +515,16 @@ int or_rule9_init(or_rule9* r, int d)
 }
 
 /* accumulates one point of set s */
-static void r9_add(const or_rule9* r, int s, double v, double* sums, double* sa)
+static void r9_add(const or_rule9* r, int s, double v, double* sums, double* sa, qshadow* qs)
 {
   for (int q = 0; q < OR_R9_RULES; ++q) sums[q] = fma(r->w[s][q], v, sums[q]);
   *sa = fma(fabs(r->w[s][0]), fabs(v), *sa);
+  if (qs) qs->s_star += (__float128)r->w[s][0] * (__float128)v;
 }
 
-void or_region_eval9(const or_rule9* r, const or_integrand* f, const double* lower,
-                     const double* width, double* est, double* err, int* axis, double* abssum,
-                     double* dvec, double* sums_out)
+static void r9_region(const or_rule9* r, const or_integrand* f, const double* lower,
+                      const double* width, double* est, double* err, int* axis, double* abssum,
+                      double* dvec, double* sums_out, qshadow* qs)
 {
   const int d = r->dim;
   double c[OR_MAXDIM] = {0}, h[OR_MAXDIM], x[OR_MAXDIM];
@@ -408,7 +538,7 @@ void or_region_eval9(const or_rule9* r, const or_integrand* f, const double* low
   }
   double sums[OR_R9_RULES] = {0, 0, 0, 0, 0}, sa = 0.0;
   const double f0 = or_f(f, c);
-  r9_add(r, 0, f0, sums, &sa);
+  r9_add(r, 0, f0, sums, &sa, qs);
   /* axis sets 1..4, fourth difference from sets 1 and 2 */
   const double ratio = (r->t[1] / r->t[0]) * (r->t[1] / r->t[0]);
   double maxd = 0.0;
@@ -420,7 +550,7 @@ void or_region_eval9(const or_rule9* r, const or_integrand* f, const double* low
         for (int m = 0; m < d; ++m) x[m] = c[m];
         x[i] = fma(h[i], sg ? -r->t[k] : r->t[k], c[i]);
         fv[k][sg] = or_f(f, x);
-        r9_add(r, 1 + k, fv[k][sg], sums, &sa);
+        r9_add(r, 1 + k, fv[k][sg], sums, &sa, qs);
       }
     const double di = fabs(2.0 * (1.0 - ratio) * f0 + ratio * (fv[0][0] + fv[0][1]) - (fv[1][0] + fv[1][1]));
     if (dvec) dvec[i] = di;
@@ -433,7 +563,7 @@ void or_region_eval9(const or_rule9* r, const or_integrand* f, const double* low
         for (int m = 0; m < d; ++m) x[m] = c[m];
         x[i] = fma(h[i], (sg & 1) ? -r->t[0] : r->t[0], c[i]);
         x[j] = fma(h[j], (sg & 2) ? -r->t[0] : r->t[0], c[j]);
-        r9_add(r, 5, or_f(f, x), sums, &sa);
+        r9_add(r, 5, or_f(f, x), sums, &sa, qs);
       }
       for (int o = 0; o < 2; ++o)
         for (int sg = 0; sg < 4; ++sg) {
@@ -441,7 +571,7 @@ void or_region_eval9(const or_rule9* r, const or_integrand* f, const double* low
           const double ti = o ? r->t[1] : r->t[0], tj = o ? r->t[0] : r->t[1];
           x[i] = fma(h[i], (sg & 1) ? -ti : ti, c[i]);
           x[j] = fma(h[j], (sg & 2) ? -tj : tj, c[j]);
-          r9_add(r, 6, or_f(f, x), sums, &sa);
+          r9_add(r, 6, or_f(f, x), sums, &sa, qs);
         }
     }
   /* triple set 7 */
@@ -453,12 +583,12 @@ void or_region_eval9(const or_rule9* r, const or_integrand* f, const double* low
           x[i] = fma(h[i], (sg & 1) ? -r->t[0] : r->t[0], c[i]);
           x[j] = fma(h[j], (sg & 2) ? -r->t[0] : r->t[0], c[j]);
           x[k] = fma(h[k], (sg & 4) ? -r->t[0] : r->t[0], c[k]);
-          r9_add(r, 7, or_f(f, x), sums, &sa);
+          r9_add(r, 7, or_f(f, x), sums, &sa, qs);
         }
   /* corner set 8 */
   for (int q = 0; q < (1 << d); ++q) {
     for (int m = 0; m < d; ++m) x[m] = fma(h[m], ((q >> m) & 1) ? -r->t[4] : r->t[4], c[m]);
-    r9_add(r, 8, or_f(f, x), sums, &sa);
+    r9_add(r, 8, or_f(f, x), sums, &sa, qs);
   }
   if (sums_out)
     for (int q = 0; q < OR_R9_RULES; ++q) sums_out[q] = sums[q];
@@ -474,6 +604,13 @@ void or_region_eval9(const or_rule9* r, const or_integrand* f, const double* low
   *err = vol * ((5.0 * e1 <= e2 && 5.0 * e2 <= e3) ? e1 : 5.0 * fmax(fmax(e1, e2), e3));
   *abssum = vol * sa;
   *axis = ax;
+}
+
+void or_region_eval9(const or_rule9* r, const or_integrand* f, const double* lower,
+                     const double* width, double* est, double* err, int* axis, double* abssum,
+                     double* dvec, double* sums_out)
+{
+  r9_region(r, f, lower, width, est, err, axis, abssum, dvec, sums_out, NULL);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -559,12 +696,85 @@ int or_integrate_bfs(const or_integrand* f, const double* lows,
   return or_integrate_bfs_key(f, lows, highs, epsrel, epsabs, maxeval, capacity, nthreads, 7, out);
 }
 
+/* The bound's derivation (standard model fl(a op b) = (a op b)(1 + e),
+ * |e| <= u; an FMA rounds once):
+ *   rule sum  s^ = sum_k w_k f_k with every term through at most p
+ *             roundings (an FMA chain, or segment chains merged by adds):
+ *             |s^ - s| <= g_p sum |w_k f_k|;
+ *   estimate  est = fl(V^ s^), V^ = fl(prod of d widths):
+ *             |est - V s| <= (g_p + g_d + g_p g_d) V sum |w f| <= g_{p+d} V sum |w f|;
+ *   abssum    A_r = fl(V^ sum^ |w f|) >= V sum |w f| (1 - g_{p+d}), and the
+ *             plain FP64 sum of the A_r is low by at most g_n: the factor
+ *             (1 + 4 (K + n) u) covers both, K = p + d + 2;
+ *   regions   T = sum est_r summed as (value, TwoSum error) pairs in any
+ *             tree of depth <= n: |I - T| <= u |T| + (1 + u) |c^ - c|, the
+ *             compensation's own rounding |c^ - c| <= g_{2n} sum |e_i|
+ *             <= 2 n^2 u^2 / (1 - 2 n u) sum |est_r|; u |T| <= 2 u |I| for
+ *             any |I - T| <= |I| / 2.
+ * (Higham, Accuracy and Stability of Numerical Algorithms, 2nd ed., 3.1,
+ * 4.2; Ogita, Rump and Oishi 2005, Sum2.) */
+double or_fp64_bound(int p, int d, int64_t n, double A, double E, double I)
+{
+  const double u = DBL_EPSILON / 2.0;
+  const double K = (double)p + (double)d + 2.0;
+  const double gK = K * u / (1.0 - K * u);
+  const double nn = (double)n;
+  return gK * A * (1.0 + 4.0 * (K + nn) * u) + 2.0 * u * fabs(I) +
+         2.0 * nn * nn * u * u / (1.0 - 2.0 * nn * u) * E;
+}
+
+static void q_split(__float128 v, double* hi, double* lo)
+{
+  *hi = (double)v;
+  *lo = (double)(v - (__float128)(*hi));
+}
+
+void or_region_check(int key, const or_integrand* f, const double* lower, const double* width,
+                     double* out)
+{
+  const int d = f->dim;
+  qshadow q;
+  q.want_rule = key != 9;
+  qs_begin(&q, d, lower, width);
+  double est, err, ab;
+  int ax;
+  if (key == 9) {
+    or_rule9 r9;
+    or_rule9_init(&r9, d);
+    r9_region(&r9, f, lower, width, &est, &err, &ax, &ab, NULL, NULL, &q);
+  } else {
+    or_rule r7;
+    or_rule_init(&r7, d);
+    gm_region(&r7, f, lower, width, &est, &err, &ax, &ab, NULL, NULL, &q);
+    or_rule_free(&r7);
+  }
+  const __float128 V = qs_volume(d, width);
+  out[0] = est;
+  out[1] = ab;
+  q_split(V * q.s_star, &out[2], &out[3]);
+  if (key == 9 || isnanq(q.s_rule)) {
+    out[4] = out[5] = NAN;
+  } else {
+    q_split(V * q.s_rule, &out[4], &out[5]);
+  }
+}
+
 int or_integrate_bfs_key(const or_integrand* f, const double* lows,
                          const double* highs, double epsrel, double epsabs,
                          int64_t maxeval, int64_t capacity, int nthreads,
                          int key, or_result* out)
 {
+  return or_integrate_bfs_check(f, lows, highs, epsrel, epsabs, maxeval, capacity, nthreads, key, 0, out,
+                                NULL);
+}
+
+int or_integrate_bfs_check(const or_integrand* f, const double* lows,
+                           const double* highs, double epsrel, double epsabs,
+                           int64_t maxeval, int64_t capacity, int nthreads,
+                           int key, int want_rule, or_result* out, or_check* chk)
+{
   memset(out, 0, sizeof(*out));
+  if (chk) memset(chk, 0, sizeof(*chk));
   const int d = f->dim;
   if (key != 0 && key != 7 && key != 9) return -1;
   or_rule rule;
@@ -584,12 +794,19 @@ int or_integrate_bfs_key(const or_integrand* f, const double* lows,
   double* ab = (double*)malloc(sizeof(double) * (size_t)cap);
   int32_t* ax = (int32_t*)malloc(sizeof(int32_t) * (size_t)cap);
   int32_t* act = (int32_t*)malloc(sizeof(int32_t) * (size_t)cap);
+  /* __float128 shadows of every region's estimate (check runs only) */
+  __float128* qstar = chk ? (__float128*)malloc(sizeof(__float128) * (size_t)cap) : NULL;
+  __float128* qrule = chk ? (__float128*)malloc(sizeof(__float128) * (size_t)cap) : NULL;
+  const int rule_q = chk && want_rule && key != 9;
+  __float128 qfin_star = 0.0Q, qfin_rule = 0.0Q;
   int rc = 0;
-  if (!lo || !wd || !lo2 || !wd2 || !est || !err || !ab || !ax || !act) { rc = -3; goto done; }
+  if (!lo || !wd || !lo2 || !wd2 || !est || !err || !ab || !ax || !act || (chk && (!qstar || !qrule))) {
+    rc = -3;
+    goto done;
+  }
   or_initial_grid(d, lows, highs, lo, wd);
   ksum ifin = {0, 0}, efin = {0, 0}, afin = {0, 0}, absfin = {0, 0};
   int64_t ncontrib_fin = 0;
-  const double u = DBL_EPSILON / 2.0;
   for (int it = 0;; ++it) {
     int nonfinite = 0;
 #ifdef _OPENMP
@@ -599,10 +816,21 @@ int or_integrate_bfs_key(const or_integrand* f, const double* lows,
       double l[OR_MAXDIM], w[OR_MAXDIM];
       for (int m = 0; m < d; ++m) { l[m] = lo[(size_t)m * n + r]; w[m] = wd[(size_t)m * n + r]; }
       int a;
+      qshadow qs, *qp = NULL;
+      if (chk) {
+        qs.want_rule = rule_q;
+        qs_begin(&qs, d, l, w);
+        qp = &qs;
+      }
       if (key == 9)
-        or_region_eval9(&rule9, f, l, w, &est[r], &err[r], &a, &ab[r], NULL, NULL);
+        r9_region(&rule9, f, l, w, &est[r], &err[r], &a, &ab[r], NULL, NULL, qp);
       else
-        or_region_eval(&rule, f, l, w, &est[r], &err[r], &a, &ab[r], NULL, NULL);
+        gm_region(&rule, f, l, w, &est[r], &err[r], &a, &ab[r], NULL, NULL, qp);
+      if (chk) {
+        const __float128 V = qs_volume(d, w);
+        qstar[r] = V * qs.s_star;
+        qrule[r] = V * qs.s_rule;
+      }
       ax[r] = a;
       if (!isfinite(est[r]) || !isfinite(err[r])) nonfinite = 1;
       double tol = fmax(epsrel * fabs(est[r]), epsabs);
@@ -614,17 +842,26 @@ int or_integrate_bfs_key(const or_integrand* f, const double* lows,
     if (n > out->peak_regions) out->peak_regions = n;
     if (nonfinite) { rc = -2; break; }
     ksum sall = ifin, eall = efin, aall = afin, absall = absfin;
+    __float128 qall_star = qfin_star, qall_rule = qfin_rule;
     int64_t A = 0;
     for (int64_t r = 0; r < n; ++r) {
       kadd(&sall, est[r]);
       kadd(&eall, err[r]);
       kadd(&aall, ab[r]);
       kadd(&absall, fabs(est[r]));
+      if (chk) {
+        qall_star += qstar[r];
+        qall_rule += qrule[r];
+      }
       if (!act[r]) {
         kadd(&ifin, est[r]);
         kadd(&efin, err[r]);
         kadd(&afin, ab[r]);
         kadd(&absfin, fabs(est[r]));
+        if (chk) {
+          qfin_star += qstar[r];
+          qfin_rule += qrule[r];
+        }
         ncontrib_fin++;
         out->nfinished++;
       } else {
@@ -634,8 +871,20 @@ int or_integrate_bfs_key(const or_integrand* f, const double* lows,
     double rG = kval(&sall), eG = kval(&eall);
     out->integral = rG;
     out->error = eG;
-    out->bound = (double)(p + 4) * u * kval(&aall) + 2.0 * u * fabs(rG) +
-                 (double)(ncontrib_fin + n) * u * u * kval(&absall);
+    out->bound = or_fp64_bound(p, d, ncontrib_fin + n, kval(&aall), kval(&absall), rG);
+    if (chk) {
+      q_split(qall_star, &chk->istar_hi, &chk->istar_lo);
+      if (rule_q && !isnanq(qall_rule)) {
+        q_split(qall_rule, &chk->irule_hi, &chk->irule_lo);
+        chk->have_rule = 1;
+      } else {
+        chk->irule_hi = chk->irule_lo = NAN;
+        chk->have_rule = 0;
+      }
+      chk->abs_total = kval(&aall);
+      chk->aest_total = kval(&absall);
+      chk->ncontrib = ncontrib_fin + n;
+    }
     if (eG <= fmax(epsrel * fabs(rG), epsabs)) { rc = 0; break; }
     if (A == 0) { rc = 1; break; }
     if (out->neval + 2 * A * (int64_t)p > maxeval || 2 * A > cap || it + 1 >= 1000) { rc = 1; break; }
@@ -648,6 +897,7 @@ int or_integrate_bfs_key(const or_integrand* f, const double* lows,
   out->fail = rc > 0 ? rc : 0;
 done:
   free(lo); free(wd); free(lo2); free(wd2); free(est); free(err); free(ab); free(ax); free(act);
+  free(qstar); free(qrule);
   or_rule_free(&rule);
   return rc;
 }
@@ -706,9 +956,13 @@ static int p2_pop(int* heap, int* hn, const double* err, const int64_t* seq)
 }
 
 /* Phase 2 of one region (GPUQuadPhases.cu:422-536). */
+/* budget[0] receives the sum of the abssums of every evaluated region,
+ * budget[1] the RES chain's sum of 2 (|child 0| + |child 1| + |parent|) + |RES|
+ * (the library's Phase-2 rounding budget); *zeroed_out 1 when the region's
+ * error passed 1e10 and it was zeroed. */
 static void phase2_region(const anyrule* R, const or_integrand* f, const double* lo0, const double* w0,
                           double epsrel, double epsabs, int pool_max, double* res_out, double* err_out,
-                          int* nreg_out, int* active_out)
+                          int* nreg_out, int* active_out, double* budget, int* zeroed_out)
 {
   const int d = R->d, cap = pool_max + 2;
   double* lo = (double*)malloc(sizeof(double) * (size_t)cap * d);
@@ -723,6 +977,7 @@ static void phase2_region(const anyrule* R, const or_integrand* f, const double*
   double ab;
   for (int m = 0; m < d; ++m) { lo[m] = lo0[m]; wd[m] = w0[m]; }
   any_eval(R, f, lo, wd, &est[0], &er[0], &ax[0], &ab);
+  double rab = ab, rres = 0.0;
   seq[0] = next++;
   p2_push(heap, &hn, 0, er, seq);
   double RES = est[0], ERR = er[0];
@@ -738,7 +993,9 @@ static void phase2_region(const anyrule* R, const or_integrand* f, const double*
     wd[(size_t)b * d + k] = hw;
     lo[(size_t)b * d + k] = lo[(size_t)a * d + k] + hw;
     any_eval(R, f, &lo[(size_t)a * d], &wd[(size_t)a * d], &est[a], &er[a], &ax[a], &ab);
+    rab += ab;
     any_eval(R, f, &lo[(size_t)b * d], &wd[(size_t)b * d], &est[b], &er[b], &ax[b], &ab);
+    rab += ab;
     /* Cuba's refinement from the parent difference (GPUQuadPhases.cu:498-515) */
     const double diff = fabs(0.25 * (est[a] + est[b] - pe));
     const double e = er[a] + er[b];
@@ -751,6 +1008,7 @@ static void phase2_region(const anyrule* R, const or_integrand* f, const double*
     er[b] += diff;
     ERR += er[a] + er[b] - pr;
     RES += est[a] + est[b] - pe;
+    rres += 2.0 * (fabs(est[a]) + fabs(est[b]) + fabs(pe)) + fabs(RES);
     seq[a] = next++;
     seq[b] = next++;
     p2_push(heap, &hn, a, er, seq);
@@ -758,11 +1016,15 @@ static void phase2_region(const anyrule* R, const or_integrand* f, const double*
     ++count;
   }
   int active = ERR > fmax(epsrel * fabs(RES), epsabs);
+  *zeroed_out = 0;
   if (ERR > 1e10) { /* the reference zeroes a diverging region and flags it */
     RES = 0.0;
     ERR = 0.0;
     active = 1;
+    *zeroed_out = 1;
   }
+  budget[0] = rab;
+  budget[1] = rres;
   *res_out = RES;
   *err_out = ERR;
   *nreg_out = count;
@@ -800,6 +1062,8 @@ int or_integrate_two_phase(const or_integrand* f, const double* lows, const doub
   or_initial_grid(d, lows, highs, lo, wd);
   {
     ksum ifin = {0, 0}, efin = {0, 0};
+    double fin_abs = 0.0, fin_aest = 0.0;
+    int64_t nfin = 0;
     int phase2 = 0;
     for (int it = 0; it < 100; ++it) {
       int nonfinite = 0;
@@ -821,17 +1085,30 @@ int or_integrate_two_phase(const or_integrand* f, const double* lows, const doub
       if (n > out->peak_regions) out->peak_regions = n;
       if (nonfinite) { rc = -2; goto done2; }
       ksum sall = ifin, eall = efin;
+      double all_abs = fin_abs, all_aest = fin_aest;
       int64_t A = 0;
       for (int64_t r = 0; r < n; ++r) {
         kadd(&sall, est[r]);
         kadd(&eall, err[r]);
-        if (!act[r]) { kadd(&ifin, est[r]); kadd(&efin, err[r]); out->nfinished++; }
-        else ++A;
+        all_abs += ab[r];
+        all_aest += fabs(est[r]);
+        if (!act[r]) {
+          kadd(&ifin, est[r]);
+          kadd(&efin, err[r]);
+          fin_abs += ab[r];
+          fin_aest += fabs(est[r]);
+          ++nfin;
+          out->nfinished++;
+        } else {
+          ++A;
+        }
       }
       const double rG = kval(&sall), eG = kval(&eall);
       if (eG <= fmax(epsrel * fabs(rG), epsabs)) { /* converged in Phase 1 */
         out->integral = rG;
         out->error = eG;
+        /* contributing: the regions finished before this iteration and all n of it */
+        out->bound = or_fp64_bound(p, d, nfin + A, all_abs, all_aest, rG);
         out->fail = 0;
         rc = 0;
         goto done2;
@@ -853,28 +1130,43 @@ int or_integrate_two_phase(const or_integrand* f, const double* lows, const doub
       double* perr = (double*)malloc(sizeof(double) * (size_t)n);
       int* pcnt = (int*)malloc(sizeof(int) * (size_t)n);
       int* pact = (int*)malloc(sizeof(int) * (size_t)n);
+      int* pzero = (int*)malloc(sizeof(int) * (size_t)n);
+      double* pbud = (double*)malloc(sizeof(double) * 2 * (size_t)n);
 #ifdef _OPENMP
 #pragma omp parallel for num_threads(nthreads > 0 ? nthreads : 1) schedule(dynamic, 4)
 #endif
       for (int64_t r = 0; r < n; ++r) {
         double l[OR_MAXDIM], w[OR_MAXDIM];
         for (int m = 0; m < d; ++m) { l[m] = lo[(size_t)m * n + r]; w[m] = wd[(size_t)m * n + r]; }
-        phase2_region(&R, f, l, w, epsrel, epsabs, pool_max, &pres[r], &perr[r], &pcnt[r], &pact[r]);
+        phase2_region(&R, f, l, w, epsrel, epsabs, pool_max, &pres[r], &perr[r], &pcnt[r], &pact[r],
+                      &pbud[2 * r], &pzero[r]);
       }
       ksum I = ifin, E = efin;
-      int64_t failed = 0;
+      int64_t failed = 0, zeroed = 0;
+      double ab_tot = fin_abs, res_round = 0.0, aest = fin_aest;
       for (int64_t r = 0; r < n; ++r) {
         kadd(&I, pres[r]);
         kadd(&E, perr[r]);
         out->nregions += pcnt[r];
         out->neval += (int64_t)(pcnt[r] - 1) * 2 * p + p;
         failed += pact[r];
+        zeroed += pzero[r];
+        ab_tot += pbud[2 * r];
+        res_round += pbud[2 * r + 1];
+        aest += fabs(pres[r]);
       }
       out->integral = kval(&I);
       out->error = kval(&E);
-      free(pres); free(perr); free(pcnt); free(pact);
-      rc = out->error <= fmax(epsrel * fabs(out->integral), epsabs) ? 0 : (int)(failed > 0 ? 1 : 0);
-      out->fail = (int32_t)(rc == 0 ? 0 : (failed > 0x7fffffff ? 0x7fffffff : failed));
+      {
+        const double u = DBL_EPSILON / 2.0;
+        out->bound = or_fp64_bound(p, d, nfin + n, ab_tot, aest, out->integral) +
+                     u * res_round * (1.0 + 4.0 * ((double)pool_max + (double)n + 2.0) * u);
+      }
+      free(pres); free(perr); free(pcnt); free(pact); free(pzero); free(pbud);
+      /* a zeroed region's contribution is dropped: the result is not certified */
+      const int ok = out->error <= fmax(epsrel * fabs(out->integral), epsabs) && zeroed == 0;
+      rc = ok ? 0 : 1;
+      out->fail = (int32_t)(rc == 0 ? 0 : (failed > 0x7fffffff ? 0x7fffffff : (failed > 0 ? failed : 1)));
     } else {
       rc = out->error <= fmax(epsrel * fabs(out->integral), epsabs) ? 0 : 1;
       out->fail = rc;

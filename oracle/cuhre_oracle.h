@@ -118,7 +118,8 @@ typedef struct {
 
 /* Breadth-first loop (the plain method).  nthreads>1 uses OpenMP over the
  * regions of an iteration.  Returns 0 (converged), 1 (not converged), <0
- * error (-2 nonfinite integrand value, -3 allocation, -1 bad argument). */
+ * error (-2 nonfinite integrand value, -3 allocation, -1 bad argument).
+ * bound = or_fp64_bound over the regions the result sums. */
 int or_integrate_bfs(const or_integrand* f, const double* lows,
                      const double* highs, double epsrel, double epsabs,
                      int64_t maxeval, int64_t capacity, int nthreads,
@@ -128,6 +129,49 @@ int or_integrate_bfs_key(const or_integrand* f, const double* lows,
                          const double* highs, double epsrel, double epsabs,
                          int64_t maxeval, int64_t capacity, int nthreads,
                          int key, or_result* out);
+
+/* FP64 rounding bound of a result over n contributing regions of a rule of
+ * p points in d dimensions (the same formula the product library reports):
+ *   g_K A (1 + 4 (K + n) u) + 2 u |I| + 2 n^2 u^2 / (1 - 2 n u) E,
+ *   K = p + d + 2, g_K = K u / (1 - K u), u = 2^-53,
+ * A = sum over the regions of volume * sum_k |w_k f_k| (abssum), E = sum
+ * |est|.  It bounds |I - sum_r V_r sum_k w_k f_k| for the FP64 integrand
+ * values f_k the loop obtained: the rule sums (each term passes through at
+ * most p roundings, g_p), the volume and the estimate's product (g_d), and
+ * the TwoSum-carried sum over regions (u |T| + g_{2n}^2-type compensation
+ * term; any summation tree of depth <= n).  The integrand's own rounding
+ * is the functor's, not the engine's, and is not included. */
+double or_fp64_bound(int p, int d, int64_t n, double A, double E, double I);
+
+/* Rounding check of one breadth-first run, evaluated beside the FP64 loop
+ * in __float128 (w_k * f_k of two doubles is exact in 113 bits):
+ *   istar = sum over the contributing regions of V_r sum_k w_k fhat_k, the
+ *           exact-arithmetic value of the loop's own FP64 integrand values;
+ *   irule = the same sums with the nodes c + h g_k and the integrand in
+ *           __float128 (families with a quad restatement; have_rule = 0
+ *           otherwise and for key 9): the rule's value free of FP64.
+ * Each as a double pair (hi + lo).  abs_total, aest_total, ncontrib are
+ * the bound's A, E and n. */
+typedef struct {
+  double istar_hi, istar_lo, irule_hi, irule_lo;
+  double abs_total, aest_total;
+  int64_t ncontrib;
+  int32_t have_rule, pad;
+} or_check;
+
+int or_integrate_bfs_check(const or_integrand* f, const double* lows,
+                           const double* highs, double epsrel, double epsabs,
+                           int64_t maxeval, int64_t capacity, int nthreads,
+                           int key, int want_rule, or_result* out, or_check* chk);
+
+/* One region with its __float128 shadow: out[0..5] = est, abssum,
+ * istar (hi, lo), irule (hi, lo; NaN where no quad restatement). */
+void or_region_check(int key, const or_integrand* f, const double* lower,
+                     const double* width, double* out);
+
+/* The integrand in __float128 at a double point (NaN where the family has
+ * no quad restatement: the tabulated ones). */
+double or_fq_residual(const or_integrand* f, const double* x, double* lo);
 
 /* The reference's two-phase method (GPUquad.h:611-727): Phase 1 is the
  * breadth-first loop (at most 100 iterations) until the global test holds,

@@ -55,6 +55,23 @@ class OrResult(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class OrCheck(C.Structure):
+    _fields_ = [
+        ("istar_hi", C.c_double),
+        ("istar_lo", C.c_double),
+        ("irule_hi", C.c_double),
+        ("irule_lo", C.c_double),
+        ("abs_total", C.c_double),
+        ("aest_total", C.c_double),
+        ("ncontrib", C.c_int64),
+        ("have_rule", C.c_int32),
+        ("pad", C.c_int32),
+    ]
+
+    def asdict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+
+
 class OrRule(C.Structure):
     _fields_ = [
         ("dim", C.c_int),
@@ -106,6 +123,14 @@ def lib():
                                       dp, dp, C.POINTER(C.c_int), dp, dp, dp]
         L.or_integrate_bfs_key.argtypes = [C.POINTER(OrIntegrand), dp, dp, C.c_double, C.c_double,
                                            C.c_int64, C.c_int64, C.c_int, C.c_int, C.POINTER(OrResult)]
+        L.or_integrate_bfs_check.argtypes = [C.POINTER(OrIntegrand), dp, dp, C.c_double, C.c_double,
+                                             C.c_int64, C.c_int64, C.c_int, C.c_int, C.c_int,
+                                             C.POINTER(OrResult), C.POINTER(OrCheck)]
+        L.or_fp64_bound.argtypes = [C.c_int, C.c_int, C.c_int64, C.c_double, C.c_double, C.c_double]
+        L.or_fp64_bound.restype = C.c_double
+        L.or_region_check.argtypes = [C.c_int, C.POINTER(OrIntegrand), dp, dp, dp]
+        L.or_fq_residual.argtypes = [C.POINTER(OrIntegrand), dp, dp]
+        L.or_fq_residual.restype = C.c_double
         L.or_integrate_two_phase.argtypes = [C.POINTER(OrIntegrand), dp, dp, C.c_double, C.c_double, C.c_int,
                                              C.c_int64, C.c_int, C.c_int, C.POINTER(OrResult)]
         L.or_integrate_heap.argtypes = [C.POINTER(OrIntegrand), dp, dp, C.c_double, C.c_double,
@@ -233,6 +258,43 @@ def integrate_bfs(p: W.Problem, nthreads=1, capacity=None, maxeval=None, epsrel=
     d = res.asdict()
     d["rc"] = rc
     return d
+
+
+def integrate_bfs_check(p: W.Problem, nthreads=8, maxeval=None, epsrel=None, epsabs=None, key=7,
+                        want_rule=True):
+    """The breadth-first loop with its __float128 shadow: the FP64 result
+    plus istar (exact-arithmetic sum of the loop's own FP64 integrand values
+    over the contributing regions) and irule (nodes and integrand in
+    __float128), each as a (hi, lo) pair, and the bound's A, E, n."""
+    s, keep = to_struct(p.f)
+    lo = np.array(p.lows, dtype=np.float64)
+    hi = np.array(p.highs, dtype=np.float64)
+    res = OrResult()
+    chk = OrCheck()
+    me = p.maxeval if maxeval is None else maxeval
+    cap = max(1, me // npoints(p.f.dim, key)) + 64
+    rc = lib().or_integrate_bfs_check(C.byref(s), _dp(lo), _dp(hi), p.epsrel if epsrel is None else epsrel,
+                                      p.epsabs if epsabs is None else epsabs, me, cap, nthreads, key,
+                                      1 if want_rule else 0, C.byref(res), C.byref(chk))
+    d = res.asdict()
+    d["rc"] = rc
+    d.update(chk.asdict())
+    return d
+
+
+def fp64_bound(p, d, n, A, E, I):
+    """The rounding bound formula the library reports (or_fp64_bound)."""
+    return lib().or_fp64_bound(p, d, n, A, E, I)
+
+
+def region_check(f: W.Integrand, lower, width, key=7):
+    """est, abssum, istar (hi, lo), irule (hi, lo) of one region."""
+    s, keep = to_struct(f)
+    lo = np.ascontiguousarray(lower, dtype=np.float64)
+    wd = np.ascontiguousarray(width, dtype=np.float64)
+    out = np.empty(6)
+    lib().or_region_check(key, C.byref(s), _dp(lo), _dp(wd), _dp(out))
+    return out
 
 
 def integrate_two_phase(p: W.Problem, key=9, phase1_max=16384, pool_max=2048, nthreads=8):

@@ -100,6 +100,17 @@ def test_batch_capacity_cut_and_errors(ctx):
     with pytest.raises(cuhre.CuhreError) as e:
         ctx.integrate_batch(bad)
     assert e.value.code == cuhre.ERR_NONFINITE
+    # the non-finite instance is marked in its own result; the others are
+    # complete and equal their plain runs
+    res, info, rc = ctx.integrate_batch(bad, raise_nonfinite=False)
+    assert rc == cuhre.ERR_NONFINITE
+    assert res[3]["fail"] == cuhre.ERR_NONFINITE
+    assert info["not_converged"] >= 1
+    for i in range(3):
+        pl = ctx.integrate(bad[i])
+        assert res[i]["fail"] == pl["fail"]
+        assert res[i]["nregions"] == pl["nregions"]
+        assert abs(res[i]["integral"] - pl["integral"]) <= 1e-12 * abs(pl["integral"])
     with pytest.raises(cuhre.CuhreError) as e:
         ctx.integrate_batch([W.c1_gaussian3(), W.c2_oscillatory8()])
     assert e.value.code == cuhre.ERR_ARG
