@@ -137,6 +137,24 @@ int cuhre_integrate(cuhre_context* ctx, const cuhre_integrand* f, int dim,
                     double epsrel, double epsabs, int64_t maxeval, int key,
                     const double* lows, const double* highs, cuhre_result* out);
 
+/* cuhre_integrate without any allocation: no counterpart in the reference;
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, kernels and results as cuhre_integrate (bit for bit), with the
+ * region lists in a static 64 MiB device array per GPU (taken once per
+ * device, held until the process ends) instead of the context's buffers;
+ * the context must exist already (its creation allocates).  A run whose
+ * list would outgrow the array -- cuhre_noalloc_capacity(dim) regions,
+ * about 233k at dim 8 and 540k at dim 3 -- stops with CUHRE_ERR_CAPACITY
+ * (out holds the partial result).  Tables of TABLE6 / LC_LT_T must fit in
+ * 1 MiB.  Calls on one GPU share the array: they are serialised by a lock
+ * inside one process; calls that overlap on one GPU otherwise race. */
+int cuhre_integrate_noalloc(cuhre_context* ctx, const cuhre_integrand* f, int dim,
+                            double epsrel, double epsabs, int64_t maxeval, int key,
+                            const double* lows, const double* highs, cuhre_result* out);
+/* Regions per list the static array holds at dim (-1 outside 2..10). */
+int64_t cuhre_noalloc_capacity(int dim);
+
 /* The reference's two-phase method (quad/GPUquad/GPUquad.h:611-727):
  * Phase 1 is the breadth-first loop above, at most 100 iterations, until
  * the global test holds, no region is active, or a split leaves

@@ -89,7 +89,12 @@ namespace cuhre {
 int fail_msg(int code, const char* msg);
 int ctx_reserve(cuhre_context* c, int64_t cap, int dim, int64_t keep_n = 0, int cur = 0, int64_t keep_nb = 0);
 int ctx_reserve_aux(cuhre_context* c, int64_t cap, int dim);
-int make_dev_integrand(cuhre_context* c, const cuhre_integrand* f, int dim, DevIntegrand* P);
+// Device copy of an integrand.  Tabulated families copy their table to the
+// device: into buffers the context allocates and keeps (table_dst null), or
+// into table_dst, a caller-provided device area of table_bytes (the
+// no-allocation entry point), in which case nothing is allocated.
+int make_dev_integrand(cuhre_context* c, const cuhre_integrand* f, int dim, DevIntegrand* P,
+                       char* table_dst = nullptr, size_t table_bytes = 0);
 int64_t list_capacity(cuhre_context* c, int dim);
 int pick_segments(const cuhre_context* c, int64_t n, int key = 7);
 // Block partials / offsets the list buffers of cap regions come with: one

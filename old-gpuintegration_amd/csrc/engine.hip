@@ -19,6 +19,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -523,7 +524,16 @@ int ctx_reserve_aux(cuhre_context* c, int64_t cap, int dim)
   return 0;
 }
 
-int make_dev_integrand(cuhre_context* c, const cuhre_integrand* f, int dim, DevIntegrand* P)
+// LC_LT_t tables are float on the device (the reference's textures): the
+// no-allocation path uploads the caller's doubles and rounds them here.
+__global__ __launch_bounds__(kBlock) void k_table_to_float(const double* src, float* dst, int64_t n)
+{
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) dst[i] = (float)src[i];
+}
+
+int make_dev_integrand(cuhre_context* c, const cuhre_integrand* f, int dim, DevIntegrand* P, char* table_dst,
+                       size_t table_bytes)
 {
   std::memset(P, 0, sizeof *P);
   P->family = f->family;
@@ -551,6 +561,16 @@ int make_dev_integrand(cuhre_context* c, const cuhre_integrand* f, int dim, DevI
     if (!f->table || f->table_nx < 2 || f->table_ny < 2)
       return fail(CUHRE_ERR_ARG, "table integrand needs a table of at least 2x2");
     const size_t cnt = (size_t)f->table_nx * f->table_ny;
+    if (table_dst) {
+      if (cnt * sizeof(double) > table_bytes)
+        return fail(CUHRE_ERR_CAPACITY, "table of %zu values exceeds the static workspace's %zu bytes", cnt,
+                    table_bytes);
+      HIP_TRY(hipMemcpyAsync(table_dst, f->table, cnt * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      P->table = reinterpret_cast<const double*>(table_dst);
+      P->nx = f->table_nx;
+      P->ny = f->table_ny;
+      return 0;
+    }
     const bool same = c->table && c->table_host.size() == cnt &&
                       std::memcmp(c->table_host.data(), f->table, cnt * sizeof(double)) == 0;
     if (!same) {
@@ -570,15 +590,33 @@ int make_dev_integrand(cuhre_context* c, const cuhre_integrand* f, int dim, DevI
     if (!f->table || nx < 2 || ny < 2 || nx > 4096 || ny > 4096)
       return fail(CUHRE_ERR_ARG, "LC_LT_t needs node arrays of 2..4096 entries and five fields");
     const size_t cnt = (size_t)nx + ny + 5 * (size_t)nx * ny;
-    std::vector<float> t32(cnt);
-    for (size_t k = 0; k < cnt; ++k) {
-      t32[k] = (float)f->table[k];
-      if (!std::isfinite(t32[k])) return fail(CUHRE_ERR_ARG, "LC_LT_t table value %zu is not finite", k);
-    }
+    // checks on the values as the device will hold them (rounded to float)
+    auto f32 = [&](size_t k) { return (float)f->table[k]; };
+    for (size_t k = 0; k < cnt; ++k)
+      if (!std::isfinite(f32(k))) return fail(CUHRE_ERR_ARG, "LC_LT_t table value %zu is not finite", k);
     for (int i = 1; i < nx; ++i)
-      if (!(t32[i] > t32[i - 1])) return fail(CUHRE_ERR_ARG, "LC_LT_t lt nodes must increase strictly");
+      if (!(f32(i) > f32(i - 1))) return fail(CUHRE_ERR_ARG, "LC_LT_t lt nodes must increase strictly");
     for (int j = 1; j < ny; ++j)
-      if (!(t32[nx + j] > t32[nx + j - 1])) return fail(CUHRE_ERR_ARG, "LC_LT_t zt nodes must increase strictly");
+      if (!(f32(nx + j) > f32(nx + j - 1))) return fail(CUHRE_ERR_ARG, "LC_LT_t zt nodes must increase strictly");
+    if (table_dst) {
+      // doubles first, then their float rounding behind them
+      const size_t dbytes = (cnt * sizeof(double) + 255) & ~(size_t)255;
+      if (dbytes + cnt * sizeof(float) > table_bytes)
+        return fail(CUHRE_ERR_CAPACITY, "LC_LT_t table of %zu values exceeds the static workspace's %zu bytes",
+                    cnt, table_bytes);
+      double* dsrc = reinterpret_cast<double*>(table_dst);
+      float* dst = reinterpret_cast<float*>(table_dst + dbytes);
+      HIP_TRY(hipMemcpyAsync(dsrc, f->table, cnt * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      hipLaunchKernelGGL(k_table_to_float, dim3((unsigned)((cnt + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                         c->stream, dsrc, dst, (int64_t)cnt);
+      HIP_TRY(hipGetLastError());
+      P->ftab = dst;
+      P->nx = nx;
+      P->ny = ny;
+      return 0;
+    }
+    std::vector<float> t32(cnt);
+    for (size_t k = 0; k < cnt; ++k) t32[k] = f32(k);
     const bool same = c->ftable && c->ftable_host.size() == cnt &&
                       std::memcmp(c->ftable_host.data(), t32.data(), cnt * sizeof(float)) == 0;
     if (!same) {
@@ -764,6 +802,10 @@ int64_t cuhre_points_per_region(int dim, int key)
   return key == 9 ? r9_points(dim) : gm_points(dim);
 }
 
+}  // extern "C"
+static int scratch_setup(int device);
+extern "C" {
+
 int cuhre_context_create(cuhre_context** out, int device, int64_t capacity_regions)
 {
   if (!out) return fail(CUHRE_ERR_ARG, "null output pointer");
@@ -793,6 +835,9 @@ int cuhre_context_create(cuhre_context** out, int device, int64_t capacity_regio
   }
   for (int i = 0; i < 4 && rc == 0; ++i)
     if (hipEventCreate(&c->ev[i]) != hipSuccess) rc = fail(CUHRE_ERR_DEVICE, "event create failed");
+  // the static workspace's address on this device (the no-allocation entry
+  // point); a failure here is reported again by that entry point
+  if (rc == 0) (void)scratch_setup(device);
   if (rc != 0) {
     cuhre_context_destroy(c);
     return rc;
@@ -876,6 +921,12 @@ namespace {
 struct LoopOpts {
   int64_t handoff_n = 0;
   int max_iter = 1000;
+  // no-allocation mode: the context's list buffers already point into the
+  // static workspace and must not grow; a run that outgrows them ends with
+  // CUHRE_ERR_CAPACITY; tables go to table_dst
+  bool noalloc = false;
+  char* table_dst = nullptr;
+  size_t table_bytes = 0;
 };
 
 struct LoopHandoff {
@@ -902,7 +953,7 @@ int plain_loop(cuhre_context* c, const cuhre_integrand* f, int dim, double epsre
       return fail(CUHRE_ERR_ARG, "empty or non-finite box along axis %d", m);
 
   DevIntegrand P;
-  if ((rc = make_dev_integrand(c, f, dim, &P))) return rc;
+  if ((rc = make_dev_integrand(c, f, dim, &P, opt.table_dst, opt.table_bytes))) return rc;
   const RuleConst R = make_rule(dim, key);
   const int64_t p = R.npts;
   const double rflops = region_flops(f, dim, key);
@@ -929,7 +980,8 @@ int plain_loop(cuhre_context* c, const cuhre_integrand* f, int dim, double epsre
     A.fin_err_s = fer_s;
     A.fin_err_c = fer_c;
     const int64_t left = maxeval - (out->neval + n * p);
-    A.max_active = std::min<int64_t>(left >= 0 ? left / (2 * p) : -1, capacity / 2);
+    const int64_t budget_active = left >= 0 ? left / (2 * p) : -1;
+    A.max_active = std::min<int64_t>(budget_active, capacity / 2);
     A.alloc_active = c->cap / 2;
     SplitArgs SA;
     fill_split_args(c, SA, cur, n, dim, epsrel, epsabs, seg_regions_per_block(S), true);
@@ -977,6 +1029,12 @@ int plain_loop(cuhre_context* c, const cuhre_integrand* f, int dim, double epsre
     }
     out->eval_launches++;
     out->eval_flops += rflops * (double)n;
+    if (st.stop == 5 && opt.noalloc) {
+      // cannot happen (max_active <= capacity / 2 = alloc_active stops first)
+      result = fail(CUHRE_ERR_CAPACITY, "static workspace too small for the children of %lld regions",
+                    (long long)st.tot.nactive);
+      break;
+    }
     if (st.stop == 5) {
       // list buffers too small for the children: grow (keeping this
       // iteration's list and results) and run the split again
@@ -1023,6 +1081,15 @@ int plain_loop(cuhre_context* c, const cuhre_integrand* f, int dim, double epsre
       result = CUHRE_OK;
       break;
     }
+    if (st.stop == 3 && opt.noalloc && budget_active >= st.tot.nactive) {
+      // the evaluation budget allowed the split; the static lists did not
+      result = fail(CUHRE_ERR_CAPACITY,
+                    "the run needs %lld regions, more than the %lld the static workspace holds at dim %d "
+                    "(cuhre_integrate has no such limit)",
+                    (long long)(2 * st.tot.nactive), (long long)capacity, dim);
+      out->fail = 1;
+      break;
+    }
     if (st.stop == 2 || st.stop == 3) {
       result = CUHRE_NOT_CONVERGED;
       break;
@@ -1044,12 +1111,70 @@ int plain_loop(cuhre_context* c, const cuhre_integrand* f, int dim, double epsre
     }
   }
   HIP_TRY(hipStreamSynchronize(c->stream));
-  out->fail = result > 0 ? result : 0;
+  if (result >= 0 || !opt.noalloc) out->fail = result > 0 ? result : 0;
   out->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return result;
 }
 
 }  // namespace
+
+// ------------------------------------------------------- static workspace
+// Device memory of cuhre_integrate_noalloc: one array per GPU (each device
+// holds its own copy at its own address until the process ends), 64 MiB, the
+// last 1 MiB reserved for integrand tables.  In host code `scratch` names a
+// host-side stand-in, not device memory: kernels and copies only ever get the
+// address hipGetSymbolAddress returns for the current device, kept in
+// g_scratch_ptr.
+constexpr size_t kScratchBytes = 64ull << 20;
+constexpr size_t kScratchTableBytes = 1ull << 20;
+static __device__ __align__(256) char scratch[kScratchBytes];
+
+constexpr int kScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static char* g_scratch_ptr[kScratchDevices];
+static std::mutex g_scratch_call_mu[kScratchDevices];  // one no-allocation call per device at a time
+
+// With `device` current: takes (once) that device's address of the array.
+static int scratch_setup(int device)
+{
+  if (device < 0 || device >= kScratchDevices)
+    return fail(CUHRE_ERR_UNSUPPORTED, "static workspace: device %d beyond %d", device, kScratchDevices);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return 0;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || ptr == nullptr) {
+    (void)hipGetLastError();
+    return fail(CUHRE_ERR_DEVICE, "hipGetSymbolAddress(scratch) on device %d failed: %s", device,
+                hipGetErrorString(e));
+  }
+  g_scratch_ptr[device] = static_cast<char*>(ptr);
+  return 0;
+}
+
+static size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Bytes of the list buffers (two ping-pong lists, est/err/abssum/axis, block
+// partials and offsets with their group tails) for cap regions of dim axes.
+static size_t scratch_list_bytes(int64_t cap, int dim)
+{
+  const int64_t nb = block_capacity(cap), ng = group_capacity(nb);
+  return 4 * al256(sizeof(double) * (size_t)cap * dim) + 3 * al256(sizeof(double) * (size_t)cap) +
+         al256((size_t)cap) + al256(sizeof(Partial) * (size_t)(nb + ng)) + al256(sizeof(int64_t) * (size_t)(nb + ng));
+}
+
+// Regions per list the workspace holds at dim (a multiple of 256).
+static int64_t scratch_capacity(int dim)
+{
+  const size_t avail = kScratchBytes - kScratchTableBytes;
+  int64_t lo = 0, hi = (int64_t)(avail / (4 * sizeof(double) * (size_t)dim)) / 256 + 1;
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) / 2;
+    if (scratch_list_bytes(mid * 256, dim) <= avail) lo = mid;
+    else hi = mid;
+  }
+  return lo * 256;
+}
 
 extern "C" {
 
@@ -1058,6 +1183,80 @@ int cuhre_integrate(cuhre_context* c, const cuhre_integrand* f, int dim, double 
 {
   LoopOpts opt;
   return plain_loop(c, f, dim, epsrel, epsabs, maxeval, key, lows, highs, out, opt, nullptr);
+}
+
+int64_t cuhre_noalloc_capacity(int dim)
+{
+  if (dim < 2 || dim > 10) return -1;
+  return scratch_capacity(dim);
+}
+
+int cuhre_integrate_noalloc(cuhre_context* c, const cuhre_integrand* f, int dim, double epsrel, double epsabs,
+                            int64_t maxeval, int key, const double* lows, const double* highs, cuhre_result* out)
+{
+  if (!c) return fail(CUHRE_ERR_ARG, "null context");
+  if (!out) return fail(CUHRE_ERR_ARG, "null result");
+  if (dim < 2 || dim > 10) return fail(CUHRE_ERR_UNSUPPORTED, "dimension %d outside 2..10", dim);
+  HIP_TRY(hipSetDevice(c->device));
+  int rc = scratch_setup(c->device);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> call_lk(g_scratch_call_mu[c->device]);
+  char* const base = g_scratch_ptr[c->device];
+  const int64_t cap = scratch_capacity(dim);
+
+  // the context's own list buffers step aside for the call
+  double *lo0 = c->lo[0], *lo1 = c->lo[1], *wd0 = c->wd[0], *wd1 = c->wd[1];
+  double *est = c->est, *err = c->err, *ab = c->abssum;
+  int8_t* ax = c->axis;
+  Partial* part = c->partials;
+  int64_t* off = c->offsets;
+  const int64_t cap0 = c->cap, capacity0 = c->capacity;
+  const int dim0 = c->dim_alloc;
+
+  char* q = base;
+  auto take = [&](size_t bytes) {
+    char* r = q;
+    q += al256(bytes);
+    return r;
+  };
+  const int64_t nb = block_capacity(cap), ng = group_capacity(nb);
+  c->lo[0] = (double*)take(sizeof(double) * (size_t)cap * dim);
+  c->lo[1] = (double*)take(sizeof(double) * (size_t)cap * dim);
+  c->wd[0] = (double*)take(sizeof(double) * (size_t)cap * dim);
+  c->wd[1] = (double*)take(sizeof(double) * (size_t)cap * dim);
+  c->est = (double*)take(sizeof(double) * (size_t)cap);
+  c->err = (double*)take(sizeof(double) * (size_t)cap);
+  c->abssum = (double*)take(sizeof(double) * (size_t)cap);
+  c->axis = (int8_t*)take((size_t)cap);
+  c->partials = (Partial*)take(sizeof(Partial) * (size_t)(nb + ng));
+  c->offsets = (int64_t*)take(sizeof(int64_t) * (size_t)(nb + ng));
+  c->cap = cap;
+  c->dim_alloc = dim;
+  c->capacity = cap;
+
+  LoopOpts opt;
+  opt.noalloc = true;
+  opt.table_dst = base + (kScratchBytes - kScratchTableBytes);
+  opt.table_bytes = kScratchTableBytes;
+  rc = plain_loop(c, f, dim, epsrel, epsabs, maxeval, key, lows, highs, out, opt, nullptr);
+  // nothing may still use the workspace once the lock is released
+  const hipError_t se = hipStreamSynchronize(c->stream);
+
+  c->lo[0] = lo0;
+  c->lo[1] = lo1;
+  c->wd[0] = wd0;
+  c->wd[1] = wd1;
+  c->est = est;
+  c->err = err;
+  c->abssum = ab;
+  c->axis = ax;
+  c->partials = part;
+  c->offsets = off;
+  c->cap = cap0;
+  c->capacity = capacity0;
+  c->dim_alloc = dim0;
+  if (rc == 0 && se != hipSuccess) return fail(CUHRE_ERR_DEVICE, "HIP error: %s", hipGetErrorString(se));
+  return rc;
 }
 
 int cuhre_integrate_two_phase(cuhre_context* c, const cuhre_integrand* f, int dim, double epsrel, double epsabs,

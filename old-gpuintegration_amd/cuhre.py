@@ -180,6 +180,10 @@ def lib():
         L.cuhre_context_set.argtypes = [ctxp, C.c_int, C.c_int]
         L.cuhre_integrate.argtypes = [ctxp, C.POINTER(Integrand), C.c_int, C.c_double, C.c_double,
                                       C.c_int64, C.c_int, dp, dp, C.POINTER(Result)]
+        L.cuhre_integrate_noalloc.argtypes = [ctxp, C.POINTER(Integrand), C.c_int, C.c_double, C.c_double,
+                                              C.c_int64, C.c_int, dp, dp, C.POINTER(Result)]
+        L.cuhre_noalloc_capacity.argtypes = [C.c_int]
+        L.cuhre_noalloc_capacity.restype = C.c_int64
         L.cuhre_integrate_two_phase.argtypes = [ctxp, C.POINTER(Integrand), C.c_int, C.c_double, C.c_double,
                                                 C.c_int, C.c_int64, C.c_int32, dp, dp, C.POINTER(Result)]
         L.cuhre_eval_regions.argtypes = [ctxp, C.POINTER(Integrand), C.c_int, C.c_int, C.c_int64,
@@ -296,6 +300,27 @@ class Context:
         _check(rc)
         d = res.asdict()
         d["rc"] = rc
+        return d
+
+    def integrate_noalloc(self, p: W.Problem, epsrel: Optional[float] = None, maxeval: Optional[int] = None,
+                          key: int = 0, check: bool = True):
+        """cuhre_integrate_noalloc: the plain method with its region lists in
+        the library's static device array (no allocation in the call).
+        With check=False the return code is returned instead of raised."""
+        s, keep = to_struct(p.f)
+        lo = np.ascontiguousarray(p.lows, dtype=np.float64)
+        hi = np.ascontiguousarray(p.highs, dtype=np.float64)
+        res = Result()
+        rc = lib().cuhre_integrate_noalloc(self.h, C.byref(s), p.f.dim,
+                                           p.epsrel if epsrel is None else epsrel, p.epsabs,
+                                           p.maxeval if maxeval is None else maxeval, key,
+                                           _dp(lo), _dp(hi), C.byref(res))
+        if check:
+            _check(rc)
+        d = res.asdict()
+        d["rc"] = rc
+        if rc < 0:
+            d["message"] = lib().cuhre_last_error().decode()
         return d
 
     def integrate_two_phase(self, p: W.Problem, key: int = 9, phase1_regions: int = 0, pool_max: int = 0):
@@ -501,6 +526,11 @@ class Guide:
         d["rc"] = rc
         d["guide"] = gst.asdict()
         return d
+
+
+def noalloc_capacity(dim: int) -> int:
+    """Regions per list the static device array of integrate_noalloc holds."""
+    return lib().cuhre_noalloc_capacity(dim)
 
 
 def points_per_region(dim: int, key: int = 0) -> int:
