@@ -30,6 +30,11 @@ sys.path.insert(0, os.path.join(ROOT, "old-gpuintegration_amd"))
 import workloads as W  # noqa: E402
 
 FP64_PEAK_TFLOPS = 78.6
+# executed FP64 flops per region of the 8-D product peak's rule evaluation,
+# from the SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 counters (add/mul 1, fma 2,
+# transcendental 1) over a bench run (profiles/eval_seg_product_peak8d.md);
+# the algorithmic model counts 24,426
+EXEC_FLOPS_PER_REGION_PP8 = 23633
 # measured HBM traffic per rule-evaluation launch of the 8-D product peak on
 # this workload (k_eval_half for lists up to 8192 regions, k_eval_seg above;
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE averaged over the 13 launches of a
@@ -92,21 +97,64 @@ def check_against_exact(problems, per, conv):
     return out
 
 
+def host_info():
+    """CPU model, logical CPUs of the machine and of this process."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = os.cpu_count() or 1
+    return {"cpu_model": model, "nproc": os.cpu_count(), "cpus_allowed": allowed}
+
+
+def _timed(fn):
+    t0 = time.perf_counter()
+    r = fn()
+    return r, time.perf_counter() - t0
+
+
 def cpu_baseline(problems):
-    """The oracle's breadth-first loop (same algorithm, C + OpenMP) on the
-    same workload, on the host cores."""
+    """The oracle on the host cores, on the full benchmark workload (both
+    8-D integrands, maxeval 1e9): its breadth-first loop (the GPU's
+    algorithm, C + OpenMP over the regions of an iteration) on the
+    process's share of the host (at most 16 threads, the CPU share of one
+    GPU on these machines) and single-threaded, and its Cuhre-style
+    global-adaptive heap (largest error first, one region at a time,
+    single-threaded by nature).  value = the multi-threaded breadth-first
+    figure."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    threads = min(16, os.cpu_count() or 1)
-    t0 = time.perf_counter()
-    ne = 0
-    for p in problems:
-        r = oracle.integrate_bfs(p, nthreads=threads)
-        ne += r["neval"]
-    dt = time.perf_counter() - t0
-    return {"value": ne / dt, "unit": "evals/s", "cores": threads, "kind": "port",
-            "sample": f"full workload: oracle breadth-first GM7/5 loop on both 8-D integrands "
-                      f"(maxeval 1e9), {ne} evals in {dt:.2f} s, OpenMP over regions"}
+    info = host_info()
+    threads = max(1, min(16, info["cpus_allowed"]))
+    flavours = {}
+    for name, fn in (("bfs_threads", lambda p: oracle.integrate_bfs(p, nthreads=threads)),
+                     ("bfs_1thread", lambda p: oracle.integrate_bfs(p, nthreads=1)),
+                     ("heap_1thread", lambda p: oracle.integrate_heap(p))):
+        ne, dt, per = 0, 0.0, []
+        for p in problems:
+            r, t = _timed(lambda: fn(p))
+            ne += r["neval"]
+            dt += t
+            per.append({"integrand": p.name, "rc": r["rc"], "integral": r["integral"], "error": r["error"],
+                        "nregions": r["nregions"], "seconds": t})
+        flavours[name] = {"evals_per_s": ne / dt, "neval": ne, "seconds": dt, "per_integrand": per}
+    main = flavours["bfs_threads"]
+    return {"value": main["evals_per_s"], "unit": "evals/s", "cores": threads, "kind": "port",
+            "sample": f"full workload: the oracle's breadth-first GM7/5 loop on both 8-D integrands "
+                      f"(maxeval 1e9), {main['neval']} evals in {main['seconds']:.2f} s on {threads} "
+                      f"threads (OpenMP over regions); single-threaded and heap flavours below",
+            "host": info, "threads": threads,
+            "bfs_1thread_evals_per_s": flavours["bfs_1thread"]["evals_per_s"],
+            "heap_1thread_evals_per_s": flavours["heap_1thread"]["evals_per_s"],
+            "flavours": flavours}
 
 
 def bench_batched(ctx, count=4096):
@@ -211,14 +259,19 @@ def bench_reference(ctx):
     on the 8-D workload and the 7-D corner peak, against the default plain
     GM7/5 method on the same integrals."""
     rows = []
+    keys = ("rc", "fail", "integral", "error", "nregions", "neval", "iterations")
     for p, me in ((W.c2_oscillatory8(), 10**9), (W.c2_product_peak8(), 10**12), (W.c5_corner7(), 10**11)):
         ref = min((ctx.integrate_two_phase(p, key=9) for _ in range(3)), key=lambda r: r["ms_total"])
+        # the same two phases with the GM7/5 rule, whose |I7 - I5| error is
+        # the certifiable one
+        ref7 = min((ctx.integrate_two_phase(p, key=7) for _ in range(2)), key=lambda r: r["ms_total"])
         plain = min((ctx.integrate(p, maxeval=me) for _ in range(3)), key=lambda r: r["ms_total"])
+        plain9 = min((ctx.integrate(p, maxeval=me, key=9) for _ in range(3)), key=lambda r: r["ms_total"])
         rows.append({"integrand": p.name,
-                     "reference_method": {k: ref[k] for k in ("rc", "fail", "integral", "error", "nregions", "neval",
-                                                              "iterations")} | {"ms": ref["ms_total"]},
-                     "plain_gm75": {k: plain[k] for k in ("rc", "integral", "error", "nregions", "neval")}
-                     | {"ms": plain["ms_total"], "maxeval": me}})
+                     "reference_method": {k: ref[k] for k in keys} | {"ms": ref["ms_total"]},
+                     "two_phase_gm75": {k: ref7[k] for k in keys} | {"ms": ref7["ms_total"]},
+                     "plain_gm75": {k: plain[k] for k in keys} | {"ms": plain["ms_total"], "maxeval": me},
+                     "plain_degree9": {k: plain9[k] for k in keys} | {"ms": plain9["ms_total"], "maxeval": me}})
     return rows
 
 
@@ -367,6 +420,7 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "fp64", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": TRAFFIC_PER_LAUNCH,
+                     "frac_executed_flops": achieved * EXEC_FLOPS_PER_REGION_PP8 / fl / FP64_PEAK_TFLOPS,
                      "traffic_note": "HBM bytes per launch of this kernel on this workload, FETCH_SIZE x2 + "
                                      "WRITE_SIZE x1 (copy-kernel calibration), profiles/eval_seg_product_peak8d.md; "
                                      "algorithmic 153 B/region x 159,134 regions = 24.3 MB",
@@ -434,7 +488,7 @@ def main():
             import oracle
             for row, p in zip(out["reference_method"], (W.c2_oscillatory8(), W.c2_product_peak8(), W.c5_corner7())):
                 ex, _ = oracle.exact(p.f, p.lows, p.highs)
-                for k in ("reference_method", "plain_gm75"):
+                for k in ("reference_method", "two_phase_gm75", "plain_gm75", "plain_degree9"):
                     row[k]["true_rel_err"] = abs(row[k]["integral"] - ex) / abs(ex)
         if "learned_vs_plain_8d" in out:
             sys.path.insert(0, os.path.join(ROOT, "oracle"))
