@@ -159,13 +159,22 @@ def cpu_baseline(problems):
 
 def bench_batched(ctx, count=4096):
     """4096 independent 5-D integrals (Gaussian / product peak / corner peak,
-    epsrel 1e-5 each): one batched call vs one plain call per integral."""
+    epsrel 1e-5 each): one batched call vs one plain call per integral, with
+    the plain per-region test (results equal the plain calls') and with the
+    shared-allotment finish policy (finish_budget 0.25)."""
     probs = W.c4_batch(count=count)
     cfg, table = ctx.batch_autotune(probs)
     ctx.integrate_batch(probs)  # warm-up (graph capture)
     t0 = time.perf_counter()
     res, info, rc = ctx.integrate_batch(probs)
     tb = time.perf_counter() - t0
+    ctx.integrate_batch(probs, finish_budget=0.25)
+    ta = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        ares, ainfo, arc = ctx.integrate_batch(probs, finish_budget=0.25)
+        ta.append(time.perf_counter() - t0)
+    ta = min(ta)
     t0 = time.perf_counter()
     plain = [ctx.integrate(p) for p in probs]
     tp = time.perf_counter() - t0
@@ -179,7 +188,15 @@ def bench_batched(ctx, count=4096):
             "identical_instances": same, "not_converged": info["not_converged"],
             "iterations": info["iterations"], "graph_used": info["graph_used"], "autotuned": cfg,
             "autotune_ms": table.tolist(),
-            "per_instance_regions_min_max": [min(r["nregions"] for r in res), max(r["nregions"] for r in res)]}
+            "per_instance_regions_min_max": [min(r["nregions"] for r in res), max(r["nregions"] for r in res)],
+            "allotment": {"finish_budget": 0.25, "batched_s": ta, "rc": arc,
+                          "regions": ainfo["total_regions"], "evals_per_s": ainfo["total_neval"] / ta,
+                          "regions_vs_plain_calls": ainfo["total_regions"] / sum(r["nregions"] for r in plain),
+                          "wall_vs_plain_calls": ta / tp, "not_converged": ainfo["not_converged"],
+                          "iterations": ainfo["iterations"],
+                          "per_instance_regions_min_max": [min(r["nregions"] for r in ares),
+                                                           max(r["nregions"] for r in ares)]},
+            "_results": {"plain_policy": res, "allotment": ares}}
 
 
 def _learned_family(ctx, dim, train, held, maxeval, train_maxeval, samples_per_run, policies, rounds=3):
@@ -490,6 +507,23 @@ def main():
                 ex, _ = oracle.exact(p.f, p.lows, p.highs)
                 for k in ("reference_method", "two_phase_gm75", "plain_gm75", "plain_degree9"):
                     row[k]["true_rel_err"] = abs(row[k]["integral"] - ex) / abs(ex)
+        if "batched" in out:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import oracle
+            probs = W.c4_batch()
+            for key in ("plain_policy", "allotment"):
+                rs = out["batched"]["_results"][key]
+                outside = sum(1 for p, r in zip(probs, rs)
+                              if r["fail"] == 0 and abs(r["integral"] - oracle.exact(p.f, p.lows, p.highs)[0])
+                              > r["error"])
+                outside_tol = sum(1 for p, r in zip(probs, rs)
+                                  if r["fail"] == 0 and abs(r["integral"] - oracle.exact(p.f, p.lows, p.highs)[0])
+                                  > p.epsrel * abs(r["integral"]))
+                certified = sum(1 for r in rs if r["fail"] == 0)
+                tgt = out["batched"] if key == "plain_policy" else out["batched"]["allotment"]
+                tgt["certified"] = certified
+                tgt["certified_outside_error_of_closed_form"] = outside
+                tgt["certified_outside_epsrel_of_closed_form"] = outside_tol
         if "learned_vs_plain_8d" in out:
             sys.path.insert(0, os.path.join(ROOT, "oracle"))
             import oracle
@@ -515,6 +549,8 @@ def main():
                                      "plain_gm75": {"integral": b["integral"], "nregions": b["nregions"],
                                                     "ms": (t2 - t1) * 1e3}, "threads": 16}
     ctx.close()
+    if "batched" in out:
+        del out["batched"]["_results"]
     print(json.dumps(out))
 
 

@@ -212,6 +212,15 @@ typedef struct {
   int32_t pad;
   int64_t initial_capacity;     /* first list allocation in regions (0: 4 Mi);
                                    the list grows on demand                   */
+  double finish_budget;         /* finish policy per instance: 0 the plain
+                                   per-region test err <= max(epsrel |est|,
+                                   epsabs) (results equal plain calls); b in
+                                   (0, 1) the shared allotment err <= max(epsrel
+                                   ((1 - b) |est| + b |I| vrel), epsabs), I the
+                                   instance's estimate in the iteration, vrel the
+                                   region's share of its box.  The global test
+                                   sum err <= max(epsrel |I|, epsabs) is the
+                                   only convergence test either way.          */
 } cuhre_batch_config;
 
 typedef struct {
@@ -264,14 +273,14 @@ typedef struct {
   int32_t axis_mode;        /* 0: fourth difference; 1: guide when confident   */
   int32_t finish_mode;      /* 0: none; 1: guide proposal + volume-share guard;
                                2: guard alone (no network);
-                               3: error budget: the per-region test is off and
-                                  each iteration retires the smallest errors
-                                  that fit in budget * (tol - frozen error)   */
+                               3: shared allotment: the per-region test
+                                  becomes err <= max(epsrel ((1 - budget) |est|
+                                  + budget |I| vrel), epsabs), I the iteration's
+                                  estimate, vrel the region's share of the box */
   float axis_confidence;    /* softmax probability needed to override the axis */
   float finish_threshold;   /* sigmoid output needed to propose a finish       */
   double budget;            /* modes 1, 2: share of tol retired regions may use;
-                               mode 3: share of the remaining slack retired
-                               per iteration                                  */
+                               mode 3: the allotment's volume share b         */
 } cuhre_guide_policy;
 
 typedef struct {

@@ -11,12 +11,14 @@
 //
 //   k_batch_eval    rule evaluation, grid-stride over 64-region groups, the
 //                   lane's integrand parameters fetched by instance id;
-//   k_batch_reduce  one block per instance: TwoSum-carried sums of its slice,
-//                   global test, budget test, active count;
+//   k_chunk_reduce  one block per 8192-region chunk of an instance's slice:
+//                   TwoSum-carried sums and active count of the chunk;
+//   k_inst_combine  one block per instance: its chunks in order, global
+//                   test, budget test, children count;
 //   k_batch_scan    one block: next slice offsets of the instances (with a
 //                   capacity cut) and the next list length;
-//   k_tile_count /  per-256-region active counts and their scan;
-//   k_tile_scan
+//   k_tile_count /  per-256-region active counts, scanned within groups
+//   k_group_scan    of 64 tiles, then the group totals;
 //   k_batch_split   stable compaction + bisection into the other buffer.
 //
 // Because no launch depends on a host-side size, two iterations (one per
@@ -42,6 +44,10 @@ RuleConst make_rule(int d, int key);
 int gm_points(int d);
 int set_error(int code, const char* msg);
 
+// Regions per reduction chunk: an instance's slice is reduced by one block
+// per chunk, then its chunks are combined in order by one block.
+constexpr int64_t kChunk = 8192;
+
 // Per-instance parameters and tolerances (uploaded once per call).
 struct BatchParams {
   int32_t family, exponent;
@@ -50,6 +56,7 @@ struct BatchParams {
   double epsrel, epsabs;
   int64_t maxeval;
   double lows[kMaxDim], highs[kMaxDim];
+  double allot;  // shared-allotment share b (0: the plain per-region test)
 };
 
 // Per-instance loop state, updated on the device.
@@ -61,6 +68,7 @@ struct InstState {
   int64_t start, count;  // slice of the current list
   int64_t next;          // children for the next iteration (2 * active)
   int32_t done, fail, iters, pad;
+  double absI;           // |estimate| of the current iteration (allotment)
 };
 
 // Integrand of one lane in a batch: family FAM with the lane's instance
@@ -91,9 +99,15 @@ struct BatchBuffers {
   double* est;
   double* err;
   double* abssum;
+  double* vrel;      // region volume / instance box volume (allotment only)
   int8_t* axis;
-  int64_t* tile_cnt;
-  int64_t* tile_off;
+  Partial* chunk_part;  // per chunk of an instance's slice
+  double2* chunk_est;   // per chunk: estimate sum (the allotment's first pass)
+  int64_t* chunk0;      // per instance: its first chunk; [count]: total chunks
+  int64_t* nch;         // total chunks of the current list
+  int64_t* tile_off;  // per 256-region tile: exclusive offset within its group of 64 tiles
+  int64_t* grp;       // per group of 64 tiles: [0, ngrp) active totals, [gcap, gcap + ngrp) bases
+  int64_t gcap;
   const BatchParams* params;
   InstState* state;
   int32_t count;
@@ -102,10 +116,26 @@ struct BatchBuffers {
   int64_t cap;
 };
 
-__device__ __forceinline__ bool batch_active(const InstState& st, const BatchParams& bp, double e,
-                                             double er)
+// Per-region test of an instance: the reference's err > max(epsrel |est|,
+// epsabs) (GPUQuadPhases.cu:63-73), or with the shared allotment b > 0
+// err > max(epsrel ((1 - b) |est| + b |I| vrel), epsabs), I the instance's
+// estimate in this iteration and vrel the region's share of its box.
+__device__ __forceinline__ bool inst_region_active(const BatchParams& bp, double absI, double e, double er,
+                                                   double vrel)
 {
-  return !st.done && isfinite(e) && isfinite(er) && region_active(e, er, bp.epsrel, bp.epsabs);
+  if (!(isfinite(e) && isfinite(er))) return false;
+  if (bp.allot > 0.0) {
+    const double a = bp.epsrel * ((1.0 - bp.allot) * fabs(e) + bp.allot * absI * vrel);
+    return er > fmax(a, bp.epsabs);
+  }
+  return region_active(e, er, bp.epsrel, bp.epsabs);
+}
+
+__device__ __forceinline__ bool batch_active(const BatchBuffers& B, const InstState& st, const BatchParams& bp,
+                                             int64_t r)
+{
+  if (st.done) return false;
+  return inst_region_active(bp, st.absI, B.est[r], B.err[r], bp.allot > 0.0 ? B.vrel[r] : 0.0);
 }
 
 // Initial k^d grid of every instance plus the instance states.
@@ -113,12 +143,18 @@ __global__ void k_batch_init(BatchBuffers B, int k, int64_t n0)
 {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t total = n0 * B.count;
+  const int64_t cpi = (n0 + kChunk - 1) / kChunk;  // chunks per instance
   if (r < B.count) {
     InstState st;
     memset(&st, 0, sizeof st);
     st.start = r * n0;
     st.count = n0;
     B.state[r] = st;
+    B.chunk0[r] = r * cpi;
+  }
+  if (r == 0) {
+    B.chunk0[B.count] = B.count * cpi;
+    *B.nch = B.count * cpi;
   }
   if (r >= total) return;
   const int i = (int)(r / n0);
@@ -179,19 +215,103 @@ __global__ __launch_bounds__(kBlock) void k_batch_eval(BatchBuffers B, RuleConst
     const bool valid = r < n;
     const int64_t rr = valid ? r : n - 1;
     const RegionOut o = batch_seg_eval<D, FAM>(B, R, r, valid, rr, S, s, wave, lane, W, seg_lds);
-    if (valid && s == 0 && B.params[B.inst[r]].family == FAM) {
-      B.est[r] = o.est;
-      B.err[r] = o.err;
-      B.abssum[r] = o.ab;
-      B.axis[r] = (int8_t)o.axis;
+    if (valid && s == 0) {
+      const BatchParams& bp = B.params[B.inst[r]];
+      if (bp.family == FAM) {
+        B.est[r] = o.est;
+        B.err[r] = o.err;
+        B.abssum[r] = o.ab;
+        B.axis[r] = (int8_t)o.axis;
+        if (bp.allot > 0.0) {
+          double v = 1.0;
+#pragma unroll
+          for (int m = 0; m < D; ++m) v *= B.width[m * B.ld + r] / (bp.highs[m] - bp.lows[m]);
+          B.vrel[r] = v;
+        }
+      }
     }
   }
 }
 
-// One block per instance: sums over the instance's slice and its loop step.
-__global__ __launch_bounds__(kBlock) void k_batch_reduce(BatchBuffers B)
+// The instance whose slice holds chunk c: the last i with chunk0[i] <= c
+// (instances without chunks share chunk0 with the next one that has some).
+__device__ __forceinline__ int chunk_owner(const int64_t* chunk0, int count, int64_t c)
 {
-  __shared__ Partial ldsw[kMaxWaves];
+  int lo = 0, hi = count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (chunk0[mid] <= c) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// One block per chunk (grid-stride): the chunk's Partial, or with
+// EST_ONLY its estimate sum alone (the allotment's test needs the
+// instance's estimate before it can classify).
+template <bool EST_ONLY>
+__global__ __launch_bounds__(kBlock) void k_chunk_reduce(BatchBuffers B)
+{
+  __shared__ Partial ldsw[kBlock / kWave];
+  if (B.flags[0] != 0) return;
+  const int64_t nch = *B.nch;
+  for (int64_t c = blockIdx.x; c < nch; c += gridDim.x) {
+    const int i = chunk_owner(B.chunk0, B.count, c);
+    const InstState& st = B.state[i];
+    const BatchParams& bp = B.params[i];
+    const int64_t r0 = st.start + (c - B.chunk0[i]) * kChunk;
+    const int64_t end = st.start + st.count;
+    const int64_t r1 = r0 + kChunk < end ? r0 + kChunk : end;
+    const double absI = st.absI;
+    Partial q;
+    partial_zero(q);
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += kBlock) {
+      const double e = B.est[r];
+      if constexpr (EST_ONLY) {
+        dd_add(q.est_s, q.est_c, e, 0.0);
+      } else {
+        const double er = B.err[r];
+        const bool act = inst_region_active(bp, absI, e, er, bp.allot > 0.0 ? B.vrel[r] : 0.0);
+        Partial one;
+        partial_from_region_act(one, e, er, B.abssum[r], act);
+        partial_add(q, one);
+      }
+    }
+    block_reduce_partial(q, ldsw, kBlock / kWave);
+    if (threadIdx.x == 0) {
+      if constexpr (EST_ONLY) B.chunk_est[c] = make_double2(q.est_s, q.est_c);
+      else B.chunk_part[c] = q;
+    }
+    __syncthreads();  // ldsw is written again by the next chunk
+  }
+}
+
+// One block per instance: |estimate| of the iteration (finished so far plus
+// every region of the iteration) for the allotment's test.
+__global__ __launch_bounds__(kBlock) void k_inst_absI(BatchBuffers B)
+{
+  __shared__ Partial ldsw[kBlock / kWave];
+  const int i = blockIdx.x;
+  if (B.flags[0] != 0) return;
+  const InstState& st = B.state[i];
+  if (st.done) return;
+  Partial q;
+  partial_zero(q);
+  for (int64_t k = B.chunk0[i] + threadIdx.x; k < B.chunk0[i + 1]; k += kBlock)
+    dd_add(q.est_s, q.est_c, B.chunk_est[k].x, B.chunk_est[k].y);
+  block_reduce_partial(q, ldsw, kBlock / kWave);
+  if (threadIdx.x == 0) {
+    double s = st.fin_s, c = st.fin_c;
+    dd_add(s, c, q.est_s, q.est_c);
+    B.state[i].absI = fabs(s + c);
+  }
+}
+
+// One block per instance: its chunks' Partials in chunk order, then the
+// instance's loop step (global test, budget, children count).
+__global__ __launch_bounds__(kBlock) void k_inst_combine(BatchBuffers B)
+{
+  __shared__ Partial ldsw[kBlock / kWave];
   const int i = blockIdx.x;
   if (B.flags[0] != 0) return;
   InstState st = B.state[i];
@@ -199,11 +319,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_reduce(BatchBuffers B)
   const BatchParams& bp = B.params[i];
   Partial q;
   partial_zero(q);
-  for (int64_t r = st.start + threadIdx.x; r < st.start + st.count; r += kBlock) {
-    Partial one;
-    partial_from_region(one, B.est[r], B.err[r], B.abssum[r], true, bp.epsrel, bp.epsabs);
-    partial_add(q, one);
-  }
+  for (int64_t k = B.chunk0[i] + threadIdx.x; k < B.chunk0[i + 1]; k += kBlock) partial_add(q, B.chunk_part[k]);
   block_reduce_partial(q, ldsw, kBlock / kWave);
   if (threadIdx.x != 0) return;
   const int64_t p = B.npts;
@@ -309,58 +425,110 @@ __global__ __launch_bounds__(1024) void k_batch_scan(BatchBuffers B)
     run += c;
   }
   atomicAdd(&s_tot, kept);
+  // chunk map of the next list: the chunk counts scanned in instance order
+  int64_t my_ch = 0;
+  for (int i = i0; i < i1; ++i)
+    if (!B.state[i].done) my_ch += (B.state[i].count + kChunk - 1) / kChunk;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    *B.n_next = (int64_t)s_tot;
-    if (s_tot > *B.n_peak) *B.n_peak = s_tot;
-  }
-}
-
-__global__ __launch_bounds__(kBlock) void k_tile_count(BatchBuffers B)
-{
-  __shared__ uint32_t wcnt[kBlock / kWave];
-  if (B.flags[0] != 0) return;
-  const int64_t n = *B.n;
-  const int64_t ntiles = (n + kBlock - 1) / kBlock;
-  for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    const int64_t r = t * kBlock + threadIdx.x;
-    bool act = false;
-    if (r < n) {
-      const int i = B.inst[r];
-      act = batch_active(B.state[i], B.params[i], B.est[r], B.err[r]);
-    }
-    const uint64_t mask = __ballot(act);
-    if ((threadIdx.x & (kWave - 1)) == 0) wcnt[threadIdx.x / kWave] = (uint32_t)__popcll(mask);
-    __syncthreads();
-    if (threadIdx.x == 0) B.tile_cnt[t] = (int64_t)wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
-    __syncthreads();
-  }
-}
-
-__global__ __launch_bounds__(1024) void k_tile_scan(BatchBuffers B)
-{
-  __shared__ int64_t sc[1024];
-  if (B.flags[0] != 0) return;
-  const int64_t n = *B.n;
-  const int64_t ntiles = (n + kBlock - 1) / kBlock;
-  const int T = blockDim.x;
-  const int64_t chunk = (ntiles + T - 1) / T;
-  const int64_t t0 = threadIdx.x * chunk;
-  const int64_t t1 = t0 + chunk < ntiles ? t0 + chunk : ntiles;
-  int64_t mine = 0;
-  for (int64_t t = t0; t < t1; ++t) mine += B.tile_cnt[t];
-  sc[threadIdx.x] = mine;
+  sc[threadIdx.x] = my_ch;
   __syncthreads();
   for (int off = 1; off < T; off <<= 1) {
-    const int64_t v = threadIdx.x >= off ? sc[threadIdx.x - off] : 0;
+    const int64_t v = (int)threadIdx.x >= off ? sc[threadIdx.x - off] : 0;
     __syncthreads();
     sc[threadIdx.x] += v;
     __syncthreads();
   }
-  int64_t run = sc[threadIdx.x] - mine;
-  for (int64_t t = t0; t < t1; ++t) {
-    B.tile_off[t] = run;
-    run += B.tile_cnt[t];
+  int64_t cb = sc[threadIdx.x] - my_ch;
+  for (int i = i0; i < i1; ++i) {
+    B.chunk0[i] = cb;
+    if (!B.state[i].done) cb += (B.state[i].count + kChunk - 1) / kChunk;
+  }
+  if (threadIdx.x == 0) {
+    *B.n_next = (int64_t)s_tot;
+    if (s_tot > *B.n_peak) *B.n_peak = s_tot;
+    B.chunk0[B.count] = sc[T - 1];
+    *B.nch = sc[T - 1];
+  }
+}
+
+// Active counts of the 256-region tiles, a group of 64 tiles per block
+// iteration: tile by tile (coalesced loads, a ballot per wave), then one
+// wavefront scans the group's 64 counts into offsets within the group and
+// the group's total.
+__global__ __launch_bounds__(kBlock) void k_tile_count(BatchBuffers B)
+{
+  __shared__ uint32_t wcnt[kBlock / kWave];
+  __shared__ int64_t gcnt[kWave];
+  if (B.flags[0] != 0) return;
+  const int64_t n = *B.n;
+  const int64_t ntiles = (n + kBlock - 1) / kBlock;
+  const int64_t ngrp = (ntiles + kWave - 1) / kWave;
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  for (int64_t g = blockIdx.x; g < ngrp; g += gridDim.x) {
+    for (int j = 0; j < kWave; ++j) {
+      const int64_t r = (g * kWave + j) * kBlock + threadIdx.x;
+      bool act = false;
+      if (r < n) {
+        const int i = B.inst[r];
+        act = batch_active(B, B.state[i], B.params[i], r);
+      }
+      const uint64_t mask = __ballot(act);
+      if (lane == 0) wcnt[wave] = (uint32_t)__popcll(mask);
+      __syncthreads();
+      if (threadIdx.x == 0) gcnt[j] = (int64_t)wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+      __syncthreads();
+    }
+    if (wave == 0) {
+      const int64_t t = g * kWave + lane;
+      const int64_t mine = gcnt[lane];
+      long long incl = (long long)mine;
+#pragma unroll
+      for (int off = 1; off < kWave; off <<= 1) {
+        const long long v = __shfl_up(incl, off, kWave);
+        if (lane >= off) incl += v;
+      }
+      if (t < ntiles) B.tile_off[t] = (int64_t)incl - mine;
+      if (lane == kWave - 1) B.grp[g] = (int64_t)incl;
+    }
+    __syncthreads();
+  }
+}
+
+// Base offsets of the groups: one block scans the group totals in rounds of
+// 1024 (coalesced), carrying the running total between rounds.
+__global__ __launch_bounds__(1024) void k_group_scan(BatchBuffers B)
+{
+  __shared__ int64_t wsum[kMaxWaves];
+  __shared__ int64_t s_carry;
+  if (B.flags[0] != 0) return;
+  const int64_t n = *B.n;
+  const int64_t ntiles = (n + kBlock - 1) / kBlock;
+  const int64_t ngrp = (ntiles + kWave - 1) / kWave;
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  const int W = blockDim.x / kWave;
+  if (threadIdx.x == 0) s_carry = 0;
+  __syncthreads();
+  for (int64_t g0 = 0; g0 < ngrp; g0 += blockDim.x) {
+    const int64_t g = g0 + threadIdx.x;
+    const int64_t mine = g < ngrp ? B.grp[g] : 0;
+    long long incl = (long long)mine;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      const long long v = __shfl_up(incl, off, kWave);
+      if (lane >= off) incl += v;
+    }
+    if (lane == kWave - 1) wsum[wave] = (int64_t)incl;
+    __syncthreads();
+    int64_t base = s_carry;
+    for (int w = 0; w < wave; ++w) base += wsum[w];
+    if (g < ngrp) B.grp[B.gcap + g] = base + (int64_t)incl - mine;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int64_t t = 0;
+      for (int w = 0; w < W; ++w) t += wsum[w];
+      s_carry += t;
+    }
+    __syncthreads();
   }
 }
 
@@ -378,7 +546,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_split(BatchBuffers B)
     int i = 0;
     if (r < n) {
       i = B.inst[r];
-      act = batch_active(B.state[i], B.params[i], B.est[r], B.err[r]);
+      act = batch_active(B, B.state[i], B.params[i], r);
     }
     const uint64_t mask = __ballot(act);
     if (lane == 0) wcnt[wave] = (uint32_t)__popcll(mask);
@@ -387,7 +555,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_split(BatchBuffers B)
     for (int w = 0; w < wave; ++w) rank += wcnt[w];
     __syncthreads();
     if (!act) continue;
-    const int64_t dst = B.tile_off[t] + rank;
+    const int64_t dst = B.tile_off[t] + B.grp[B.gcap + (t >> 6)] + rank;
     const int a = B.axis[r];
     for (int m = 0; m < B.dim; ++m) {
       const double lo = B.lower[m * B.ld + r];
@@ -478,10 +646,13 @@ struct BatchEngine {
   double* lo[2] = {nullptr, nullptr};
   double* wd[2] = {nullptr, nullptr};
   int32_t* inst[2] = {nullptr, nullptr};
-  double *est = nullptr, *err = nullptr, *abssum = nullptr;
+  double *est = nullptr, *err = nullptr, *abssum = nullptr, *vrel = nullptr;
   int8_t* axis = nullptr;
-  int64_t *tile_cnt = nullptr, *tile_off = nullptr;
-  int64_t* n = nullptr;  // [0,1] list lengths, [2] peak, [3] overflow, [4] needed length
+  int64_t *tile_off = nullptr, *grp = nullptr;
+  Partial* chunk_part = nullptr;
+  double2* chunk_est = nullptr;
+  int64_t* chunk0 = nullptr;  // per instance (+1), kept with the states
+  int64_t* n = nullptr;  // [0,1] list lengths, [2] peak, [3] overflow, [4] needed length, [5] chunks
   BatchParams* params = nullptr;
   InstState* state = nullptr;
   std::vector<InstState> h_state;
@@ -489,11 +660,12 @@ struct BatchEngine {
   std::vector<int> perm;  // list position k holds caller instance perm[k]
   unsigned fam_mask = 0;
   int allow_cut = 0;
-  int64_t h_n[5] = {0, 0, 0, 0, 0};
+  int allot = 0;  // the batch uses the shared allotment (extra first pass)
+  int64_t h_n[6] = {0, 0, 0, 0, 0, 0};
   // cached graph of two iterations for one buffer set and launch shape
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
-  int g_count = -1, g_dim = -1, g_S = -1, g_grid = -1, g_cut = -1;
+  int g_count = -1, g_dim = -1, g_S = -1, g_grid = -1, g_cut = -1, g_allot = -1;
   unsigned g_mask = 0;
   int64_t g_cap = -1;
   // autotune cache
@@ -513,9 +685,11 @@ struct ListArrays {
   double* lo[2] = {nullptr, nullptr};
   double* wd[2] = {nullptr, nullptr};
   int32_t* inst[2] = {nullptr, nullptr};
-  double *est = nullptr, *err = nullptr, *abssum = nullptr;
+  double *est = nullptr, *err = nullptr, *abssum = nullptr, *vrel = nullptr;
   int8_t* axis = nullptr;
-  int64_t *tile_cnt = nullptr, *tile_off = nullptr;
+  int64_t *tile_off = nullptr, *grp = nullptr;
+  Partial* chunk_part = nullptr;
+  double2* chunk_est = nullptr;
 };
 
 static void free_arrays(ListArrays& a)
@@ -525,24 +699,31 @@ static void free_arrays(ListArrays& a)
     if (a.wd[p]) (void)hipFree(a.wd[p]);
     if (a.inst[p]) (void)hipFree(a.inst[p]);
   }
-  for (void* q : {(void*)a.est, (void*)a.err, (void*)a.abssum, (void*)a.axis, (void*)a.tile_cnt,
-                  (void*)a.tile_off})
+  for (void* q : {(void*)a.est, (void*)a.err, (void*)a.abssum, (void*)a.vrel, (void*)a.axis, (void*)a.grp,
+                  (void*)a.tile_off, (void*)a.chunk_part, (void*)a.chunk_est})
     if (q) (void)hipFree(q);
   a = ListArrays();
 }
 
-static bool alloc_arrays(ListArrays& a, int64_t cap, int dim)
+// Chunks a list of cap regions split among count instances can have.
+static int64_t chunk_capacity(int64_t cap, int count) { return cap / kChunk + count + 2; }
+
+static bool alloc_arrays(ListArrays& a, int64_t cap, int dim, int count)
 {
   const size_t lb = sizeof(double) * (size_t)cap * dim;
   const int64_t ntiles = cap / kBlock + 1;
+  const int64_t gcap = ntiles / kWave + 2;
   bool ok = true;
   for (int p = 0; p < 2; ++p)
     ok = ok && hipMalloc(&a.lo[p], lb) == hipSuccess && hipMalloc(&a.wd[p], lb) == hipSuccess &&
          hipMalloc(&a.inst[p], sizeof(int32_t) * cap) == hipSuccess;
   ok = ok && hipMalloc(&a.est, sizeof(double) * cap) == hipSuccess &&
        hipMalloc(&a.err, sizeof(double) * cap) == hipSuccess &&
-       hipMalloc(&a.abssum, sizeof(double) * cap) == hipSuccess && hipMalloc(&a.axis, cap) == hipSuccess &&
-       hipMalloc(&a.tile_cnt, sizeof(int64_t) * ntiles) == hipSuccess &&
+       hipMalloc(&a.abssum, sizeof(double) * cap) == hipSuccess &&
+       hipMalloc(&a.vrel, sizeof(double) * cap) == hipSuccess && hipMalloc(&a.axis, cap) == hipSuccess &&
+       hipMalloc(&a.grp, sizeof(int64_t) * 2 * gcap) == hipSuccess &&
+       hipMalloc(&a.chunk_part, sizeof(Partial) * chunk_capacity(cap, count)) == hipSuccess &&
+       hipMalloc(&a.chunk_est, sizeof(double2) * chunk_capacity(cap, count)) == hipSuccess &&
        hipMalloc(&a.tile_off, sizeof(int64_t) * ntiles) == hipSuccess;
   if (!ok) {
     (void)hipGetLastError();
@@ -562,9 +743,12 @@ static ListArrays take_arrays(BatchEngine* e)
   a.est = e->est;
   a.err = e->err;
   a.abssum = e->abssum;
+  a.vrel = e->vrel;
   a.axis = e->axis;
-  a.tile_cnt = e->tile_cnt;
+  a.grp = e->grp;
   a.tile_off = e->tile_off;
+  a.chunk_part = e->chunk_part;
+  a.chunk_est = e->chunk_est;
   return a;
 }
 
@@ -578,9 +762,12 @@ static void put_arrays(BatchEngine* e, const ListArrays& a)
   e->est = a.est;
   e->err = a.err;
   e->abssum = a.abssum;
+  e->vrel = a.vrel;
   e->axis = a.axis;
-  e->tile_cnt = a.tile_cnt;
+  e->grp = a.grp;
   e->tile_off = a.tile_off;
+  e->chunk_part = a.chunk_part;
+  e->chunk_est = a.chunk_est;
 }
 
 static void free_batch(BatchEngine* e)
@@ -589,8 +776,9 @@ static void free_batch(BatchEngine* e)
   ListArrays a = take_arrays(e);
   free_arrays(a);
   put_arrays(e, a);
-  for (void* q : {(void*)e->n, (void*)e->params, (void*)e->state})
+  for (void* q : {(void*)e->n, (void*)e->params, (void*)e->state, (void*)e->chunk0})
     if (q) (void)hipFree(q);
+  e->chunk0 = nullptr;
   e->n = nullptr;
   e->params = nullptr;
   e->state = nullptr;
@@ -609,7 +797,7 @@ void batch_engine_delete(BatchEngine* e)
 
 static size_t bytes_per_region(int dim)
 {
-  return (size_t)dim * 4 * sizeof(double) + 2 * sizeof(int32_t) + 3 * sizeof(double) + 1 +
+  return (size_t)dim * 4 * sizeof(double) + 2 * sizeof(int32_t) + 4 * sizeof(double) + 1 +
          2 * sizeof(int64_t) / kBlock + 1;
 }
 
@@ -632,10 +820,11 @@ static int batch_reserve(BatchEngine* e, int64_t cap, int dim, int count)
   count = std::max(count, e->count_alloc);
   free_batch(e);
   ListArrays a;
-  bool ok = alloc_arrays(a, cap, dim);
-  ok = ok && hipMalloc(&e->n, sizeof(int64_t) * 5) == hipSuccess &&
+  bool ok = alloc_arrays(a, cap, dim, count);
+  ok = ok && hipMalloc(&e->n, sizeof(int64_t) * 6) == hipSuccess &&
        hipMalloc(&e->params, sizeof(BatchParams) * count) == hipSuccess &&
-       hipMalloc(&e->state, sizeof(InstState) * count) == hipSuccess;
+       hipMalloc(&e->state, sizeof(InstState) * count) == hipSuccess &&
+       hipMalloc(&e->chunk0, sizeof(int64_t) * (count + 1)) == hipSuccess;
   put_arrays(e, a);
   if (!ok) {
     (void)hipGetLastError();
@@ -654,7 +843,8 @@ static int batch_grow(BatchEngine* e, int64_t cap, int parity, int64_t n, hipStr
 {
   cap = (cap + 255) & ~255ll;
   ListArrays a;
-  if (!alloc_arrays(a, cap, e->dim_alloc)) return set_error(CUHRE_ERR_ALLOC, "cannot grow the batch list");
+  if (!alloc_arrays(a, cap, e->dim_alloc, e->count_alloc))
+    return set_error(CUHRE_ERR_ALLOC, "cannot grow the batch list");
   bool ok = true;
   if (n > 0) {
     const size_t rb = sizeof(double) * (size_t)n;
@@ -667,6 +857,7 @@ static int batch_grow(BatchEngine* e, int64_t cap, int parity, int64_t n, hipStr
          hipMemcpyAsync(a.est, e->est, rb, hipMemcpyDeviceToDevice, st) == hipSuccess &&
          hipMemcpyAsync(a.err, e->err, rb, hipMemcpyDeviceToDevice, st) == hipSuccess &&
          hipMemcpyAsync(a.abssum, e->abssum, rb, hipMemcpyDeviceToDevice, st) == hipSuccess &&
+         hipMemcpyAsync(a.vrel, e->vrel, rb, hipMemcpyDeviceToDevice, st) == hipSuccess &&
          hipMemcpyAsync(a.axis, e->axis, n, hipMemcpyDeviceToDevice, st) == hipSuccess;
   }
   ok = ok && hipStreamSynchronize(st) == hipSuccess;
@@ -701,9 +892,15 @@ static BatchBuffers make_buffers(BatchEngine* e, int parity, int count, int dim)
   B.est = e->est;
   B.err = e->err;
   B.abssum = e->abssum;
+  B.vrel = e->vrel;
   B.axis = e->axis;
-  B.tile_cnt = e->tile_cnt;
   B.tile_off = e->tile_off;
+  B.grp = e->grp;
+  B.gcap = (e->cap / kBlock + 1) / kWave + 2;
+  B.chunk_part = e->chunk_part;
+  B.chunk_est = e->chunk_est;
+  B.chunk0 = e->chunk0;
+  B.nch = e->n + 5;
   B.params = e->params;
   B.state = e->state;
   B.count = count;
@@ -718,7 +915,12 @@ static void enqueue_eval_reduce(BatchEngine* e, int parity, int count, int dim, 
 {
   const BatchBuffers B = make_buffers(e, parity, count, dim);
   batch_eval_launcher(dim)(B, R, S, grid, e->fam_mask, st);
-  hipLaunchKernelGGL(k_batch_reduce, dim3(count), dim3(kBlock), 0, st, B);
+  if (e->allot) {
+    hipLaunchKernelGGL(k_chunk_reduce<true>, dim3(grid), dim3(kBlock), 0, st, B);
+    hipLaunchKernelGGL(k_inst_absI, dim3(count), dim3(kBlock), 0, st, B);
+  }
+  hipLaunchKernelGGL(k_chunk_reduce<false>, dim3(grid), dim3(kBlock), 0, st, B);
+  hipLaunchKernelGGL(k_inst_combine, dim3(count), dim3(kBlock), 0, st, B);
 }
 
 static void enqueue_scan_split(BatchEngine* e, int parity, int count, int dim, int grid, hipStream_t st)
@@ -726,7 +928,7 @@ static void enqueue_scan_split(BatchEngine* e, int parity, int count, int dim, i
   const BatchBuffers B = make_buffers(e, parity, count, dim);
   hipLaunchKernelGGL(k_batch_scan, dim3(1), dim3(1024), 0, st, B);
   hipLaunchKernelGGL(k_tile_count, dim3(grid), dim3(kBlock), 0, st, B);
-  hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, st, B);
+  hipLaunchKernelGGL(k_group_scan, dim3(1), dim3(1024), 0, st, B);
   hipLaunchKernelGGL(k_batch_split, dim3(grid), dim3(kBlock), 0, st, B);
 }
 
@@ -756,7 +958,7 @@ static int check_batch_args(int count, const cuhre_integrand* fs, int dim)
 // Parameters in list order: instances sorted (stably) by family.
 static void fill_params(std::vector<BatchParams>& hp, std::vector<int>& perm, unsigned& mask, int count,
                         const cuhre_integrand* fs, int dim, const double* epsrel, const double* epsabs,
-                        const int64_t* maxeval, const double* lows, const double* highs)
+                        const int64_t* maxeval, const double* lows, const double* highs, double allot)
 {
   perm.resize(count);
   for (int i = 0; i < count; ++i) perm[i] = i;
@@ -780,6 +982,7 @@ static void fill_params(std::vector<BatchParams>& hp, std::vector<int>& perm, un
     b.epsrel = epsrel ? epsrel[i] : 1e-3;
     b.epsabs = epsabs ? epsabs[i] : 0.0;
     b.maxeval = maxeval ? maxeval[i] : 1000000000LL;
+    b.allot = allot;
     for (int m = 0; m < dim; ++m) {
       b.lows[m] = lows ? lows[(size_t)i * dim + m] : 0.0;
       b.highs[m] = highs ? highs[(size_t)i * dim + m] : 1.0;
@@ -792,7 +995,8 @@ static int grid_k_batch(int d) { return d <= 4 ? 4 : (d <= 10 ? 2 : 1); }
 // Uploads the batch and builds its initial list in buffer 0.
 static int batch_setup(BatchEngine* e, hipStream_t st, int count, const cuhre_integrand* fs, int dim,
                        const double* epsrel, const double* epsabs, const int64_t* maxeval,
-                       const double* lows, const double* highs, int64_t cap, int64_t* n0_out)
+                       const double* lows, const double* highs, int64_t cap, int64_t* n0_out,
+                       double allot = 0.0)
 {
   const int k = grid_k_batch(dim);
   int64_t n0 = 1;
@@ -802,7 +1006,7 @@ static int batch_setup(BatchEngine* e, hipStream_t st, int count, const cuhre_in
   int rc = batch_reserve(e, cap, dim, count);
   if (rc) return rc;
   std::vector<BatchParams>& hp = e->h_params;
-  fill_params(hp, e->perm, e->fam_mask, count, fs, dim, epsrel, epsabs, maxeval, lows, highs);
+  fill_params(hp, e->perm, e->fam_mask, count, fs, dim, epsrel, epsabs, maxeval, lows, highs, allot);
   if (hipMemcpyAsync(e->params, hp.data(), sizeof(BatchParams) * count, hipMemcpyHostToDevice, st) !=
       hipSuccess)
     return set_error(CUHRE_ERR_DEVICE, "batch parameter upload failed");
@@ -814,7 +1018,9 @@ static int batch_setup(BatchEngine* e, hipStream_t st, int count, const cuhre_in
   e->h_n[2] = total;
   e->h_n[3] = 0;
   e->h_n[4] = 0;
-  if (hipMemcpyAsync(e->n, e->h_n, sizeof e->h_n, hipMemcpyHostToDevice, st) != hipSuccess)
+  e->h_n[5] = 0;  // the chunk count is written by the init kernel
+  e->allot = allot > 0.0 ? 1 : 0;
+  if (hipMemcpyAsync(e->n, e->h_n, 5 * sizeof(int64_t), hipMemcpyHostToDevice, st) != hipSuccess)
     return set_error(CUHRE_ERR_DEVICE, "batch setup copy failed");
   if (hipGetLastError() != hipSuccess) return set_error(CUHRE_ERR_DEVICE, "batch init launch failed");
   *n0_out = total;
@@ -824,7 +1030,8 @@ static int batch_setup(BatchEngine* e, hipStream_t st, int count, const cuhre_in
 static bool capture_graph(BatchEngine* e, int count, int dim, const RuleConst& R, int S, int G, hipStream_t st)
 {
   const bool same = e->gexec && e->g_count == count && e->g_dim == dim && e->g_S == S && e->g_grid == G &&
-                    e->g_cap == e->cap && e->g_mask == e->fam_mask && e->g_cut == e->allow_cut;
+                    e->g_cap == e->cap && e->g_mask == e->fam_mask && e->g_cut == e->allow_cut &&
+                    e->g_allot == e->allot;
   if (same) return true;
   drop_graph(e);
   bool ok = false;
@@ -841,6 +1048,7 @@ static bool capture_graph(BatchEngine* e, int count, int dim, const RuleConst& R
       e->g_cap = e->cap;
       e->g_mask = e->fam_mask;
       e->g_cut = e->allow_cut;
+      e->g_allot = e->allot;
     }
   }
   (void)hipGetLastError();
@@ -922,6 +1130,7 @@ int cuhre_batch_autotune(cuhre_context* ctx, int32_t count, const cuhre_integran
   best->use_graph = 1;
   best->pad = 0;
   best->initial_capacity = 0;
+  best->finish_budget = 0.0;
   e->tuned_dim = dim;
   e->tuned_S = bs;
   e->tuned_grid = bg;
@@ -952,6 +1161,8 @@ int cuhre_integrate_batch(cuhre_context* ctx, int32_t count, const cuhre_integra
   G = G ? G : 2048;
   if (S != 1 && S != 2 && S != 4) return set_error(CUHRE_ERR_ARG, "batch point_segments must be 1, 2 or 4");
   const int per_check = cfg && cfg->iterations_per_check > 0 ? cfg->iterations_per_check : 2;
+  const double allot = cfg ? cfg->finish_budget : 0.0;
+  if (!(allot >= 0.0) || allot >= 1.0) return set_error(CUHRE_ERR_ARG, "batch finish_budget must be in [0, 1)");
   const bool use_graph = !cfg || cfg->use_graph != 0;
   // list capacity: an explicit engine capacity is a hard limit; otherwise the
   // list starts small and grows, up to three quarters of the device memory
@@ -960,7 +1171,7 @@ int cuhre_integrate_batch(cuhre_context* ctx, int32_t count, const cuhre_integra
   const int64_t first = cfg && cfg->initial_capacity > 0 ? cfg->initial_capacity : (1 << 22);
   if (cfg && cfg->initial_capacity > 0 && e->cap > first) free_batch(e);  // honour a smaller start
   rc = batch_setup(e, st, count, fs, dim, epsrel, epsabs, maxeval, lows, highs,
-                   std::min<int64_t>(limit, std::max<int64_t>(e->cap, first)), &n0);
+                   std::min<int64_t>(limit, std::max<int64_t>(e->cap, first)), &n0, allot);
   if (rc) return rc;
   e->allow_cut = e->cap >= limit ? 1 : 0;
   const RuleConst R = make_rule(dim, 7);

@@ -118,6 +118,25 @@ __device__ __forceinline__ void partial_from_region(Partial& p, double est, doub
   p.nnonfinite = finite ? 0 : 1;
 }
 
+// The same with the region's classification given (act: still active).
+__device__ __forceinline__ void partial_from_region_act(Partial& p, double est, double err, double ab, bool act)
+{
+  partial_zero(p);
+  const bool finite = isfinite(est) && isfinite(err);
+  p.est_s = est;
+  p.err_s = err;
+  p.abs_all = ab;
+  p.aest_all = fabs(est);
+  if (!act) {
+    p.fest_s = est;
+    p.ferr_s = err;
+    p.abs_fin = ab;
+    p.aest_fin = fabs(est);
+  }
+  p.nactive = act ? 1 : 0;
+  p.nnonfinite = finite ? 0 : 1;
+}
+
 // Reduce one Partial per thread to thread 0 of a block of W waves.  Waves
 // whose lanes hold no region (contributes == false, wave-uniform) skip the
 // shuffles and add zeros; thread 0 adds the waves 0, wstep, 2 wstep, ...

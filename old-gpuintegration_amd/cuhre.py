@@ -92,6 +92,7 @@ class BatchConfig(C.Structure):
         ("use_graph", C.c_int32),
         ("pad", C.c_int32),
         ("initial_capacity", C.c_int64),
+        ("finish_budget", C.c_double),
     ]
 
     def asdict(self):
@@ -390,7 +391,7 @@ class Context:
         return count, d, arr, lows, highs, keep
 
     def integrate_batch(self, problems, point_segments=0, grid_blocks=0, iterations_per_check=0,
-                        use_graph=True, initial_capacity=0, raise_nonfinite=True):
+                        use_graph=True, initial_capacity=0, raise_nonfinite=True, finish_budget=0.0):
         """Integrates every problem (same dimension) in one batched call.
         Returns (results list in input order, info dict, return code).  With
         raise_nonfinite=False a non-finite integrand value returns rc -2 with
@@ -400,7 +401,7 @@ class Context:
         ea = np.ascontiguousarray([p.epsabs for p in problems], dtype=np.float64)
         me = np.ascontiguousarray([p.maxeval for p in problems], dtype=np.int64)
         cfg = BatchConfig(point_segments, grid_blocks, iterations_per_check, int(bool(use_graph)), 0,
-                          initial_capacity)
+                          initial_capacity, finish_budget)
         out = (Result * count)()
         info = BatchInfo()
         rc = lib().cuhre_integrate_batch(self.h, count, arr, d, _dp(er), _dp(ea),

@@ -164,6 +164,15 @@ int or_integrate_bfs_check(const or_integrand* f, const double* lows,
                            int64_t maxeval, int64_t capacity, int nthreads,
                            int key, int want_rule, or_result* out, or_check* chk);
 
+/* The breadth-first loop (GM7/5) with the shared-allotment finish policy:
+ * a region is active iff err > max(epsrel ((1 - allot) |est| + allot |I|
+ * vrel), epsabs), I the iteration's estimate (finished so far plus the
+ * iteration's regions), vrel the region's share of the box volume;
+ * allot = 0 is the plain loop.  The Genz-Malik global test is unchanged. */
+int or_integrate_bfs_allot(const or_integrand* f, const double* lows, const double* highs, double epsrel,
+                           double epsabs, int64_t maxeval, int64_t capacity, int nthreads, double allot,
+                           or_result* out);
+
 /* One region with its __float128 shadow: out[0..5] = est, abssum,
  * istar (hi, lo), irule (hi, lo; NaN where no quad restatement). */
 void or_region_check(int key, const or_integrand* f, const double* lower,

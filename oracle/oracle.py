@@ -126,6 +126,8 @@ def lib():
         L.or_integrate_bfs_check.argtypes = [C.POINTER(OrIntegrand), dp, dp, C.c_double, C.c_double,
                                              C.c_int64, C.c_int64, C.c_int, C.c_int, C.c_int,
                                              C.POINTER(OrResult), C.POINTER(OrCheck)]
+        L.or_integrate_bfs_allot.argtypes = [C.POINTER(OrIntegrand), dp, dp, C.c_double, C.c_double,
+                                             C.c_int64, C.c_int64, C.c_int, C.c_double, C.POINTER(OrResult)]
         L.or_fp64_bound.argtypes = [C.c_int, C.c_int, C.c_int64, C.c_double, C.c_double, C.c_double]
         L.or_fp64_bound.restype = C.c_double
         L.or_region_check.argtypes = [C.c_int, C.POINTER(OrIntegrand), dp, dp, dp]
@@ -246,13 +248,20 @@ def npoints(d, key=7):
     return lib().or_r9_npoints(d) if key == 9 else W.gm_npoints(d)
 
 
+def _default_cap(maxeval, d, key):
+    """List capacity of the oracle's loop: what maxeval allows, at most 2^25
+    regions (the oracle allocates its lists up front; no workload of the
+    tests or the bench needs more in one iteration)."""
+    return min(max(1, maxeval // npoints(d, key)) + 64, 1 << 25)
+
+
 def integrate_bfs(p: W.Problem, nthreads=1, capacity=None, maxeval=None, epsrel=None, key=7):
     s, keep = to_struct(p.f)
     lo = np.array(p.lows, dtype=np.float64)
     hi = np.array(p.highs, dtype=np.float64)
     res = OrResult()
     me = p.maxeval if maxeval is None else maxeval
-    cap = capacity if capacity is not None else max(1, me // npoints(p.f.dim, key)) + 64
+    cap = capacity if capacity is not None else _default_cap(me, p.f.dim, key)
     rc = lib().or_integrate_bfs_key(C.byref(s), _dp(lo), _dp(hi), p.epsrel if epsrel is None else epsrel,
                                     p.epsabs, me, cap, nthreads, key, C.byref(res))
     d = res.asdict()
@@ -272,13 +281,29 @@ def integrate_bfs_check(p: W.Problem, nthreads=8, maxeval=None, epsrel=None, eps
     res = OrResult()
     chk = OrCheck()
     me = p.maxeval if maxeval is None else maxeval
-    cap = max(1, me // npoints(p.f.dim, key)) + 64
+    cap = _default_cap(me, p.f.dim, key)
     rc = lib().or_integrate_bfs_check(C.byref(s), _dp(lo), _dp(hi), p.epsrel if epsrel is None else epsrel,
                                       p.epsabs if epsabs is None else epsabs, me, cap, nthreads, key,
                                       1 if want_rule else 0, C.byref(res), C.byref(chk))
     d = res.asdict()
     d["rc"] = rc
     d.update(chk.asdict())
+    return d
+
+
+def integrate_bfs_allot(p: W.Problem, allot, nthreads=8, maxeval=None, epsrel=None):
+    """The breadth-first GM7/5 loop with the shared-allotment finish policy
+    (allot = b; 0 is the plain loop)."""
+    s, keep = to_struct(p.f)
+    lo = np.array(p.lows, dtype=np.float64)
+    hi = np.array(p.highs, dtype=np.float64)
+    res = OrResult()
+    me = p.maxeval if maxeval is None else maxeval
+    cap = _default_cap(me, p.f.dim, 7)
+    rc = lib().or_integrate_bfs_allot(C.byref(s), _dp(lo), _dp(hi), p.epsrel if epsrel is None else epsrel,
+                                      p.epsabs, me, cap, nthreads, allot, C.byref(res))
+    d = res.asdict()
+    d["rc"] = rc
     return d
 
 
