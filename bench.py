@@ -292,6 +292,42 @@ def bench_reference(ctx):
     return rows
 
 
+def bench_table6(ctx):
+    """The table-driven 6-D integrand (256 x 256 FP64 bilinear field) at its
+    stated epsrel 1e-5 (maxeval 1e9): wall time, evals/s and the rule
+    evaluation's roofline from HIP events (the integrand's flops as counted
+    in region_flops plus the 4 table values read per point, 32 B, served by
+    the cache: the 512 KiB table stays resident); and at epsrel 1e-3 the
+    plain per-region test against the normalised allotment (guided loop,
+    finish_mode 4, no network), which certifies it."""
+    import cuhre
+    p = W.c3_table6()
+    ctx.integrate(p)  # warm-up
+    best = min((ctx.integrate(p) for _ in range(3)), key=lambda r: r["ms_total"])
+    ctx.set(True, 0)
+    prof = ctx.integrate(p)
+    ctx.set(False, 0)
+    tf = prof["eval_flops"] / (prof["ms_eval"] * 1e-3) / 1e12
+    pts = W.gm_npoints(6)
+    row = {"epsrel": 1e-5, "maxeval": 10**9, "rc": best["rc"], "integral": best["integral"],
+           "error": best["error"], "nregions": best["nregions"], "neval": best["neval"],
+           "iterations": best["iterations"], "ms": best["ms_total"], "evals_per_s": best["neval"] / best["ms_total"] * 1e3,
+           "eval_roofline": {"bound": "fp64", "achieved_tflops": tf, "frac": tf / FP64_PEAK_TFLOPS,
+                             "ms_eval": prof["ms_eval"], "eval_launches": prof["eval_launches"],
+                             "flops_per_region": prof["eval_flops"] / prof["nregions"],
+                             "table_bytes_per_region": 32 * pts,
+                             "table_GBps_from_cache": prof["nregions"] * 32 * pts / (prof["ms_eval"] * 1e-3) / 1e9}}
+    g = cuhre.Guide(ctx, seed=1)
+    plain3 = ctx.integrate(p, epsrel=1e-3)
+    norm = min((g.integrate(p, axis_mode=0, finish_mode=4, budget=0.25, epsrel=1e-3) for _ in range(2)),
+               key=lambda r: r["ms_total"])
+    g.close()
+    keys = ("rc", "integral", "error", "nregions", "iterations")
+    return {"table6d_1e-5": row,
+            "table6d_1e-3": {"plain": {k: plain3[k] for k in keys} | {"ms": plain3["ms_total"]},
+                             "normalised_allotment_b0.25": {k: norm[k] for k in keys} | {"ms": norm["ms_total"]}}}
+
+
 SWEEP_SIZES = (256, 4096, 16384, 65536, 262144, 1048576)
 SWEEP_SEGMENTS = (1, 2, 4, 8, 16, 32)
 
@@ -365,6 +401,8 @@ def main():
                          "older command lines)")
     ap.add_argument("--no-reference", action="store_true",
                     help="skip the default run of the reference's own method (degree-9 rule, two phases)")
+    ap.add_argument("--no-table", action="store_true",
+                    help="skip the default run of the table-driven 6-D integrand")
     ap.add_argument("--sweep", action="store_true",
                     help="also run the point-segment sweep of the rule evaluation (for a kernel profiler)")
     ap.add_argument("--lclt", action="store_true",
@@ -490,6 +528,8 @@ def main():
                for name in PEAK8_POLICIES}}
     if not args.no_reference:
         out["reference_method"] = bench_reference(ctx)
+    if not args.no_table:
+        out["table6d"] = bench_table6(ctx)
     if args.lclt:
         out["lclt_driver"] = bench_lclt(ctx)
     if args.sweep:
@@ -507,6 +547,14 @@ def main():
                 ex, _ = oracle.exact(p.f, p.lows, p.highs)
                 for k in ("reference_method", "two_phase_gm75", "plain_gm75", "plain_degree9"):
                     row[k]["true_rel_err"] = abs(row[k]["integral"] - ex) / abs(ex)
+        if "table6d" in out:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import oracle
+            p = W.c3_table6()
+            ex, _ = oracle.exact(p.f, p.lows, p.highs)
+            t = out["table6d"]
+            for row in (t["table6d_1e-5"], t["table6d_1e-3"]["plain"], t["table6d_1e-3"]["normalised_allotment_b0.25"]):
+                row["true_rel_err"] = abs(row["integral"] - ex) / abs(ex)
         if "batched" in out:
             sys.path.insert(0, os.path.join(ROOT, "oracle"))
             import oracle

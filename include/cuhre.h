@@ -276,11 +276,18 @@ typedef struct {
                                3: shared allotment: the per-region test
                                   becomes err <= max(epsrel ((1 - budget) |est|
                                   + budget |I| vrel), epsabs), I the iteration's
-                                  estimate, vrel the region's share of the box */
+                                  estimate, vrel the region's share of the box;
+                               4: normalised allotment: err <= max(epsrel |I|
+                                  ((1 - budget) |est| / E + budget vrel),
+                                  epsabs), E = sum |est| over the finished
+                                  regions and the iteration's: the tolerances
+                                  add up to epsrel |I| even when the
+                                  integrand's signs cancel; for f >= 0 and
+                                  budget 0 it is the plain test                */
   float axis_confidence;    /* softmax probability needed to override the axis */
   float finish_threshold;   /* sigmoid output needed to propose a finish       */
   double budget;            /* modes 1, 2: share of tol retired regions may use;
-                               mode 3: the allotment's volume share b         */
+                               modes 3, 4: the allotment's volume share b     */
 } cuhre_guide_policy;
 
 typedef struct {

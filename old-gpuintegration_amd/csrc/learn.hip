@@ -371,6 +371,8 @@ struct GuideArgs {
   int dim;
   double lows[kMaxDim], highs[kMaxDim];
   double absI;  // |current estimate of the integral|
+  double absE;  // sum |est| over the finished regions and the iteration's (normalized allotment)
+  double fin_aest;  // sum |est| of the regions finished before this iteration
   double epsrel, epsabs;
   double tol;   // max(epsrel * absI, epsabs)
   const IterState* state;  // when set: absI and tol come from this iteration's state,
@@ -378,7 +380,8 @@ struct GuideArgs {
   double loc_epsrel, loc_epsabs;  // the per-region test (0, 0 under the shared allotment)
   int axis_mode;    // 0: fourth difference, 1: guide when confident
   int finish_mode;  // 0: none, 1: guide proposal + volume-share guard, 2: guard alone,
-                    // 3: shared value / volume allotment replaces the per-region test
+                    // 3: shared value / volume allotment replaces the per-region test,
+                    // 4: the same normalised so the tolerances add up to epsrel |I|
   int use_mlp;
   float axis_conf, finish_thr;
   double budget;  // fraction of epsrel*|I| the retired regions may spend
@@ -486,10 +489,11 @@ __global__ __launch_bounds__(64 * guide_tiles<MLP>()) void k_guide_apply(GuideAr
   constexpr int kL = MLP ? 1 : 0;  // network buffers only when the network runs
   __shared__ __align__(16) uint16_t BufA[kMlpWaveBuf * kL + 1], BufB[kMlpWaveBuf * kL + 1];
   if (G.state != nullptr && G.state->stop != 0) return;  // block-uniform
-  double absI = G.absI, tol = G.tol;
+  double absI = G.absI, tol = G.tol, absE = G.absE;
   if (G.state != nullptr) {
     absI = fabs(G.state->rG_s + G.state->rG_c);
     tol = fmax(G.epsrel * absI, G.epsabs);
+    absE = G.fin_aest + G.state->tot.aest_all;
   }
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t ntiles = (G.n + kMlpRows - 1) / kMlpRows;
@@ -553,6 +557,14 @@ __global__ __launch_bounds__(64 * guide_tiles<MLP>()) void k_guide_apply(GuideAr
     if (G.finish_mode == 3) {
       // shared allotment: a (1 - budget) share of tol by value, a budget share by volume
       const double a = G.epsrel * ((1.0 - G.budget) * fabs(e) + G.budget * absI * vrel);
+      if (er <= fmax(a, G.epsabs)) retire = 1;
+    }
+    if (G.finish_mode == 4) {
+      // normalised allotment: value shares |est| / E and volume shares of
+      // epsrel |I|, so the tolerances add up to the global one even when the
+      // integrand's signs cancel (|I| << E); for f >= 0 and budget 0 it is
+      // the plain test
+      const double a = G.epsrel * absI * ((1.0 - G.budget) * fabs(e) / absE + G.budget * vrel);
       if (er <= fmax(a, G.epsabs)) retire = 1;
     }
     ab = G.abssum[r];
@@ -708,7 +720,7 @@ __global__ __launch_bounds__(256) void k_gscan_top(const ChunkPart* part, int64_
     const IterState& st = *RA.state;
     const double tol = fmax(RA.epsrel * fabs(st.rG_s + st.rG_c), RA.epsabs);
     const double frozen = RA.frozen_prev + (st.tot.ferr_s + st.tot.ferr_c);
-    room = RA.mode == 3   ? 1e300
+    room = RA.mode >= 3   ? 1e300
            : RA.mode != 0 ? fmin(RA.budget * tol - RA.ret_prev, (1.0 - RA.budget) * tol - frozen)
                           : -1.0;
   }
@@ -1196,7 +1208,7 @@ int guided_run(cuhre_context* c, cuhre_guide* g, const cuhre_integrand* f, int d
   // the network and the sample log read I5 and the fourth differences
   const bool need_aux = run.use_mlp || run.collect_max > 0;
   // the error-budget finish replaces the per-region test
-  const double loc_rel = run.finish_mode == 3 ? 0.0 : epsrel, loc_abs = run.finish_mode == 3 ? 0.0 : epsabs;
+  const double loc_rel = run.finish_mode >= 3 ? 0.0 : epsrel, loc_abs = run.finish_mode >= 3 ? 0.0 : epsabs;
   if ((rc = ctx_reserve(c, std::min<int64_t>(capacity, std::max<int64_t>(4 * n0, 1 << 16)), dim))) return rc;
   if (need_aux && (rc = ctx_reserve_aux(c, c->cap, dim))) return rc;
   if ((rc = guide_reserve_regions(g, c->cap))) return rc;
@@ -1256,6 +1268,8 @@ int guided_run(cuhre_context* c, cuhre_guide* g, const cuhre_integrand* f, int d
         G.highs[m] = highs[m];
       }
       G.absI = std::fabs(out->integral);
+      G.fin_aest = fin_aest;
+      G.absE = fin_aest + c->h_state->tot.aest_all;  // read only when !dev (synchronised)
       G.epsrel = epsrel;
       G.epsabs = epsabs;
       G.tol = std::max(epsrel * G.absI, epsabs);
@@ -1299,7 +1313,7 @@ int guided_run(cuhre_context* c, cuhre_guide* g, const cuhre_integrand* f, int d
       if (!dev) {
         const IterState& hs = *c->h_state;
         const double frozen = RA.frozen_prev + (hs.tot.ferr_s + hs.tot.ferr_c);
-        RA.room = run.finish_mode == 3 ? 1e300
+        RA.room = run.finish_mode >= 3 ? 1e300
                   : run.finish_mode    ? std::min(run.budget * G.tol - ret_err, (1.0 - run.budget) * G.tol - frozen)
                                        : -1.0;
       }
@@ -1722,7 +1736,7 @@ int cuhre_integrate_guided(cuhre_context* ctx, cuhre_guide* g, const cuhre_integ
   if (g->ctx != ctx) return set_error(CUHRE_ERR_ARG, "the guide belongs to another context");
   cuhre_guide_policy d = {1, 1, 0.6f, 0.5f, 0.25};
   const cuhre_guide_policy& P = pol ? *pol : d;
-  if (P.axis_mode < 0 || P.axis_mode > 1 || P.finish_mode < 0 || P.finish_mode > 3 || !(P.budget >= 0.0) ||
+  if (P.axis_mode < 0 || P.axis_mode > 1 || P.finish_mode < 0 || P.finish_mode > 4 || !(P.budget >= 0.0) ||
       P.budget > 1.0)
     return set_error(CUHRE_ERR_ARG, "bad guide policy");
   GuidedRun run{P.axis_mode, P.finish_mode, (P.axis_mode == 1 || P.finish_mode == 1) ? 1 : 0, P.axis_confidence,

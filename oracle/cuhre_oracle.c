@@ -770,14 +770,14 @@ int or_integrate_bfs_key(const or_integrand* f, const double* lows,
 
 static int bfs_core(const or_integrand* f, const double* lows, const double* highs, double epsrel,
                     double epsabs, int64_t maxeval, int64_t capacity, int nthreads, int key, int want_rule,
-                    double allot, or_result* out, or_check* chk);
+                    double allot, int normalized, or_result* out, or_check* chk);
 
 int or_integrate_bfs_check(const or_integrand* f, const double* lows,
                            const double* highs, double epsrel, double epsabs,
                            int64_t maxeval, int64_t capacity, int nthreads,
                            int key, int want_rule, or_result* out, or_check* chk)
 {
-  return bfs_core(f, lows, highs, epsrel, epsabs, maxeval, capacity, nthreads, key, want_rule, 0.0, out, chk);
+  return bfs_core(f, lows, highs, epsrel, epsabs, maxeval, capacity, nthreads, key, want_rule, 0.0, 0, out, chk);
 }
 
 int or_integrate_bfs_allot(const or_integrand* f, const double* lows, const double* highs, double epsrel,
@@ -785,17 +785,28 @@ int or_integrate_bfs_allot(const or_integrand* f, const double* lows, const doub
                            or_result* out)
 {
   if (!(allot >= 0.0) || allot > 1.0) return -1;
-  return bfs_core(f, lows, highs, epsrel, epsabs, maxeval, capacity, nthreads, 7, 0, allot, out, NULL);
+  return bfs_core(f, lows, highs, epsrel, epsabs, maxeval, capacity, nthreads, 7, 0, allot, 0, out, NULL);
+}
+
+int or_integrate_bfs_normalized(const or_integrand* f, const double* lows, const double* highs, double epsrel,
+                                double epsabs, int64_t maxeval, int64_t capacity, int nthreads, double allot,
+                                or_result* out)
+{
+  if (!(allot >= 0.0) || allot > 1.0) return -1;
+  return bfs_core(f, lows, highs, epsrel, epsabs, maxeval, capacity, nthreads, 7, 0, allot, 1, out, NULL);
 }
 
 /* The loop.  allot = b > 0 replaces the per-region test by the shared
  * allotment: active iff err > max(epsrel ((1 - b) |est| + b |I| vrel),
  * epsabs), with I the iteration's estimate (finished so far plus every
  * region of the iteration) and vrel the region's share of the box volume;
- * b = 0 is the reference's test (GPUQuadPhases.cu:63-73). */
+ * b = 0 is the reference's test (GPUQuadPhases.cu:63-73).  normalized: the
+ * tolerance is epsrel |I| ((1 - b) |est| / E + b vrel) instead, E = sum
+ * |est| over the finished regions and the iteration's, so the tolerances of
+ * all regions add up to the global epsrel |I| (any b, cancellation or not). */
 static int bfs_core(const or_integrand* f, const double* lows, const double* highs, double epsrel,
                     double epsabs, int64_t maxeval, int64_t capacity, int nthreads, int key, int want_rule,
-                    double allot, or_result* out, or_check* chk)
+                    double allot, int normalized, or_result* out, or_check* chk)
 {
   memset(out, 0, sizeof(*out));
   if (chk) memset(chk, 0, sizeof(*chk));
@@ -860,17 +871,21 @@ static int bfs_core(const or_integrand* f, const double* lows, const double* hig
       double tol = fmax(epsrel * fabs(est[r]), epsabs);
       act[r] = err[r] > tol ? 1 : 0;
     }
-    if (allot > 0.0) {
+    if (allot > 0.0 || normalized) {
       /* the shared allotment needs the iteration's estimate first */
-      ksum all = ifin;
-      for (int64_t r = 0; r < n; ++r) kadd(&all, est[r]);
-      const double absI = fabs(kval(&all));
+      ksum all = ifin, aall = absfin;
+      for (int64_t r = 0; r < n; ++r) {
+        kadd(&all, est[r]);
+        kadd(&aall, fabs(est[r]));
+      }
+      const double absI = fabs(kval(&all)), E = kval(&aall);
       double boxv = 1.0;
       for (int m = 0; m < d; ++m) boxv *= highs[m] - lows[m];
       for (int64_t r = 0; r < n; ++r) {
         double vrel = 1.0;
         for (int m = 0; m < d; ++m) vrel *= wd[(size_t)m * n + r] / (highs[m] - lows[m]);
-        const double a = epsrel * ((1.0 - allot) * fabs(est[r]) + allot * absI * vrel);
+        const double a = normalized ? epsrel * absI * ((1.0 - allot) * fabs(est[r]) / E + allot * vrel)
+                                    : epsrel * ((1.0 - allot) * fabs(est[r]) + allot * absI * vrel);
         act[r] = isfinite(est[r]) && isfinite(err[r]) && err[r] > fmax(a, epsabs) ? 1 : 0;
       }
       (void)boxv;

@@ -173,6 +173,14 @@ int or_integrate_bfs_allot(const or_integrand* f, const double* lows, const doub
                            double epsabs, int64_t maxeval, int64_t capacity, int nthreads, double allot,
                            or_result* out);
 
+/* The same loop with the normalized allotment: active iff err >
+ * max(epsrel |I| ((1 - allot) |est| / E + allot vrel), epsabs), E = sum |est|
+ * over the finished regions and the iteration's; the tolerances add up to
+ * epsrel |I| whatever the signs (for f >= 0 and allot = 0 the plain test). */
+int or_integrate_bfs_normalized(const or_integrand* f, const double* lows, const double* highs, double epsrel,
+                                double epsabs, int64_t maxeval, int64_t capacity, int nthreads, double allot,
+                                or_result* out);
+
 /* One region with its __float128 shadow: out[0..5] = est, abssum,
  * istar (hi, lo), irule (hi, lo; NaN where no quad restatement). */
 void or_region_check(int key, const or_integrand* f, const double* lower,

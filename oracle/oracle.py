@@ -128,6 +128,8 @@ def lib():
                                              C.POINTER(OrResult), C.POINTER(OrCheck)]
         L.or_integrate_bfs_allot.argtypes = [C.POINTER(OrIntegrand), dp, dp, C.c_double, C.c_double,
                                              C.c_int64, C.c_int64, C.c_int, C.c_double, C.POINTER(OrResult)]
+        L.or_integrate_bfs_normalized.argtypes = [C.POINTER(OrIntegrand), dp, dp, C.c_double, C.c_double,
+                                                  C.c_int64, C.c_int64, C.c_int, C.c_double, C.POINTER(OrResult)]
         L.or_fp64_bound.argtypes = [C.c_int, C.c_int, C.c_int64, C.c_double, C.c_double, C.c_double]
         L.or_fp64_bound.restype = C.c_double
         L.or_region_check.argtypes = [C.c_int, C.POINTER(OrIntegrand), dp, dp, dp]
@@ -302,6 +304,22 @@ def integrate_bfs_allot(p: W.Problem, allot, nthreads=8, maxeval=None, epsrel=No
     cap = _default_cap(me, p.f.dim, 7)
     rc = lib().or_integrate_bfs_allot(C.byref(s), _dp(lo), _dp(hi), p.epsrel if epsrel is None else epsrel,
                                       p.epsabs, me, cap, nthreads, allot, C.byref(res))
+    d = res.asdict()
+    d["rc"] = rc
+    return d
+
+
+def integrate_bfs_normalized(p: W.Problem, allot, nthreads=8, maxeval=None, epsrel=None):
+    """The breadth-first GM7/5 loop with the normalized allotment (per-region
+    tolerances adding up to epsrel |I|)."""
+    s, keep = to_struct(p.f)
+    lo = np.array(p.lows, dtype=np.float64)
+    hi = np.array(p.highs, dtype=np.float64)
+    res = OrResult()
+    me = p.maxeval if maxeval is None else maxeval
+    cap = _default_cap(me, p.f.dim, 7)
+    rc = lib().or_integrate_bfs_normalized(C.byref(s), _dp(lo), _dp(hi), p.epsrel if epsrel is None else epsrel,
+                                           p.epsabs, me, cap, nthreads, allot, C.byref(res))
     d = res.asdict()
     d["rc"] = rc
     return d
