@@ -183,6 +183,14 @@ int cuhre_eval_regions(cuhre_context* ctx, const cuhre_integrand* f, int dim,
                        const double* width, double* est, double* err,
                        int32_t* axis, double* abssum, double* i5, double* dvec);
 
+/* Phase 2 of the two-phase method (quad/GPUquad/GPUQuadPhases.cu:422-536) on
+ * n regions given on the host (SoA as cuhre_eval_regions): per region the
+ * refined result, error and pool size (1 + bisections).  key 0/7 or 9;
+ * pool_max 0: 2048. */
+int cuhre_phase2_regions(cuhre_context* ctx, const cuhre_integrand* f, int dim, int key, int64_t n,
+                         const double* lower, const double* width, double epsrel, double epsabs,
+                         int32_t pool_max, double* res, double* err, int32_t* count);
+
 /* One classify + compact + bisect step on a host list: regions with
  * err > max(epsrel |est|, epsabs) are split on axis[r]; child 0 of the j-th
  * such region lands at 2j, child 1 at 2j+1 (SoA with stride 2A).  out_*

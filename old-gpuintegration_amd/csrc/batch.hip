@@ -141,6 +141,7 @@ __device__ __forceinline__ bool batch_active(const BatchBuffers& B, const InstSt
 // Initial k^d grid of every instance plus the instance states.
 __global__ void k_batch_init(BatchBuffers B, int k, int64_t n0)
 {
+#pragma clang fp contract(off)  // low + id * w rounded twice, as the oracle computes it
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t total = n0 * B.count;
   const int64_t cpi = (n0 + kChunk - 1) / kChunk;  // chunks per instance

@@ -86,7 +86,24 @@ struct cuhre_context {
 
 namespace cuhre {
 
+// Error reporting: sets the thread-local message cuhre_last_error returns
+// and returns code.
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
 int fail_msg(int code, const char* msg);
+int check_integrand_args(const cuhre_integrand* f, int dim, int key);
+int check_bounds(int dim, double epsrel, double epsabs, const double* lows, const double* highs);
+double region_flops(const cuhre_integrand* f, int d, int key);
+double family_flops(const cuhre_integrand* f, int d);
+double ipow_flops(int e, int d);
+int grid_k(int d);  // initial grid k^d: k = 4 (d <= 4), 2 (5..10), 1 above (GPUKernelquad.cu:300-316)
+
+// The static workspace of cuhre_integrate_noalloc: 64 MiB per device, the
+// last 1 MiB for integrand tables.
+constexpr size_t kScratchBytes = 64ull << 20;
+constexpr size_t kScratchTableBytes = 1ull << 20;
+size_t al256(size_t b);
+size_t scratch_list_bytes(int64_t cap, int dim);  // list buffers of cap regions
+int64_t scratch_capacity(int dim);                // regions per list the workspace holds
 int ctx_reserve(cuhre_context* c, int64_t cap, int dim, int64_t keep_n = 0, int cur = 0, int64_t keep_nb = 0);
 int ctx_reserve_aux(cuhre_context* c, int64_t cap, int dim);
 // Device copy of an integrand.  Tabulated families copy their table to the

@@ -87,21 +87,6 @@ EvalLauncher find_eval_launcher(int family, int dim, int key)
   }
 }
 
-// ------------------------------------------------------------------ errors
-
-static thread_local std::string g_err;
-
-static int fail(int code, const char* fmt, ...)
-{
-  char buf[512];
-  va_list ap;
-  va_start(ap, fmt);
-  vsnprintf(buf, sizeof buf, fmt, ap);
-  va_end(ap);
-  g_err = buf;
-  return code;
-}
-
 #define HIP_TRY(expr)                                                                   \
   do {                                                                                  \
     hipError_t e_ = (expr);                                                             \
@@ -110,157 +95,9 @@ static int fail(int code, const char* fmt, ...)
                   __FILE__, __LINE__);                                                  \
   } while (0)
 
-int set_error(int code, const char* msg) { return fail(code, "%s", msg); }
-
 struct BatchEngine;
 BatchEngine* batch_engine_new();
 void batch_engine_delete(BatchEngine* e);
-
-// ------------------------------------------------------------------ rule
-
-int gm_points(int d) { return (1 << d) + 2 * d * d + 2 * d + 1; }
-
-// Genz-Malik (1980) degree-7 rule and embedded degree-5 rule; weights on
-// [-1,1]^d divided by the volume 2^d.
-int r9_points(int d)
-{
-  return 1 + 8 * d + 2 * d * (d - 1) + 4 * d * (d - 1) + 4 * d * (d - 1) * (d - 2) / 3 + (1 << d);
-}
-
-// The reference's degree-9 rule (KEY 9): radii and weights as
-// quad/GPUquad/GPUQuadRule.cu:33-176 defines them (they define the rule),
-// null-rule scale/norm as :209-225.
-static Rule9Const make_rule9(int d)
-{
-  Rule9Const R;
-  const double n = (double)d;
-  const double a[5] = {.47795365790226950619, .20302858736911986780, .44762735462617812882, .125,
-                       .34303789878087814570};
-  for (int k = 0; k < 5; ++k) R.t[k] = 2.0 * a[k];  // fractions of the width -> half-width units
-  const double A = n * (n * (n * (-.0023611709677855117884) + .11415390023857325268) + (-.63833920076702389094)) +
-                   .74849988504685208004;
-  const double B = n * (n * (n * (-.0014324017033399125142) + .057471507864489725949) + (-.14225104571434243234)) -
-                   (-.062875028738286979989);
-  const double C = n * (n * .0035417564516782676826 + (-.072609367395893679605)) + .10557491625218991012;
-  const double D = n * (n * .0021486025550098687713 + (-.032268563892953949998)) + .010636783990231217481;
-  const double E = n * (-.04508628929435784076) + .21415883524352793401;
-  const double F = n * (-.027351546526545644722) + .054941067048711234101;
-  const double G = .015532417276607053264 - n * .0035417564516782676826;
-  const double H = .0035328099607090870236 - n * .0021486025550098687713;
-  const double c = (double)(1 << d);
-  const double w[9][5] = {
-      {A, B, n * .254591133248959089 - B, n * (n * (-1.207328566678236261) + .89567365764160676508) - 1 + A,
-       n * (-.36479356986049146661) + 1 - A},
-      {C, D, .014689102496143490175 - D, n * .51134708346467591431 + .45976448120806344646 + C,
-       .18239678493024573331 - C},
-      {E, F, .11937596202570775297 - F, n * .65089519391920250593 + .14744939829434460168, -E},
-      {.057693384490973483573, .034999626602143583822, -.057693384490973483573, -1.3868627719278281436,
-       -.057693384490973483573},
-      {0, 0, -.2386668732575008879, 0, 0},
-      {G, H, -H, .09231719987444221619 + G, -G},
-      {.02254314464717892038, .013675773263272822361, -.013675773263272822361, -.32544759695960125297,
-       -.02254314464717892038},
-      {.0017708782258391338413, .0010743012775049343856, -.0010743012775049343856, .0017708782258391338413,
-       -.0017708782258391338413},
-      {.25150011495314791996 / c, -.062875028738286979989 / c, .062875028738286979989 / c,
-       .25150011495314791996 / c, -.25150011495314791996 / c}};
-  const int cnt[9] = {1, 2 * d, 2 * d, 2 * d, 2 * d, 2 * d * (d - 1), 4 * d * (d - 1),
-                      4 * d * (d - 1) * (d - 2) / 3, 1 << d};
-  for (int s = 0; s < 9; ++s)
-    for (int q = 0; q < 5; ++q) {
-      R.w[s][q] = w[s][q];
-      R.scale[s][q] = 0.0;
-      R.norm[s][q] = 0.0;
-    }
-  for (int s = 0; s < 9; ++s)
-    for (int q = 1; q < 4; ++q) {
-      const double sc = w[s][q] == 0.0 ? 100.0 : -w[s][q + 1] / w[s][q];
-      double sum = 0.0;
-      for (int x = 0; x < 9; ++x) sum += (double)cnt[x] * std::fabs(w[x][q + 1] + sc * w[x][q]);
-      R.scale[s][q] = sc;
-      R.norm[s][q] = 1.0 / sum;
-    }
-  R.ratio = (R.t[1] / R.t[0]) * (R.t[1] / R.t[0]);
-  return R;
-}
-
-RuleConst make_rule(int d, int key)
-{
-  RuleConst R;
-  std::memset(&R, 0, sizeof R);
-  const double dd = (double)d;
-  R.l2 = std::sqrt(9.0 / 70.0);
-  R.l3 = std::sqrt(9.0 / 10.0);
-  R.l4 = std::sqrt(9.0 / 10.0);
-  R.l5 = std::sqrt(9.0 / 19.0);
-  R.w7c = (12824.0 - 9120.0 * dd + 400.0 * dd * dd) / 19683.0;
-  R.w7a2 = 980.0 / 6561.0;
-  R.w7a3 = (1820.0 - 400.0 * dd) / 19683.0;
-  R.w7p = 200.0 / 19683.0;
-  R.w7v = 6859.0 / 19683.0 / (double)(1 << d);
-  R.w5c = (729.0 - 950.0 * dd + 50.0 * dd * dd) / 729.0;
-  R.w5a2 = 245.0 / 486.0;
-  R.w5a3 = (265.0 - 100.0 * dd) / 1458.0;
-  R.w5p = 25.0 / 729.0;
-  R.ratio = 1.0 / 7.0;
-  R.key = key == 9 ? 9 : 7;
-  R.npts = key == 9 ? r9_points(d) : gm_points(d);
-  if (key == 9) R.r9 = make_rule9(d);
-  return R;
-}
-
-// Algorithmic FP64 flops of one region's evaluation for a family, by the
-// counting convention add/sub/mul = 1, fma = 2, and div / exp / cos / sqrt
-// charged the FP64 VALU flops of their gfx950 instruction sequences
-// (measured with the SQ_INSTS_VALU_*_F64 counters on a one-op kernel).
-// FP64 flops per call (add/sub/mul 1, fma 2, transcendental estimate 1) of
-// the device math each family uses, counted with the SQ_INSTS_VALU_*_F64
-// counters on one-operation kernels (flopcost.cpp); recip is recip_dev.
-struct FlopCosts { double div = 14, exp = 27, cos = 57, sqrt = 17, recip = 9, erfc = 115; };
-static FlopCosts g_costs;
-
-// multiplies of b^e: ipow_const for Genz's exponent d + 1, else the loop
-double ipow_flops(int e, int d)
-{
-  if (e <= 0) return 0.0;
-  int sq = 0, bits = 0;
-  for (int t = e; t > 1; t >>= 1) ++sq;
-  for (int t = e; t; t >>= 1) bits += t & 1;
-  return e == d + 1 ? (double)(sq + bits - 1) : (double)(sq + bits);
-}
-
-double family_flops(const cuhre_integrand* f, int d)
-{
-  switch (f->family) {
-  case CUHRE_GAUSSIAN: return 4.0 * d + g_costs.exp;
-  case CUHRE_PRODUCT_PEAK: return 4.0 * d - 1 + g_costs.recip;
-  case CUHRE_OSCILLATORY: return 2.0 * d + 1 + g_costs.cos;
-  case CUHRE_CORNER_PEAK: return 2.0 * d + ipow_flops(f->exponent, d) + g_costs.recip;
-  case CUHRE_CONTINUOUS: return 4.0 * d + g_costs.exp;
-  case CUHRE_COSINE_SUM: return d * (2.0 + g_costs.cos + g_costs.div) + 1;
-  case CUHRE_TABLE6: return 20.0 + g_costs.exp + g_costs.sqrt + 2 * g_costs.div;
-  // five bilinear fields (8 flops each, 2 interpolation weights), the closed
-  // form's ~40 flops, 3 exp, 4 erfc and 11 divisions
-  case CUHRE_LC_LT_T: return 84.0 + 3 * g_costs.exp + 4 * g_costs.erfc + 11 * g_costs.div;
-  default: return 4.0 * d;
-  }
-}
-
-// Per region: every point costs its integrand plus d coordinate FMAs.
-// GM7/5: the sums take three FMAs (I7, I5, sum |w f|) at centre/axis/pair
-// points and two at the corners (no degree-5 weight).  Degree 9: two adds
-// per point (f and |f| into the set group), six FMAs per set group, the
-// null-rule combination.  Plus the per-region setup.
-double region_flops(const cuhre_integrand* f, int d, int key)
-{
-  if (key == 9) {
-    const double p = r9_points(d);
-    const double groups = 1 + 4.0 * d + d * (d - 1) + d * (d - 1) * (d - 2) / 6.0 + 1;
-    return p * (family_flops(f, d) + 2.0 * d + 2.0) + 12.0 * groups + 3 * 9 * 4.0 + (8.0 * d + 12.0);
-  }
-  const double p = gm_points(d), corners = (double)(1 << d);
-  return p * (family_flops(f, d) + 2.0 * d) + 6.0 * (p - corners) + 4.0 * corners + (8.0 * d + 12.0);
-}
 
 // ------------------------------------------------------------------ split
 
@@ -459,9 +296,13 @@ static int ctx_free_lists(cuhre_context* c)
 int ctx_reserve(cuhre_context* c, int64_t cap, int dim, int64_t keep_n, int cur, int64_t keep_nb)
 {
   if (cap <= c->cap && dim <= c->dim_alloc) return 0;
-  cap = std::max(cap, c->cap);
+  // a longer list keeps the capacity it grew to; a list of more dimensions
+  // is sized by the request (at the old capacity it may not fit beside the
+  // old buffers), and without contents to carry the old buffers go first
+  if (keep_n > 0 || dim <= c->dim_alloc) cap = std::max(cap, c->cap);
   dim = std::max(dim, c->dim_alloc);
   cap = (cap + 255) & ~255ll;
+  if (keep_n == 0) ctx_free_lists(c);
   const int64_t nbmax = block_capacity(cap);
   double *lo[2], *wd[2], *est, *err, *ab;
   int8_t* ax;
@@ -664,18 +505,6 @@ int pick_segments(const cuhre_context* c, int64_t n, int key)
   return S;
 }
 
-static int grid_k(int d) { return d <= 4 ? 4 : (d <= 10 ? 2 : 1); }
-
-// Initial uniform k^d grid over the box (GPUKernelquad.cu:23-61, 300-316),
-// region r has coordinate index (r / k^m) % k along axis m.
-int64_t initial_grid_count(int d)
-{
-  const int k = grid_k(d);
-  int64_t n = 1;
-  for (int m = 0; m < d; ++m) n *= k;
-  return n;
-}
-
 // The initial grid written on the device (lower = low + id * w with the
 // product rounded first, width w = (high - low) / k, as the oracle's
 // or_initial_grid), instead of two host-to-device copies per call.
@@ -689,6 +518,7 @@ struct GridArgs {
 
 __global__ __launch_bounds__(kBlock) void k_initial_grid(GridArgs G)
 {
+#pragma clang fp contract(off)  // low + id * w rounded twice, as the oracle computes it
   const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (r >= G.n) return;
   int64_t q = r;
@@ -696,7 +526,7 @@ __global__ __launch_bounds__(kBlock) void k_initial_grid(GridArgs G)
     const int64_t id = q % G.k;
     q /= G.k;
     const double w = (G.highs[m] - G.lows[m]) / (double)G.k;
-    G.lo[m * G.ld + r] = __dadd_rn(G.lows[m], __dmul_rn((double)id, w));
+    G.lo[m * G.ld + r] = G.lows[m] + (double)id * w;
     G.wd[m * G.ld + r] = w;
   }
 }
@@ -721,11 +551,8 @@ int upload_initial_grid(cuhre_context* c, int list, int d, const double* lows, c
 int check_common(cuhre_context* ctx, const cuhre_integrand* f, int dim, int key)
 {
   if (!ctx) return fail(CUHRE_ERR_ARG, "null context");
-  if (!f) return fail(CUHRE_ERR_ARG, "null integrand");
-  if (dim < 2 || dim > 10) return fail(CUHRE_ERR_UNSUPPORTED, "dimension %d outside 2..10", dim);
-  if (f->dim != dim) return fail(CUHRE_ERR_ARG, "integrand dim %d != dim %d", f->dim, dim);
-  if (key != 0 && key != 7 && key != 9)
-    return fail(CUHRE_ERR_UNSUPPORTED, "rule key %d not available (0 or 7: Genz-Malik 7/5; 9: degree 9)", key);
+  int rc = check_integrand_args(f, dim, key);
+  if (rc) return rc;
   if (!find_eval_launcher(f->family, dim, key))
     return fail(CUHRE_ERR_UNSUPPORTED, "family %d not compiled for dim %d", f->family, dim);
   HIP_TRY(hipSetDevice(ctx->device));
@@ -793,14 +620,6 @@ void dd_add_host(double& s1, double& c1, double s2, double c2)
 }  // namespace cuhre
 
 extern "C" {
-
-const char* cuhre_last_error(void) { return g_err.c_str(); }
-
-int64_t cuhre_points_per_region(int dim, int key)
-{
-  if (dim < 2 || dim > kMaxDim || (key != 0 && key != 7 && key != 9)) return -1;
-  return key == 9 ? r9_points(dim) : gm_points(dim);
-}
 
 }  // extern "C"
 static int scratch_setup(int device);
@@ -946,11 +765,7 @@ int plain_loop(cuhre_context* c, const cuhre_integrand* f, int dim, double epsre
   std::memset(out, 0, sizeof *out);
   int rc = check_common(c, f, dim, key);
   if (rc) return rc;
-  if (!lows || !highs) return fail(CUHRE_ERR_ARG, "null bounds");
-  if (!(epsrel >= 0.0) || !(epsabs >= 0.0)) return fail(CUHRE_ERR_ARG, "tolerances must be >= 0");
-  for (int m = 0; m < dim; ++m)
-    if (!(highs[m] > lows[m]) || !std::isfinite(lows[m]) || !std::isfinite(highs[m]))
-      return fail(CUHRE_ERR_ARG, "empty or non-finite box along axis %d", m);
+  if ((rc = check_bounds(dim, epsrel, epsabs, lows, highs))) return rc;
 
   DevIntegrand P;
   if ((rc = make_dev_integrand(c, f, dim, &P, opt.table_dst, opt.table_bytes))) return rc;
@@ -1118,6 +933,99 @@ int plain_loop(cuhre_context* c, const cuhre_integrand* f, int dim, double epsre
 
 }  // namespace
 
+// Phase 2 of the reference's two-phase method on the n regions of list
+// `cur` (SoA, ld = the context's capacity): one wavefront per region
+// (k_phase2), per-region outputs copied to the host.  The workspace is the
+// context's and kept between calls.
+struct Phase2Host {
+  std::vector<double> res, err, rab, rres;
+  std::vector<int32_t> cnt;
+  std::vector<int8_t> act, bad, zero;
+};
+
+static int run_phase2(cuhre_context* c, const cuhre_integrand* f, int dim, int key, int cur, int64_t n,
+                      double epsrel, double epsabs, int32_t pool_max, Phase2Host& H)
+{
+  int rc;
+  const Phase2Launcher launch = find_phase2_launcher(f->family, dim, key);
+  if (!launch) return fail(CUHRE_ERR_UNSUPPORTED, "no Phase-2 kernel for family %d, dim %d", f->family, dim);
+  DevIntegrand P;
+  if ((rc = make_dev_integrand(c, f, dim, &P))) return rc;
+  const RuleConst R = make_rule(dim, key);
+  const int64_t slots = (int64_t)pool_max + 2;
+  const size_t per_slot = sizeof(double) * (2 * (size_t)dim + 2) + sizeof(int8_t) + sizeof(int32_t);
+  const size_t ws = (size_t)n * (size_t)slots * per_slot;
+  const size_t outb = (size_t)n * (4 * sizeof(double) + sizeof(int32_t) + 3) + 8 * 256;
+  const size_t need = ws + outb + 8 * 256;
+  if (c->p2_bytes < need) {
+    size_t free_b = 0, total_b = 0;
+    if (c->p2_ws) {
+      (void)hipFree(c->p2_ws);
+      c->p2_ws = nullptr;
+      c->p2_bytes = 0;
+    }
+    HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+    if (need > free_b / 4 * 3)
+      return fail(CUHRE_ERR_CAPACITY, "Phase 2 needs %.1f GB for %lld regions x %d slots", need / 1e9, (long long)n,
+                  pool_max);
+    if (hipMalloc(&c->p2_ws, need) != hipSuccess) {
+      (void)hipGetLastError();
+      c->p2_ws = nullptr;
+      return fail(CUHRE_ERR_ALLOC, "cannot allocate the Phase-2 workspace (%.1f GB)", need / 1e9);
+    }
+    c->p2_bytes = need;
+  }
+  Phase2Args A;
+  A.lower = c->lo[cur];
+  A.width = c->wd[cur];
+  A.ld = c->cap;
+  A.n = n;
+  A.slots = slots;
+  char* q = c->p2_ws;
+  auto take = [&](size_t bytes) {
+    char* r = q;
+    q += (bytes + 255) & ~(size_t)255;
+    return r;
+  };
+  A.geo = (double*)take(sizeof(double) * (size_t)n * slots * 2 * dim);
+  A.sest = (double*)take(sizeof(double) * (size_t)n * slots);
+  A.sax = (int8_t*)take((size_t)n * slots);
+  A.gkerr = (double*)take(sizeof(double) * (size_t)n * slots);
+  A.gkseq = (int32_t*)take(sizeof(int32_t) * (size_t)n * slots);
+  A.res = (double*)take(sizeof(double) * n);
+  A.err = (double*)take(sizeof(double) * n);
+  A.cnt = (int32_t*)take(sizeof(int32_t) * n);
+  A.act = (int8_t*)take(n);
+  A.nonfinite = (int8_t*)take(n);
+  A.zeroed = (int8_t*)take(n);
+  A.rab = (double*)take(sizeof(double) * n);
+  A.rres = (double*)take(sizeof(double) * n);
+  A.epsrel = epsrel;
+  A.epsabs = epsabs;
+  A.pool_max = pool_max;
+  H.res.resize(n);
+  H.err.resize(n);
+  H.rab.resize(n);
+  H.rres.resize(n);
+  H.cnt.resize(n);
+  H.act.resize(n);
+  H.bad.resize(n);
+  H.zero.resize(n);
+  launch(A, P, R, c->stream);
+  if (hipGetLastError() != hipSuccess ||
+      hipMemcpyAsync(H.res.data(), A.res, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      hipMemcpyAsync(H.err.data(), A.err, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      hipMemcpyAsync(H.rab.data(), A.rab, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      hipMemcpyAsync(H.rres.data(), A.rres, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      hipMemcpyAsync(H.cnt.data(), A.cnt, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      hipMemcpyAsync(H.act.data(), A.act, n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      hipMemcpyAsync(H.bad.data(), A.nonfinite, n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      hipMemcpyAsync(H.zero.data(), A.zeroed, n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    return fail(CUHRE_ERR_DEVICE, "Phase 2 failed: %s", hipGetErrorString(hipGetLastError()));
+  return 0;
+}
+
 // ------------------------------------------------------- static workspace
 // Device memory of cuhre_integrate_noalloc: one array per GPU (each device
 // holds its own copy at its own address until the process ends), 64 MiB, the
@@ -1125,8 +1033,6 @@ int plain_loop(cuhre_context* c, const cuhre_integrand* f, int dim, double epsre
 // host-side stand-in, not device memory: kernels and copies only ever get the
 // address hipGetSymbolAddress returns for the current device, kept in
 // g_scratch_ptr.
-constexpr size_t kScratchBytes = 64ull << 20;
-constexpr size_t kScratchTableBytes = 1ull << 20;
 static __device__ __align__(256) char scratch[kScratchBytes];
 
 constexpr int kScratchDevices = 64;
@@ -1152,30 +1058,6 @@ static int scratch_setup(int device)
   return 0;
 }
 
-static size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
-
-// Bytes of the list buffers (two ping-pong lists, est/err/abssum/axis, block
-// partials and offsets with their group tails) for cap regions of dim axes.
-static size_t scratch_list_bytes(int64_t cap, int dim)
-{
-  const int64_t nb = block_capacity(cap), ng = group_capacity(nb);
-  return 4 * al256(sizeof(double) * (size_t)cap * dim) + 3 * al256(sizeof(double) * (size_t)cap) +
-         al256((size_t)cap) + al256(sizeof(Partial) * (size_t)(nb + ng)) + al256(sizeof(int64_t) * (size_t)(nb + ng));
-}
-
-// Regions per list the workspace holds at dim (a multiple of 256).
-static int64_t scratch_capacity(int dim)
-{
-  const size_t avail = kScratchBytes - kScratchTableBytes;
-  int64_t lo = 0, hi = (int64_t)(avail / (4 * sizeof(double) * (size_t)dim)) / 256 + 1;
-  while (hi - lo > 1) {
-    const int64_t mid = (lo + hi) / 2;
-    if (scratch_list_bytes(mid * 256, dim) <= avail) lo = mid;
-    else hi = mid;
-  }
-  return lo * 256;
-}
-
 extern "C" {
 
 int cuhre_integrate(cuhre_context* c, const cuhre_integrand* f, int dim, double epsrel, double epsabs,
@@ -1183,12 +1065,6 @@ int cuhre_integrate(cuhre_context* c, const cuhre_integrand* f, int dim, double 
 {
   LoopOpts opt;
   return plain_loop(c, f, dim, epsrel, epsabs, maxeval, key, lows, highs, out, opt, nullptr);
-}
-
-int64_t cuhre_noalloc_capacity(int dim)
-{
-  if (dim < 2 || dim > 10) return -1;
-  return scratch_capacity(dim);
 }
 
 int cuhre_integrate_noalloc(cuhre_context* c, const cuhre_integrand* f, int dim, double epsrel, double epsabs,
@@ -1281,81 +1157,14 @@ int cuhre_integrate_two_phase(cuhre_context* c, const cuhre_integrand* f, int di
     return rc;
   }
   // Phase 2 on the ho.n regions of list ho.cur (split, not yet evaluated)
-  const Phase2Launcher launch = find_phase2_launcher(f->family, dim, key);
-  if (!launch) return fail(CUHRE_ERR_UNSUPPORTED, "no Phase-2 kernel for family %d, dim %d", f->family, dim);
-  DevIntegrand P;
-  if ((rc = make_dev_integrand(c, f, dim, &P))) return rc;
-  const RuleConst R = make_rule(dim, key);
-  const int64_t p = R.npts;
+  const int64_t p = make_rule(dim, key).npts;
   const int64_t n = ho.n;
-  const int64_t slots = (int64_t)pool_max + 2;
-  const size_t per_slot = sizeof(double) * (2 * (size_t)dim + 2) + sizeof(int8_t) + sizeof(int32_t);
-  const size_t ws = (size_t)n * (size_t)slots * per_slot;
-  const size_t outb = (size_t)n * (4 * sizeof(double) + sizeof(int32_t) + 3) + 8 * 256;
-  const size_t need = ws + outb + 8 * 256;
-  if (c->p2_bytes < need) {
-    size_t free_b = 0, total_b = 0;
-    if (c->p2_ws) {
-      (void)hipFree(c->p2_ws);
-      c->p2_ws = nullptr;
-      c->p2_bytes = 0;
-    }
-    HIP_TRY(hipMemGetInfo(&free_b, &total_b));
-    if (need > free_b / 4 * 3)
-      return fail(CUHRE_ERR_CAPACITY, "Phase 2 needs %.1f GB for %lld regions x %d slots", need / 1e9, (long long)n,
-                  pool_max);
-    if (hipMalloc(&c->p2_ws, need) != hipSuccess) {
-      (void)hipGetLastError();
-      c->p2_ws = nullptr;
-      return fail(CUHRE_ERR_ALLOC, "cannot allocate the Phase-2 workspace (%.1f GB)", need / 1e9);
-    }
-    c->p2_bytes = need;
-  }
-  Phase2Args A;
-  A.lower = c->lo[ho.cur];
-  A.width = c->wd[ho.cur];
-  A.ld = c->cap;
-  A.n = n;
-  A.slots = slots;
-  char* q = c->p2_ws;
-  auto take = [&](size_t bytes) {
-    char* r = q;
-    q += (bytes + 255) & ~(size_t)255;
-    return r;
-  };
-  A.geo = (double*)take(sizeof(double) * (size_t)n * slots * 2 * dim);
-  A.sest = (double*)take(sizeof(double) * (size_t)n * slots);
-  A.sax = (int8_t*)take((size_t)n * slots);
-  A.gkerr = (double*)take(sizeof(double) * (size_t)n * slots);
-  A.gkseq = (int32_t*)take(sizeof(int32_t) * (size_t)n * slots);
-  A.res = (double*)take(sizeof(double) * n);
-  A.err = (double*)take(sizeof(double) * n);
-  A.cnt = (int32_t*)take(sizeof(int32_t) * n);
-  A.act = (int8_t*)take(n);
-  A.nonfinite = (int8_t*)take(n);
-  A.zeroed = (int8_t*)take(n);
-  A.rab = (double*)take(sizeof(double) * n);
-  A.rres = (double*)take(sizeof(double) * n);
-  A.epsrel = epsrel;
-  A.epsabs = epsabs;
-  A.pool_max = pool_max;
+  Phase2Host H;
+  if ((rc = run_phase2(c, f, dim, key, ho.cur, n, epsrel, epsabs, pool_max, H))) return rc;
+  const std::vector<double>&hres = H.res, &herr = H.err, &hrab = H.rab, &hrres = H.rres;
+  const std::vector<int32_t>& hcnt = H.cnt;
+  const std::vector<int8_t>&hact = H.act, &hbad = H.bad, &hzero = H.zero;
   int result = 0;
-  std::vector<double> hres(n), herr(n), hrab(n), hrres(n);
-  std::vector<int32_t> hcnt(n);
-  std::vector<int8_t> hact(n), hbad(n), hzero(n);
-  launch(A, P, R, c->stream);
-  if (hipGetLastError() != hipSuccess ||
-      hipMemcpyAsync(hres.data(), A.res, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-      hipMemcpyAsync(herr.data(), A.err, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-      hipMemcpyAsync(hrab.data(), A.rab, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-      hipMemcpyAsync(hrres.data(), A.rres, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-      hipMemcpyAsync(hcnt.data(), A.cnt, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-      hipMemcpyAsync(hact.data(), A.act, n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-      hipMemcpyAsync(hbad.data(), A.nonfinite, n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-      hipMemcpyAsync(hzero.data(), A.zeroed, n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-      hipStreamSynchronize(c->stream) != hipSuccess)
-    result = fail(CUHRE_ERR_DEVICE, "Phase 2 failed: %s", hipGetErrorString(hipGetLastError()));
-  if (result) return result;
   double I_s = ho.fin_s, I_c = ho.fin_c, E_s = ho.fer_s, E_c = ho.fer_c;
   double ab_tot = ho.fin_abs, res_round = 0.0, aest = ho.fin_aest;
   int64_t failed = 0, bad = 0, zeroed = 0;
@@ -1396,6 +1205,35 @@ int cuhre_integrate_two_phase(cuhre_context* c, const cuhre_integrand* f, int di
   out->fail = ok && zeroed == 0 ? 0 : (int32_t)std::min<int64_t>(std::max<int64_t>(failed, 1), INT32_MAX);
   out->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return result;
+}
+
+int cuhre_phase2_regions(cuhre_context* c, const cuhre_integrand* f, int dim, int key, int64_t n,
+                         const double* lower, const double* width, double epsrel, double epsabs, int32_t pool_max,
+                         double* res, double* err, int32_t* count)
+{
+  int rc = check_common(c, f, dim, key);
+  if (rc) return rc;
+  if (n <= 0) return 0;
+  if (!lower || !width || !res || !err || !count) return fail(CUHRE_ERR_ARG, "null buffer");
+  if (pool_max <= 0) pool_max = 2048;
+  if (pool_max > (1 << 24)) return fail(CUHRE_ERR_ARG, "pool_max %d above 2^24", pool_max);
+  if ((rc = ctx_reserve(c, n, dim))) return rc;
+  HIP_TRY(hipMemcpy2DAsync(c->lo[0], sizeof(double) * c->cap, lower, sizeof(double) * n, sizeof(double) * n, dim,
+                           hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpy2DAsync(c->wd[0], sizeof(double) * c->cap, width, sizeof(double) * n, sizeof(double) * n, dim,
+                           hipMemcpyHostToDevice, c->stream));
+  Phase2Host H;
+  if ((rc = run_phase2(c, f, dim, key == 0 ? 7 : key, 0, n, epsrel, epsabs, pool_max, H))) return rc;
+  int64_t bad = 0;
+  for (int64_t r = 0; r < n; ++r) {
+    res[r] = H.res[r];
+    err[r] = H.err[r];
+    count[r] = H.cnt[r];
+    bad += H.bad[r];
+  }
+  if (bad) return fail(CUHRE_ERR_NONFINITE, "integrand returned a non-finite value in Phase 2 (%lld regions)",
+                       (long long)bad);
+  return 0;
 }
 
 int cuhre_eval_regions(cuhre_context* c, const cuhre_integrand* f, int dim, int key, int64_t n,

@@ -151,16 +151,35 @@ struct Family<CUHRE_COSINE_SUM, D> {
 // (512 KiB at 256x256) exceeds one CU's LDS, so it is read through L2.
 template <int D>
 struct Family<CUHRE_TABLE6, D> {
-  __device__ __forceinline__ static double eval(const DevIntegrand& P, const double (&x)[D])
+  // Cell of the table holding x (the same clamp the oracle applies).
+  __device__ __forceinline__ static void cell(const DevIntegrand& P, double x0, double x1, int& ix, int& iy,
+                                              double& tx, double& ty)
   {
+    // no contraction: fusing fx - ix with the product into an FMA gives
+    // other interpolation weights than the oracle's (by up to an ulp of fx)
+#pragma clang fp contract(off)
     const int nx = P.nx, ny = P.ny;
-    double fx = x[0] * (double)(nx - 1), fy = x[1] * (double)(ny - 1);
-    int ix = (int)floor(fx), iy = (int)floor(fy);
+    const double fx = x0 * (double)(nx - 1), fy = x1 * (double)(ny - 1);
+    ix = (int)floor(fx);
+    iy = (int)floor(fy);
     ix = ix < 0 ? 0 : (ix > nx - 2 ? nx - 2 : ix);
     iy = iy < 0 ? 0 : (iy > ny - 2 ? ny - 2 : iy);
-    double tx = fx - (double)ix, ty = fy - (double)iy;
+    tx = fx - (double)ix;
+    ty = fy - (double)iy;
+  }
+  __device__ __forceinline__ static double eval(const DevIntegrand& P, const double (&x)[D])
+  {
+    int ix, iy;
+    double tx, ty;
+    cell(P, x[0], x[1], ix, iy, tx, ty);
+    const int ny = P.ny;
     const double* T = P.table + (size_t)ix * ny + iy;
-    double t00 = T[0], t01 = T[1], t10 = T[ny], t11 = T[ny + 1];
+    return eval_corners(x, tx, ty, T[0], T[1], T[ny], T[ny + 1]);
+  }
+  // Bilinear value from the cell's four corners, times the other factors.
+  __device__ __forceinline__ static double eval_corners(const double (&x)[D], double tx, double ty, double t00,
+                                                        double t01, double t10, double t11)
+  {
     double b0 = fma(tx, t10 - t00, t00);
     double b1 = fma(tx, t11 - t01, t01);
     double B = fma(ty, b1 - b0, b0);

@@ -187,6 +187,8 @@ def lib():
         L.cuhre_noalloc_capacity.restype = C.c_int64
         L.cuhre_integrate_two_phase.argtypes = [ctxp, C.POINTER(Integrand), C.c_int, C.c_double, C.c_double,
                                                 C.c_int, C.c_int64, C.c_int32, dp, dp, C.POINTER(Result)]
+        L.cuhre_phase2_regions.argtypes = [ctxp, C.POINTER(Integrand), C.c_int, C.c_int, C.c_int64, dp, dp,
+                                           C.c_double, C.c_double, C.c_int32, dp, dp, C.POINTER(C.c_int32)]
         L.cuhre_eval_regions.argtypes = [ctxp, C.POINTER(Integrand), C.c_int, C.c_int, C.c_int64,
                                          dp, dp, dp, dp, C.POINTER(C.c_int32), dp, dp, dp]
         L.cuhre_split_regions.argtypes = [ctxp, C.c_int, C.c_int64, dp, dp, dp, dp,
@@ -358,6 +360,20 @@ class Context:
         if want_aux:
             return est, err, ax, ab, i5, dv.T.copy()
         return est, err, ax, ab
+
+    def phase2_regions(self, f: W.Integrand, lower, width, key=9, epsrel=1e-6, epsabs=0.0, pool_max=0):
+        """Phase 2 of the two-phase method on each region: (res, err, count)."""
+        lower = np.asarray(lower, dtype=np.float64)
+        width = np.asarray(width, dtype=np.float64)
+        n, d = lower.shape
+        L = np.ascontiguousarray(lower.T)
+        Wd = np.ascontiguousarray(width.T)
+        res = np.empty(n); err = np.empty(n)
+        cnt = np.empty(n, dtype=np.int32)
+        s, keep = to_struct(f)
+        _check(lib().cuhre_phase2_regions(self.h, C.byref(s), d, key, n, _dp(L), _dp(Wd), epsrel, epsabs, pool_max,
+                                          _dp(res), _dp(err), cnt.ctypes.data_as(C.POINTER(C.c_int32))))
+        return res, err, cnt
 
     def split_regions(self, lower, width, est, err, axis, epsrel, epsabs=0.0):
         lower = np.asarray(lower, dtype=np.float64)

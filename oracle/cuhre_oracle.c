@@ -514,6 +514,25 @@ int or_rule9_init(or_rule9* r, int d)
   return 0;
 }
 
+/* Error of the degree-9 rule from its five rule sums (GPUQuadSample.cu:
+ * 195-216): for each pair of successive null rules (q, q+1), q = 1..3, the
+ * largest over the sets t of the normalised combination |N_{q+1} +
+ * scale[t][q] N_q| norm[t][q]; then with errcoeff {5, 1, 5}: e1 when the
+ * three fall off geometrically (5 e1 <= e2 and 5 e2 <= e3), else 5 max. */
+double or_r9_error(const or_rule9* r, const double* sums, double* e_out)
+{
+  double e[OR_R9_RULES] = {0, 0, 0, 0, 0};
+  for (int q = 1; q < OR_R9_RULES - 1; ++q) {
+    double m = 0.0;
+    for (int s = 0; s < OR_R9_SETS; ++s) m = fmax(m, fabs(sums[q + 1] + r->scale[s][q] * sums[q]) * r->norm[s][q]);
+    e[q] = m;
+  }
+  if (e_out)
+    for (int q = 1; q < 4; ++q) e_out[q - 1] = e[q];
+  const double e1 = e[1], e2 = e[2], e3 = e[3];
+  return (5.0 * e1 <= e2 && 5.0 * e2 <= e3) ? e1 : 5.0 * fmax(fmax(e1, e2), e3);
+}
+
 /* accumulates one point of set s */
 static void r9_add(const or_rule9* r, int s, double v, double* sums, double* sa, qshadow* qs)
 {
@@ -592,16 +611,8 @@ static void r9_region(const or_rule9* r, const or_integrand* f, const double* lo
   }
   if (sums_out)
     for (int q = 0; q < OR_R9_RULES; ++q) sums_out[q] = sums[q];
-  /* the largest normalised null rule in the span of two successive ones */
-  double e[OR_R9_RULES];
-  for (int q = 1; q < OR_R9_RULES - 1; ++q) {
-    double m = 0.0;
-    for (int s = 0; s < OR_R9_SETS; ++s) m = fmax(m, fabs(sums[q + 1] + r->scale[s][q] * sums[q]) * r->norm[s][q]);
-    e[q] = m;
-  }
-  const double e1 = e[1], e2 = e[2], e3 = e[3];
   *est = vol * sums[0];
-  *err = vol * ((5.0 * e1 <= e2 && 5.0 * e2 <= e3) ? e1 : 5.0 * fmax(fmax(e1, e2), e3));
+  *err = vol * or_r9_error(r, sums, NULL);
   *abssum = vol * sa;
   *axis = ax;
 }
@@ -1084,6 +1095,25 @@ static void phase2_region(const anyrule* R, const or_integrand* f, const double*
   *nreg_out = count;
   *active_out = active;
   free(lo); free(wd); free(est); free(er); free(ax); free(seq); free(heap);
+}
+
+void or_phase2_region(int key, const or_integrand* f, const double* lower, const double* width, double epsrel,
+                      double epsabs, int pool_max, double* out)
+{
+  anyrule R;
+  R.key = key == 9 ? 9 : 7;
+  R.d = f->dim;
+  or_rule_init(&R.r7, f->dim);
+  if (R.key == 9) or_rule9_init(&R.r9, f->dim);
+  double res, err, budget[2];
+  int cnt, act, zero;
+  phase2_region(&R, f, lower, width, epsrel, epsabs, pool_max, &res, &err, &cnt, &act, budget, &zero);
+  out[0] = res;
+  out[1] = err;
+  out[2] = (double)cnt;
+  out[3] = (double)act;
+  out[4] = (double)zero;
+  or_rule_free(&R.r7);
 }
 
 int or_integrate_two_phase(const or_integrand* f, const double* lows, const double* highs,

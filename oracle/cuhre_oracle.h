@@ -108,6 +108,10 @@ void or_region_eval9(const or_rule9* r, const or_integrand* f,
                      double* est, double* err, int* axis, double* abssum,
                      double* dvec, double* sums);
 
+/* The degree-9 rule's error (before the volume factor) from its five rule
+ * sums; e_out (may be NULL) receives e1, e2, e3. */
+double or_r9_error(const or_rule9* r, const double* sums, double* e_out);
+
 typedef struct {
   double integral, error, bound;
   int64_t nregions, neval;
@@ -203,6 +207,11 @@ double or_fq_residual(const or_integrand* f, const double* x, double* lo);
 int or_integrate_two_phase(const or_integrand* f, const double* lows, const double* highs,
                            double epsrel, double epsabs, int key, int64_t phase1_max,
                            int pool_max, int nthreads, or_result* out);
+
+/* Phase 2 of one region (GPUQuadPhases.cu:422-536) as the two-phase method
+ * runs it: out[0..4] = result, error, pool size, still active, zeroed. */
+void or_phase2_region(int key, const or_integrand* f, const double* lower, const double* width, double epsrel,
+                      double epsabs, int pool_max, double* out);
 
 /* Cuhre-style global adaptive (heap of regions, always bisect the region
  * with the largest error), same GM7/5 rule: the CPU baseline flavour (i). */

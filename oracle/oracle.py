@@ -130,6 +130,9 @@ def lib():
                                              C.c_int64, C.c_int64, C.c_int, C.c_double, C.POINTER(OrResult)]
         L.or_integrate_bfs_normalized.argtypes = [C.POINTER(OrIntegrand), dp, dp, C.c_double, C.c_double,
                                                   C.c_int64, C.c_int64, C.c_int, C.c_double, C.POINTER(OrResult)]
+        L.or_r9_error.argtypes = [C.POINTER(OrRule9), dp, dp]
+        L.or_r9_error.restype = C.c_double
+        L.or_phase2_region.argtypes = [C.c_int, C.POINTER(OrIntegrand), dp, dp, C.c_double, C.c_double, C.c_int, dp]
         L.or_fp64_bound.argtypes = [C.c_int, C.c_int, C.c_int64, C.c_double, C.c_double, C.c_double]
         L.or_fp64_bound.restype = C.c_double
         L.or_region_check.argtypes = [C.c_int, C.POINTER(OrIntegrand), dp, dp, dp]
@@ -323,6 +326,27 @@ def integrate_bfs_normalized(p: W.Problem, allot, nthreads=8, maxeval=None, epsr
     d = res.asdict()
     d["rc"] = rc
     return d
+
+
+def r9_error(d, sums):
+    """Error of the degree-9 rule (before the volume) from five rule sums;
+    returns (err, (e1, e2, e3))."""
+    r = OrRule9()
+    assert lib().or_rule9_init(C.byref(r), d) == 0
+    s = np.ascontiguousarray(sums, dtype=np.float64)
+    e = np.zeros(3)
+    err = lib().or_r9_error(C.byref(r), _dp(s), _dp(e))
+    return err, e
+
+
+def phase2_region(f: W.Integrand, lower, width, key=9, epsrel=1e-6, epsabs=0.0, pool_max=2048):
+    """Phase 2 of one region: (result, error, pool size, active, zeroed)."""
+    s, keep = to_struct(f)
+    lo = np.ascontiguousarray(lower, dtype=np.float64)
+    wd = np.ascontiguousarray(width, dtype=np.float64)
+    out = np.empty(5)
+    lib().or_phase2_region(key, C.byref(s), _dp(lo), _dp(wd), epsrel, epsabs, pool_max, _dp(out))
+    return out[0], out[1], int(out[2]), int(out[3]), int(out[4])
 
 
 def fp64_bound(p, d, n, A, E, I):

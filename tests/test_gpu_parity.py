@@ -283,3 +283,38 @@ def test_capacity_stop(ctx):
     o = O.integrate_bfs(p, nthreads=8, capacity=4096)
     assert g["rc"] == 1
     _compare_integrations(g, o, p)
+
+
+@pytest.mark.parametrize("lanes", [1, 4, 16])
+def test_table6_clustered_small_regions_match_oracle(ctx, lanes):
+    """Short clustered regions of the table-driven integrand (the late
+    iterations' shape, where rule points fall close to the table's cell
+    edges): per-region values, fourth differences included, against the
+    oracle.  The cell weights x (n - 1) - floor(...) are computed without
+    FMA contraction, as the oracle computes them; with contraction they
+    differed by up to an ulp of x (n - 1) and moved D_i by ~1e-14."""
+    rng = np.random.default_rng(31 + lanes)
+    p = W.c3_table6()
+    n = 4096
+    L = np.empty((n, 6)); Wd = np.empty((n, 6))
+    # 16 clusters of 256 regions, each cluster inside a 0.1 x 0.1 patch of (x1, x2)
+    for c in range(16):
+        x0, y0 = rng.uniform(0, 0.9, 2)
+        sl = slice(256 * c, 256 * (c + 1))
+        Wd[sl, 0] = rng.uniform(0.001, 0.02, 256)
+        Wd[sl, 1] = rng.uniform(0.001, 0.02, 256)
+        L[sl, 0] = x0 + rng.uniform(0, 0.08, 256)
+        L[sl, 1] = y0 + rng.uniform(0, 0.08, 256)
+    Wd[:, 2:] = rng.uniform(0.05, 0.2, (n, 4))
+    L[:, 2] = 0.1 + rng.uniform(0, 0.7, n)
+    L[:, 3:] = rng.uniform(0, 0.8, (n, 3))
+    # and a few wide ones
+    Wd[-8:, 0] = 0.9
+    L[-8:, 0] = 0.05
+    ctx.set(False, lanes)
+    try:
+        g = ctx.eval_regions(p.f, L, Wd, want_aux=True)
+    finally:
+        ctx.set(False, 0)
+    o = O.region_eval(p.f, L, Wd)
+    assert_region_parity(g, o, 6, ("table6 clustered", lanes), np.prod(Wd, axis=1))

@@ -7,7 +7,8 @@ Phase 1 is the breadth-first loop (same region counts as the oracle).  In
 Phase 2 a region's pool picks its next region by the largest error; GPU and
 CPU errors differ in the last bits (the GPU sums a rule's points in another
 order), so a near-tie can pick a different region and change that region's
-count: totals are compared within 0.5%, estimates within the reported error.
+count: totals are compared within 0.2%, errors within 1%, estimates within a
+tenth of the reported error (per-region Phase-2 parity is pinned below).
 """
 import numpy as np
 import pytest
@@ -34,11 +35,12 @@ def test_two_phase_matches_oracle(ctx, case):
     p, key, p1 = CASES[case]
     g = ctx.integrate_two_phase(p, key=key, phase1_regions=p1)
     o = O.integrate_two_phase(p, key=key, phase1_max=p1)
+    print(f"{case}: regions {g['nregions']} / {o['nregions']}, error {g['error']:.6e} / {o['error']:.6e}")
     assert g["rc"] == o["rc"], (g, o)
     assert g["iterations"] == o["iterations"]
-    assert abs(g["nregions"] - o["nregions"]) <= 5e-3 * o["nregions"], (g["nregions"], o["nregions"])
-    assert abs(g["integral"] - o["integral"]) <= max(g["error"], o["error"]) + 1e-12 * abs(o["integral"])
-    assert abs(g["error"] - o["error"]) <= 0.05 * o["error"]
+    assert abs(g["nregions"] - o["nregions"]) <= 2e-3 * o["nregions"], (g["nregions"], o["nregions"])
+    assert abs(g["integral"] - o["integral"]) <= 0.1 * max(g["error"], o["error"]) + 1e-12 * abs(o["integral"])
+    assert abs(g["error"] - o["error"]) <= 0.01 * o["error"]
     if g["rc"] == 0:
         assert g["fail"] == 0
 
@@ -104,3 +106,47 @@ def test_two_phase_large_pool_spills_keys_to_global(ctx):
     assert abs(g["nregions"] - o["nregions"]) <= 5e-3 * o["nregions"], (g["nregions"], o["nregions"])
     assert (g["nregions"] - 64) / 64 > 256  # pools go past the LDS slots on average
     assert abs(g["integral"] - o["integral"]) <= max(g["error"], o["error"]) + 1e-12 * abs(o["integral"])
+
+
+@pytest.mark.parametrize("key,mk", [(9, W.c2_product_peak8), (7, W.c2_product_peak8), (7, W.c1_gaussian3)])
+def test_phase2_per_region_matches_oracle(ctx, key, mk):
+    """Phase 2 region by region (cuhre_phase2_regions against the oracle's
+    or_phase2_region) on the initial grid: the same pool size and, where the
+    bisection sequences agree, the same refined result and error to
+    rounding.  A pool whose largest-error choice meets a near-tie (last-bit
+    differences between the GPU's and the host's rule sums) may take another
+    region; at most 2% of the regions may differ in size."""
+    p = mk()
+    d = p.f.dim
+    L, Wd = O.initial_grid(d, p.lows, p.highs)
+    res, err, cnt = ctx.phase2_regions(p.f, L, Wd, key=key, epsrel=p.epsrel, pool_max=64)
+    same, worst_r, worst_e = 0, 0.0, 0.0
+    for i in range(L.shape[0]):
+        r, e, c, act, zero = O.phase2_region(p.f, L[i], Wd[i], key=key, epsrel=p.epsrel, pool_max=64)
+        if c != cnt[i]:
+            continue
+        same += 1
+        worst_r = max(worst_r, abs(res[i] - r) / abs(r))
+        worst_e = max(worst_e, abs(err[i] - e) / e)
+    print(f"phase2 key {key} {p.name}: {same}/{L.shape[0]} equal pools, worst rel res {worst_r:.1e}, err {worst_e:.1e}")
+    assert same >= 0.98 * L.shape[0]
+    # the errors are differences of rule sums that cancel to ~1e-8 of
+    # sum |w f| (the degree-9 null rules, |I7 - I5|), so their last-bit
+    # differences weigh ~1e-8 relative; measured 9e-10 (key 9)
+    assert worst_r <= 1e-12 and worst_e <= 1e-8
+
+
+def test_two_phase_reference_caps_on_product_peak8(ctx):
+    """The reference's own caps (2^14 Phase-1 regions, quad/quad.h:9,21;
+    2048 bisections per Phase-2 region, GPUQuadPhases.cu:456-458) on the
+    8-D product peak, degree-9 rule, against the oracle."""
+    p = W.c2_product_peak8()
+    g = ctx.integrate_two_phase(p, key=9)
+    o = O.integrate_two_phase(p, key=9, phase1_max=1 << 14, pool_max=2048, nthreads=8)
+    print("reference caps:", g["nregions"], o["nregions"], g["integral"], o["integral"], g["error"], o["error"])
+    assert g["rc"] == o["rc"] == 0 and g["iterations"] == o["iterations"]
+    assert abs(g["nregions"] - o["nregions"]) <= 2e-3 * o["nregions"]
+    assert abs(g["integral"] - o["integral"]) <= 1e-3 * g["error"] + 1e-12 * abs(o["integral"])
+    assert abs(g["error"] - o["error"]) <= 0.01 * o["error"]
+    ex, _ = O.exact(p.f, p.lows, p.highs)
+    assert abs(g["integral"] - ex) <= g["error"]

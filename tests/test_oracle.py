@@ -485,3 +485,124 @@ def test_lclt_density_integrates_to_one(lt, zt):
     fz = lambda k: O.f_values(W.lc_lt_t(dict(t, **{k: [0.0] * len(t[k])})),  # noqa: E731
                               np.array([[lt, zt, c] for c in (0.5, lt / 2)]))
     assert np.max(np.abs(fz("fmsk") - O.f_values(f, np.array([[lt, zt, 0.5], [lt, zt, lt / 2]])))) > 1e-5
+
+
+# The degree-9 rule's error (GPUQuadSample.cu:195-216; constants
+# GPUQuadRule.cu:209-225): pinned by what the construction means, not by
+# restating it.
+
+@pytest.mark.parametrize("d", range(2, 11))
+def test_rule9_null_combinations_are_normalised_null_rules(d):
+    """For each set t and null-rule pair (q, q+1) the combination
+    N_{q+1} + scale[t][q] N_q removes set t (unless set t has no weight in
+    N_q, where the reference uses scale 100), still integrates constants to
+    zero, and norm[t][q] makes its weighted 1-norm exactly 1."""
+    R = O.rule9(d)
+    w, cnt, sc, nm = R["w"], R["count"], R["scale"], R["norm"]
+    for q in range(1, 4):
+        for t in range(9):
+            comb = w[:, q + 1] + sc[t, q] * w[:, q]
+            if w[t, q] != 0.0:
+                assert abs(comb[t]) <= 1e-15 * (abs(w[t, q + 1]) + 1.0), (d, q, t)
+            else:
+                assert sc[t, q] == 100.0
+            assert abs((cnt * comb).sum()) <= 1e-12 * (cnt * np.abs(comb)).sum()
+            assert (cnt * np.abs(comb)).sum() * nm[t, q] == pytest.approx(1.0, rel=1e-14)
+
+
+@pytest.mark.parametrize("d", [3, 8])
+def test_rule9_error_branch_on_chosen_sums(d):
+    """errcoeff {5, 1, 5}: the error is e1 when e1, e2, e3 fall off by 5x
+    each, else 5 max(e1, e2, e3).  Sums that leave only the last null rule
+    give e1 = e2 = 0 < e3 and an error of exactly 0; sums that leave only
+    the first give e2 = e3 = 0 < e1 and an error of 5 e1 (a swapped
+    coefficient or branch fails one of the two)."""
+    R = O.rule9(d)
+    err, e = O.r9_error(d, [1.0, 0.0, 0.0, 0.0, 1.0])
+    assert e[0] == 0.0 and e[1] == 0.0 and e[2] == pytest.approx(R["norm"][:, 3].max())
+    assert err == 0.0
+    err, e = O.r9_error(d, [1.0, 1.0, 0.0, 0.0, 0.0])
+    assert e[1] == 0.0 and e[2] == 0.0
+    assert e[0] == pytest.approx((np.abs(R["scale"][:, 1]) * R["norm"][:, 1]).max())
+    assert err == pytest.approx(5.0 * e[0])
+    # geometric fall-off taken from real sums: the error is homogeneous of
+    # degree 1 in the sums
+    err1, _ = O.r9_error(d, [0.3, 1e-3, -2e-3, 5e-2, 0.7])
+    err2, _ = O.r9_error(d, [0.6, 2e-3, -4e-3, 1e-1, 1.4])
+    assert err2 == pytest.approx(2.0 * err1, rel=1e-14)
+
+
+# Phase 2 (GPUQuadPhases.cu:422-536): the Cuba refinement of the children's
+# errors from the parent difference (GPUQuadPhases.cu:498-515), checked by
+# hand on one and two bisections.
+
+def _refine(pe, l, r, el, er):
+    diff = abs(0.25 * (l + r - pe))
+    e = el + er
+    if e > 0:
+        c = 1 + 2 * diff / e
+        el, er = el * c, er * c
+    return el + diff, er + diff
+
+
+def _eval(key, f, lo, wd):
+    if key == 9:
+        est, err, ax, ab, dv, sums = O.region_eval9(f, lo[None, :], wd[None, :])
+    else:
+        est, err, ax, ab, i5, dv = O.region_eval(f, lo[None, :], wd[None, :])
+    return est[0], err[0], int(ax[0])
+
+
+def _bisect(lo, wd, k):
+    hw = 0.5 * wd[k]
+    lo0, wd0 = lo.copy(), wd.copy()
+    wd0[k] = hw
+    lo1, wd1 = lo.copy(), wd.copy()
+    wd1[k] = hw
+    lo1[k] = lo[k] + hw
+    return (lo0, wd0), (lo1, wd1)
+
+
+@pytest.mark.parametrize("key", [7, 9])
+def test_phase2_one_bisection_by_hand(key):
+    d = 4
+    f = W.product_peak([6.0, 3.0, 5.0, 2.0], [0.3, 0.6, 0.45, 0.2])
+    lo = np.array([0.25, 0.5, 0.25, 0.0])
+    wd = np.array([0.25, 0.25, 0.25, 0.5])
+    pe, pr, k = _eval(key, f, lo, wd)
+    (l0, w0), (l1, w1) = _bisect(lo, wd, k)
+    a, ea, _ = _eval(key, f, l0, w0)
+    b, eb, _ = _eval(key, f, l1, w1)
+    ea2, eb2 = _refine(pe, a, b, ea, eb)
+    res, err, cnt, act, zero = O.phase2_region(f, lo, wd, key=key, pool_max=1)
+    assert cnt == 2 and zero == 0
+    assert res == pe + (a + b - pe)
+    assert err == pr + (ea2 + eb2 - pr)
+    # the refinement only grows errors, by at least the parent difference
+    assert ea2 >= ea + abs(0.25 * (a + b - pe)) and eb2 >= eb
+
+
+@pytest.mark.parametrize("key", [7, 9])
+def test_phase2_two_bisections_take_the_larger_error(key):
+    d = 3
+    f = W.gaussian([7.0, 4.0, 9.0], [0.31, 0.62, 0.17])
+    lo = np.array([0.0, 0.5, 0.0])
+    wd = np.array([0.5, 0.5, 0.5])
+    pe, pr, k = _eval(key, f, lo, wd)
+    (l0, w0), (l1, w1) = _bisect(lo, wd, k)
+    a, ea, ka = _eval(key, f, l0, w0)
+    b, eb, kb = _eval(key, f, l1, w1)
+    ea, eb = _refine(pe, a, b, ea, eb)
+    RES, ERR = pe + (a + b - pe), pr + (ea + eb - pr)
+    # second bisection: the child with the larger refined error (the first
+    # created on a tie)
+    (lp, wp, ep, rp, kp) = (l0, w0, a, ea, ka) if ea >= eb else (l1, w1, b, eb, kb)
+    (m0, v0), (m1, v1) = _bisect(lp, wp, kp)
+    c0, ec0, _ = _eval(key, f, m0, v0)
+    c1, ec1, _ = _eval(key, f, m1, v1)
+    ec0, ec1 = _refine(ep, c0, c1, ec0, ec1)
+    RES += c0 + c1 - ep
+    ERR += ec0 + ec1 - rp
+    res, err, cnt, act, zero = O.phase2_region(f, lo, wd, key=key, epsrel=1e-12, pool_max=2)
+    assert cnt == 3
+    assert res == RES and err == ERR
