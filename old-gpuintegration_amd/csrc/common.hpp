@@ -136,4 +136,53 @@ using EvalLauncher = void (*)(const EvalArgs&, const DevIntegrand&, const RuleCo
 // Returns the launcher for (family, dim, rule key 0/7 or 9) or nullptr.
 EvalLauncher find_eval_launcher(int family, int dim, int key = 7);
 
+// A pseudo-random permutation of [0, n) for the order of training samples,
+// computed per row with no stored order: a bijection of [0, 2^bits) built
+// from four rounds of a key addition, a multiplication by an odd constant
+// and a xor-shift (each invertible modulo 2^bits), walked along its cycles
+// until it lands below n.  The walk ends (the cycle through a start below n
+// comes back to it), and the first value below n after each start is itself
+// a bijection of [0, n) (cycle walking, Black and Rogaway 2002).
+struct Shuffle {
+  uint64_t n, mask;
+  uint64_t key[4];
+  int shift;
+};
+
+__host__ __device__ inline uint64_t shuffle_index(const Shuffle& P, uint64_t x)
+{
+  const auto round = [&](uint64_t k) {
+    x = (x + k) & P.mask;
+    x = (x * 0xD1342543DE82EF95ull) & P.mask;
+    x ^= x >> P.shift;
+  };
+  do {
+    round(P.key[0]);
+    round(P.key[1]);
+    round(P.key[2]);
+    round(P.key[3]);
+  } while (x >= P.n);
+  return x;
+}
+
+// The permutation of [0, n) (n >= 1) drawn from `seed`.
+inline Shuffle make_shuffle(uint64_t n, uint64_t seed)
+{
+  Shuffle P;
+  int bits = 1;
+  while (bits < 63 && (1ull << bits) < n) ++bits;
+  P.n = n;
+  P.mask = (1ull << bits) - 1;
+  P.shift = (bits + 1) / 2;
+  uint64_t s = seed;
+  for (int i = 0; i < 4; ++i) {
+    s += 0x9E3779B97F4A7C15ull;
+    uint64_t z = s;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    P.key[i] = z ^ (z >> 31);
+  }
+  return P;
+}
+
 }  // namespace cuhre

@@ -43,11 +43,16 @@ int set_error(int code, const char* msg);
 
 // ---------------------------------------------------------------- training
 
-// Master FP32 parameters -> BF16 W ([K][N]) and W^T ([N][K]) copies.
-__global__ void k_mlp_prepare(const float* params, uint16_t* W, uint16_t* WT)
+// Master FP32 parameters -> BF16 W ([K][N]) and W^T ([N][K]) copies and the
+// training image (whose padding the caller has zeroed).
+__global__ void k_mlp_prepare(const float* params, uint16_t* W, uint16_t* WT, uint16_t* img)
 {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= kMlpBOff[0]) return;
+  if (p >= kMlpParams) return;
+  if (p >= kMlpBOff[0]) {
+    reinterpret_cast<float*>(reinterpret_cast<char*>(img) + kImgBiasByte)[p - kMlpBOff[0]] = params[p];
+    return;
+  }
   int l = 3;
   while (p < kMlpWOff[l]) --l;
   const int K = kMlpK[l], N = kMlpN[l];
@@ -56,6 +61,8 @@ __global__ void k_mlp_prepare(const float* params, uint16_t* W, uint16_t* WT)
   const uint16_t h = f2bf(params[p]);
   W[kMlpWOff[l] + k * N + n] = h;
   WT[kMlpWOff[l] + n * K + k] = h;
+  img[img_wt(l, k, n)] = h;
+  img[img_w(l, k, n)] = h;
 }
 
 // Inference on rows given as FP32 features (n x 32) -> logits (n x 16): one
@@ -79,9 +86,10 @@ __global__ __launch_bounds__(64) void k_mlp_forward(MlpDev M, const float* feat,
 }
 
 // dH[16 x K] = dZ[16 x N] . W^T for this wave's rows, then dZprev = dH * (H > 0)
-// stored as BF16.  W row-major [K][N] (global) gives contiguous B operands;
-// dZ, H and dZprev are LDS rows with padded strides.
-template <int K, int N>
+// stored as BF16.  W row-major [K][N] with row stride LDW (the training
+// image in LDS) gives contiguous B operands; dZ, H and dZprev are LDS rows
+// with padded strides.
+template <int K, int N, int LDW>
 __device__ __forceinline__ void mlp_layer_bwd(const uint16_t* dZs, const uint16_t* W, const uint16_t* Hs,
                                               uint16_t* dZprev)
 {
@@ -94,7 +102,7 @@ __device__ __forceinline__ void mlp_layer_bwd(const uint16_t* dZs, const uint16_
     const bf16x4 a = *reinterpret_cast<const bf16x4*>(dZs + row * (N + kPad) + kt * 16 + kq);
 #pragma unroll
     for (int jt = 0; jt < K / 16; ++jt) {
-      const bf16x4 b = *reinterpret_cast<const bf16x4*>(W + (jt * 16 + row) * N + kt * 16 + kq);
+      const bf16x4 b = *reinterpret_cast<const bf16x4*>(W + (jt * 16 + row) * LDW + kt * 16 + kq);
       acc[jt] = mfma16(a, b, acc[jt]);
     }
   }
@@ -136,88 +144,124 @@ __host__ __device__ __forceinline__ uint64_t mix64(uint64_t z)
   return z ^ (z >> 31);
 }
 
+// A training block's slot in the partial buffer: the gradient over its rows
+// (kMlpParams), then its rows' summed loss, axis hits and finish hits.
+constexpr int kPartStride = kMlpParams + 4;
+
 struct TrainArgs {
-  MlpDev M;
+  const uint4* img;      // training image of the weights (kImgBytes)
   const float* feat;     // samples x 32
   const int8_t* axis;    // samples
   const float* finish;   // samples
-  const int32_t* order;  // batch rows -> sample index
+  const int32_t* order;  // batch rows -> sample index; null: row r is sample
+                         // shuffle_index(shuf, first + r)
+  Shuffle shuf;
+  int64_t first;
   int64_t rows;          // rows in this batch
   int dim;
   float inv_batch;
-  float* part;           // blocks x kMlpParams partial gradients
-  float* loss;           // rows
-  int8_t* correct;       // rows: bit 0 axis argmax right, bit 1 finish right
+  float* part;           // blocks x kPartStride
   uint64_t perm_seed;    // 0: samples as logged; else a random axis relabelling per row
 };
 
-// Forward + backward of 64 rows per block; the block's weight gradient is
-// written to its own slot (reduced in a fixed order by k_mlp_reduce_adam).
+__device__ __forceinline__ int64_t train_sample(const TrainArgs& T, int64_t r)
+{
+  return T.order != nullptr ? (int64_t)T.order[r] : (int64_t)shuffle_index(T.shuf, (uint64_t)(T.first + r));
+}
+
+// Forward + backward of 64 rows per block, 16 per wavefront; the block's
+// weight gradient goes to its own slot (reduced in a fixed order by
+// k_mlp_reduce_adam).  The weights are staged in LDS by one linear copy of
+// the training image, issued together with the row gathers; the forward and
+// backward passes are wave-local, and one block barrier orders the weight
+// gradient after them.
 __global__ __launch_bounds__(256) void k_mlp_train(TrainArgs T)
 {
+  constexpr int kChunks = kImgBytes / 16, kFull = kChunks / 256, kTail = kChunks % 256;
+  __shared__ __align__(16) uint4 s_img[kChunks];
   __shared__ __align__(16) uint16_t X[kMlpRows * kLdIn];
   __shared__ __align__(16) uint16_t H1[kMlpRows * kLdHid], H2[kMlpRows * kLdHid], H3[kMlpRows * kLdHid];
-  __shared__ float Y[kMlpRows * kMlpOut];
+  __shared__ __align__(16) float Y[kMlpRows * kMlpOut];
   __shared__ __align__(16) uint16_t dZ4[kMlpRows * kLdOut];
   __shared__ __align__(16) uint16_t dZ3[kMlpRows * kLdHid], dZ2[kMlpRows * kLdHid], dZ1[kMlpRows * kLdHid];
-  __shared__ int8_t s_perm[kMlpRows][8];  // new axis j reads logged axis s_perm[r][j]
+  __shared__ int8_t s_dst[kMlpRows][8];  // logged axis m of row rr sits at new axis s_dst[rr][m]
   __shared__ int s_label[kMlpRows];
+  __shared__ float s_fin[kMlpRows];
+  __shared__ float s_st[4][3];
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63;
   const int64_t r0 = (int64_t)blockIdx.x * kMlpRows;
-  if (threadIdx.x < kMlpRows) {
+  // thread t gathers features 8 (t & 3) .. 8 (t & 3) + 7 of row t >> 2 (a
+  // wave gathers its own 16 rows)
+  const int gr = t >> 2, gq = (t & 3) * 8;
+  float4 fa = {0.f, 0.f, 0.f, 0.f}, fb = {0.f, 0.f, 0.f, 0.f};
+  if (r0 + gr < T.rows) {
+    const float4* src = reinterpret_cast<const float4*>(T.feat + train_sample(T, r0 + gr) * kMlpIn + gq);
+    fa = src[0];
+    fb = src[1];
+  }
+  uint4 wv[kFull], wt = {0u, 0u, 0u, 0u};
+#pragma unroll
+  for (int i = 0; i < kFull; ++i) wv[i] = T.img[t + i * 256];
+  if (t < kTail) wt = T.img[kFull * 256 + t];
+  if (t < kMlpRows) {
     // Axis relabelling (the rule is symmetric under permuting the axes, so
-    // is the label): position, width and fourth-difference features of axis
-    // j come from axis perm[j], the label moves with them.  Fisher-Yates
-    // from a per-sample hash.
-    const int rr = threadIdx.x;
-    const int64_t r = r0 + rr;
-    int8_t* pm = s_perm[rr];
-    for (int j = 0; j < 8; ++j) pm[j] = (int8_t)j;
+    // is the label): position, width and fourth-difference features of
+    // logged axis m move to new axis dst[m], the label with them.
+    // Fisher-Yates from a per-sample hash.
+    const int64_t r = r0 + t;
+    uint32_t pm = 0x76543210u;  // nibble j: the logged axis new axis j holds
     int lab = 0;
+    float fin = 0.f;
     if (r < T.rows) {
-      const int64_t smp = T.order[r];
+      const int64_t smp = train_sample(T, r);
       lab = T.axis[smp];
+      fin = T.finish[smp];
       if (T.perm_seed != 0) {
         uint64_t h = mix64(T.perm_seed ^ (uint64_t)smp * 0x9E3779B97F4A7C15ull);
         for (int j = T.dim - 1; j > 0; --j) {
-          const int k = (int)(h % (uint64_t)(j + 1));
+          const int k = (int)(((h >> 32) * (uint64_t)(j + 1)) >> 32);  // uniform in [0, j]
           h = mix64(h);
-          const int8_t t = pm[j];
-          pm[j] = pm[k];
-          pm[k] = t;
+          const uint32_t a = (pm >> (4 * j)) & 15u, b = (pm >> (4 * k)) & 15u;
+          pm &= ~((15u << (4 * j)) | (15u << (4 * k)));
+          pm |= (b << (4 * j)) | (a << (4 * k));
         }
-        for (int j = 0; j < T.dim; ++j)
-          if (pm[j] == lab) {
-            lab = j;
-            break;
-          }
       }
     }
-    s_label[rr] = lab;
+    for (int j = 0; j < 8; ++j) s_dst[t][(pm >> (4 * j)) & 15u] = (int8_t)j;
+    s_label[t] = s_dst[t][lab];
+    s_fin[t] = fin;
   }
+#pragma unroll
+  for (int i = 0; i < kFull; ++i) s_img[t + i * 256] = wv[i];
+  if (t < kTail) s_img[kFull * 256 + t] = wt;
   __syncthreads();
-  for (int e = threadIdx.x; e < kMlpRows * kMlpIn; e += blockDim.x) {
-    const int rr = e / kMlpIn;
-    const int64_t r = r0 + rr;
-    int q = e % kMlpIn;
-    const int q0 = q;
-    if (q < 24 && (q & 7) < T.dim) q = (q & ~7) | s_perm[rr][q & 7];
-    X[rr * kLdIn + q0] = r < T.rows ? f2bf(T.feat[(int64_t)T.order[r] * kMlpIn + q]) : (uint16_t)0;
+  const uint16_t* img = reinterpret_cast<const uint16_t*>(s_img);
+  const float* bias = reinterpret_cast<const float*>(reinterpret_cast<const char*>(s_img) + kImgBiasByte);
+  {
+    const float v[8] = {fa.x, fa.y, fa.z, fa.w, fb.x, fb.y, fb.z, fb.w};
+    uint16_t* xr = X + gr * kLdIn;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int q = gq + i;
+      const int dst = (q < 24 && (q & 7) < T.dim) ? ((q & ~7) | s_dst[gr][q & 7]) : q;
+      xr[dst] = f2bf(v[i]);
+    }
   }
-  __syncthreads();
-  const int w = threadIdx.x >> 6;
-  mlp_forward16(T.M, X + w * 16 * kLdIn, H1 + w * 16 * kLdHid, H2 + w * 16 * kLdHid,
-                H3 + w * 16 * kLdHid, Y + w * 16 * kMlpOut);
-  // loss and dY (softmax cross-entropy over the d axis logits + BCE finish)
-  if (threadIdx.x < kMlpRows) {
-    const int rr = threadIdx.x;
+  __syncwarp();
+  mlp_forward16_wave(img, bias, X + w * 16 * kLdIn, H1 + w * 16 * kLdHid, H2 + w * 16 * kLdHid,
+                     H3 + w * 16 * kLdHid, Y + w * 16 * kMlpOut);
+  // loss and dY of the wave's rows (softmax cross-entropy over the d axis
+  // logits + BCE finish), one row per lane 0..15
+  float st_l = 0.f, st_a = 0.f, st_f = 0.f;
+  if (lane < 16) {
+    const int rr = w * 16 + lane;
     const int64_t r = r0 + rr;
     uint16_t* dz = dZ4 + rr * kLdOut;
     for (int j = 0; j < kMlpOut; ++j) dz[j] = 0;
     if (r < T.rows) {
-      const int64_t s = T.order[r];
       const float* y = Y + rr * kMlpOut;
       const int a = s_label[rr];
-      const float f = T.finish[s];
+      const float f = s_fin[rr];
       float m = y[0];
       int am = 0;
       for (int j = 1; j < T.dim; ++j)
@@ -231,8 +275,9 @@ __global__ __launch_bounds__(256) void k_mlp_train(TrainArgs T)
       const float z = y[kMlpFinish];
       const float q = 1.f / (1.f + expf(-z));
       const float bce = fmaxf(z, 0.f) - z * f + log1pf(expf(-fabsf(z)));
-      T.loss[r] = (lse - y[a]) + bce;
-      T.correct[r] = (int8_t)((am == a ? 1 : 0) | (((q >= 0.5f) == (f >= 0.5f)) ? 2 : 0));
+      st_l = (lse - y[a]) + bce;
+      st_a = am == a ? 1.f : 0.f;
+      st_f = (q >= 0.5f) == (f >= 0.5f) ? 1.f : 0.f;
       for (int j = 0; j < T.dim; ++j) {
         const float pj = expf(y[j] - lse);
         dz[j] = f2bf((pj - (j == a ? 1.f : 0.f)) * T.inv_batch);
@@ -240,25 +285,36 @@ __global__ __launch_bounds__(256) void k_mlp_train(TrainArgs T)
       dz[kMlpFinish] = f2bf((q - f) * T.inv_batch);
     }
   }
-  __syncthreads();
-  mlp_layer_bwd<64, 16>(dZ4 + w * 16 * kLdOut, T.M.W + kMlpWOff[3], H3 + w * 16 * kLdHid,
-                        dZ3 + w * 16 * kLdHid);
-  __syncthreads();
-  mlp_layer_bwd<64, 64>(dZ3 + w * 16 * kLdHid, T.M.W + kMlpWOff[2], H2 + w * 16 * kLdHid,
-                        dZ2 + w * 16 * kLdHid);
-  __syncthreads();
-  mlp_layer_bwd<64, 64>(dZ2 + w * 16 * kLdHid, T.M.W + kMlpWOff[1], H1 + w * 16 * kLdHid,
-                        dZ1 + w * 16 * kLdHid);
+#pragma unroll
+  for (int o = 8; o > 0; o >>= 1) {
+    st_l += __shfl_xor(st_l, o);
+    st_a += __shfl_xor(st_a, o);
+    st_f += __shfl_xor(st_f, o);
+  }
+  if (lane == 0) {
+    s_st[w][0] = st_l;
+    s_st[w][1] = st_a;
+    s_st[w][2] = st_f;
+  }
+  __syncwarp();
+  mlp_layer_bwd<64, 16, 16 + kPad>(dZ4 + w * 16 * kLdOut, img + kImgW[3], H3 + w * 16 * kLdHid,
+                                   dZ3 + w * 16 * kLdHid);
+  __syncwarp();
+  mlp_layer_bwd<64, 64, 64 + kPad>(dZ3 + w * 16 * kLdHid, img + kImgW[2], H2 + w * 16 * kLdHid,
+                                   dZ2 + w * 16 * kLdHid);
+  __syncwarp();
+  mlp_layer_bwd<64, 64, 64 + kPad>(dZ2 + w * 16 * kLdHid, img + kImgW[1], H1 + w * 16 * kLdHid,
+                                   dZ1 + w * 16 * kLdHid);
   __syncthreads();
   // weight gradients: 44 tiles of 16x16 shared out over the 4 waves
-  float* part = T.part + (int64_t)blockIdx.x * kMlpParams;
-  const int l = __lane_id(), row = l & 15, kq = (l >> 4) * 4;
-  for (int t = w; t < 44; t += 4) {
+  float* part = T.part + (int64_t)blockIdx.x * kPartStride;
+  const int row = lane & 15, kq = (lane >> 4) * 4;
+  for (int tl = w; tl < 44; tl += 4) {
     int layer, mt, nt;
-    if (t < 8) { layer = 0; mt = t / 4; nt = t % 4; }
-    else if (t < 24) { layer = 1; mt = (t - 8) / 4; nt = (t - 8) % 4; }
-    else if (t < 40) { layer = 2; mt = (t - 24) / 4; nt = (t - 24) % 4; }
-    else { layer = 3; mt = t - 40; nt = 0; }
+    if (tl < 8) { layer = 0; mt = tl / 4; nt = tl % 4; }
+    else if (tl < 24) { layer = 1; mt = (tl - 8) / 4; nt = (tl - 8) % 4; }
+    else if (tl < 40) { layer = 2; mt = (tl - 24) / 4; nt = (tl - 24) % 4; }
+    else { layer = 3; mt = tl - 40; nt = 0; }
     const uint16_t* Hs = layer == 0 ? X : (layer == 1 ? H1 : (layer == 2 ? H2 : H3));
     const uint16_t* dZs = layer == 0 ? dZ1 : (layer == 1 ? dZ2 : (layer == 2 ? dZ3 : dZ4));
     const int K = kMlpK[layer], N = kMlpN[layer];
@@ -267,7 +323,7 @@ __global__ __launch_bounds__(256) void k_mlp_train(TrainArgs T)
     for (int v = 0; v < 4; ++v) part[kMlpWOff[layer] + (mt * 16 + kq + v) * N + nt * 16 + row] = acc[v];
   }
   // bias gradients: column sums of the BF16 upstream gradients
-  for (int c = threadIdx.x; c < kMlpParams - kMlpBOff[0]; c += blockDim.x) {
+  for (int c = t; c < kMlpParams - kMlpBOff[0]; c += blockDim.x) {
     int layer = 3;
     while (c < kMlpBOff[layer] - kMlpBOff[0]) --layer;
     const int N = kMlpN[layer], col = c - (kMlpBOff[layer] - kMlpBOff[0]);
@@ -276,80 +332,94 @@ __global__ __launch_bounds__(256) void k_mlp_train(TrainArgs T)
     for (int s = 0; s < kMlpRows; ++s) sacc += bf2f(dZs[s * (N + kPad) + col]);
     part[kMlpBOff[0] + c] = sacc;
   }
+  if (t < 3) part[kMlpParams + t] = ((s_st[0][t] + s_st[1][t]) + s_st[2][t]) + s_st[3][t];
 }
 
 // grad[p] = sum over the training blocks' partials in a fixed order (each of
 // kRedWaves wavefronts sums every kRedWaves-th block for 64 parameters, then
 // wavefront 0 adds the kRedWaves sums in order); optional Adam step on the
-// FP32 master parameters followed by the BF16 copies.  With `acc` set, one
-// extra block adds the step's loss and correct counts of `rows` rows into
-// acc[0..2].
+// FP32 master parameters followed by the BF16 copies (W, W^T and the
+// training image).  With `acc` set, one extra block adds the blocks' loss
+// and hit counts, in a fixed order, into acc[0..2].
 constexpr int kRedWaves = 8;
 
-__global__ __launch_bounds__(kRedWaves * 64) void k_mlp_reduce_adam(const float* part, int nblocks, float* grad,
-                                                                     float* params, float* m, float* v, int step,
-                                                                     float lr, int do_update, uint16_t* W,
-                                                                     uint16_t* WT, const float* loss,
-                                                                     const int8_t* correct, int64_t rows,
-                                                                     double* acc)
+struct AdamArgs {
+  const float* part;
+  int nblocks;
+  float* grad;
+  float* params;
+  float* m;
+  float* v;
+  int step;
+  float lr;
+  int do_update;
+  uint16_t* W;
+  uint16_t* WT;
+  uint16_t* img;
+  double* acc;
+};
+
+__global__ __launch_bounds__(kRedWaves * 64) void k_mlp_reduce_adam(AdamArgs A)
 {
   __shared__ float s_g[kRedWaves][64];
-  __shared__ double s_st[kRedWaves][3];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  if (acc != nullptr && blockIdx.x == gridDim.x - 1) {
-    double l = 0, a = 0, f = 0;
-    for (int64_t r = threadIdx.x; r < rows; r += blockDim.x) {
-      l += loss[r];
-      a += (correct[r] & 1) ? 1 : 0;
-      f += (correct[r] & 2) ? 1 : 0;
-    }
-    for (int o = 32; o > 0; o >>= 1) {
-      l += shfl_xor_d(l, o);
-      a += shfl_xor_d(a, o);
-      f += shfl_xor_d(f, o);
-    }
-    if (lane == 0) {
-      s_st[w][0] = l;
-      s_st[w][1] = a;
-      s_st[w][2] = f;
-    }
-    __syncthreads();
-    if (threadIdx.x < 3) {
-      double t = 0;
-      for (int j = 0; j < kRedWaves; ++j) t += s_st[j][threadIdx.x];
-      acc[threadIdx.x] += t;
-    }
+  if (A.acc != nullptr && blockIdx.x == gridDim.x - 1) {
+    if (w != 0) return;
+    double s[3] = {0.0, 0.0, 0.0};
+    for (int b = lane; b < A.nblocks; b += 64)
+      for (int j = 0; j < 3; ++j) s[j] += A.part[(int64_t)b * kPartStride + kMlpParams + j];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+      for (int j = 0; j < 3; ++j) s[j] += shfl_xor_d(s[j], o);
+    if (lane < 3) A.acc[lane] += s[lane];
     return;
   }
   const int p = blockIdx.x * 64 + lane;
   float g = 0.f;
-  if (p < kMlpParams)
-    for (int b = w; b < nblocks; b += kRedWaves) g += part[(int64_t)b * kMlpParams + p];
+  if (p < kMlpParams) {
+    const float* q = A.part + p;
+    int b = w;
+    // the same left-to-right sum as one block at a time, its loads batched
+    for (; b + 3 * kRedWaves < A.nblocks; b += 4 * kRedWaves) {
+      const float x0 = q[(int64_t)b * kPartStride], x1 = q[(int64_t)(b + kRedWaves) * kPartStride];
+      const float x2 = q[(int64_t)(b + 2 * kRedWaves) * kPartStride];
+      const float x3 = q[(int64_t)(b + 3 * kRedWaves) * kPartStride];
+      g += x0;
+      g += x1;
+      g += x2;
+      g += x3;
+    }
+    for (; b < A.nblocks; b += kRedWaves) g += q[(int64_t)b * kPartStride];
+  }
   s_g[w][lane] = g;
   __syncthreads();
   if (w != 0 || p >= kMlpParams) return;
   g = s_g[0][lane];
   for (int j = 1; j < kRedWaves; ++j) g += s_g[j][lane];
-  grad[p] = g;
-  if (!do_update) return;
+  A.grad[p] = g;
+  if (!A.do_update) return;
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-  const float mm = b1 * m[p] + (1.f - b1) * g;
-  const float vv = b2 * v[p] + (1.f - b2) * g * g;
-  m[p] = mm;
-  v[p] = vv;
-  const float mh = mm / (1.f - powf(b1, (float)step));
-  const float vh = vv / (1.f - powf(b2, (float)step));
-  const float np = params[p] - lr * mh / (sqrtf(vh) + eps);
-  params[p] = np;
+  const float mm = b1 * A.m[p] + (1.f - b1) * g;
+  const float vv = b2 * A.v[p] + (1.f - b2) * g * g;
+  A.m[p] = mm;
+  A.v[p] = vv;
+  const float mh = mm / (1.f - powf(b1, (float)A.step));
+  const float vh = vv / (1.f - powf(b2, (float)A.step));
+  const float np = A.params[p] - A.lr * mh / (sqrtf(vh) + eps);
+  A.params[p] = np;
   if (p < kMlpBOff[0]) {
     int l = 3;
     while (p < kMlpWOff[l]) --l;
     const int K = kMlpK[l], N = kMlpN[l];
-    const int q = p - kMlpWOff[l];
-    const int k = q / N, n = q - k * N;
+    const int qq = p - kMlpWOff[l];
+    const int k = qq / N, n = qq - k * N;
     const uint16_t h = f2bf(np);
-    W[kMlpWOff[l] + k * N + n] = h;
-    WT[kMlpWOff[l] + n * K + k] = h;
+    A.W[kMlpWOff[l] + k * N + n] = h;
+    A.WT[kMlpWOff[l] + n * K + k] = h;
+    A.img[img_wt(l, k, n)] = h;
+    A.img[img_w(l, k, n)] = h;
+  } else {
+    reinterpret_cast<float*>(reinterpret_cast<char*>(A.img) + kImgBiasByte)[p - kMlpBOff[0]] = np;
   }
 }
 
@@ -928,6 +998,7 @@ struct cuhre_guide {
   float* v = nullptr;
   uint16_t* W = nullptr;
   uint16_t* WT = nullptr;
+  uint16_t* img = nullptr;  // training image (kImgBytes)
   float* grad = nullptr;
   float* part = nullptr;
   int part_blocks = 0;
@@ -961,10 +1032,9 @@ struct cuhre_guide {
   uint32_t seq = 0;
   // training scratch
   int64_t train_cap = 0;
-  int32_t* order = nullptr;
-  float* loss = nullptr;
-  int8_t* correct = nullptr;
-  double* d_stats = nullptr;
+  int32_t* order = nullptr;  // explicit row order (cuhre_guide_gradient)
+  double* d_stats = nullptr;  // per epoch: loss, axis hits, finish hits, pad
+  int stats_cap = 0;          // epochs d_stats holds
 };
 
 namespace {
@@ -1101,66 +1171,84 @@ int upload_params(cuhre_guide* g, const float* host)
   (void)hipMemsetAsync(g->m, 0, sizeof(float) * kMlpParams, st);
   (void)hipMemsetAsync(g->v, 0, sizeof(float) * kMlpParams, st);
   g->adam_step = 0;
-  hipLaunchKernelGGL(k_mlp_prepare, dim3((kMlpBOff[0] + 255) / 256), dim3(256), 0, st, g->params, g->W, g->WT);
+  hipLaunchKernelGGL(k_mlp_prepare, dim3((kMlpParams + 255) / 256), dim3(256), 0, st, g->params, g->W, g->WT,
+                     g->img);
   if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
     return set_error(CUHRE_ERR_DEVICE, "parameter preparation failed");
   return 0;
 }
 
-// Scratch for steps of `rows` rows and an order array of `order_len` entries.
-int reserve_train(cuhre_guide* g, int64_t rows, int64_t order_len)
+// Scratch for steps of `rows` rows, an explicit order of `order_len`
+// entries and the statistics of `epochs` epochs.
+int reserve_train(cuhre_guide* g, int64_t rows, int64_t order_len, int epochs)
 {
   const int blocks = (int)((rows + kMlpRows - 1) / kMlpRows);
   if (blocks > g->part_blocks) {
     gfree(g->part);
-    if (hipMalloc(&g->part, sizeof(float) * kMlpParams * (size_t)blocks) != hipSuccess) {
+    if (hipMalloc(&g->part, sizeof(float) * kPartStride * (size_t)blocks) != hipSuccess) {
       g->part_blocks = 0;
       return set_error(CUHRE_ERR_ALLOC, "cannot allocate gradient partials");
     }
     g->part_blocks = blocks;
   }
-  const int64_t need = std::max(rows, order_len);
-  if (need > g->train_cap) {
+  if (order_len > g->train_cap) {
     gfree(g->order);
-    gfree(g->loss);
-    gfree(g->correct);
-    if (hipMalloc(&g->order, sizeof(int32_t) * need) != hipSuccess ||
-        hipMalloc(&g->loss, sizeof(float) * need) != hipSuccess || hipMalloc(&g->correct, need) != hipSuccess) {
+    if (hipMalloc(&g->order, sizeof(int32_t) * order_len) != hipSuccess) {
       g->train_cap = 0;
-      return set_error(CUHRE_ERR_ALLOC, "cannot allocate training buffers");
+      return set_error(CUHRE_ERR_ALLOC, "cannot allocate the row order");
     }
-    g->train_cap = need;
+    g->train_cap = order_len;
+  }
+  if (epochs > g->stats_cap) {
+    gfree(g->d_stats);
+    if (hipMalloc(&g->d_stats, sizeof(double) * 4 * epochs) != hipSuccess) {
+      g->stats_cap = 0;
+      return set_error(CUHRE_ERR_ALLOC, "cannot allocate training statistics");
+    }
+    g->stats_cap = epochs;
   }
   return 0;
 }
 
-// One optimisation step (or gradient only) over `rows` sample indices.
-// With `stats` set, the step's loss and correct counts are added into
-// stats[0..2] (the device's epoch sums).
+// One optimisation step (or gradient only) over `rows` rows: row r is sample
+// order[r], or with order null sample shuffle_index(shuf, first + r).  With
+// `stats` set, the step's loss and hit counts are added into stats[0..2].
 void train_step(cuhre_guide* g, const float* feat, const int8_t* ax, const float* fin, const int32_t* order,
-                int64_t rows, int dim, float lr, bool update, hipStream_t st, uint64_t perm_seed = 0,
-                double* stats = nullptr)
+                const Shuffle& shuf, int64_t first, int64_t rows, int dim, float lr, bool update, hipStream_t st,
+                uint64_t perm_seed, double* stats)
 {
   TrainArgs T;
-  T.perm_seed = perm_seed;
-  T.M = mlp_dev(g);
+  T.img = reinterpret_cast<const uint4*>(g->img);
   T.feat = feat;
   T.axis = ax;
   T.finish = fin;
   T.order = order;
+  T.shuf = shuf;
+  T.first = first;
   T.rows = rows;
   T.dim = dim;
   T.inv_batch = 1.0f / (float)rows;
   T.part = g->part;
-  T.loss = g->loss;
-  T.correct = g->correct;
+  T.perm_seed = perm_seed;
   const int blocks = (int)((rows + kMlpRows - 1) / kMlpRows);
   hipLaunchKernelGGL(k_mlp_train, dim3(blocks), dim3(256), 0, st, T);
   if (update) ++g->adam_step;
+  AdamArgs A;
+  A.part = g->part;
+  A.nblocks = blocks;
+  A.grad = g->grad;
+  A.params = g->params;
+  A.m = g->m;
+  A.v = g->v;
+  A.step = std::max(g->adam_step, 1);
+  A.lr = lr;
+  A.do_update = update ? 1 : 0;
+  A.W = g->W;
+  A.WT = g->WT;
+  A.img = g->img;
+  A.acc = stats;
   const unsigned rblocks = (unsigned)((kMlpParams + 63) / 64) + (stats ? 1u : 0u);
-  hipLaunchKernelGGL(k_mlp_reduce_adam, dim3(rblocks), dim3(kRedWaves * 64), 0, st, g->part, blocks, g->grad,
-                     g->params, g->m, g->v, std::max(g->adam_step, 1), lr, update ? 1 : 0, g->W, g->WT, g->loss,
-                     g->correct, rows, stats);
+  hipLaunchKernelGGL(k_mlp_reduce_adam, dim3(rblocks), dim3(kRedWaves * 64), 0, st, A);
 }
 
 struct GuidedRun {
@@ -1511,10 +1599,9 @@ int cuhre_guide_create(cuhre_context* ctx, uint64_t seed, cuhre_guide** out)
       hipMalloc(&g->grad, sizeof(float) * kMlpParams) != hipSuccess ||
       hipMalloc(&g->W, sizeof(uint16_t) * kMlpBOff[0]) != hipSuccess ||
       hipMalloc(&g->WT, sizeof(uint16_t) * kMlpBOff[0]) != hipSuccess ||
+      hipMalloc(&g->img, kImgBytes) != hipSuccess || hipMemset(g->img, 0, kImgBytes) != hipSuccess ||
       hipMalloc(&g->d_sums, sizeof(GuideSums)) != hipSuccess ||
       hipMalloc(&g->d_retire_on, sizeof(int)) != hipSuccess ||
-
-      hipMalloc(&g->d_stats, sizeof(double) * 4) != hipSuccess ||
       hipHostMalloc(&g->h_sums, sizeof(GuideSums), hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc(&g->pub, sizeof(GuidePub), hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
       hipHostMalloc(&g->pub_seq, sizeof(uint32_t), hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
@@ -1545,13 +1632,13 @@ int cuhre_guide_destroy(cuhre_guide* g)
     auto& v = g->ctx->guides;
     v.erase(std::remove(v.begin(), v.end(), g), v.end());
   }
-  for (void* q : {(void*)g->params, (void*)g->m, (void*)g->v, (void*)g->W, (void*)g->WT, (void*)g->grad,
+  for (void* q : {(void*)g->params, (void*)g->m, (void*)g->v, (void*)g->W, (void*)g->WT, (void*)g->img, (void*)g->grad,
                   (void*)g->part, (void*)g->ds_feat, (void*)g->ds_axis, (void*)g->ds_fin, (void*)g->ds_err,
                   (void*)g->ds_vrel, (void*)g->la_lo, (void*)g->la_wd, (void*)g->la_est, (void*)g->la_err,
                   (void*)g->la_ab, (void*)g->la_axis, (void*)g->retire, (void*)g->tile_cnt, (void*)g->tile_off,
                   (void*)g->tile_logcnt, (void*)g->tile_logoff, (void*)g->tile_ret, (void*)g->d_sums, (void*)g->d_retire_on,
                   (void*)g->chunk_part, (void*)g->chunk_off, (void*)g->chunk_log,
-                  (void*)g->order, (void*)g->loss, (void*)g->correct, (void*)g->d_stats})
+                  (void*)g->order, (void*)g->d_stats})
     gfree(q);
   if (g->h_sums) (void)hipHostFree(g->h_sums);
   if (g->pub) (void)hipHostFree(g->pub);
@@ -1603,9 +1690,9 @@ int cuhre_guide_gradient(cuhre_guide* g, int64_t n, const float* features, const
 {
   if (guide_check(g)) return CUHRE_ERR_ARG;
   if (!features || !axis || !finish || !grad) return set_error(CUHRE_ERR_ARG, "null argument");
-  if (n <= 0 || dim < 2 || dim > 8) return set_error(CUHRE_ERR_ARG, "bad size or dimension");
+  if (n <= 0 || n > INT32_MAX || dim < 2 || dim > 8) return set_error(CUHRE_ERR_ARG, "bad size or dimension");
   hipStream_t st = g->ctx->stream;
-  int rc = reserve_train(g, n, n);
+  int rc = reserve_train(g, n, n, 1);
   if (rc) return rc;
   float *dx, *df;
   int8_t* da;
@@ -1619,22 +1706,19 @@ int cuhre_guide_gradient(cuhre_guide* g, int64_t n, const float* features, const
   if (hipMalloc(&dx, sizeof(float) * 32 * n) != hipSuccess || hipMalloc(&df, sizeof(float) * n) != hipSuccess ||
       hipMalloc(&da, n) != hipSuccess)
     return set_error(CUHRE_ERR_ALLOC, "cannot allocate gradient inputs");
+  double acc[4] = {0, 0, 0, 0};
   bool ok = hipMemcpyAsync(dx, features, sizeof(float) * 32 * n, hipMemcpyHostToDevice, st) == hipSuccess &&
             hipMemcpyAsync(df, finish, sizeof(float) * n, hipMemcpyHostToDevice, st) == hipSuccess &&
             hipMemcpyAsync(da, ax8.data(), n, hipMemcpyHostToDevice, st) == hipSuccess &&
-            hipMemcpyAsync(g->order, ord.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, st) == hipSuccess;
+            hipMemcpyAsync(g->order, ord.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, st) == hipSuccess &&
+            hipMemsetAsync(g->d_stats, 0, sizeof(double) * 4, st) == hipSuccess;
   if (ok) {
-    train_step(g, dx, da, df, g->order, n, dim, 0.f, false, st);
-    std::vector<float> lh(n);
+    train_step(g, dx, da, df, g->order, Shuffle{}, 0, n, dim, 0.f, false, st, 0, g->d_stats);
     ok = hipGetLastError() == hipSuccess &&
          hipMemcpyAsync(grad, g->grad, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, st) == hipSuccess &&
-         hipMemcpyAsync(lh.data(), g->loss, sizeof(float) * n, hipMemcpyDeviceToHost, st) == hipSuccess &&
+         hipMemcpyAsync(acc, g->d_stats, sizeof acc, hipMemcpyDeviceToHost, st) == hipSuccess &&
          hipStreamSynchronize(st) == hipSuccess;
-    if (ok && loss) {
-      double s = 0;
-      for (float v : lh) s += v;
-      *loss = s / (double)n;
-    }
+    if (ok && loss) *loss = acc[0] / (double)n;
   }
   (void)hipFree(dx);
   (void)hipFree(df);
@@ -1687,40 +1771,31 @@ int cuhre_guide_train(cuhre_guide* g, int dim, int epochs, int batch, float lr, 
   const int64_t n = g->ds_n;
   if (n < batch) return set_error(CUHRE_ERR_ARG, "fewer samples than one batch");
   hipStream_t st = g->ctx->stream;
-  int rc = reserve_train(g, batch, n);
+  int rc = reserve_train(g, batch, 0, epochs);
   if (rc) return rc;
-  std::vector<int32_t> ord(n);
-  uint64_t sm = seed;
-  auto next = [&]() {
-    sm += 0x9E3779B97F4A7C15ull;
-    uint64_t z = sm;
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    return z ^ (z >> 31);
-  };
+  // Every epoch visits the samples in the order of its own permutation,
+  // computed per row on the device; the host queues the whole run's steps
+  // and reads the per-epoch statistics once at the end.
   const int64_t steps = n / batch;
   const auto t0 = std::chrono::steady_clock::now();
+  if (hipMemsetAsync(g->d_stats, 0, sizeof(double) * 4 * epochs, st) != hipSuccess)
+    return set_error(CUHRE_ERR_DEVICE, "statistics reset failed");
   for (int ep = 0; ep < epochs; ++ep) {
-    for (int64_t i = 0; i < n; ++i) ord[i] = (int32_t)i;
-    for (int64_t i = n - 1; i > 0; --i) std::swap(ord[i], ord[next() % (uint64_t)(i + 1)]);
-    if (hipMemcpyAsync(g->order, ord.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, st) != hipSuccess)
-      return set_error(CUHRE_ERR_DEVICE, "order upload failed");
-    double acc_h[4] = {0, 0, 0, 0};
-    (void)hipMemsetAsync(g->d_stats, 0, sizeof(double) * 4, st);
-    for (int64_t k = 0; k < steps; ++k) {
-      train_step(g, g->ds_feat, g->ds_axis, g->ds_fin, g->order + k * batch, batch, dim, lr, true, st,
-                 mix64(seed ^ (0xA0761D6478BD642Full * (uint64_t)(ep * steps + k + 1))), g->d_stats);
-    }
-    if (hipGetLastError() != hipSuccess ||
-        hipMemcpyAsync(acc_h, g->d_stats, sizeof acc_h, hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess)
-      return set_error(CUHRE_ERR_DEVICE, "training step failed");
-    const double rows = (double)(steps * batch);
-    if (ep == 0) stats->loss_first = acc_h[0] / rows;
-    stats->loss_last = acc_h[0] / rows;
-    stats->axis_accuracy = acc_h[1] / rows;
-    stats->finish_accuracy = acc_h[2] / rows;
+    const Shuffle P = make_shuffle((uint64_t)n, mix64(seed ^ (0x2545F4914F6CDD1Dull * (uint64_t)(ep + 1))));
+    for (int64_t k = 0; k < steps; ++k)
+      train_step(g, g->ds_feat, g->ds_axis, g->ds_fin, nullptr, P, k * batch, batch, dim, lr, true, st,
+                 mix64(seed ^ (0xA0761D6478BD642Full * (uint64_t)(ep * steps + k + 1))), g->d_stats + 4 * ep);
   }
+  std::vector<double> acc((size_t)4 * epochs);
+  if (hipGetLastError() != hipSuccess ||
+      hipMemcpyAsync(acc.data(), g->d_stats, sizeof(double) * 4 * epochs, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return set_error(CUHRE_ERR_DEVICE, "training step failed");
+  const double rows = (double)(steps * batch);
+  stats->loss_first = acc[0] / rows;
+  stats->loss_last = acc[4 * (epochs - 1)] / rows;
+  stats->axis_accuracy = acc[4 * (epochs - 1) + 1] / rows;
+  stats->finish_accuracy = acc[4 * (epochs - 1) + 2] / rows;
   stats->epochs = epochs;
   stats->steps = (int32_t)(steps * epochs);
   stats->samples = n;

@@ -3,8 +3,9 @@
 // 32 -> 64 -> 64 -> 64 -> 16, ReLU.  BF16 weights and BF16 activations enter
 // v_mfma_f32_16x16x16_bf16 (FP32 accumulation).  Two forms of the forward
 // pass: inference (mlp_forward64_wave) runs 64 rows on one wavefront, layer
-// by layer, in LDS of its own; training (mlp_forward16) gives each wavefront
-// of a 256-thread block 16 of the block's 64 rows.  Output columns 0..d-1
+// by layer, in LDS of its own; training (mlp_forward16_wave) gives each
+// wavefront of a 256-thread block 16 of the block's 64 rows, with the
+// weights staged in LDS.  Output columns 0..d-1
 // are split-axis logits, column 15 the finish logit.
 //
 // Operand layout of v_mfma_f32_16x16x16_bf16 (lane l, item v = 0..3):
@@ -89,40 +90,64 @@ __device__ __forceinline__ void mlp_store_relu(const f32x4 (&acc)[N / 16], const
     }
 }
 
+// Training image: the weights as the training kernel holds them in LDS, so
+// one linear copy stages them.  W^T of layer l as [N][K + kPad] BF16 at
+// element kImgWT[l], W of layer l as [K][N + kPad] BF16 at kImgW[l] (the
+// padding spreads the 16 rows of an operand read over the banks), then the
+// 208 FP32 biases at byte kImgBiasByte.  Padding elements stay zero.
+constexpr int kImgWT[4] = {0, 2304, 6656, 11008};
+constexpr int kImgW[4] = {12096, 14272, 18624, 22976};
+constexpr int kImgBf16 = 24256;
+constexpr int kImgBiasByte = kImgBf16 * 2;
+constexpr int kImgBytes = kImgBiasByte + (kMlpParams - kMlpBOff[0]) * 4;
+static_assert(kImgWT[1] == kImgWT[0] + kMlpN[0] * (kMlpK[0] + kPad) &&
+                  kImgWT[2] == kImgWT[1] + kMlpN[1] * (kMlpK[1] + kPad) &&
+                  kImgWT[3] == kImgWT[2] + kMlpN[2] * (kMlpK[2] + kPad) &&
+                  kImgW[0] == kImgWT[3] + kMlpN[3] * (kMlpK[3] + kPad) &&
+                  kImgW[1] == kImgW[0] + kMlpK[0] * (kMlpN[0] + kPad) &&
+                  kImgW[2] == kImgW[1] + kMlpK[1] * (kMlpN[1] + kPad) &&
+                  kImgW[3] == kImgW[2] + kMlpK[2] * (kMlpN[2] + kPad) &&
+                  kImgBf16 == kImgW[3] + kMlpK[3] * (kMlpN[3] + kPad),
+              "training image layout");
+static_assert(kImgBiasByte % 16 == 0 && kImgBytes % 16 == 0, "training image alignment");
+
+__host__ __device__ constexpr int img_wt(int l, int k, int n) { return kImgWT[l] + n * (kMlpK[l] + kPad) + k; }
+__host__ __device__ constexpr int img_w(int l, int k, int n) { return kImgW[l] + k * (kMlpN[l] + kPad) + n; }
+
 // Forward pass of this wave's 16 rows (the training kernel's form, whose
-// backward pass reuses each wave's activations).  Xs..H3s are this wave's
-// LDS rows (padded strides); Ys receives the 16 x 16 logits in FP32.  The
-// weights come from W^T in global memory (L2-resident, 22 KB).  Every wave
-// of the block calls it together (block barriers between layers).
-__device__ __forceinline__ void mlp_forward16(const MlpDev& M, const uint16_t* Xs, uint16_t* H1s, uint16_t* H2s,
-                                              uint16_t* H3s, float* Ys)
+// backward pass reuses the activations).  Xs..H3s are this wave's LDS rows
+// (padded strides); Ys receives the 16 x 16 logits in FP32.  The weights
+// and biases come from the training image in LDS.  Wave-local: the layers
+// are ordered by wavefront fences.
+__device__ __forceinline__ void mlp_forward16_wave(const uint16_t* img, const float* bias, const uint16_t* Xs,
+                                                   uint16_t* H1s, uint16_t* H2s, uint16_t* H3s, float* Ys)
 {
   {
     f32x4 acc[4];
-    mlp_layer_fwd<32, 64, 32>(Xs, M.WT + kMlpWOff[0], acc);
-    mlp_store_relu<64>(acc, M.b + 0, H1s);
+    mlp_layer_fwd<32, 64, 32 + kPad>(Xs, img + kImgWT[0], acc);
+    mlp_store_relu<64>(acc, bias + 0, H1s);
   }
-  __syncthreads();
+  __syncwarp();
   {
     f32x4 acc[4];
-    mlp_layer_fwd<64, 64, 64>(H1s, M.WT + kMlpWOff[1], acc);
-    mlp_store_relu<64>(acc, M.b + 64, H2s);
+    mlp_layer_fwd<64, 64, 64 + kPad>(H1s, img + kImgWT[1], acc);
+    mlp_store_relu<64>(acc, bias + 64, H2s);
   }
-  __syncthreads();
+  __syncwarp();
   {
     f32x4 acc[4];
-    mlp_layer_fwd<64, 64, 64>(H2s, M.WT + kMlpWOff[2], acc);
-    mlp_store_relu<64>(acc, M.b + 128, H3s);
+    mlp_layer_fwd<64, 64, 64 + kPad>(H2s, img + kImgWT[2], acc);
+    mlp_store_relu<64>(acc, bias + 128, H3s);
   }
-  __syncthreads();
+  __syncwarp();
   {
     f32x4 acc[1];
-    mlp_layer_fwd<64, 16, 64>(H3s, M.WT + kMlpWOff[3], acc);
+    mlp_layer_fwd<64, 16, 64 + kPad>(H3s, img + kImgWT[3], acc);
     const int l = __lane_id(), col = l & 15, r0 = (l >> 4) * 4;
 #pragma unroll
-    for (int v = 0; v < 4; ++v) Ys[(r0 + v) * 16 + col] = acc[0][v] + M.b[192 + col];
+    for (int v = 0; v < 4; ++v) Ys[(r0 + v) * 16 + col] = acc[0][v] + bias[192 + col];
   }
-  __syncthreads();
+  __syncwarp();
 }
 
 // One layer over a wave's 64 rows: the B operands (W^T rows) are loaded once

@@ -14,6 +14,10 @@ do.
 The functions mirror csrc/learn.hip step by step:
   forward(Wb, X)            -> (H0, H1, H2, H3, Y)
   backward(Wb, cache, ...)  -> gradients of the four weight matrices and biases
+and the training loop's sample order and axis relabelling:
+  make_shuffle / shuffle_index -> the rows of an epoch (common.hpp Shuffle)
+  relabel                      -> a row's features and label under a random
+                                  permutation of its axes
 """
 import numpy as np
 
@@ -96,3 +100,67 @@ def grads(Ws, bs, X, axis, finish, d, round_bf16=True):
 def loss_only(Ws, bs, X, axis, finish, d):
     H, Y = forward(Ws, bs, X, round_bf16=False)
     return loss_and_dY(Y, axis, finish, d)[0]
+
+
+# ------------------------------------------------- training order, relabelling
+
+M64 = (1 << 64) - 1
+GOLDEN = 0x9E3779B97F4A7C15
+
+
+def mix64(z):
+    """SplitMix64 step: the output for state z (z advanced by the golden gamma
+    first), as learn.hip's mix64."""
+    z = (z + GOLDEN) & M64
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M64
+    return z ^ (z >> 31)
+
+
+def make_shuffle(n, seed):
+    """The permutation of [0, n) drawn from seed: four rounds of (add key,
+    multiply by an odd constant, xor-shift) on [0, 2^bits), cycle-walked
+    below n."""
+    bits = 1
+    while bits < 63 and (1 << bits) < n:
+        bits += 1
+    keys = [mix64((seed + i * GOLDEN) & M64) for i in range(4)]
+    return n, (1 << bits) - 1, keys, (bits + 1) // 2
+
+
+def shuffle_index(P, x):
+    n, mask, keys, shift = P
+    x = np.array(x, dtype=np.uint64, ndmin=1)
+    mult = np.uint64(0xD1342543DE82EF95)
+    todo = np.ones(x.shape, dtype=bool)
+    while todo.any():
+        y = x[todo]
+        for k in keys:
+            y = (y + np.uint64(k)) & np.uint64(mask)
+            y = (y * mult) & np.uint64(mask)
+            y ^= y >> np.uint64(shift)
+        x[todo] = y
+        todo[todo] = y >= np.uint64(n)
+    return x.astype(np.int64)
+
+
+def relabel(x, label, perm_seed, sample, dim):
+    """Features and label of one row after the random relabelling of its
+    axes: new axis j takes position, width and fourth-difference features
+    (blocks 0-7, 8-15, 16-23) of logged axis pm[j]; pm is a Fisher-Yates
+    shuffle of the first dim axes from a hash of (perm_seed, sample)."""
+    pm = list(range(8))
+    if perm_seed != 0:
+        h = mix64(perm_seed ^ ((sample * GOLDEN) & M64))
+        for j in range(dim - 1, 0, -1):
+            k = ((h >> 32) * (j + 1)) >> 32
+            h = mix64(h)
+            pm[j], pm[k] = pm[k], pm[j]
+    dst = [0] * 8
+    for j in range(8):
+        dst[pm[j]] = j
+    y = np.array(x, copy=True)
+    for q in range(24):
+        if (q & 7) < dim:
+            y[(q & ~7) | dst[q & 7]] = x[q]
+    return y, dst[label]

@@ -2,13 +2,14 @@
 // and UndefinedBehaviorSanitizer (make -C old-gpuintegration_amd sanitize)
 // and run by tests/test_sanitize.py: the rule tables of both rules in every
 // dimension, the flop model, the rounding bound, the initial grid and static
-// workspace arithmetic, the argument checks with their error codes and
-// messages.  Exits non-zero on the first failed check; a sanitizer report
+// workspace arithmetic, the training order's permutation, the argument
+// checks with their error codes and messages.  Exits non-zero on the first failed check; a sanitizer report
 // aborts the program.
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <vector>
 
 #include "context.hpp"
 
@@ -100,6 +101,23 @@ int main()
   std::memset(longmsg, 'x', sizeof longmsg - 1);
   longmsg[sizeof longmsg - 1] = 0;
   CHECK(set_error(CUHRE_ERR_DEVICE, longmsg) == CUHRE_ERR_DEVICE && std::strlen(cuhre_last_error()) < 600);
+
+  // training order: a permutation of [0, n) for every n, and the values the
+  // Python restatement (oracle/mlp_oracle.py) gives
+  for (const uint64_t n : {1ull, 2ull, 3ull, 64ull, 1000ull, 65537ull}) {
+    const Shuffle P = make_shuffle(n, n * 977);
+    std::vector<char> seen(n, 0);
+    for (uint64_t i = 0; i < n; ++i) {
+      const uint64_t v = shuffle_index(P, i);
+      CHECK(v < n && !seen[v]);
+      if (v < n) seen[v] = 1;
+    }
+  }
+  {
+    const Shuffle P = make_shuffle(1000, 12345);
+    const uint64_t want[6] = {195, 337, 220, 19, 681, 514};
+    for (int i = 0; i < 6; ++i) CHECK(shuffle_index(P, (uint64_t)i) == want[i]);
+  }
 
   std::printf(g_fail ? "host checks FAILED\n" : "host checks ok\n");
   return g_fail;

@@ -75,6 +75,65 @@ def test_gradient_matches_oracle(ctx):
         assert np.linalg.norm(a - b) <= 3e-2 * np.linalg.norm(b) + 1e-7, l
 
 
+def test_training_step_matches_oracle(ctx):
+    """One step of cuhre_guide_train against the oracle: the rows the epoch's
+    device-side permutation of the sample log puts in the batch, each row's
+    axis relabelling, the batch loss, and the first Adam step, which moves
+    every parameter by lr against the sign of its gradient (zero where the
+    gradient is zero)."""
+    g = cuhre.Guide(ctx, seed=3)
+    dim = 5
+    f = W.corner_peak([0.7, 1.5, 2.0, 3.0, 4.5], 6)
+    p = W.Problem("corner5small", f, [0.0] * dim, [1.0] * dim, 1e-6, 0.0, 10**8)
+    g.collect(p, max_samples=6000, maxeval=10**8)
+    n = g.samples()
+    assert n >= 2000
+    X, a, fin = g.get_samples()
+    p0 = g.params()
+    batch = 64 * ((n // 2) // 64 + 1)  # one step: batch <= n < 2 batch
+    assert batch <= n < 2 * batch
+    seed, lr = 5, 1e-3
+    st = g.train(dim, epochs=1, batch=batch, lr=lr, seed=seed)
+    assert st["steps"] == 1
+    p1 = g.params()
+    Ws, bs = from_vec(p0)
+
+    def oracle_grad(rows, perm_seed):
+        Xr = np.empty((len(rows), 32))
+        ar = np.empty(len(rows), dtype=np.int64)
+        for i, smp in enumerate(rows):
+            Xr[i], ar[i] = M.relabel(X[smp].astype(np.float64), int(a[smp]), perm_seed, int(smp), dim)
+        L, gW, gb, _ = M.grads(Ws, bs, Xr, ar, fin[rows].astype(np.float64), dim)
+        return L, to_vec(gW, gb).astype(np.float64)
+
+    rows = M.shuffle_index(M.make_shuffle(n, M.mix64(seed ^ 0x2545F4914F6CDD1D)), np.arange(batch))
+    L, go = oracle_grad(rows, M.mix64(seed ^ 0xA0761D6478BD642F))
+    assert abs(st["loss_first"] - L) <= 1e-3 * abs(L), (st["loss_first"], L)
+    delta = p1.astype(np.float64) - p0.astype(np.float64)
+    assert np.all(np.abs(delta) <= lr * (1 + 1e-3))
+    assert np.all(delta[go == 0.0] == 0.0)
+
+    def disagreement(gr):
+        bad = tot = 0
+        for lo, hi in zip(WOFF + BOFF, WOFF[1:] + [BOFF[0]] + BOFF[1:] + [cuhre.GUIDE_PARAMS]):
+            gl, dl = gr[lo:hi], delta[lo:hi]
+            clear = np.abs(gl) > 0.02 * np.abs(gl).max() if np.abs(gl).max() > 0 else np.zeros(gl.shape, bool)
+            bad += int(np.sum(np.sign(dl[clear]) != -np.sign(gl[clear])))
+            tot += int(clear.sum())
+        return bad / tot, tot
+
+    frac, tot = disagreement(go)
+    # the same check against a batch of other rows, or without the
+    # relabelling, fails: the test sees which rows were drawn and how
+    frac_rows, _ = disagreement(oracle_grad(np.arange(batch), M.mix64(seed ^ 0xA0761D6478BD642F))[1])
+    frac_perm, _ = disagreement(oracle_grad(rows, 0)[1])
+    print(f"training step: {tot} clear gradient signs, {frac:.4f} disagree; "
+          f"other rows {frac_rows:.3f}, no relabelling {frac_perm:.3f}")
+    assert tot >= 2000 and frac <= 0.01
+    assert frac_rows >= 0.05 and frac_perm >= 0.05
+    g.close()
+
+
 def test_guided_without_guide_is_the_plain_method(ctx):
     g = cuhre.Guide(ctx, seed=2)
     for p in (W.c1_gaussian3(), W.c5_corner7()):

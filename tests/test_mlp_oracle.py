@@ -1,7 +1,9 @@
 """Pins the CPU oracle of the learned guide's network: BF16 rounding on known
 values, and its backward pass against central finite differences of the
-loss (float64, no rounding)."""
+loss (float64, no rounding); the training order (a permutation of the
+sample log per epoch) and the axis relabelling of training rows."""
 import numpy as np
+import pytest
 
 import mlp_oracle as M
 
@@ -51,3 +53,47 @@ def test_loss_is_cross_entropy_plus_bce():
     ce = -np.log(np.exp(2.0) / np.exp([1.0, 2.0, 0.5]).sum())
     bce = -np.log(1 / (1 + np.exp(-0.3)))
     assert abs(L - (ce + bce)) < 1e-12
+
+
+def test_mix64_is_splitmix64():
+    import workloads as W
+    for s in (0, 1, 7, 0xA0761D6478BD642F):
+        assert M.mix64(s) == W.SplitMix64(s).next_u64()
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 64, 1000, 8192, 65537, 300_000])
+def test_shuffle_is_a_permutation(n):
+    P = M.make_shuffle(n, M.mix64(n))
+    v = M.shuffle_index(P, np.arange(n))
+    assert np.array_equal(np.sort(v), np.arange(n))
+
+
+def test_shuffle_mixes_large_logs():
+    # neighbours in the sample log land in the same 8192-row batch about as
+    # often as under a uniform permutation (8192 / n)
+    n = 600_000
+    v = M.shuffle_index(M.make_shuffle(n, 11), np.arange(n))
+    inv = np.empty(n, np.int64)
+    inv[v] = np.arange(n)
+    b = inv // 8192
+    assert abs(np.mean(b[1:] == b[:-1]) - 8192 / n) <= 0.2 * 8192 / n
+    assert abs(np.corrcoef(np.arange(n), v)[0, 1]) < 0.01
+
+
+@pytest.mark.parametrize("dim", [2, 5, 8])
+def test_relabel_moves_axis_blocks_with_the_label(dim):
+    x = np.arange(32, dtype=np.float64)
+    for smp in range(50):
+        for lab in range(dim):
+            y, nl = M.relabel(x, lab, 0xC0FFEE, smp, dim)
+            # the label's features follow it, in all three blocks
+            for blk in range(3):
+                assert y[8 * blk + nl] == x[8 * blk + lab]
+            # axes beyond dim and the scalar features stay put
+            assert np.array_equal(y[24:], x[24:])
+            for q in range(24):
+                if (q & 7) >= dim:
+                    assert y[q] == x[q]
+            assert sorted(y[:8][:dim]) == list(x[:dim])
+    y, nl = M.relabel(x, 3, 0, 5, 8)
+    assert nl == 3 and np.array_equal(y, x)
