@@ -137,63 +137,197 @@ __device__ __forceinline__ RegionOut seg_eval(const F& f, const double* __restri
   double maxd = 0.0;
   int ax = -1;
 
+  // Point coordinates: a coordinate that sits at the centre is c[m] itself
+  // (equal to fma(h, 0, c), c never being -0), so each point recomputes only
+  // the coordinates its generator moves: one per axis point, one per pair
+  // point (the row's coordinate i is shared by its pairs), and per group of
+  // corners only the coordinates above the group's unrolled low bits.  The
+  // points and their order are those of the plain formula x = c + h * g.
+
   // axis points: four per axis, then the fourth difference of that axis
+  if constexpr (HALF) {
+    // s varies across the lanes of a half: axis i = s (lanes s < D)
 #pragma unroll 1
-  for (int i = s; i < D; i += S) {
-    double fv[4];
+    for (int i = s; i < D; i += S) {
+      double fv[4];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      const double gv = t == 0 ? R.l2 : (t == 1 ? -R.l2 : (t == 2 ? R.l3 : -R.l3));
+      for (int t = 0; t < 4; ++t) {
+        const double gv = t == 0 ? R.l2 : (t == 1 ? -R.l2 : (t == 2 ? R.l3 : -R.l3));
 #pragma unroll
-      for (int m = 0; m < D; ++m) x[m] = fma(h[m], m == i ? gv : 0.0, c[m]);
-      fv[t] = f(x);
-      const double w7 = t < 2 ? R.w7a2 : R.w7a3;
-      const double w5 = t < 2 ? R.w5a2 : R.w5a3;
-      s7 = fma(w7, fv[t], s7);
-      s5 = fma(w5, fv[t], s5);
-      sa = fma(fabs(w7), fabs(fv[t]), sa);
+        for (int m = 0; m < D; ++m) x[m] = fma(h[m], m == i ? gv : 0.0, c[m]);
+        fv[t] = f(x);
+        const double w7 = t < 2 ? R.w7a2 : R.w7a3;
+        const double w5 = t < 2 ? R.w5a2 : R.w5a3;
+        s7 = fma(w7, fv[t], s7);
+        s5 = fma(w5, fv[t], s5);
+        sa = fma(fabs(w7), fabs(fv[t]), sa);
+      }
+      const double t2 = fma(-2.0, f0, fv[0] + fv[1]);
+      const double t3 = fma(-2.0, f0, fv[2] + fv[3]);
+      const double di = fabs(fma(-R.ratio, t3, t2));
+      if (dvec != nullptr && valid) dvec[i * ld + r] = di;
+      if (di > maxd) {
+        maxd = di;
+        ax = i;
+      }
     }
-    const double t2 = fma(-2.0, f0, fv[0] + fv[1]);
-    const double t3 = fma(-2.0, f0, fv[2] + fv[3]);
-    const double di = fabs(fma(-R.ratio, t3, t2));
-    if (dvec != nullptr && valid) dvec[i * ld + r] = di;
-    if (di > maxd) {
-      maxd = di;
-      ax = i;
+  } else {
+    // wave-uniform segment: axes i = s, s + S, ...; i is a compile-time
+    // index in the unrolled loop, so x[i] is the only coordinate computed
+    int next = s;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      if (i != next) continue;
+      next += S;
+      double fv[4];
+#pragma unroll
+      for (int m = 0; m < D; ++m) x[m] = c[m];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const double gv = t == 0 ? R.l2 : (t == 1 ? -R.l2 : (t == 2 ? R.l3 : -R.l3));
+        x[i] = fma(h[i], gv, c[i]);
+        fv[t] = f(x);
+        const double w7 = t < 2 ? R.w7a2 : R.w7a3;
+        const double w5 = t < 2 ? R.w5a2 : R.w5a3;
+        s7 = fma(w7, fv[t], s7);
+        s5 = fma(w5, fv[t], s5);
+        sa = fma(fabs(w7), fabs(fv[t]), sa);
+      }
+      const double t2 = fma(-2.0, f0, fv[0] + fv[1]);
+      const double t3 = fma(-2.0, f0, fv[2] + fv[3]);
+      const double di = fabs(fma(-R.ratio, t3, t2));
+      if (dvec != nullptr && valid) dvec[i * ld + r] = di;
+      if (di > maxd) {
+        maxd = di;
+        ax = i;
+      }
     }
   }
 
-  // pair points at l4, pair q = (i, j) in the order (0,1),(0,2),...,(D-2,D-1)
+  // pair points at l4, pair q = (i, j) in the order (0,1),(0,2),...,(D-2,D-1),
+  // each pair's four points in the order (+,+), (+,-), (-,+), (-,-) of (x_i, x_j)
   constexpr int NPAIR = D * (D - 1) / 2;
+  if constexpr (HALF) {
 #pragma unroll 2
-  for (int q = s; q < NPAIR; q += S) {
-    int i = 0, rem = q;
-    while (rem >= D - 1 - i) {
-      rem -= D - 1 - i;
-      ++i;
+    for (int q = s; q < NPAIR; q += S) {
+      int i = 0, rem = q;
+      while (rem >= D - 1 - i) {
+        rem -= D - 1 - i;
+        ++i;
+      }
+      const int j = i + 1 + rem;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const double gi = t < 2 ? R.l4 : -R.l4;
+        const double gj = (t & 1) ? -R.l4 : R.l4;
+#pragma unroll
+        for (int m = 0; m < D; ++m) x[m] = fma(h[m], m == i ? gi : (m == j ? gj : 0.0), c[m]);
+        const double v = f(x);
+        s7 = fma(R.w7p, v, s7);
+        s5 = fma(R.w5p, v, s5);
+        sa = fma(fabs(R.w7p), fabs(v), sa);
+      }
     }
-    const int j = i + 1 + rem;
+  } else {
+    // rows i (runtime, wave-uniform) hold the row vectors xp / xm (x_i at
+    // +-l4, the rest at the centre); j is a compile-time index, so each point
+    // computes only x_j
+    int next = s;  // next pair index of this segment
+    int q0 = 0;    // index of the pair (i, i + 1)
+#pragma unroll 1
+    for (int i = 0; i < D - 1; ++i) {
+      const int row = D - 1 - i;
+      if (next < q0 + row) {
+        double xp[D], xm[D];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      const double gi = t < 2 ? R.l4 : -R.l4;
-      const double gj = (t & 1) ? -R.l4 : R.l4;
+        for (int m = 0; m < D; ++m) {
+          const double g = m == i ? R.l4 : 0.0;
+          xp[m] = fma(h[m], g, c[m]);
+          xm[m] = fma(h[m], -g, c[m]);
+        }
 #pragma unroll
-      for (int m = 0; m < D; ++m) x[m] = fma(h[m], m == i ? gi : (m == j ? gj : 0.0), c[m]);
-      const double v = f(x);
-      s7 = fma(R.w7p, v, s7);
-      s5 = fma(R.w5p, v, s5);
-      sa = fma(fabs(R.w7p), fabs(v), sa);
+        for (int j = 1; j < D; ++j) {
+          if (j <= i || q0 + (j - i - 1) != next) continue;
+          next += S;
+          const double xjp = fma(h[j], R.l4, c[j]);
+          const double xjm = fma(h[j], -R.l4, c[j]);
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+#pragma unroll
+            for (int m = 0; m < D; ++m) x[m] = t < 2 ? xp[m] : xm[m];
+            x[j] = (t & 1) ? xjm : xjp;
+            const double v = f(x);
+            s7 = fma(R.w7p, v, s7);
+            s5 = fma(R.w5p, v, s5);
+            sa = fma(fabs(R.w7p), fabs(v), sa);
+          }
+        }
+      }
+      q0 += row;
     }
   }
 
-  // 2^D corner points at l5 (degree-7 rule only)
-#pragma unroll 2
-  for (int k = s; k < (1 << D); k += S) {
+  // 2^D corner points at l5 (degree-7 rule only), corner k has x_m at -l5
+  // where bit m of k is set
+  if constexpr (HALF) {
+    // corners k = s, s + 32, ...: bits 0..4 of k are the lane's s
+    if constexpr (D <= 5) {
+      if (s < (1 << D)) {
 #pragma unroll
-    for (int m = 0; m < D; ++m) x[m] = fma(h[m], ((k >> m) & 1) ? -R.l5 : R.l5, c[m]);
-    const double v = f(x);
-    s7 = fma(R.w7v, v, s7);
-    sa = fma(fabs(R.w7v), fabs(v), sa);
+        for (int m = 0; m < D; ++m) x[m] = fma(h[m], ((s >> m) & 1) ? -R.l5 : R.l5, c[m]);
+        const double v = f(x);
+        s7 = fma(R.w7v, v, s7);
+        sa = fma(fabs(R.w7v), fabs(v), sa);
+      }
+    } else {
+      constexpr int LT = D - 5;              // bits of k above the lane's five
+      constexpr int LU = LT < 3 ? LT : 3;    // of which unrolled
+      double cpu[LU], cmu[LU];
+#pragma unroll
+      for (int m = 0; m < 5; ++m) x[m] = fma(h[m], ((s >> m) & 1) ? -R.l5 : R.l5, c[m]);
+#pragma unroll
+      for (int b = 0; b < LU; ++b) {
+        cpu[b] = fma(h[5 + b], R.l5, c[5 + b]);
+        cmu[b] = fma(h[5 + b], -R.l5, c[5 + b]);
+      }
+#pragma unroll 1
+      for (int g = 0; g < (1 << (LT - LU)); ++g) {
+#pragma unroll
+        for (int m = 5 + LU; m < D; ++m)
+          x[m] = fma(h[m], ((g >> (m - 5 - LU)) & 1) ? -R.l5 : R.l5, c[m]);
+#pragma unroll
+        for (int u = 0; u < (1 << LU); ++u) {
+#pragma unroll
+          for (int b = 0; b < LU; ++b) x[5 + b] = ((u >> b) & 1) ? cmu[b] : cpu[b];
+          const double v = f(x);
+          s7 = fma(R.w7v, v, s7);
+          sa = fma(fabs(R.w7v), fabs(v), sa);
+        }
+      }
+    }
+  } else {
+    // groups of 2^LU consecutive corners, group g = s, s + S, ...; the low LU
+    // coordinates come from precomputed c +- l5 h
+    constexpr int LU = D < 3 ? D : 3;
+    double cpu[LU], cmu[LU];
+#pragma unroll
+    for (int b = 0; b < LU; ++b) {
+      cpu[b] = fma(h[b], R.l5, c[b]);
+      cmu[b] = fma(h[b], -R.l5, c[b]);
+    }
+#pragma unroll 1
+    for (int g = s; g < (1 << (D - LU)); g += S) {
+#pragma unroll
+      for (int m = LU; m < D; ++m) x[m] = fma(h[m], ((g >> (m - LU)) & 1) ? -R.l5 : R.l5, c[m]);
+#pragma unroll
+      for (int u = 0; u < (1 << LU); ++u) {
+#pragma unroll
+        for (int b = 0; b < LU; ++b) x[b] = ((u >> b) & 1) ? cmu[b] : cpu[b];
+        const double v = f(x);
+        s7 = fma(R.w7v, v, s7);
+        sa = fma(fabs(R.w7v), fabs(v), sa);
+      }
+    }
   }
 
   if constexpr (HALF) {
