@@ -99,12 +99,44 @@ __device__ __forceinline__ void half_argmax(double& d, int& a)
   }
 }
 
+// The four axis points of axis i (a compile-time index once inlined in an
+// unrolled loop: only x[i] leaves the centre) and that axis' fourth
+// difference.
+template <int D, class F>
+__device__ __forceinline__ void gm_axis(const F& f, const double (&c)[D], const double (&h)[D], int i, double f0,
+                                        const RuleConst& R, double& s7, double& s5, double& sa, double& maxd,
+                                        int& ax, double* dvec, int64_t ld, int64_t r, bool valid)
+{
+  double x[D], fv[4];
+#pragma unroll
+  for (int m = 0; m < D; ++m) x[m] = c[m];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const double gv = t == 0 ? R.l2 : (t == 1 ? -R.l2 : (t == 2 ? R.l3 : -R.l3));
+    x[i] = fma(h[i], gv, c[i]);
+    fv[t] = f(x);
+    const double w7 = t < 2 ? R.w7a2 : R.w7a3;
+    const double w5 = t < 2 ? R.w5a2 : R.w5a3;
+    s7 = fma(w7, fv[t], s7);
+    s5 = fma(w5, fv[t], s5);
+    sa = fma(fabs(w7), fabs(fv[t]), sa);
+  }
+  const double t2 = fma(-2.0, f0, fv[0] + fv[1]);
+  const double t3 = fma(-2.0, f0, fv[2] + fv[3]);
+  const double di = fabs(fma(-R.ratio, t3, t2));
+  if (dvec != nullptr && valid) dvec[i * ld + r] = di;
+  if (di > maxd) {
+    maxd = di;
+    ax = i;
+  }
+}
+
 // Segmented Genz-Malik evaluation of the region held by this lane (see the
 // file comment).  All waves of the block must call it together when S > 1
 // (it contains a block barrier).  F(x) evaluates the lane's integrand.
 // HALF: S = 32 segments of one region in the 32 lanes of a half wavefront
 // (s = lane mod 32), merged by half_sum / half_argmax instead of LDS.
-template <int D, bool HALF = false, class F>
+template <int D, bool HALF = false, bool ONE = false, class F>
 __device__ __forceinline__ RegionOut seg_eval(const F& f, const double* __restrict__ lower,
                                               const double* __restrict__ width, int64_t ld,
                                               int64_t r, bool valid, int64_t rr, int S, int s,
@@ -138,11 +170,15 @@ __device__ __forceinline__ RegionOut seg_eval(const F& f, const double* __restri
   int ax = -1;
 
   // Point coordinates: a coordinate that sits at the centre is c[m] itself
-  // (equal to fma(h, 0, c), c never being -0), so each point recomputes only
-  // the coordinates its generator moves: one per axis point, one per pair
-  // point (the row's coordinate i is shared by its pairs), and per group of
-  // corners only the coordinates above the group's unrolled low bits.  The
-  // points and their order are those of the plain formula x = c + h * g.
+  // (equal to fma(h, 0, c), c never being -0), so each point computes only
+  // the coordinates its generator moves.  With one segment (ONE) the axis
+  // points are straight-line code whose moved coordinate is a compile-time
+  // index, so an integrand that accumulates over the coordinates in order
+  // shares its partial results with the centre's (the compiler's common
+  // subexpressions); corners run in groups of 16 whose last four
+  // coordinates are compile-time, so such an integrand's partial result over
+  // the group's first coordinates is computed once per group.  (Pair points
+  // unrolled the same way needed ~380 registers.)
 
   // axis points: four per axis, then the fourth difference of that axis
   if constexpr (HALF) {
@@ -171,36 +207,18 @@ __device__ __forceinline__ RegionOut seg_eval(const F& f, const double* __restri
         ax = i;
       }
     }
+  } else if (ONE) {
+#pragma unroll
+    for (int i = 0; i < D; ++i) gm_axis<D>(f, c, h, i, f0, R, s7, s5, sa, maxd, ax, dvec, ld, r, valid);
   } else {
-    // wave-uniform segment: axes i = s, s + S, ...; i is a compile-time
-    // index in the unrolled loop, so x[i] is the only coordinate computed
+    // wave-uniform segment: axes i = s, s + S, ... (i compile-time in the
+    // unrolled loop)
     int next = s;
 #pragma unroll
     for (int i = 0; i < D; ++i) {
       if (i != next) continue;
       next += S;
-      double fv[4];
-#pragma unroll
-      for (int m = 0; m < D; ++m) x[m] = c[m];
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const double gv = t == 0 ? R.l2 : (t == 1 ? -R.l2 : (t == 2 ? R.l3 : -R.l3));
-        x[i] = fma(h[i], gv, c[i]);
-        fv[t] = f(x);
-        const double w7 = t < 2 ? R.w7a2 : R.w7a3;
-        const double w5 = t < 2 ? R.w5a2 : R.w5a3;
-        s7 = fma(w7, fv[t], s7);
-        s5 = fma(w5, fv[t], s5);
-        sa = fma(fabs(w7), fabs(fv[t]), sa);
-      }
-      const double t2 = fma(-2.0, f0, fv[0] + fv[1]);
-      const double t3 = fma(-2.0, f0, fv[2] + fv[3]);
-      const double di = fabs(fma(-R.ratio, t3, t2));
-      if (dvec != nullptr && valid) dvec[i * ld + r] = di;
-      if (di > maxd) {
-        maxd = di;
-        ax = i;
-      }
+      gm_axis<D>(f, c, h, i, f0, R, s7, s5, sa, maxd, ax, dvec, ld, r, valid);
     }
   }
 
@@ -267,10 +285,11 @@ __device__ __forceinline__ RegionOut seg_eval(const F& f, const double* __restri
     }
   }
 
-  // 2^D corner points at l5 (degree-7 rule only), corner k has x_m at -l5
-  // where bit m of k is set
+  // 2^D corner points at l5 (degree-7 rule only)
   if constexpr (HALF) {
-    // corners k = s, s + 32, ...: bits 0..4 of k are the lane's s
+    // corner k has x_m at -l5 where bit m of k is set; corners k = s, s + 32,
+    // ...: bits 0..4 of k are the lane's s, the next (up to three) are
+    // unrolled
     if constexpr (D <= 5) {
       if (s < (1 << D)) {
 #pragma unroll
@@ -306,23 +325,26 @@ __device__ __forceinline__ RegionOut seg_eval(const F& f, const double* __restri
       }
     }
   } else {
-    // groups of 2^LU consecutive corners, group g = s, s + S, ...; the low LU
-    // coordinates come from precomputed c +- l5 h
-    constexpr int LU = D < 3 ? D : 3;
+    // corners in lexicographic order of their signs, the last coordinate
+    // fastest: corner k has x_m at -l5 where bit D-1-m of k is set.  Groups
+    // of 2^LU corners (group g = s, s + S, ...) share the first D - LU
+    // coordinates; the last LU come from precomputed c +- l5 h.
+    constexpr int LU = D < 4 ? D : 4;
+    constexpr int M0 = D - LU;  // first unrolled coordinate
     double cpu[LU], cmu[LU];
 #pragma unroll
     for (int b = 0; b < LU; ++b) {
-      cpu[b] = fma(h[b], R.l5, c[b]);
-      cmu[b] = fma(h[b], -R.l5, c[b]);
+      cpu[b] = fma(h[M0 + b], R.l5, c[M0 + b]);
+      cmu[b] = fma(h[M0 + b], -R.l5, c[M0 + b]);
     }
 #pragma unroll 1
-    for (int g = s; g < (1 << (D - LU)); g += S) {
+    for (int g = ONE ? 0 : s; g < (1 << M0); g += ONE ? 1 : S) {
 #pragma unroll
-      for (int m = LU; m < D; ++m) x[m] = fma(h[m], ((g >> (m - LU)) & 1) ? -R.l5 : R.l5, c[m]);
+      for (int m = 0; m < M0; ++m) x[m] = fma(h[m], ((g >> (M0 - 1 - m)) & 1) ? -R.l5 : R.l5, c[m]);
 #pragma unroll
       for (int u = 0; u < (1 << LU); ++u) {
 #pragma unroll
-        for (int b = 0; b < LU; ++b) x[b] = ((u >> b) & 1) ? cmu[b] : cpu[b];
+        for (int b = 0; b < LU; ++b) x[M0 + b] = ((u >> (LU - 1 - b)) & 1) ? cmu[b] : cpu[b];
         const double v = f(x);
         s7 = fma(R.w7v, v, s7);
         sa = fma(fabs(R.w7v), fabs(v), sa);
@@ -406,6 +428,36 @@ __global__ __launch_bounds__(1024) void k_eval_seg(EvalArgs A, DevIntegrand P, R
   }
 }
 
+// One segment (long lists): 256-thread blocks, a region per lane, the point
+// loops straight-line code (seg_eval with ONE).  Its own launch bounds give
+// the compiler the registers that code needs.
+template <int D, int FAM>
+__global__ __launch_bounds__(kBlock) void k_eval_one(EvalArgs A, DevIntegrand P, RuleConst R)
+{
+  __shared__ Partial ldsw[kBlock / kWave];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const bool valid = r < A.n;
+  const int64_t rr = valid ? r : A.n - 1;
+  const PlainIntegrand<D, FAM> f{P};
+  const RegionOut o = seg_eval<D, false, true>(f, A.lower, A.width, A.ld, r, valid, rr, 1, 0, wave, lane,
+                                               kBlock / kWave, nullptr, R, A.dvec);
+  if (valid) {
+    A.est[r] = o.est;
+    A.err[r] = o.err;
+    A.axis[r] = (int8_t)o.axis;
+    if (A.abssum) A.abssum[r] = o.ab;
+    if (A.i5) A.i5[r] = o.i5;
+  }
+  if (A.finalize) {
+    Partial p;
+    partial_from_region(p, o.est, o.err, o.ab, valid, A.loc_epsrel, A.loc_epsabs);
+    block_reduce_partial(p, ldsw, kBlock / kWave);
+    store_block_partial(A, p);
+  }
+}
+
 // Half-wavefront form (S = kHalfS), defined after the degree-9 evaluator.
 template <int D, int FAM, int KEY>
 void launch_half(const EvalArgs& A, const DevIntegrand& P, const RuleConst& R, hipStream_t st);
@@ -417,6 +469,11 @@ void launch_eval(const EvalArgs& A, const DevIntegrand& P, const RuleConst& R, i
   if (S == kHalfS) return launch_half<D, FAM, 7>(A, P, R, st);
   const int rpb = seg_regions_per_block(S);
   const int64_t nb = (A.n + rpb - 1) / rpb;
+  if (S == 1) {
+    hipExtLaunchKernelGGL((k_eval_one<D, FAM>), dim3((unsigned)nb), dim3(kBlock), 0, st, A.prof_start,
+                          A.prof_stop, 0, A, P, R);
+    return;
+  }
   hipExtLaunchKernelGGL((k_eval_seg<D, FAM>), dim3((unsigned)nb), dim3(kWave * seg_waves(S)),
                         (uint32_t)seg_lds_bytes(S), st, A.prof_start, A.prof_stop, 0, A, P, R, S);
 }
