@@ -30,16 +30,21 @@ sys.path.insert(0, os.path.join(ROOT, "old-gpuintegration_amd"))
 import workloads as W  # noqa: E402
 
 FP64_PEAK_TFLOPS = 78.6
-# executed FP64 flops per region of the 8-D product peak's rule evaluation,
-# from the SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 counters (add/mul 1, fma 2,
-# transcendental 1) over a bench run (profiles/eval_seg_product_peak8d.md);
-# the algorithmic model counts 24,426
-EXEC_FLOPS_PER_REGION_PP8 = 23633
+# Executed FP64 flops per region of the rule evaluation, from the
+# SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 counters (add/mul/transcendental 1,
+# fma 2) over every evaluation launch of a run of the workload
+# (profiles/eval_product_peak8d.md).  These are the roofline's work
+# count: the evaluation shares an integrand's per-coordinate partial
+# results between points with equal coordinates, so it executes fewer flops
+# than a point-by-point evaluation (region_flops, the model, reported beside
+# it as model_flops_per_region).
+EXEC_FLOPS_PER_REGION_PP8 = 8453  # 8-D product peak, bench workload (model 24,426)
+EXEC_FLOPS_PER_REGION_T6 = 4941   # table-driven 6-D integrand, epsrel 1e-5 (model 16,322)
 # measured HBM traffic per rule-evaluation launch of the 8-D product peak on
-# this workload (k_eval_half for lists up to 8192 regions, k_eval_seg above;
-# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE averaged over the 13 launches of a
-# run, FETCH_SIZE scaled x2 and WRITE_SIZE x1 as a 32 MiB copy kernel reads)
-TRAFFIC_PER_LAUNCH = 2 * 10144.6 * 1024 + 4080.8 * 1024
+# this workload (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE averaged over the
+# 13 launches of a run, FETCH_SIZE scaled x2 and WRITE_SIZE x1 as a 32 MiB
+# copy kernel reads)
+TRAFFIC_PER_LAUNCH = 2 * 10152.2 * 1024 + 4229.4 * 1024
 
 
 def flop_costs():
@@ -307,14 +312,15 @@ def bench_table6(ctx):
     ctx.set(True, 0)
     prof = ctx.integrate(p)
     ctx.set(False, 0)
-    tf = prof["eval_flops"] / (prof["ms_eval"] * 1e-3) / 1e12
+    tf = EXEC_FLOPS_PER_REGION_T6 * prof["nregions"] / (prof["ms_eval"] * 1e-3) / 1e12
     pts = W.gm_npoints(6)
     row = {"epsrel": 1e-5, "maxeval": 10**9, "rc": best["rc"], "integral": best["integral"],
            "error": best["error"], "nregions": best["nregions"], "neval": best["neval"],
            "iterations": best["iterations"], "ms": best["ms_total"], "evals_per_s": best["neval"] / best["ms_total"] * 1e3,
            "eval_roofline": {"bound": "fp64", "achieved_tflops": tf, "frac": tf / FP64_PEAK_TFLOPS,
                              "ms_eval": prof["ms_eval"], "eval_launches": prof["eval_launches"],
-                             "flops_per_region": prof["eval_flops"] / prof["nregions"],
+                             "flops_per_region": EXEC_FLOPS_PER_REGION_T6,
+                             "model_flops_per_region": prof["eval_flops"] / prof["nregions"],
                              "table_bytes_per_region": 32 * pts,
                              "table_GBps_from_cache": prof["nregions"] * 32 * pts / (prof["ms_eval"] * 1e-3) / 1e9}}
     g = cuhre.Guide(ctx, seed=1)
@@ -442,10 +448,12 @@ def main():
     ev_launches = sum(r["eval_launches"] for r in pp)
     ev_regions = sum(r["nregions"] for r in pp)
     fl = region_flops(problems[1].f, 8, costs)
+    xfl = EXEC_FLOPS_PER_REGION_PP8
     avg_ms = ev_ms / ev_launches
-    achieved = (fl * ev_regions / ev_launches) / (avg_ms * 1e-3) / 1e12
+    achieved = (xfl * ev_regions / ev_launches) / (avg_ms * 1e-3) / 1e12
+    model_tf = (fl * ev_regions / ev_launches) / (avg_ms * 1e-3) / 1e12
     big = max(pp, key=lambda r: r["largest_regions"])
-    big_tf = fl * big["largest_regions"] / (big["ms_eval_largest"] * 1e-3) / 1e12
+    big_tf = xfl * big["largest_regions"] / (big["ms_eval_largest"] * 1e-3) / 1e12
 
     last = runs[-1]
     per = []
@@ -475,14 +483,21 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "fp64", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": TRAFFIC_PER_LAUNCH,
-                     "frac_executed_flops": achieved * EXEC_FLOPS_PER_REGION_PP8 / fl / FP64_PEAK_TFLOPS,
+                     "flops_per_region": xfl,
+                     "flops_note": "executed FP64 flops per region counted by SQ_INSTS_VALU_*_F64 "
+                                   "(profiles/eval_product_peak8d.md); the point-by-point model "
+                                   "(model_flops_per_region) charges every point the whole integrand, which "
+                                   "this evaluation does not execute: it shares the integrand's "
+                                   "per-coordinate partial results between points",
+                     "model_flops_per_region": fl, "model_equivalent_tflops": model_tf,
                      "traffic_note": "HBM bytes per launch of this kernel on this workload, FETCH_SIZE x2 + "
-                                     "WRITE_SIZE x1 (copy-kernel calibration), profiles/eval_seg_product_peak8d.md; "
+                                     "WRITE_SIZE x1 (copy-kernel calibration), profiles/eval_product_peak8d.md; "
                                      "algorithmic 153 B/region x 159,134 regions = 24.3 MB",
-                     "kernel": "rule evaluation of product_peak8d: k_eval_half<8, product_peak, 7> (lists up "
-                               "to 8192 regions) + k_eval_seg<8, product_peak>, every launch",
+                     "kernel": "rule evaluation of product_peak8d, every launch: k_eval_half<8, product_peak, 7> "
+                               "(lists up to 8192 regions), k_eval_seg<8, product_peak> (2..16 segments), "
+                               "k_eval_one<8, product_peak> (one segment, from 64k regions)",
                      "launches": ev_launches, "avg_launch_ms": avg_ms,
-                     "flops_per_region": fl, "regions_per_launch": ev_regions / ev_launches,
+                     "regions_per_launch": ev_regions / ev_launches,
                      "largest_launch": {"regions": big["largest_regions"], "ms": big["ms_eval_largest"],
                                         "tflops": big_tf, "frac": big_tf / FP64_PEAK_TFLOPS}},
         "per_integrand": per,

@@ -75,6 +75,7 @@ struct InstState {
 // parameters (held in registers).
 template <int D, int FAM>
 struct LaneIntegrand {
+  static constexpr bool kUnroll = Family<FAM, D>::kUnroll;
   DevIntegrand P;
   __device__ __forceinline__ double operator()(const double (&x)[D]) const
   {
@@ -171,7 +172,7 @@ __global__ void k_batch_init(BatchBuffers B, int k, int64_t n0)
   B.inst[r] = i;
 }
 
-template <int D, int FAM>
+template <int D, int FAM, bool ONE>
 __device__ __forceinline__ RegionOut batch_seg_eval(const BatchBuffers& B, const RuleConst& R,
                                                     int64_t r, bool valid, int64_t rr, int S, int s,
                                                     int wave, int lane, int W, double* seg_lds)
@@ -187,7 +188,7 @@ __device__ __forceinline__ RegionOut batch_seg_eval(const BatchBuffers& B, const
     f.P.b[m] = bp.b[m];
   }
   f.P.s0 = bp.s0;
-  return seg_eval<D>(f, B.lower, B.width, B.ld, r, valid, rr, S, s, wave, lane, W, seg_lds, R, nullptr);
+  return seg_eval<D, false, ONE>(f, B.lower, B.width, B.ld, r, valid, rr, S, s, wave, lane, W, seg_lds, R, nullptr);
 }
 
 // One launch per family present in the batch.  Instances are ordered by
@@ -195,7 +196,8 @@ __device__ __forceinline__ RegionOut batch_seg_eval(const BatchBuffers& B, const
 // first and last regions bracket FAM may hold it, any other tile is skipped
 // (a block-uniform test, which keeps the segment barriers convergent).
 // Lanes of another family in a boundary tile compute but do not store.
-template <int D, int FAM>
+// ONE: one segment (S = 1), the straight-line point loops of seg_eval.
+template <int D, int FAM, bool ONE>
 __global__ __launch_bounds__(kBlock) void k_batch_eval(BatchBuffers B, RuleConst R, int S)
 {
   extern __shared__ double seg_lds[];
@@ -215,7 +217,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_eval(BatchBuffers B, RuleConst
     const int64_t r = t0 + g * kWave + lane;
     const bool valid = r < n;
     const int64_t rr = valid ? r : n - 1;
-    const RegionOut o = batch_seg_eval<D, FAM>(B, R, r, valid, rr, S, s, wave, lane, W, seg_lds);
+    const RegionOut o = batch_seg_eval<D, FAM, ONE>(B, R, r, valid, rr, S, s, wave, lane, W, seg_lds);
     if (valid && s == 0) {
       const BatchParams& bp = B.params[B.inst[r]];
       if (bp.family == FAM) {
@@ -608,16 +610,29 @@ static void launch_batch_eval(const BatchBuffers& B, const RuleConst& R, int S, 
 {
   const dim3 g(grid), b(kWave * seg_waves(S));
   const size_t lds = seg_lds_bytes(S);
+  if (S == 1) {
+    if (fam_mask & (1u << CUHRE_GAUSSIAN))
+      hipLaunchKernelGGL((k_batch_eval<D, CUHRE_GAUSSIAN, true>), g, b, lds, st, B, R, S);
+    if (fam_mask & (1u << CUHRE_PRODUCT_PEAK))
+      hipLaunchKernelGGL((k_batch_eval<D, CUHRE_PRODUCT_PEAK, true>), g, b, lds, st, B, R, S);
+    if (fam_mask & (1u << CUHRE_OSCILLATORY))
+      hipLaunchKernelGGL((k_batch_eval<D, CUHRE_OSCILLATORY, true>), g, b, lds, st, B, R, S);
+    if (fam_mask & (1u << CUHRE_CORNER_PEAK))
+      hipLaunchKernelGGL((k_batch_eval<D, CUHRE_CORNER_PEAK, true>), g, b, lds, st, B, R, S);
+    if (fam_mask & (1u << CUHRE_CONTINUOUS))
+      hipLaunchKernelGGL((k_batch_eval<D, CUHRE_CONTINUOUS, true>), g, b, lds, st, B, R, S);
+    return;
+  }
   if (fam_mask & (1u << CUHRE_GAUSSIAN))
-    hipLaunchKernelGGL((k_batch_eval<D, CUHRE_GAUSSIAN>), g, b, lds, st, B, R, S);
+    hipLaunchKernelGGL((k_batch_eval<D, CUHRE_GAUSSIAN, false>), g, b, lds, st, B, R, S);
   if (fam_mask & (1u << CUHRE_PRODUCT_PEAK))
-    hipLaunchKernelGGL((k_batch_eval<D, CUHRE_PRODUCT_PEAK>), g, b, lds, st, B, R, S);
+    hipLaunchKernelGGL((k_batch_eval<D, CUHRE_PRODUCT_PEAK, false>), g, b, lds, st, B, R, S);
   if (fam_mask & (1u << CUHRE_OSCILLATORY))
-    hipLaunchKernelGGL((k_batch_eval<D, CUHRE_OSCILLATORY>), g, b, lds, st, B, R, S);
+    hipLaunchKernelGGL((k_batch_eval<D, CUHRE_OSCILLATORY, false>), g, b, lds, st, B, R, S);
   if (fam_mask & (1u << CUHRE_CORNER_PEAK))
-    hipLaunchKernelGGL((k_batch_eval<D, CUHRE_CORNER_PEAK>), g, b, lds, st, B, R, S);
+    hipLaunchKernelGGL((k_batch_eval<D, CUHRE_CORNER_PEAK, false>), g, b, lds, st, B, R, S);
   if (fam_mask & (1u << CUHRE_CONTINUOUS))
-    hipLaunchKernelGGL((k_batch_eval<D, CUHRE_CONTINUOUS>), g, b, lds, st, B, R, S);
+    hipLaunchKernelGGL((k_batch_eval<D, CUHRE_CONTINUOUS, false>), g, b, lds, st, B, R, S);
 }
 
 using BatchEvalLauncher = void (*)(const BatchBuffers&, const RuleConst&, int, int, unsigned,

@@ -113,7 +113,7 @@ int ctx_reserve_aux(cuhre_context* c, int64_t cap, int dim);
 int make_dev_integrand(cuhre_context* c, const cuhre_integrand* f, int dim, DevIntegrand* P,
                        char* table_dst = nullptr, size_t table_bytes = 0);
 int64_t list_capacity(cuhre_context* c, int dim);
-int pick_segments(const cuhre_context* c, int64_t n, int key = 7);
+int pick_segments(const cuhre_context* c, int64_t n, int key = 7, int family = -1, int dim = 0);
 // Block partials / offsets the list buffers of cap regions come with: one
 // per 64 regions, and at least 1024 (the half-wavefront form on 8192 regions).
 inline int64_t block_capacity(int64_t cap) { return cap / kWave + 1 > 1024 ? cap / kWave + 1 : 1024; }

@@ -491,7 +491,30 @@ int64_t list_capacity(cuhre_context* c, int dim)
   return (int64_t)(((double)fr + held) * 0.75 / per);
 }
 
-int pick_segments(const cuhre_context* c, int64_t n, int key)
+// Whether the family's rule evaluation unrolls its point loops at dim (kUnroll).
+bool family_unrolls(int family, int dim)
+{
+  switch (family) {
+  case CUHRE_GAUSSIAN: return Family<CUHRE_GAUSSIAN, 2>::unrolls(dim);
+  case CUHRE_PRODUCT_PEAK: return Family<CUHRE_PRODUCT_PEAK, 2>::unrolls(dim);
+  case CUHRE_OSCILLATORY: return Family<CUHRE_OSCILLATORY, 2>::unrolls(dim);
+  case CUHRE_CORNER_PEAK: return Family<CUHRE_CORNER_PEAK, 2>::unrolls(dim);
+  case CUHRE_CONTINUOUS: return Family<CUHRE_CONTINUOUS, 2>::unrolls(dim);
+  case CUHRE_POLYNOMIAL: return Family<CUHRE_POLYNOMIAL, 2>::unrolls(dim);
+  case CUHRE_COSINE_SUM: return Family<CUHRE_COSINE_SUM, 2>::unrolls(dim);
+  case CUHRE_TABLE6: return Family<CUHRE_TABLE6, 6>::unrolls(dim);
+  case CUHRE_LC_LT_T: return Family<CUHRE_LC_LT_T, 3>::unrolls(dim);
+  default: return false;
+  }
+}
+
+// Segment count of an evaluation launch over n regions (the point-segment
+// sweep of the 8-D product peak and oscillatory integrand on MI355X): the
+// half-wavefront form up to half_max_regions, above it the smallest S with
+// n*S >= fill_lanes lanes, a quarter of that for families whose one-segment
+// kernel unrolls its point loops (it is the fastest from 64k regions up);
+// the degree-9 rule keeps S >= 2 below 2M regions.
+int pick_segments(const cuhre_context* c, int64_t n, int key, int family, int dim)
 {
   const int smax = key == 9 ? 8 : kMaxWaves;  // the degree-9 merge needs more LDS per wave
   // the half-wavefront form needs n/8 block partials
@@ -499,9 +522,10 @@ int pick_segments(const cuhre_context* c, int64_t n, int key)
   if (c->segments == kHalfS && half_fits) return kHalfS;
   if (c->segments > 0) return std::min(c->segments, smax);
   if (n <= c->half_max_regions && half_fits) return kHalfS;
+  const int64_t lanes = key != 9 && family_unrolls(family, dim) ? c->fill_lanes / 4 : c->fill_lanes;
   int S = 1;
-  while (S < smax && n * S < c->fill_lanes) S *= 2;
-  if (S == 1 && n < (2ll << 20)) S = 2;
+  while (S < smax && n * S < lanes) S *= 2;
+  if (key == 9 && S == 1 && n < (2ll << 20)) S = 2;
   return S;
 }
 
@@ -785,7 +809,7 @@ int plain_loop(cuhre_context* c, const cuhre_integrand* f, int dim, double epsre
   int64_t nfin = 0, n = n0;
   int result = CUHRE_NOT_CONVERGED;
   for (int it = 0; it < opt.max_iter; ++it) {
-    const int S = pick_segments(c, n, key);
+    const int S = pick_segments(c, n, key, f->family, dim);
     const EvalLauncher launch = find_eval_launcher(f->family, dim, key);
     EvalArgs A;
     fill_eval_args(c, A, cur, n, dim, epsrel, epsabs);

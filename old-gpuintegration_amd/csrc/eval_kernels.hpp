@@ -60,6 +60,7 @@ inline int seg_regions_per_block(int S)
 // Integrand of the plain kernel: one family, parameters in kernel arguments.
 template <int D, int FAM>
 struct PlainIntegrand {
+  static constexpr bool kUnroll = Family<FAM, D>::kUnroll;
   const DevIntegrand& P;
   __device__ __forceinline__ double operator()(const double (&x)[D]) const
   {
@@ -178,11 +179,14 @@ __device__ __forceinline__ RegionOut seg_eval(const F& f, const double* __restri
   // subexpressions); corners run in groups of 16 whose last four
   // coordinates are compile-time, so such an integrand's partial result over
   // the group's first coordinates is computed once per group.  (Pair points
-  // unrolled the same way needed ~380 registers.)
+  // unrolled the same way needed ~380 registers.)  Families without kUnroll
+  // keep runtime point loops (x = c + h * g at every point), as does the
+  // half-wavefront form for its axes and pairs.
 
   // axis points: four per axis, then the fourth difference of that axis
-  if constexpr (HALF) {
-    // s varies across the lanes of a half: axis i = s (lanes s < D)
+  constexpr bool COMPACT = HALF || !F::kUnroll;  // runtime point loops
+  if constexpr (COMPACT) {
+    // axes i = s, s + S, ... (in a half, s varies across its lanes)
 #pragma unroll 1
     for (int i = s; i < D; i += S) {
       double fv[4];
@@ -225,7 +229,7 @@ __device__ __forceinline__ RegionOut seg_eval(const F& f, const double* __restri
   // pair points at l4, pair q = (i, j) in the order (0,1),(0,2),...,(D-2,D-1),
   // each pair's four points in the order (+,+), (+,-), (-,+), (-,-) of (x_i, x_j)
   constexpr int NPAIR = D * (D - 1) / 2;
-  if constexpr (HALF) {
+  if constexpr (COMPACT) {
 #pragma unroll 2
     for (int q = s; q < NPAIR; q += S) {
       int i = 0, rem = q;
@@ -286,7 +290,17 @@ __device__ __forceinline__ RegionOut seg_eval(const F& f, const double* __restri
   }
 
   // 2^D corner points at l5 (degree-7 rule only)
-  if constexpr (HALF) {
+  if constexpr (!F::kUnroll) {
+    // corner k has x_m at -l5 where bit m of k is set; k = s, s + S, ...
+#pragma unroll 2
+    for (int k = s; k < (1 << D); k += S) {
+#pragma unroll
+      for (int m = 0; m < D; ++m) x[m] = fma(h[m], ((k >> m) & 1) ? -R.l5 : R.l5, c[m]);
+      const double v = f(x);
+      s7 = fma(R.w7v, v, s7);
+      sa = fma(fabs(R.w7v), fabs(v), sa);
+    }
+  } else if constexpr (HALF) {
     // corner k has x_m at -l5 where bit m of k is set; corners k = s, s + 32,
     // ...: bits 0..4 of k are the lane's s, the next (up to three) are
     // unrolled

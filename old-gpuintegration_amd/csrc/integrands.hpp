@@ -53,11 +53,23 @@ __device__ __forceinline__ double recip_dev(double p)
   return fma(r, e, r);
 }
 
+// Each family states unrolls(d) (kUnroll at its D): whether the rule's
+// point loops unroll around its evaluation (seg_eval).  True where the
+// integrand accumulates per-coordinate terms that the compiler can share
+// between points with equal coordinates (products and sums of
+// one-coordinate factors, the table lookups in (x1, x2)); false where one
+// expensive function of all coordinates dominates, so unrolled copies of it
+// would only fill the instruction cache: cos of a sum, exp of a sum above
+// six dimensions (up to six its shared sum still pays), the polynomial's
+// runtime term loop, the cosine sum's d cosines per point and LC_LT_t's
+// erfc and exponentials.
 template <int FAM, int D>
 struct Family;
 
 template <int D>
 struct Family<CUHRE_GAUSSIAN, D> {
+  static constexpr bool unrolls(int d) { return d <= 6; }
+  static constexpr bool kUnroll = unrolls(D);
   __device__ __forceinline__ static double eval(const DevIntegrand& P, const double (&x)[D])
   {
     double s = 0.0;
@@ -72,6 +84,8 @@ struct Family<CUHRE_GAUSSIAN, D> {
 
 template <int D>
 struct Family<CUHRE_PRODUCT_PEAK, D> {
+  static constexpr bool unrolls(int) { return true; }
+  static constexpr bool kUnroll = unrolls(D);
   __device__ __forceinline__ static double eval(const DevIntegrand& P, const double (&x)[D])
   {
     double p = 1.0;
@@ -86,6 +100,8 @@ struct Family<CUHRE_PRODUCT_PEAK, D> {
 
 template <int D>
 struct Family<CUHRE_OSCILLATORY, D> {
+  static constexpr bool unrolls(int) { return false; }
+  static constexpr bool kUnroll = unrolls(D);
   __device__ __forceinline__ static double eval(const DevIntegrand& P, const double (&x)[D])
   {
     double s = P.s0;
@@ -97,6 +113,8 @@ struct Family<CUHRE_OSCILLATORY, D> {
 
 template <int D>
 struct Family<CUHRE_CORNER_PEAK, D> {
+  static constexpr bool unrolls(int) { return true; }
+  static constexpr bool kUnroll = unrolls(D);
   __device__ __forceinline__ static double eval(const DevIntegrand& P, const double (&x)[D])
   {
     double s = 1.0;
@@ -110,6 +128,8 @@ struct Family<CUHRE_CORNER_PEAK, D> {
 
 template <int D>
 struct Family<CUHRE_CONTINUOUS, D> {
+  static constexpr bool unrolls(int d) { return d <= 6; }
+  static constexpr bool kUnroll = unrolls(D);
   __device__ __forceinline__ static double eval(const DevIntegrand& P, const double (&x)[D])
   {
     double s = 0.0;
@@ -121,6 +141,8 @@ struct Family<CUHRE_CONTINUOUS, D> {
 
 template <int D>
 struct Family<CUHRE_POLYNOMIAL, D> {
+  static constexpr bool unrolls(int) { return false; }
+  static constexpr bool kUnroll = unrolls(D);
   __device__ __forceinline__ static double eval(const DevIntegrand& P, const double (&x)[D])
   {
     double s = 0.0;
@@ -137,6 +159,8 @@ struct Family<CUHRE_POLYNOMIAL, D> {
 // The reference driver's test functor (cudaCuhre.cu:11-22, function.cu FUN 5).
 template <int D>
 struct Family<CUHRE_COSINE_SUM, D> {
+  static constexpr bool unrolls(int) { return false; }
+  static constexpr bool kUnroll = unrolls(D);
   __device__ __forceinline__ static double eval(const DevIntegrand&, const double (&x)[D])
   {
     double s = 0.0;
@@ -151,6 +175,8 @@ struct Family<CUHRE_COSINE_SUM, D> {
 // (512 KiB at 256x256) exceeds one CU's LDS, so it is read through L2.
 template <int D>
 struct Family<CUHRE_TABLE6, D> {
+  static constexpr bool unrolls(int) { return true; }
+  static constexpr bool kUnroll = unrolls(D);
   // Cell of the table holding x (the same clamp the oracle applies).
   __device__ __forceinline__ static void cell(const DevIntegrand& P, double x0, double x1, int& ix, int& iy,
                                               double& tx, double& ty)
@@ -227,6 +253,8 @@ __device__ __forceinline__ double lclt_gaussian(double x, double mu, double sigm
 
 template <int D>
 struct Family<CUHRE_LC_LT_T, D> {
+  static constexpr bool unrolls(int) { return false; }
+  static constexpr bool kUnroll = unrolls(D);
   __device__ __forceinline__ static double eval(const DevIntegrand& P, const double (&x)[D])
   {
 #pragma clang fp contract(off)

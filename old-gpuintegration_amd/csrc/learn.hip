@@ -1309,7 +1309,7 @@ int guided_run(cuhre_context* c, cuhre_guide* g, const cuhre_integrand* f, int d
   int64_t nfin = 0, n = n0, logged = 0;
   int result = CUHRE_NOT_CONVERGED;
   for (int it = 0; it < 1000; ++it) {
-    const int S = pick_segments(c, n);
+    const int S = pick_segments(c, n, 7, f->family, dim);
     EvalArgs A;
     fill_eval_args(c, A, cur, n, dim, epsrel, epsabs);
     A.loc_epsrel = loc_rel;
@@ -1517,7 +1517,7 @@ int guided_run(cuhre_context* c, cuhre_guide* g, const cuhre_integrand* f, int d
       LA.axis = g->la_axis;
       LA.abssum = g->la_ab;
       LA.finalize = 0;
-      find_eval_launcher(f->family, dim)(LA, P, R, pick_segments(c, LA.n), st);
+      find_eval_launcher(f->family, dim)(LA, P, R, pick_segments(c, LA.n, 7, f->family, dim), st);
       hipLaunchKernelGGL(k_label_axis, dim3((unsigned)((gsum.n_logged + 255) / 256)), dim3(256), 0, st, g->la_err,
                          gsum.n_logged, dim, g->ds_n, g->ds_axis);
       if (hipGetLastError() != hipSuccess) return set_error(CUHRE_ERR_DEVICE, "sample collection failed");
