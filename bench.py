@@ -180,9 +180,32 @@ def bench_batched(ctx, count=4096):
         ares, ainfo, arc = ctx.integrate_batch(probs, finish_budget=0.25)
         ta.append(time.perf_counter() - t0)
     ta = min(ta)
+    # the persistent form: one launch runs the instances' loops in
+    # per-workgroup region queues, the largest are handed over
+    pers = {}
+    for budget in (0.0, 0.25):
+        ctx.integrate_batch(probs, finish_budget=budget, persistent=True)
+        best = None
+        for _ in range(3):
+            t0 = time.perf_counter()
+            pres, pinfo, prc = ctx.integrate_batch(probs, finish_budget=budget, persistent=True)
+            dt = time.perf_counter() - t0
+            if best is None or dt < best[0]:
+                best = (dt, pres, pinfo, prc)
+        dt, pres, pinfo, prc = best
+        ref = res if budget == 0.0 else ares
+        pers["plain_policy" if budget == 0.0 else "allotment"] = {
+            "batched_s": dt, "rc": prc, "regions": pinfo["total_regions"],
+            "evals_per_s": pinfo["total_neval"] / dt, "not_converged": pinfo["not_converged"],
+            "finished_in_persistent_launch": pinfo["persistent_finished"],
+            "same_as_replayed": sum(1 for a, b in zip(pres, ref) if a["nregions"] == b["nregions"]
+                                    and a["iterations"] == b["iterations"]
+                                    and abs(a["integral"] - b["integral"]) <= 1e-12 * abs(b["integral"]))}
     t0 = time.perf_counter()
     plain = [ctx.integrate(p) for p in probs]
     tp = time.perf_counter() - t0
+    for v in pers.values():
+        v["wall_vs_plain_calls"] = v["batched_s"] / tp
     same = sum(1 for a, b in zip(res, plain) if a["nregions"] == b["nregions"]
                and abs(a["integral"] - b["integral"]) <= 1e-12 * abs(b["integral"]))
     regions = sum(r["nregions"] for r in res)
@@ -201,6 +224,7 @@ def bench_batched(ctx, count=4096):
                           "iterations": ainfo["iterations"],
                           "per_instance_regions_min_max": [min(r["nregions"] for r in ares),
                                                            max(r["nregions"] for r in ares)]},
+            "persistent": pers,
             "_results": {"plain_policy": res, "allotment": ares}}
 
 

@@ -217,7 +217,14 @@ typedef struct {
   int32_t grid_blocks;          /* 0: from cuhre_batch_autotune               */
   int32_t iterations_per_check; /* graph replays between host checks (0: 2)   */
   int32_t use_graph;            /* 1: hipGraph replay, 0: direct launches     */
-  int32_t pad;
+  int32_t persistent;           /* 1: first one persistent launch in which each
+                                   workgroup takes instances from a queue and
+                                   runs each one's whole loop in its own region
+                                   queue (2 x 16384 regions); an instance that
+                                   outgrows its queue is handed over, with its
+                                   list and sums, to the replayed iterations.
+                                   0: replayed iterations only.  Same results
+                                   per instance either way (sums to rounding). */
   int64_t initial_capacity;     /* first list allocation in regions (0: 4 Mi);
                                    the list grows on demand                   */
   double finish_budget;         /* finish policy per instance: 0 the plain
@@ -244,7 +251,8 @@ typedef struct {
   int64_t peak_list;            /* longest region list of the run            */
   int64_t capacity;             /* regions per list buffer at the end        */
   int32_t grows;                /* list growths during the run               */
-  int32_t pad;
+  int32_t persistent_finished;  /* instances the persistent launch finished
+                                   (-1: no persistent launch)                 */
 } cuhre_batch_info;
 
 /* count instances; fs[count]; epsrel/epsabs/maxeval per instance (NULL:

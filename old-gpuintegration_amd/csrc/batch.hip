@@ -686,6 +686,11 @@ struct BatchEngine {
   int64_t g_cap = -1;
   // autotune cache
   int tuned_dim = -1, tuned_S = 0, tuned_grid = 0;
+  // persistent form: queue head + staging offset, per-instance staging positions
+  int64_t* p_head = nullptr;
+  int64_t* p_stage = nullptr;
+  int p_count = 0;
+  std::vector<int64_t> h_stage, h_chunk0;
 };
 
 static void drop_graph(BatchEngine* e)
@@ -792,8 +797,12 @@ static void free_batch(BatchEngine* e)
   ListArrays a = take_arrays(e);
   free_arrays(a);
   put_arrays(e, a);
-  for (void* q : {(void*)e->n, (void*)e->params, (void*)e->state, (void*)e->chunk0})
+  for (void* q : {(void*)e->n, (void*)e->params, (void*)e->state, (void*)e->chunk0, (void*)e->p_head,
+                  (void*)e->p_stage})
     if (q) (void)hipFree(q);
+  e->p_head = nullptr;
+  e->p_stage = nullptr;
+  e->p_count = 0;
   e->chunk0 = nullptr;
   e->n = nullptr;
   e->params = nullptr;
@@ -887,6 +896,389 @@ static int batch_grow(BatchEngine* e, int64_t cap, int parity, int64_t n, hipStr
   put_arrays(e, a);
   e->cap = cap;
   return 0;
+}
+
+// ------------------------------------------------------------ persistent form
+// One launch runs the instances' loops.  Each workgroup takes the next
+// instance from an atomic queue head and runs its whole plain loop in its
+// own region queue -- two ping-pong lists of Q regions (slice w of the list
+// buffers: lists 0 and 1, with est/err/abssum/vrel/axis beside them), a
+// region per lane, block barriers between the evaluation, the reduction,
+// the classification and the split -- then takes the next instance.  The
+// loop step is k_inst_combine's (same sums, tests and counters).  An
+// instance whose children would not fit in Q regions is handed over: its
+// list (before that iteration) is copied to a staging area in list 1 beyond
+// the queues, with its sums, and the replayed iterations continue it; when
+// the staging area is full the instance restarts there from its initial
+// grid.  No block waits for another: the queue head and the staging offset
+// are the only shared state.
+struct PersistArgs {
+  const BatchParams* params;
+  InstState* state;
+  double* lo[2];
+  double* wd[2];
+  double *est, *err, *abssum, *vrel;
+  int8_t* axis;
+  int32_t* head;       // the family's queue head (next instance - first)
+  unsigned long long* stage_off;  // staging area used so far (all families)
+  int64_t* stage;      // per instance: staging position of a handed-over list, -1 restart, -2 done
+  int64_t ld;          // list leading dimension (the engine's capacity)
+  int64_t Q;           // regions per queue list
+  int64_t stage0, stage_cap;  // staging area [stage0, stage0 + stage_cap) of list 1
+  int32_t first, last;  // the family's instances
+  int32_t npts, k0;
+};
+
+// A value the caller knows to be equal in every lane, in scalar registers.
+__device__ __forceinline__ double uniform_d(double v)
+{
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)b);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Block-shared state of the instance a persistent workgroup runs (thread 0
+// takes the loop steps; the other threads read n, |I| and the decision).
+struct PersistShared {
+  Partial part[kBlock / kWave];
+  Partial tot;
+  InstState st, before;
+  int64_t run, n;
+  double absI;
+  long long pos;
+  uint32_t wcnt[kBlock / kWave];
+  int ctl;  // 0: next iteration, 1: finished, 2: handed over
+  int inst;
+};
+
+template <int D, int FAM>
+__device__ void persist_instance(const PersistArgs& A, const RuleConst& R, const BatchParams& bp, int i,
+                                 PersistShared& S)
+{
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  const int64_t Q = A.Q, p = A.npts, ld = A.ld, q0 = (int64_t)blockIdx.x * Q;
+  double* const lo0 = A.lo[0] + q0;
+  double* const lo1 = A.lo[1] + q0;
+  double* const wd0 = A.wd[0] + q0;
+  double* const wd1 = A.wd[1] + q0;
+  double* const est = A.est + q0;
+  double* const err = A.err + q0;
+  double* const abss = A.abssum + q0;
+  double* const vrel = A.vrel + q0;
+  int8_t* const axis = A.axis + q0;
+  // the instance's parameters are uniform across the block: scalar registers
+  LaneIntegrand<D, FAM> f;
+  f.P.family = FAM;
+  f.P.exponent = __builtin_amdgcn_readfirstlane(bp.exponent);
+  double vol[D];  // widths of the instance's box (the allotment's vrel)
+#pragma unroll
+  for (int m = 0; m < D; ++m) {
+    f.P.a[m] = uniform_d(bp.a[m]);
+    f.P.w[m] = uniform_d(bp.w[m]);
+    f.P.b[m] = uniform_d(bp.b[m]);
+    vol[m] = uniform_d(bp.highs[m] - bp.lows[m]);
+  }
+  f.P.s0 = uniform_d(bp.s0);
+  const bool allot = bp.allot > 0.0;
+
+  // initial k^D grid in list 0 (low + id w rounded twice, as k_batch_init)
+  int64_t n = 1;
+  for (int m = 0; m < D; ++m) n *= A.k0;
+  {
+#pragma clang fp contract(off)
+    for (int64_t r = tid; r < n; r += kBlock) {
+      int64_t q = r;
+      for (int m = 0; m < D; ++m) {
+        const int64_t id = q % A.k0;
+        q /= A.k0;
+        const double w = (bp.highs[m] - bp.lows[m]) / (double)A.k0;
+        lo0[m * ld + r] = bp.lows[m] + (double)id * w;
+        wd0[m * ld + r] = w;
+      }
+    }
+  }
+  if (tid == 0) memset(&S.st, 0, sizeof S.st);
+  int cur = 0;
+  const uint64_t below = lane ? (~0ull >> (kWave - lane)) : 0ull;
+  for (;;) {
+    __syncthreads();  // the list written by the previous step is visible
+    double* const lo = cur ? lo1 : lo0;
+    double* const wd = cur ? wd1 : wd0;
+    double* const clo = cur ? lo0 : lo1;
+    double* const cwd = cur ? wd0 : wd1;
+    // evaluation: a region per lane, passes of 256
+    Partial acc;
+    partial_zero(acc);
+#pragma unroll 1
+    for (int64_t b = 0; b < n; b += kBlock) {
+      const int64_t r = b + tid;
+      const bool valid = r < n;
+      const int64_t rr = valid ? r : n - 1;
+      const RegionOut o = seg_eval<D, false, true>(f, lo, wd, ld, r, valid, rr, 1, 0, wave, lane, kBlock / kWave,
+                                                   nullptr, R, nullptr);
+      if (valid) {
+        est[r] = o.est;
+        err[r] = o.err;
+        abss[r] = o.ab;
+        axis[r] = (int8_t)o.axis;
+        if (allot) {
+          double v = 1.0;
+#pragma unroll
+          for (int m = 0; m < D; ++m) v *= wd[m * ld + r] / vol[m];
+          vrel[r] = v;
+        }
+        dd_add(acc.est_s, acc.est_c, o.est, 0.0);
+        dd_add(acc.err_s, acc.err_c, o.err, 0.0);
+        acc.abs_all += o.ab;
+        acc.aest_all += fabs(o.est);
+        if (!(isfinite(o.est) && isfinite(o.err))) acc.nnonfinite += 1;
+      }
+    }
+    block_reduce_partial(acc, S.part, kBlock / kWave);
+    if (tid == 0) {
+      // the iteration's sums (k_inst_combine's step, before the classification)
+      S.tot = acc;
+      S.before = S.st;
+      InstState& st = S.st;
+      double rs = st.fin_s, rc = st.fin_c;
+      dd_add(rs, rc, acc.est_s, acc.est_c);
+      double es = st.fer_s, ec = st.fer_c;
+      dd_add(es, ec, acc.err_s, acc.err_c);
+      const double rG = rs + rc;
+      st.iters += 1;
+      st.nregions += n;
+      st.neval += n * p;
+      st.peak = n > st.peak ? n : st.peak;
+      st.integral = rG;
+      st.error = es + ec;
+      st.bound = fp64_rounding_bound(p, D, st.nfin + n, st.abs_fin + acc.abs_all, st.aest_fin + acc.aest_all, rG);
+      S.absI = fabs(rG);
+      S.run = 0;
+    }
+    __syncthreads();
+    const double absI = S.absI;
+    // classification (the iteration's finished sums), stable compaction and
+    // bisection into the other list; children that would not fit are not
+    // written (the instance is then handed over)
+    Partial fa;
+    partial_zero(fa);
+#pragma unroll 1
+    for (int64_t b = 0; b < n; b += kBlock) {
+      const int64_t r = b + tid;
+      bool act = false;
+      if (r < n) {
+        const double e = est[r], er = err[r];
+        act = inst_region_active(bp, absI, e, er, allot ? vrel[r] : 0.0);
+        if (!act) {
+          dd_add(fa.fest_s, fa.fest_c, e, 0.0);
+          dd_add(fa.ferr_s, fa.ferr_c, er, 0.0);
+          fa.abs_fin += abss[r];
+          fa.aest_fin += fabs(e);
+        }
+      }
+      const uint64_t mask = __ballot(act);
+      if (lane == 0) S.wcnt[wave] = (uint32_t)__popcll(mask);
+      __syncthreads();
+      int64_t rank = S.run + __popcll(mask & below), blk = 0;
+      for (int w = 0; w < kBlock / kWave; ++w) {
+        if (w < wave) rank += S.wcnt[w];
+        blk += S.wcnt[w];
+      }
+      if (act && 2 * rank + 1 < Q) {
+        const int a = axis[r];
+        for (int m = 0; m < D; ++m) {
+          const double l = lo[m * ld + r], w = wd[m * ld + r];
+          double2 L, W;
+          if (m == a) {
+            const double hw = 0.5 * w;
+            L = make_double2(l, l + hw);
+            W = make_double2(hw, hw);
+          } else {
+            L = make_double2(l, l);
+            W = make_double2(w, w);
+          }
+          *reinterpret_cast<double2*>(clo + m * ld + 2 * rank) = L;
+          *reinterpret_cast<double2*>(cwd + m * ld + 2 * rank) = W;
+        }
+      }
+      __syncthreads();  // every wave has read run and wcnt
+      if (tid == 0) S.run += blk;
+    }
+    block_reduce_partial(fa, S.part, kBlock / kWave);
+    if (tid == 0) {
+      InstState& st = S.st;
+      const Partial& tot = S.tot;
+      const int64_t nact = S.run;
+      dd_add(st.fin_s, st.fin_c, fa.fest_s, fa.fest_c);
+      dd_add(st.fer_s, st.fer_c, fa.ferr_s, fa.ferr_c);
+      st.abs_fin += fa.abs_fin;
+      st.aest_fin += fa.aest_fin;
+      st.nfin += n - nact;
+      st.next = 0;
+      int ctl = 0;
+      if (tot.nnonfinite > 0) {
+        st.done = 1;
+        st.fail = CUHRE_ERR_NONFINITE;
+        ctl = 1;
+      } else if (st.error <= fmax(bp.epsrel * fabs(st.integral), bp.epsabs)) {
+        st.done = 1;
+        st.fail = 0;
+        ctl = 1;
+      } else if (nact == 0 || st.neval + 2 * nact * p > bp.maxeval || st.iters >= 1000) {
+        st.done = 1;
+        st.fail = 1;
+        ctl = 1;
+      } else if (2 * nact > Q) {  // hand over the list of this iteration, before its step
+        st = S.before;
+        ctl = 2;
+      }
+      S.n = 2 * nact;
+      S.ctl = ctl;
+    }
+    __syncthreads();
+    const int ctl = __builtin_amdgcn_readfirstlane(S.ctl);
+    if (ctl == 1) {
+      if (tid == 0) {
+        S.st.count = 0;
+        A.state[i] = S.st;
+        A.stage[i] = -2;
+      }
+      __syncthreads();
+      return;
+    }
+    if (ctl == 2) break;
+    n = S.n;
+    cur ^= 1;
+  }
+  // hand-over: one atomic reservation of n regions in the staging area
+  if (tid == 0) {
+    const long long pos = (long long)atomicAdd(A.stage_off, (unsigned long long)n);
+    S.pos = pos + n <= A.stage_cap ? pos : -1;
+  }
+  __syncthreads();
+  const long long pos = S.pos;
+  if (pos >= 0) {
+    const double* const lo = cur ? lo1 : lo0;
+    const double* const wd = cur ? wd1 : wd0;
+    for (int64_t r = tid; r < n; r += kBlock)
+      for (int m = 0; m < D; ++m) {
+        A.lo[1][m * ld + A.stage0 + pos + r] = lo[m * ld + r];
+        A.wd[1][m * ld + A.stage0 + pos + r] = wd[m * ld + r];
+      }
+  }
+  if (tid == 0) {
+    if (pos < 0) memset(&S.st, 0, sizeof S.st);  // restarts from its initial grid
+    S.st.count = pos >= 0 ? n : 0;
+    A.state[i] = S.st;
+    A.stage[i] = pos >= 0 ? pos : -1;
+  }
+  __syncthreads();
+}
+
+// One persistent kernel per family: the family's instances are the range
+// [first, last) of the family-sorted list order, taken through its own
+// queue head.
+template <int D, int FAM>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_batch_persist(PersistArgs A, RuleConst R)
+{
+  __shared__ PersistShared S;
+  for (;;) {
+    if (threadIdx.x == 0) S.inst = A.first + atomicAdd(A.head, 1);
+    __syncthreads();
+    const int i = __builtin_amdgcn_readfirstlane(S.inst);
+    __syncthreads();  // read before thread 0 takes the next one
+    if (i >= A.last) return;
+    persist_instance<D, FAM>(A, R, A.params[i], i, S);
+  }
+}
+
+// After the persistent launch: the instances still running, in instance
+// order, into list 0 at their new starts (their lists from the staging
+// area, or their initial grids), one block per instance.
+__global__ __launch_bounds__(kBlock) void k_batch_gather(BatchBuffers B, const int64_t* stage, int64_t stage0,
+                                                        int k)
+{
+#pragma clang fp contract(off)  // low + id * w rounded twice, as the oracle computes it
+  const int i = blockIdx.x;
+  const InstState& st = B.state[i];
+  if (st.done) return;
+  const int64_t s0 = st.start, c = st.count;
+  const BatchParams& bp = B.params[i];
+  for (int64_t r = threadIdx.x; r < c; r += kBlock) {
+    if (stage[i] >= 0) {
+      for (int m = 0; m < B.dim; ++m) {
+        B.lower[m * B.ld + s0 + r] = B.out_lower[m * B.ld + stage0 + stage[i] + r];
+        B.width[m * B.ld + s0 + r] = B.out_width[m * B.ld + stage0 + stage[i] + r];
+      }
+    } else {
+      int64_t q = r;
+      for (int m = 0; m < B.dim; ++m) {
+        const int64_t id = q % k;
+        q /= k;
+        const double w = (bp.highs[m] - bp.lows[m]) / (double)k;
+        B.lower[m * B.ld + s0 + r] = bp.lows[m] + (double)id * w;
+        B.width[m * B.ld + s0 + r] = w;
+      }
+    }
+    B.inst[s0 + r] = i;
+  }
+}
+
+using PersistLauncher = void (*)(const PersistArgs&, const RuleConst&, int, hipStream_t);
+
+template <int D, int FAM>
+static void launch_persist(const PersistArgs& A, const RuleConst& R, int grid, hipStream_t st)
+{
+  hipLaunchKernelGGL((k_batch_persist<D, FAM>), dim3(grid), dim3(kBlock), 0, st, A, R);
+}
+
+// The family's persistent kernel at D and its resident blocks per CU.
+template <int D>
+static PersistLauncher persist_launcher_d(int family, int* per_cu)
+{
+  PersistLauncher l = nullptr;
+  const void* k = nullptr;
+  switch (family) {
+  case CUHRE_GAUSSIAN: l = &launch_persist<D, CUHRE_GAUSSIAN>; k = (const void*)k_batch_persist<D, CUHRE_GAUSSIAN>; break;
+  case CUHRE_PRODUCT_PEAK:
+    l = &launch_persist<D, CUHRE_PRODUCT_PEAK>;
+    k = (const void*)k_batch_persist<D, CUHRE_PRODUCT_PEAK>;
+    break;
+  case CUHRE_OSCILLATORY:
+    l = &launch_persist<D, CUHRE_OSCILLATORY>;
+    k = (const void*)k_batch_persist<D, CUHRE_OSCILLATORY>;
+    break;
+  case CUHRE_CORNER_PEAK:
+    l = &launch_persist<D, CUHRE_CORNER_PEAK>;
+    k = (const void*)k_batch_persist<D, CUHRE_CORNER_PEAK>;
+    break;
+  case CUHRE_CONTINUOUS:
+    l = &launch_persist<D, CUHRE_CONTINUOUS>;
+    k = (const void*)k_batch_persist<D, CUHRE_CONTINUOUS>;
+    break;
+  default: return nullptr;
+  }
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, kBlock, 0) != hipSuccess) return nullptr;
+  *per_cu = nb;
+  return l;
+}
+
+static PersistLauncher persist_launcher(int dim, int family, int* per_cu)
+{
+  switch (dim) {
+  case 2: return persist_launcher_d<2>(family, per_cu);
+  case 3: return persist_launcher_d<3>(family, per_cu);
+  case 4: return persist_launcher_d<4>(family, per_cu);
+  case 5: return persist_launcher_d<5>(family, per_cu);
+  case 6: return persist_launcher_d<6>(family, per_cu);
+  case 7: return persist_launcher_d<7>(family, per_cu);
+  case 8: return persist_launcher_d<8>(family, per_cu);
+  case 9: return persist_launcher_d<9>(family, per_cu);
+  case 10: return persist_launcher_d<10>(family, per_cu);
+  default: return nullptr;
+  }
 }
 
 static BatchBuffers make_buffers(BatchEngine* e, int parity, int count, int dim)
@@ -1071,6 +1463,150 @@ static bool capture_graph(BatchEngine* e, int count, int dim, const RuleConst& R
   return ok;
 }
 
+// Families present in a batch (bit f for family f).
+static unsigned batch_family_mask(int count, const cuhre_integrand* fs)
+{
+  unsigned m = 0;
+  for (int i = 0; i < count; ++i) m |= 1u << fs[i].family;
+  return m;
+}
+
+// Resident workgroups of the families' persistent kernels (the largest)
+// and the list length their queues and staging area want: 2 x kPersistQ
+// regions per workgroup.
+constexpr int64_t kPersistQ = 16384;
+
+static int persist_grid(int device, int dim, unsigned fam_mask)
+{
+  int ncu = 0, most = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
+  for (int f = 0; f < 32; ++f) {
+    if (!(fam_mask & (1u << f))) continue;
+    int per_cu = 0;
+    if (!persist_launcher(dim, f, &per_cu)) return 0;
+    most = std::max(most, ncu * std::max(per_cu, 1));
+  }
+  return most;
+}
+
+static int64_t persist_regions(int device, int dim, unsigned fam_mask)
+{
+  return 2 * kPersistQ * persist_grid(device, dim, fam_mask);
+}
+
+// The persistent launches (one per family), then the instances they handed
+// over gathered into list 0 in instance order with their states and chunk
+// map, ready for the replayed iterations (an empty list when every instance
+// finished).
+static int run_persistent(BatchEngine* e, hipStream_t st, int device, int count, int dim, const RuleConst& R,
+                          int* finished)
+{
+  const int gmax = std::min(persist_grid(device, dim, e->fam_mask), count);
+  if (gmax <= 0) return 0;
+  const int64_t Q = std::min<int64_t>(kPersistQ, (e->cap / 2 / gmax) & ~(int64_t)(kBlock - 1));
+  if (Q < 1024) return 0;  // lists too short for useful queues: replayed iterations only
+  if (e->p_count < count) {
+    if (e->p_head) (void)hipFree(e->p_head);
+    if (e->p_stage) (void)hipFree(e->p_stage);
+    e->p_head = nullptr;
+    e->p_stage = nullptr;
+    e->p_count = 0;
+    if (hipMalloc(&e->p_head, 16 * sizeof(int64_t)) != hipSuccess ||
+        hipMalloc(&e->p_stage, sizeof(int64_t) * count) != hipSuccess) {
+      (void)hipGetLastError();
+      return set_error(CUHRE_ERR_ALLOC, "cannot allocate the persistent queue state");
+    }
+    e->p_count = count;
+  }
+  if (hipMemsetAsync(e->p_head, 0, 16 * sizeof(int64_t), st) != hipSuccess)
+    return set_error(CUHRE_ERR_DEVICE, "persistent queue reset failed");
+  PersistArgs A;
+  A.params = e->params;
+  A.state = e->state;
+  for (int q = 0; q < 2; ++q) {
+    A.lo[q] = e->lo[q];
+    A.wd[q] = e->wd[q];
+  }
+  A.est = e->est;
+  A.err = e->err;
+  A.abssum = e->abssum;
+  A.vrel = e->vrel;
+  A.axis = e->axis;
+  A.stage_off = reinterpret_cast<unsigned long long*>(e->p_head + 8);
+  A.stage = e->p_stage;
+  A.ld = e->cap;
+  A.Q = Q;
+  A.stage0 = (int64_t)gmax * Q;
+  A.stage_cap = e->cap - A.stage0;
+  A.npts = gm_points(dim);
+  A.k0 = grid_k_batch(dim);
+  // each family's instances are a contiguous range of the family-sorted order
+  for (int k = 0; k < count;) {
+    const int f = e->h_params[k].family;
+    int j = k;
+    while (j < count && e->h_params[j].family == f) ++j;
+    int per_cu = 0, ncu = 0;
+    const PersistLauncher launch = persist_launcher(dim, f, &per_cu);
+    if (!launch || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+      return set_error(CUHRE_ERR_DEVICE, "no persistent kernel for a family");
+    A.head = reinterpret_cast<int32_t*>(e->p_head + f);
+    A.first = k;
+    A.last = j;
+    launch(A, R, std::min(std::min(ncu * std::max(per_cu, 1), gmax), j - k), st);
+    k = j;
+  }
+  if (hipGetLastError() != hipSuccess) return set_error(CUHRE_ERR_DEVICE, "persistent launch failed");
+  e->h_state.resize(count);
+  e->h_stage.resize(count);
+  if (hipMemcpyAsync(e->h_state.data(), e->state, sizeof(InstState) * count, hipMemcpyDeviceToHost, st) !=
+          hipSuccess ||
+      hipMemcpyAsync(e->h_stage.data(), e->p_stage, sizeof(int64_t) * count, hipMemcpyDeviceToHost, st) !=
+          hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return set_error(CUHRE_ERR_DEVICE, "persistent result copy failed");
+  // the handed-over instances' slices of list 0 in instance order
+  int64_t n0 = 1;
+  for (int m = 0; m < dim; ++m) n0 *= A.k0;
+  e->h_chunk0.assign(count + 1, 0);
+  int64_t run = 0, nch = 0;
+  int done = 0;
+  for (int i = 0; i < count; ++i) {
+    InstState& s = e->h_state[i];
+    e->h_chunk0[i] = nch;
+    if (s.done) {
+      ++done;
+      s.count = 0;
+      continue;
+    }
+    s.start = run;
+    s.count = e->h_stage[i] >= 0 ? s.count : n0;
+    run += s.count;
+    nch += (s.count + kChunk - 1) / kChunk;
+  }
+  e->h_chunk0[count] = nch;
+  *finished = done;
+  if (run > e->cap) return set_error(CUHRE_ERR_CAPACITY, "handed-over lists exceed the list buffers");
+  e->h_n[0] = run;
+  e->h_n[1] = 0;
+  e->h_n[2] = run;
+  e->h_n[3] = 0;
+  e->h_n[4] = 0;
+  e->h_n[5] = nch;
+  if (hipMemcpyAsync(e->state, e->h_state.data(), sizeof(InstState) * count, hipMemcpyHostToDevice, st) !=
+          hipSuccess ||
+      hipMemcpyAsync(e->chunk0, e->h_chunk0.data(), sizeof(int64_t) * (count + 1), hipMemcpyHostToDevice, st) !=
+          hipSuccess ||
+      hipMemcpyAsync(e->n, e->h_n, 6 * sizeof(int64_t), hipMemcpyHostToDevice, st) != hipSuccess)
+    return set_error(CUHRE_ERR_DEVICE, "persistent hand-over upload failed");
+  if (run > 0) {
+    const BatchBuffers B = make_buffers(e, 0, count, dim);
+    hipLaunchKernelGGL(k_batch_gather, dim3((unsigned)count), dim3(kBlock), 0, st, B, (const int64_t*)e->p_stage,
+                       A.stage0, A.k0);
+    if (hipGetLastError() != hipSuccess) return set_error(CUHRE_ERR_DEVICE, "hand-over gather launch failed");
+  }
+  return 0;
+}
+
 }  // namespace cuhre
 
 using namespace cuhre;
@@ -1144,7 +1680,7 @@ int cuhre_batch_autotune(cuhre_context* ctx, int32_t count, const cuhre_integran
   best->grid_blocks = bg;
   best->iterations_per_check = 2;
   best->use_graph = 1;
-  best->pad = 0;
+  best->persistent = 0;
   best->initial_capacity = 0;
   best->finish_budget = 0.0;
   e->tuned_dim = dim;
@@ -1180,20 +1716,27 @@ int cuhre_integrate_batch(cuhre_context* ctx, int32_t count, const cuhre_integra
   const double allot = cfg ? cfg->finish_budget : 0.0;
   if (!(allot >= 0.0) || allot >= 1.0) return set_error(CUHRE_ERR_ARG, "batch finish_budget must be in [0, 1)");
   const bool use_graph = !cfg || cfg->use_graph != 0;
+  const bool persistent = cfg && cfg->persistent != 0;
   // list capacity: an explicit engine capacity is a hard limit; otherwise the
   // list starts small and grows, up to three quarters of the device memory
   const int64_t limit = capacity > 0 ? capacity : memory_cap(e, dim);
   int64_t n0 = 0;
   const int64_t first = cfg && cfg->initial_capacity > 0 ? cfg->initial_capacity : (1 << 22);
   if (cfg && cfg->initial_capacity > 0 && e->cap > first) free_batch(e);  // honour a smaller start
+  // the persistent form's queues (2 x 16384 regions per resident workgroup)
+  // and its staging area share the list buffers
+  const int64_t want = persistent ? std::max<int64_t>(first, persist_regions(device, dim, batch_family_mask(count, fs))) : first;
   rc = batch_setup(e, st, count, fs, dim, epsrel, epsabs, maxeval, lows, highs,
-                   std::min<int64_t>(limit, std::max<int64_t>(e->cap, first)), &n0, allot);
+                   std::min<int64_t>(limit, std::max<int64_t>(e->cap, want)), &n0, allot);
   if (rc) return rc;
   e->allow_cut = e->cap >= limit ? 1 : 0;
   const RuleConst R = make_rule(dim, 7);
+  int persist_finished = -1;
+  if (persistent && (rc = run_persistent(e, st, device, count, dim, R, &persist_finished))) return rc;
   bool graph_ok = use_graph && capture_graph(e, count, dim, R, S, G, st);
   int iters = 0, grows = 0;
-  for (int check = 0; check < 2000; ++check) {
+  const bool all_done = persist_finished >= 0 && e->h_n[0] == 0;  // nothing handed over
+  for (int check = 0; check < 2000 && !all_done; ++check) {
     for (int k = 0; k < per_check; ++k) {
       if (graph_ok) {
         if (hipGraphLaunch(e->gexec, st) != hipSuccess) return set_error(CUHRE_ERR_DEVICE, "graph launch failed");
@@ -1263,6 +1806,7 @@ int cuhre_integrate_batch(cuhre_context* ctx, int32_t count, const cuhre_integra
     info->total_neval = total_regions * p;
     info->not_converged = nonconv;
     info->ms_total = ms;
+    info->persistent_finished = persist_finished;
     info->graph_used = graph_ok ? 1 : 0;
     info->point_segments = S;
     info->grid_blocks = G;

@@ -161,6 +161,95 @@ __global__ __launch_bounds__(kBlock) void k_split(SplitArgs S)
   }
 }
 
+// The split with the iteration's finalisation folded in (one launch fewer
+// per iteration).  part[0..np) are the evaluation's block partials
+// (one level) or, after k_finish_groups, the sums of its groups of 64 blocks
+// (two levels), np <= kWaveFinalize.  Every block scans the np active
+// counts into LDS for its own offsets; block 0 also reduces the partials in
+// full, writes the global offsets (one level) or group bases (two levels)
+// for a re-run of the split, and writes and publishes the iteration state.
+// Blocks skip the split when a count alone stops the loop (non-finite
+// values, the budget, the buffers); when the global test is met the
+// children written are unused.
+__global__ __launch_bounds__(kBlock) void k_split_fin(SplitArgs S, EvalArgs A, const Partial* part, uint32_t np,
+                                                      int two_level)
+{
+  __shared__ int64_t s_base[kWaveFinalize];
+  __shared__ int64_t s_wsum[kBlock / kWave], s_wbad[kBlock / kWave];
+  __shared__ uint32_t wcnt[kBlock / kWave];
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  // exclusive scan of the active counts (np <= 256 = one per thread)
+  const int64_t mine = threadIdx.x < np ? part[threadIdx.x].nactive : 0;
+  const int64_t bad = threadIdx.x < np ? part[threadIdx.x].nnonfinite : 0;
+  long long incl = (long long)mine, nbad = (long long)bad;
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const long long v = __shfl_up(incl, off, kWave);
+    if (lane >= off) incl += v;
+  }
+#pragma unroll
+  for (int mask = 1; mask < kWave; mask <<= 1) nbad += __shfl_xor(nbad, mask, kWave);
+  if (lane == kWave - 1) s_wsum[wave] = (int64_t)incl;
+  if (lane == 0) s_wbad[wave] = (int64_t)nbad;
+  __syncthreads();
+  int64_t before = 0, nact = 0, nonfinite = 0;
+  for (int w = 0; w < kBlock / kWave; ++w) {
+    if (w < wave) before += s_wsum[w];
+    nact += s_wsum[w];
+    nonfinite += s_wbad[w];
+  }
+  if (threadIdx.x < np) s_base[threadIdx.x] = before + (int64_t)incl - mine;
+  if (blockIdx.x == 0) {
+    Partial q;
+    reduce_partials(part, np, two_level ? A.gbase : A.offsets, q);
+    if (threadIdx.x == 0) write_iteration_state(A, q);
+  }
+  __syncthreads();
+  if (nonfinite != 0 || nact > A.max_active || nact > A.alloc_active) return;
+
+  const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const bool valid = r < S.n;
+  bool act = false;
+  if (valid) {
+    const double e = S.est[r], er = S.err[r];
+    act = isfinite(e) && isfinite(er) && region_active(e, er, S.epsrel, S.epsabs);
+  }
+  const uint64_t mask = __ballot(act);
+  const uint64_t below = lane ? (~0ull >> (kWave - lane)) : 0ull;
+  int64_t rank;
+  if (S.rpb >= kWave) {
+    if (lane == 0) wcnt[wave] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    const int wpb = S.rpb / kWave;
+    const int first = wave - wave % wpb;
+    rank = __popcll(mask & below);
+    for (int w = first; w < wave; ++w) rank += wcnt[w];
+  } else {
+    const int g0 = lane - lane % S.rpb;
+    const uint64_t grp = (S.rpb == kWave ? ~0ull : ((1ull << S.rpb) - 1ull)) << g0;
+    rank = __popcll(mask & grp & below);
+  }
+  if (!act) return;
+  const int64_t blk = r / S.rpb;
+  const int64_t dst = (two_level ? s_base[blk / kWave] + S.offsets[blk] : s_base[blk]) + rank;
+  const int a = S.axis[r];
+  for (int m = 0; m < S.dim; ++m) {
+    const double lo = S.lower[m * S.ld + r];
+    const double w = S.width[m * S.ld + r];
+    double2 L, W;
+    if (m == a) {
+      const double hw = 0.5 * w;
+      L = make_double2(lo, lo + hw);
+      W = make_double2(hw, hw);
+    } else {
+      L = make_double2(lo, lo);
+      W = make_double2(w, w);
+    }
+    *reinterpret_cast<double2*>(S.out_lower + m * S.out_ld + 2 * dst) = L;
+    *reinterpret_cast<double2*>(S.out_width + m * S.out_ld + 2 * dst) = W;
+  }
+}
+
 void launch_split(const SplitArgs& S, hipStream_t st)
 {
   hipLaunchKernelGGL(k_split, dim3((unsigned)((S.n + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, S);
@@ -843,8 +932,19 @@ int plain_loop(cuhre_context* c, const cuhre_integrand* f, int dim, double epsre
     }
     launch(A, P, R, S, c->stream);
     A.prof_start = A.prof_stop = nullptr;
-    launch_finish(A, nb, c->stream);
-    hipLaunchKernelGGL(k_split, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, c->stream, SA);
+    const dim3 sgrid((unsigned)((n + kBlock - 1) / kBlock));
+    const int64_t G = (nb + kWave - 1) / kWave;
+    if (nb <= (int64_t)kWaveFinalize) {
+      hipLaunchKernelGGL(k_split_fin, sgrid, dim3(kBlock), 0, c->stream, SA, A, (const Partial*)A.partials,
+                         (uint32_t)nb, 0);
+    } else if (G <= (int64_t)kWaveFinalize) {
+      hipLaunchKernelGGL(k_finish_groups, dim3((unsigned)G), dim3(kWave), 0, c->stream, A, (uint32_t)nb);
+      hipLaunchKernelGGL(k_split_fin, sgrid, dim3(kBlock), 0, c->stream, SA, A, (const Partial*)A.gpart,
+                         (uint32_t)G, 1);
+    } else {
+      launch_finish(A, nb, c->stream);
+      hipLaunchKernelGGL(k_split, sgrid, dim3(kBlock), 0, c->stream, SA);
+    }
     if (c->profile) HIP_TRY(hipEventRecord(c->ev[2], c->stream));
     HIP_TRY(hipGetLastError());
     IterState st;

@@ -90,7 +90,7 @@ class BatchConfig(C.Structure):
         ("grid_blocks", C.c_int32),
         ("iterations_per_check", C.c_int32),
         ("use_graph", C.c_int32),
-        ("pad", C.c_int32),
+        ("persistent", C.c_int32),
         ("initial_capacity", C.c_int64),
         ("finish_budget", C.c_double),
     ]
@@ -113,7 +113,7 @@ class BatchInfo(C.Structure):
         ("peak_list", C.c_int64),
         ("capacity", C.c_int64),
         ("grows", C.c_int32),
-        ("pad", C.c_int32),
+        ("persistent_finished", C.c_int32),
     ]
 
     def asdict(self):
@@ -407,17 +407,20 @@ class Context:
         return count, d, arr, lows, highs, keep
 
     def integrate_batch(self, problems, point_segments=0, grid_blocks=0, iterations_per_check=0,
-                        use_graph=True, initial_capacity=0, raise_nonfinite=True, finish_budget=0.0):
+                        use_graph=True, initial_capacity=0, raise_nonfinite=True, finish_budget=0.0,
+                        persistent=False):
         """Integrates every problem (same dimension) in one batched call.
         Returns (results list in input order, info dict, return code).  With
         raise_nonfinite=False a non-finite integrand value returns rc -2 with
-        the per-instance results (that instance's fail is -2)."""
+        the per-instance results (that instance's fail is -2).  persistent:
+        one persistent launch runs the instances' loops in per-workgroup
+        region queues first."""
         count, d, arr, lows, highs, keep = self._batch_inputs(problems)
         er = np.ascontiguousarray([p.epsrel for p in problems], dtype=np.float64)
         ea = np.ascontiguousarray([p.epsabs for p in problems], dtype=np.float64)
         me = np.ascontiguousarray([p.maxeval for p in problems], dtype=np.int64)
-        cfg = BatchConfig(point_segments, grid_blocks, iterations_per_check, int(bool(use_graph)), 0,
-                          initial_capacity, finish_budget)
+        cfg = BatchConfig(point_segments, grid_blocks, iterations_per_check, int(bool(use_graph)),
+                          int(bool(persistent)), initial_capacity, finish_budget)
         out = (Result * count)()
         info = BatchInfo()
         rc = lib().cuhre_integrate_batch(self.h, count, arr, d, _dp(er), _dp(ea),

@@ -178,3 +178,62 @@ def test_batch_allotment_on_the_4096_integrals(ctx):
     # measured: 52 (plain) and 56 (allotment) of 4096 certified results lie
     # beyond epsrel of the closed form
     assert am[1] <= pm[1] + max(8, pm[1] // 4)
+
+
+# ------------------------------------------------------------ persistent form
+# One persistent launch in which each workgroup takes instances from a queue
+# and runs each one's loop in its own region queue; an instance that
+# outgrows its queue is handed over (its list and sums) to the replayed
+# iterations.  Per instance the result equals the replayed form's: same
+# region counts, iterations and flags, sums within rounding.
+
+
+@pytest.mark.parametrize("budget", [0.0, 0.25])
+def test_persistent_matches_replayed_iterations(ctx, budget):
+    probs = W.c4_batch(seed=31, count=96)
+    ref, rinfo, rrc = ctx.integrate_batch(probs, finish_budget=budget)
+    res, info, rc = ctx.integrate_batch(probs, finish_budget=budget, persistent=True)
+    assert rinfo["persistent_finished"] == -1
+    # most instances finish inside the persistent launch, the largest are
+    # handed over (their lists outgrow the 16384-region queues)
+    print("persistent_finished", info["persistent_finished"], "of", len(probs))
+    assert 0 < info["persistent_finished"] < len(probs)
+    assert rc == rrc and info["not_converged"] == rinfo["not_converged"]
+    assert info["total_regions"] == rinfo["total_regions"]
+    for i, (a, b) in enumerate(zip(res, ref)):
+        _same(a, b, ("persistent", budget, i))
+
+
+def test_persistent_only_small_instances_and_oracle(ctx):
+    """Instances small enough to finish inside the persistent launch (no
+    hand-over): the plain method per instance and the oracle."""
+    pool = [W.Problem(p.name, p.f, p.lows, p.highs, 1e-4, 0.0, p.maxeval) for p in W.c4_batch(seed=32, count=64)]
+    # lists that stay well inside a 16384-region queue
+    probs = [p for p in pool if ctx.integrate(p)["peak_regions"] <= 4096][:24]
+    assert len(probs) >= 12
+    res, info, rc = ctx.integrate_batch(probs, persistent=True)
+    assert info["persistent_finished"] == len(probs)
+    for i, (p, b) in enumerate(zip(probs, res)):
+        g = ctx.integrate(p)
+        _same(b, g, ("plain", i))
+        o = O.integrate_bfs(p, nthreads=8)
+        assert b["nregions"] == o["nregions"] and b["iterations"] == o["iterations"], (i, b, o)
+        assert abs(b["integral"] - o["integral"]) <= 1e-12 * abs(o["integral"])
+
+
+def test_persistent_nonfinite_and_small_lists(ctx):
+    bad = W.c4_batch(seed=33, count=3) + [
+        W.Problem("bad", W.corner_peak([-1.0] * 5, 700), [0] * 5, [1] * 5, 1e-5, 0, 10**8)]
+    res, info, rc = ctx.integrate_batch(bad, raise_nonfinite=False, persistent=True)
+    assert rc == cuhre.ERR_NONFINITE and res[3]["fail"] == cuhre.ERR_NONFINITE
+    for i in range(3):
+        pl = ctx.integrate(bad[i])
+        _same(res[i], pl, ("nonfinite batch", i))
+    # lists too short for useful queues: the replayed iterations alone
+    probs = W.c4_batch(seed=34, count=16)
+    with cuhre.Context(0, capacity=8192) as small:
+        a, ia, _ = small.integrate_batch(probs, persistent=True)
+        b, ib, _ = small.integrate_batch(probs)
+    assert ia["persistent_finished"] == -1
+    for x, y in zip(a, b):
+        assert x["nregions"] == y["nregions"] and x["integral"] == y["integral"]
