@@ -460,6 +460,17 @@ def main():
     neval = sum(r["neval"] for step in runs for r in step)
     value = neval / dt
 
+    # the same steps with the loop queueing one iteration at a time (not
+    # the metric: the loop above queues each iteration before the host has
+    # read the previous state; results are identical)
+    ctx.queue_ahead(False)
+    t1 = time.perf_counter()
+    serial = [[ctx.integrate(p) for p in problems] for _ in range(args.steps)]
+    dt_serial = time.perf_counter() - t1
+    ctx.queue_ahead(True)
+    same_serial = all(a["integral"] == b["integral"] and a["nregions"] == b["nregions"]
+                      for sa, sb in zip(runs, serial) for a, b in zip(sa, sb))
+
     # kernel timing with HIP events on the engine's stream over the same steps
     ctx.set(True, 0)
     prof = [[ctx.integrate(p) for p in problems] for _ in range(args.steps)]
@@ -525,6 +536,9 @@ def main():
                      "largest_launch": {"regions": big["largest_regions"], "ms": big["ms_eval_largest"],
                                         "tflops": big_tf, "frac": big_tf / FP64_PEAK_TFLOPS}},
         "per_integrand": per,
+        "queue_ahead": {"ms_per_step": dt / args.steps * 1e3,
+                        "ms_per_step_one_iteration_at_a_time": dt_serial / args.steps * 1e3,
+                        "speedup": dt_serial / dt, "results_identical": same_serial},
     }
     if args.converge:
         p = W.c2_product_peak8()

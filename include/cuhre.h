@@ -128,6 +128,16 @@ int cuhre_context_destroy(cuhre_context* ctx);
  * automatic choice for the shortest lists). */
 int cuhre_context_set(cuhre_context* ctx, int profile, int point_segments);
 
+/* Plain-loop scheduling (no counterpart in the reference, whose host loop
+ * reads every iteration's sums before the next launch).  on != 0, the
+ * default: each iteration's evaluation and split enter the stream before
+ * the host has read the previous iteration's state, their grids sized for
+ * the largest list that iteration can have and the real length read on the
+ * device, so the GPU does not wait on the host between iterations; 0: one
+ * iteration at a time.  Results are identical either way.  Profiling
+ * (cuhre_context_set) runs one iteration at a time. */
+int cuhre_context_queue_ahead(cuhre_context* ctx, int on);
+
 /* Plain adaptive integration over the box [lows, highs].  key selects the
  * rule: 0 or 7 = Genz-Malik degree 7/5 (err = |I7 - I5|); 9 = the
  * reference's degree-9 rule with four null rules (quad/GPUquad/GPUQuadRule.cu:30-226,

@@ -99,10 +99,31 @@ struct EvalArgs {
   uint32_t seq;        // published with the state when host_state is set
   IterState* host_state;  // optional: coherent host copy of *state
   uint32_t* host_seq;     // written (system-scope release) after host_state
+  const int64_t* n_dev;   // when set: the list length, read on the device (n is then
+                          // an upper bound: the launch of a queued-ahead iteration)
   // host side only: when set, the evaluation kernel's own start and end are
   // recorded in these events (hipExtLaunchKernel), so a timed launch does
   // not include the gap before it reaches the GPU
   hipEvent_t prof_start, prof_stop;
+};
+
+// Device-side loop state of a list parity when iterations are queued ahead
+// of the host: the list's length and the finished-region sums before it.
+struct SpecSlot {
+  int64_t n, pad;
+  double fin_s, fin_c, fer_s, fer_c;  // est and err of the finished regions (value, carry)
+};
+
+// What the split of a queued-ahead iteration needs on the device: its
+// list's slot, the next list's slot, the evaluation budget left before the
+// iteration (maxeval minus the evaluations of earlier iterations) and the
+// regions per evaluation block (for the number of block partials).
+struct SpecArgs {
+  const SpecSlot* in;  // nullptr: not queued ahead
+  SpecSlot* out;
+  int64_t left0, cap2;  // budget before the iteration; capacity / 2
+  int64_t npts;
+  int32_t rpb, pad;
 };
 
 // FP64 rounding bound of a result that sums n regions of a p-point rule in

@@ -61,12 +61,18 @@ struct cuhre_context {
   cuhre::IterState* state = nullptr;
   cuhre::IterState* h_state = nullptr;
   // coherent host memory the finalisation kernel publishes the state to
-  // (host and device views), with a sequence number written last
+  // (host and device views; two buffers, the launch numbered seq uses
+  // buffer seq & 1), with the sequence number written last
   cuhre::IterState* pub_state = nullptr;
   uint32_t* pub_seq = nullptr;
   cuhre::IterState* pub_state_dev = nullptr;
   uint32_t* pub_seq_dev = nullptr;
   uint32_t seq = 0;
+  // plain loop: queue each iteration before the previous state is read,
+  // with the list lengths and finished sums in these device slots (one per
+  // list buffer)
+  int queue_ahead = 1;
+  cuhre::SpecSlot* spec = nullptr;
   double* aux = nullptr;  // i5 (cap) + dvec (cap*dim) on request
   int64_t aux_cap = 0;
   char* p2_ws = nullptr;  // Phase-2 workspace of the two-phase method, kept between calls

@@ -171,13 +171,36 @@ __global__ __launch_bounds__(kBlock) void k_split(SplitArgs S)
 // Blocks skip the split when a count alone stops the loop (non-finite
 // values, the budget, the buffers); when the global test is met the
 // children written are unused.
+// Queued ahead of the host (P.in set): the list length and the finished
+// sums come from the device slot P.in, the number of partials and the
+// levels of the finalisation follow from that length (the launch grids
+// were sized for an upper bound), the budget test uses it, and block 0
+// also writes the next list's slot: its length (0 when the loop stops, so
+// that an iteration queued behind this one does nothing) and the finished
+// sums after this iteration.
 __global__ __launch_bounds__(kBlock) void k_split_fin(SplitArgs S, EvalArgs A, const Partial* part, uint32_t np,
-                                                      int two_level)
+                                                      int two_level, SpecArgs P)
 {
   __shared__ int64_t s_base[kWaveFinalize];
   __shared__ int64_t s_wsum[kBlock / kWave], s_wbad[kBlock / kWave];
   __shared__ uint32_t wcnt[kBlock / kWave];
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  int64_t n = S.n;
+  if (P.in != nullptr) {
+    n = P.in->n;
+    if (n == 0) return;  // the loop stopped before this iteration
+    const int64_t nb = (n + P.rpb - 1) / P.rpb;
+    two_level = nb > (int64_t)kWaveFinalize;
+    part = two_level ? A.gpart : A.partials;
+    np = (uint32_t)(two_level ? (nb + kWave - 1) / kWave : nb);
+    A.fin_est_s = P.in->fin_s;
+    A.fin_est_c = P.in->fin_c;
+    A.fin_err_s = P.in->fer_s;
+    A.fin_err_c = P.in->fer_c;
+    const int64_t left = P.left0 - n * P.npts;
+    const int64_t budget = left >= 0 ? left / (2 * P.npts) : -1;
+    A.max_active = budget < P.cap2 ? budget : P.cap2;
+  }
   // exclusive scan of the active counts (np <= 256 = one per thread)
   const int64_t mine = threadIdx.x < np ? part[threadIdx.x].nactive : 0;
   const int64_t bad = threadIdx.x < np ? part[threadIdx.x].nnonfinite : 0;
@@ -202,13 +225,28 @@ __global__ __launch_bounds__(kBlock) void k_split_fin(SplitArgs S, EvalArgs A, c
   if (blockIdx.x == 0) {
     Partial q;
     reduce_partials(part, np, two_level ? A.gbase : A.offsets, q);
-    if (threadIdx.x == 0) write_iteration_state(A, q);
+    if (threadIdx.x == 0) {
+      const IterState st = iteration_state(A, q);
+      write_state(A, st);
+      if (P.out != nullptr) {
+        SpecSlot o;
+        o.n = st.stop == 0 ? 2 * q.nactive : 0;
+        o.pad = 0;
+        o.fin_s = A.fin_est_s;
+        o.fin_c = A.fin_est_c;
+        o.fer_s = A.fin_err_s;
+        o.fer_c = A.fin_err_c;
+        dd_add(o.fin_s, o.fin_c, q.fest_s, q.fest_c);
+        dd_add(o.fer_s, o.fer_c, q.ferr_s, q.ferr_c);
+        *P.out = o;
+      }
+    }
   }
   __syncthreads();
   if (nonfinite != 0 || nact > A.max_active || nact > A.alloc_active) return;
 
   const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  const bool valid = r < S.n;
+  const bool valid = r < n;
   bool act = false;
   if (valid) {
     const double e = S.est[r], er = S.err[r];
@@ -247,6 +285,19 @@ __global__ __launch_bounds__(kBlock) void k_split_fin(SplitArgs S, EvalArgs A, c
     }
     *reinterpret_cast<double2*>(S.out_lower + m * S.out_ld + 2 * dst) = L;
     *reinterpret_cast<double2*>(S.out_width + m * S.out_ld + 2 * dst) = W;
+  }
+}
+
+// The list length and finished sums of the first list of a loop whose
+// iterations are queued ahead (slot), or only the length (after the host
+// re-ran a split).
+__global__ void k_spec_set(SpecSlot* slot, int64_t n, int zero_sums)
+{
+  if (threadIdx.x != 0) return;
+  slot->n = n;
+  if (zero_sums) {
+    slot->fin_s = slot->fin_c = slot->fer_s = slot->fer_c = 0.0;
+    slot->pad = 0;
   }
 }
 
@@ -758,13 +809,15 @@ int cuhre_context_create(cuhre_context** out, int device, int64_t capacity_regio
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc(&c->state, sizeof(IterState)) != hipSuccess ||
       hipHostMalloc(&c->h_state, sizeof(IterState), hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc(&c->pub_state, sizeof(IterState), hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
+      hipMalloc(&c->spec, 2 * sizeof(SpecSlot)) != hipSuccess ||
+      hipHostMalloc(&c->pub_state, 2 * sizeof(IterState), hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
       hipHostMalloc(&c->pub_seq, sizeof(uint32_t), hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
       hipHostGetDevicePointer((void**)&c->pub_state_dev, c->pub_state, 0) != hipSuccess ||
       hipHostGetDevicePointer((void**)&c->pub_seq_dev, c->pub_seq, 0) != hipSuccess ||
       hipMalloc(&c->sum_out, sizeof(double)) != hipSuccess) {
     rc = fail(CUHRE_ERR_DEVICE, "context setup failed: %s", hipGetErrorString(hipGetLastError()));
   }
+  if (rc == 0) *c->pub_seq = 0;  // launches are numbered from 1
   for (int i = 0; i < 4 && rc == 0; ++i)
     if (hipEventCreate(&c->ev[i]) != hipSuccess) rc = fail(CUHRE_ERR_DEVICE, "event create failed");
   // the static workspace's address on this device (the no-allocation entry
@@ -791,6 +844,7 @@ int cuhre_context_destroy(cuhre_context* c)
   if (c->aux) (void)hipFree(c->aux);
   if (c->p2_ws) (void)hipFree(c->p2_ws);
   if (c->state) (void)hipFree(c->state);
+  if (c->spec) (void)hipFree(c->spec);
   if (c->h_state) (void)hipHostFree(c->h_state);
   if (c->pub_state) (void)hipHostFree(c->pub_state);
   if (c->pub_seq) (void)hipHostFree(c->pub_seq);
@@ -816,16 +870,25 @@ int cuhre_context_set(cuhre_context* c, int profile, int point_segments)
   return 0;
 }
 
+int cuhre_context_queue_ahead(cuhre_context* c, int on)
+{
+  if (!c) return fail(CUHRE_ERR_ARG, "null context");
+  c->queue_ahead = on != 0;
+  return 0;
+}
+
 namespace {
 
-// Waits for the evaluation launch numbered seq to publish its state (its
-// k_finish writes the state, then the number, to coherent host memory).
-// Meanwhile the split already queued behind it keeps the GPU busy.
+// Waits for the launch numbered seq to publish its state (its finalisation
+// writes the state to buffer seq & 1, then the number, to coherent host
+// memory).  Meanwhile the launches queued behind it keep the GPU busy; the
+// next one may already have published (number seq + 1, the other buffer).
 int wait_published(cuhre_context* c, uint32_t seq, IterState* out)
 {
+  const auto reached = [&] { return (int32_t)(__atomic_load_n(c->pub_seq, __ATOMIC_ACQUIRE) - seq) >= 0; };
   for (uint64_t spin = 0;; ++spin) {
-    if (__atomic_load_n(c->pub_seq, __ATOMIC_ACQUIRE) == seq) {
-      std::memcpy(out, (const void*)c->pub_state, sizeof *out);
+    if (reached()) {
+      std::memcpy(out, (const void*)(c->pub_state + (seq & 1)), sizeof *out);
       return 0;
     }
     __builtin_ia32_pause();
@@ -833,7 +896,7 @@ int wait_published(cuhre_context* c, uint32_t seq, IterState* out)
     // never publishes; asked every ~2^18 spins it stays off the hot path
     if ((spin & 262143) == 262143) {
       const hipError_t q = hipStreamQuery(c->stream);
-      if (q == hipSuccess && __atomic_load_n(c->pub_seq, __ATOMIC_ACQUIRE) != seq)
+      if (q == hipSuccess && !reached())
         return fail(CUHRE_ERR_DEVICE, "evaluation ended without publishing its state");
       if (q != hipSuccess && q != hipErrorNotReady)
         return fail(CUHRE_ERR_DEVICE, "HIP error: %s", hipGetErrorString(q));
@@ -897,64 +960,118 @@ int plain_loop(cuhre_context* c, const cuhre_integrand* f, int dim, double epsre
   double fin_s = 0.0, fin_c = 0.0, fer_s = 0.0, fer_c = 0.0, fin_abs = 0.0, fin_aest = 0.0;
   int64_t nfin = 0, n = n0;
   int result = CUHRE_NOT_CONVERGED;
-  for (int it = 0; it < opt.max_iter; ++it) {
-    const int S = pick_segments(c, n, key, f->family, dim);
-    const EvalLauncher launch = find_eval_launcher(f->family, dim, key);
+  const EvalLauncher launch = find_eval_launcher(f->family, dim, key);
+  // Queue-ahead: iteration it+1 (its evaluation, sized for the bound 2 n of
+  // its list, and its split) enters the stream before the host has read
+  // iteration it's state, so the GPU does not idle through the host's round
+  // trip.  Its list length and finished sums come from the device slot the
+  // split of iteration it writes (length 0 when the loop stops there).  The
+  // segment count of every launch follows from that bound (2 n of the
+  // previous list, the initial grid first) and the reductions from the real
+  // length, so results do not depend on which launches were queued ahead.
+  const bool ahead = c->queue_ahead && !c->profile && opt.handoff_n == 0;
+  struct Launched {
+    int S = 1;
+    int64_t nb = 0;  // evaluation blocks of the launch
+    uint32_t seq = 0;
+    bool slot = false;  // its split writes the next list's slot
+  };
+  const auto fused = [](int64_t nb) {
+    return nb <= (int64_t)kWaveFinalize || (nb + kWave - 1) / kWave <= (int64_t)kWaveFinalize;
+  };
+  // Queues the evaluation and split of list lc: nl regions (exact) or at
+  // most nl, the length then read on the device; bnd picks the segments;
+  // nev: evaluations of the iterations before.
+  const auto queue = [&](int lc, int64_t nl, int64_t bnd, bool exact, int64_t nev, Launched& L) -> int {
+    L.S = pick_segments(c, bnd, key, f->family, dim);
+    const int rpb = seg_regions_per_block(L.S);
+    L.nb = blocks_for(nl, L.S);
+    if (L.nb > 0x7fffffffll) return fail(CUHRE_ERR_CAPACITY, "grid too large");
     EvalArgs A;
-    fill_eval_args(c, A, cur, n, dim, epsrel, epsabs);
+    fill_eval_args(c, A, lc, nl, dim, epsrel, epsabs);
     A.finalize = 1;
-    A.fin_est_s = fin_s;
-    A.fin_est_c = fin_c;
-    A.fin_err_s = fer_s;
-    A.fin_err_c = fer_c;
-    const int64_t left = maxeval - (out->neval + n * p);
-    const int64_t budget_active = left >= 0 ? left / (2 * p) : -1;
-    A.max_active = std::min<int64_t>(budget_active, capacity / 2);
     A.alloc_active = c->cap / 2;
-    SplitArgs SA;
-    fill_split_args(c, SA, cur, n, dim, epsrel, epsabs, seg_regions_per_block(S), true);
-    const int64_t nb = blocks_for(n, S);
-    if (nb > (int64_t)kWaveFinalize) SA.gbase = A.gbase;
-    if (nb > 0x7fffffffll) return fail(CUHRE_ERR_CAPACITY, "grid too large");
-
-    // without profiling the state comes back through coherent host memory,
-    // so the next launches are queued while this split still runs
-    const bool publish = !c->profile;
-    if (publish) {
-      A.seq = ++c->seq;
-      A.host_state = c->pub_state_dev;
-      A.host_seq = c->pub_seq_dev;
+    SpecArgs Q;
+    std::memset(&Q, 0, sizeof Q);
+    if (exact) {
+      A.fin_est_s = fin_s;
+      A.fin_est_c = fin_c;
+      A.fin_err_s = fer_s;
+      A.fin_err_c = fer_c;
+      const int64_t left = maxeval - (nev + nl * p);
+      const int64_t budget_active = left >= 0 ? left / (2 * p) : -1;
+      A.max_active = std::min<int64_t>(budget_active, capacity / 2);
+    } else {
+      A.n_dev = &c->spec[lc].n;
+      Q.in = &c->spec[lc];
+      Q.left0 = maxeval - nev;
+      Q.cap2 = capacity / 2;
+      Q.npts = p;
     }
-    if (c->profile) {
+    L.slot = ahead && fused(L.nb);
+    if (L.slot) Q.out = &c->spec[1 - lc];
+    Q.rpb = rpb;
+    SplitArgs SA;
+    fill_split_args(c, SA, lc, nl, dim, epsrel, epsabs, rpb, true);
+    if (L.nb > (int64_t)kWaveFinalize) SA.gbase = A.gbase;
+    // without profiling the state comes back through coherent host memory
+    // (two buffers: the split queued behind may publish before the host reads)
+    if (!c->profile) {
+      L.seq = A.seq = ++c->seq;
+      A.host_state = c->pub_state_dev + (A.seq & 1);
+      A.host_seq = c->pub_seq_dev;
+    } else {
       // the kernel's own start and end (no launch gap in the interval)
       A.prof_start = c->ev[0];
       A.prof_stop = c->ev[1];
     }
-    launch(A, P, R, S, c->stream);
+    launch(A, P, R, L.S, c->stream);
     A.prof_start = A.prof_stop = nullptr;
-    const dim3 sgrid((unsigned)((n + kBlock - 1) / kBlock));
-    const int64_t G = (nb + kWave - 1) / kWave;
-    if (nb <= (int64_t)kWaveFinalize) {
+    const dim3 sgrid((unsigned)((nl + kBlock - 1) / kBlock));
+    const int64_t G = (L.nb + kWave - 1) / kWave;
+    if (L.nb <= (int64_t)kWaveFinalize) {
       hipLaunchKernelGGL(k_split_fin, sgrid, dim3(kBlock), 0, c->stream, SA, A, (const Partial*)A.partials,
-                         (uint32_t)nb, 0);
+                         (uint32_t)L.nb, 0, Q);
     } else if (G <= (int64_t)kWaveFinalize) {
-      hipLaunchKernelGGL(k_finish_groups, dim3((unsigned)G), dim3(kWave), 0, c->stream, A, (uint32_t)nb);
+      hipLaunchKernelGGL(k_finish_groups, dim3((unsigned)G), dim3(kWave), 0, c->stream, A, (uint32_t)L.nb);
       hipLaunchKernelGGL(k_split_fin, sgrid, dim3(kBlock), 0, c->stream, SA, A, (const Partial*)A.gpart,
-                         (uint32_t)G, 1);
+                         (uint32_t)G, 1, Q);
     } else {
-      launch_finish(A, nb, c->stream);
+      launch_finish(A, L.nb, c->stream);
       hipLaunchKernelGGL(k_split, sgrid, dim3(kBlock), 0, c->stream, SA);
     }
     if (c->profile) HIP_TRY(hipEventRecord(c->ev[2], c->stream));
     HIP_TRY(hipGetLastError());
+    return 0;
+  };
+
+  int64_t bound = n0;  // 2 n of the previous list (the initial grid first)
+  Launched L, next;
+  bool queued = false;  // iteration it is already in the stream (as next)
+  for (int it = 0; it < opt.max_iter; ++it) {
+    if (queued) L = next;
+    else if ((rc = queue(cur, n, bound, true, out->neval, L))) return rc;
+    queued = false;
+    if (L.slot && it + 1 < opt.max_iter) {
+      const int S2 = pick_segments(c, 2 * n, key, f->family, dim);
+      const int64_t nb2 = blocks_for(2 * n, S2);
+      if (fused(nb2) && nb2 <= block_capacity(c->cap)) {
+        if ((rc = queue(1 - cur, 2 * n, 2 * n, false, out->neval + n * p, next))) return rc;
+        queued = true;
+      }
+    }
     IterState st;
-    if (publish) {
-      if ((rc = wait_published(c, A.seq, &st))) return rc;
+    if (!c->profile) {
+      if ((rc = wait_published(c, L.seq, &st))) return rc;
     } else {
       HIP_TRY(hipMemcpyAsync(c->h_state, c->state, sizeof(IterState), hipMemcpyDeviceToHost, c->stream));
       HIP_TRY(hipStreamSynchronize(c->stream));
       st = *c->h_state;
     }
+    const int64_t left = maxeval - (out->neval + n * p);
+    const int64_t budget_active = left >= 0 ? left / (2 * p) : -1;
+    const int S = L.S;
+    const int64_t nb = blocks_for(n, S);
     if (c->profile) {
       float ms_e = 0.f, ms_s = 0.f;
       HIP_TRY(hipEventElapsedTime(&ms_e, c->ev[0], c->ev[1]));
@@ -976,10 +1093,13 @@ int plain_loop(cuhre_context* c, const cuhre_integrand* f, int dim, double epsre
     }
     if (st.stop == 5) {
       // list buffers too small for the children: grow (keeping this
-      // iteration's list and results) and run the split again
+      // iteration's list and results) and run the split again; an
+      // iteration queued behind this one read length 0 and did nothing
+      queued = false;
       const int64_t A2 = 2 * st.tot.nactive;
       const int64_t ncap = std::min<int64_t>(capacity, std::max<int64_t>(A2, 2 * c->cap));
       if ((rc = ctx_reserve(c, ncap, dim, n, cur, nb))) return rc;
+      SplitArgs SA;
       fill_split_args(c, SA, cur, n, dim, epsrel, epsabs, seg_regions_per_block(S), false);
       if (nb > (int64_t)kWaveFinalize) SA.gbase = c->offsets + block_capacity(c->cap);
       if (c->profile) HIP_TRY(hipEventRecord(c->ev[1], c->stream));
@@ -1033,6 +1153,7 @@ int plain_loop(cuhre_context* c, const cuhre_integrand* f, int dim, double epsre
       result = CUHRE_NOT_CONVERGED;
       break;
     }
+    bound = 2 * n;
     n = 2 * st.tot.nactive;
     cur = 1 - cur;
     if (opt.handoff_n > 0 && (n >= opt.handoff_n || it + 1 == opt.max_iter)) {

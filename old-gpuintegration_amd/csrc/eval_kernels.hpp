@@ -57,6 +57,25 @@ inline int seg_regions_per_block(int S)
   return S == kHalfS ? kBlock / kHalfS : kWave * (seg_waves(S) / S);
 }
 
+// The launch's list length: A.n, or the length on the device when the
+// iteration was queued ahead of the host (A.n is then an upper bound and
+// the blocks beyond the list store zero partials).
+__device__ __forceinline__ int64_t eval_len(const EvalArgs& A) { return A.n_dev != nullptr ? *A.n_dev : A.n; }
+
+// A block of a queued-ahead launch that lies wholly beyond the list (first:
+// its first region) stores a zero partial and exits; when the list is empty
+// (the loop stopped before this iteration) it writes nothing.
+__device__ __forceinline__ bool block_beyond(const EvalArgs& A, int64_t n, int64_t first)
+{
+  if (A.n_dev == nullptr || first < n) return false;
+  if (n > 0 && A.finalize && threadIdx.x == 0) {
+    Partial z;
+    partial_zero(z);
+    A.partials[blockIdx.x] = z;
+  }
+  return true;
+}
+
 // Integrand of the plain kernel: one family, parameters in kernel arguments.
 template <int D, int FAM>
 struct PlainIntegrand {
@@ -421,8 +440,10 @@ __global__ __launch_bounds__(1024) void k_eval_seg(EvalArgs A, DevIntegrand P, R
   const int lane = threadIdx.x & (kWave - 1);
   const int g = wave / S, s = wave - g * S;
   const int64_t r = (int64_t)blockIdx.x * (kWave * (W / S)) + g * kWave + lane;
-  const bool valid = r < A.n;
-  const int64_t rr = valid ? r : A.n - 1;
+  const int64_t n = eval_len(A);
+  if (block_beyond(A, n, (int64_t)blockIdx.x * (kWave * (W / S)))) return;
+  const bool valid = r < n;
+  const int64_t rr = valid ? r : (n > 0 ? n - 1 : 0);
   const PlainIntegrand<D, FAM> f{P};
   const RegionOut o = seg_eval<D>(f, A.lower, A.width, A.ld, r, valid, rr, S, s, wave, lane, W,
                                   seg_lds, R, A.dvec);
@@ -452,8 +473,10 @@ __global__ __launch_bounds__(kBlock) void k_eval_one(EvalArgs A, DevIntegrand P,
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  const bool valid = r < A.n;
-  const int64_t rr = valid ? r : A.n - 1;
+  const int64_t n = eval_len(A);
+  if (block_beyond(A, n, (int64_t)blockIdx.x * kBlock)) return;
+  const bool valid = r < n;
+  const int64_t rr = valid ? r : (n > 0 ? n - 1 : 0);
   const PlainIntegrand<D, FAM> f{P};
   const RegionOut o = seg_eval<D, false, true>(f, A.lower, A.width, A.ld, r, valid, rr, 1, 0, wave, lane,
                                                kBlock / kWave, nullptr, R, A.dvec);
@@ -714,8 +737,10 @@ __global__ __launch_bounds__(256) void k_eval_half(EvalArgs A, DevIntegrand P, R
   const int lane = threadIdx.x & (kWave - 1);
   const int s = threadIdx.x & (kHalfS - 1);
   const int64_t r = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kHalfS;
-  const bool valid = r < A.n;
-  const int64_t rr = valid ? r : A.n - 1;
+  const int64_t n = eval_len(A);
+  if (block_beyond(A, n, (int64_t)blockIdx.x * (kBlock / kHalfS))) return;
+  const bool valid = r < n;
+  const int64_t rr = valid ? r : (n > 0 ? n - 1 : 0);
   const PlainIntegrand<D, FAM> f{P};
   RegionOut o;
   if constexpr (KEY == 9)
@@ -756,8 +781,10 @@ __global__ __launch_bounds__(512) void k_eval9_seg(EvalArgs A, DevIntegrand P, R
   const int lane = threadIdx.x & (kWave - 1);
   const int g = wave / S, s = wave - g * S;
   const int64_t r = (int64_t)blockIdx.x * (kWave * (W / S)) + g * kWave + lane;
-  const bool valid = r < A.n;
-  const int64_t rr = valid ? r : A.n - 1;
+  const int64_t n = eval_len(A);
+  if (block_beyond(A, n, (int64_t)blockIdx.x * (kWave * (W / S)))) return;
+  const bool valid = r < n;
+  const int64_t rr = valid ? r : (n > 0 ? n - 1 : 0);
   const PlainIntegrand<D, FAM> f{P};
   const RegionOut o = seg_eval9<D>(f, A.lower, A.width, A.ld, r, valid, rr, S, s, wave, lane, W, seg_lds, R.r9,
                                    A.dvec);

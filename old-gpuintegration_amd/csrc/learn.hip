@@ -1307,9 +1307,10 @@ int guided_run(cuhre_context* c, cuhre_guide* g, const cuhre_integrand* f, int d
   const int64_t ds_base0 = g->ds_n;
   double fin_s = 0, fin_c = 0, fer_s = 0, fer_c = 0, fin_abs = 0, fin_aest = 0, ret_err = 0;
   int64_t nfin = 0, n = n0, logged = 0;
+  int64_t bound = n0;  // 2 n of the previous list: picks the segments, as in the plain loop
   int result = CUHRE_NOT_CONVERGED;
   for (int it = 0; it < 1000; ++it) {
-    const int S = pick_segments(c, n, 7, f->family, dim);
+    const int S = pick_segments(c, bound, 7, f->family, dim);
     EvalArgs A;
     fill_eval_args(c, A, cur, n, dim, epsrel, epsabs);
     A.loc_epsrel = loc_rel;
@@ -1546,6 +1547,7 @@ int guided_run(cuhre_context* c, cuhre_guide* g, const cuhre_integrand* f, int d
       result = out->error <= std::max(epsrel * std::fabs(out->integral), epsabs) ? CUHRE_OK : CUHRE_NOT_CONVERGED;
       break;
     }
+    bound = 2 * n;
     n = 2 * gsum.n_split;
     cur = 1 - cur;
   }

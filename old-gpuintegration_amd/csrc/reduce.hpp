@@ -160,9 +160,9 @@ __device__ __forceinline__ void block_reduce_partial(Partial& p, Partial* ldsw, 
   }
 }
 
-// Global stop decision from the iteration's total q; one thread writes the
-// state (and publishes it to the host when asked).
-__device__ __forceinline__ void write_iteration_state(const EvalArgs& A, const Partial& q)
+// The iteration's state from its total q: the sums with the finished
+// regions before it and the global stop decision.
+__device__ __forceinline__ IterState iteration_state(const EvalArgs& A, const Partial& q)
 {
   IterState st;
   st.tot = q;
@@ -181,12 +181,23 @@ __device__ __forceinline__ void write_iteration_state(const EvalArgs& A, const P
   else if (q.nactive > A.alloc_active) stop = 5;
   st.stop = stop;
   st.pad = 0;
+  return st;
+}
+
+// One thread writes the state (and publishes it to the host when asked).
+__device__ __forceinline__ void write_state(const EvalArgs& A, const IterState& st)
+{
   *A.state = st;
   if (A.host_state != nullptr) {
     // publish to the host, which polls host_seq instead of synchronising
     *A.host_state = st;
     __hip_atomic_store(A.host_seq, A.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
   }
+}
+
+__device__ __forceinline__ void write_iteration_state(const EvalArgs& A, const Partial& q)
+{
+  write_state(A, iteration_state(A, q));
 }
 
 // Launches up to this many blocks are finalised by one wavefront, larger
@@ -201,7 +212,8 @@ __device__ __forceinline__ void store_block_partial(const EvalArgs& A, const Par
 }
 
 // Reduces the n partials src[0..n) with one block of W <= 16 wavefronts:
-// dst[b] = exclusive scan of the active counts, the total lands in thread 0.
+// dst[b] = exclusive scan of the active counts (unless dst is null), the
+// total lands in thread 0.
 // Each thread sums a contiguous chunk in order; each wave scans its counts
 // and reduces its partials with shuffles; one barrier hands the wave totals
 // over and wave 0 reduces them.  Every thread returns (callers may barrier).
@@ -231,10 +243,11 @@ __device__ __forceinline__ void reduce_partials(const Partial* src, uint32_t n, 
   __syncthreads();
   int64_t run = (int64_t)incl - mine;
   for (uint32_t w = 0; w < wave; ++w) run += s_cnt[w];
-  for (uint32_t b = b0; b < b1; ++b) {
-    dst[b] = run;
-    run += src[b].nactive;
-  }
+  if (dst != nullptr)
+    for (uint32_t b = b0; b < b1; ++b) {
+      dst[b] = run;
+      run += src[b].nactive;
+    }
   if (wave == 0) {
     Partial t;
     if (lane < W) t = ldsw[lane];
@@ -260,6 +273,8 @@ __device__ __forceinline__ void finish_iteration(const EvalArgs& A, uint32_t nb)
 // parallel.
 __device__ __forceinline__ void finish_groups(const EvalArgs& A, uint32_t nb)
 {
+  // a launch queued ahead of a loop that stopped before it: touch nothing
+  if (A.n_dev != nullptr && *A.n_dev == 0) return;
   const uint32_t lane = threadIdx.x;
   const uint32_t b = blockIdx.x * kWave + lane;
   Partial q;

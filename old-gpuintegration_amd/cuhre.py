@@ -30,7 +30,7 @@ ERR_CAPACITY = -6
 
 # Symbols every build exports (kept in step with include/cuhre.h).
 EXPORTS = [
-    "cuhre_context_create", "cuhre_context_destroy", "cuhre_context_set",
+    "cuhre_context_create", "cuhre_context_destroy", "cuhre_context_set", "cuhre_context_queue_ahead",
     "cuhre_integrate", "cuhre_integrate_two_phase", "cuhre_eval_regions", "cuhre_split_regions", "cuhre_sum",
     "cuhre_last_error", "cuhre_points_per_region", "cuhre_integrate_batch", "cuhre_batch_autotune",
     "cuhre_guide_create", "cuhre_guide_destroy", "cuhre_guide_set_params", "cuhre_guide_get_params",
@@ -179,6 +179,7 @@ def lib():
         L.cuhre_context_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_int64]
         L.cuhre_context_destroy.argtypes = [ctxp]
         L.cuhre_context_set.argtypes = [ctxp, C.c_int, C.c_int]
+        L.cuhre_context_queue_ahead.argtypes = [ctxp, C.c_int]
         L.cuhre_integrate.argtypes = [ctxp, C.POINTER(Integrand), C.c_int, C.c_double, C.c_double,
                                       C.c_int64, C.c_int, dp, dp, C.POINTER(Result)]
         L.cuhre_integrate_noalloc.argtypes = [ctxp, C.POINTER(Integrand), C.c_int, C.c_double, C.c_double,
@@ -272,6 +273,11 @@ class Context:
         """profile: HIP-event timing of every launch; segments: 0 automatic or
         1/2/4/8/16 wavefronts per group of 64 regions."""
         _check(lib().cuhre_context_set(self.h, int(profile), segments))
+
+    def queue_ahead(self, on: bool = True):
+        """Plain loop: queue each iteration before the previous state is read
+        (the default) or one at a time; results are identical."""
+        _check(lib().cuhre_context_queue_ahead(self.h, int(on)))
 
     def close(self):
         if self.h:
