@@ -164,10 +164,12 @@ __global__ __launch_bounds__(kBlock) void k_split(SplitArgs S)
 // The split with the iteration's finalisation folded in (one launch fewer
 // per iteration).  part[0..np) are the evaluation's block partials
 // (one level) or, after k_finish_groups, the sums of its groups of 64 blocks
-// (two levels), np <= kWaveFinalize.  Every block scans the np active
-// counts into LDS for its own offsets; block 0 also reduces the partials in
-// full, writes the global offsets (one level) or group bases (two levels)
-// for a re-run of the split, and writes and publishes the iteration state.
+// (two levels), np <= kWaveFinalize.  Every block but the last scans the
+// np active counts into LDS for its own offsets and splits its regions; the
+// last block (one beyond the list's blocks) reduces the partials in full,
+// writes the global offsets (one level) or group bases (two levels) for a
+// re-run of the split, and writes and publishes the iteration state, in
+// parallel with the split.
 // Blocks skip the split when a count alone stops the loop (non-finite
 // values, the budget, the buffers); when the global test is met the
 // children written are unused.
@@ -201,6 +203,27 @@ __global__ __launch_bounds__(kBlock) void k_split_fin(SplitArgs S, EvalArgs A, c
     const int64_t budget = left >= 0 ? left / (2 * P.npts) : -1;
     A.max_active = budget < P.cap2 ? budget : P.cap2;
   }
+  if (blockIdx.x == gridDim.x - 1) {
+    Partial q;
+    reduce_partials(part, np, two_level ? A.gbase : A.offsets, q);
+    if (threadIdx.x == 0) {
+      const IterState st = iteration_state(A, q);
+      write_state(A, st);
+      if (P.out != nullptr) {
+        SpecSlot o;
+        o.n = st.stop == 0 ? 2 * q.nactive : 0;
+        o.pad = 0;
+        o.fin_s = A.fin_est_s;
+        o.fin_c = A.fin_est_c;
+        o.fer_s = A.fin_err_s;
+        o.fer_c = A.fin_err_c;
+        dd_add(o.fin_s, o.fin_c, q.fest_s, q.fest_c);
+        dd_add(o.fer_s, o.fer_c, q.ferr_s, q.ferr_c);
+        *P.out = o;
+      }
+    }
+    return;
+  }
   // exclusive scan of the active counts (np <= 256 = one per thread)
   const int64_t mine = threadIdx.x < np ? part[threadIdx.x].nactive : 0;
   const int64_t bad = threadIdx.x < np ? part[threadIdx.x].nnonfinite : 0;
@@ -222,26 +245,6 @@ __global__ __launch_bounds__(kBlock) void k_split_fin(SplitArgs S, EvalArgs A, c
     nonfinite += s_wbad[w];
   }
   if (threadIdx.x < np) s_base[threadIdx.x] = before + (int64_t)incl - mine;
-  if (blockIdx.x == 0) {
-    Partial q;
-    reduce_partials(part, np, two_level ? A.gbase : A.offsets, q);
-    if (threadIdx.x == 0) {
-      const IterState st = iteration_state(A, q);
-      write_state(A, st);
-      if (P.out != nullptr) {
-        SpecSlot o;
-        o.n = st.stop == 0 ? 2 * q.nactive : 0;
-        o.pad = 0;
-        o.fin_s = A.fin_est_s;
-        o.fin_c = A.fin_est_c;
-        o.fer_s = A.fin_err_s;
-        o.fer_c = A.fin_err_c;
-        dd_add(o.fin_s, o.fin_c, q.fest_s, q.fest_c);
-        dd_add(o.fer_s, o.fer_c, q.ferr_s, q.ferr_c);
-        *P.out = o;
-      }
-    }
-  }
   __syncthreads();
   if (nonfinite != 0 || nact > A.max_active || nact > A.alloc_active) return;
 
@@ -1027,7 +1030,8 @@ int plain_loop(cuhre_context* c, const cuhre_integrand* f, int dim, double epsre
     }
     launch(A, P, R, L.S, c->stream);
     A.prof_start = A.prof_stop = nullptr;
-    const dim3 sgrid((unsigned)((nl + kBlock - 1) / kBlock));
+    // the split's blocks and one more for the finalisation
+    const dim3 sgrid((unsigned)((nl + kBlock - 1) / kBlock + 1));
     const int64_t G = (L.nb + kWave - 1) / kWave;
     if (L.nb <= (int64_t)kWaveFinalize) {
       hipLaunchKernelGGL(k_split_fin, sgrid, dim3(kBlock), 0, c->stream, SA, A, (const Partial*)A.partials,
@@ -1038,7 +1042,7 @@ int plain_loop(cuhre_context* c, const cuhre_integrand* f, int dim, double epsre
                          (uint32_t)G, 1, Q);
     } else {
       launch_finish(A, L.nb, c->stream);
-      hipLaunchKernelGGL(k_split, sgrid, dim3(kBlock), 0, c->stream, SA);
+      hipLaunchKernelGGL(k_split, dim3(sgrid.x - 1), dim3(kBlock), 0, c->stream, SA);
     }
     if (c->profile) HIP_TRY(hipEventRecord(c->ev[2], c->stream));
     HIP_TRY(hipGetLastError());

@@ -211,35 +211,66 @@ __device__ __forceinline__ void store_block_partial(const EvalArgs& A, const Par
   if (threadIdx.x == 0) A.partials[blockIdx.x] = p;
 }
 
-// Reduces the n partials src[0..n) with one block of W <= 16 wavefronts:
-// dst[b] = exclusive scan of the active counts (unless dst is null), the
-// total lands in thread 0.
-// Each thread sums a contiguous chunk in order; each wave scans its counts
-// and reduces its partials with shuffles; one barrier hands the wave totals
-// over and wave 0 reduces them.  Every thread returns (callers may barrier).
+// Reduces the n partials src[0..n) with one block of T = 64 W threads
+// (W <= 16 wavefronts): dst[b] = exclusive scan of the active counts (unless
+// dst is null), the total lands in thread 0.  The counts are summed exactly
+// over contiguous chunks.  The sums follow one order whatever T is: field
+// by field, partials k, k+16, k+32, ... are added in order into 16 sums
+// (k = 0..15), which are then combined pairwise (partners k xor 1, 2, 4, 8,
+// in a 16-lane row); the eight double fields go to 8 rows of 16 threads,
+// so the whole block works on one short chain instead of shuffling the
+// partial's 14 fields through a 64-lane tree.  Every thread returns
+// (callers may barrier).
 __device__ __forceinline__ void reduce_partials(const Partial* src, uint32_t n, int64_t* dst, Partial& total)
 {
-  __shared__ Partial ldsw[kMaxWaves];
-  __shared__ int64_t s_cnt[kMaxWaves];
+  constexpr int kFields = 8;  // est, err, fest, ferr (value + carry), abs_all, abs_fin, aest_all, aest_fin
+  __shared__ double s_f[kFields][2];
+  __shared__ int64_t s_cnt[kMaxWaves], s_bad[kMaxWaves];
   const uint32_t T = blockDim.x, W = T / kWave;
   const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
   const uint32_t chunk = (n + T - 1) / T;
   const uint32_t b0 = threadIdx.x * chunk < n ? threadIdx.x * chunk : n;
   const uint32_t b1 = b0 + chunk < n ? b0 + chunk : n;
-  Partial q;
-  partial_zero(q);
-  for (uint32_t b = b0; b < b1; ++b) partial_add(q, src[b]);
-  const int64_t mine = q.nactive;
-  long long incl = (long long)mine;
+  int64_t mine = 0, bad = 0;
+  for (uint32_t b = b0; b < b1; ++b) {
+    mine += src[b].nactive;
+    bad += src[b].nnonfinite;
+  }
+  long long incl = (long long)mine, nbad = (long long)bad;
 #pragma unroll
   for (int off = 1; off < kWave; off <<= 1) {
     const long long v = __shfl_up(incl, off, kWave);
     if ((int)lane >= off) incl += v;
   }
 #pragma unroll
-  for (int mask = 1; mask < kWave; mask <<= 1) partial_shfl_add(q, mask);
+  for (int mask = 1; mask < kWave; mask <<= 1) nbad += __shfl_xor(nbad, mask, kWave);
   if (lane == kWave - 1) s_cnt[wave] = (int64_t)incl;
-  if (lane == 0) ldsw[wave] = q;
+  if (lane == 0) s_bad[wave] = (int64_t)nbad;
+  for (uint32_t base = 0; base < kFields * 16; base += T) {
+    const uint32_t idx = base + threadIdx.x;
+    const uint32_t f = idx >> 4, k = idx & 15;
+    double v = 0.0, c = 0.0;
+    if (f < kFields) {
+      const bool dd = f < 4;
+      const uint32_t off = dd ? 2 * f : 4 + f;
+      for (uint32_t j = k; j < n; j += 16) {
+        const double* q = reinterpret_cast<const double*>(src + j) + off;
+        if (dd) dd_add(v, c, q[0], q[1]);
+        else v += q[0];
+      }
+    }
+#pragma unroll
+    for (int m = 1; m < 16; m <<= 1) {
+      const double vo = shfl_xor_d(v, m), co = shfl_xor_d(c, m);
+      // TwoSum's error term is exact, so both partners get the same bits
+      if (f < 4) dd_add(v, c, vo, co);
+      else v += vo;
+    }
+    if (f < kFields && k == 0) {
+      s_f[f][0] = v;
+      s_f[f][1] = c;
+    }
+  }
   __syncthreads();
   int64_t run = (int64_t)incl - mine;
   for (uint32_t w = 0; w < wave; ++w) run += s_cnt[w];
@@ -248,11 +279,25 @@ __device__ __forceinline__ void reduce_partials(const Partial* src, uint32_t n, 
       dst[b] = run;
       run += src[b].nactive;
     }
-  if (wave == 0) {
+  if (threadIdx.x == 0) {
     Partial t;
-    if (lane < W) t = ldsw[lane];
-    else partial_zero(t);
-    for (uint32_t mask = 1; mask < W; mask <<= 1) partial_shfl_add(t, (int)mask);
+    t.est_s = s_f[0][0];
+    t.est_c = s_f[0][1];
+    t.err_s = s_f[1][0];
+    t.err_c = s_f[1][1];
+    t.fest_s = s_f[2][0];
+    t.fest_c = s_f[2][1];
+    t.ferr_s = s_f[3][0];
+    t.ferr_c = s_f[3][1];
+    t.abs_all = s_f[4][0];
+    t.abs_fin = s_f[5][0];
+    t.aest_all = s_f[6][0];
+    t.aest_fin = s_f[7][0];
+    t.nactive = t.nnonfinite = 0;
+    for (uint32_t w = 0; w < W; ++w) {
+      t.nactive += s_cnt[w];
+      t.nnonfinite += s_bad[w];
+    }
     total = t;
   }
 }
