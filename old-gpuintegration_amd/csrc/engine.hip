@@ -162,28 +162,29 @@ __global__ __launch_bounds__(kBlock) void k_split(SplitArgs S)
 }
 
 // The split with the iteration's finalisation folded in (one launch fewer
-// per iteration).  part[0..np) are the evaluation's block partials
-// (one level) or, after k_finish_groups, the sums of its groups of 64 blocks
-// (two levels), np <= kWaveFinalize.  Every block but the last scans the
-// np active counts into LDS for its own offsets and splits its regions; the
-// last block (one beyond the list's blocks) reduces the partials in full,
-// writes the global offsets (one level) or group bases (two levels) for a
-// re-run of the split, and writes and publishes the iteration state, in
-// parallel with the split.
+// per iteration).  part[0..np) are the evaluation's block partials, np <=
+// kOneLevel (one level), or, after k_finish_groups, the sums of its groups
+// of 64 blocks, np <= kWaveFinalize (two levels).  Every block but the last
+// scans the np active counts (one level: the dense counts the evaluation
+// blocks stored in offsets[]) into LDS for its own offsets and splits its
+// regions; the last block (one beyond the list's blocks) reduces the
+// partials in full, writes the group bases (two levels) for a re-run of the
+// split, and writes and publishes the iteration state, in parallel with the
+// split.
 // Blocks skip the split when a count alone stops the loop (non-finite
 // values, the budget, the buffers); when the global test is met the
 // children written are unused.
 // Queued ahead of the host (P.in set): the list length and the finished
 // sums come from the device slot P.in, the number of partials and the
 // levels of the finalisation follow from that length (the launch grids
-// were sized for an upper bound), the budget test uses it, and block 0
-// also writes the next list's slot: its length (0 when the loop stops, so
+// were sized for an upper bound), the budget test uses it, and the last
+// block also writes the next list's slot: its length (0 when the loop stops, so
 // that an iteration queued behind this one does nothing) and the finished
 // sums after this iteration.
 __global__ __launch_bounds__(kBlock) void k_split_fin(SplitArgs S, EvalArgs A, const Partial* part, uint32_t np,
                                                       int two_level, SpecArgs P)
 {
-  __shared__ int64_t s_base[kWaveFinalize];
+  __shared__ int64_t s_base[kOneLevel];
   __shared__ int64_t s_wsum[kBlock / kWave], s_wbad[kBlock / kWave];
   __shared__ uint32_t wcnt[kBlock / kWave];
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
@@ -192,7 +193,7 @@ __global__ __launch_bounds__(kBlock) void k_split_fin(SplitArgs S, EvalArgs A, c
     n = P.in->n;
     if (n == 0) return;  // the loop stopped before this iteration
     const int64_t nb = (n + P.rpb - 1) / P.rpb;
-    two_level = nb > (int64_t)kWaveFinalize;
+    two_level = nb > (int64_t)kOneLevel;
     part = two_level ? A.gpart : A.partials;
     np = (uint32_t)(two_level ? (nb + kWave - 1) / kWave : nb);
     A.fin_est_s = P.in->fin_s;
@@ -205,7 +206,7 @@ __global__ __launch_bounds__(kBlock) void k_split_fin(SplitArgs S, EvalArgs A, c
   }
   if (blockIdx.x == gridDim.x - 1) {
     Partial q;
-    reduce_partials(part, np, two_level ? A.gbase : A.offsets, q);
+    reduce_partials(part, np, two_level ? A.gbase : nullptr, q);
     if (threadIdx.x == 0) {
       const IterState st = iteration_state(A, q);
       write_state(A, st);
@@ -224,9 +225,24 @@ __global__ __launch_bounds__(kBlock) void k_split_fin(SplitArgs S, EvalArgs A, c
     }
     return;
   }
-  // exclusive scan of the active counts (np <= 256 = one per thread)
-  const int64_t mine = threadIdx.x < np ? part[threadIdx.x].nactive : 0;
-  const int64_t bad = threadIdx.x < np ? part[threadIdx.x].nnonfinite : 0;
+  // exclusive scan of the active counts, four per thread
+  int64_t cnt[4], mine = 0, bad = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const uint32_t j = 4 * threadIdx.x + i;
+    cnt[i] = 0;
+    if (j < np) {
+      if (two_level) {
+        cnt[i] = part[j].nactive;
+        bad += part[j].nnonfinite;
+      } else {
+        const int64_t v = A.offsets[j];
+        cnt[i] = v & kCountMask;
+        bad += v >> 40;
+      }
+    }
+    mine += cnt[i];
+  }
   long long incl = (long long)mine, nbad = (long long)bad;
 #pragma unroll
   for (int off = 1; off < kWave; off <<= 1) {
@@ -244,7 +260,12 @@ __global__ __launch_bounds__(kBlock) void k_split_fin(SplitArgs S, EvalArgs A, c
     nact += s_wsum[w];
     nonfinite += s_wbad[w];
   }
-  if (threadIdx.x < np) s_base[threadIdx.x] = before + (int64_t)incl - mine;
+  int64_t run = before + (int64_t)incl - mine;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    if (4 * threadIdx.x + i < np) s_base[4 * threadIdx.x + i] = run;
+    run += cnt[i];
+  }
   __syncthreads();
   if (nonfinite != 0 || nact > A.max_active || nact > A.alloc_active) return;
 
@@ -291,16 +312,35 @@ __global__ __launch_bounds__(kBlock) void k_split_fin(SplitArgs S, EvalArgs A, c
   }
 }
 
-// The list length and finished sums of the first list of a loop whose
-// iterations are queued ahead (slot), or only the length (after the host
-// re-ran a split).
-__global__ void k_spec_set(SpecSlot* slot, int64_t n, int zero_sums)
+// In place: the dense active counts of nb <= kOneLevel evaluation blocks
+// (offsets[b] = count + kBadFlag when non-finite) to their exclusive scan,
+// the offsets a re-run of the split takes.  One block, four per thread.
+__global__ __launch_bounds__(kBlock) void k_scan_counts(int64_t* off, uint32_t nb)
 {
-  if (threadIdx.x != 0) return;
-  slot->n = n;
-  if (zero_sums) {
-    slot->fin_s = slot->fin_c = slot->fer_s = slot->fer_c = 0.0;
-    slot->pad = 0;
+  __shared__ int64_t s_w[kBlock / kWave];
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  int64_t cnt[4], mine = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const uint32_t j = 4 * threadIdx.x + i;
+    cnt[i] = j < nb ? off[j] & kCountMask : 0;
+    mine += cnt[i];
+  }
+  long long incl = (long long)mine;
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const long long v = __shfl_up(incl, o, kWave);
+    if (lane >= o) incl += v;
+  }
+  if (lane == kWave - 1) s_w[wave] = (int64_t)incl;
+  __syncthreads();
+  int64_t run = (int64_t)incl - mine;
+  for (int w = 0; w < wave; ++w) run += s_w[w];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const uint32_t j = 4 * threadIdx.x + i;
+    if (j < nb) off[j] = run;
+    run += cnt[i];
   }
 }
 
@@ -391,7 +431,10 @@ extern "C" __attribute__((visibility("hidden"))) BatchEngine* cuhre_ctx_batch(cu
 }
 
 __global__ __launch_bounds__(kWave) void k_finish(EvalArgs A, uint32_t nb) { finish_iteration(A, nb); }
-__global__ __launch_bounds__(kWave) void k_finish_groups(EvalArgs A, uint32_t nb) { finish_groups(A, nb); }
+__global__ __launch_bounds__(kWave) void k_finish_groups(EvalArgs A, uint32_t nb, int rpb)
+{
+  finish_groups(A, nb, rpb);
+}
 __global__ __launch_bounds__(kMaxWaves * kWave) void k_finish_top(EvalArgs A, uint32_t G) { finish_top(A, G); }
 
 void launch_finish(const EvalArgs& A, int64_t nb, hipStream_t st)
@@ -401,7 +444,7 @@ void launch_finish(const EvalArgs& A, int64_t nb, hipStream_t st)
     return;
   }
   const int64_t G = (nb + kWave - 1) / kWave;
-  hipLaunchKernelGGL(k_finish_groups, dim3((unsigned)G), dim3(kWave), 0, st, A, (uint32_t)nb);
+  hipLaunchKernelGGL(k_finish_groups, dim3((unsigned)G), dim3(kWave), 0, st, A, (uint32_t)nb, 0);
   hipLaunchKernelGGL(k_finish_top, dim3(1), dim3(G <= (int64_t)kWaveFinalize ? kWave : kMaxWaves * kWave), 0, st,
                      A, (uint32_t)G);
 }
@@ -980,7 +1023,7 @@ int plain_loop(cuhre_context* c, const cuhre_integrand* f, int dim, double epsre
     bool slot = false;  // its split writes the next list's slot
   };
   const auto fused = [](int64_t nb) {
-    return nb <= (int64_t)kWaveFinalize || (nb + kWave - 1) / kWave <= (int64_t)kWaveFinalize;
+    return nb <= (int64_t)kOneLevel || (nb + kWave - 1) / kWave <= (int64_t)kWaveFinalize;
   };
   // Queues the evaluation and split of list lc: nl regions (exact) or at
   // most nl, the length then read on the device; bnd picks the segments;
@@ -1016,7 +1059,7 @@ int plain_loop(cuhre_context* c, const cuhre_integrand* f, int dim, double epsre
     Q.rpb = rpb;
     SplitArgs SA;
     fill_split_args(c, SA, lc, nl, dim, epsrel, epsabs, rpb, true);
-    if (L.nb > (int64_t)kWaveFinalize) SA.gbase = A.gbase;
+    if (L.nb > (int64_t)kOneLevel) SA.gbase = A.gbase;
     // without profiling the state comes back through coherent host memory
     // (two buffers: the split queued behind may publish before the host reads)
     if (!c->profile) {
@@ -1033,11 +1076,11 @@ int plain_loop(cuhre_context* c, const cuhre_integrand* f, int dim, double epsre
     // the split's blocks and one more for the finalisation
     const dim3 sgrid((unsigned)((nl + kBlock - 1) / kBlock + 1));
     const int64_t G = (L.nb + kWave - 1) / kWave;
-    if (L.nb <= (int64_t)kWaveFinalize) {
+    if (L.nb <= (int64_t)kOneLevel) {
       hipLaunchKernelGGL(k_split_fin, sgrid, dim3(kBlock), 0, c->stream, SA, A, (const Partial*)A.partials,
                          (uint32_t)L.nb, 0, Q);
     } else if (G <= (int64_t)kWaveFinalize) {
-      hipLaunchKernelGGL(k_finish_groups, dim3((unsigned)G), dim3(kWave), 0, c->stream, A, (uint32_t)L.nb);
+      hipLaunchKernelGGL(k_finish_groups, dim3((unsigned)G), dim3(kWave), 0, c->stream, A, (uint32_t)L.nb, rpb);
       hipLaunchKernelGGL(k_split_fin, sgrid, dim3(kBlock), 0, c->stream, SA, A, (const Partial*)A.gpart,
                          (uint32_t)G, 1, Q);
     } else {
@@ -1105,8 +1148,11 @@ int plain_loop(cuhre_context* c, const cuhre_integrand* f, int dim, double epsre
       if ((rc = ctx_reserve(c, ncap, dim, n, cur, nb))) return rc;
       SplitArgs SA;
       fill_split_args(c, SA, cur, n, dim, epsrel, epsabs, seg_regions_per_block(S), false);
-      if (nb > (int64_t)kWaveFinalize) SA.gbase = c->offsets + block_capacity(c->cap);
+      if (nb > (int64_t)kOneLevel) SA.gbase = c->offsets + block_capacity(c->cap);
       if (c->profile) HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+      // one level: offsets[] still holds the blocks' dense counts
+      if (nb <= (int64_t)kOneLevel)
+        hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(kBlock), 0, c->stream, c->offsets, (uint32_t)nb);
       hipLaunchKernelGGL(k_split, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, c->stream, SA);
       if (c->profile) HIP_TRY(hipEventRecord(c->ev[2], c->stream));
       HIP_TRY(hipGetLastError());

@@ -72,6 +72,7 @@ __device__ __forceinline__ bool block_beyond(const EvalArgs& A, int64_t n, int64
     Partial z;
     partial_zero(z);
     A.partials[blockIdx.x] = z;
+    A.offsets[blockIdx.x] = 0;
   }
   return true;
 }

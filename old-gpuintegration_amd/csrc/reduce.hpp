@@ -204,23 +204,35 @@ __device__ __forceinline__ void write_iteration_state(const EvalArgs& A, const P
 // ones in groups of 64 blocks.
 constexpr uint32_t kWaveFinalize = 4 * kWave;
 
+// Evaluation launches of up to this many blocks are finalised by the split
+// in one level: every evaluation block also stores its active count, plus
+// kBadFlag when it holds a non-finite value, into offsets[block], a dense
+// array the split's blocks scan (a finalisation kernel later overwrites it
+// with offsets).
+constexpr uint32_t kOneLevel = 1024;
+constexpr int64_t kBadFlag = 1ll << 40;
+constexpr int64_t kCountMask = kBadFlag - 1;
+
 // Called by every evaluation block after its thread 0 holds the block's
 // Partial in p.
 __device__ __forceinline__ void store_block_partial(const EvalArgs& A, const Partial& p)
 {
-  if (threadIdx.x == 0) A.partials[blockIdx.x] = p;
+  if (threadIdx.x == 0) {
+    A.partials[blockIdx.x] = p;
+    A.offsets[blockIdx.x] = p.nactive + (p.nnonfinite > 0 ? kBadFlag : 0);
+  }
 }
 
 // Reduces the n partials src[0..n) with one block of T = 64 W threads
 // (W <= 16 wavefronts): dst[b] = exclusive scan of the active counts (unless
 // dst is null), the total lands in thread 0.  The counts are summed exactly
 // over contiguous chunks.  The sums follow one order whatever T is: field
-// by field, partials k, k+16, k+32, ... are added in order into 16 sums
-// (k = 0..15), which are then combined pairwise (partners k xor 1, 2, 4, 8,
-// in a 16-lane row); the eight double fields go to 8 rows of 16 threads,
-// so the whole block works on one short chain instead of shuffling the
-// partial's 14 fields through a 64-lane tree.  Every thread returns
-// (callers may barrier).
+// by field, partials k, k+32, k+64, ... are added in order into 32 sums
+// (k = 0..31), which are then combined pairwise (partners k xor 1, 2, 4, 8,
+// 16 in a 32-lane row); the eight double fields go to 8 rows of 32
+// threads, so the whole block works on short chains (loads issued 16 at a
+// time) instead of shuffling the partial's 14 fields through a 64-lane
+// tree.  Every thread returns (callers may barrier).
 __device__ __forceinline__ void reduce_partials(const Partial* src, uint32_t n, int64_t* dst, Partial& total)
 {
   constexpr int kFields = 8;  // est, err, fest, ferr (value + carry), abs_all, abs_fin, aest_all, aest_fin
@@ -246,21 +258,37 @@ __device__ __forceinline__ void reduce_partials(const Partial* src, uint32_t n, 
   for (int mask = 1; mask < kWave; mask <<= 1) nbad += __shfl_xor(nbad, mask, kWave);
   if (lane == kWave - 1) s_cnt[wave] = (int64_t)incl;
   if (lane == 0) s_bad[wave] = (int64_t)nbad;
-  for (uint32_t base = 0; base < kFields * 16; base += T) {
+  constexpr uint32_t kRows = 32, kBatch = 16;
+  for (uint32_t base = 0; base < kFields * kRows; base += T) {
     const uint32_t idx = base + threadIdx.x;
-    const uint32_t f = idx >> 4, k = idx & 15;
+    const uint32_t f = idx / kRows, k = idx % kRows;
     double v = 0.0, c = 0.0;
     if (f < kFields) {
-      const bool dd = f < 4;
+      const bool dd = f < 4;  // uniform in a wavefront (two rows of one kind)
       const uint32_t off = dd ? 2 * f : 4 + f;
-      for (uint32_t j = k; j < n; j += 16) {
-        const double* q = reinterpret_cast<const double*>(src + j) + off;
-        if (dd) dd_add(v, c, q[0], q[1]);
-        else v += q[0];
+      for (uint32_t j0 = k; j0 < n; j0 += kBatch * kRows) {
+        double a[kBatch], b[kBatch];
+#pragma unroll
+        for (uint32_t u = 0; u < kBatch; ++u) {
+          const uint32_t j = j0 + u * kRows;
+          a[u] = b[u] = 0.0;
+          if (j < n) {
+            const double* q = reinterpret_cast<const double*>(src + j) + off;
+            a[u] = q[0];
+            if (dd) b[u] = q[1];
+          }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kBatch; ++u) {
+          if (j0 + u * kRows < n) {
+            if (dd) dd_add(v, c, a[u], b[u]);
+            else v += a[u];
+          }
+        }
       }
     }
 #pragma unroll
-    for (int m = 1; m < 16; m <<= 1) {
+    for (int m = 1; m < (int)kRows; m <<= 1) {
       const double vo = shfl_xor_d(v, m), co = shfl_xor_d(c, m);
       // TwoSum's error term is exact, so both partners get the same bits
       if (f < 4) dd_add(v, c, vo, co);
@@ -316,10 +344,16 @@ __device__ __forceinline__ void finish_iteration(const EvalArgs& A, uint32_t nb)
 // One block reading all partials of a large launch took ~40 us at 8k blocks
 // (its load stream is serial per thread); in groups they are read in
 // parallel.
-__device__ __forceinline__ void finish_groups(const EvalArgs& A, uint32_t nb)
+// Queued ahead of the host (A.n_dev set, rpb regions per evaluation block):
+// nothing to do when the list is empty (the loop stopped before it) or
+// when its real length needs only one level (the split then scans the
+// blocks' dense counts, which must stay in offsets[]).
+__device__ __forceinline__ void finish_groups(const EvalArgs& A, uint32_t nb, int rpb = 0)
 {
-  // a launch queued ahead of a loop that stopped before it: touch nothing
-  if (A.n_dev != nullptr && *A.n_dev == 0) return;
+  if (A.n_dev != nullptr) {
+    const int64_t n = *A.n_dev;
+    if (n == 0 || (rpb > 0 && (n + rpb - 1) / rpb <= (int64_t)kOneLevel)) return;
+  }
   const uint32_t lane = threadIdx.x;
   const uint32_t b = blockIdx.x * kWave + lane;
   Partial q;
