@@ -435,7 +435,6 @@ template <int D, int FAM>
 __global__ __launch_bounds__(1024) void k_eval_seg(EvalArgs A, DevIntegrand P, RuleConst R, int S)
 {
   extern __shared__ double seg_lds[];
-  __shared__ Partial ldsw[kMaxWaves];
   const int W = S < 4 ? 4 : S;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & (kWave - 1);
@@ -458,8 +457,8 @@ __global__ __launch_bounds__(1024) void k_eval_seg(EvalArgs A, DevIntegrand P, R
   }
   if (A.finalize) {
     Partial p;
-    partial_from_region(p, o.est, o.err, o.ab, valid && owner, A.loc_epsrel, A.loc_epsabs);
-    block_reduce_partial(p, ldsw, W, owner, S);
+    block_reduce_regions(owner, g * kWave + lane, kWave * (W / S), o.est, o.err, o.ab, valid, A.loc_epsrel,
+                         A.loc_epsabs, p);
     store_block_partial(A, p);
   }
 }
@@ -470,7 +469,6 @@ __global__ __launch_bounds__(1024) void k_eval_seg(EvalArgs A, DevIntegrand P, R
 template <int D, int FAM>
 __global__ __launch_bounds__(kBlock) void k_eval_one(EvalArgs A, DevIntegrand P, RuleConst R)
 {
-  __shared__ Partial ldsw[kBlock / kWave];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -490,8 +488,7 @@ __global__ __launch_bounds__(kBlock) void k_eval_one(EvalArgs A, DevIntegrand P,
   }
   if (A.finalize) {
     Partial p;
-    partial_from_region(p, o.est, o.err, o.ab, valid, A.loc_epsrel, A.loc_epsabs);
-    block_reduce_partial(p, ldsw, kBlock / kWave);
+    block_reduce_regions(true, threadIdx.x, kBlock, o.est, o.err, o.ab, valid, A.loc_epsrel, A.loc_epsabs, p);
     store_block_partial(A, p);
   }
 }
@@ -734,7 +731,6 @@ __device__ __forceinline__ RegionOut seg_eval9(const F& f, const double* __restr
 template <int D, int FAM, int KEY>
 __global__ __launch_bounds__(256) void k_eval_half(EvalArgs A, DevIntegrand P, RuleConst R)
 {
-  __shared__ Partial ldsw[kBlock / kWave];
   const int lane = threadIdx.x & (kWave - 1);
   const int s = threadIdx.x & (kHalfS - 1);
   const int64_t r = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kHalfS;
@@ -758,8 +754,8 @@ __global__ __launch_bounds__(256) void k_eval_half(EvalArgs A, DevIntegrand P, R
   }
   if (A.finalize) {
     Partial p;
-    partial_from_region(p, o.est, o.err, o.ab, valid && owner, A.loc_epsrel, A.loc_epsabs);
-    block_reduce_partial(p, ldsw, kBlock / kWave);
+    block_reduce_regions(owner, threadIdx.x / kHalfS, kBlock / kHalfS, o.est, o.err, o.ab, valid, A.loc_epsrel,
+                         A.loc_epsabs, p);
     store_block_partial(A, p);
   }
 }
@@ -776,7 +772,6 @@ template <int D, int FAM>
 __global__ __launch_bounds__(512) void k_eval9_seg(EvalArgs A, DevIntegrand P, RuleConst R, int S)
 {
   extern __shared__ double seg_lds[];
-  __shared__ Partial ldsw[kMaxWaves];
   const int W = S < 4 ? 4 : S;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & (kWave - 1);
@@ -799,8 +794,8 @@ __global__ __launch_bounds__(512) void k_eval9_seg(EvalArgs A, DevIntegrand P, R
   }
   if (A.finalize) {
     Partial p;
-    partial_from_region(p, o.est, o.err, o.ab, valid && owner, A.loc_epsrel, A.loc_epsabs);
-    block_reduce_partial(p, ldsw, W, owner, S);
+    block_reduce_regions(owner, g * kWave + lane, kWave * (W / S), o.est, o.err, o.ab, valid, A.loc_epsrel,
+                         A.loc_epsabs, p);
     store_block_partial(A, p);
   }
 }

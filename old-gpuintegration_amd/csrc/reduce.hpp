@@ -160,6 +160,88 @@ __device__ __forceinline__ void block_reduce_partial(Partial& p, Partial* ldsw, 
   }
 }
 
+// Sums of an evaluation block's regions (at most 256, in slots 0..nslots-1
+// of LDS) into thread 0's Partial, for blocks of at least 256 threads.  The
+// owner thread of each slot stores the region's est, err and volume *
+// sum|w7 f| (zeros for a slot beyond the list) and whether it stays active;
+// then eight rows of 32 threads add one field each (est, err, their
+// finished parts, the two |.| sums and their finished parts): slots k,
+// k+32, ... in order into 32 sums, combined pairwise (partners k xor 1, 2,
+// 4, 8, 16).  The counts come from wavefront ballots.  Each region's values
+// and classification are those of partial_from_region.
+struct RegionSlots {
+  double est[kBlock], err[kBlock], ab[kBlock];
+  uint8_t fin[kBlock];  // the region is finished (its values count in the finished sums)
+};
+
+__device__ __forceinline__ void block_reduce_regions(bool owner, int slot, int nslots, double est, double err,
+                                                     double ab, bool valid, double epsrel, double epsabs, Partial& out)
+{
+  __shared__ RegionSlots L;
+  __shared__ double s_f[8][2];
+  __shared__ int64_t s_wact[kMaxWaves], s_wbad[kMaxWaves];
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  const bool finite = isfinite(est) && isfinite(err);
+  const bool act = finite && region_active(est, err, epsrel, epsabs);
+  if (owner) {
+    L.est[slot] = valid ? est : 0.0;
+    L.err[slot] = valid ? err : 0.0;
+    L.ab[slot] = valid ? ab : 0.0;
+    L.fin[slot] = valid && !act;
+  }
+  const uint64_t ma = __ballot(owner && valid && act), mb = __ballot(owner && valid && !finite);
+  if (lane == 0) {
+    s_wact[wave] = __popcll(ma);
+    s_wbad[wave] = __popcll(mb);
+  }
+  __syncthreads();
+  if (threadIdx.x < 8 * 32) {  // whole wavefronts
+    const int f = threadIdx.x >> 5, k = threadIdx.x & 31;
+    double v = 0.0, c = 0.0;
+    for (int j = k; j < nslots; j += 32) {
+      const bool keep = (f != 2 && f != 3 && f != 5 && f != 7) || L.fin[j];
+      const double e = L.est[j];
+      double x = f == 0 || f == 2 ? e : f == 1 || f == 3 ? L.err[j] : f == 4 || f == 5 ? L.ab[j] : fabs(e);
+      x = keep ? x : 0.0;
+      if (f < 4) dd_add(v, c, x, 0.0);
+      else v += x;
+    }
+#pragma unroll
+    for (int m = 1; m < 32; m <<= 1) {
+      const double vo = shfl_xor_d(v, m), co = shfl_xor_d(c, m);
+      if (f < 4) dd_add(v, c, vo, co);
+      else v += vo;
+    }
+    if (k == 0) {
+      s_f[f][0] = v;
+      s_f[f][1] = c;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    Partial t;
+    t.est_s = s_f[0][0];
+    t.est_c = s_f[0][1];
+    t.err_s = s_f[1][0];
+    t.err_c = s_f[1][1];
+    t.fest_s = s_f[2][0];
+    t.fest_c = s_f[2][1];
+    t.ferr_s = s_f[3][0];
+    t.ferr_c = s_f[3][1];
+    t.abs_all = s_f[4][0];
+    t.abs_fin = s_f[5][0];
+    t.aest_all = s_f[6][0];
+    t.aest_fin = s_f[7][0];
+    t.nactive = t.nnonfinite = 0;
+    const int W = blockDim.x / kWave;
+    for (int w = 0; w < W; ++w) {
+      t.nactive += s_wact[w];
+      t.nnonfinite += s_wbad[w];
+    }
+    out = t;
+  }
+}
+
 // The iteration's state from its total q: the sums with the finished
 // regions before it and the global stop decision.
 __device__ __forceinline__ IterState iteration_state(const EvalArgs& A, const Partial& q)
