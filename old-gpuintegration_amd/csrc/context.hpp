@@ -49,6 +49,9 @@ struct cuhre_context {
   int64_t half_max_regions = 8192;  // auto: half-wavefront form up to this many regions
 
   int64_t cap = 0;  // allocated regions per list buffer
+  // free HBM plus what the list buffers hold (bytes), measured when first
+  // needed and again after the lists are reallocated (< 0: measure)
+  double avail_bytes = -1.0;
   int dim_alloc = 0;
   double* lo[2] = {nullptr, nullptr};
   double* wd[2] = {nullptr, nullptr};

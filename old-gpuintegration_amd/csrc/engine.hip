@@ -503,6 +503,7 @@ int ctx_reserve(cuhre_context* c, int64_t cap, int dim, int64_t keep_n, int cur,
       hipMalloc(&part, sizeof(Partial) * (nbmax + group_capacity(nbmax))) != hipSuccess ||
       hipMalloc(&off, sizeof(int64_t) * (nbmax + group_capacity(nbmax))) != hipSuccess) {
     (void)hipGetLastError();
+    c->avail_bytes = -1.0;
     return fail(CUHRE_ERR_ALLOC, "cannot allocate a region list of %lld regions x %d dims",
                 (long long)cap, dim);
   }
@@ -536,6 +537,7 @@ int ctx_reserve(cuhre_context* c, int64_t cap, int dim, int64_t keep_n, int cur,
   c->offsets = off;
   c->cap = cap;
   c->dim_alloc = dim;
+  c->avail_bytes = -1.0;
   return 0;
 }
 
@@ -670,11 +672,17 @@ int make_dev_integrand(cuhre_context* c, const cuhre_integrand* f, int dim, DevI
 int64_t list_capacity(cuhre_context* c, int dim)
 {
   if (c->capacity > 0) return c->capacity;
-  size_t fr = 0, tot = 0;
-  if (hipMemGetInfo(&fr, &tot) != hipSuccess) return 64ll << 20;
+  // the context's own list allocations move bytes between "free" and
+  // "held", so the sum is measured (a runtime query of several us) only
+  // when first needed and after the lists are reallocated
+  if (c->avail_bytes < 0) {
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) return 64ll << 20;
+    const double held = (double)c->cap * (4.0 * sizeof(double) * c->dim_alloc + 3 * sizeof(double) + 1);
+    c->avail_bytes = (double)fr + held;
+  }
   const double per = 4.0 * sizeof(double) * dim + 3 * sizeof(double) + 1 + sizeof(Partial) / 64.0 + 1;
-  const double held = (double)c->cap * (4.0 * sizeof(double) * c->dim_alloc + 3 * sizeof(double) + 1);
-  return (int64_t)(((double)fr + held) * 0.75 / per);
+  return (int64_t)(c->avail_bytes * 0.75 / per);
 }
 
 // Whether the family's rule evaluation unrolls its point loops at dim (kUnroll).
@@ -1220,7 +1228,13 @@ int plain_loop(cuhre_context* c, const cuhre_integrand* f, int dim, double epsre
       break;
     }
   }
-  HIP_TRY(hipStreamSynchronize(c->stream));
+  // Queued ahead, the result is complete once the last state is read: what
+  // the stream still holds writes only the context's own buffers (the tail
+  // of the last split, an iteration that read length 0), and every later
+  // use of the context is ordered behind it on its stream (destroy
+  // synchronises).  The static workspace is shared by every context on the
+  // device, so the no-allocation form drains its stream before returning.
+  if (!ahead || opt.noalloc) HIP_TRY(hipStreamSynchronize(c->stream));
   if (result >= 0 || !opt.noalloc) out->fail = result > 0 ? result : 0;
   out->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return result;

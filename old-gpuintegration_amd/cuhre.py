@@ -232,6 +232,12 @@ def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
+def _box(v):
+    """A box bound as a ctypes double array (cheaper per call than a numpy
+    array's pointer)."""
+    return (C.c_double * len(v))(*v)
+
+
 def _check(rc):
     if rc < 0:
         raise CuhreError(rc, lib().cuhre_last_error().decode())
@@ -299,13 +305,11 @@ class Context:
     def integrate(self, p: W.Problem, epsrel: Optional[float] = None, maxeval: Optional[int] = None,
                   key: int = 0):
         s, keep = to_struct(p.f)
-        lo = np.ascontiguousarray(p.lows, dtype=np.float64)
-        hi = np.ascontiguousarray(p.highs, dtype=np.float64)
         res = Result()
         rc = lib().cuhre_integrate(self.h, C.byref(s), p.f.dim,
                                    p.epsrel if epsrel is None else epsrel, p.epsabs,
                                    p.maxeval if maxeval is None else maxeval, key,
-                                   _dp(lo), _dp(hi), C.byref(res))
+                                   _box(p.lows), _box(p.highs), C.byref(res))
         _check(rc)
         d = res.asdict()
         d["rc"] = rc
@@ -317,13 +321,11 @@ class Context:
         the library's static device array (no allocation in the call).
         With check=False the return code is returned instead of raised."""
         s, keep = to_struct(p.f)
-        lo = np.ascontiguousarray(p.lows, dtype=np.float64)
-        hi = np.ascontiguousarray(p.highs, dtype=np.float64)
         res = Result()
         rc = lib().cuhre_integrate_noalloc(self.h, C.byref(s), p.f.dim,
                                            p.epsrel if epsrel is None else epsrel, p.epsabs,
                                            p.maxeval if maxeval is None else maxeval, key,
-                                           _dp(lo), _dp(hi), C.byref(res))
+                                           _box(p.lows), _box(p.highs), C.byref(res))
         if check:
             _check(rc)
         d = res.asdict()
