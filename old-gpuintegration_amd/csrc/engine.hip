@@ -112,6 +112,34 @@ void batch_engine_delete(BatchEngine* e);
 // of the j-th active region goes to slot 2j and child 1 to 2j+1, so the
 // two children of a region are adjacent and written with one 16-byte store
 // per coordinate.
+// Lists up to this many regions load a region's split inputs before the
+// split's count scan (see k_split_fin).
+constexpr int64_t kPrefetchSplit = 262144;
+
+// The two children of a region with bounds (lo, wd), bisected along axis
+// a, to slots 2 dst and 2 dst + 1 of the output list: one 16-byte store per
+// coordinate and list.  The bounds were loaded for every coordinate first,
+// so their loads are in flight together.
+__device__ __forceinline__ void write_children(const SplitArgs& S, int64_t dst, int a, const double (&lo)[kMaxDim],
+                                               const double (&wd)[kMaxDim])
+{
+#pragma unroll
+  for (int m = 0; m < kMaxDim; ++m) {
+    if (m >= S.dim) break;
+    double2 L, W;
+    if (m == a) {
+      const double hw = 0.5 * wd[m];
+      L = make_double2(lo[m], lo[m] + hw);
+      W = make_double2(hw, hw);
+    } else {
+      L = make_double2(lo[m], lo[m]);
+      W = make_double2(wd[m], wd[m]);
+    }
+    *reinterpret_cast<double2*>(S.out_lower + m * S.out_ld + 2 * dst) = L;
+    *reinterpret_cast<double2*>(S.out_width + m * S.out_ld + 2 * dst) = W;
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_split(SplitArgs S)
 {
   if (S.state != nullptr && S.state->stop != 0) return;
@@ -144,21 +172,14 @@ __global__ __launch_bounds__(kBlock) void k_split(SplitArgs S)
   const int64_t blk = r / S.rpb;
   const int64_t dst = S.offsets[blk] + (S.gbase != nullptr ? S.gbase[blk / kWave] : 0) + rank;
   const int a = S.axis[r];
-  for (int m = 0; m < S.dim; ++m) {
-    const double lo = S.lower[m * S.ld + r];
-    const double w = S.width[m * S.ld + r];
-    double2 L, W;
-    if (m == a) {
-      const double hw = 0.5 * w;
-      L = make_double2(lo, lo + hw);
-      W = make_double2(hw, hw);
-    } else {
-      L = make_double2(lo, lo);
-      W = make_double2(w, w);
+  double lo[kMaxDim], wd[kMaxDim];
+#pragma unroll
+  for (int m = 0; m < kMaxDim; ++m)
+    if (m < S.dim) {
+      lo[m] = S.lower[m * S.ld + r];
+      wd[m] = S.width[m * S.ld + r];
     }
-    *reinterpret_cast<double2*>(S.out_lower + m * S.out_ld + 2 * dst) = L;
-    *reinterpret_cast<double2*>(S.out_width + m * S.out_ld + 2 * dst) = W;
-  }
+  write_children(S, dst, a, lo, wd);
 }
 
 // The split with the iteration's finalisation folded in (one launch fewer
@@ -225,6 +246,25 @@ __global__ __launch_bounds__(kBlock) void k_split_fin(SplitArgs S, EvalArgs A, c
     }
     return;
   }
+  // Lists up to kPrefetchSplit regions are latency-bound: the region's est,
+  // err, axis and bounds are loaded before the count scan so that their
+  // latency overlaps it (larger lists load them after the stop test, and
+  // the bounds only for active regions).
+  const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const bool valid = r < n, early = n <= kPrefetchSplit;
+  double e = 0.0, er = 0.0, lo[kMaxDim], wd[kMaxDim];
+  int a = 0;
+  if (early && valid) {
+    e = S.est[r];
+    er = S.err[r];
+    a = S.axis[r];
+#pragma unroll
+    for (int m = 0; m < kMaxDim; ++m)
+      if (m < S.dim) {
+        lo[m] = S.lower[m * S.ld + r];
+        wd[m] = S.width[m * S.ld + r];
+      }
+  }
   // exclusive scan of the active counts, four per thread
   int64_t cnt[4], mine = 0, bad = 0;
 #pragma unroll
@@ -269,13 +309,11 @@ __global__ __launch_bounds__(kBlock) void k_split_fin(SplitArgs S, EvalArgs A, c
   __syncthreads();
   if (nonfinite != 0 || nact > A.max_active || nact > A.alloc_active) return;
 
-  const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  const bool valid = r < n;
-  bool act = false;
-  if (valid) {
-    const double e = S.est[r], er = S.err[r];
-    act = isfinite(e) && isfinite(er) && region_active(e, er, S.epsrel, S.epsabs);
+  if (!early && valid) {
+    e = S.est[r];
+    er = S.err[r];
   }
+  const bool act = valid && isfinite(e) && isfinite(er) && region_active(e, er, S.epsrel, S.epsabs);
   const uint64_t mask = __ballot(act);
   const uint64_t below = lane ? (~0ull >> (kWave - lane)) : 0ull;
   int64_t rank;
@@ -294,22 +332,16 @@ __global__ __launch_bounds__(kBlock) void k_split_fin(SplitArgs S, EvalArgs A, c
   if (!act) return;
   const int64_t blk = r / S.rpb;
   const int64_t dst = (two_level ? s_base[blk / kWave] + S.offsets[blk] : s_base[blk]) + rank;
-  const int a = S.axis[r];
-  for (int m = 0; m < S.dim; ++m) {
-    const double lo = S.lower[m * S.ld + r];
-    const double w = S.width[m * S.ld + r];
-    double2 L, W;
-    if (m == a) {
-      const double hw = 0.5 * w;
-      L = make_double2(lo, lo + hw);
-      W = make_double2(hw, hw);
-    } else {
-      L = make_double2(lo, lo);
-      W = make_double2(w, w);
-    }
-    *reinterpret_cast<double2*>(S.out_lower + m * S.out_ld + 2 * dst) = L;
-    *reinterpret_cast<double2*>(S.out_width + m * S.out_ld + 2 * dst) = W;
+  if (!early) {
+    a = S.axis[r];
+#pragma unroll
+    for (int m = 0; m < kMaxDim; ++m)
+      if (m < S.dim) {
+        lo[m] = S.lower[m * S.ld + r];
+        wd[m] = S.width[m * S.ld + r];
+      }
   }
+  write_children(S, dst, a, lo, wd);
 }
 
 // In place: the dense active counts of nb <= kOneLevel evaluation blocks
