@@ -38,13 +38,13 @@ FP64_PEAK_TFLOPS = 78.6
 # results between points with equal coordinates, so it executes fewer flops
 # than a point-by-point evaluation (region_flops, the model, reported beside
 # it as model_flops_per_region).
-EXEC_FLOPS_PER_REGION_PP8 = 8453  # 8-D product peak, bench workload (model 24,426)
-EXEC_FLOPS_PER_REGION_T6 = 4941   # table-driven 6-D integrand, epsrel 1e-5 (model 16,322)
+EXEC_FLOPS_PER_REGION_PP8 = 8240  # 8-D product peak, bench workload (model 24,426)
+EXEC_FLOPS_PER_REGION_T6 = 4754   # table-driven 6-D integrand, epsrel 1e-5 (model 16,322)
 # measured HBM traffic per rule-evaluation launch of the 8-D product peak on
 # this workload (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE averaged over the
-# 13 launches of a run, FETCH_SIZE scaled x2 and WRITE_SIZE x1 as a 32 MiB
-# copy kernel reads)
-TRAFFIC_PER_LAUNCH = 2 * 10152.2 * 1024 + 4229.4 * 1024
+# 13 launches of a run that evaluate regions, FETCH_SIZE scaled x2 and
+# WRITE_SIZE x1 as a 32 MiB copy kernel reads)
+TRAFFIC_PER_LAUNCH = 2 * 10130.0 * 1024 + 4412.9 * 1024
 
 
 def flop_costs():
