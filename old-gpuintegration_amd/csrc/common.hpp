@@ -101,6 +101,8 @@ struct EvalArgs {
   uint32_t* host_seq;     // written (system-scope release) after host_state
   const int64_t* n_dev;   // when set: the list length, read on the device (n is then
                           // an upper bound: the launch of a queued-ahead iteration)
+  int8_t* axis_out;       // optional (half-wavefront form): a second copy of the split axes
+  uint8_t* mask_out;      // optional (half-wavefront form): per block, bit j = region j stays active
   // host side only: when set, the evaluation kernel's own start and end are
   // recorded in these events (hipExtLaunchKernel), so a timed launch does
   // not include the gap before it reaches the GPU
@@ -124,6 +126,25 @@ struct SpecArgs {
   int64_t left0, cap2;  // budget before the iteration; capacity / 2
   int64_t npts;
   int32_t rpb, pad;
+};
+
+// A half-wavefront evaluation that first does the previous iteration's
+// split for its own regions (k_eval_fused): the previous iteration's
+// finalisation arguments (R: its block partials in R.partials, its dense
+// active counts in R.offsets, the state to publish; P: its queued-ahead
+// arguments, P.in null when its length is R.n) and its list (bounds, split
+// axes and activity masks, 8 regions per mask).  stop_seq: the previous
+// iteration's number, stored there when its finalisation stops the loop.
+struct FuseArgs {
+  EvalArgs R;
+  SpecArgs P;
+  const double* plo;
+  const double* pwd;
+  const int8_t* paxis;
+  const uint8_t* pmask;
+  double* clo;  // this iteration's list (the children's bounds are written there)
+  double* cwd;
+  uint32_t* stop_seq;
 };
 
 // FP64 rounding bound of a result that sums n regions of a p-point rule in
@@ -156,6 +177,11 @@ using EvalLauncher = void (*)(const EvalArgs&, const DevIntegrand&, const RuleCo
 
 // Returns the launcher for (family, dim, rule key 0/7 or 9) or nullptr.
 EvalLauncher find_eval_launcher(int family, int dim, int key = 7);
+
+// Launches k_eval_fused over at most A.n regions (the degree-7/5 rule);
+// nullptr where the family has no such kernel.
+using FusedLauncher = void (*)(const EvalArgs&, const FuseArgs&, const DevIntegrand&, const RuleConst&, hipStream_t);
+FusedLauncher find_fused_launcher(int family, int dim);
 
 // A pseudo-random permutation of [0, n) for the order of training samples,
 // computed per row with no stored order: a bijection of [0, 2^bits) built

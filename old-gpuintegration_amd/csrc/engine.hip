@@ -41,6 +41,15 @@ EvalLauncher launcher_cosine_sum(int, int);
 EvalLauncher launcher_table6(int, int);
 EvalLauncher launcher_lclt(int, int);
 
+FusedLauncher fused_gaussian(int);
+FusedLauncher fused_product_peak(int);
+FusedLauncher fused_oscillatory(int);
+FusedLauncher fused_corner_peak(int);
+FusedLauncher fused_continuous(int);
+FusedLauncher fused_polynomial(int);
+FusedLauncher fused_cosine_sum(int);
+FusedLauncher fused_table6(int);
+
 Phase2Launcher p2launcher_gaussian(int, int);
 Phase2Launcher p2launcher_product_peak(int, int);
 Phase2Launcher p2launcher_oscillatory(int, int);
@@ -84,6 +93,21 @@ EvalLauncher find_eval_launcher(int family, int dim, int key)
   case CUHRE_TABLE6: return launcher_table6(dim, key);
   case CUHRE_LC_LT_T: return launcher_lclt(dim, key);
   default: return nullptr;
+  }
+}
+
+FusedLauncher find_fused_launcher(int family, int dim)
+{
+  switch (family) {
+  case CUHRE_GAUSSIAN: return fused_gaussian(dim);
+  case CUHRE_PRODUCT_PEAK: return fused_product_peak(dim);
+  case CUHRE_OSCILLATORY: return fused_oscillatory(dim);
+  case CUHRE_CORNER_PEAK: return fused_corner_peak(dim);
+  case CUHRE_CONTINUOUS: return fused_continuous(dim);
+  case CUHRE_POLYNOMIAL: return fused_polynomial(dim);
+  case CUHRE_COSINE_SUM: return fused_cosine_sum(dim);
+  case CUHRE_TABLE6: return fused_table6(dim);
+  default: return nullptr;  // LC_LT_t: its iterations keep the split launch
   }
 }
 
@@ -904,6 +928,26 @@ int cuhre_context_create(cuhre_context** out, int device, int64_t capacity_regio
     rc = fail(CUHRE_ERR_DEVICE, "context setup failed: %s", hipGetErrorString(hipGetLastError()));
   }
   if (rc == 0) *c->pub_seq = 0;  // launches are numbered from 1
+  if (rc == 0) {
+    // two sets of per-block partials, dense counts, split axes and masks for
+    // the fused half-wavefront iterations, and the stop word they share
+    const size_t part = al256(sizeof(Partial) * kOneLevel), cnt = al256(sizeof(int64_t) * kOneLevel);
+    const size_t ax = al256((size_t)kOneLevel * (kBlock / kHalfS)), mk = al256(kOneLevel);
+    const size_t set = part + cnt + ax + mk;
+    if (hipMalloc(&c->fuse_mem, 2 * set + 256) != hipSuccess ||
+        hipMemset(c->fuse_mem, 0, 2 * set + 256) != hipSuccess) {
+      rc = fail(CUHRE_ERR_DEVICE, "context setup failed: %s", hipGetErrorString(hipGetLastError()));
+    } else {
+      for (int i = 0; i < 2; ++i) {
+        char* b = c->fuse_mem + i * set;
+        c->fuse[i].part = (Partial*)b;
+        c->fuse[i].cnt = (int64_t*)(b + part);
+        c->fuse[i].axis = (int8_t*)(b + part + cnt);
+        c->fuse[i].mask = (uint8_t*)(b + part + cnt + ax);
+      }
+      c->stop_seq = (uint32_t*)(c->fuse_mem + 2 * set);
+    }
+  }
   for (int i = 0; i < 4 && rc == 0; ++i)
     if (hipEventCreate(&c->ev[i]) != hipSuccess) rc = fail(CUHRE_ERR_DEVICE, "event create failed");
   // the static workspace's address on this device (the no-allocation entry
@@ -931,6 +975,7 @@ int cuhre_context_destroy(cuhre_context* c)
   if (c->p2_ws) (void)hipFree(c->p2_ws);
   if (c->state) (void)hipFree(c->state);
   if (c->spec) (void)hipFree(c->spec);
+  if (c->fuse_mem) (void)hipFree(c->fuse_mem);
   if (c->h_state) (void)hipHostFree(c->h_state);
   if (c->pub_state) (void)hipHostFree(c->pub_state);
   if (c->pub_seq) (void)hipHostFree(c->pub_seq);
@@ -1060,19 +1105,36 @@ int plain_loop(cuhre_context* c, const cuhre_integrand* f, int dim, double epsre
     int S = 1;
     int64_t nb = 0;  // evaluation blocks of the launch
     uint32_t seq = 0;
-    bool slot = false;  // its split writes the next list's slot
+    bool slot = false;       // its finalisation writes the next list's slot
+    bool fuse_next = false;  // the next iteration's k_eval_fused splits and finalises it
+    int lc = 0;              // its list
+    int set = 0;             // its fused-chain buffers (c->fuse[set])
+    EvalArgs A;              // its finalisation's arguments
+    SpecArgs Q;
   };
   const auto fused = [](int64_t nb) {
     return nb <= (int64_t)kOneLevel || (nb + kWave - 1) / kWave <= (int64_t)kWaveFinalize;
   };
+  const FusedLauncher fuse_launch = find_fused_launcher(f->family, dim);
+  // Half-wavefront iterations queued ahead form a chain: each writes its
+  // partials, dense counts, split axes and activity masks to one of two
+  // buffer sets, and the next one (k_eval_fused) finds its regions from
+  // them, finalising and splitting it in the same launch.
+  const auto chains = [&](int S, int64_t bnd) {
+    return ahead && fuse_launch != nullptr && key != 9 && c->segments == 0 && S == kHalfS && bnd <= c->cap / 2;
+  };
   // Queues the evaluation and split of list lc: nl regions (exact) or at
   // most nl, the length then read on the device; bnd picks the segments;
-  // nev: evaluations of the iterations before.
-  const auto queue = [&](int lc, int64_t nl, int64_t bnd, bool exact, int64_t nev, Launched& L) -> int {
+  // nev: evaluations of the iterations before; iter: the iteration.  With
+  // parent set, the evaluation is k_eval_fused over the parent's children.
+  const auto queue = [&](int lc, int64_t nl, int64_t bnd, bool exact, int64_t nev, int iter, const Launched* parent,
+                         Launched& L) -> int {
     L.S = pick_segments(c, bnd, key, f->family, dim);
     const int rpb = seg_regions_per_block(L.S);
     L.nb = blocks_for(nl, L.S);
     if (L.nb > 0x7fffffffll) return fail(CUHRE_ERR_CAPACITY, "grid too large");
+    L.lc = lc;
+    L.set = iter & 1;
     EvalArgs A;
     fill_eval_args(c, A, lc, nl, dim, epsrel, epsabs);
     A.finalize = 1;
@@ -1094,6 +1156,15 @@ int plain_loop(cuhre_context* c, const cuhre_integrand* f, int dim, double epsre
       Q.cap2 = capacity / 2;
       Q.npts = p;
     }
+    const bool chain = chains(L.S, bnd);
+    L.fuse_next = chain && iter + 1 < opt.max_iter && 2 * bnd <= c->half_max_regions &&
+                  pick_segments(c, 2 * bnd, key, f->family, dim) == kHalfS;
+    if (chain) {
+      A.partials = c->fuse[L.set].part;
+      A.offsets = c->fuse[L.set].cnt;
+      A.axis_out = c->fuse[L.set].axis;
+      A.mask_out = c->fuse[L.set].mask;
+    }
     L.slot = ahead && fused(L.nb);
     if (L.slot) Q.out = &c->spec[1 - lc];
     Q.rpb = rpb;
@@ -1111,8 +1182,28 @@ int plain_loop(cuhre_context* c, const cuhre_integrand* f, int dim, double epsre
       A.prof_start = c->ev[0];
       A.prof_stop = c->ev[1];
     }
-    launch(A, P, R, L.S, c->stream);
+    if (parent != nullptr) {
+      FuseArgs F;
+      F.R = parent->A;
+      F.P = parent->Q;
+      F.plo = c->lo[parent->lc];
+      F.pwd = c->wd[parent->lc];
+      F.paxis = c->fuse[parent->set].axis;
+      F.pmask = c->fuse[parent->set].mask;
+      F.clo = c->lo[lc];
+      F.cwd = c->wd[lc];
+      F.stop_seq = c->stop_seq;
+      fuse_launch(A, F, P, R, c->stream);
+    } else {
+      launch(A, P, R, L.S, c->stream);
+    }
     A.prof_start = A.prof_stop = nullptr;
+    L.A = A;
+    L.Q = Q;
+    if (L.fuse_next) {  // split and finalised by the next iteration's evaluation
+      HIP_TRY(hipGetLastError());
+      return 0;
+    }
     // the split's blocks and one more for the finalisation
     const dim3 sgrid((unsigned)((nl + kBlock - 1) / kBlock + 1));
     const int64_t G = (L.nb + kWave - 1) / kWave;
@@ -1137,13 +1228,17 @@ int plain_loop(cuhre_context* c, const cuhre_integrand* f, int dim, double epsre
   bool queued = false;  // iteration it is already in the stream (as next)
   for (int it = 0; it < opt.max_iter; ++it) {
     if (queued) L = next;
-    else if ((rc = queue(cur, n, bound, true, out->neval, L))) return rc;
+    else if ((rc = queue(cur, n, bound, true, out->neval, it, nullptr, L))) return rc;
     queued = false;
-    if (L.slot && it + 1 < opt.max_iter) {
+    // iteration it+1 is queued now; it must be when its evaluation is to
+    // split and finalise iteration it (its state is published from there)
+    if (L.fuse_next || (L.slot && it + 1 < opt.max_iter)) {
       const int S2 = pick_segments(c, 2 * n, key, f->family, dim);
       const int64_t nb2 = blocks_for(2 * n, S2);
-      if (fused(nb2) && nb2 <= block_capacity(c->cap)) {
-        if ((rc = queue(1 - cur, 2 * n, 2 * n, false, out->neval + n * p, next))) return rc;
+      if (L.fuse_next || (fused(nb2) && nb2 <= block_capacity(c->cap))) {
+        if ((rc = queue(1 - cur, 2 * n, 2 * n, false, out->neval + n * p, it + 1, L.fuse_next ? &L : nullptr,
+                        next)))
+          return rc;
         queued = true;
       }
     }

@@ -5,5 +5,6 @@
 
 namespace cuhre {
 EvalLauncher launcher_continuous(int dim, int key) { return family_launcher<CUHRE_CONTINUOUS>(dim, key); }
+FusedLauncher fused_continuous(int dim) { return family_fused_launcher<CUHRE_CONTINUOUS>(dim); }
 Phase2Launcher p2launcher_continuous(int dim, int key) { return phase2_launcher<CUHRE_CONTINUOUS>(dim, key); }
 }  // namespace cuhre

@@ -5,5 +5,6 @@
 
 namespace cuhre {
 EvalLauncher launcher_corner_peak(int dim, int key) { return family_launcher<CUHRE_CORNER_PEAK>(dim, key); }
+FusedLauncher fused_corner_peak(int dim) { return family_fused_launcher<CUHRE_CORNER_PEAK>(dim); }
 Phase2Launcher p2launcher_corner_peak(int dim, int key) { return phase2_launcher<CUHRE_CORNER_PEAK>(dim, key); }
 }  // namespace cuhre

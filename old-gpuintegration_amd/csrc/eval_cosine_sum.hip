@@ -5,5 +5,6 @@
 
 namespace cuhre {
 EvalLauncher launcher_cosine_sum(int dim, int key) { return family_launcher<CUHRE_COSINE_SUM>(dim, key); }
+FusedLauncher fused_cosine_sum(int dim) { return family_fused_launcher<CUHRE_COSINE_SUM>(dim); }
 Phase2Launcher p2launcher_cosine_sum(int dim, int key) { return phase2_launcher<CUHRE_COSINE_SUM>(dim, key); }
 }  // namespace cuhre

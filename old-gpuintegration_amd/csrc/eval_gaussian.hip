@@ -5,5 +5,6 @@
 
 namespace cuhre {
 EvalLauncher launcher_gaussian(int dim, int key) { return family_launcher<CUHRE_GAUSSIAN>(dim, key); }
+FusedLauncher fused_gaussian(int dim) { return family_fused_launcher<CUHRE_GAUSSIAN>(dim); }
 Phase2Launcher p2launcher_gaussian(int dim, int key) { return phase2_launcher<CUHRE_GAUSSIAN>(dim, key); }
 }  // namespace cuhre

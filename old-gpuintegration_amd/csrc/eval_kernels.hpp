@@ -727,6 +727,41 @@ __device__ __forceinline__ RegionOut seg_eval9(const F& f, const double* __restr
   return o;
 }
 
+// Outputs of a half-wavefront evaluation block b (8 regions, owner lane s
+// = 0 of each): the per-region results, optionally a second copy of the
+// split axes and the block's activity mask (bit j: region j stays active,
+// the split's test), and the block's partial sums and dense active count.
+__device__ __forceinline__ void half_outputs(const EvalArgs& A, const RegionOut& o, int64_t r, bool valid,
+                                             bool owner, uint32_t b)
+{
+  if (valid && owner) {
+    A.est[r] = o.est;
+    A.err[r] = o.err;
+    A.axis[r] = (int8_t)o.axis;
+    if (A.abssum) A.abssum[r] = o.ab;
+    if (A.i5) A.i5[r] = o.i5;
+    if (A.axis_out) A.axis_out[r] = (int8_t)o.axis;
+  }
+  if (A.mask_out != nullptr) {
+    __shared__ uint32_t s_mask;
+    if (threadIdx.x == 0) s_mask = 0;
+    __syncthreads();
+    if (valid && owner && isfinite(o.est) && isfinite(o.err) && region_active(o.est, o.err, A.epsrel, A.epsabs))
+      atomicOr(&s_mask, 1u << (threadIdx.x / kHalfS));
+    __syncthreads();
+    if (threadIdx.x == 0) A.mask_out[b] = (uint8_t)s_mask;
+  }
+  if (A.finalize) {
+    Partial p;
+    block_reduce_regions(owner, threadIdx.x / kHalfS, kBlock / kHalfS, o.est, o.err, o.ab, valid, A.loc_epsrel,
+                         A.loc_epsabs, p);
+    if (threadIdx.x == 0) {
+      A.partials[b] = p;
+      A.offsets[b] = p.nactive + (p.nnonfinite > 0 ? kBadFlag : 0);
+    }
+  }
+}
+
 // Half-wavefront form (S = kHalfS): 8 regions per 256-thread block.
 template <int D, int FAM, int KEY>
 __global__ __launch_bounds__(256) void k_eval_half(EvalArgs A, DevIntegrand P, RuleConst R)
@@ -744,20 +779,7 @@ __global__ __launch_bounds__(256) void k_eval_half(EvalArgs A, DevIntegrand P, R
     o = seg_eval9<D, true>(f, A.lower, A.width, A.ld, r, valid, rr, kHalfS, s, 0, lane, 0, nullptr, R.r9, A.dvec);
   else
     o = seg_eval<D, true>(f, A.lower, A.width, A.ld, r, valid, rr, kHalfS, s, 0, lane, 0, nullptr, R, A.dvec);
-  const bool owner = s == 0;
-  if (valid && owner) {
-    A.est[r] = o.est;
-    A.err[r] = o.err;
-    A.axis[r] = (int8_t)o.axis;
-    if (A.abssum) A.abssum[r] = o.ab;
-    if (A.i5) A.i5[r] = o.i5;
-  }
-  if (A.finalize) {
-    Partial p;
-    block_reduce_regions(owner, threadIdx.x / kHalfS, kBlock / kHalfS, o.est, o.err, o.ab, valid, A.loc_epsrel,
-                         A.loc_epsabs, p);
-    store_block_partial(A, p);
-  }
+  half_outputs(A, o, r, valid, s == 0, blockIdx.x);
 }
 
 template <int D, int FAM, int KEY>
@@ -766,6 +788,183 @@ void launch_half(const EvalArgs& A, const DevIntegrand& P, const RuleConst& R, h
   const int64_t nb = (A.n + kBlock / kHalfS - 1) / (kBlock / kHalfS);
   hipExtLaunchKernelGGL((k_eval_half<D, FAM, KEY>), dim3((unsigned)nb), dim3(kBlock), 0, st, A.prof_start,
                         A.prof_stop, 0, A, P, R);
+}
+
+// The half-wavefront evaluation of a list whose regions are the children of
+// the previous list's active regions, found here instead of by a separate
+// split launch (lists of at most 8192 regions queued ahead of the host: one
+// launch per iteration instead of two).  Block 0 finalises the previous
+// iteration as k_split_fin's extra block does (its partials, the stop
+// decision, the published state, the next list's slot).  Every other block
+// scans the previous iteration's dense active counts, takes the same count
+// test as the split (budget, buffers, non-finite values; the loop then
+// stops and the block exits), and for each of its 8 regions r finds the
+// parent: the (r/2)-th active region, in the previous block whose count
+// scan covers r/2, at the position its activity mask gives.  It bisects the
+// parent along its split axis exactly as the split does (child r/2*2 takes
+// the lower half), writes the child's bounds to this list, and evaluates
+// the child from those values; results, axes, mask and partials follow as
+// in k_eval_half.  Regions, their order and every result are those of the
+// split followed by k_eval_half.  When block 0 finds that the previous
+// iteration stops the loop (its global test) it stores that iteration's
+// number in stop_seq, and blocks that see it before their point loops
+// skip them (the outputs are then unused).
+template <int D, int FAM>
+__global__ __launch_bounds__(256) void k_eval_fused(EvalArgs A, FuseArgs F, DevIntegrand P, RuleConst R)
+{
+  __shared__ int64_t s_base[kOneLevel];
+  __shared__ int64_t s_wsum[kBlock / kWave], s_wbad[kBlock / kWave];
+  __shared__ int s_stop;
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  const int s = threadIdx.x & (kHalfS - 1);
+  constexpr int kRpb = kBlock / kHalfS;  // regions per block, both lists
+  EvalArgs RA = F.R;
+  int64_t pn = RA.n;
+  if (F.P.in != nullptr) {
+    pn = F.P.in->n;
+    if (pn == 0) return;  // the loop stopped before the previous iteration
+    RA.fin_est_s = F.P.in->fin_s;
+    RA.fin_est_c = F.P.in->fin_c;
+    RA.fin_err_s = F.P.in->fer_s;
+    RA.fin_err_c = F.P.in->fer_c;
+    const int64_t left = F.P.left0 - pn * F.P.npts;
+    const int64_t budget = left >= 0 ? left / (2 * F.P.npts) : -1;
+    RA.max_active = budget < F.P.cap2 ? budget : F.P.cap2;
+  }
+  const uint32_t pnb = (uint32_t)((pn + kRpb - 1) / kRpb);
+  if (blockIdx.x == 0) {
+    Partial q;
+    reduce_partials(RA.partials, pnb, nullptr, q);
+    if (threadIdx.x == 0) {
+      const IterState st = iteration_state(RA, q);
+      if (st.stop != 0) __hip_atomic_store(F.stop_seq, RA.seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      write_state(RA, st);
+      if (F.P.out != nullptr) {
+        SpecSlot o;
+        o.n = st.stop == 0 ? 2 * q.nactive : 0;
+        o.pad = 0;
+        o.fin_s = RA.fin_est_s;
+        o.fin_c = RA.fin_est_c;
+        o.fer_s = RA.fin_err_s;
+        o.fer_c = RA.fin_err_c;
+        dd_add(o.fin_s, o.fin_c, q.fest_s, q.fest_c);
+        dd_add(o.fer_s, o.fer_c, q.ferr_s, q.ferr_c);
+        *F.P.out = o;
+      }
+    }
+    return;
+  }
+  const uint32_t b = blockIdx.x - 1;
+  // exclusive scan of the previous iteration's dense counts, four per thread
+  int64_t cnt[4], mine = 0, bad = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const uint32_t j = 4 * threadIdx.x + i;
+    cnt[i] = 0;
+    if (j < pnb) {
+      const int64_t v = RA.offsets[j];
+      cnt[i] = v & kCountMask;
+      bad += v >> 40;
+    }
+    mine += cnt[i];
+  }
+  long long incl = (long long)mine, nbad = (long long)bad;
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const long long v = __shfl_up(incl, off, kWave);
+    if (lane >= off) incl += v;
+  }
+#pragma unroll
+  for (int mask = 1; mask < kWave; mask <<= 1) nbad += __shfl_xor(nbad, mask, kWave);
+  if (lane == kWave - 1) s_wsum[wave] = (int64_t)incl;
+  if (lane == 0) s_wbad[wave] = (int64_t)nbad;
+  __syncthreads();
+  int64_t before = 0, nact = 0, nonfinite = 0;
+  for (int w = 0; w < kBlock / kWave; ++w) {
+    if (w < wave) before += s_wsum[w];
+    nact += s_wsum[w];
+    nonfinite += s_wbad[w];
+  }
+  int64_t run = before + (int64_t)incl - mine;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    if (4 * threadIdx.x + i < pnb) s_base[4 * threadIdx.x + i] = run;
+    run += cnt[i];
+  }
+  __syncthreads();
+  if (nonfinite != 0 || nact == 0 || nact > RA.max_active || nact > RA.alloc_active) return;
+  const int64_t n = 2 * nact;
+  if ((int64_t)b * kRpb >= n) {  // beyond this list
+    if (threadIdx.x == 0) {
+      Partial z;
+      partial_zero(z);
+      A.partials[b] = z;
+      A.offsets[b] = 0;
+      if (A.mask_out) A.mask_out[b] = 0;
+    }
+    return;
+  }
+  const int64_t r = (int64_t)b * kRpb + threadIdx.x / kHalfS;
+  const bool valid = r < n;
+  const int64_t rr = valid ? r : n - 1;
+  // the parent: the (rr/2)-th active region of the previous list
+  const int64_t k = rr >> 1;
+  uint32_t lo_b = 0, hi_b = pnb;
+  while (hi_b - lo_b > 1) {
+    const uint32_t mid = (lo_b + hi_b) >> 1;
+    if (s_base[mid] <= k) lo_b = mid;
+    else hi_b = mid;
+  }
+  int kk = (int)(k - s_base[lo_b]);
+  const uint32_t pm = F.pmask[lo_b];
+  int pos = 0;
+  for (int j = 0; j < kRpb; ++j)
+    if ((pm >> j) & 1u) {
+      if (kk == 0) {
+        pos = j;
+        break;
+      }
+      --kk;
+    }
+  const int64_t par = (int64_t)lo_b * kRpb + pos;
+  const int a = F.paxis[par];
+  double lo[D], wd[D];
+#pragma unroll
+  for (int m = 0; m < D; ++m) {
+    lo[m] = F.plo[m * A.ld + par];
+    wd[m] = F.pwd[m * A.ld + par];
+  }
+  const bool upper = (rr & 1) != 0;
+#pragma unroll
+  for (int m = 0; m < D; ++m)
+    if (m == a) {
+      const double hw = 0.5 * wd[m];
+      if (upper) lo[m] = lo[m] + hw;
+      wd[m] = hw;
+    }
+  if (valid) {
+#pragma unroll
+    for (int m = 0; m < D; ++m)
+      if (s == m) {
+        F.clo[m * A.ld + r] = lo[m];
+        F.cwd[m * A.ld + r] = wd[m];
+      }
+  }
+  if (threadIdx.x == 0)
+    s_stop = __hip_atomic_load(F.stop_seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == RA.seq;
+  __syncthreads();
+  if (s_stop) return;
+  const PlainIntegrand<D, FAM> f{P};
+  const RegionOut o = seg_eval<D, true>(f, lo, wd, 1, r, valid, 0, kHalfS, s, 0, lane, 0, nullptr, R, A.dvec);
+  half_outputs(A, o, r, valid, s == 0, b);
+}
+
+template <int D, int FAM>
+void launch_fused(const EvalArgs& A, const FuseArgs& F, const DevIntegrand& P, const RuleConst& R, hipStream_t st)
+{
+  // the list's blocks and block 0 for the previous iteration's finalisation
+  const int64_t nb = (A.n + kBlock / kHalfS - 1) / (kBlock / kHalfS) + 1;
+  hipLaunchKernelGGL((k_eval_fused<D, FAM>), dim3((unsigned)nb), dim3(kBlock), 0, st, A, F, P, R);
 }
 
 template <int D, int FAM>
@@ -812,6 +1011,23 @@ void launch_eval9(const EvalArgs& A, const DevIntegrand& P, const RuleConst& R, 
 }
 
 // Launchers for one family over dimensions 2..10 (key 9: the degree-9 rule).
+template <int FAM>
+FusedLauncher family_fused_launcher(int dim)
+{
+  switch (dim) {
+  case 2: return &launch_fused<2, FAM>;
+  case 3: return &launch_fused<3, FAM>;
+  case 4: return &launch_fused<4, FAM>;
+  case 5: return &launch_fused<5, FAM>;
+  case 6: return &launch_fused<6, FAM>;
+  case 7: return &launch_fused<7, FAM>;
+  case 8: return &launch_fused<8, FAM>;
+  case 9: return &launch_fused<9, FAM>;
+  case 10: return &launch_fused<10, FAM>;
+  default: return nullptr;
+  }
+}
+
 template <int FAM>
 EvalLauncher family_launcher(int dim, int key = 7)
 {

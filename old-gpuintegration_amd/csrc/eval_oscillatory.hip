@@ -5,5 +5,6 @@
 
 namespace cuhre {
 EvalLauncher launcher_oscillatory(int dim, int key) { return family_launcher<CUHRE_OSCILLATORY>(dim, key); }
+FusedLauncher fused_oscillatory(int dim) { return family_fused_launcher<CUHRE_OSCILLATORY>(dim); }
 Phase2Launcher p2launcher_oscillatory(int dim, int key) { return phase2_launcher<CUHRE_OSCILLATORY>(dim, key); }
 }  // namespace cuhre

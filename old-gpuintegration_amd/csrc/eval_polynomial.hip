@@ -5,5 +5,6 @@
 
 namespace cuhre {
 EvalLauncher launcher_polynomial(int dim, int key) { return family_launcher<CUHRE_POLYNOMIAL>(dim, key); }
+FusedLauncher fused_polynomial(int dim) { return family_fused_launcher<CUHRE_POLYNOMIAL>(dim); }
 Phase2Launcher p2launcher_polynomial(int dim, int key) { return phase2_launcher<CUHRE_POLYNOMIAL>(dim, key); }
 }  // namespace cuhre

@@ -5,5 +5,6 @@
 
 namespace cuhre {
 EvalLauncher launcher_product_peak(int dim, int key) { return family_launcher<CUHRE_PRODUCT_PEAK>(dim, key); }
+FusedLauncher fused_product_peak(int dim) { return family_fused_launcher<CUHRE_PRODUCT_PEAK>(dim); }
 Phase2Launcher p2launcher_product_peak(int dim, int key) { return phase2_launcher<CUHRE_PRODUCT_PEAK>(dim, key); }
 }  // namespace cuhre

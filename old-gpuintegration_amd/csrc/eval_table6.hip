@@ -12,6 +12,7 @@ EvalLauncher launcher_table6(int dim, int key)
   if (dim != 6) return nullptr;
   return key == 9 ? &launch_eval9<6, CUHRE_TABLE6> : &launch_eval<6, CUHRE_TABLE6>;
 }
+FusedLauncher fused_table6(int dim) { return dim == 6 ? &launch_fused<6, CUHRE_TABLE6> : nullptr; }
 Phase2Launcher p2launcher_table6(int dim, int key)
 {
   if (dim != 6) return nullptr;
