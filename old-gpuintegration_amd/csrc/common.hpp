@@ -101,8 +101,8 @@ struct EvalArgs {
   uint32_t* host_seq;     // written (system-scope release) after host_state
   const int64_t* n_dev;   // when set: the list length, read on the device (n is then
                           // an upper bound: the launch of a queued-ahead iteration)
-  int8_t* axis_out;       // optional (half-wavefront form): a second copy of the split axes
-  uint8_t* mask_out;      // optional (half-wavefront form): per block, bit j = region j stays active
+  uint8_t* act_out;       // optional (half-wavefront form): per block of 8, its active
+                          // regions in order, (position << 4) | split axis
   // host side only: when set, the evaluation kernel's own start and end are
   // recorded in these events (hipExtLaunchKernel), so a timed launch does
   // not include the gap before it reaches the GPU
@@ -132,19 +132,16 @@ struct SpecArgs {
 // split for its own regions (k_eval_fused): the previous iteration's
 // finalisation arguments (R: its block partials in R.partials, its dense
 // active counts in R.offsets, the state to publish; P: its queued-ahead
-// arguments, P.in null when its length is R.n) and its list (bounds, split
-// axes and activity masks, 8 regions per mask).  stop_seq: the previous
-// iteration's number, stored there when its finalisation stops the loop.
+// arguments, P.in null when its length is R.n) and its list (bounds, and
+// per block of 8 regions the list of its active ones, act_out above).
 struct FuseArgs {
   EvalArgs R;
   SpecArgs P;
   const double* plo;
   const double* pwd;
-  const int8_t* paxis;
-  const uint8_t* pmask;
+  const uint8_t* pact;
   double* clo;  // this iteration's list (the children's bounds are written there)
   double* cwd;
-  uint32_t* stop_seq;
 };
 
 // FP64 rounding bound of a result that sums n regions of a p-point rule in

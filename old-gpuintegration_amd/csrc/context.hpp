@@ -77,16 +77,14 @@ struct cuhre_context {
   int queue_ahead = 1;
   cuhre::SpecSlot* spec = nullptr;
   // half-wavefront iterations queued ahead alternate between these two sets
-  // of block partials, dense counts, split axes and activity masks, so an
+  // of block partials, dense counts and active-region lists, so an
   // evaluation can split its parent list while it writes its own
   struct FuseSet {
     cuhre::Partial* part = nullptr;  // kOneLevel blocks
     int64_t* cnt = nullptr;
-    int8_t* axis = nullptr;  // kOneLevel * 8 regions
-    uint8_t* mask = nullptr;
+    uint8_t* act = nullptr;  // kOneLevel * 8 regions
   } fuse[2];
   char* fuse_mem = nullptr;
-  uint32_t* stop_seq = nullptr;
   double* aux = nullptr;  // i5 (cap) + dvec (cap*dim) on request
   int64_t aux_cap = 0;
   char* p2_ws = nullptr;  // Phase-2 workspace of the two-phase method, kept between calls

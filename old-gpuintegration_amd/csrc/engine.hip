@@ -929,23 +929,20 @@ int cuhre_context_create(cuhre_context** out, int device, int64_t capacity_regio
   }
   if (rc == 0) *c->pub_seq = 0;  // launches are numbered from 1
   if (rc == 0) {
-    // two sets of per-block partials, dense counts, split axes and masks for
-    // the fused half-wavefront iterations, and the stop word they share
+    // two sets of per-block partials, dense counts and active-region lists
+    // for the fused half-wavefront iterations
     const size_t part = al256(sizeof(Partial) * kOneLevel), cnt = al256(sizeof(int64_t) * kOneLevel);
-    const size_t ax = al256((size_t)kOneLevel * (kBlock / kHalfS)), mk = al256(kOneLevel);
-    const size_t set = part + cnt + ax + mk;
-    if (hipMalloc(&c->fuse_mem, 2 * set + 256) != hipSuccess ||
-        hipMemset(c->fuse_mem, 0, 2 * set + 256) != hipSuccess) {
+    const size_t act = al256((size_t)kOneLevel * (kBlock / kHalfS));
+    const size_t set = part + cnt + act;
+    if (hipMalloc(&c->fuse_mem, 2 * set) != hipSuccess || hipMemset(c->fuse_mem, 0, 2 * set) != hipSuccess) {
       rc = fail(CUHRE_ERR_DEVICE, "context setup failed: %s", hipGetErrorString(hipGetLastError()));
     } else {
       for (int i = 0; i < 2; ++i) {
         char* b = c->fuse_mem + i * set;
         c->fuse[i].part = (Partial*)b;
         c->fuse[i].cnt = (int64_t*)(b + part);
-        c->fuse[i].axis = (int8_t*)(b + part + cnt);
-        c->fuse[i].mask = (uint8_t*)(b + part + cnt + ax);
+        c->fuse[i].act = (uint8_t*)(b + part + cnt);
       }
-      c->stop_seq = (uint32_t*)(c->fuse_mem + 2 * set);
     }
   }
   for (int i = 0; i < 4 && rc == 0; ++i)
@@ -1162,8 +1159,7 @@ int plain_loop(cuhre_context* c, const cuhre_integrand* f, int dim, double epsre
     if (chain) {
       A.partials = c->fuse[L.set].part;
       A.offsets = c->fuse[L.set].cnt;
-      A.axis_out = c->fuse[L.set].axis;
-      A.mask_out = c->fuse[L.set].mask;
+      A.act_out = c->fuse[L.set].act;
     }
     L.slot = ahead && fused(L.nb);
     if (L.slot) Q.out = &c->spec[1 - lc];
@@ -1188,11 +1184,9 @@ int plain_loop(cuhre_context* c, const cuhre_integrand* f, int dim, double epsre
       F.P = parent->Q;
       F.plo = c->lo[parent->lc];
       F.pwd = c->wd[parent->lc];
-      F.paxis = c->fuse[parent->set].axis;
-      F.pmask = c->fuse[parent->set].mask;
+      F.pact = c->fuse[parent->set].act;
       F.clo = c->lo[lc];
       F.cwd = c->wd[lc];
-      F.stop_seq = c->stop_seq;
       fuse_launch(A, F, P, R, c->stream);
     } else {
       launch(A, P, R, L.S, c->stream);

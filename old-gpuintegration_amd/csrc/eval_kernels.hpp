@@ -728,9 +728,10 @@ __device__ __forceinline__ RegionOut seg_eval9(const F& f, const double* __restr
 }
 
 // Outputs of a half-wavefront evaluation block b (8 regions, owner lane s
-// = 0 of each): the per-region results, optionally a second copy of the
-// split axes and the block's activity mask (bit j: region j stays active,
-// the split's test), and the block's partial sums and dense active count.
+// = 0 of each): the per-region results, optionally the block's list of
+// active regions (the split's test) in order, entry i = (position in the
+// block << 4) | split axis of its i-th active region, and the block's
+// partial sums and dense active count.
 __device__ __forceinline__ void half_outputs(const EvalArgs& A, const RegionOut& o, int64_t r, bool valid,
                                              bool owner, uint32_t b)
 {
@@ -740,16 +741,17 @@ __device__ __forceinline__ void half_outputs(const EvalArgs& A, const RegionOut&
     A.axis[r] = (int8_t)o.axis;
     if (A.abssum) A.abssum[r] = o.ab;
     if (A.i5) A.i5[r] = o.i5;
-    if (A.axis_out) A.axis_out[r] = (int8_t)o.axis;
   }
-  if (A.mask_out != nullptr) {
+  if (A.act_out != nullptr) {
     __shared__ uint32_t s_mask;
     if (threadIdx.x == 0) s_mask = 0;
     __syncthreads();
-    if (valid && owner && isfinite(o.est) && isfinite(o.err) && region_active(o.est, o.err, A.epsrel, A.epsabs))
-      atomicOr(&s_mask, 1u << (threadIdx.x / kHalfS));
+    const int j = threadIdx.x / kHalfS;
+    const bool act =
+        valid && owner && isfinite(o.est) && isfinite(o.err) && region_active(o.est, o.err, A.epsrel, A.epsabs);
+    if (act) atomicOr(&s_mask, 1u << j);
     __syncthreads();
-    if (threadIdx.x == 0) A.mask_out[b] = (uint8_t)s_mask;
+    if (act) A.act_out[b * (kBlock / kHalfS) + __popc(s_mask & ((1u << j) - 1u))] = (uint8_t)((j << 4) | o.axis);
   }
   if (A.finalize) {
     Partial p;
@@ -800,21 +802,18 @@ void launch_half(const EvalArgs& A, const DevIntegrand& P, const RuleConst& R, h
 // test as the split (budget, buffers, non-finite values; the loop then
 // stops and the block exits), and for each of its 8 regions r finds the
 // parent: the (r/2)-th active region, in the previous block whose count
-// scan covers r/2, at the position its activity mask gives.  It bisects the
+// scan covers r/2, at the position (and with the split axis) that block's
+// list of active regions gives.  It bisects the
 // parent along its split axis exactly as the split does (child r/2*2 takes
 // the lower half), writes the child's bounds to this list, and evaluates
-// the child from those values; results, axes, mask and partials follow as
-// in k_eval_half.  Regions, their order and every result are those of the
-// split followed by k_eval_half.  When block 0 finds that the previous
-// iteration stops the loop (its global test) it stores that iteration's
-// number in stop_seq, and blocks that see it before their point loops
-// skip them (the outputs are then unused).
+// the child from those values; results, active lists and partials follow
+// as in k_eval_half.  Regions, their order and every result are those of
+// the split followed by k_eval_half.
 template <int D, int FAM>
 __global__ __launch_bounds__(256) void k_eval_fused(EvalArgs A, FuseArgs F, DevIntegrand P, RuleConst R)
 {
   __shared__ int64_t s_base[kOneLevel];
   __shared__ int64_t s_wsum[kBlock / kWave], s_wbad[kBlock / kWave];
-  __shared__ int s_stop;
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
   const int s = threadIdx.x & (kHalfS - 1);
   constexpr int kRpb = kBlock / kHalfS;  // regions per block, both lists
@@ -837,7 +836,6 @@ __global__ __launch_bounds__(256) void k_eval_fused(EvalArgs A, FuseArgs F, DevI
     reduce_partials(RA.partials, pnb, nullptr, q);
     if (threadIdx.x == 0) {
       const IterState st = iteration_state(RA, q);
-      if (st.stop != 0) __hip_atomic_store(F.stop_seq, RA.seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       write_state(RA, st);
       if (F.P.out != nullptr) {
         SpecSlot o;
@@ -900,7 +898,6 @@ __global__ __launch_bounds__(256) void k_eval_fused(EvalArgs A, FuseArgs F, DevI
       partial_zero(z);
       A.partials[b] = z;
       A.offsets[b] = 0;
-      if (A.mask_out) A.mask_out[b] = 0;
     }
     return;
   }
@@ -915,19 +912,9 @@ __global__ __launch_bounds__(256) void k_eval_fused(EvalArgs A, FuseArgs F, DevI
     if (s_base[mid] <= k) lo_b = mid;
     else hi_b = mid;
   }
-  int kk = (int)(k - s_base[lo_b]);
-  const uint32_t pm = F.pmask[lo_b];
-  int pos = 0;
-  for (int j = 0; j < kRpb; ++j)
-    if ((pm >> j) & 1u) {
-      if (kk == 0) {
-        pos = j;
-        break;
-      }
-      --kk;
-    }
-  const int64_t par = (int64_t)lo_b * kRpb + pos;
-  const int a = F.paxis[par];
+  const uint32_t e = F.pact[lo_b * kRpb + (uint32_t)(k - s_base[lo_b])];
+  const int64_t par = (int64_t)lo_b * kRpb + (e >> 4);
+  const int a = (int)(e & 15u);
   double lo[D], wd[D];
 #pragma unroll
   for (int m = 0; m < D; ++m) {
@@ -950,10 +937,6 @@ __global__ __launch_bounds__(256) void k_eval_fused(EvalArgs A, FuseArgs F, DevI
         F.cwd[m * A.ld + r] = wd[m];
       }
   }
-  if (threadIdx.x == 0)
-    s_stop = __hip_atomic_load(F.stop_seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == RA.seq;
-  __syncthreads();
-  if (s_stop) return;
   const PlainIntegrand<D, FAM> f{P};
   const RegionOut o = seg_eval<D, true>(f, lo, wd, 1, r, valid, 0, kHalfS, s, 0, lane, 0, nullptr, R, A.dvec);
   half_outputs(A, o, r, valid, s == 0, b);
