@@ -4,7 +4,9 @@ C-ABI: every result field must be identical (the segment count of a launch
 follows from the bound 2 n of the previous list and the reductions from
 the real list length in both forms).  The cases stop on every condition:
 convergence, the evaluation budget, list growth on the way (the lists
-start small and grow when the children do not fit) and a non-finite value.
+start small and grow when the children do not fit) and a non-finite value,
+also while the first iterations run as fused half-wavefront launches (each
+finding its regions from the previous list instead of a split launch).
 """
 import json
 import os
@@ -44,6 +46,14 @@ def _cases():
         ("table6d", W.c3_table6(), {}),
         ("corner7d", W.c5_corner7(), {"maxeval": 10**9}),
         ("lc_lt_t3", lclt, {}),
+        # the first iterations run as a chain of fused half-wavefront launches:
+        # the smallest and largest initial grids, and a budget that stops the
+        # loop inside the chain
+        ("gaussian2d", W.Problem("g2", W.gaussian([6.0, 9.0], [0.3, 0.6]), [0.0] * 2, [1.0] * 2, 1e-9, 0.0,
+                                 10**9), {}),
+        ("product_peak10d", W.Problem("pp10", W.product_peak([3.0] * 10, [0.2 + 0.05 * i for i in range(10)]),
+                                      [0.0] * 10, [1.0] * 10, 1e-6, 0.0, 10**8), {}),
+        ("product_peak8d_budget_in_chain", W.c2_product_peak8(), {"maxeval": 401 * 1500}),
     ]
 
 
