@@ -40,6 +40,10 @@ FP64_PEAK_TFLOPS = 78.6
 # it as model_flops_per_region).
 EXEC_FLOPS_PER_REGION_PP8 = 8240  # 8-D product peak, bench workload (model 24,426)
 EXEC_FLOPS_PER_REGION_T6 = 4754   # table-driven 6-D integrand, epsrel 1e-5 (model 16,322)
+# the batched evaluation (k_batch_eval, the three 5-D families of the
+# batch workload) on the autotuner's sample: executed FP64 flops per region,
+# same counters (profiles/batch_c4_5d.md)
+EXEC_FLOPS_PER_REGION_BATCH5 = 2988
 # measured HBM traffic per rule-evaluation launch of the 8-D product peak on
 # this workload (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE averaged over the
 # 13 launches of a run that evaluate regions, FETCH_SIZE scaled x2 and
@@ -209,7 +213,22 @@ def bench_batched(ctx, count=4096):
     same = sum(1 for a, b in zip(res, plain) if a["nregions"] == b["nregions"]
                and abs(a["integral"] - b["integral"]) <= 1e-12 * abs(b["integral"]))
     regions = sum(r["nregions"] for r in res)
+    # roofline of the batched evaluation: the autotuner times one evaluation
+    # of its sample (every instance's initial grid refined uniformly until
+    # the list holds 2^19 regions or more) per launch shape with HIP events
+    d = probs[0].f.dim
+    k = 4 if d <= 4 else 2
+    n_sample = count * k ** d
+    while n_sample < (1 << 19):
+        n_sample *= 2
+    best_ms = float(min(table))
+    eval_tf = EXEC_FLOPS_PER_REGION_BATCH5 * n_sample / (best_ms * 1e-3) / 1e12
     return {"instances": count, "batched_s": tb, "plain_sequential_s": tp, "wall_ratio": tb / tp,
+            "eval_roofline": {"bound": "fp64", "achieved_tflops": eval_tf, "frac": eval_tf / FP64_PEAK_TFLOPS,
+                              "flops_per_region": EXEC_FLOPS_PER_REGION_BATCH5, "sample_regions": n_sample,
+                              "ms_per_evaluation": best_ms,
+                              "note": "k_batch_eval over the autotuner's sample at its chosen shape (HIP "
+                                      "events); executed FP64 flops per region from the VALU counters"},
             "batched_evals_per_s": info["total_neval"] / tb,
             "plain_evals_per_s": sum(r["neval"] for r in plain) / tp,
             "regions": regions, "regions_plain": sum(r["nregions"] for r in plain),
