@@ -48,7 +48,7 @@ EXEC_FLOPS_PER_REGION_BATCH5 = 2988
 # this workload (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE averaged over the
 # 13 launches of a run that evaluate regions, FETCH_SIZE scaled x2 and
 # WRITE_SIZE x1 as a 32 MiB copy kernel reads)
-TRAFFIC_PER_LAUNCH = 2 * 10130.0 * 1024 + 4412.9 * 1024
+TRAFFIC_PER_LAUNCH = 2 * 10142.1 * 1024 + 4465.5 * 1024
 
 
 def flop_costs():
