@@ -63,8 +63,9 @@ inline int seg_regions_per_block(int S)
 __device__ __forceinline__ int64_t eval_len(const EvalArgs& A) { return A.n_dev != nullptr ? *A.n_dev : A.n; }
 
 // A block of a queued-ahead launch that lies wholly beyond the list (first:
-// its first region) stores a zero partial and exits; when the list is empty
-// (the loop stopped before this iteration) it writes nothing.
+// its first region) stores a zero partial and a zero active count and
+// exits; when the list is empty (the loop stopped before this iteration)
+// it writes nothing.
 __device__ __forceinline__ bool block_beyond(const EvalArgs& A, int64_t n, int64_t first)
 {
   if (A.n_dev == nullptr || first < n) return false;

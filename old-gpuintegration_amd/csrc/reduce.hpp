@@ -5,14 +5,16 @@
 // copy per iteration (quad/GPUquad/GPUKernelquad.cu:565-577) and counts the
 // active regions with thrust::exclusive_scan + two device-to-host copies
 // (:364-386).  Here each evaluation block folds its regions into one Partial
-// (wave shuffles, then LDS across the waves) and stores it; a one-block
-// kernel queued behind the evaluation (k_finish) reduces the partials in a
-// fixed order, scans the per-block active counts into split offsets and
-// takes the stop decision, so an iteration needs no host sync.  (A fused
-// last-block ticket was measured first: the acq_rel atomic on one counter
-// serialises at ~23 ns per block, 12 us at 512 blocks and ~190 us at 8192,
-// against ~3 us for the extra launch.)  Sums are carried as (value,
-// rounding error) pairs built with TwoSum.
+// (block_reduce_regions: field by field in rows of 32 lanes from LDS) and
+// stores it with its dense active count; one block launched behind the
+// evaluation (the split's extra block in the plain loop, k_finish in the
+// guided loop) reduces the partials in a fixed order (reduce_partials) and
+// takes the stop decision, while the split's blocks scan the counts for
+// their offsets, so an iteration needs no host sync.  (A fused last-block
+// ticket was measured first: the acq_rel atomic on one counter serialises
+// at ~23 ns per block, 12 us at 512 blocks and ~190 us at 8192, against ~3
+// us for the extra launch.)  Sums are carried as (value, rounding error)
+// pairs built with TwoSum.
 #pragma once
 
 #include "common.hpp"
@@ -96,29 +98,9 @@ __device__ __forceinline__ bool region_active(double est, double err, double eps
   return err > fmax(epsrel * fabs(est), epsabs);
 }
 
-__device__ __forceinline__ void partial_from_region(Partial& p, double est, double err,
-                                                    double ab, bool valid, double epsrel,
-                                                    double epsabs)
-{
-  partial_zero(p);
-  if (!valid) return;
-  const bool finite = isfinite(est) && isfinite(err);
-  const bool act = finite && region_active(est, err, epsrel, epsabs);
-  p.est_s = est;
-  p.err_s = err;
-  p.abs_all = ab;
-  p.aest_all = fabs(est);
-  if (!act) {
-    p.fest_s = est;
-    p.ferr_s = err;
-    p.abs_fin = ab;
-    p.aest_fin = fabs(est);
-  }
-  p.nactive = act ? 1 : 0;
-  p.nnonfinite = finite ? 0 : 1;
-}
-
-// The same with the region's classification given (act: still active).
+// One region's contribution to a Partial, its classification given (act:
+// still active; a region with a non-finite est or err counts in nnonfinite
+// and in the finished sums).
 __device__ __forceinline__ void partial_from_region_act(Partial& p, double est, double err, double ab, bool act)
 {
   partial_zero(p);
@@ -167,8 +149,10 @@ __device__ __forceinline__ void block_reduce_partial(Partial& p, Partial* ldsw, 
 // then eight rows of 32 threads add one field each (est, err, their
 // finished parts, the two |.| sums and their finished parts): slots k,
 // k+32, ... in order into 32 sums, combined pairwise (partners k xor 1, 2,
-// 4, 8, 16).  The counts come from wavefront ballots.  Each region's values
-// and classification are those of partial_from_region.
+// 4, 8, 16).  The counts come from wavefront ballots.  A region counts as
+// active when its est and err are finite and err > max(epsrel |est|,
+// epsabs); one with a non-finite value counts in nnonfinite and, like every
+// inactive region, in the finished sums.
 struct RegionSlots {
   double est[kBlock], err[kBlock], ab[kBlock];
   uint8_t fin[kBlock];  // the region is finished (its values count in the finished sums)
