@@ -121,6 +121,21 @@ __device__ __forceinline__ void half_argmax(double& d, int& a)
   }
 }
 
+// e[m] = h[m] at the coordinate i that a point set moves and 0 elsewhere.
+// With i a per-lane value (the half-wavefront form), a point's coordinates
+// x[m] = fma(e[m], g, c[m]) are c[m] + h[m] g where m == i and c[m] plus a
+// signed zero, i.e. c[m] itself, elsewhere: the same values as
+// fma(h[m], m == i ? g : 0, c[m]), with the index test paid once per set
+// instead of at every coordinate of every point.  Nested for two or three
+// moved coordinates (pairs, triples), the inner results are c[m] exactly
+// wherever an outer coordinate moves.
+template <int D>
+__device__ __forceinline__ void moved_coord(double (&e)[D], const double (&h)[D], int i)
+{
+#pragma unroll
+  for (int m = 0; m < D; ++m) e[m] = m == i ? h[m] : 0.0;
+}
+
 // The four axis points of axis i (a compile-time index once inlined in an
 // unrolled loop: only x[i] leaves the centre) and that axis' fourth
 // difference.
@@ -569,7 +584,8 @@ __device__ __forceinline__ RegionOut seg_eval9(const F& f, const double* __restr
   // axis sets 1..4
 #pragma unroll 1
   for (int i = s; i < D; i += S) {
-    double fv[2][2];
+    double fv[2][2], e[D];
+    if constexpr (HALF) moved_coord(e, h, i);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       double fs = 0.0, fa = 0.0;
@@ -577,7 +593,12 @@ __device__ __forceinline__ RegionOut seg_eval9(const F& f, const double* __restr
       for (int sg = 0; sg < 2; ++sg) {
         const double gv = sg ? -R.t[k] : R.t[k];
 #pragma unroll
-        for (int m = 0; m < D; ++m) x[m] = fma(h[m], m == i ? gv : 0.0, c[m]);
+        for (int m = 0; m < D; ++m) {
+          if constexpr (HALF)
+            x[m] = fma(e[m], gv, c[m]);
+          else
+            x[m] = fma(h[m], m == i ? gv : 0.0, c[m]);
+        }
         const double v = f(x);
         if (k < 2) fv[k][sg] = v;
         fs += v;
@@ -603,12 +624,26 @@ __device__ __forceinline__ RegionOut seg_eval9(const F& f, const double* __restr
       ++i;
     }
     const int j = i + 1 + rem;
+    double ei[D], ej[D];
+    if constexpr (HALF) {
+      moved_coord(ei, h, i);
+      moved_coord(ej, h, j);
+    }
+    // x at (gi, gj) on axes (i, j), the centre elsewhere
+    const auto pair_point = [&](double gi, double gj) {
+#pragma unroll
+      for (int m = 0; m < D; ++m) {
+        if constexpr (HALF)
+          x[m] = fma(ei[m], gi, fma(ej[m], gj, c[m]));
+        else
+          x[m] = fma(h[m], m == i ? gi : (m == j ? gj : 0.0), c[m]);
+      }
+    };
     double fs = 0.0, fa = 0.0;
 #pragma unroll
     for (int sg = 0; sg < 4; ++sg) {
       const double gi = (sg & 1) ? -R.t[0] : R.t[0], gj = (sg & 2) ? -R.t[0] : R.t[0];
-#pragma unroll
-      for (int m = 0; m < D; ++m) x[m] = fma(h[m], m == i ? gi : (m == j ? gj : 0.0), c[m]);
+      pair_point(gi, gj);
       const double v = f(x);
       fs += v;
       fa += fabs(v);
@@ -622,8 +657,7 @@ __device__ __forceinline__ RegionOut seg_eval9(const F& f, const double* __restr
       for (int sg = 0; sg < 4; ++sg) {
         const double ti = o ? R.t[1] : R.t[0], tj = o ? R.t[0] : R.t[1];
         const double gi = (sg & 1) ? -ti : ti, gj = (sg & 2) ? -tj : tj;
-#pragma unroll
-        for (int m = 0; m < D; ++m) x[m] = fma(h[m], m == i ? gi : (m == j ? gj : 0.0), c[m]);
+        pair_point(gi, gj);
         const double v = f(x);
         fs += v;
         fa += fabs(v);
@@ -646,14 +680,24 @@ __device__ __forceinline__ RegionOut seg_eval9(const F& f, const double* __restr
       ++j;
     }
     const int k = j + 1 + rem;
+    double ei[D], ej[D], ek[D];
+    if constexpr (HALF) {
+      moved_coord(ei, h, i);
+      moved_coord(ej, h, j);
+      moved_coord(ek, h, k);
+    }
     double fs = 0.0, fa = 0.0;
 #pragma unroll 2
     for (int sg = 0; sg < 8; ++sg) {
       const double gi = (sg & 1) ? -R.t[0] : R.t[0], gj = (sg & 2) ? -R.t[0] : R.t[0],
                    gk = (sg & 4) ? -R.t[0] : R.t[0];
 #pragma unroll
-      for (int m = 0; m < D; ++m)
-        x[m] = fma(h[m], m == i ? gi : (m == j ? gj : (m == k ? gk : 0.0)), c[m]);
+      for (int m = 0; m < D; ++m) {
+        if constexpr (HALF)
+          x[m] = fma(ei[m], gi, fma(ej[m], gj, fma(ek[m], gk, c[m])));
+        else
+          x[m] = fma(h[m], m == i ? gi : (m == j ? gj : (m == k ? gk : 0.0)), c[m]);
+      }
       const double v = f(x);
       fs += v;
       fa += fabs(v);
@@ -664,13 +708,44 @@ __device__ __forceinline__ RegionOut seg_eval9(const F& f, const double* __restr
   // corner set 8
   {
     double fs = 0.0, fa = 0.0;
-#pragma unroll 2
-    for (int k = s; k < (1 << D); k += S) {
+    if constexpr (HALF && D > 5) {
+      // corners k = s, s + 32, ... in increasing order: bits 0..4 of k are
+      // the lane's s (those coordinates are set once), the next (up to
+      // three) are unrolled, so an integrand that accumulates over the
+      // coordinates in order computes its partial result over the first five
+      // once for the unrolled corners
+      constexpr int LT = D - 5;            // bits of k above the lane's five
+      constexpr int LU = LT < 3 ? LT : 3;  // of which unrolled
+      double cpu[LU], cmu[LU];
 #pragma unroll
-      for (int m = 0; m < D; ++m) x[m] = fma(h[m], ((k >> m) & 1) ? -R.t[4] : R.t[4], c[m]);
-      const double v = f(x);
-      fs += v;
-      fa += fabs(v);
+      for (int m = 0; m < 5; ++m) x[m] = fma(h[m], ((s >> m) & 1) ? -R.t[4] : R.t[4], c[m]);
+#pragma unroll
+      for (int b = 0; b < LU; ++b) {
+        cpu[b] = fma(h[5 + b], R.t[4], c[5 + b]);
+        cmu[b] = fma(h[5 + b], -R.t[4], c[5 + b]);
+      }
+#pragma unroll 1
+      for (int g = 0; g < (1 << (LT - LU)); ++g) {
+#pragma unroll
+        for (int m = 5 + LU; m < D; ++m) x[m] = fma(h[m], ((g >> (m - 5 - LU)) & 1) ? -R.t[4] : R.t[4], c[m]);
+#pragma unroll
+        for (int u = 0; u < (1 << LU); ++u) {
+#pragma unroll
+          for (int b = 0; b < LU; ++b) x[5 + b] = ((u >> b) & 1) ? cmu[b] : cpu[b];
+          const double v = f(x);
+          fs += v;
+          fa += fabs(v);
+        }
+      }
+    } else {
+#pragma unroll 2
+      for (int k = s; k < (1 << D); k += S) {
+#pragma unroll
+        for (int m = 0; m < D; ++m) x[m] = fma(h[m], ((k >> m) & 1) ? -R.t[4] : R.t[4], c[m]);
+        const double v = f(x);
+        fs += v;
+        fa += fabs(v);
+      }
     }
     group(8, fs, fa);
   }

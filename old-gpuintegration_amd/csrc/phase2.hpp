@@ -54,6 +54,8 @@ struct Phase2Args {
 
 // Pool slots whose keys (error, creation number) live in LDS.
 constexpr int kP2LdsSlots = 256;
+// Rows of 64 pool slots whose keys one step of the max-error scan loads.
+constexpr int kScanBatch = 8;
 
 __device__ __forceinline__ double read_lane_d(double v, int l)
 {
@@ -127,15 +129,39 @@ __global__ __launch_bounds__(64) void k_phase2(Phase2Args A, DevIntegrand P, Rul
   while (count <= A.pool_max && (count == 1 || ERR > fmax(A.epsrel * fabs(RES), A.epsabs))) {
     __syncthreads();  // the records written by lanes 0 and 32 are visible to the wave
     // the slot to bisect: largest error, earliest created among equal errors
+    // (a pool of more than 64 slots is read kScanBatch rows of 64 slots at
+    // a time, so that a deep pool's keys in global memory arrive together
+    // instead of one row per round trip; the order among candidates is a
+    // strict total order, so the result does not depend on the order of the
+    // comparisons)
     double be = -INFINITY;
     int32_t bq = INT32_MAX, bs = -1;
-    for (int32_t t = lane; t < count; t += kWave) {
-      const double e = key_err(t);
-      const int32_t q = key_seq(t);
-      if (e > be || (e == be && q < bq)) {
-        be = e;
-        bq = q;
-        bs = t;
+    if (count <= kWave) {
+      if (lane < count) {
+        const double e = key_err(lane);
+        if (e > be) {  // false for a NaN key, as in the batched form
+          be = e;
+          bq = key_seq(lane);
+          bs = lane;
+        }
+      }
+    } else {
+      for (int32_t t0 = lane; t0 < count; t0 += kScanBatch * kWave) {
+        double e[kScanBatch];
+        int32_t q[kScanBatch];
+#pragma unroll
+        for (int u = 0; u < kScanBatch; ++u) {
+          const int32_t t = t0 + u * kWave;
+          e[u] = t < count ? key_err(t) : -INFINITY;
+          q[u] = t < count ? key_seq(t) : INT32_MAX;
+        }
+#pragma unroll
+        for (int u = 0; u < kScanBatch; ++u)
+          if (e[u] > be || (e[u] == be && q[u] < bq)) {
+            be = e[u];
+            bq = q[u];
+            bs = t0 + u * kWave;
+          }
       }
     }
 #pragma unroll
