@@ -1363,10 +1363,63 @@ int plain_loop(cuhre_context* c, const cuhre_integrand* f, int dim, double epsre
 
 }  // namespace
 
+// Launch order of Phase 2.  A region's run ends when its own error meets
+// its tolerance; its length grows with the ratio of its first error to that
+// tolerance, so the regions are dispatched by decreasing log2 of the ratio
+// (kP2Buckets buckets, counting sort): the longest chains start first and
+// the short ones fill the chip around them instead of a long chain starting
+// last.  Results are written per region, so the order changes only when
+// each one runs.
+constexpr int kP2Buckets = 64;
+
+__global__ void k_p2_zero(uint32_t* p, int count)
+{
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < count) p[t] = 0u;
+}
+
+__global__ __launch_bounds__(256) void k_p2_bucket(const double* est, const double* err, int64_t n, double epsrel,
+                                                   double epsabs, uint8_t* bucket, uint32_t* hist)
+{
+  __shared__ uint32_t h[kP2Buckets];
+  if (threadIdx.x < kP2Buckets) h[threadIdx.x] = 0u;
+  __syncthreads();
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) {
+    // ratio <= 1 or NaN: bucket 0; above 2^62 (a zero tolerance): the last
+    const double q = err[r] / fmax(epsrel * fabs(est[r]), epsabs);
+    const int b = q > 1.0 ? (q <= 0x1p62 ? ilogb(q) + 1 : kP2Buckets - 1) : 0;
+    bucket[r] = (uint8_t)b;
+    atomicAdd(&h[b], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x < kP2Buckets && h[threadIdx.x] != 0u) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
+}
+
+__global__ __launch_bounds__(256) void k_p2_scatter(const uint8_t* bucket, int64_t n, const uint32_t* hist,
+                                                    uint32_t* cursor, int32_t* order)
+{
+  __shared__ uint32_t base[kP2Buckets];
+  if (threadIdx.x == 0) {
+    uint32_t s = 0;
+    for (int b = kP2Buckets - 1; b >= 0; --b) {
+      base[b] = s;
+      s += hist[b];
+    }
+  }
+  __syncthreads();
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) {
+    const int b = bucket[r];
+    order[base[b] + atomicAdd(&cursor[b], 1u)] = (int32_t)r;
+  }
+}
+
 // Phase 2 of the reference's two-phase method on the n regions of list
-// `cur` (SoA, ld = the context's capacity): one wavefront per region
-// (k_phase2), per-region outputs copied to the host.  The workspace is the
-// context's and kept between calls.
+// `cur` (SoA, ld = the context's capacity): the regions' first evaluation by
+// the half-wavefront kernel, their launch order, then one wavefront per
+// region (k_phase2), per-region outputs copied to the host.  The workspace
+// is the context's and kept between calls.
 struct Phase2Host {
   std::vector<double> res, err, rab, rres;
   std::vector<int32_t> cnt;
@@ -1385,7 +1438,11 @@ static int run_phase2(cuhre_context* c, const cuhre_integrand* f, int dim, int k
   const int64_t slots = (int64_t)pool_max + 2;
   const size_t per_slot = sizeof(double) * (2 * (size_t)dim + 2) + sizeof(int8_t) + sizeof(int32_t);
   const size_t ws = (size_t)n * (size_t)slots * per_slot;
-  const size_t outb = (size_t)n * (4 * sizeof(double) + sizeof(int32_t) + 3) + 8 * 256;
+  // outputs, then the first evaluations (est, err, abssum, axis), the
+  // launch order with its buckets and the bucket counters
+  const size_t outb = (size_t)n * (4 * sizeof(double) + sizeof(int32_t) + 3) +
+                      (size_t)n * (3 * sizeof(double) + sizeof(int8_t) + sizeof(int32_t) + 1) +
+                      2 * kP2Buckets * sizeof(uint32_t) + 16 * 256;
   const size_t need = ws + outb + 8 * 256;
   if (c->p2_bytes < need) {
     size_t free_b = 0, total_b = 0;
@@ -1433,6 +1490,45 @@ static int run_phase2(cuhre_context* c, const cuhre_integrand* f, int dim, int k
   A.epsrel = epsrel;
   A.epsabs = epsabs;
   A.pool_max = pool_max;
+  A.order = nullptr;
+  A.est0 = A.err0 = A.ab0 = nullptr;
+  A.ax0 = nullptr;
+  const EvalLauncher first = find_eval_launcher(f->family, dim, key);
+  if (first != nullptr) {
+    double* est0 = (double*)take(sizeof(double) * n);
+    double* err0 = (double*)take(sizeof(double) * n);
+    double* ab0 = (double*)take(sizeof(double) * n);
+    int8_t* ax0 = (int8_t*)take(n);
+    int32_t* order = (int32_t*)take(sizeof(int32_t) * n);
+    uint8_t* bucket = (uint8_t*)take(n);
+    uint32_t* hist = (uint32_t*)take(2 * kP2Buckets * sizeof(uint32_t));
+    uint32_t* cursor = hist + kP2Buckets;
+    EvalArgs E;
+    std::memset(&E, 0, sizeof E);
+    E.lower = A.lower;
+    E.width = A.width;
+    E.ld = A.ld;
+    E.n = n;
+    E.epsrel = E.loc_epsrel = epsrel;
+    E.epsabs = E.loc_epsabs = epsabs;
+    E.est = est0;
+    E.err = err0;
+    E.axis = ax0;
+    E.abssum = ab0;
+    E.finalize = 0;  // per-region outputs only
+    first(E, P, R, kHalfS, c->stream);
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_p2_zero, dim3(1), dim3(2 * kP2Buckets), 0, c->stream, hist, 2 * kP2Buckets);
+    hipLaunchKernelGGL(k_p2_bucket, dim3(nb), dim3(256), 0, c->stream, (const double*)est0, (const double*)err0, n,
+                       epsrel, epsabs, bucket, hist);
+    hipLaunchKernelGGL(k_p2_scatter, dim3(nb), dim3(256), 0, c->stream, (const uint8_t*)bucket, n,
+                       (const uint32_t*)hist, cursor, order);
+    A.order = order;
+    A.est0 = est0;
+    A.err0 = err0;
+    A.ab0 = ab0;
+    A.ax0 = ax0;
+  }
   H.res.resize(n);
   H.err.resize(n);
   H.rab.resize(n);

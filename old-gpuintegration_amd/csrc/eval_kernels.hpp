@@ -606,7 +606,9 @@ __device__ __forceinline__ RegionOut seg_eval9(const F& f, const double* __restr
       }
       group(1 + k, fs, fa);
     }
-    const double di = fabs(2.0 * (1.0 - R.ratio) * f0 + R.ratio * (fv[0][0] + fv[0][1]) - (fv[1][0] + fv[1][1]));
+    // (products and sums fused explicitly, so that every kernel built on
+    // this evaluator rounds them alike)
+    const double di = fabs(fma(R.ratio, fv[0][0] + fv[0][1], 2.0 * (1.0 - R.ratio) * f0) - (fv[1][0] + fv[1][1]));
     if (dvec != nullptr && valid) dvec[i * ld + r] = di;
     if (di > maxd) {
       maxd = di;
@@ -791,7 +793,7 @@ __device__ __forceinline__ RegionOut seg_eval9(const F& f, const double* __restr
   for (int q = 1; q < 4; ++q) {
     double mx = 0.0;
 #pragma unroll
-    for (int t = 0; t < 9; ++t) mx = fmax(mx, fabs(sum[q + 1] + R.scale[t][q] * sum[q]) * R.norm[t][q]);
+    for (int t = 0; t < 9; ++t) mx = fmax(mx, fabs(fma(R.scale[t][q], sum[q], sum[q + 1])) * R.norm[t][q]);
     e[q] = mx;
   }
   RegionOut o;

@@ -18,8 +18,12 @@
 // slice of global memory ([slot][lower D, width D]), written by lanes 0 and
 // 32 and read back by the whole wave after a barrier.  A region stops when its own
 // sum meets max(epsrel |RESULT|, epsabs) or after pool_max bisections.
-// Per-region run lengths differ by orders of magnitude; with one wave per
-// region the hardware dispatcher balances them.
+// Per-region run lengths differ by orders of magnitude: the host evaluates
+// every region once beforehand with the half-wavefront kernel (the same
+// evaluator) and launches the regions by decreasing ratio of that first
+// error to the region's tolerance (Phase2Args::order), so the longest runs
+// start first and the hardware dispatcher fills the chip with the short
+// ones around them.
 #pragma once
 
 #include "eval_kernels.hpp"
@@ -50,6 +54,15 @@ struct Phase2Args {
   // 2 (|child 0| + |child 1| + |parent|) + |RES| (each update rounds three times)
   double* rab;
   double* rres;
+  // optional: block b refines region order[b] (the regions expected to run
+  // longest first, so that their chains do not start last)
+  const int32_t* order;
+  // optional: each region's first evaluation, made beforehand by the
+  // half-wavefront kernel (the same evaluator, so the same values)
+  const double* est0;
+  const double* err0;
+  const double* ab0;
+  const int8_t* ax0;
 };
 
 // Pool slots whose keys (error, creation number) live in LDS.
@@ -82,8 +95,10 @@ __global__ __launch_bounds__(64) void k_phase2(Phase2Args A, DevIntegrand P, Rul
 {
   __shared__ double s_err[kP2LdsSlots];
   __shared__ int32_t s_seq[kP2LdsSlots];
-  double* const gerr = A.gkerr + blockIdx.x * A.slots;
-  int32_t* const gseq = A.gkseq + blockIdx.x * A.slots;
+  // the region of the input list this block refines, and its workspace slice
+  const int64_t g = A.order != nullptr ? (int64_t)A.order[blockIdx.x] : (int64_t)blockIdx.x;
+  double* const gerr = A.gkerr + g * A.slots;
+  int32_t* const gseq = A.gkseq + g * A.slots;
   auto key_err = [&](int32_t t) { return t < kP2LdsSlots ? s_err[t] : gerr[t]; };
   auto key_seq = [&](int32_t t) { return t < kP2LdsSlots ? s_seq[t] : gseq[t]; };
   auto set_key = [&](int32_t t, double e, int32_t q) {
@@ -95,7 +110,6 @@ __global__ __launch_bounds__(64) void k_phase2(Phase2Args A, DevIntegrand P, Rul
       gseq[t] = q;
     }
   };
-  const int64_t g = blockIdx.x;  // one block per region of the input list
   const int lane = threadIdx.x;
   const int half = lane >> 5;
   double* const geo = A.geo + g * A.slots * (2 * D);
@@ -109,11 +123,24 @@ __global__ __launch_bounds__(64) void k_phase2(Phase2Args A, DevIntegrand P, Rul
     lo[m] = A.lower[m * A.ld + g];
     wd[m] = A.width[m * A.ld + g];
   }
-  // both halves evaluate the input region; lane 0 keeps it in slot 0
-  const RegionOut o = p2_eval<D, FAM, KEY>(f, R, lo, wd, lane & 31);
-  const double e0 = read_lane_d(o.est, 0), r0 = read_lane_d(o.err, 0);
+  // the input region's first evaluation (given, or by both halves here);
+  // lane 0 keeps it in slot 0
+  double e0, r0, rab;
+  int ax0;
+  if (A.est0 != nullptr) {
+    e0 = A.est0[g];
+    r0 = A.err0[g];
+    rab = A.ab0[g];
+    ax0 = A.ax0[g];
+  } else {
+    const RegionOut o = p2_eval<D, FAM, KEY>(f, R, lo, wd, lane & 31);
+    e0 = read_lane_d(o.est, 0);
+    r0 = read_lane_d(o.err, 0);
+    rab = read_lane_d(o.ab, 0);
+    ax0 = o.axis;
+  }
   bool bad = !(isfinite(e0) && isfinite(r0));
-  double rab = read_lane_d(o.ab, 0), rres = 0.0;
+  double rres = 0.0;
   if (lane == 0) {
 #pragma unroll
     for (int m = 0; m < D; ++m) {
@@ -121,7 +148,7 @@ __global__ __launch_bounds__(64) void k_phase2(Phase2Args A, DevIntegrand P, Rul
       geo[D + m] = wd[m];
     }
     sest[0] = e0;
-    sax[0] = (int8_t)o.axis;
+    sax[0] = (int8_t)ax0;
     set_key(0, r0, 0);
   }
   int32_t count = 1, next = 1;
