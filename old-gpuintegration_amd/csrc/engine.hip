@@ -1396,10 +1396,14 @@ __global__ __launch_bounds__(256) void k_p2_bucket(const double* est, const doub
   if (threadIdx.x < kP2Buckets && h[threadIdx.x] != 0u) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
 }
 
+// Each block ranks its regions within their buckets in LDS and reserves
+// one range per bucket in the global order (one global atomic per block
+// and bucket: the regions crowd into a few buckets).
 __global__ __launch_bounds__(256) void k_p2_scatter(const uint8_t* bucket, int64_t n, const uint32_t* hist,
                                                     uint32_t* cursor, int32_t* order)
 {
-  __shared__ uint32_t base[kP2Buckets];
+  __shared__ uint32_t base[kP2Buckets], cnt[kP2Buckets];
+  if (threadIdx.x < kP2Buckets) cnt[threadIdx.x] = 0u;
   if (threadIdx.x == 0) {
     uint32_t s = 0;
     for (int b = kP2Buckets - 1; b >= 0; --b) {
@@ -1409,10 +1413,17 @@ __global__ __launch_bounds__(256) void k_p2_scatter(const uint8_t* bucket, int64
   }
   __syncthreads();
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int b = -1;
+  uint32_t rank = 0;
   if (r < n) {
-    const int b = bucket[r];
-    order[base[b] + atomicAdd(&cursor[b], 1u)] = (int32_t)r;
+    b = bucket[r];
+    rank = atomicAdd(&cnt[b], 1u);
   }
+  __syncthreads();
+  if (threadIdx.x < kP2Buckets && cnt[threadIdx.x] != 0u)
+    base[threadIdx.x] += atomicAdd(&cursor[threadIdx.x], cnt[threadIdx.x]);
+  __syncthreads();
+  if (b >= 0) order[base[b] + rank] = (int32_t)r;
 }
 
 // Phase 2 of the reference's two-phase method on the n regions of list
