@@ -173,7 +173,10 @@ __device__ __forceinline__ void gm_axis(const F& f, const double (&c)[D], const 
 // (it contains a block barrier).  F(x) evaluates the lane's integrand.
 // HALF: S = 32 segments of one region in the 32 lanes of a half wavefront
 // (s = lane mod 32), merged by half_sum / half_argmax instead of LDS.
-template <int D, bool HALF = false, bool ONE = false, class F>
+// MOVED (half form only): axis and pair points take their coordinates from
+// per-set moved-coordinate vectors (moved_coord) instead of per-coordinate
+// selects: fewer instructions, more registers.
+template <int D, bool HALF = false, bool ONE = false, bool MOVED = HALF, class F>
 __device__ __forceinline__ RegionOut seg_eval(const F& f, const double* __restrict__ lower,
                                               const double* __restrict__ width, int64_t ld,
                                               int64_t r, bool valid, int64_t rr, int S, int s,
@@ -225,12 +228,18 @@ __device__ __forceinline__ RegionOut seg_eval(const F& f, const double* __restri
     // axes i = s, s + S, ... (in a half, s varies across its lanes)
 #pragma unroll 1
     for (int i = s; i < D; i += S) {
-      double fv[4];
+      double fv[4], e[D];
+      if constexpr (MOVED) moved_coord(e, h, i);
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         const double gv = t == 0 ? R.l2 : (t == 1 ? -R.l2 : (t == 2 ? R.l3 : -R.l3));
 #pragma unroll
-        for (int m = 0; m < D; ++m) x[m] = fma(h[m], m == i ? gv : 0.0, c[m]);
+        for (int m = 0; m < D; ++m) {
+          if constexpr (MOVED)
+            x[m] = fma(e[m], gv, c[m]);
+          else
+            x[m] = fma(h[m], m == i ? gv : 0.0, c[m]);
+        }
         fv[t] = f(x);
         const double w7 = t < 2 ? R.w7a2 : R.w7a3;
         const double w5 = t < 2 ? R.w5a2 : R.w5a3;
@@ -274,12 +283,22 @@ __device__ __forceinline__ RegionOut seg_eval(const F& f, const double* __restri
         ++i;
       }
       const int j = i + 1 + rem;
+      double ei[D], ej[D];
+      if constexpr (MOVED) {
+        moved_coord(ei, h, i);
+        moved_coord(ej, h, j);
+      }
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         const double gi = t < 2 ? R.l4 : -R.l4;
         const double gj = (t & 1) ? -R.l4 : R.l4;
 #pragma unroll
-        for (int m = 0; m < D; ++m) x[m] = fma(h[m], m == i ? gi : (m == j ? gj : 0.0), c[m]);
+        for (int m = 0; m < D; ++m) {
+          if constexpr (MOVED)
+            x[m] = fma(ei[m], gi, fma(ej[m], gj, c[m]));
+          else
+            x[m] = fma(h[m], m == i ? gi : (m == j ? gj : 0.0), c[m]);
+        }
         const double v = f(x);
         s7 = fma(R.w7p, v, s7);
         s5 = fma(R.w5p, v, s5);

@@ -87,7 +87,9 @@ __device__ __forceinline__ RegionOut p2_eval(const PlainIntegrand<D, FAM>& f, co
   if constexpr (KEY == 9)
     return seg_eval9<D, true>(f, lo, wd, 1, 0, true, 0, 32, s, 0, s, 0, nullptr, R.r9, nullptr);
   else
-    return seg_eval<D, true>(f, lo, wd, 1, 0, true, 0, 32, s, 0, s, 0, nullptr, R, nullptr);
+    // (without the moved-coordinate vectors: their registers cost this
+    // kernel a wavefront per SIMD)
+    return seg_eval<D, true, false, false>(f, lo, wd, 1, 0, true, 0, 32, s, 0, s, 0, nullptr, R, nullptr);
 }
 
 template <int D, int FAM, int KEY>
