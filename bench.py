@@ -39,6 +39,13 @@ FP64_PEAK_TFLOPS = 78.6
 # than a point-by-point evaluation (region_flops, the model, reported beside
 # it as model_flops_per_region).
 EXEC_FLOPS_PER_REGION_PP8 = 8240  # 8-D product peak, bench workload (model 24,426)
+# Issue ceiling of that evaluation: per region it issues 5,158 full-rate
+# VALU instructions and 402 v_rcp_f64 (one reciprocal per point), and the
+# reciprocal issues at about a quarter of the FMA rate (flopcost's issue-rate
+# kernels: 4.24 against 1.24 CU-cycles per wave instruction,
+# profiles/flop_costs_gfx950.md), so at most 8,240 flops per
+# 2 * (5,158 + 4 * 402) issue slots, 0.61 of the FP64 peak, is reachable.
+ISSUE_CEILING_PP8 = 8240 / (2 * (5158 + 4 * 402))
 EXEC_FLOPS_PER_REGION_T6 = 4754   # table-driven 6-D integrand, epsrel 1e-5 (model 16,322)
 # the batched evaluation (k_batch_eval, the three 5-D families of the
 # batch workload) on the autotuner's sample: executed FP64 flops per region,
@@ -537,6 +544,8 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "fp64", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": TRAFFIC_PER_LAUNCH,
+                     "issue_ceiling_frac": ISSUE_CEILING_PP8,
+                     "frac_of_issue_ceiling": achieved / FP64_PEAK_TFLOPS / ISSUE_CEILING_PP8,
                      "flops_per_region": xfl,
                      "flops_note": "executed FP64 flops per region counted by SQ_INSTS_VALU_*_F64 "
                                    "(profiles/eval_product_peak8d.md); the point-by-point model "
@@ -553,7 +562,8 @@ def main():
                      "launches": ev_launches, "avg_launch_ms": avg_ms,
                      "regions_per_launch": ev_regions / ev_launches,
                      "largest_launch": {"regions": big["largest_regions"], "ms": big["ms_eval_largest"],
-                                        "tflops": big_tf, "frac": big_tf / FP64_PEAK_TFLOPS}},
+                                        "tflops": big_tf, "frac": big_tf / FP64_PEAK_TFLOPS,
+                                        "frac_of_issue_ceiling": big_tf / FP64_PEAK_TFLOPS / ISSUE_CEILING_PP8}},
         "per_integrand": per,
         "queue_ahead": {"ms_per_step": dt / args.steps * 1e3,
                         "ms_per_step_one_iteration_at_a_time": dt_serial / args.steps * 1e3,
