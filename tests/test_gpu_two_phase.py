@@ -136,6 +136,27 @@ def test_phase2_per_region_matches_oracle(ctx, key, mk):
     assert worst_r <= 1e-12 and worst_e <= 1e-8
 
 
+@pytest.mark.parametrize("key", [9, 7])
+def test_phase2_results_independent_of_list_order(ctx, key):
+    """Phase 2 launches the regions by decreasing ratio of first error to
+    tolerance and writes each region's results at its own index: the same
+    regions given in another order give every region's result, error and
+    pool size bit for bit."""
+    p = W.c2_product_peak8()
+    d = p.f.dim
+    rng = np.random.default_rng(17)
+    n = 3000
+    L = rng.uniform(0.0, 0.8, (n, d))
+    Wd = rng.uniform(0.02, 0.2, (n, d))
+    res, err, cnt = ctx.phase2_regions(p.f, L, Wd, key=key, epsrel=p.epsrel, pool_max=256)
+    perm = rng.permutation(n)
+    res2, err2, cnt2 = ctx.phase2_regions(p.f, L[perm], Wd[perm], key=key, epsrel=p.epsrel, pool_max=256)
+    assert np.array_equal(cnt[perm], cnt2)
+    assert np.array_equal(res[perm].view(np.int64), res2.view(np.int64))
+    assert np.array_equal(err[perm].view(np.int64), err2.view(np.int64))
+    assert cnt.max() > 4 * np.median(cnt)  # run lengths differ, so the order matters for the launch
+
+
 def test_two_phase_reference_caps_on_product_peak8(ctx):
     """The reference's own caps (2^14 Phase-1 regions, quad/quad.h:9,21;
     2048 bisections per Phase-2 region, GPUQuadPhases.cu:456-458) on the
