@@ -560,21 +560,30 @@ __global__ __launch_bounds__(kBlock) void k_batch_split(BatchBuffers B)
     if (!act) continue;
     const int64_t dst = B.tile_off[t] + B.grp[B.gcap + (t >> 6)] + rank;
     const int a = B.axis[r];
-    for (int m = 0; m < B.dim; ++m) {
-      const double lo = B.lower[m * B.ld + r];
-      const double w = B.width[m * B.ld + r];
-      double2 L, Wd;
-      if (m == a) {
-        const double hw = 0.5 * w;
-        L = make_double2(lo, lo + hw);
-        Wd = make_double2(hw, hw);
-      } else {
-        L = make_double2(lo, lo);
-        Wd = make_double2(w, w);
+    // every coordinate's bounds loaded before the first store (one memory
+    // round trip per region instead of one per coordinate)
+    double lo[kMaxDim], wd[kMaxDim];
+#pragma unroll
+    for (int m = 0; m < kMaxDim; ++m)
+      if (m < B.dim) {
+        lo[m] = B.lower[m * B.ld + r];
+        wd[m] = B.width[m * B.ld + r];
       }
-      *reinterpret_cast<double2*>(B.out_lower + m * B.ld + 2 * dst) = L;
-      *reinterpret_cast<double2*>(B.out_width + m * B.ld + 2 * dst) = Wd;
-    }
+#pragma unroll
+    for (int m = 0; m < kMaxDim; ++m)
+      if (m < B.dim) {
+        double2 L, Wd;
+        if (m == a) {
+          const double hw = 0.5 * wd[m];
+          L = make_double2(lo[m], lo[m] + hw);
+          Wd = make_double2(hw, hw);
+        } else {
+          L = make_double2(lo[m], lo[m]);
+          Wd = make_double2(wd[m], wd[m]);
+        }
+        *reinterpret_cast<double2*>(B.out_lower + m * B.ld + 2 * dst) = L;
+        *reinterpret_cast<double2*>(B.out_width + m * B.ld + 2 * dst) = Wd;
+      }
     *reinterpret_cast<int2*>(B.out_inst + 2 * dst) = make_int2(i, i);
   }
 }
