@@ -1432,9 +1432,10 @@ __global__ __launch_bounds__(256) void k_p2_scatter(const uint8_t* bucket, int64
 // region (k_phase2), per-region outputs copied to the host.  The workspace
 // is the context's and kept between calls.
 struct Phase2Host {
-  std::vector<double> res, err, rab, rres;
-  std::vector<int32_t> cnt;
-  std::vector<int8_t> act, bad, zero;
+  std::vector<char> buf;  // the per-region outputs, copied as one span
+  const double *res, *err, *rab, *rres;
+  const int32_t* cnt;
+  const int8_t *act, *bad, *zero;
 };
 
 static int run_phase2(cuhre_context* c, const cuhre_integrand* f, int dim, int key, int cur, int64_t n,
@@ -1540,24 +1541,22 @@ static int run_phase2(cuhre_context* c, const cuhre_integrand* f, int dim, int k
     A.ab0 = ab0;
     A.ax0 = ax0;
   }
-  H.res.resize(n);
-  H.err.resize(n);
-  H.rab.resize(n);
-  H.rres.resize(n);
-  H.cnt.resize(n);
-  H.act.resize(n);
-  H.bad.resize(n);
-  H.zero.resize(n);
+  // the outputs res .. rres lie in this order in the workspace: one copy
+  const char* const d0 = reinterpret_cast<const char*>(A.res);
+  const size_t span = (size_t)(reinterpret_cast<const char*>(A.rres + n) - d0);
+  H.buf.resize(span);
+  const auto at = [&](const void* d) { return H.buf.data() + (reinterpret_cast<const char*>(d) - d0); };
+  H.res = reinterpret_cast<const double*>(at(A.res));
+  H.err = reinterpret_cast<const double*>(at(A.err));
+  H.rab = reinterpret_cast<const double*>(at(A.rab));
+  H.rres = reinterpret_cast<const double*>(at(A.rres));
+  H.cnt = reinterpret_cast<const int32_t*>(at(A.cnt));
+  H.act = reinterpret_cast<const int8_t*>(at(A.act));
+  H.bad = reinterpret_cast<const int8_t*>(at(A.nonfinite));
+  H.zero = reinterpret_cast<const int8_t*>(at(A.zeroed));
   launch(A, P, R, c->stream);
   if (hipGetLastError() != hipSuccess ||
-      hipMemcpyAsync(H.res.data(), A.res, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-      hipMemcpyAsync(H.err.data(), A.err, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-      hipMemcpyAsync(H.rab.data(), A.rab, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-      hipMemcpyAsync(H.rres.data(), A.rres, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-      hipMemcpyAsync(H.cnt.data(), A.cnt, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-      hipMemcpyAsync(H.act.data(), A.act, n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-      hipMemcpyAsync(H.bad.data(), A.nonfinite, n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-      hipMemcpyAsync(H.zero.data(), A.zeroed, n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      hipMemcpyAsync(H.buf.data(), d0, span, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess)
     return fail(CUHRE_ERR_DEVICE, "Phase 2 failed: %s", hipGetErrorString(hipGetLastError()));
   return 0;
@@ -1698,9 +1697,9 @@ int cuhre_integrate_two_phase(cuhre_context* c, const cuhre_integrand* f, int di
   const int64_t n = ho.n;
   Phase2Host H;
   if ((rc = run_phase2(c, f, dim, key, ho.cur, n, epsrel, epsabs, pool_max, H))) return rc;
-  const std::vector<double>&hres = H.res, &herr = H.err, &hrab = H.rab, &hrres = H.rres;
-  const std::vector<int32_t>& hcnt = H.cnt;
-  const std::vector<int8_t>&hact = H.act, &hbad = H.bad, &hzero = H.zero;
+  const double *hres = H.res, *herr = H.err, *hrab = H.rab, *hrres = H.rres;
+  const int32_t* hcnt = H.cnt;
+  const int8_t *hact = H.act, *hbad = H.bad, *hzero = H.zero;
   int result = 0;
   double I_s = ho.fin_s, I_c = ho.fin_c, E_s = ho.fer_s, E_c = ho.fer_c;
   double ab_tot = ho.fin_abs, res_round = 0.0, aest = ho.fin_aest;
